@@ -1,0 +1,180 @@
+/*
+ * qmb_hip.h -- C ABI of libqmb_hip.so, the MI355X (gfx950) implementation of qmb's sparse
+ * second-quantized Hamiltonian operators on bit-packed configurations.
+ *
+ * Conventions shared by every entry point:
+ *   - All array arguments are DEVICE pointers (hipMalloc / torch HIP tensors), row-major, contiguous.
+ *   - A config is n_qubytes bytes; site s is bit (s % 8) of byte s / 8 (qmb/bitspack.py:50).
+ *   - A term table is site int16[n_terms][4], kind uint8[n_terms][4] (0 = a, 1 = a^dagger,
+ *     2 = identity pad), coef float64[n_terms][2] (re, im) -- the layout of qmb/_hamiltonian.cpp:29-64.
+ *   - Amplitudes are complex128 stored as float64 pairs (re, im).
+ *   - particle_cut 1 = fermions (Jordan-Wigner sign), 2 = hard-core bosons (no sign).
+ *   - n_qubytes is 1..32.
+ *   - `stream` is a hipStream_t passed as void* (0 = the null stream).  Every call only enqueues
+ *     work on that stream and returns; no entry point synchronises the device or allocates device
+ *     memory.  Scratch memory comes from the caller through `workspace`, whose size the matching
+ *     *_workspace_size function returns.
+ *   - Return value: QMB_OK or an error code; qmb_last_error() gives the message of the last failure
+ *     on the calling host thread.
+ *   - Preconditions the reference also assumes: target / exclude config sets are duplicate-free
+ *     (with duplicates, the lowest row index receives the contribution).
+ */
+#ifndef QMB_HIP_H
+#define QMB_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QMB_OK 0
+#define QMB_ERR_ARGUMENT 1
+#define QMB_ERR_HIP 2
+#define QMB_ERR_WORKSPACE 3
+#define QMB_ERR_UNSUPPORTED 4
+
+/* Message of the last failing call on this host thread ("" if none). */
+const char* qmb_last_error(void);
+
+/* Library version as major * 10000 + minor * 100 + patch. */
+int qmb_version(void);
+
+/* Host-only validation of a term table held in HOST memory: counts terms whose kind is outside
+ * {0,1,2} or whose site lies outside [0, 8 * n_qubytes).  Replaces nothing in the reference
+ * (which leaves both undefined, qmb/_hamiltonian.cpp:29-64); the Python layer calls it once per
+ * table and n_qubytes. */
+int qmb_validate_terms_host(int n_qubytes, const int16_t* site, const uint8_t* kind, int64_t n_terms, int64_t* n_bad);
+
+/* Host-only compilation of one term into the library's 64-bit record (for unit tests of the
+ * term semantics without a GPU).  status: 0 ok, 1 never succeeds, 2 bad site, 3 bad kind. */
+int qmb_compile_term_host(const int16_t* site4, const uint8_t* kind4, int particle_cut, int n_qubytes, uint64_t* rec,
+                          int32_t* status, int64_t* bucket, int32_t* diagonal);
+
+/* Host-only application of a compiled record to one config (n_qubytes bytes, host memory).
+ * Returns 1 on success and writes the resulting config and parity bit. */
+int qmb_apply_record_host(uint64_t rec, int n_qubytes, const uint8_t* config, uint8_t* result, int32_t* parity);
+
+/* Host-only Philox4x32-10 block (for known-answer tests) and the (0,1] uniform built on it. */
+void qmb_philox4x32_10_host(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double qmb_philox_uniform_host(uint64_t seed, uint64_t index);
+
+/* ============================== drop-in operators ============================================ */
+
+/* apply_within: psi_j[k] = sum over (i, t) with t(configs_i[i]) == configs_j[k] of
+ * sign * coef[t] * psi_i[i]; psi_j is written in configs_j's own row order.
+ * Replaces apply_within_interface, qmb/_hamiltonian_cuda.cu:171-274 (CPU: _hamiltonian_cpu.cpp:164-257),
+ * bound by qmb/hamiltonian.py:104-131. */
+int qmb_apply_within_workspace_size(int n_qubytes, int64_t n_i, int64_t n_j, int64_t n_terms, size_t* bytes);
+int qmb_apply_within(int n_qubytes, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i,
+                     const uint8_t* configs_j, int64_t n_j, const int16_t* site, const uint8_t* kind, const double* coef,
+                     int64_t n_terms, double* psi_j, void* workspace, size_t workspace_bytes, void* stream);
+
+/* find_relative: the configs reached from configs_i that are not in configs_exclude, ranked per
+ * contribution by |sign * coef[t] * psi_i[i]|^2; the count_selected largest contributions are kept,
+ * zero weights dropped, and the distinct configs among them are written sorted by bytes into
+ * out (capacity count_selected rows).  *out_rows (device int64) receives the row count.
+ * Entries tied with the count_selected-th weight are taken in ascending config order.
+ * Replaces find_relative_interface, qmb/_hamiltonian_cuda.cu:551-675 (CPU :442-546),
+ * bound by qmb/hamiltonian.py:133-166. */
+int qmb_find_relative_workspace_size(int n_qubytes, int64_t n_i, int64_t n_exclude, int64_t n_terms, int64_t count_selected,
+                                     size_t* bytes);
+int qmb_find_relative(int n_qubytes, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i,
+                      int64_t count_selected, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
+                      const uint8_t* configs_exclude, int64_t n_exclude, uint8_t* out, int64_t* out_rows, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
+/* diagonal_term: out[k] = sum over terms t with t(configs[k]) == configs[k] of sign * coef[t].
+ * Replaces diagonal_term_interface, qmb/_hamiltonian_cuda.cu:738-803, bound by
+ * qmb/hamiltonian.py:168-186. */
+int qmb_diagonal_term_workspace_size(int n_qubytes, int64_t n, int64_t n_terms, size_t* bytes);
+int qmb_diagonal_term(int n_qubytes, int particle_cut, const uint8_t* configs, int64_t n, const int16_t* site,
+                      const uint8_t* kind, const double* coef, int64_t n_terms, double* out, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
+/* single_relative: for each config, among terms whose result is not one of the input configs, the
+ * result of the term with the largest key u_t^(1/|coef_t|) (Efraimidis-Spirakis), u_t uniform in
+ * (0,1] from Philox4x32-10 keyed by `seed` at counter t; an all-zero row when there is none.
+ * Ties go to the lower term index.  Replaces single_relative_interface,
+ * qmb/_hamiltonian_cuda.cu:907-996 (whose keys come from curand), bound by
+ * qmb/hamiltonian.py:188-206.  If `keys` is non-NULL it supplies the n_terms keys (device
+ * float64) instead of the Philox draw. */
+int qmb_single_relative_workspace_size(int n_qubytes, int64_t n, int64_t n_terms, size_t* bytes);
+int qmb_single_relative(int n_qubytes, int particle_cut, const uint8_t* configs, int64_t n, const int16_t* site,
+                        const uint8_t* kind, const double* coef, int64_t n_terms, uint64_t seed, const double* keys,
+                        uint8_t* out, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ============================== extensions (no reference counterpart) ========================= */
+
+/* Adjoint of apply_within with respect to psi_i: grad_i[i] = sum over (t, k) with
+ * t(configs_i[i]) == configs_j[k] of conj(sign * coef[t]) * grad_j[k] -- a gather, no atomics on
+ * the output rows of other configs.  New functionality: the reference never differentiates the
+ * path (qmb/vmc.py:104-116 calls apply_within under no_grad). */
+int qmb_apply_within_adjoint(int n_qubytes, int particle_cut, const uint8_t* configs_i, int64_t n_i,
+                             const uint8_t* configs_j, const double* grad_j, int64_t n_j, const int16_t* site,
+                             const uint8_t* kind, const double* coef, int64_t n_terms, double* grad_i, void* workspace,
+                             size_t workspace_bytes, void* stream);
+
+/* The reference's own scheme on the GPU, kept as the comparator: one thread per (term, config),
+ * a lexicographic sort of configs_j, a binary search per connection and FP64 atomics
+ * (qmb/_hamiltonian_cuda.cu:86-169 restated).  Same results as qmb_apply_within. */
+int qmb_apply_within_plain_workspace_size(int n_qubytes, int64_t n_j, size_t* bytes);
+int qmb_apply_within_plain(int n_qubytes, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i,
+                           const uint8_t* configs_j, int64_t n_j, const int16_t* site, const uint8_t* kind,
+                           const double* coef, int64_t n_terms, double* psi_j, void* workspace, size_t workspace_bytes,
+                           void* stream);
+
+/* ---------------- prepared objects: build once, apply many times (Krylov loops, autograd) ------ */
+
+/* A compiled, bucketed term table living in a caller-owned device buffer. */
+typedef struct qmb_terms {
+    int32_t n_qubytes, particle_cut, n_words, n_bits;
+    int64_t n_terms, n_buckets;
+    void* buffer;      /* device buffer the fields below point into */
+    size_t bytes;
+    uint64_t* recs;    /* [n_terms] compiled records, bucket order (valid prefix: counts[0]) */
+    double* coefs;     /* [n_terms][2] matching coefficients */
+    uint32_t* tidx;    /* [n_terms] original term index */
+    uint32_t* bucket_start; /* [n_buckets + 1] */
+    uint64_t* drecs;   /* [n_terms] diagonal terms in original order (valid prefix: counts[1]) */
+    double* dcoefs;    /* [n_terms][2] */
+    int64_t* counts;   /* device: {valid terms, diagonal terms, invalid terms, 0} */
+} qmb_terms;
+
+int qmb_terms_workspace_size(int n_qubytes, int64_t n_terms, size_t* bytes);
+int qmb_terms_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
+                    int64_t n_terms, void* buffer, size_t bytes, qmb_terms* out, void* stream);
+
+/* A hash index over a config set: padded key words plus an open-addressing table. */
+typedef struct qmb_targets {
+    int32_t n_qubytes, n_words;
+    int64_t n;
+    uint64_t capacity; /* power of two */
+    void* buffer;
+    size_t bytes;
+    uint64_t* keys;    /* [n][n_words] */
+    uint64_t* table;   /* [capacity] (index << 32 | fingerprint) or ~0 */
+} qmb_targets;
+
+int qmb_targets_workspace_size(int n_qubytes, int64_t n, size_t* bytes);
+int qmb_targets_build(int n_qubytes, const uint8_t* configs, int64_t n, void* buffer, size_t bytes, qmb_targets* out,
+                      void* stream);
+
+/* Launch shape for the enumeration kernels (0 = library default). */
+typedef struct qmb_launch {
+    int32_t tile;    /* configs per workgroup tile: 256..4096, power of two */
+    int32_t groups;  /* term-bucket groups per tile (8 keeps each group on one XCD's L2) */
+} qmb_launch;
+
+/* mode 0: forward scatter  out[k] += sum sign*coef*in[i]      (in = psi_i [n_i], out = psi_j [n_j])
+ * mode 1: adjoint gather   out[i] += sum conj(sign*coef)*in[k] (in = grad_j [n_j], out = grad_i [n_i])
+ * `out` is accumulated into (zero it first).  configs_i are n_qubytes-byte rows. */
+int qmb_apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const uint8_t* configs_i, int64_t n_i,
+                       const double* in, double* out, int mode, const qmb_launch* launch, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QMB_HIP_H */

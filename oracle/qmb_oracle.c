@@ -1,0 +1,433 @@
+/*
+ * qmb_oracle.c -- CPU restatement of qmb's Hamiltonian operators, used ONLY as a test checker.
+ *
+ * This file is test infrastructure.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it, and only to check or time against; the product path
+ * (qmb-public_amd/) never links, loads or calls it.
+ *
+ * It restates, in plain C and in the same loop order, the serial algorithms of the reference:
+ *   - term application + fermionic parity   qmb/_hamiltonian_cpu.cpp:48-78
+ *   - byte-lexicographic order               qmb/_hamiltonian_cpu.cpp:7-20
+ *   - apply_within (sort, term-outer/config-inner loop, binary search, +=)
+ *                                            qmb/_hamiltonian_cpu.cpp:80-162, :164-257
+ *   - find_relative (exclude search, weight, K-slot min-heap, nonzero filter, unique)
+ *                                            qmb/_hamiltonian_cpu.cpp:259-324, :326-339, :341-546
+ *   - diagonal_term (CUDA-only in the reference)   qmb/_hamiltonian_cuda.cu:677-803
+ *   - single_relative with caller-supplied per-term keys instead of curand
+ *                                            qmb/_hamiltonian_cuda.cu:805-996
+ *
+ * Pinning (what checks each function; see tests/test_oracle_*.py):
+ *   apply_term / apply_within : golden vectors from the compiled reference CPU op (tests/golden),
+ *                               dense Jordan-Wigner matrices built from Kronecker products,
+ *                               Hermiticity and closed-form energies.
+ *   find_relative             : golden vectors from the compiled reference CPU op, brute-force top-K.
+ *   diagonal_term             : identity diagonal_term(c) == apply_within([c],[1],[c]) and the
+ *                               closed form sum_p h_pp n_p + ... on normal-ordered molecular terms,
+ *                               and the dense JW diagonal.
+ *   single_relative           : brute force with injected keys (arg-max definition), dense JW
+ *                               connectivity.
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, no fast-math, so results are portable).
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MAX_OP 4
+#define MAX_NQ 64
+
+/* qmb/_hamiltonian_cpu.cpp:36-46: bit s lives in byte s/8 at position s%8 (LSB first). */
+static int get_bit(const uint8_t* cfg, int s) { return (cfg[s >> 3] >> (s & 7)) & 1; }
+
+static void set_bit(uint8_t* cfg, int s, int v) {
+    if (v) {
+        cfg[s >> 3] |= (uint8_t)(1u << (s & 7));
+    } else {
+        cfg[s >> 3] &= (uint8_t)~(1u << (s & 7));
+    }
+}
+
+/*
+ * qmb/_hamiltonian_cpu.cpp:48-78.  Ops are applied right to left (op index 3..0); kind 2 is an
+ * identity pad.  An op of kind k fails when the bit already equals k (a needs 1, a^dagger needs 0),
+ * otherwise it sets the bit to k.  For particle_cut 1 the parity flips by the number of occupied
+ * sites strictly below the op's site, counted on the config as it is after the op.
+ * Returns 1 on success; *parity receives the accumulated parity bit.
+ */
+int oracle_apply_term(uint8_t* cfg, const int16_t* site, const uint8_t* kind, int particle_cut, int* parity) {
+    int p = 0;
+    for (int op = MAX_OP; op-- > 0;) {
+        int k = kind[op];
+        if (k == 2) {
+            continue;
+        }
+        int s = site[op];
+        if (get_bit(cfg, s) == k) {
+            *parity = p;
+            return 0;
+        }
+        set_bit(cfg, s, k);
+        if (particle_cut == 1) {
+            for (int b = 0; b < s; ++b) {
+                p ^= get_bit(cfg, b);
+            }
+        }
+    }
+    *parity = p;
+    return 1;
+}
+
+/* qmb/_hamiltonian_cpu.cpp:7-20: unsigned bytes, byte 0 most significant. */
+static int cfg_cmp(const uint8_t* a, const uint8_t* b, int nq) {
+    for (int i = 0; i < nq; ++i) {
+        if (a[i] < b[i]) {
+            return -1;
+        }
+        if (a[i] > b[i]) {
+            return 1;
+        }
+    }
+    return 0;
+}
+
+/* Sorting context (qsort has no user pointer in C99). Ties broken by row index. */
+static const uint8_t* g_sort_base;
+static int g_sort_nq;
+
+static int idx_cmp(const void* pa, const void* pb) {
+    int64_t a = *(const int64_t*)pa;
+    int64_t b = *(const int64_t*)pb;
+    int c = cfg_cmp(g_sort_base + a * g_sort_nq, g_sort_base + b * g_sort_nq, g_sort_nq);
+    if (c != 0) {
+        return c;
+    }
+    return (a > b) - (a < b);
+}
+
+static int64_t* sorted_order(const uint8_t* configs, int64_t n, int nq) {
+    int64_t* order = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; ++i) {
+        order[i] = i;
+    }
+    g_sort_base = configs;
+    g_sort_nq = nq;
+    qsort(order, (size_t)n, sizeof(int64_t), idx_cmp);
+    return order;
+}
+
+/*
+ * qmb/_hamiltonian_cpu.cpp:108-122: the reference's binary search (low/high/mid).  Returns the
+ * position in the sorted view, or -1 when absent.
+ */
+static int64_t bsearch_sorted(const uint8_t* sorted, int64_t n, int nq, const uint8_t* key) {
+    int64_t low = 0, high = n - 1;
+    while (low <= high) {
+        int64_t mid = (low + high) / 2;
+        int c = cfg_cmp(key, sorted + mid * nq, nq);
+        if (c < 0) {
+            high = mid - 1;
+        } else if (c > 0) {
+            low = mid + 1;
+        } else {
+            return mid;
+        }
+    }
+    return -1;
+}
+
+static uint8_t* gather_rows(const uint8_t* configs, const int64_t* order, int64_t n, int nq) {
+    uint8_t* out = (uint8_t*)malloc((size_t)(n > 0 ? n : 1) * (size_t)nq);
+    for (int64_t i = 0; i < n; ++i) {
+        memcpy(out + i * nq, configs + order[i] * nq, (size_t)nq);
+    }
+    return out;
+}
+
+/*
+ * apply_within: qmb/_hamiltonian_cpu.cpp:80-162 (kernel + term-outer loop) and :164-257 (sort of
+ * configs_j by index, zero accumulator, un-permute).  psi arrays are interleaved (re, im).
+ * abs_sum (optional, [n_j]) receives sum |contribution| per target, used for condition-aware
+ * tolerances.  counts (optional, [2]) receives {M_gen, M_hit}: successful applications and those
+ * whose target is present.
+ */
+int oracle_apply_within(int nq, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i,
+                        const uint8_t* configs_j, int64_t n_j, const int16_t* site, const uint8_t* kind, const double* coef,
+                        int64_t n_terms, double* psi_j, double* abs_sum, int64_t* counts) {
+    if (nq <= 0 || nq > MAX_NQ) {
+        return 1;
+    }
+    int64_t* order = sorted_order(configs_j, n_j, nq);
+    uint8_t* sorted = gather_rows(configs_j, order, n_j, nq);
+    double* acc = (double*)calloc((size_t)(n_j > 0 ? n_j : 1) * 2, sizeof(double));
+    double* aacc = (double*)calloc((size_t)(n_j > 0 ? n_j : 1), sizeof(double));
+    int64_t m_gen = 0, m_hit = 0;
+    uint8_t cur[MAX_NQ];
+    for (int64_t t = 0; t < n_terms; ++t) {
+        for (int64_t i = 0; i < n_i; ++i) {
+            memcpy(cur, configs_i + i * nq, (size_t)nq);
+            int parity = 0;
+            if (!oracle_apply_term(cur, site + t * MAX_OP, kind + t * MAX_OP, particle_cut, &parity)) {
+                continue;
+            }
+            ++m_gen;
+            int64_t mid = bsearch_sorted(sorted, n_j, nq, cur);
+            if (mid < 0) {
+                continue;
+            }
+            ++m_hit;
+            double sign = parity ? -1.0 : 1.0;
+            double c0 = coef[2 * t], c1 = coef[2 * t + 1];
+            double p0 = psi_i[2 * i], p1 = psi_i[2 * i + 1];
+            double re = sign * (c0 * p0 - c1 * p1);
+            double im = sign * (c0 * p1 + c1 * p0);
+            acc[2 * mid] += re;
+            acc[2 * mid + 1] += im;
+            aacc[mid] += __builtin_sqrt(re * re + im * im);
+        }
+    }
+    for (int64_t k = 0; k < n_j; ++k) {
+        psi_j[2 * order[k]] = acc[2 * k];
+        psi_j[2 * order[k] + 1] = acc[2 * k + 1];
+        if (abs_sum) {
+            abs_sum[order[k]] = aacc[k];
+        }
+    }
+    if (counts) {
+        counts[0] = m_gen;
+        counts[1] = m_hit;
+    }
+    free(order);
+    free(sorted);
+    free(acc);
+    free(aacc);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * find_relative.  A heap entry is 8 weight bytes followed by nq config bytes
+ * (qmb/_hamiltonian_cpu.cpp:393-399); the heap orders by the leading double only (:326-339).
+ */
+typedef struct {
+    double w;
+    uint8_t cfg[MAX_NQ];
+} heap_entry;
+
+/* qmb/_hamiltonian_cpu.cpp:259-324: replace-root-and-sift-down on a K-slot min-heap; an entry
+ * strictly below the root is dropped, ties with the root enter. */
+static void heap_add(heap_entry* heap, int64_t size, const heap_entry* v) {
+    int64_t idx = 0;
+    if (v->w < heap[0].w) {
+        return;
+    }
+    for (;;) {
+        int64_t left = 2 * idx + 1, right = 2 * idx + 2;
+        int lp = left < size, rp = right < size;
+        if (lp && rp) {
+            if (v->w < heap[left].w) {
+                if (v->w < heap[right].w) {
+                    break;
+                }
+                heap[idx] = heap[right];
+                idx = right;
+            } else {
+                if (v->w < heap[right].w) {
+                    heap[idx] = heap[left];
+                    idx = left;
+                } else if (heap[left].w < heap[right].w) {
+                    heap[idx] = heap[left];
+                    idx = left;
+                } else {
+                    heap[idx] = heap[right];
+                    idx = right;
+                }
+            }
+        } else if (lp) {
+            if (v->w < heap[left].w) {
+                break;
+            }
+            heap[idx] = heap[left];
+            idx = left;
+        } else if (rp) {
+            if (v->w < heap[right].w) {
+                break;
+            }
+            heap[idx] = heap[right];
+            idx = right;
+        } else {
+            break;
+        }
+    }
+    heap[idx] = *v;
+}
+
+static int g_nq_unique;
+static int row_cmp(const void* a, const void* b) { return cfg_cmp((const uint8_t*)a, (const uint8_t*)b, g_nq_unique); }
+
+/*
+ * qmb/_hamiltonian_cpu.cpp:341-546.  Writes up to count_selected unique configs, sorted by bytes,
+ * into out (capacity count_selected rows) and returns the row count (or -1 on bad arguments).
+ * out_weights (optional, [count_selected]) receives the heap's weights (heap order) and
+ * kth_weight (optional) the root weight after the run (the K-th largest entry weight).
+ * count_selected must be >= 1 (0 reads heap[0] out of bounds in the reference).
+ */
+int64_t oracle_find_relative(int nq, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i,
+                             int64_t count_selected, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
+                             const uint8_t* configs_exclude, int64_t n_exclude, uint8_t* out, double* kth_weight) {
+    if (nq <= 0 || nq > MAX_NQ || count_selected <= 0) {
+        return -1;
+    }
+    int64_t* order = sorted_order(configs_exclude, n_exclude, nq);
+    uint8_t* sorted = gather_rows(configs_exclude, order, n_exclude, nq);
+    heap_entry* heap = (heap_entry*)calloc((size_t)count_selected, sizeof(heap_entry));
+    heap_entry v;
+    memset(&v, 0, sizeof(v));
+    for (int64_t t = 0; t < n_terms; ++t) {
+        for (int64_t i = 0; i < n_i; ++i) {
+            uint8_t cur[MAX_NQ];
+            memcpy(cur, configs_i + i * nq, (size_t)nq);
+            int parity = 0;
+            if (!oracle_apply_term(cur, site + t * MAX_OP, kind + t * MAX_OP, particle_cut, &parity)) {
+                continue;
+            }
+            if (bsearch_sorted(sorted, n_exclude, nq, cur) >= 0) {
+                continue;
+            }
+            double sign = parity ? -1.0 : 1.0;
+            double c0 = coef[2 * t], c1 = coef[2 * t + 1];
+            double p0 = psi_i[2 * i], p1 = psi_i[2 * i + 1];
+            double re = sign * (c0 * p0 - c1 * p1);
+            double im = sign * (c0 * p1 + c1 * p0);
+            v.w = re * re + im * im;
+            memcpy(v.cfg, cur, (size_t)nq);
+            heap_add(heap, count_selected, &v);
+        }
+    }
+    if (kth_weight) {
+        *kth_weight = heap[0].w;
+    }
+    /* :537-545: keep rows whose weight bytes are not all zero, then unique (sorted). */
+    uint8_t* rows = (uint8_t*)malloc((size_t)count_selected * (size_t)nq);
+    int64_t m = 0;
+    for (int64_t k = 0; k < count_selected; ++k) {
+        uint64_t bits;
+        memcpy(&bits, &heap[k].w, sizeof(bits));
+        if (bits != 0) {
+            memcpy(rows + m * nq, heap[k].cfg, (size_t)nq);
+            ++m;
+        }
+    }
+    g_nq_unique = nq;
+    qsort(rows, (size_t)m, (size_t)nq, row_cmp);
+    int64_t u = 0;
+    for (int64_t k = 0; k < m; ++k) {
+        if (u == 0 || cfg_cmp(rows + k * nq, out + (u - 1) * nq, nq) != 0) {
+            memcpy(out + u * nq, rows + k * nq, (size_t)nq);
+            ++u;
+        }
+    }
+    free(rows);
+    free(heap);
+    free(order);
+    free(sorted);
+    return u;
+}
+
+/*
+ * qmb/_hamiltonian_cuda.cu:677-708: d[c] = sum over terms whose application succeeds and returns
+ * c itself of sign * coef, summed here in term order.
+ */
+int oracle_diagonal_term(int nq, int particle_cut, const uint8_t* configs, int64_t n, const int16_t* site, const uint8_t* kind,
+                         const double* coef, int64_t n_terms, double* out) {
+    if (nq <= 0 || nq > MAX_NQ) {
+        return 1;
+    }
+    uint8_t cur[MAX_NQ];
+    for (int64_t i = 0; i < n; ++i) {
+        double re = 0.0, im = 0.0;
+        for (int64_t t = 0; t < n_terms; ++t) {
+            memcpy(cur, configs + i * nq, (size_t)nq);
+            int parity = 0;
+            if (!oracle_apply_term(cur, site + t * MAX_OP, kind + t * MAX_OP, particle_cut, &parity)) {
+                continue;
+            }
+            if (cfg_cmp(cur, configs + i * nq, nq) != 0) {
+                continue;
+            }
+            double sign = parity ? -1.0 : 1.0;
+            re += sign * coef[2 * t];
+            im += sign * coef[2 * t + 1];
+        }
+        out[2 * i] = re;
+        out[2 * i + 1] = im;
+    }
+    return 0;
+}
+
+/*
+ * qmb/_hamiltonian_cuda.cu:805-996 with the random keys supplied by the caller: for each config,
+ * among terms whose application succeeds and whose result is not in the (sorted) input set, pick
+ * the one with the largest key (the first such term in term order on ties).  Configs with no
+ * candidate get an all-zero row (:966).
+ */
+int oracle_single_relative(int nq, int particle_cut, const uint8_t* configs, int64_t n, const int16_t* site, const uint8_t* kind,
+                           const double* keys, int64_t n_terms, uint8_t* out) {
+    if (nq <= 0 || nq > MAX_NQ) {
+        return 1;
+    }
+    int64_t* order = sorted_order(configs, n, nq);
+    uint8_t* sorted = gather_rows(configs, order, n, nq);
+    uint8_t cur[MAX_NQ];
+    for (int64_t i = 0; i < n; ++i) {
+        int found = 0;
+        double best = 0.0;
+        memset(out + i * nq, 0, (size_t)nq);
+        for (int64_t t = 0; t < n_terms; ++t) {
+            memcpy(cur, configs + i * nq, (size_t)nq);
+            int parity = 0;
+            if (!oracle_apply_term(cur, site + t * MAX_OP, kind + t * MAX_OP, particle_cut, &parity)) {
+                continue;
+            }
+            if (bsearch_sorted(sorted, n, nq, cur) >= 0) {
+                continue;
+            }
+            if (!found || keys[t] > best) {
+                found = 1;
+                best = keys[t];
+                memcpy(out + i * nq, cur, (size_t)nq);
+            }
+        }
+    }
+    free(order);
+    free(sorted);
+    return 0;
+}
+
+/* Counts for the roofline model: M_gen and M_hit without computing values (same loops). */
+int oracle_count_connections(int nq, int particle_cut, const uint8_t* configs_i, int64_t n_i, const uint8_t* configs_j, int64_t n_j,
+                             const int16_t* site, const uint8_t* kind, int64_t n_terms, int64_t* counts) {
+    if (nq <= 0 || nq > MAX_NQ) {
+        return 1;
+    }
+    int64_t* order = sorted_order(configs_j, n_j, nq);
+    uint8_t* sorted = gather_rows(configs_j, order, n_j, nq);
+    int64_t m_gen = 0, m_hit = 0;
+    uint8_t cur[MAX_NQ];
+    for (int64_t i = 0; i < n_i; ++i) {
+        for (int64_t t = 0; t < n_terms; ++t) {
+            memcpy(cur, configs_i + i * nq, (size_t)nq);
+            int parity = 0;
+            if (!oracle_apply_term(cur, site + t * MAX_OP, kind + t * MAX_OP, particle_cut, &parity)) {
+                continue;
+            }
+            ++m_gen;
+            if (bsearch_sorted(sorted, n_j, nq, cur) >= 0) {
+                ++m_hit;
+            }
+        }
+    }
+    counts[0] = m_gen;
+    counts[1] = m_hit;
+    free(order);
+    free(sorted);
+    return 0;
+}

@@ -1,0 +1,483 @@
+// qmb_apply.hip -- the hot path: Hamiltonian application on bit-packed configs for gfx950.
+//
+// apply_within forward (scatter) and adjoint (gather) run on the tiled, bucketed enumeration of
+// qmb_enum.hpp; each success is looked up in the hash index of configs_j and accumulated in FP64.
+// Also here: the reference-scheme comparator kernel, diagonal_term and single_relative.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "qmb_enum.hpp"
+
+namespace qmb {
+
+namespace {
+
+constexpr int kBlock = kEnumBlock;
+
+inline unsigned blocks_for(int64_t n, int block = kBlock) {
+    int64_t b = (n + block - 1) / block;
+    return static_cast<unsigned>(b < 1 ? 1 : b);
+}
+
+struct ApplyParams {
+    EnumSource src;
+    const double2* in;
+    double2* out;
+    const double2* coefs;
+    const uint64_t* tkeys;
+    const uint64_t* table;
+    uint64_t mask;
+    int tile_log2;
+    int groups;
+};
+
+// out[j] += sign * coef * in[i]: scatter with hardware FP64 atomics.
+template <int W>
+struct ForwardAction {
+    const ApplyParams& p;
+    double2 v;
+    __device__ ForwardAction(const ApplyParams& pp) : p(pp), v(make_double2(0.0, 0.0)) {}
+    __device__ __forceinline__ void begin_member(int64_t i) { v = p.in[i]; }
+    __device__ __forceinline__ void on_success(int64_t, uint32_t t, const Cfg<W>& o, int par) {
+        const int64_t j = lookup<W>(p.table, p.mask, p.tkeys, o);
+        if (j < 0) {
+            return;
+        }
+        const double2 cf = p.coefs[t];
+        double re = cf.x * v.x - cf.y * v.y;
+        double im = cf.x * v.y + cf.y * v.x;
+        if (par) {
+            re = -re;
+            im = -im;
+        }
+        unsafeAtomicAdd(&p.out[j].x, re);
+        unsafeAtomicAdd(&p.out[j].y, im);
+    }
+    __device__ __forceinline__ void end_member(int64_t) {}
+};
+
+// out[i] += conj(sign * coef) * in[j]: gather into registers, one atomic per (thread, member).
+template <int W>
+struct AdjointAction {
+    const ApplyParams& p;
+    double re, im;
+    __device__ AdjointAction(const ApplyParams& pp) : p(pp), re(0.0), im(0.0) {}
+    __device__ __forceinline__ void begin_member(int64_t) {
+        re = 0.0;
+        im = 0.0;
+    }
+    __device__ __forceinline__ void on_success(int64_t, uint32_t t, const Cfg<W>& o, int par) {
+        const int64_t j = lookup<W>(p.table, p.mask, p.tkeys, o);
+        if (j < 0) {
+            return;
+        }
+        const double2 cf = p.coefs[t];
+        const double2 g = p.in[j];
+        const double a = cf.x * g.x + cf.y * g.y;
+        const double b = cf.x * g.y - cf.y * g.x;
+        re += par ? -a : a;
+        im += par ? -b : b;
+    }
+    __device__ __forceinline__ void end_member(int64_t i) {
+        if (re != 0.0 || im != 0.0) {
+            unsafeAtomicAdd(&p.out[i].x, re);
+            unsafeAtomicAdd(&p.out[i].y, im);
+        }
+    }
+};
+
+template <int W, int MODE>
+__global__ __launch_bounds__(kBlock) void apply_kernel(ApplyParams p) {
+    extern __shared__ uint64_t smem[];
+    const int64_t tile = blockIdx.x / p.groups;
+    const int g = blockIdx.x % p.groups;
+    if (MODE == 0) {
+        ForwardAction<W> act(p);
+        enumerate_tile<W>(p.src, tile, p.tile_log2, g, p.groups, smem, act);
+    } else {
+        AdjointAction<W> act(p);
+        enumerate_tile<W>(p.src, tile, p.tile_log2, g, p.groups, smem, act);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// The reference's scheme on the GPU (comparator): thread (term, config) over the full grid,
+// binary search in configs_j sorted by bytes, FP64 atomics (qmb/_hamiltonian_cuda.cu:86-169).
+// ------------------------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(kBlock) void plain_kernel(const uint8_t* __restrict__ configs, const double2* __restrict__ psi,
+                                                      int64_t n_i, int nq, int particle_cut, const int16_t* __restrict__ site,
+                                                      const uint8_t* __restrict__ kind, const double2* __restrict__ coef,
+                                                      const uint64_t* __restrict__ sorted_be, const uint32_t* __restrict__ perm,
+                                                      int64_t n_j, double2* __restrict__ out) {
+    const int64_t t = blockIdx.x;
+    int16_t ts[kMaxOps];
+    uint8_t tk[kMaxOps];
+#pragma unroll
+    for (int op = 0; op < kMaxOps; ++op) {
+        ts[op] = site[4 * t + op];
+        tk[op] = kind[4 * t + op];
+    }
+    const double2 cf = coef[t];
+    for (int64_t i = blockIdx.y * static_cast<int64_t>(kBlock) + threadIdx.x; i < n_i;
+         i += static_cast<int64_t>(gridDim.y) * kBlock) {
+        Cfg<W> c = load_row<W>(configs + i * nq, nq);
+        bool ok = true;
+        int parity = 0;
+#pragma unroll
+        for (int op = kMaxOps; op-- > 0;) {
+            const int k = tk[op];
+            if (k == 2 || !ok) {
+                continue;
+            }
+            const int s = ts[op];
+            const int wi = s >> 6;
+            const uint64_t bit = 1ull << (s & 63);
+            uint64_t word = 0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                word = (w == wi) ? c.w[w] : word;
+            }
+            if (((word & bit) ? 1 : 0) == k) {
+                ok = false;
+                continue;
+            }
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                if (w == wi) {
+                    c.w[w] = k ? (c.w[w] | bit) : (c.w[w] & ~bit);
+                }
+                if (particle_cut == 1) {
+                    parity ^= __popcll(c.w[w] & low_mask_word(s, w)) & 1;
+                }
+            }
+        }
+        if (!ok) {
+            continue;
+        }
+        uint64_t key[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            key[w] = bswap64(c.w[w]);
+        }
+        int64_t lo = 0, hi = n_j - 1, found = -1;
+        while (lo <= hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            int cmp = 0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const uint64_t s = sorted_be[mid * W + w];
+                if (cmp == 0) {
+                    cmp = key[w] < s ? -1 : (key[w] > s ? 1 : 0);
+                }
+            }
+            if (cmp < 0) {
+                hi = mid - 1;
+            } else if (cmp > 0) {
+                lo = mid + 1;
+            } else {
+                found = mid;
+                break;
+            }
+        }
+        if (found < 0) {
+            continue;
+        }
+        const int64_t j = perm[found];
+        const double2 v = psi[i];
+        double re = cf.x * v.x - cf.y * v.y;
+        double im = cf.x * v.y + cf.y * v.x;
+        if (parity) {
+            re = -re;
+            im = -im;
+        }
+        unsafeAtomicAdd(&out[j].x, re);
+        unsafeAtomicAdd(&out[j].y, im);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// diagonal_term: one thread per config, diagonal terms staged through LDS in original order,
+// summed serially so the result does not depend on scheduling (qmb/_hamiltonian_cuda.cu:677-708).
+// ------------------------------------------------------------------------------------------
+constexpr int kDiagTile = 1024;
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void diag_kernel(const uint8_t* __restrict__ configs, int64_t n, int nq,
+                                                     const uint64_t* __restrict__ drecs, const double2* __restrict__ dcoefs,
+                                                     const int64_t* __restrict__ counts, double2* __restrict__ out) {
+    __shared__ uint64_t s_rec[kDiagTile];
+    __shared__ double2 s_coef[kDiagTile];
+    const int64_t i = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x;
+    const bool valid = i < n;
+    Cfg<W> c;
+    if (valid) {
+        c = load_row<W>(configs + i * nq, nq);
+    } else {
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            c.w[k] = 0;
+        }
+    }
+    const int64_t td = counts[1];
+    double re = 0.0, im = 0.0;
+    for (int64_t base = 0; base < td; base += kDiagTile) {
+        const int cnt = static_cast<int>(td - base < kDiagTile ? td - base : kDiagTile);
+        __syncthreads();
+        for (int u = threadIdx.x; u < cnt; u += kBlock) {
+            s_rec[u] = drecs[base + u];
+            s_coef[u] = dcoefs[base + u];
+        }
+        __syncthreads();
+        for (int u = 0; u < cnt; ++u) {
+            Cfg<W> o;
+            int par;
+            if (apply_rec<W>(s_rec[u], c, o, par)) {
+                const double2 cf = s_coef[u];
+                re += par ? -cf.x : cf.x;
+                im += par ? -cf.y : cf.y;
+            }
+        }
+    }
+    if (valid) {
+        out[i] = make_double2(re, im);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// single_relative: per-term Efraimidis-Spirakis keys, terms sorted by key (descending, ties by
+// index), then each config takes the first term that succeeds with a result outside the input set.
+// ------------------------------------------------------------------------------------------
+__global__ void sr_keys_kernel(const double2* __restrict__ coef, int64_t n_terms, uint64_t seed, const double* __restrict__ keys_in,
+                               uint64_t* __restrict__ keybits, uint32_t* __restrict__ vals, uint8_t* __restrict__ usable) {
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= n_terms) {
+        return;
+    }
+    double key;
+    if (keys_in) {
+        key = keys_in[t];
+    } else {
+        const double2 cf = coef[t];
+        const double weight = pow(cf.x * cf.x + cf.y * cf.y, 0.5);
+        key = pow(philox_uniform(seed, static_cast<uint64_t>(t)), 1.0 / weight);
+    }
+    const bool ok = !(key != key) && key >= 0.0;  // a NaN key never wins (score < NaN is false)
+    keybits[t] = ok ? static_cast<uint64_t>(__double_as_longlong(key)) : 0ull;
+    usable[t] = ok ? 1 : 0;
+    vals[t] = static_cast<uint32_t>(t);
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void single_rel_kernel(const uint8_t* __restrict__ configs, int64_t n, int nq,
+                                                           const uint32_t* __restrict__ order, const uint64_t* __restrict__ orecs,
+                                                           const uint8_t* __restrict__ usable, int64_t n_terms,
+                                                           const uint64_t* __restrict__ tkeys, const uint64_t* __restrict__ table,
+                                                           uint64_t mask, uint8_t* __restrict__ out) {
+    const int64_t i = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const Cfg<W> c = load_row<W>(configs + i * nq, nq);
+    for (int64_t q = 0; q < n_terms; ++q) {
+        const uint32_t t = order[q];
+        const uint64_t rec = orecs[t];
+        if ((rec & kNeverSucceeds) || !usable[t]) {
+            continue;
+        }
+        Cfg<W> o;
+        int par;
+        if (!apply_rec<W>(rec, c, o, par)) {
+            continue;
+        }
+        if (lookup<W>(table, mask, tkeys, o) >= 0) {
+            continue;
+        }
+        store_row<W>(out + i * nq, nq, o);
+        return;
+    }
+    for (int b = 0; b < nq; ++b) {
+        out[i * nq + b] = 0;
+    }
+}
+
+}  // namespace
+
+LaunchShape resolve_launch(const qmb_launch* launch, int n_words, int64_t n_i) {
+    LaunchShape s{};
+    int tile = 2048 / n_words;
+    int groups = 8;
+    if (launch && launch->tile > 0) {
+        tile = launch->tile;
+    } else {
+        // keep at least ~128 tiles in flight for small inputs (tile >= 256)
+        while (tile > 256 && (n_i + tile - 1) / tile < 128) {
+            tile >>= 1;
+        }
+    }
+    if (launch && launch->groups > 0) {
+        groups = launch->groups;
+    }
+    if (tile < 64) {
+        tile = 64;
+    }
+    if (tile > 4096) {
+        tile = 4096;
+    }
+    int lg = 0;
+    while ((1 << lg) < tile) {
+        ++lg;
+    }
+    s.tile_log2 = lg;
+    s.groups = groups < 1 ? 1 : (groups > 64 ? 64 : groups);
+    return s;
+}
+
+size_t enum_lds_bytes(int n_words, int n_bits, int tile_log2) { return enum_lds_bytes_of(n_words, n_bits, tile_log2); }
+
+int apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const uint8_t* configs_i, int64_t n_i, const double* in,
+                   double* out, int mode, const qmb_launch* launch, hipStream_t stream) {
+    if (!terms || !targets) {
+        return set_error(QMB_ERR_ARGUMENT, "terms and targets must be non-null");
+    }
+    if (terms->n_qubytes != targets->n_qubytes) {
+        return set_error(QMB_ERR_ARGUMENT, "terms built for %d bytes, targets for %d", terms->n_qubytes, targets->n_qubytes);
+    }
+    if (mode != 0 && mode != 1) {
+        return set_error(QMB_ERR_ARGUMENT, "mode must be 0 (forward) or 1 (adjoint)");
+    }
+    if (n_i < 0 || n_i >= (1ll << 40)) {
+        return set_error(QMB_ERR_ARGUMENT, "n_i out of range");
+    }
+    if (n_i == 0 || terms->n_terms == 0) {
+        return QMB_OK;
+    }
+    const LaunchShape ls = resolve_launch(launch, terms->n_words, n_i);
+    ApplyParams p{};
+    p.src = make_source(terms, configs_i, n_i);
+    p.in = reinterpret_cast<const double2*>(in);
+    p.out = reinterpret_cast<double2*>(out);
+    p.coefs = reinterpret_cast<const double2*>(terms->coefs);
+    p.tkeys = targets->keys;
+    p.table = targets->table;
+    p.mask = targets->capacity - 1;
+    p.tile_log2 = ls.tile_log2;
+    p.groups = ls.groups;
+    const int64_t tiles = (n_i + (1ll << ls.tile_log2) - 1) >> ls.tile_log2;
+    const int64_t grid = tiles * ls.groups;
+    if (grid >= (1ll << 31)) {
+        return set_error(QMB_ERR_ARGUMENT, "grid too large");
+    }
+    const size_t lds = enum_lds_bytes(terms->n_words, terms->n_bits, ls.tile_log2);
+    if (lds > 160 * 1024 - 1024) {
+        return set_error(QMB_ERR_ARGUMENT, "tile of %d configs needs %zu bytes of LDS", 1 << ls.tile_log2, lds);
+    }
+    return dispatch_words(terms->n_words, [&](auto wc) -> int {
+        constexpr int W = decltype(wc)::value;
+        if (mode == 0) {
+            QMB_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&apply_kernel<W, 0>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+            apply_kernel<W, 0><<<static_cast<unsigned>(grid), kBlock, lds, stream>>>(p);
+        } else {
+            QMB_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&apply_kernel<W, 1>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+            apply_kernel<W, 1><<<static_cast<unsigned>(grid), kBlock, lds, stream>>>(p);
+        }
+        QMB_CHECK_LAUNCH();
+        return QMB_OK;
+    });
+}
+
+int apply_plain(int n_qubytes, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i,
+                const uint64_t* sorted_keys_be, const uint32_t* perm, int64_t n_j, const int16_t* site, const uint8_t* kind,
+                const double* coef, int64_t n_terms, double* psi_j, hipStream_t stream) {
+    if (n_i == 0 || n_terms == 0 || n_j == 0) {
+        return QMB_OK;
+    }
+    if (n_terms >= (1ll << 31)) {
+        return set_error(QMB_ERR_ARGUMENT, "too many terms for the plain grid");
+    }
+    int64_t gy = (n_i + kBlock - 1) / kBlock;
+    if (gy > 65535) {
+        gy = 65535;
+    }
+    dim3 grid(static_cast<unsigned>(n_terms), static_cast<unsigned>(gy));
+    return dispatch_words(n_words_of(n_qubytes), [&](auto wc) -> int {
+        constexpr int W = decltype(wc)::value;
+        plain_kernel<W><<<grid, kBlock, 0, stream>>>(configs_i, reinterpret_cast<const double2*>(psi_i), n_i, n_qubytes,
+                                                      particle_cut, site, kind, reinterpret_cast<const double2*>(coef),
+                                                      sorted_keys_be, perm, n_j, reinterpret_cast<double2*>(psi_j));
+        QMB_CHECK_LAUNCH();
+        return QMB_OK;
+    });
+}
+
+int diagonal_prepared(const qmb_terms* terms, const uint8_t* configs, int64_t n, double* out, hipStream_t stream) {
+    if (n == 0) {
+        return QMB_OK;
+    }
+    return dispatch_words(terms->n_words, [&](auto wc) -> int {
+        constexpr int W = decltype(wc)::value;
+        diag_kernel<W><<<blocks_for(n), kBlock, 0, stream>>>(configs, n, terms->n_qubytes, terms->drecs,
+                                                             reinterpret_cast<const double2*>(terms->dcoefs), terms->counts,
+                                                             reinterpret_cast<double2*>(out));
+        QMB_CHECK_LAUNCH();
+        return QMB_OK;
+    });
+}
+
+size_t single_relative_scratch_bytes(int64_t n_terms) {
+    const size_t T = static_cast<size_t>(n_terms > 0 ? n_terms : 1);
+    size_t sort_bytes = 0;
+    uint64_t* k = nullptr;
+    uint32_t* v = nullptr;
+    (void)rocprim::radix_sort_pairs_desc(nullptr, sort_bytes, k, k, v, v, static_cast<unsigned>(T));
+    Carver cv;
+    cv.take<uint64_t>(T);  // orecs
+    cv.take<uint64_t>(T);  // keybits
+    cv.take<uint64_t>(T);  // sorted keybits
+    cv.take<uint32_t>(T);  // vals
+    cv.take<uint32_t>(T);  // order
+    cv.take<uint8_t>(T);   // usable
+    cv.take_bytes(sort_bytes);
+    return cv.off + 256;
+}
+
+int single_relative_prepared(int n_qubytes, int particle_cut, const uint8_t* configs, int64_t n, const int16_t* site,
+                             const uint8_t* kind, const double* coef, int64_t n_terms, uint64_t seed, const double* keys,
+                             const qmb_targets* exclude, uint8_t* out, void* scratch, size_t scratch_bytes, hipStream_t stream) {
+    if (n == 0) {
+        return QMB_OK;
+    }
+    const size_t T = static_cast<size_t>(n_terms > 0 ? n_terms : 1);
+    Carver cv{static_cast<char*>(scratch), 0};
+    uint64_t* orecs = cv.take<uint64_t>(T);
+    uint64_t* keybits = cv.take<uint64_t>(T);
+    uint64_t* skeybits = cv.take<uint64_t>(T);
+    uint32_t* vals = cv.take<uint32_t>(T);
+    uint32_t* order = cv.take<uint32_t>(T);
+    uint8_t* usable = cv.take<uint8_t>(T);
+    size_t sort_bytes = 0;
+    (void)rocprim::radix_sort_pairs_desc(nullptr, sort_bytes, keybits, skeybits, vals, order, static_cast<unsigned>(T));
+    void* tmp = cv.take_bytes(sort_bytes);
+    if (cv.off > scratch_bytes) {
+        return set_error(QMB_ERR_WORKSPACE, "single_relative scratch too small");
+    }
+    if (n_terms > 0) {
+        QMB_TRY(compile_records_original(n_qubytes, particle_cut, site, kind, n_terms, orecs, stream));
+        sr_keys_kernel<<<blocks_for(n_terms), kBlock, 0, stream>>>(reinterpret_cast<const double2*>(coef), n_terms, seed, keys,
+                                                                  keybits, vals, usable);
+        QMB_CHECK_LAUNCH();
+        QMB_HIP_TRY(rocprim::radix_sort_pairs_desc(tmp, sort_bytes, keybits, skeybits, vals, order,
+                                                   static_cast<unsigned>(n_terms), 0, 64, stream));
+    }
+    return dispatch_words(n_words_of(n_qubytes), [&](auto wc) -> int {
+        constexpr int W = decltype(wc)::value;
+        single_rel_kernel<W><<<blocks_for(n), kBlock, 0, stream>>>(configs, n, n_qubytes, order, orecs, usable, n_terms,
+                                                                   exclude->keys, exclude->table, exclude->capacity - 1, out);
+        QMB_CHECK_LAUNCH();
+        return QMB_OK;
+    });
+}
+
+}  // namespace qmb

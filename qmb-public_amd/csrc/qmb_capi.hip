@@ -1,0 +1,444 @@
+// qmb_capi.hip -- the extern "C" boundary of libqmb_hip.so (declared in include/qmb_hip.h).
+//
+// One-shot entry points mirror the reference torch ops (qmb/_hamiltonian.cpp:83-89): each one lays
+// the caller's workspace out into a compiled term table and a hash index over the config set it
+// needs, enqueues the builders and the kernel on the caller's stream, and returns.  Nothing here
+// allocates device memory or synchronises.
+#include <cstdio>
+#include <cstring>
+
+#include "qmb_internal.hpp"
+
+namespace qmb {
+
+namespace {
+thread_local std::string g_error;
+}
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+void clear_error() { g_error.clear(); }
+
+namespace {
+
+int check_ptr(const void* p, int64_t n, const char* name) {
+    if (n > 0 && p == nullptr) {
+        return set_error(QMB_ERR_ARGUMENT, "%s must be non-null when its size is positive", name);
+    }
+    if (n < 0) {
+        return set_error(QMB_ERR_ARGUMENT, "%s size must be non-negative", name);
+    }
+    return QMB_OK;
+}
+
+int check_particle_cut(int pc) {
+    if (pc != 1 && pc != 2) {
+        return set_error(QMB_ERR_UNSUPPORTED, "particle_cut must be 1 (fermi) or 2 (bose2), got %d", pc);
+    }
+    return QMB_OK;
+}
+
+size_t terms_bytes(int nq, int64_t n_terms) {
+    Carver cv;
+    qmb_terms t{};
+    if (terms_layout(cv, nq, n_terms, &t, nullptr, nullptr) != QMB_OK) {
+        return 0;
+    }
+    return cv.off + 256;
+}
+
+size_t targets_bytes(int nq, int64_t n) {
+    Carver cv;
+    qmb_targets t{};
+    if (targets_layout(cv, nq, n, &t) != QMB_OK) {
+        return 0;
+    }
+    return cv.off + 256;
+}
+
+// Splits a workspace into consecutive regions of the given sizes.
+struct Regions {
+    char* base;
+    size_t off;
+    size_t cap;
+    void* next(size_t bytes) {
+        off = (off + 255) & ~static_cast<size_t>(255);
+        void* p = base + off;
+        off += bytes;
+        return p;
+    }
+};
+
+int need_workspace(size_t need, size_t have, void* ws) {
+    if (need > have || (need > 0 && ws == nullptr)) {
+        return set_error(QMB_ERR_WORKSPACE, "workspace too small: need %zu bytes, have %zu", need, have);
+    }
+    return QMB_OK;
+}
+
+}  // namespace
+}  // namespace qmb
+
+using namespace qmb;
+
+extern "C" {
+
+const char* qmb_last_error(void) { return g_error.c_str(); }
+
+int qmb_version(void) { return 10000; }
+
+int qmb_validate_terms_host(int n_qubytes, const int16_t* site, const uint8_t* kind, int64_t n_terms, int64_t* n_bad) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    QMB_TRY(check_ptr(site, n_terms, "site"));
+    QMB_TRY(check_ptr(kind, n_terms, "kind"));
+    int64_t bad = 0;
+    const int n_bits = 8 * n_qubytes;
+    for (int64_t t = 0; t < n_terms; ++t) {
+        bool ok = true;
+        for (int op = 0; op < kMaxOps; ++op) {
+            const int k = kind[4 * t + op];
+            if (k > 2) {
+                ok = false;
+            } else if (k != 2 && (site[4 * t + op] < 0 || site[4 * t + op] >= n_bits)) {
+                ok = false;
+            }
+        }
+        bad += ok ? 0 : 1;
+    }
+    if (n_bad) {
+        *n_bad = bad;
+    }
+    return QMB_OK;
+}
+
+int qmb_compile_term_host(const int16_t* site4, const uint8_t* kind4, int particle_cut, int n_qubytes, uint64_t* rec,
+                          int32_t* status, int64_t* bucket, int32_t* diagonal) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    QMB_TRY(check_particle_cut(particle_cut));
+    const CompiledTerm c = compile_term(site4, kind4, particle_cut, 8 * n_qubytes);
+    if (rec) {
+        *rec = c.status == kCompileOk ? c.rec : kNeverSucceeds;
+    }
+    if (status) {
+        *status = c.status;
+    }
+    if (bucket) {
+        *bucket = c.status == kCompileOk ? bucket_of(c, 8 * n_qubytes) : -1;
+    }
+    if (diagonal) {
+        *diagonal = c.status == kCompileOk && c.diagonal ? 1 : 0;
+    }
+    return QMB_OK;
+}
+
+int qmb_apply_record_host(uint64_t rec, int n_qubytes, const uint8_t* config, uint8_t* result, int32_t* parity) {
+    clear_error();
+    if (check_qubytes(n_qubytes) != QMB_OK || (rec & kNeverSucceeds)) {
+        return 0;
+    }
+    Cfg<kMaxWords> c{};
+    for (int b = 0; b < n_qubytes; ++b) {
+        c.w[b >> 3] |= static_cast<uint64_t>(config[b]) << (8 * (b & 7));
+    }
+    Cfg<kMaxWords> o{};
+    int par = 0;
+    const bool ok = apply_rec<kMaxWords>(rec, c, o, par);
+    if (ok) {
+        for (int b = 0; b < n_qubytes; ++b) {
+            result[b] = static_cast<uint8_t>(o.w[b >> 3] >> (8 * (b & 7)));
+        }
+        if (parity) {
+            *parity = par;
+        }
+    }
+    return ok ? 1 : 0;
+}
+
+void qmb_philox4x32_10_host(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    U32x4 c{{ctr[0], ctr[1], ctr[2], ctr[3]}};
+    U32x4 r = philox4x32_10(c, key[0], key[1]);
+    for (int i = 0; i < 4; ++i) {
+        out[i] = r.v[i];
+    }
+}
+
+double qmb_philox_uniform_host(uint64_t seed, uint64_t index) { return philox_uniform(seed, index); }
+
+// ------------------------------------------------------------------------ prepared objects ----
+int qmb_terms_workspace_size(int n_qubytes, int64_t n_terms, size_t* bytes) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    if (!bytes) {
+        return set_error(QMB_ERR_ARGUMENT, "bytes must be non-null");
+    }
+    *bytes = terms_bytes(n_qubytes, n_terms);
+    return *bytes ? QMB_OK : QMB_ERR_ARGUMENT;
+}
+
+int qmb_terms_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
+                    void* buffer, size_t bytes, qmb_terms* out, void* stream) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    QMB_TRY(check_particle_cut(particle_cut));
+    QMB_TRY(check_ptr(site, n_terms, "site"));
+    QMB_TRY(check_ptr(kind, n_terms, "kind"));
+    QMB_TRY(check_ptr(coef, n_terms, "coef"));
+    if (!out || !buffer) {
+        return set_error(QMB_ERR_ARGUMENT, "buffer and out must be non-null");
+    }
+    return terms_build(n_qubytes, particle_cut, site, kind, coef, n_terms, buffer, bytes, out, as_stream(stream));
+}
+
+int qmb_targets_workspace_size(int n_qubytes, int64_t n, size_t* bytes) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    if (!bytes) {
+        return set_error(QMB_ERR_ARGUMENT, "bytes must be non-null");
+    }
+    *bytes = targets_bytes(n_qubytes, n);
+    return *bytes ? QMB_OK : QMB_ERR_ARGUMENT;
+}
+
+int qmb_targets_build(int n_qubytes, const uint8_t* configs, int64_t n, void* buffer, size_t bytes, qmb_targets* out,
+                      void* stream) {
+    clear_error();
+    QMB_TRY(check_ptr(configs, n, "configs"));
+    if (!out || !buffer) {
+        return set_error(QMB_ERR_ARGUMENT, "buffer and out must be non-null");
+    }
+    return targets_build(n_qubytes, configs, n, buffer, bytes, out, as_stream(stream));
+}
+
+int qmb_apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const uint8_t* configs_i, int64_t n_i, const double* in,
+                       double* out, int mode, const qmb_launch* launch, void* stream) {
+    clear_error();
+    QMB_TRY(check_ptr(configs_i, n_i, "configs_i"));
+    return apply_prepared(terms, targets, configs_i, n_i, in, out, mode, launch, as_stream(stream));
+}
+
+// ------------------------------------------------------------------------------ apply_within --
+int qmb_apply_within_workspace_size(int n_qubytes, int64_t n_i, int64_t n_j, int64_t n_terms, size_t* bytes) {
+    clear_error();
+    (void)n_i;
+    QMB_TRY(check_qubytes(n_qubytes));
+    if (!bytes) {
+        return set_error(QMB_ERR_ARGUMENT, "bytes must be non-null");
+    }
+    *bytes = terms_bytes(n_qubytes, n_terms) + targets_bytes(n_qubytes, n_j) + 512;
+    return QMB_OK;
+}
+
+static int apply_oneshot(int n_qubytes, int particle_cut, const uint8_t* configs_i, int64_t n_i, const uint8_t* configs_j,
+                         int64_t n_j, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
+                         const double* in, double* out, int mode, void* workspace, size_t workspace_bytes, void* stream) {
+    QMB_TRY(check_qubytes(n_qubytes));
+    QMB_TRY(check_particle_cut(particle_cut));
+    QMB_TRY(check_ptr(configs_i, n_i, "configs_i"));
+    QMB_TRY(check_ptr(configs_j, n_j, "configs_j"));
+    QMB_TRY(check_ptr(site, n_terms, "site"));
+    QMB_TRY(check_ptr(kind, n_terms, "kind"));
+    QMB_TRY(check_ptr(coef, n_terms, "coef"));
+    const size_t tb = terms_bytes(n_qubytes, n_terms);
+    const size_t gb = targets_bytes(n_qubytes, n_j);
+    QMB_TRY(need_workspace(tb + gb + 512, workspace_bytes, workspace));
+    hipStream_t s = as_stream(stream);
+    const int64_t n_out = mode == 0 ? n_j : n_i;
+    QMB_TRY(launch_zero_f64(out, 2 * n_out, s));
+    if (n_i == 0 || n_j == 0 || n_terms == 0) {
+        return QMB_OK;
+    }
+    Regions r{static_cast<char*>(workspace), 0, workspace_bytes};
+    qmb_terms terms{};
+    qmb_targets targets{};
+    QMB_TRY(terms_build(n_qubytes, particle_cut, site, kind, coef, n_terms, r.next(tb), tb, &terms, s));
+    QMB_TRY(targets_build(n_qubytes, configs_j, n_j, r.next(gb), gb, &targets, s));
+    return apply_prepared(&terms, &targets, configs_i, n_i, in, out, mode, nullptr, s);
+}
+
+int qmb_apply_within(int n_qubytes, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i,
+                     const uint8_t* configs_j, int64_t n_j, const int16_t* site, const uint8_t* kind, const double* coef,
+                     int64_t n_terms, double* psi_j, void* workspace, size_t workspace_bytes, void* stream) {
+    clear_error();
+    QMB_TRY(check_ptr(psi_i, n_i, "psi_i"));
+    QMB_TRY(check_ptr(psi_j, n_j, "psi_j"));
+    return apply_oneshot(n_qubytes, particle_cut, configs_i, n_i, configs_j, n_j, site, kind, coef, n_terms, psi_i, psi_j, 0,
+                         workspace, workspace_bytes, stream);
+}
+
+int qmb_apply_within_adjoint(int n_qubytes, int particle_cut, const uint8_t* configs_i, int64_t n_i, const uint8_t* configs_j,
+                             const double* grad_j, int64_t n_j, const int16_t* site, const uint8_t* kind, const double* coef,
+                             int64_t n_terms, double* grad_i, void* workspace, size_t workspace_bytes, void* stream) {
+    clear_error();
+    QMB_TRY(check_ptr(grad_j, n_j, "grad_j"));
+    QMB_TRY(check_ptr(grad_i, n_i, "grad_i"));
+    return apply_oneshot(n_qubytes, particle_cut, configs_i, n_i, configs_j, n_j, site, kind, coef, n_terms, grad_j, grad_i, 1,
+                         workspace, workspace_bytes, stream);
+}
+
+// ------------------------------------------------------------------------------------ plain --
+int qmb_apply_within_plain_workspace_size(int n_qubytes, int64_t n_j, size_t* bytes) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    if (!bytes) {
+        return set_error(QMB_ERR_ARGUMENT, "bytes must be non-null");
+    }
+    const int W = n_words_of(n_qubytes);
+    const size_t N = static_cast<size_t>(n_j > 0 ? n_j : 1);
+    *bytes = N * W * 8 + 256 + N * 4 + 256 + sort_configs_scratch_bytes(W, n_j) + 512;
+    return QMB_OK;
+}
+
+int qmb_apply_within_plain(int n_qubytes, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i,
+                           const uint8_t* configs_j, int64_t n_j, const int16_t* site, const uint8_t* kind, const double* coef,
+                           int64_t n_terms, double* psi_j, void* workspace, size_t workspace_bytes, void* stream) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    QMB_TRY(check_particle_cut(particle_cut));
+    QMB_TRY(check_ptr(configs_i, n_i, "configs_i"));
+    QMB_TRY(check_ptr(psi_i, n_i, "psi_i"));
+    QMB_TRY(check_ptr(configs_j, n_j, "configs_j"));
+    QMB_TRY(check_ptr(psi_j, n_j, "psi_j"));
+    QMB_TRY(check_ptr(site, n_terms, "site"));
+    size_t need = 0;
+    QMB_TRY(qmb_apply_within_plain_workspace_size(n_qubytes, n_j, &need));
+    QMB_TRY(need_workspace(need, workspace_bytes, workspace));
+    // The plain kernel reads raw (site, kind) ops, so reject sites it would mis-handle.
+    hipStream_t s = as_stream(stream);
+    QMB_TRY(launch_zero_f64(psi_j, 2 * n_j, s));
+    if (n_i == 0 || n_j == 0 || n_terms == 0) {
+        return QMB_OK;
+    }
+    const int W = n_words_of(n_qubytes);
+    const size_t N = static_cast<size_t>(n_j);
+    Regions r{static_cast<char*>(workspace), 0, workspace_bytes};
+    uint64_t* keys_be = static_cast<uint64_t*>(r.next(N * W * 8));
+    uint32_t* perm = static_cast<uint32_t*>(r.next(N * 4));
+    const size_t sb = sort_configs_scratch_bytes(W, n_j);
+    void* scratch = r.next(sb);
+    QMB_TRY(sort_configs(n_qubytes, configs_j, n_j, keys_be, perm, scratch, sb, s));
+    return apply_plain(n_qubytes, particle_cut, configs_i, psi_i, n_i, keys_be, perm, n_j, site, kind, coef, n_terms, psi_j, s);
+}
+
+// --------------------------------------------------------------------------- diagonal_term ----
+int qmb_diagonal_term_workspace_size(int n_qubytes, int64_t n, int64_t n_terms, size_t* bytes) {
+    clear_error();
+    (void)n;
+    QMB_TRY(check_qubytes(n_qubytes));
+    if (!bytes) {
+        return set_error(QMB_ERR_ARGUMENT, "bytes must be non-null");
+    }
+    *bytes = terms_bytes(n_qubytes, n_terms) + 256;
+    return QMB_OK;
+}
+
+int qmb_diagonal_term(int n_qubytes, int particle_cut, const uint8_t* configs, int64_t n, const int16_t* site, const uint8_t* kind,
+                      const double* coef, int64_t n_terms, double* out, void* workspace, size_t workspace_bytes, void* stream) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    QMB_TRY(check_particle_cut(particle_cut));
+    QMB_TRY(check_ptr(configs, n, "configs"));
+    QMB_TRY(check_ptr(out, n, "out"));
+    QMB_TRY(check_ptr(site, n_terms, "site"));
+    QMB_TRY(check_ptr(kind, n_terms, "kind"));
+    QMB_TRY(check_ptr(coef, n_terms, "coef"));
+    const size_t tb = terms_bytes(n_qubytes, n_terms);
+    QMB_TRY(need_workspace(tb, workspace_bytes, workspace));
+    hipStream_t s = as_stream(stream);
+    Regions r{static_cast<char*>(workspace), 0, workspace_bytes};
+    qmb_terms terms{};
+    QMB_TRY(terms_build(n_qubytes, particle_cut, site, kind, coef, n_terms, r.next(tb), tb, &terms, s));
+    return diagonal_prepared(&terms, configs, n, out, s);
+}
+
+// ------------------------------------------------------------------------- single_relative ----
+int qmb_single_relative_workspace_size(int n_qubytes, int64_t n, int64_t n_terms, size_t* bytes) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    if (!bytes) {
+        return set_error(QMB_ERR_ARGUMENT, "bytes must be non-null");
+    }
+    *bytes = targets_bytes(n_qubytes, n) + single_relative_scratch_bytes(n_terms) + 512;
+    return QMB_OK;
+}
+
+int qmb_single_relative(int n_qubytes, int particle_cut, const uint8_t* configs, int64_t n, const int16_t* site, const uint8_t* kind,
+                        const double* coef, int64_t n_terms, uint64_t seed, const double* keys, uint8_t* out, void* workspace,
+                        size_t workspace_bytes, void* stream) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    QMB_TRY(check_particle_cut(particle_cut));
+    QMB_TRY(check_ptr(configs, n, "configs"));
+    QMB_TRY(check_ptr(out, n, "out"));
+    QMB_TRY(check_ptr(site, n_terms, "site"));
+    QMB_TRY(check_ptr(kind, n_terms, "kind"));
+    QMB_TRY(check_ptr(coef, n_terms, "coef"));
+    const size_t gb = targets_bytes(n_qubytes, n);
+    const size_t sb = single_relative_scratch_bytes(n_terms);
+    QMB_TRY(need_workspace(gb + sb + 512, workspace_bytes, workspace));
+    hipStream_t s = as_stream(stream);
+    Regions r{static_cast<char*>(workspace), 0, workspace_bytes};
+    qmb_targets exclude{};
+    QMB_TRY(targets_build(n_qubytes, configs, n, r.next(gb), gb, &exclude, s));
+    return single_relative_prepared(n_qubytes, particle_cut, configs, n, site, kind, coef, n_terms, seed, keys, &exclude, out,
+                                    r.next(sb), sb, s);
+}
+
+// --------------------------------------------------------------------------- find_relative ----
+int qmb_find_relative_workspace_size(int n_qubytes, int64_t n_i, int64_t n_exclude, int64_t n_terms, int64_t count_selected,
+                                     size_t* bytes) {
+    clear_error();
+    (void)n_i;
+    QMB_TRY(check_qubytes(n_qubytes));
+    if (!bytes) {
+        return set_error(QMB_ERR_ARGUMENT, "bytes must be non-null");
+    }
+    if (count_selected <= 0) {
+        return set_error(QMB_ERR_ARGUMENT, "count_selected must be positive");
+    }
+    *bytes = terms_bytes(n_qubytes, n_terms) + targets_bytes(n_qubytes, n_exclude) +
+             find_relative_scratch_bytes(n_qubytes, count_selected) + 768;
+    return QMB_OK;
+}
+
+int qmb_find_relative(int n_qubytes, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i,
+                      int64_t count_selected, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
+                      const uint8_t* configs_exclude, int64_t n_exclude, uint8_t* out, int64_t* out_rows, void* workspace,
+                      size_t workspace_bytes, void* stream) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    QMB_TRY(check_particle_cut(particle_cut));
+    QMB_TRY(check_ptr(configs_i, n_i, "configs_i"));
+    QMB_TRY(check_ptr(psi_i, n_i, "psi_i"));
+    QMB_TRY(check_ptr(site, n_terms, "site"));
+    QMB_TRY(check_ptr(kind, n_terms, "kind"));
+    QMB_TRY(check_ptr(coef, n_terms, "coef"));
+    QMB_TRY(check_ptr(configs_exclude, n_exclude, "configs_exclude"));
+    if (!out || !out_rows) {
+        return set_error(QMB_ERR_ARGUMENT, "out and out_rows must be non-null");
+    }
+    size_t need = 0;
+    QMB_TRY(qmb_find_relative_workspace_size(n_qubytes, n_i, n_exclude, n_terms, count_selected, &need));
+    QMB_TRY(need_workspace(need, workspace_bytes, workspace));
+    hipStream_t s = as_stream(stream);
+    const size_t tb = terms_bytes(n_qubytes, n_terms);
+    const size_t gb = targets_bytes(n_qubytes, n_exclude);
+    const size_t fb = find_relative_scratch_bytes(n_qubytes, count_selected);
+    Regions r{static_cast<char*>(workspace), 0, workspace_bytes};
+    qmb_terms terms{};
+    qmb_targets exclude{};
+    QMB_TRY(terms_build(n_qubytes, particle_cut, site, kind, coef, n_terms, r.next(tb), tb, &terms, s));
+    QMB_TRY(targets_build(n_qubytes, configs_exclude, n_exclude, r.next(gb), gb, &exclude, s));
+    return find_relative_prepared(&terms, &exclude, configs_i, psi_i, n_i, count_selected, out, out_rows, r.next(fb), fb, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,500 @@
+// qmb_find_relative.hip -- find_relative as a deterministic radix select on gfx950.
+//
+// The reference keeps a K-slot min-heap of (weight, config) entries, one entry per contribution
+// |sign * coef * psi_i|^2 whose target is not excluded (qmb/_hamiltonian_cpu.cpp:341-405), guarded
+// on the GPU by per-node spin locks with racy pre-reads (qmb/_hamiltonian_cuda.cu:276-427); then it
+// drops zero weights and returns the distinct configs sorted by bytes (:537-545).
+//
+// Here the K largest contributions are found without locks.  Positive doubles order like their bit
+// patterns, so the K-th largest weight is located digit by digit (12-bit digits of the weight bits):
+// every pass re-runs the enumeration and histograms the next digit of the entries still tied with
+// the threshold, and a single-thread kernel picks the digit at which the running count reaches K.
+// Once the tied class fits the candidate buffer, one more pass collects every entry strictly above
+// the threshold prefix and the tied candidates; the candidates are ordered by (weight desc, config
+// asc) and the first K - above are kept.  All control lives in device memory, so the whole call is
+// enqueued without host synchronisation.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "qmb_enum.hpp"
+
+namespace qmb {
+
+namespace {
+
+constexpr int kHistBits = 12;
+constexpr int kHistBins = 1 << kHistBits;
+constexpr int kLevels = 6;  // digits [51,63) [39,51) [27,39) [15,27) [3,15) [0,3)
+
+__host__ __device__ inline int fr_shift(int level) { return level < 5 ? 51 - 12 * level : 0; }
+__host__ __device__ inline int fr_width(int level) { return level < 5 ? 12 : 3; }
+
+inline unsigned blocks_for(int64_t n, int block = 256) {
+    int64_t b = (n + block - 1) / block;
+    return static_cast<unsigned>(b < 1 ? 1 : b);
+}
+
+struct FrState {
+    int64_t k_rem;
+    uint64_t prefix;
+    int32_t level;
+    int32_t shift;
+    int32_t done;
+    int32_t all;
+    unsigned long long n_above;
+    unsigned long long n_cand;
+    int64_t cand_needed;
+};
+
+struct FrParams {
+    EnumSource src;
+    const double2* psi;
+    const double2* coefs;
+    const uint64_t* xkeys;
+    const uint64_t* xtable;
+    uint64_t xmask;
+    FrState* state;
+    unsigned long long* hist;
+    uint64_t* sel_c;   // [K][W] selected configs ("above" entries, then the kept candidates)
+    uint64_t* cand_w;  // [cap]
+    uint64_t* cand_c;  // [cap][W]
+    int64_t k;
+    int64_t cand_cap;
+    int tile_log2;
+    int groups;
+};
+
+// |sign * coef * psi|^2 with every product and sum rounded separately (no contraction), so the
+// bits match a plain IEEE evaluation of the reference expression (qmb/_hamiltonian_cpu.cpp:388-392).
+__device__ __forceinline__ uint64_t weight_bits(const double2 cf, const double2 v) {
+    const double re = __dsub_rn(__dmul_rn(cf.x, v.x), __dmul_rn(cf.y, v.y));
+    const double im = __dadd_rn(__dmul_rn(cf.x, v.y), __dmul_rn(cf.y, v.x));
+    const double w = __dadd_rn(__dmul_rn(re, re), __dmul_rn(im, im));
+    return static_cast<uint64_t>(__double_as_longlong(w));
+}
+
+template <int W>
+struct HistAction {
+    const FrParams& p;
+    uint32_t* hist;
+    uint64_t prefix;
+    int shift;
+    int width;
+    double2 v;
+    __device__ HistAction(const FrParams& pp, uint32_t* h, uint64_t pre, int sh, int wd)
+        : p(pp), hist(h), prefix(pre), shift(sh), width(wd), v(make_double2(0.0, 0.0)) {}
+    __device__ __forceinline__ void begin_member(int64_t i) { v = p.psi[i]; }
+    __device__ __forceinline__ void on_success(int64_t, uint32_t t, const Cfg<W>& o, int) {
+        const uint64_t wb = weight_bits(p.coefs[t], v);
+        if (wb == 0 || (wb >> (shift + width)) != prefix) {
+            return;
+        }
+        if (lookup<W>(p.xtable, p.xmask, p.xkeys, o) >= 0) {
+            return;
+        }
+        atomicAdd(&hist[(wb >> shift) & ((1u << width) - 1u)], 1u);
+    }
+    __device__ __forceinline__ void end_member(int64_t) {}
+};
+
+template <int W>
+struct CollectAction {
+    const FrParams& p;
+    uint64_t prefix;
+    int shift;
+    bool all;
+    double2 v;
+    __device__ CollectAction(const FrParams& pp, uint64_t pre, int sh, bool a)
+        : p(pp), prefix(pre), shift(sh), all(a), v(make_double2(0.0, 0.0)) {}
+    __device__ __forceinline__ void begin_member(int64_t i) { v = p.psi[i]; }
+    __device__ __forceinline__ void on_success(int64_t, uint32_t t, const Cfg<W>& o, int) {
+        const uint64_t wb = weight_bits(p.coefs[t], v);
+        if (wb == 0) {
+            return;
+        }
+        const uint64_t top = all ? 1 : (wb >> shift);
+        const uint64_t pre = all ? 0 : prefix;
+        if (top < pre) {
+            return;
+        }
+        if (lookup<W>(p.xtable, p.xmask, p.xkeys, o) >= 0) {
+            return;
+        }
+        if (top > pre) {
+            const unsigned long long slot = atomicAdd(&p.state->n_above, 1ull);
+            if (slot < static_cast<unsigned long long>(p.k)) {
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    p.sel_c[slot * W + w] = o.w[w];
+                }
+            }
+        } else {
+            const unsigned long long slot = atomicAdd(&p.state->n_cand, 1ull);
+            if (slot < static_cast<unsigned long long>(p.cand_cap)) {
+                p.cand_w[slot] = wb;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    p.cand_c[slot * W + w] = o.w[w];
+                }
+            }
+        }
+    }
+    __device__ __forceinline__ void end_member(int64_t) {}
+};
+
+template <int W>
+__global__ __launch_bounds__(kEnumBlock) void fr_hist_kernel(FrParams p) {
+    extern __shared__ uint64_t smem[];
+    __shared__ uint32_t s_hist[kHistBins];
+    if (p.state->done) {
+        return;  // uniform: every thread reads the same flag
+    }
+    const int level = p.state->level;
+    const int shift = fr_shift(level);
+    const int width = fr_width(level);
+    const uint64_t prefix = p.state->prefix;
+    for (int b = threadIdx.x; b < kHistBins; b += kEnumBlock) {
+        s_hist[b] = 0;
+    }
+    __syncthreads();
+    HistAction<W> act(p, s_hist, prefix, shift, width);
+    enumerate_tile<W>(p.src, blockIdx.x / p.groups, p.tile_log2, blockIdx.x % p.groups, p.groups, smem, act);
+    __syncthreads();
+    for (int b = threadIdx.x; b < (1 << width); b += kEnumBlock) {
+        if (s_hist[b]) {
+            atomicAdd(&p.hist[b], static_cast<unsigned long long>(s_hist[b]));
+        }
+    }
+}
+
+template <int W>
+__global__ __launch_bounds__(kEnumBlock) void fr_collect_kernel(FrParams p) {
+    extern __shared__ uint64_t smem[];
+    CollectAction<W> act(p, p.state->prefix, p.state->shift, p.state->all != 0);
+    enumerate_tile<W>(p.src, blockIdx.x / p.groups, p.tile_log2, blockIdx.x % p.groups, p.groups, smem, act);
+}
+
+__global__ void fr_init_kernel(FrState* st, unsigned long long* hist, int64_t k) {
+    for (int b = threadIdx.x; b < kHistBins; b += blockDim.x) {
+        hist[b] = 0;
+    }
+    if (threadIdx.x == 0) {
+        st->k_rem = k;
+        st->prefix = 0;
+        st->level = 0;
+        st->shift = 0;
+        st->done = 0;
+        st->all = 0;
+        st->n_above = 0;
+        st->n_cand = 0;
+        st->cand_needed = 0;
+    }
+}
+
+// Chooses the threshold digit of the current level (single thread; 4096 bins).
+__global__ void fr_select_kernel(FrState* st, unsigned long long* hist, int64_t cand_cap) {
+    if (threadIdx.x != 0 || st->done) {
+        return;
+    }
+    const int level = st->level;
+    const int width = fr_width(level);
+    const int bins = 1 << width;
+    unsigned long long total = 0;
+    for (int b = 0; b < bins; ++b) {
+        total += hist[b];
+    }
+    const unsigned long long need = static_cast<unsigned long long>(st->k_rem);
+    if (level == 0 && total <= need) {
+        st->all = 1;
+        st->done = 1;
+        st->cand_needed = 0;
+    } else {
+        unsigned long long cum = 0;
+        int d = bins - 1;
+        for (; d > 0; --d) {
+            if (cum + hist[d] >= need) {
+                break;
+            }
+            cum += hist[d];
+        }
+        st->k_rem = static_cast<int64_t>(need - cum);
+        st->prefix = (st->prefix << width) | static_cast<uint64_t>(d);
+        st->shift = fr_shift(level);
+        if (hist[d] <= static_cast<unsigned long long>(cand_cap) || level == kLevels - 1) {
+            st->done = 1;
+            st->cand_needed = st->k_rem;
+        } else {
+            st->level = level + 1;
+        }
+    }
+    for (int b = 0; b < bins; ++b) {
+        hist[b] = 0;
+    }
+}
+
+template <int W>
+__global__ void be_word_gather_kernel(const uint64_t* __restrict__ words, const uint32_t* __restrict__ perm, int64_t n, int w,
+                                      uint64_t* __restrict__ out) {
+    const int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (p < n) {
+        out[p] = bswap64(words[static_cast<int64_t>(perm[p]) * W + w]);
+    }
+}
+
+__global__ void u64_gather_kernel(const uint64_t* __restrict__ v, const uint32_t* __restrict__ perm, int64_t n,
+                                  uint64_t* __restrict__ out) {
+    const int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (p < n) {
+        out[p] = v[perm[p]];
+    }
+}
+
+__global__ void valid_key_kernel(const uint32_t* __restrict__ perm, int64_t n, const FrState* __restrict__ st, int64_t k,
+                                 uint64_t* __restrict__ out) {
+    const int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (p < n) {
+        const unsigned long long above = st->n_above < static_cast<unsigned long long>(k) ? st->n_above : k;
+        const int64_t n_sel = static_cast<int64_t>(above) + st->cand_needed;
+        out[p] = static_cast<int64_t>(perm[p]) < n_sel ? 0ull : 1ull;
+    }
+}
+
+__global__ void iota_u32_kernel(uint32_t* p, int64_t n) {
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i < n) {
+        p[i] = static_cast<uint32_t>(i);
+    }
+}
+
+// Appends the first cand_needed candidates (in sorted order) after the "above" entries.
+template <int W>
+__global__ void fr_take_kernel(const uint32_t* __restrict__ perm, int64_t cap, const uint64_t* __restrict__ cand_c,
+                               const FrState* __restrict__ st, int64_t k, uint64_t* __restrict__ sel_c) {
+    const int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    const unsigned long long above = st->n_above < static_cast<unsigned long long>(k) ? st->n_above : k;
+    const int64_t found = static_cast<int64_t>(st->n_cand < static_cast<unsigned long long>(cap) ? st->n_cand : cap);
+    const int64_t need = st->cand_needed < found ? st->cand_needed : found;
+    if (p < cap && p < need && static_cast<int64_t>(above) + p < k) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            sel_c[(above + p) * W + w] = cand_c[static_cast<int64_t>(perm[p]) * W + w];
+        }
+    }
+}
+
+template <int W>
+__global__ void fr_unique_flag_kernel(const uint32_t* __restrict__ perm, int64_t n, const uint64_t* __restrict__ sel_c,
+                                      const FrState* __restrict__ st, int64_t k, uint32_t* __restrict__ flag) {
+    const int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (p >= n) {
+        return;
+    }
+    const unsigned long long above = st->n_above < static_cast<unsigned long long>(k) ? st->n_above : k;
+    const int64_t found = static_cast<int64_t>(st->n_cand);
+    const int64_t need = st->cand_needed < found ? st->cand_needed : found;
+    int64_t n_sel = static_cast<int64_t>(above) + need;
+    if (n_sel > k) {
+        n_sel = k;
+    }
+    const int64_t a = perm[p];
+    uint32_t f = a < n_sel ? 1u : 0u;
+    if (f && p > 0) {
+        const int64_t b = perm[p - 1];
+        bool eq = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            eq = eq && (sel_c[a * W + w] == sel_c[b * W + w]);
+        }
+        if (eq) {
+            f = 0u;
+        }
+    }
+    flag[p] = f;
+}
+
+template <int W>
+__global__ void fr_scatter_kernel(const uint32_t* __restrict__ perm, int64_t n, const uint64_t* __restrict__ sel_c,
+                                  const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, int nq,
+                                  uint8_t* __restrict__ out, int64_t* __restrict__ out_rows) {
+    const int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (p >= n) {
+        return;
+    }
+    if (flag[p]) {
+        Cfg<W> c;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            c.w[w] = sel_c[static_cast<int64_t>(perm[p]) * W + w];
+        }
+        store_row<W>(out + static_cast<int64_t>(pos[p]) * nq, nq, c);
+    }
+    if (p == n - 1) {
+        *out_rows = static_cast<int64_t>(pos[p]) + flag[p];
+    }
+}
+
+int64_t cand_capacity(int64_t k) {
+    int64_t cap = 2 * k;
+    return cap < (1 << 20) ? (1 << 20) : cap;
+}
+
+struct FrScratch {
+    FrState* state;
+    unsigned long long* hist;
+    uint64_t* sel_c;
+    uint64_t* cand_w;
+    uint64_t* cand_c;
+    uint32_t* perm_a;
+    uint32_t* perm_b;
+    uint64_t* key_a;
+    uint64_t* key_b;
+    uint32_t* flag;
+    uint32_t* pos;
+    void* tmp;
+    size_t tmp_bytes;
+};
+
+FrScratch fr_layout(Carver& cv, int W, int64_t k) {
+    const int64_t cap = cand_capacity(k);
+    const size_t M = static_cast<size_t>(cap > k ? cap : k);
+    FrScratch s{};
+    s.state = cv.take<FrState>(1);
+    s.hist = cv.take<unsigned long long>(kHistBins);
+    s.sel_c = cv.take<uint64_t>(static_cast<size_t>(k) * W);
+    s.cand_w = cv.take<uint64_t>(static_cast<size_t>(cap));
+    s.cand_c = cv.take<uint64_t>(static_cast<size_t>(cap) * W);
+    s.perm_a = cv.take<uint32_t>(M);
+    s.perm_b = cv.take<uint32_t>(M);
+    s.key_a = cv.take<uint64_t>(M);
+    s.key_b = cv.take<uint64_t>(M);
+    s.flag = cv.take<uint32_t>(static_cast<size_t>(k));
+    s.pos = cv.take<uint32_t>(static_cast<size_t>(k));
+    size_t sort_bytes = 0, sort_desc_bytes = 0, scan_bytes = 0;
+    (void)rocprim::radix_sort_pairs(nullptr, sort_bytes, s.key_a, s.key_b, s.perm_a, s.perm_b, static_cast<unsigned>(M));
+    (void)rocprim::radix_sort_pairs_desc(nullptr, sort_desc_bytes, s.key_a, s.key_b, s.perm_a, s.perm_b, static_cast<unsigned>(M));
+    (void)rocprim::exclusive_scan(nullptr, scan_bytes, s.flag, s.pos, 0u, static_cast<size_t>(k), rocprim::plus<uint32_t>());
+    size_t t = sort_bytes > sort_desc_bytes ? sort_bytes : sort_desc_bytes;
+    t = t > scan_bytes ? t : scan_bytes;
+    s.tmp = cv.take_bytes(t);
+    s.tmp_bytes = t;
+    return s;
+}
+
+// Sorts perm (an index permutation over n entries) stably by config words ascending.
+template <int W>
+int sort_by_config(const uint64_t* words, int64_t n, FrScratch& s, hipStream_t stream) {
+    for (int w = W - 1; w >= 0; --w) {
+        be_word_gather_kernel<W><<<blocks_for(n), 256, 0, stream>>>(words, s.perm_a, n, w, s.key_a);
+        QMB_CHECK_LAUNCH();
+        size_t need = s.tmp_bytes;
+        QMB_HIP_TRY(rocprim::radix_sort_pairs(s.tmp, need, s.key_a, s.key_b, s.perm_a, s.perm_b, static_cast<unsigned>(n), 0, 64,
+                                              stream));
+        QMB_HIP_TRY(hipMemcpyAsync(s.perm_a, s.perm_b, static_cast<size_t>(n) * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+    }
+    return QMB_OK;
+}
+
+template <int W>
+int find_relative_impl(const qmb_terms* terms, const qmb_targets* exclude, const uint8_t* configs_i, const double* psi_i,
+                       int64_t n_i, int64_t k, uint8_t* out, int64_t* out_rows, void* scratch, size_t scratch_bytes,
+                       hipStream_t stream) {
+    Carver cv{static_cast<char*>(scratch), 0};
+    FrScratch s = fr_layout(cv, W, k);
+    if (cv.off > scratch_bytes) {
+        return set_error(QMB_ERR_WORKSPACE, "find_relative scratch too small: need %zu, have %zu", cv.off, scratch_bytes);
+    }
+    const int64_t cap = cand_capacity(k);
+    const LaunchShape ls = resolve_launch(nullptr, W, n_i);
+    FrParams p{};
+    p.src = make_source(terms, configs_i, n_i);
+    p.psi = reinterpret_cast<const double2*>(psi_i);
+    p.coefs = reinterpret_cast<const double2*>(terms->coefs);
+    p.xkeys = exclude->keys;
+    p.xtable = exclude->table;
+    p.xmask = exclude->capacity - 1;
+    p.state = s.state;
+    p.hist = s.hist;
+    p.sel_c = s.sel_c;
+    p.cand_w = s.cand_w;
+    p.cand_c = s.cand_c;
+    p.k = k;
+    p.cand_cap = cap;
+    p.tile_log2 = ls.tile_log2;
+    p.groups = ls.groups;
+    const int64_t tiles = (n_i + (1ll << ls.tile_log2) - 1) >> ls.tile_log2;
+    const int64_t grid = n_i > 0 ? tiles * ls.groups : 0;
+    const size_t lds = enum_lds_bytes_of(W, terms->n_bits, ls.tile_log2);
+
+    fr_init_kernel<<<1, 256, 0, stream>>>(s.state, s.hist, k);
+    QMB_CHECK_LAUNCH();
+    QMB_TRY(launch_fill_u64(s.cand_w, 0ull, cap, stream));
+    if (grid > 0 && terms->n_terms > 0) {
+        QMB_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&fr_hist_kernel<W>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        QMB_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&fr_collect_kernel<W>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        for (int level = 0; level < kLevels; ++level) {
+            fr_hist_kernel<W><<<static_cast<unsigned>(grid), kEnumBlock, lds, stream>>>(p);
+            QMB_CHECK_LAUNCH();
+            fr_select_kernel<<<1, 64, 0, stream>>>(s.state, s.hist, cap);
+            QMB_CHECK_LAUNCH();
+        }
+        fr_collect_kernel<W><<<static_cast<unsigned>(grid), kEnumBlock, lds, stream>>>(p);
+        QMB_CHECK_LAUNCH();
+    }
+    // Candidates: order by (weight desc, config asc), keep the first cand_needed.
+    iota_u32_kernel<<<blocks_for(cap), 256, 0, stream>>>(s.perm_a, cap);
+    QMB_CHECK_LAUNCH();
+    QMB_TRY(sort_by_config<W>(s.cand_c, cap, s, stream));
+    u64_gather_kernel<<<blocks_for(cap), 256, 0, stream>>>(s.cand_w, s.perm_a, cap, s.key_a);
+    QMB_CHECK_LAUNCH();
+    size_t need = s.tmp_bytes;
+    QMB_HIP_TRY(rocprim::radix_sort_pairs_desc(s.tmp, need, s.key_a, s.key_b, s.perm_a, s.perm_b, static_cast<unsigned>(cap), 0, 64,
+                                               stream));
+    fr_take_kernel<W><<<blocks_for(cap), 256, 0, stream>>>(s.perm_b, cap, s.cand_c, s.state, k, s.sel_c);
+    QMB_CHECK_LAUNCH();
+    // Selected entries: sort by config (valid entries first), drop repeats, write rows.
+    iota_u32_kernel<<<blocks_for(k), 256, 0, stream>>>(s.perm_a, k);
+    QMB_CHECK_LAUNCH();
+    QMB_TRY(sort_by_config<W>(s.sel_c, k, s, stream));
+    valid_key_kernel<<<blocks_for(k), 256, 0, stream>>>(s.perm_a, k, s.state, k, s.key_a);
+    QMB_CHECK_LAUNCH();
+    need = s.tmp_bytes;
+    QMB_HIP_TRY(rocprim::radix_sort_pairs(s.tmp, need, s.key_a, s.key_b, s.perm_a, s.perm_b, static_cast<unsigned>(k), 0, 1, stream));
+    fr_unique_flag_kernel<W><<<blocks_for(k), 256, 0, stream>>>(s.perm_b, k, s.sel_c, s.state, k, s.flag);
+    QMB_CHECK_LAUNCH();
+    need = s.tmp_bytes;
+    QMB_HIP_TRY(rocprim::exclusive_scan(s.tmp, need, s.flag, s.pos, 0u, static_cast<size_t>(k), rocprim::plus<uint32_t>(), stream));
+    fr_scatter_kernel<W><<<blocks_for(k), 256, 0, stream>>>(s.perm_b, k, s.sel_c, s.flag, s.pos, terms->n_qubytes, out, out_rows);
+    QMB_CHECK_LAUNCH();
+    return QMB_OK;
+}
+
+}  // namespace
+
+size_t find_relative_scratch_bytes(int n_qubytes, int64_t count_selected) {
+    Carver cv;
+    (void)fr_layout(cv, n_words_of(n_qubytes), count_selected);
+    return cv.off + 256;
+}
+
+int find_relative_prepared(const qmb_terms* terms, const qmb_targets* exclude, const uint8_t* configs_i, const double* psi_i,
+                           int64_t n_i, int64_t count_selected, uint8_t* out, int64_t* out_rows, void* scratch,
+                           size_t scratch_bytes, hipStream_t stream) {
+    if (count_selected <= 0) {
+        return set_error(QMB_ERR_ARGUMENT, "count_selected must be positive, got %lld", static_cast<long long>(count_selected));
+    }
+    if (count_selected >= (1ll << 31)) {
+        return set_error(QMB_ERR_ARGUMENT, "count_selected too large");
+    }
+    if (terms->n_qubytes != exclude->n_qubytes) {
+        return set_error(QMB_ERR_ARGUMENT, "exclude set has a different config width");
+    }
+    return dispatch_words(terms->n_words, [&](auto wc) -> int {
+        constexpr int W = decltype(wc)::value;
+        return find_relative_impl<W>(terms, exclude, configs_i, psi_i, n_i, count_selected, out, out_rows, scratch, scratch_bytes,
+                                     stream);
+    });
+}
+
+}  // namespace qmb

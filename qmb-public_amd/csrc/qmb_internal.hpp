@@ -1,0 +1,137 @@
+// qmb_internal.hpp -- declarations shared by the qmb_hip translation units (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <type_traits>
+
+#include "qmb_common.hpp"
+#include "qmb_hip.h"
+
+namespace qmb {
+
+// ---------------------------------------------------------------------------------------------
+// Errors: the C ABI returns a code and keeps the message per host thread.
+// ---------------------------------------------------------------------------------------------
+int set_error(int code, const char* fmt, ...);
+void clear_error();
+
+#define QMB_HIP_TRY(expr)                                                                                  \
+    do {                                                                                                   \
+        hipError_t qmb_e_ = (expr);                                                                        \
+        if (qmb_e_ != hipSuccess) {                                                                        \
+            return ::qmb::set_error(QMB_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(qmb_e_), \
+                                    __FILE__, __LINE__);                                                   \
+        }                                                                                                  \
+    } while (0)
+
+#define QMB_CHECK_LAUNCH() QMB_HIP_TRY(hipGetLastError())
+
+#define QMB_TRY(expr)             \
+    do {                          \
+        int qmb_rc_ = (expr);     \
+        if (qmb_rc_ != QMB_OK) {  \
+            return qmb_rc_;       \
+        }                         \
+    } while (0)
+
+// ---------------------------------------------------------------------------------------------
+// Workspace carving.  The same routine runs with base == nullptr to size a workspace and with the
+// caller's buffer to lay it out, so sizes and layouts cannot drift apart.
+// ---------------------------------------------------------------------------------------------
+struct Carver {
+    char* base = nullptr;
+    size_t off = 0;
+
+    template <typename T>
+    T* take(size_t count) {
+        off = (off + 255) & ~static_cast<size_t>(255);
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += count * sizeof(T);
+        return p;
+    }
+    void* take_bytes(size_t bytes) { return take<char>(bytes); }
+};
+
+inline int n_words_of(int n_qubytes) { return (n_qubytes + 7) / 8; }
+
+inline uint64_t next_pow2(uint64_t x) {
+    uint64_t p = 1;
+    while (p < x) {
+        p <<= 1;
+    }
+    return p;
+}
+
+inline int check_qubytes(int n_qubytes) {
+    if (n_qubytes < 1 || n_qubytes > kMaxQubytes) {
+        return set_error(QMB_ERR_UNSUPPORTED, "n_qubytes must be in [1, %d], got %d", kMaxQubytes, n_qubytes);
+    }
+    return QMB_OK;
+}
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Launch geometry of the enumeration kernels.
+constexpr int kEnumThreads = 256;
+
+struct LaunchShape {
+    int tile_log2;
+    int groups;
+};
+
+LaunchShape resolve_launch(const qmb_launch* launch, int n_words, int64_t n_i);
+size_t enum_lds_bytes(int n_words, int n_bits, int tile_log2);
+
+// ---------------------------------------------------------------------------------------------
+// Builders (qmb_build.hip).
+// ---------------------------------------------------------------------------------------------
+int terms_layout(Carver& cv, int n_qubytes, int64_t n_terms, qmb_terms* out, void** scratch, size_t* scratch_bytes);
+int terms_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
+                int64_t n_terms, void* buffer, size_t bytes, qmb_terms* out, hipStream_t stream);
+int targets_layout(Carver& cv, int n_qubytes, int64_t n, qmb_targets* out);
+int targets_build(int n_qubytes, const uint8_t* configs, int64_t n, void* buffer, size_t bytes, qmb_targets* out,
+                  hipStream_t stream);
+
+// Sorting of config rows by bytes (radix over byte-swapped words).  keys_be: [n][W] output in
+// sorted order, perm: original row of each sorted position.
+size_t sort_configs_scratch_bytes(int n_words, int64_t n);
+int sort_configs(int n_qubytes, const uint8_t* configs, int64_t n, uint64_t* keys_be, uint32_t* perm, void* scratch,
+                 size_t scratch_bytes, hipStream_t stream);
+
+// Original-order compiled records (for single_relative), top bit set for never-succeeding terms.
+constexpr uint64_t kNeverSucceeds = 1ull << 63;
+int compile_records_original(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, int64_t n_terms,
+                             uint64_t* orecs, hipStream_t stream);
+
+// ---------------------------------------------------------------------------------------------
+// Kernels launchers (qmb_apply.hip, qmb_find_relative.hip).
+// ---------------------------------------------------------------------------------------------
+int launch_fill_u64(uint64_t* p, uint64_t value, int64_t n, hipStream_t stream);
+int launch_zero_f64(double* p, int64_t n, hipStream_t stream);
+
+int apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const uint8_t* configs_i, int64_t n_i,
+                   const double* in, double* out, int mode, const qmb_launch* launch, hipStream_t stream);
+
+int apply_plain(int n_qubytes, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i,
+                const uint64_t* sorted_keys_be, const uint32_t* perm, int64_t n_j, const int16_t* site, const uint8_t* kind,
+                const double* coef, int64_t n_terms, double* psi_j, hipStream_t stream);
+
+int diagonal_prepared(const qmb_terms* terms, const uint8_t* configs, int64_t n, double* out, hipStream_t stream);
+
+int single_relative_prepared(int n_qubytes, int particle_cut, const uint8_t* configs, int64_t n, const int16_t* site,
+                             const uint8_t* kind, const double* coef, int64_t n_terms, uint64_t seed, const double* keys,
+                             const qmb_targets* exclude, uint8_t* out, void* scratch, size_t scratch_bytes,
+                             hipStream_t stream);
+size_t single_relative_scratch_bytes(int64_t n_terms);
+
+size_t find_relative_scratch_bytes(int n_qubytes, int64_t count_selected);
+int find_relative_prepared(const qmb_terms* terms, const qmb_targets* exclude, const uint8_t* configs_i, const double* psi_i,
+                           int64_t n_i, int64_t count_selected, uint8_t* out, int64_t* out_rows, void* scratch,
+                           size_t scratch_bytes, hipStream_t stream);
+
+}  // namespace qmb

@@ -1,0 +1,142 @@
+"""
+ctypes binding of libqmb_hip.so, the C ABI declared in include/qmb_hip.h.
+
+The library is built in-tree (``make -C qmb-public_amd/csrc``, or ``__graft_entry__.build()``) and
+loaded from this directory.  There is no fallback: if the library is missing or fails to load, every
+entry point raises, so a GPU run can never silently compute on another path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+
+_HERE = pathlib.Path(__file__).resolve().parent
+LIBRARY_PATH = pathlib.Path(os.environ.get("QMB_HIP_LIBRARY", str(_HERE / "libqmb_hip.so")))
+
+QMB_OK = 0
+
+_c_int = ctypes.c_int
+_c_i32 = ctypes.c_int32
+_c_i64 = ctypes.c_int64
+_c_u64 = ctypes.c_uint64
+_c_size = ctypes.c_size_t
+_vp = ctypes.c_void_p
+
+
+class QmbTerms(ctypes.Structure):
+    """Mirror of ``struct qmb_terms`` (include/qmb_hip.h)."""
+
+    _fields_ = [
+        ("n_qubytes", _c_i32),
+        ("particle_cut", _c_i32),
+        ("n_words", _c_i32),
+        ("n_bits", _c_i32),
+        ("n_terms", _c_i64),
+        ("n_buckets", _c_i64),
+        ("buffer", _vp),
+        ("bytes", _c_size),
+        ("recs", _vp),
+        ("coefs", _vp),
+        ("tidx", _vp),
+        ("bucket_start", _vp),
+        ("drecs", _vp),
+        ("dcoefs", _vp),
+        ("counts", _vp),
+    ]
+
+
+class QmbTargets(ctypes.Structure):
+    """Mirror of ``struct qmb_targets`` (include/qmb_hip.h)."""
+
+    _fields_ = [
+        ("n_qubytes", _c_i32),
+        ("n_words", _c_i32),
+        ("n", _c_i64),
+        ("capacity", _c_u64),
+        ("buffer", _vp),
+        ("bytes", _c_size),
+        ("keys", _vp),
+        ("table", _vp),
+    ]
+
+
+class QmbLaunch(ctypes.Structure):
+    """Mirror of ``struct qmb_launch`` (include/qmb_hip.h)."""
+
+    _fields_ = [("tile", _c_i32), ("groups", _c_i32)]
+
+
+# name -> (restype, argtypes); kept in the order of include/qmb_hip.h
+SIGNATURES: dict[str, tuple[object, list[object]]] = {
+    "qmb_last_error": (ctypes.c_char_p, []),
+    "qmb_version": (_c_int, []),
+    "qmb_validate_terms_host": (_c_int, [_c_int, _vp, _vp, _c_i64, ctypes.POINTER(_c_i64)]),
+    "qmb_compile_term_host": (
+        _c_int,
+        [_vp, _vp, _c_int, _c_int, ctypes.POINTER(_c_u64), ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i32)],
+    ),
+    "qmb_apply_record_host": (_c_int, [_c_u64, _c_int, _vp, _vp, ctypes.POINTER(_c_i32)]),
+    "qmb_philox4x32_10_host": (None, [_vp, _vp, _vp]),
+    "qmb_philox_uniform_host": (ctypes.c_double, [_c_u64, _c_u64]),
+    "qmb_apply_within_workspace_size": (_c_int, [_c_int, _c_i64, _c_i64, _c_i64, ctypes.POINTER(_c_size)]),
+    "qmb_apply_within": (_c_int, [_c_int, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _vp, _c_size, _vp]),
+    "qmb_find_relative_workspace_size": (_c_int, [_c_int, _c_i64, _c_i64, _c_i64, _c_i64, ctypes.POINTER(_c_size)]),
+    "qmb_find_relative": (
+        _c_int,
+        [_c_int, _c_int, _vp, _vp, _c_i64, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _c_i64, _vp, _vp, _vp, _c_size, _vp],
+    ),
+    "qmb_diagonal_term_workspace_size": (_c_int, [_c_int, _c_i64, _c_i64, ctypes.POINTER(_c_size)]),
+    "qmb_diagonal_term": (_c_int, [_c_int, _c_int, _vp, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _vp, _c_size, _vp]),
+    "qmb_single_relative_workspace_size": (_c_int, [_c_int, _c_i64, _c_i64, ctypes.POINTER(_c_size)]),
+    "qmb_single_relative": (_c_int, [_c_int, _c_int, _vp, _c_i64, _vp, _vp, _vp, _c_i64, _c_u64, _vp, _vp, _vp, _c_size, _vp]),
+    "qmb_apply_within_adjoint": (_c_int, [_c_int, _c_int, _vp, _c_i64, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _vp, _c_size, _vp]),
+    "qmb_apply_within_plain_workspace_size": (_c_int, [_c_int, _c_i64, ctypes.POINTER(_c_size)]),
+    "qmb_apply_within_plain": (_c_int, [_c_int, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _vp, _c_size, _vp]),
+    "qmb_terms_workspace_size": (_c_int, [_c_int, _c_i64, ctypes.POINTER(_c_size)]),
+    "qmb_terms_build": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _c_i64, _vp, _c_size, ctypes.POINTER(QmbTerms), _vp]),
+    "qmb_targets_workspace_size": (_c_int, [_c_int, _c_i64, ctypes.POINTER(_c_size)]),
+    "qmb_targets_build": (_c_int, [_c_int, _vp, _c_i64, _vp, _c_size, ctypes.POINTER(QmbTargets), _vp]),
+    "qmb_apply_prepared": (
+        _c_int,
+        [ctypes.POINTER(QmbTerms), ctypes.POINTER(QmbTargets), _vp, _c_i64, _vp, _vp, _c_int, ctypes.POINTER(QmbLaunch), _vp],
+    ),
+}
+
+
+class QmbError(RuntimeError):
+    """A nonzero status from libqmb_hip.so, carrying qmb_last_error()."""
+
+
+_LIB: ctypes.CDLL | None = None
+
+
+def library() -> ctypes.CDLL:
+    """Loads libqmb_hip.so once and binds every declared entry point; raises if it is absent."""
+    global _LIB  # pylint: disable=global-statement
+    if _LIB is None:
+        if not LIBRARY_PATH.exists():
+            raise ImportError(
+                f"libqmb_hip.so not found at {LIBRARY_PATH}; build it with `make -C qmb-public_amd/csrc` "
+                "(or __graft_entry__.build()). There is no non-HIP fallback."
+            )
+        lib = ctypes.CDLL(str(LIBRARY_PATH))
+        for name, (restype, argtypes) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = restype
+            fn.argtypes = argtypes
+        _LIB = lib
+    return _LIB
+
+
+def check(status: int, what: str) -> None:
+    """Raises QmbError with the library's message when status is nonzero."""
+    if status != QMB_OK:
+        message = library().qmb_last_error().decode("utf-8", "replace")
+        raise QmbError(f"{what}: {message} (status {status})")
+
+
+def exported_symbols() -> list[str]:
+    """Names of the entry points this binding expects the library to export."""
+    return list(SIGNATURES)
