@@ -1,0 +1,270 @@
+#!/usr/bin/env python3
+"""
+apply_within forward + adjoint throughput on one MI355X, in matrix elements per second.
+
+A step is one differentiable application of a seeded synthetic Hamiltonian to a seeded config set:
+build the hash index over configs_j, forward psi_j = H psi_i (scatter kernel), then the adjoint
+grad_i = H^H g for a random upstream gradient g (gather kernel) through torch.autograd.  The term
+table is compiled once outside the timed region (it belongs to the Hamiltonian, like the reference's
+one-off _prepare_data).  A matrix element is a contribution: a (config, term) pair whose application
+succeeds and whose target is in configs_j; a step computes M_hit of them forward and M_hit backward.
+
+Workloads (--workload):
+  molecular52 (default): 52 spin orbitals, 7a+7b, |(pq|rs)| < 1e-10 dropped (669,579 terms),
+                         1,000,000 configs = HF + all singles/doubles + seeded triples/quadruples,
+                         configs_j = configs_i.
+  hubbard128:            open 1-D Hubbard chain, 64 sites (128 spin orbitals), t=1, U=4, half
+                         filling, 4,000,000 configs (random seeds + single hops), configs_j = configs_i.
+  molecular20:           20 spin orbitals, 5a+5b, 10,000 configs uniform in the sector.
+
+Prints one JSON line with the metric, a roofline entry for the forward kernel (algorithmic bytes per
+launch / its HIP-event time) and the reference CPU op timed on this host on a bounded sample.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import platform
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path[:0] = [str(ROOT / "qmb-public_amd"), str(ROOT)]
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(name: str) -> dict[str, object]:
+    import numpy as np  # pylint: disable=import-outside-toplevel
+
+    from qmb_hip import synthetic as sy  # pylint: disable=import-outside-toplevel
+
+    t0 = time.time()
+    if name == "molecular52":
+        site, kind, coef = sy.molecular_terms(26, 2001, threshold=1e-10)
+        configs = sy.excitation_configs(26, 7, 7, 1_000_000, 2002)
+        pc, seeds, sample = 1, (2003, 2004), 1000
+        desc = {"workload": "molecular52", "spin_orbitals": 52, "electrons": "7a+7b", "n_configs": 1_000_000,
+                "configs": "HF + singles + doubles + seeded triples/quadruples", "threshold": 1e-10}
+    elif name == "hubbard128":
+        site, kind, coef = sy.terms_from_dict(sy.hubbard_dict(64, 1, 1.0, 4.0))
+        configs = sy.hubbard_chain_configs(64, 4_000_000, 3000)
+        pc, seeds, sample = 1, (3001, 3002), 400_000
+        desc = {"workload": "hubbard128", "sites": 64, "spin_orbitals": 128, "t": 1.0, "U": 4.0, "filling": "half",
+                "n_configs": int(len(configs))}
+    elif name == "molecular20":
+        site, kind, coef = sy.molecular_terms(10, 1001)
+        configs = sy.sector_configs(10, 5, 5, 10_000, 1002)
+        pc, seeds, sample = 1, (1003, 1004), 10_000
+        desc = {"workload": "molecular20", "spin_orbitals": 20, "electrons": "5a+5b", "n_configs": 10_000}
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    order = np.argsort(-np.linalg.norm(coef.numpy(), axis=1), kind="stable")
+    site, kind, coef = site[order], kind[order], coef[order]
+    psi = sy.random_psi(len(configs), seeds[0])
+    grad = sy.random_psi(len(configs), seeds[1])
+    log(f"workload {name}: {len(site)} terms, {len(configs)} configs x {configs.shape[1]} bytes, built in {time.time() - t0:.1f}s")
+    return {"site": site, "kind": kind, "coef": coef, "configs": configs, "psi": psi, "grad": grad, "pc": pc,
+            "sample": min(sample, len(configs)), "desc": desc}
+
+
+def cpu_baseline(w: dict[str, object], m_hit_sample: int) -> dict[str, object]:
+    """The reference CPU op (oracle/_ref) on the first `sample` configs_i against all of configs_j."""
+    import numpy as np  # pylint: disable=import-outside-toplevel
+
+    configs = w["configs"]
+    n = int(w["sample"])
+    nq = configs.shape[1]
+    pc = int(w["pc"])
+    ref = ROOT / "oracle" / "_ref" / f"qmb_ref_{nq}_{pc}.so"
+    sample_desc = f"apply_within of the first {n} of {len(configs)} configs_i against all {len(configs)} configs_j"
+    with tempfile.TemporaryDirectory() as d:
+        inp = pathlib.Path(d) / "in.npz"
+        out = pathlib.Path(d) / "out.npz"
+        np.savez(inp, configs_i=configs[:n], psi_i=w["psi"][:n], configs_j=configs, site=w["site"].numpy(), kind=w["kind"].numpy(),
+                 coef=w["coef"].numpy(), particle_cut=pc)
+        if ref.exists():
+            env = dict(os.environ, OMP_NUM_THREADS="1")
+            proc = subprocess.run([sys.executable, str(ROOT / "oracle" / "ref_runner.py"), "apply_within", str(inp), str(out),
+                                   "--repeat", "1", "--threads", "1"], env=env, capture_output=True, text=True, check=False)
+            if proc.returncode == 0:
+                t = float(np.load(out)["times"][0])
+                return {"value": m_hit_sample / t, "unit": "matrix elements/s", "cores": 1, "kind": "reference",
+                        "sample": sample_desc + " (qmb/_hamiltonian_cpu.cpp built from source, -O3 -ffast-math, 1 thread)",
+                        "seconds": t, "host_cpu": platform.processor() or platform.machine()}
+            log(f"reference op failed: {proc.stderr[-500:]}")
+        from oracle import oracle as orc  # pylint: disable=import-outside-toplevel
+
+        t0 = time.perf_counter()
+        orc.apply_within(pc, configs[:n], w["psi"][:n], configs, w["site"].numpy(), w["kind"].numpy(), w["coef"].numpy())
+        t = time.perf_counter() - t0
+        return {"value": m_hit_sample / t, "unit": "matrix elements/s", "cores": 1, "kind": "port",
+                "sample": sample_desc + " (oracle/qmb_oracle.c, 1 thread)", "seconds": t}
+
+
+def main() -> int:
+    parser = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=10)
+    parser.add_argument("--warmup", type=int, default=3)
+    parser.add_argument("--workload", default="molecular52", choices=["molecular52", "hubbard128", "molecular20"])
+    parser.add_argument("--tile", type=int, default=0)
+    parser.add_argument("--groups", type=int, default=0)
+    parser.add_argument("--no-plain", action="store_true", help="skip the reference-scheme comparator")
+    parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = parser.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is described in DESIGN.md.",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np  # pylint: disable=import-outside-toplevel
+    import torch  # pylint: disable=import-outside-toplevel
+
+    from qmb_hip import autograd as ag  # pylint: disable=import-outside-toplevel
+    from qmb_hip import ops  # pylint: disable=import-outside-toplevel
+    from qmb_hip import prepared as pp  # pylint: disable=import-outside-toplevel
+
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a HIP GPU")
+    dev = torch.device("cuda:0")
+    w = build_workload(args.workload)
+    configs = torch.from_numpy(w["configs"]).to(dev)
+    nq = configs.shape[1]
+    pc = int(w["pc"])
+    site, kind, coef = (t.contiguous().to(dev) for t in (w["site"], w["kind"], w["coef"]))
+    psi = torch.from_numpy(w["psi"]).to(dev)
+    grad = torch.from_numpy(w["grad"]).to(dev)
+    n = configs.shape[0]
+    n_terms = site.shape[0]
+    terms = pp.PreparedTerms(site, kind, coef, nq, pc)
+    torch.cuda.synchronize()
+    valid_terms, diag_terms, bad_terms = terms.counts()
+    assert bad_terms == 0
+
+    def step() -> torch.Tensor:
+        targets = pp.PreparedTargets(configs, nq)
+        x = psi.detach().requires_grad_(True)
+        y = ag.apply_within(x, configs, terms, targets, args.tile, args.groups)
+        y.backward(grad)
+        return x.grad
+
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i + 1}/{args.warmup}")
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for i in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = wall * 1e3 / args.steps
+    log(f"timed {args.steps} steps: {step_ms:.3f} ms/step (events {ev0.elapsed_time(ev1) / args.steps:.3f} ms)")
+
+    # Dominant kernel: the forward enumeration, timed alone with events on the stream it runs on.
+    targets = pp.PreparedTargets(configs, nq)
+    m_gen, m_hit = pp.count(terms, targets, configs, args.tile, args.groups)
+    out = torch.zeros((n, 2), dtype=torch.float64, device=dev)
+    vec = torch.view_as_real(psi).contiguous()
+    reps = max(3, min(20, args.steps))
+    k_ms = []
+    for _ in range(reps):
+        out.zero_()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        pp.apply(terms, targets, configs, vec, pp.MODE_FORWARD, out=out, tile=args.tile, groups=args.groups, zero=False)
+        b.record(stream)
+        b.synchronize()
+        k_ms.append(a.elapsed_time(b))
+    kernel_ms = float(np.median(k_ms))
+    adj_ms = []
+    gvec = torch.view_as_real(grad).contiguous()
+    gout = torch.zeros((n, 2), dtype=torch.float64, device=dev)
+    for _ in range(reps):
+        gout.zero_()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        pp.apply(terms, targets, configs, gvec, pp.MODE_ADJOINT, out=gout, tile=args.tile, groups=args.groups, zero=False)
+        b.record(stream)
+        b.synchronize()
+        adj_ms.append(a.elapsed_time(b))
+    adjoint_ms = float(np.median(adj_ms))
+    algo_bytes = n * (nq + 16) + n * (nq + 16) + n_terms * 28 + m_gen * nq + m_hit * 16
+    achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
+    log(f"M_gen={m_gen} M_hit={m_hit}; forward kernel {kernel_ms:.3f} ms, adjoint kernel {adjoint_ms:.3f} ms")
+
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_{args.workload}.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_forward_launch")
+
+    plain = None
+    if not args.no_plain:
+        # The reference scheme (full term x config grid, per-call sort, binary search, atomics);
+        # its backward is the same kernel on swapped arguments with conjugated coefficients, which
+        # equals the adjoint because the synthetic tables are Hermitian.
+        ns = ops.load(nq, pc)
+        psi_r = torch.view_as_real(psi).contiguous()
+        coef_conj = coef.clone()
+        coef_conj[:, 1].neg_()
+        small = configs[:64]
+        ns.apply_within_plain(small, psi_r[:64], small, site, kind, coef)
+        torch.cuda.synchronize()
+        p0 = time.perf_counter()
+        ns.apply_within_plain(configs, psi_r, configs, site, kind, coef)
+        ns.apply_within_plain(configs, gvec, configs, site, kind, coef_conj)
+        torch.cuda.synchronize()
+        plain_s = time.perf_counter() - p0
+        plain = {"ms_per_step": plain_s * 1e3, "value": 2 * m_hit / plain_s, "speedup": plain_s * 1e3 / step_ms}
+        log(f"plain comparator: {plain_s * 1e3:.1f} ms per fwd+bwd")
+
+    cpu = None
+    if not args.no_cpu:
+        s_n = int(w["sample"])
+        sample_targets = targets
+        g_s, h_s = pp.count(terms, sample_targets, configs[:s_n])
+        cpu = cpu_baseline(w, h_s)
+        cpu["m_hit_sample"] = h_s
+        log(f"cpu baseline: {cpu['value']:.3e} elements/s over {cpu['seconds']:.2f}s ({cpu['kind']})")
+
+    value = 2 * m_hit / (step_ms * 1e-3)
+    result = {
+        "metric": "apply_within fwd+bwd matrix elements/s",
+        "value": value,
+        "unit": "matrix elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded)",
+        "config": dict(w["desc"], n_terms=n_terms, n_qubytes=nq, M_gen=m_gen, M_hit=m_hit, valid_terms=valid_terms,
+                       diagonal_terms=diag_terms, tile=args.tile or "auto", groups=args.groups or 8),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": traffic, "kernel": f"apply_kernel<W={(nq + 7) // 8},forward>", "kernel_ms": kernel_ms,
+                     "adjoint_kernel_ms": adjoint_ms, "algorithmic_bytes": algo_bytes},
+        "cpu_baseline": cpu,
+        "plain_kernel": plain,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

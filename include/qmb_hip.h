@@ -163,12 +163,17 @@ int qmb_targets_build(int n_qubytes, const uint8_t* configs, int64_t n, void* bu
 
 /* Launch shape for the enumeration kernels (0 = library default). */
 typedef struct qmb_launch {
-    int32_t tile;    /* configs per workgroup tile: 256..4096, power of two */
-    int32_t groups;  /* term-bucket groups per tile (8 keeps each group on one XCD's L2) */
+    int32_t tile;     /* bucketed strategy: configs per workgroup tile, 64..4096, power of two */
+    int32_t groups;   /* bucketed strategy: bucket groups per tile (bucket b goes to group b % groups) */
+    int32_t strategy; /* 0 auto (direct for <= 4096 terms), 1 bucketed tiles, 2 direct (every config
+                         walks every term from LDS) */
 } qmb_launch;
 
 /* mode 0: forward scatter  out[k] += sum sign*coef*in[i]      (in = psi_i [n_i], out = psi_j [n_j])
  * mode 1: adjoint gather   out[i] += sum conj(sign*coef)*in[k] (in = grad_j [n_j], out = grad_i [n_i])
+ * mode 2: count            ((uint64_t*)out)[0] += connections (successful term applications),
+ *                          ((uint64_t*)out)[1] += contributions (connections whose target is indexed);
+ *                          `in` is unused.  These are M_gen and M_hit of the traffic model.
  * `out` is accumulated into (zero it first).  configs_i are n_qubytes-byte rows. */
 int qmb_apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const uint8_t* configs_i, int64_t n_i,
                        const double* in, double* out, int mode, const qmb_launch* launch, void* stream);

@@ -35,15 +35,14 @@ struct ApplyParams {
 template <int W>
 struct ForwardAction {
     const ApplyParams& p;
-    double2 v;
-    __device__ ForwardAction(const ApplyParams& pp) : p(pp), v(make_double2(0.0, 0.0)) {}
-    __device__ __forceinline__ void begin_member(int64_t i) { v = p.in[i]; }
-    __device__ __forceinline__ void on_success(int64_t, uint32_t t, const Cfg<W>& o, int par) {
+    __device__ explicit ForwardAction(const ApplyParams& pp) : p(pp) {}
+    __device__ __forceinline__ void on_success(int64_t i, uint32_t t, const Cfg<W>& o, int par) {
         const int64_t j = lookup<W>(p.table, p.mask, p.tkeys, o);
         if (j < 0) {
             return;
         }
         const double2 cf = p.coefs[t];
+        const double2 v = p.in[i];
         double re = cf.x * v.x - cf.y * v.y;
         double im = cf.x * v.y + cf.y * v.x;
         if (par) {
@@ -53,50 +52,76 @@ struct ForwardAction {
         unsafeAtomicAdd(&p.out[j].x, re);
         unsafeAtomicAdd(&p.out[j].y, im);
     }
-    __device__ __forceinline__ void end_member(int64_t) {}
 };
 
-// out[i] += conj(sign * coef) * in[j]: gather into registers, one atomic per (thread, member).
+// out[i] += conj(sign * coef) * in[j]: gather from the target's gradient.
 template <int W>
 struct AdjointAction {
     const ApplyParams& p;
-    double re, im;
-    __device__ AdjointAction(const ApplyParams& pp) : p(pp), re(0.0), im(0.0) {}
-    __device__ __forceinline__ void begin_member(int64_t) {
-        re = 0.0;
-        im = 0.0;
-    }
-    __device__ __forceinline__ void on_success(int64_t, uint32_t t, const Cfg<W>& o, int par) {
+    __device__ explicit AdjointAction(const ApplyParams& pp) : p(pp) {}
+    __device__ __forceinline__ void on_success(int64_t i, uint32_t t, const Cfg<W>& o, int par) {
         const int64_t j = lookup<W>(p.table, p.mask, p.tkeys, o);
         if (j < 0) {
             return;
         }
         const double2 cf = p.coefs[t];
         const double2 g = p.in[j];
-        const double a = cf.x * g.x + cf.y * g.y;
-        const double b = cf.x * g.y - cf.y * g.x;
-        re += par ? -a : a;
-        im += par ? -b : b;
+        double re = cf.x * g.x + cf.y * g.y;
+        double im = cf.x * g.y - cf.y * g.x;
+        if (par) {
+            re = -re;
+            im = -im;
+        }
+        unsafeAtomicAdd(&p.out[i].x, re);
+        unsafeAtomicAdd(&p.out[i].y, im);
     }
-    __device__ __forceinline__ void end_member(int64_t i) {
-        if (re != 0.0 || im != 0.0) {
-            unsafeAtomicAdd(&p.out[i].x, re);
-            unsafeAtomicAdd(&p.out[i].y, im);
+};
+
+// Counts connections (successful applications) and contributions (connections whose target is in
+// the index) for the traffic model; `out` is reinterpreted as two unsigned 64-bit counters.
+template <int W>
+struct CountAction {
+    const ApplyParams& p;
+    unsigned long long gen, hit;
+    __device__ explicit CountAction(const ApplyParams& pp) : p(pp), gen(0), hit(0) {}
+    __device__ __forceinline__ void on_success(int64_t, uint32_t, const Cfg<W>& o, int) {
+        ++gen;
+        hit += lookup<W>(p.table, p.mask, p.tkeys, o) >= 0 ? 1 : 0;
+    }
+    __device__ __forceinline__ void flush() {
+        unsigned long long g = gen, h = hit;
+        for (int d = 32; d > 0; d >>= 1) {
+            g += __shfl_down(g, d);
+            h += __shfl_down(h, d);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            unsigned long long* c = reinterpret_cast<unsigned long long*>(p.out);
+            atomicAdd(&c[0], g);
+            atomicAdd(&c[1], h);
         }
     }
 };
 
-template <int W, int MODE>
+template <int W, int MODE, int DIRECT>
 __global__ __launch_bounds__(kBlock) void apply_kernel(ApplyParams p) {
     extern __shared__ uint64_t smem[];
-    const int64_t tile = blockIdx.x / p.groups;
-    const int g = blockIdx.x % p.groups;
+    auto run = [&](auto& act) {
+        if (DIRECT) {
+            enumerate_direct<W>(p.src, blockIdx.x, *reinterpret_cast<DirectLds<W>*>(smem), act);
+        } else {
+            enumerate_tile<W>(p.src, blockIdx.x / p.groups, p.tile_log2, blockIdx.x % p.groups, p.groups, smem, act);
+        }
+    };
     if (MODE == 0) {
         ForwardAction<W> act(p);
-        enumerate_tile<W>(p.src, tile, p.tile_log2, g, p.groups, smem, act);
-    } else {
+        run(act);
+    } else if (MODE == 1) {
         AdjointAction<W> act(p);
-        enumerate_tile<W>(p.src, tile, p.tile_log2, g, p.groups, smem, act);
+        run(act);
+    } else {
+        CountAction<W> act(p);
+        run(act);
+        act.flush();
     }
 }
 
@@ -303,10 +328,16 @@ __global__ __launch_bounds__(kBlock) void single_rel_kernel(const uint8_t* __res
 
 }  // namespace
 
-LaunchShape resolve_launch(const qmb_launch* launch, int n_words, int64_t n_i) {
+LaunchShape resolve_launch(const qmb_launch* launch, int n_words, int64_t n_i, int64_t n_terms) {
     LaunchShape s{};
     int tile = 2048 / n_words;
     int groups = 8;
+    int strategy = launch ? launch->strategy : 0;
+    if (strategy == 0) {
+        // Small tables: every config walks every term from LDS.  Large tables: bucketed tiles.
+        strategy = n_terms <= kDirectMaxTerms ? 2 : 1;
+    }
+    s.direct = strategy == 2;
     if (launch && launch->tile > 0) {
         tile = launch->tile;
     } else {
@@ -333,6 +364,8 @@ LaunchShape resolve_launch(const qmb_launch* launch, int n_words, int64_t n_i) {
     return s;
 }
 
+size_t direct_lds_bytes(int n_words) { return static_cast<size_t>(kDirectChunk) * (32 * n_words + 4); }
+
 size_t enum_lds_bytes(int n_words, int n_bits, int tile_log2) { return enum_lds_bytes_of(n_words, n_bits, tile_log2); }
 
 int apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const uint8_t* configs_i, int64_t n_i, const double* in,
@@ -343,8 +376,8 @@ int apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const uin
     if (terms->n_qubytes != targets->n_qubytes) {
         return set_error(QMB_ERR_ARGUMENT, "terms built for %d bytes, targets for %d", terms->n_qubytes, targets->n_qubytes);
     }
-    if (mode != 0 && mode != 1) {
-        return set_error(QMB_ERR_ARGUMENT, "mode must be 0 (forward) or 1 (adjoint)");
+    if (mode != 0 && mode != 1 && mode != 2) {
+        return set_error(QMB_ERR_ARGUMENT, "mode must be 0 (forward), 1 (adjoint) or 2 (count)");
     }
     if (n_i < 0 || n_i >= (1ll << 40)) {
         return set_error(QMB_ERR_ARGUMENT, "n_i out of range");
@@ -352,7 +385,7 @@ int apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const uin
     if (n_i == 0 || terms->n_terms == 0) {
         return QMB_OK;
     }
-    const LaunchShape ls = resolve_launch(launch, terms->n_words, n_i);
+    const LaunchShape ls = resolve_launch(launch, terms->n_words, n_i, terms->n_terms);
     ApplyParams p{};
     p.src = make_source(terms, configs_i, n_i);
     p.in = reinterpret_cast<const double2*>(in);
@@ -363,28 +396,39 @@ int apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const uin
     p.mask = targets->capacity - 1;
     p.tile_log2 = ls.tile_log2;
     p.groups = ls.groups;
-    const int64_t tiles = (n_i + (1ll << ls.tile_log2) - 1) >> ls.tile_log2;
-    const int64_t grid = tiles * ls.groups;
+    int64_t grid;
+    size_t lds;
+    if (ls.direct) {
+        grid = (n_i + kBlock - 1) / kBlock;
+        lds = direct_lds_bytes(terms->n_words);
+    } else {
+        const int64_t tiles = (n_i + (1ll << ls.tile_log2) - 1) >> ls.tile_log2;
+        grid = tiles * ls.groups;
+        lds = enum_lds_bytes(terms->n_words, terms->n_bits, ls.tile_log2);
+    }
     if (grid >= (1ll << 31)) {
         return set_error(QMB_ERR_ARGUMENT, "grid too large");
     }
-    const size_t lds = enum_lds_bytes(terms->n_words, terms->n_bits, ls.tile_log2);
     if (lds > 160 * 1024 - 1024) {
         return set_error(QMB_ERR_ARGUMENT, "tile of %d configs needs %zu bytes of LDS", 1 << ls.tile_log2, lds);
     }
     return dispatch_words(terms->n_words, [&](auto wc) -> int {
         constexpr int W = decltype(wc)::value;
-        if (mode == 0) {
-            QMB_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&apply_kernel<W, 0>),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-            apply_kernel<W, 0><<<static_cast<unsigned>(grid), kBlock, lds, stream>>>(p);
-        } else {
-            QMB_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&apply_kernel<W, 1>),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-            apply_kernel<W, 1><<<static_cast<unsigned>(grid), kBlock, lds, stream>>>(p);
+        auto launch_one = [&](auto kernel) -> int {
+            QMB_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            static_cast<int>(lds)));
+            kernel<<<static_cast<unsigned>(grid), kBlock, lds, stream>>>(p);
+            QMB_CHECK_LAUNCH();
+            return QMB_OK;
+        };
+        if (ls.direct) {
+            if (mode == 0) return launch_one(&apply_kernel<W, 0, 1>);
+            if (mode == 1) return launch_one(&apply_kernel<W, 1, 1>);
+            return launch_one(&apply_kernel<W, 2, 1>);
         }
-        QMB_CHECK_LAUNCH();
-        return QMB_OK;
+        if (mode == 0) return launch_one(&apply_kernel<W, 0, 0>);
+        if (mode == 1) return launch_one(&apply_kernel<W, 1, 0>);
+        return launch_one(&apply_kernel<W, 2, 0>);
     });
 }
 

@@ -171,7 +171,7 @@ __global__ void insert_kernel(const uint64_t* __restrict__ keys, int64_t n, uint
     const uint32_t fp = static_cast<uint32_t>(h);
     const uint64_t entry = (static_cast<uint64_t>(i) << 32) | fp;
     uint64_t slot = (h >> 32) & mask;
-    // Bounded by the capacity: the table is at most half full, so an empty slot always exists.
+    // Bounded by the capacity: the table is at most a quarter full, so an empty slot always exists.
     for (uint64_t probe = 0; probe <= mask; ++probe) {
         unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&table[slot]),
                                             static_cast<unsigned long long>(kEmptySlot), static_cast<unsigned long long>(entry));
@@ -418,7 +418,7 @@ int targets_layout(Carver& cv, int n_qubytes, int64_t n, qmb_targets* out) {
     t.n_qubytes = n_qubytes;
     t.n_words = n_words_of(n_qubytes);
     t.n = n;
-    t.capacity = next_pow2(static_cast<uint64_t>(2 * n > 64 ? 2 * n : 64));
+    t.capacity = next_pow2(static_cast<uint64_t>(4 * n > 64 ? 4 * n : 64));  // load factor <= 1/4
     t.buffer = cv.base;
     t.keys = cv.take<uint64_t>(static_cast<size_t>(n > 0 ? n : 1) * t.n_words);
     t.table = cv.take<uint64_t>(t.capacity);

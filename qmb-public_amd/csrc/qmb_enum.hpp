@@ -49,7 +49,7 @@ __device__ __forceinline__ void store_row(uint8_t* __restrict__ row, int nq, con
     }
 }
 
-// Hash-index lookup: row of the indexed set equal to c, or -1.  The table is at most half full,
+// Hash-index lookup: row of the indexed set equal to c, or -1.  The table is at most 1/4 full,
 // so the probe sequence reaches an empty slot; the loop is additionally capped at the capacity.
 template <int W>
 __device__ __forceinline__ int64_t lookup(const uint64_t* __restrict__ table, uint64_t mask, const uint64_t* __restrict__ keys,
@@ -125,15 +125,75 @@ __device__ __forceinline__ EnumLds carve_lds(uint64_t* smem, int tile_log2, int 
     return l;
 }
 
-// Processes tile `tile` x bucket group `g` (of `groups`).  Action provides
-//   begin_member(i)                   before the terms of member config i (global row)
-//   on_success(i, t, target, parity)  for every term position t that succeeds on config i
-//   end_member(i)                     after them
-// Every thread of the workgroup must call this (it synchronises the workgroup).
+// A compiled record expanded to word masks: the term succeeds on c iff (c & m) == v for every word;
+// the target is c ^ f and the parity popcount(c & p) ^ pconst (p excludes the touched sites).
+template <int W>
+struct Masks {
+    uint64_t m[W], v[W], f[W], p[W];
+    int pconst;
+};
+
+template <int W>
+__device__ __forceinline__ Masks<W> expand_rec(uint64_t rec) {
+    Masks<W> x;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        x.m[w] = 0;
+        x.v[w] = 0;
+        x.f[w] = 0;
+        x.p[w] = 0;
+    }
+    const uint32_t valid = static_cast<uint32_t>(rec >> 32) & 0xf;
+    const uint32_t req = static_cast<uint32_t>(rec >> 36) & 0xf;
+    const uint32_t fin = static_cast<uint32_t>(rec >> 40) & 0xf;
+    const uint32_t odd = static_cast<uint32_t>(rec >> 44) & 0xf;
+#pragma unroll
+    for (int k = 0; k < kMaxOps; ++k) {
+        if ((valid >> k) & 1u) {
+            const int s = static_cast<int>((rec >> (8 * k)) & 0xff);
+            const uint64_t bit = 1ull << (s & 63);
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                if (w == (s >> 6)) {
+                    x.m[w] |= bit;
+                    x.v[w] |= ((req >> k) & 1u) ? bit : 0ull;
+                    x.f[w] |= (((req ^ fin) >> k) & 1u) ? bit : 0ull;
+                }
+                if ((odd >> k) & 1u) {
+                    x.p[w] ^= low_mask_word(s, w);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        x.p[w] &= ~x.m[w];
+    }
+    x.pconst = static_cast<int>((rec >> 48) & 1);
+    return x;
+}
+
+template <int W>
+__device__ __forceinline__ bool apply_masks(const Masks<W>& x, const Cfg<W>& c, Cfg<W>& o, int& parity) {
+    bool ok = true;
+    int par = x.pconst;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        ok = ok && ((c.w[w] & x.m[w]) == x.v[w]);
+        o.w[w] = c.w[w] ^ x.f[w];
+        par ^= __popcll(c.w[w] & x.p[w]) & 1;
+    }
+    parity = par;
+    return ok;
+}
+
+// Processes tile `tile` x bucket group `g` (of `groups`): buckets b with b % groups == g, so the
+// heavy buckets (pairs of low sites that nearly every config occupies) spread over all groups.
+// Action provides on_success(i, t, target, parity) for every term position t that succeeds on
+// config row i.  Every thread of the workgroup must call this (it synchronises the workgroup).
 template <int W, typename Action>
 __device__ void enumerate_tile(const EnumSource& p, int64_t tile, int tile_log2, int g, int groups, uint64_t* smem, Action& act) {
     __shared__ int s_count;
-    __shared__ uint32_t s_range[2];
     const int C = 1 << tile_log2;
     const int CW = C >> 6;
     const EnumLds L = carve_lds<W>(smem, tile_log2, p.n_bits);
@@ -144,13 +204,6 @@ __device__ void enumerate_tile(const EnumSource& p, int64_t tile, int tile_log2,
     const int64_t left = p.n_i - i0;
     const int nc = left < C ? static_cast<int>(left) : C;
 
-    if (tid == 0) {
-        const uint64_t tb = static_cast<uint64_t>(p.counts[0]);
-        const uint64_t lo_t = tb * g / groups;
-        const uint64_t hi_t = tb * (g + 1) / groups;
-        s_range[0] = lower_bound_u32(p.bucket_start, p.n_buckets, lo_t);
-        s_range[1] = (g == groups - 1) ? static_cast<uint32_t>(p.n_buckets) : lower_bound_u32(p.bucket_start, p.n_buckets, hi_t);
-    }
     for (int r = tid; r < C; r += kEnumBlock) {
         Cfg<W> c;
         if (r < nc) {
@@ -190,11 +243,9 @@ __device__ void enumerate_tile(const EnumSource& p, int64_t tile, int tile_log2,
     }
     __syncthreads();
 
-    const uint32_t b_lo = s_range[0];
-    const uint32_t b_hi = s_range[1];
     const int64_t single_end = 1 + p.n_bits;
     const int64_t residual = p.n_buckets - 1;
-    for (uint32_t b = b_lo; b < b_hi; ++b) {
+    for (int64_t b = g; b < p.n_buckets; b += groups) {
         const uint32_t ts = p.bucket_start[b];
         const uint32_t te = p.bucket_start[b + 1];
         if (ts == te) {
@@ -205,7 +256,7 @@ __device__ void enumerate_tile(const EnumSource& p, int64_t tile, int tile_log2,
             if (b >= 1 && b < single_end) {
                 r0 = static_cast<int>(b) - 1;
             } else if (b >= single_end && b < residual) {
-                decode_pair(static_cast<int64_t>(b) - single_end, &r0, &r1);
+                decode_pair(b - single_end, &r0, &r1);
             }
             uint64_t m = 0;
             if (lane < CW) {
@@ -239,32 +290,93 @@ __device__ void enumerate_tile(const EnumSource& p, int64_t tile, int tile_log2,
         __syncthreads();
         const int mcount = s_count;
         const int k = static_cast<int>(te - ts);
-        // 2-D split of the (member, term) pairs: TX threads along the bucket's terms.
+        // 2-D split of the (term, member) pairs: TX threads along the bucket's terms, each holding
+        // its term expanded to masks while it walks TY-strided members (configs are LDS broadcasts).
         const int kk = k < kEnumBlock ? k : kEnumBlock;
         const int lg = kk <= 1 ? 0 : 32 - __clz(kk - 1);
         const int TX = 1 << lg;
         const int TY = kEnumBlock >> lg;
         const int tx = tid & (TX - 1);
         const int ty = tid >> lg;
-        for (int q = ty; q < mcount; q += TY) {
-            const int r = L.mem[q];
-            Cfg<W> c;
+        for (uint32_t t = ts + tx; t < te; t += TX) {
+            const Masks<W> x = expand_rec<W>(p.recs[t]);
+            for (int q = ty; q < mcount; q += TY) {
+                const int r = L.mem[q];
+                Cfg<W> c;
 #pragma unroll
-            for (int w = 0; w < W; ++w) {
-                c.w[w] = L.cfg[r * W + w];
-            }
-            const int64_t i = i0 + r;
-            act.begin_member(i);
-            for (uint32_t t = ts + tx; t < te; t += TX) {
+                for (int w = 0; w < W; ++w) {
+                    c.w[w] = L.cfg[r * W + w];
+                }
                 Cfg<W> o;
                 int par;
-                if (apply_rec<W>(p.recs[t], c, o, par)) {
-                    act.on_success(i, t, o, par);
+                if (apply_masks<W>(x, c, o, par)) {
+                    act.on_success(i0 + r, t, o, par);
                 }
             }
-            act.end_member(i);
         }
         __syncthreads();
+    }
+}
+
+// Direct enumeration for small term tables: each thread owns one config in registers and walks
+// every term, the terms staged through LDS (expanded to masks) in chunks shared by the workgroup.
+constexpr int kDirectChunk = 256;
+
+template <int W>
+struct DirectLds {
+    uint64_t m[kDirectChunk * W];
+    uint64_t v[kDirectChunk * W];
+    uint64_t f[kDirectChunk * W];
+    uint64_t p[kDirectChunk * W];
+    int32_t pc[kDirectChunk];
+};
+
+template <int W, typename Action>
+__device__ void enumerate_direct(const EnumSource& p, int64_t block, DirectLds<W>& L, Action& act) {
+    const int tid = threadIdx.x;
+    const int64_t i = block * kEnumBlock + tid;
+    const bool valid = i < p.n_i;
+    Cfg<W> c;
+    if (valid) {
+        c = load_row<W>(p.configs + i * p.nq, p.nq);
+    } else {
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            c.w[k] = 0;
+        }
+    }
+    const int64_t tb = p.counts[0];
+    for (int64_t base = 0; base < tb; base += kDirectChunk) {
+        const int cnt = static_cast<int>(tb - base < kDirectChunk ? tb - base : kDirectChunk);
+        __syncthreads();
+        for (int u = tid; u < cnt; u += kEnumBlock) {
+            const Masks<W> x = expand_rec<W>(p.recs[base + u]);
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                L.m[u * W + w] = x.m[w];
+                L.v[u * W + w] = x.v[w];
+                L.f[u * W + w] = x.f[w];
+                L.p[u * W + w] = x.p[w];
+            }
+            L.pc[u] = x.pconst;
+        }
+        __syncthreads();
+        if (valid) {
+            for (int u = 0; u < cnt; ++u) {
+                bool ok = true;
+                int par = L.pc[u];
+                Cfg<W> o;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    ok = ok && ((c.w[w] & L.m[u * W + w]) == L.v[u * W + w]);
+                    o.w[w] = c.w[w] ^ L.f[u * W + w];
+                    par ^= __popcll(c.w[w] & L.p[u * W + w]) & 1;
+                }
+                if (ok) {
+                    act.on_success(i, static_cast<uint32_t>(base + u), o, par);
+                }
+            }
+        }
     }
 }
 

@@ -62,6 +62,7 @@ struct FrParams {
     int64_t cand_cap;
     int tile_log2;
     int groups;
+    int direct;
 };
 
 // |sign * coef * psi|^2 with every product and sum rounded separately (no contraction), so the
@@ -80,12 +81,10 @@ struct HistAction {
     uint64_t prefix;
     int shift;
     int width;
-    double2 v;
     __device__ HistAction(const FrParams& pp, uint32_t* h, uint64_t pre, int sh, int wd)
-        : p(pp), hist(h), prefix(pre), shift(sh), width(wd), v(make_double2(0.0, 0.0)) {}
-    __device__ __forceinline__ void begin_member(int64_t i) { v = p.psi[i]; }
-    __device__ __forceinline__ void on_success(int64_t, uint32_t t, const Cfg<W>& o, int) {
-        const uint64_t wb = weight_bits(p.coefs[t], v);
+        : p(pp), hist(h), prefix(pre), shift(sh), width(wd) {}
+    __device__ __forceinline__ void on_success(int64_t i, uint32_t t, const Cfg<W>& o, int) {
+        const uint64_t wb = weight_bits(p.coefs[t], p.psi[i]);
         if (wb == 0 || (wb >> (shift + width)) != prefix) {
             return;
         }
@@ -94,7 +93,6 @@ struct HistAction {
         }
         atomicAdd(&hist[(wb >> shift) & ((1u << width) - 1u)], 1u);
     }
-    __device__ __forceinline__ void end_member(int64_t) {}
 };
 
 template <int W>
@@ -103,12 +101,9 @@ struct CollectAction {
     uint64_t prefix;
     int shift;
     bool all;
-    double2 v;
-    __device__ CollectAction(const FrParams& pp, uint64_t pre, int sh, bool a)
-        : p(pp), prefix(pre), shift(sh), all(a), v(make_double2(0.0, 0.0)) {}
-    __device__ __forceinline__ void begin_member(int64_t i) { v = p.psi[i]; }
-    __device__ __forceinline__ void on_success(int64_t, uint32_t t, const Cfg<W>& o, int) {
-        const uint64_t wb = weight_bits(p.coefs[t], v);
+    __device__ CollectAction(const FrParams& pp, uint64_t pre, int sh, bool a) : p(pp), prefix(pre), shift(sh), all(a) {}
+    __device__ __forceinline__ void on_success(int64_t i, uint32_t t, const Cfg<W>& o, int) {
+        const uint64_t wb = weight_bits(p.coefs[t], p.psi[i]);
         if (wb == 0) {
             return;
         }
@@ -139,8 +134,16 @@ struct CollectAction {
             }
         }
     }
-    __device__ __forceinline__ void end_member(int64_t) {}
 };
+
+template <int W, typename Action>
+__device__ __forceinline__ void fr_enumerate(const FrParams& p, uint64_t* smem, Action& act) {
+    if (p.direct) {
+        enumerate_direct<W>(p.src, blockIdx.x, *reinterpret_cast<DirectLds<W>*>(smem), act);
+    } else {
+        enumerate_tile<W>(p.src, blockIdx.x / p.groups, p.tile_log2, blockIdx.x % p.groups, p.groups, smem, act);
+    }
+}
 
 template <int W>
 __global__ __launch_bounds__(kEnumBlock) void fr_hist_kernel(FrParams p) {
@@ -158,7 +161,7 @@ __global__ __launch_bounds__(kEnumBlock) void fr_hist_kernel(FrParams p) {
     }
     __syncthreads();
     HistAction<W> act(p, s_hist, prefix, shift, width);
-    enumerate_tile<W>(p.src, blockIdx.x / p.groups, p.tile_log2, blockIdx.x % p.groups, p.groups, smem, act);
+    fr_enumerate<W>(p, smem, act);
     __syncthreads();
     for (int b = threadIdx.x; b < (1 << width); b += kEnumBlock) {
         if (s_hist[b]) {
@@ -171,7 +174,7 @@ template <int W>
 __global__ __launch_bounds__(kEnumBlock) void fr_collect_kernel(FrParams p) {
     extern __shared__ uint64_t smem[];
     CollectAction<W> act(p, p.state->prefix, p.state->shift, p.state->all != 0);
-    enumerate_tile<W>(p.src, blockIdx.x / p.groups, p.tile_log2, blockIdx.x % p.groups, p.groups, smem, act);
+    fr_enumerate<W>(p, smem, act);
 }
 
 __global__ void fr_init_kernel(FrState* st, unsigned long long* hist, int64_t k) {
@@ -404,7 +407,7 @@ int find_relative_impl(const qmb_terms* terms, const qmb_targets* exclude, const
         return set_error(QMB_ERR_WORKSPACE, "find_relative scratch too small: need %zu, have %zu", cv.off, scratch_bytes);
     }
     const int64_t cap = cand_capacity(k);
-    const LaunchShape ls = resolve_launch(nullptr, W, n_i);
+    const LaunchShape ls = resolve_launch(nullptr, W, n_i, terms->n_terms);
     FrParams p{};
     p.src = make_source(terms, configs_i, n_i);
     p.psi = reinterpret_cast<const double2*>(psi_i);
@@ -421,9 +424,10 @@ int find_relative_impl(const qmb_terms* terms, const qmb_targets* exclude, const
     p.cand_cap = cap;
     p.tile_log2 = ls.tile_log2;
     p.groups = ls.groups;
+    p.direct = ls.direct ? 1 : 0;
     const int64_t tiles = (n_i + (1ll << ls.tile_log2) - 1) >> ls.tile_log2;
-    const int64_t grid = n_i > 0 ? tiles * ls.groups : 0;
-    const size_t lds = enum_lds_bytes_of(W, terms->n_bits, ls.tile_log2);
+    const int64_t grid = n_i <= 0 ? 0 : (ls.direct ? (n_i + kEnumBlock - 1) / kEnumBlock : tiles * ls.groups);
+    const size_t lds = ls.direct ? direct_lds_bytes(W) : enum_lds_bytes_of(W, terms->n_bits, ls.tile_log2);
 
     fr_init_kernel<<<1, 256, 0, stream>>>(s.state, s.hist, k);
     QMB_CHECK_LAUNCH();

@@ -82,10 +82,15 @@ constexpr int kEnumThreads = 256;
 struct LaunchShape {
     int tile_log2;
     int groups;
+    bool direct;
 };
 
-LaunchShape resolve_launch(const qmb_launch* launch, int n_words, int64_t n_i);
+// Term tables up to this size use the direct strategy by default.
+constexpr int64_t kDirectMaxTerms = 4096;
+
+LaunchShape resolve_launch(const qmb_launch* launch, int n_words, int64_t n_i, int64_t n_terms);
 size_t enum_lds_bytes(int n_words, int n_bits, int tile_log2);
+size_t direct_lds_bytes(int n_words);
 
 // ---------------------------------------------------------------------------------------------
 // Builders (qmb_build.hip).

@@ -65,7 +65,7 @@ class QmbTargets(ctypes.Structure):
 class QmbLaunch(ctypes.Structure):
     """Mirror of ``struct qmb_launch`` (include/qmb_hip.h)."""
 
-    _fields_ = [("tile", _c_i32), ("groups", _c_i32)]
+    _fields_ = [("tile", _c_i32), ("groups", _c_i32), ("strategy", _c_i32)]
 
 
 # name -> (restype, argtypes); kept in the order of include/qmb_hip.h

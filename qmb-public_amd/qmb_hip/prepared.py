@@ -1,0 +1,119 @@
+"""
+Prepared objects: a compiled term table and a hash index over a config set, built once on the GPU
+and reused across many applications (Krylov loops, forward + adjoint of one autograd step, the
+batched launch).  Thin holders around ``qmb_terms`` / ``qmb_targets`` of include/qmb_hip.h whose
+device buffers come from PyTorch's caching allocator.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from .ops import _check_terms, _ptr, _stream, _workspace
+
+MODE_FORWARD = 0
+MODE_ADJOINT = 1
+MODE_COUNT = 2
+
+
+STRATEGY_AUTO = 0
+STRATEGY_TILES = 1
+STRATEGY_DIRECT = 2
+
+
+def _launch(tile: int = 0, groups: int = 0, strategy: int = 0) -> _lib.QmbLaunch | None:
+    if tile == 0 and groups == 0 and strategy == 0:
+        return None
+    return _lib.QmbLaunch(tile, groups, strategy)
+
+
+class PreparedTerms:
+    """Compiled, bucketed term table on one device (qmb_terms_build)."""
+
+    def __init__(self, site: torch.Tensor, kind: torch.Tensor, coef: torch.Tensor, n_qubytes: int, particle_cut: int) -> None:
+        device = site.device
+        site, kind, coef = site.contiguous(), kind.contiguous(), coef.contiguous()
+        self.n_terms = _check_terms(site, kind, coef, device, n_qubytes)
+        self.n_qubytes = n_qubytes
+        self.particle_cut = particle_cut
+        self.device = device
+        lib = _lib.library()
+        size = ctypes.c_size_t(0)
+        _lib.check(lib.qmb_terms_workspace_size(n_qubytes, self.n_terms, ctypes.byref(size)), "qmb_terms_workspace_size")
+        self.buffer = _workspace(size.value, device)
+        self.desc = _lib.QmbTerms()
+        # the raw table must outlive the asynchronous build
+        self._inputs = (site, kind, coef)
+        with torch.cuda.device(device):
+            _lib.check(lib.qmb_terms_build(n_qubytes, particle_cut, _ptr(site), _ptr(kind), _ptr(coef), self.n_terms, _ptr(self.buffer),
+                                           self.buffer.numel(), ctypes.byref(self.desc), _stream(device)), "qmb_terms_build")
+
+    def counts(self) -> tuple[int, int, int]:
+        """(valid terms, diagonal terms, invalid terms) -- synchronises."""
+        offset = int(self.desc.counts) - self.buffer.data_ptr()
+        v = self.buffer[offset:offset + 32].view(torch.int64).tolist()
+        return int(v[0]), int(v[1]), int(v[2])
+
+
+class PreparedTargets:
+    """Hash index over a config set on one device (qmb_targets_build)."""
+
+    def __init__(self, configs: torch.Tensor, n_qubytes: int) -> None:
+        configs = configs.contiguous()
+        if configs.dtype != torch.uint8 or configs.dim() != 2 or configs.size(1) != n_qubytes:
+            raise RuntimeError("configs must be a uint8 [n, n_qubytes] tensor.")
+        device = configs.device
+        self.configs = configs
+        self.n = configs.size(0)
+        self.n_qubytes = n_qubytes
+        self.device = device
+        lib = _lib.library()
+        size = ctypes.c_size_t(0)
+        _lib.check(lib.qmb_targets_workspace_size(n_qubytes, self.n, ctypes.byref(size)), "qmb_targets_workspace_size")
+        self.buffer = _workspace(size.value, device)
+        self.desc = _lib.QmbTargets()
+        with torch.cuda.device(device):
+            _lib.check(lib.qmb_targets_build(n_qubytes, _ptr(configs), self.n, _ptr(self.buffer), self.buffer.numel(),
+                                             ctypes.byref(self.desc), _stream(device)), "qmb_targets_build")
+
+
+def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tensor, vec: torch.Tensor, mode: int,
+          out: torch.Tensor | None = None, tile: int = 0, groups: int = 0, zero: bool = True, strategy: int = 0) -> torch.Tensor:
+    """mode 0: psi_j = A psi_i (vec = psi_i as float64 [n_i, 2]); mode 1: grad_i = A^H grad_j.
+
+    `out` (float64 [n, 2]) is accumulated into when given, after zeroing unless zero=False.
+    """
+    configs_i = configs_i.contiguous()
+    vec = vec.contiguous()
+    n_i = configs_i.size(0)
+    n_out = targets.n if mode == MODE_FORWARD else n_i
+    device = configs_i.device
+    if out is None:
+        out = torch.zeros((n_out, 2), dtype=torch.float64, device=device)
+    elif zero:
+        out.zero_()
+    launch = _launch(tile, groups, strategy)
+    lib = _lib.library()
+    with torch.cuda.device(device):
+        _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.desc), ctypes.byref(targets.desc), _ptr(configs_i), n_i, _ptr(vec), _ptr(out),
+                                          mode, ctypes.byref(launch) if launch else None, _stream(device)), "qmb_apply_prepared")
+    return out
+
+
+def count(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tensor, tile: int = 0, groups: int = 0,
+          strategy: int = 0) -> tuple[int, int]:
+    """(M_gen, M_hit): successful term applications and those whose target is indexed -- synchronises."""
+    configs_i = configs_i.contiguous()
+    device = configs_i.device
+    counters = torch.zeros(2, dtype=torch.int64, device=device)
+    launch = _launch(tile, groups, strategy)
+    lib = _lib.library()
+    with torch.cuda.device(device):
+        _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.desc), ctypes.byref(targets.desc), _ptr(configs_i), configs_i.size(0), None,
+                                          _ptr(counters), MODE_COUNT, ctypes.byref(launch) if launch else None, _stream(device)),
+                   "qmb_apply_prepared(count)")
+    gen, hit = counters.tolist()
+    return int(gen), int(hit)

@@ -374,7 +374,9 @@ def excitation_configs(n_orbit: int, n_alpha: int, n_beta: int, count: int, seed
         return base[keep]
     extra = np.zeros((0, base.shape[1]), dtype=np.uint8)
     need = count - len(base)
+    stalled = 0
     while len(extra) < need:
+        before = len(extra)
         m = int((need - len(extra)) * 1.3) + 64
         level = rng.integers(3, 5, size=m)
         cap_a = min(n_alpha, n_orbit - n_alpha)
@@ -393,6 +395,9 @@ def excitation_configs(n_orbit: int, n_alpha: int, n_beta: int, count: int, seed
         occ[:, 2 * occ_b + 1] = np.where(rb_h, 0, occ[:, 2 * occ_b + 1])
         occ[:, 2 * vir_b + 1] = np.where(rb_p, 1, occ[:, 2 * vir_b + 1])
         extra = sort_unique_configs(np.concatenate([extra, pack_occupation(occ)]))
+        stalled = stalled + 1 if len(extra) == before else 0
+        if stalled >= 8:
+            raise ValueError(f"only {len(base) + len(extra)} configs up to quadruple excitations exist; asked for {count}")
     keep = np.sort(rng.choice(len(extra), size=need, replace=False))
     return sort_unique_configs(np.concatenate([base, extra[keep]]))
 

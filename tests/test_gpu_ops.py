@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 import torch
 
+from brute import contribution_weights
 from conftest import sorted_terms
 from oracle import oracle as orc
 from qmb_hip import Hamiltonian, ops
@@ -38,22 +39,10 @@ def _rows(a: np.ndarray) -> set[bytes]:
 def _weights(pc: int, cfg: np.ndarray, psi: np.ndarray, exclude: np.ndarray, s: np.ndarray, k: np.ndarray,
              c: np.ndarray) -> dict[bytes, float]:
     """Max contribution weight per reachable non-excluded config (brute force over terms x configs)."""
-    ex = _rows(exclude)
     best: dict[bytes, float] = {}
-    for i in range(len(cfg)):
-        for t in range(len(s)):
-            ok, out, par = orc.apply_term(cfg[i], s[t], k[t], pc)
-            if not ok:
-                continue
-            key = bytes(out)
-            if key in ex:
-                continue
-            sign = -1.0 if par else 1.0
-            re = sign * (c[t, 0] * psi[i].real - c[t, 1] * psi[i].imag)
-            im = sign * (c[t, 0] * psi[i].imag + c[t, 1] * psi[i].real)
-            w = re * re + im * im
-            if w > best.get(key, -1.0):
-                best[key] = w
+    for w, key in contribution_weights(pc, cfg, psi, exclude, s, k, c):
+        if w > best.get(key, -1.0):
+            best[key] = w
     return best
 
 

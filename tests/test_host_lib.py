@@ -1,0 +1,144 @@
+"""
+Host-side checks of libqmb_hip.so that need no GPU: the library loads, it exports every entry point
+include/qmb_hip.h declares, and its term compiler + record application (the code every kernel runs)
+agree bit for bit with the oracle's restatement of qmb/_hamiltonian_cpu.cpp:48-78.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from qmb_hip import _lib
+
+HEADER = pathlib.Path(__file__).resolve().parent.parent / "include" / "qmb_hip.h"
+
+
+def declared_functions() -> list[str]:
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(qmb_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function() -> None:
+    lib = _lib.library()
+    names = declared_functions()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(_lib.exported_symbols())
+
+
+def test_version_and_error_channel() -> None:
+    lib = _lib.library()
+    assert lib.qmb_version() >= 10000
+    out = ctypes.c_size_t(0)
+    assert lib.qmb_terms_workspace_size(0, 10, ctypes.byref(out)) != 0
+    assert b"n_qubytes" in lib.qmb_last_error()
+
+
+def _compile(site4: np.ndarray, kind4: np.ndarray, pc: int, nq: int) -> tuple[int, int, int, int]:
+    lib = _lib.library()
+    s = np.ascontiguousarray(site4, dtype=np.int16)
+    k = np.ascontiguousarray(kind4, dtype=np.uint8)
+    rec = ctypes.c_uint64(0)
+    status = ctypes.c_int32(0)
+    bucket = ctypes.c_int64(0)
+    diag = ctypes.c_int32(0)
+    _lib.check(lib.qmb_compile_term_host(s.ctypes.data, k.ctypes.data, pc, nq, ctypes.byref(rec), ctypes.byref(status),
+                                         ctypes.byref(bucket), ctypes.byref(diag)), "compile")
+    return rec.value, status.value, bucket.value, diag.value
+
+
+def _apply(rec: int, config: np.ndarray) -> tuple[bool, np.ndarray, int]:
+    lib = _lib.library()
+    cfg = np.ascontiguousarray(config, dtype=np.uint8)
+    out = np.zeros_like(cfg)
+    par = ctypes.c_int32(0)
+    ok = lib.qmb_apply_record_host(rec, len(cfg), cfg.ctypes.data, out.ctypes.data, ctypes.byref(par))
+    return bool(ok), out, par.value
+
+
+@pytest.mark.parametrize("pc", [1, 2])
+@pytest.mark.parametrize("nq", [1, 2, 9, 17, 32])
+def test_compiled_records_match_oracle(pc: int, nq: int) -> None:
+    rng = np.random.default_rng(1000 * pc + nq)
+    n_bits = 8 * nq
+    for _ in range(400):
+        n_ops = int(rng.integers(0, 5))
+        pool = rng.integers(0, n_bits, size=3)  # few distinct sites -> repeated sites are common
+        site = np.zeros(4, dtype=np.int16)
+        kind = np.full(4, 2, dtype=np.uint8)
+        site[:n_ops] = rng.choice(pool, size=n_ops)
+        kind[:n_ops] = rng.integers(0, 2, size=n_ops)
+        rec, status, bucket, diag = _compile(site, kind, pc, nq)
+        assert status in (0, 1)
+        for _ in range(8):
+            cfg = rng.integers(0, 256, size=nq, dtype=np.uint8)
+            if rng.random() < 0.5:  # bias toward configs on which the term can succeed
+                for op in range(n_ops):
+                    s = int(site[op])
+                    want_bit = 1 - int(kind[op])
+                    cfg[s // 8] = (int(cfg[s // 8]) & (255 ^ (1 << (s % 8)))) | (want_bit << (s % 8))
+            ok_o, out_o, par_o = orc.apply_term(cfg, site, kind, pc)
+            ok_l, out_l, par_l = _apply(rec, cfg) if status == 0 else (False, cfg, 0)
+            assert ok_o == ok_l, (site, kind, cfg)
+            if ok_o:
+                assert np.array_equal(out_o, out_l) and par_o == par_l, (site, kind, cfg)
+                if diag:
+                    assert np.array_equal(out_o, cfg)
+        assert (bucket >= 0) == (status == 0)
+
+
+def test_compile_rejects_bad_terms() -> None:
+    assert _compile(np.array([9, 0, 0, 0]), np.array([1, 2, 2, 2]), 1, 1)[1] == 2  # site outside one byte
+    assert _compile(np.array([1, 0, 0, 0]), np.array([3, 2, 2, 2]), 1, 1)[1] == 3  # kind 3
+    assert _compile(np.array([1, 1, 0, 0]), np.array([0, 0, 2, 2]), 1, 1)[1] == 1  # a_1 a_1 is zero
+
+
+def test_bucket_is_required_occupied_set() -> None:
+    nq = 2
+    n_bits = 16
+    # a+_5 a+_3 a_9 a_2: requires 2 and 9 occupied -> pair bucket of (2, 9)
+    _, status, bucket, _ = _compile(np.array([5, 3, 9, 2]), np.array([1, 1, 0, 0]), 1, nq)
+    assert status == 0 and bucket == 1 + n_bits + 9 * 8 // 2 + 2
+    # n_4 = a+_4 a_4: requires 4 -> single bucket
+    assert _compile(np.array([4, 4, 0, 0]), np.array([1, 0, 2, 2]), 1, nq)[2] == 1 + 4
+    # identity -> bucket 0
+    assert _compile(np.array([0, 0, 0, 0]), np.array([2, 2, 2, 2]), 1, nq)[2] == 0
+
+
+def test_philox_known_answers() -> None:
+    """Philox4x32-10 known-answer vectors of the Random123 distribution (Salmon et al., SC'11)."""
+    lib = _lib.library()
+    cases = [
+        ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0], [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, want in cases:
+        c = (ctypes.c_uint32 * 4)(*ctr)
+        k = (ctypes.c_uint32 * 2)(*key)
+        out = (ctypes.c_uint32 * 4)()
+        lib.qmb_philox4x32_10_host(c, k, out)
+        assert list(out) == want
+
+
+def test_philox_uniform_range_and_moments() -> None:
+    lib = _lib.library()
+    u = np.array([lib.qmb_philox_uniform_host(12345, i) for i in range(20000)])
+    assert np.all(u > 0) and np.all(u <= 1)
+    assert abs(u.mean() - 0.5) < 0.01 and abs(u.var() - 1 / 12) < 0.005
+
+
+def test_validate_terms_host() -> None:
+    lib = _lib.library()
+    site = np.array([[1, 2, 0, 0], [9, 0, 0, 0], [3, 0, 0, 0]], dtype=np.int16)
+    kind = np.array([[1, 0, 2, 2], [1, 2, 2, 2], [7, 2, 2, 2]], dtype=np.uint8)
+    bad = ctypes.c_int64(0)
+    _lib.check(lib.qmb_validate_terms_host(1, site.ctypes.data, kind.ctypes.data, 3, ctypes.byref(bad)), "validate")
+    assert bad.value == 2
