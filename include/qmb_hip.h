@@ -56,6 +56,12 @@ int qmb_compile_term_host(const int16_t* site4, const uint8_t* kind4, int partic
  * Returns 1 on success and writes the resulting config and parity bit. */
 int qmb_apply_record_host(uint64_t rec, int n_qubytes, const uint8_t* config, uint8_t* result, int32_t* parity);
 
+/* Host-only Hermiticity test of a term table held in HOST memory: *hermitian = 1 when the operator
+ * sum_t coef_t O_t equals its adjoint, comparing the compiled form of every term with that of its
+ * adjoint (terms with the same action are summed first) to a relative tolerance `tol`. */
+int qmb_terms_hermitian_host(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
+                             int64_t n_terms, double tol, int32_t* hermitian);
+
 /* Host-only Philox4x32-10 block (for known-answer tests) and the (0,1] uniform built on it. */
 void qmb_philox4x32_10_host(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 double qmb_philox_uniform_host(uint64_t seed, uint64_t index);
@@ -146,15 +152,20 @@ int qmb_terms_workspace_size(int n_qubytes, int64_t n_terms, size_t* bytes);
 int qmb_terms_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
                     int64_t n_terms, void* buffer, size_t bytes, qmb_terms* out, void* stream);
 
-/* A hash index over a config set: padded key words plus an open-addressing table. */
+/* A hash index over a config set: padded key words, an open-addressing table and a blocked Bloom
+ * filter (16 bits per key) that answers most misses from L2 before the table is touched. */
 typedef struct qmb_targets {
     int32_t n_qubytes, n_words;
     int64_t n;
-    uint64_t capacity; /* power of two */
+    uint64_t capacity;     /* power of two, >= 4 n */
+    uint64_t filter_words; /* power of two, ~ n / 4 */
     void* buffer;
     size_t bytes;
     uint64_t* keys;    /* [n][n_words] */
     uint64_t* table;   /* [capacity] (index << 32 | fingerprint) or ~0 */
+    uint64_t* filter;  /* [filter_words] */
+    uint64_t* cols;    /* [256] hash column per site: H(config) = XOR of cols over occupied sites */
+    int64_t* duplicates; /* device: number of rows equal to an earlier row */
 } qmb_targets;
 
 int qmb_targets_workspace_size(int n_qubytes, int64_t n, size_t* bytes);
@@ -167,6 +178,10 @@ typedef struct qmb_launch {
     int32_t groups;   /* bucketed strategy: bucket groups per tile (bucket b goes to group b % groups) */
     int32_t strategy; /* 0 auto (direct for <= 4096 terms), 1 bucketed tiles, 2 direct (every config
                          walks every term from LDS) */
+    int32_t symmetric; /* 1: the caller asserts the operator is Hermitian (see qmb_terms_hermitian_host)
+                          and configs_i is the indexed set itself with no repeated rows; each connected
+                          pair is then looked up once and both A_ji and conj(A_ji) are applied.  Mode 1
+                          then computes the same map as mode 0 (A^H = A). */
 } qmb_launch;
 
 /* mode 0: forward scatter  out[k] += sum sign*coef*in[i]      (in = psi_i [n_i], out = psi_j [n_j])

@@ -27,30 +27,39 @@ struct ApplyParams {
     const uint64_t* tkeys;
     const uint64_t* table;
     uint64_t mask;
+    const uint64_t* filter;
+    uint64_t fmask;
+    const uint64_t* cols;
     int tile_log2;
     int groups;
 };
 
 // out[j] += sign * coef * in[i]: scatter with hardware FP64 atomics.
-template <int W>
+// SYM: the operator is Hermitian and the index covers the source set itself, so each connected pair
+// (i, j) is handled once, from the end with the smaller index hash, applying both A_ji and
+// A_ij = conj(A_ji); connections seen from the other end are skipped before their lookup.
+template <int W, bool SYM>
 struct ForwardAction {
     const ApplyParams& p;
     __device__ explicit ForwardAction(const ApplyParams& pp) : p(pp) {}
-    __device__ __forceinline__ void on_success(int64_t i, uint32_t t, const Cfg<W>& o, int par) {
-        const int64_t j = lookup<W>(p.table, p.mask, p.tkeys, o);
+    __device__ __forceinline__ bool pre(int64_t, uint32_t, const Cfg<W>&, int, uint64_t hs, uint64_t ht) {
+        return !SYM || ht >= hs;
+    }
+    __device__ __forceinline__ void post(int64_t i, uint32_t t, const Cfg<W>&, int par, int64_t j, bool distinct) {
         if (j < 0) {
             return;
         }
         const double2 cf = p.coefs[t];
         const double2 v = p.in[i];
-        double re = cf.x * v.x - cf.y * v.y;
-        double im = cf.x * v.y + cf.y * v.x;
-        if (par) {
-            re = -re;
-            im = -im;
+        const double sc_re = par ? -cf.x : cf.x;
+        const double sc_im = par ? -cf.y : cf.y;
+        unsafeAtomicAdd(&p.out[j].x, sc_re * v.x - sc_im * v.y);
+        unsafeAtomicAdd(&p.out[j].y, sc_re * v.y + sc_im * v.x);
+        if (SYM && distinct) {
+            const double2 g = p.in[j];
+            unsafeAtomicAdd(&p.out[i].x, sc_re * g.x + sc_im * g.y);
+            unsafeAtomicAdd(&p.out[i].y, sc_re * g.y - sc_im * g.x);
         }
-        unsafeAtomicAdd(&p.out[j].x, re);
-        unsafeAtomicAdd(&p.out[j].y, im);
     }
 };
 
@@ -59,8 +68,8 @@ template <int W>
 struct AdjointAction {
     const ApplyParams& p;
     __device__ explicit AdjointAction(const ApplyParams& pp) : p(pp) {}
-    __device__ __forceinline__ void on_success(int64_t i, uint32_t t, const Cfg<W>& o, int par) {
-        const int64_t j = lookup<W>(p.table, p.mask, p.tkeys, o);
+    __device__ __forceinline__ bool pre(int64_t, uint32_t, const Cfg<W>&, int, uint64_t, uint64_t) { return true; }
+    __device__ __forceinline__ void post(int64_t i, uint32_t t, const Cfg<W>&, int par, int64_t j, bool) {
         if (j < 0) {
             return;
         }
@@ -84,10 +93,11 @@ struct CountAction {
     const ApplyParams& p;
     unsigned long long gen, hit;
     __device__ explicit CountAction(const ApplyParams& pp) : p(pp), gen(0), hit(0) {}
-    __device__ __forceinline__ void on_success(int64_t, uint32_t, const Cfg<W>& o, int) {
+    __device__ __forceinline__ bool pre(int64_t, uint32_t, const Cfg<W>&, int, uint64_t, uint64_t) {
         ++gen;
-        hit += lookup<W>(p.table, p.mask, p.tkeys, o) >= 0 ? 1 : 0;
+        return true;
     }
+    __device__ __forceinline__ void post(int64_t, uint32_t, const Cfg<W>&, int, int64_t j, bool) { hit += j >= 0 ? 1 : 0; }
     __device__ __forceinline__ void flush() {
         unsigned long long g = gen, h = hit;
         for (int d = 32; d > 0; d >>= 1) {
@@ -105,18 +115,22 @@ struct CountAction {
 template <int W, int MODE, int DIRECT>
 __global__ __launch_bounds__(kBlock) void apply_kernel(ApplyParams p) {
     extern __shared__ uint64_t smem[];
+    const HashIndex ix{p.table, p.tkeys, p.mask, p.filter, p.fmask, p.cols};
     auto run = [&](auto& act) {
         if (DIRECT) {
-            enumerate_direct<W>(p.src, blockIdx.x, *reinterpret_cast<DirectLds<W>*>(smem), act);
+            enumerate_direct<W>(p.src, ix, blockIdx.x, *reinterpret_cast<DirectLds<W>*>(smem), act);
         } else {
-            enumerate_tile<W>(p.src, blockIdx.x / p.groups, p.tile_log2, blockIdx.x % p.groups, p.groups, smem, act);
+            enumerate_tile<W>(p.src, ix, blockIdx.x / p.groups, p.tile_log2, blockIdx.x % p.groups, p.groups, smem, act);
         }
     };
     if (MODE == 0) {
-        ForwardAction<W> act(p);
+        ForwardAction<W, false> act(p);
         run(act);
     } else if (MODE == 1) {
         AdjointAction<W> act(p);
+        run(act);
+    } else if (MODE == 3) {
+        ForwardAction<W, true> act(p);
         run(act);
     } else {
         CountAction<W> act(p);
@@ -298,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void single_rel_kernel(const uint8_t* __res
                                                            const uint32_t* __restrict__ order, const uint64_t* __restrict__ orecs,
                                                            const uint8_t* __restrict__ usable, int64_t n_terms,
                                                            const uint64_t* __restrict__ tkeys, const uint64_t* __restrict__ table,
-                                                           uint64_t mask, uint8_t* __restrict__ out) {
+                                                           uint64_t mask, const uint64_t* __restrict__ cols, uint8_t* __restrict__ out) {
     const int64_t i = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x;
     if (i >= n) {
         return;
@@ -315,7 +329,7 @@ __global__ __launch_bounds__(kBlock) void single_rel_kernel(const uint8_t* __res
         if (!apply_rec<W>(rec, c, o, par)) {
             continue;
         }
-        if (lookup<W>(table, mask, tkeys, o) >= 0) {
+        if (lookup<W>(table, mask, tkeys, cols, o) >= 0) {
             continue;
         }
         store_row<W>(out + i * nq, nq, o);
@@ -338,6 +352,7 @@ LaunchShape resolve_launch(const qmb_launch* launch, int n_words, int64_t n_i, i
         strategy = n_terms <= kDirectMaxTerms ? 2 : 1;
     }
     s.direct = strategy == 2;
+    s.symmetric = launch != nullptr && launch->symmetric != 0;
     if (launch && launch->tile > 0) {
         tile = launch->tile;
     } else {
@@ -364,7 +379,13 @@ LaunchShape resolve_launch(const qmb_launch* launch, int n_words, int64_t n_i, i
     return s;
 }
 
-size_t direct_lds_bytes(int n_words) { return static_cast<size_t>(kDirectChunk) * (32 * n_words + 4); }
+size_t direct_lds_bytes(int n_words) {
+    static_assert(sizeof(DirectLds<1>) <= static_cast<size_t>(kDirectChunk) * (32 * 1 + 12), "DirectLds<1> layout");
+    static_assert(sizeof(DirectLds<2>) <= static_cast<size_t>(kDirectChunk) * (32 * 2 + 12), "DirectLds<2> layout");
+    static_assert(sizeof(DirectLds<3>) <= static_cast<size_t>(kDirectChunk) * (32 * 3 + 12), "DirectLds<3> layout");
+    static_assert(sizeof(DirectLds<4>) <= static_cast<size_t>(kDirectChunk) * (32 * 4 + 12), "DirectLds<4> layout");
+    return static_cast<size_t>(kDirectChunk) * (32 * n_words + 12);
+}
 
 size_t enum_lds_bytes(int n_words, int n_bits, int tile_log2) { return enum_lds_bytes_of(n_words, n_bits, tile_log2); }
 
@@ -394,6 +415,9 @@ int apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const uin
     p.tkeys = targets->keys;
     p.table = targets->table;
     p.mask = targets->capacity - 1;
+    p.filter = targets->filter;
+    p.fmask = targets->filter_words - 1;
+    p.cols = targets->cols;
     p.tile_log2 = ls.tile_log2;
     p.groups = ls.groups;
     int64_t grid;
@@ -421,13 +445,17 @@ int apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const uin
             QMB_CHECK_LAUNCH();
             return QMB_OK;
         };
+        // A Hermitian operator on its own index: forward and adjoint are the same map.
+        const int m = (ls.symmetric && mode != 2) ? 3 : mode;
         if (ls.direct) {
-            if (mode == 0) return launch_one(&apply_kernel<W, 0, 1>);
-            if (mode == 1) return launch_one(&apply_kernel<W, 1, 1>);
+            if (m == 0) return launch_one(&apply_kernel<W, 0, 1>);
+            if (m == 1) return launch_one(&apply_kernel<W, 1, 1>);
+            if (m == 3) return launch_one(&apply_kernel<W, 3, 1>);
             return launch_one(&apply_kernel<W, 2, 1>);
         }
-        if (mode == 0) return launch_one(&apply_kernel<W, 0, 0>);
-        if (mode == 1) return launch_one(&apply_kernel<W, 1, 0>);
+        if (m == 0) return launch_one(&apply_kernel<W, 0, 0>);
+        if (m == 1) return launch_one(&apply_kernel<W, 1, 0>);
+        if (m == 3) return launch_one(&apply_kernel<W, 3, 0>);
         return launch_one(&apply_kernel<W, 2, 0>);
     });
 }
@@ -518,7 +546,7 @@ int single_relative_prepared(int n_qubytes, int particle_cut, const uint8_t* con
     return dispatch_words(n_words_of(n_qubytes), [&](auto wc) -> int {
         constexpr int W = decltype(wc)::value;
         single_rel_kernel<W><<<blocks_for(n), kBlock, 0, stream>>>(configs, n, n_qubytes, order, orecs, usable, n_terms,
-                                                                   exclude->keys, exclude->table, exclude->capacity - 1, out);
+                                                                   exclude->keys, exclude->table, exclude->capacity - 1, exclude->cols, out);
         QMB_CHECK_LAUNCH();
         return QMB_OK;
     });

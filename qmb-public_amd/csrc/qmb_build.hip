@@ -156,8 +156,17 @@ __global__ void keys_kernel(const uint8_t* __restrict__ configs, int64_t n, int 
     }
 }
 
+__global__ void hash_columns_kernel(uint64_t* __restrict__ cols, int n) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < n) {
+        cols[b] = hash_column(b);
+    }
+}
+
 template <int W>
-__global__ void insert_kernel(const uint64_t* __restrict__ keys, int64_t n, uint64_t* __restrict__ table, uint64_t mask) {
+__global__ void insert_kernel(const uint64_t* __restrict__ keys, int64_t n, uint64_t* __restrict__ table, uint64_t mask,
+                              uint64_t* __restrict__ filter, uint64_t fmask, const uint64_t* __restrict__ cols,
+                              unsigned long long* __restrict__ dups) {
     int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i >= n) {
         return;
@@ -167,10 +176,11 @@ __global__ void insert_kernel(const uint64_t* __restrict__ keys, int64_t n, uint
     for (int k = 0; k < W; ++k) {
         c.w[k] = keys[i * W + k];
     }
-    const uint64_t h = hash_cfg<W>(c);
+    const uint64_t h = hash_cfg<W>(c, cols);
     const uint32_t fp = static_cast<uint32_t>(h);
     const uint64_t entry = (static_cast<uint64_t>(i) << 32) | fp;
     uint64_t slot = (h >> 32) & mask;
+    atomicOr(reinterpret_cast<unsigned long long*>(&filter[filter_word(h, fmask)]), static_cast<unsigned long long>(filter_bits(h)));
     // Bounded by the capacity: the table is at most a quarter full, so an empty slot always exists.
     for (uint64_t probe = 0; probe <= mask; ++probe) {
         unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&table[slot]),
@@ -188,6 +198,7 @@ __global__ void insert_kernel(const uint64_t* __restrict__ keys, int64_t n, uint
             if (cfg_equal<W>(o, c)) {
                 // duplicate row: the lowest row index keeps the slot
                 atomicMin(reinterpret_cast<unsigned long long*>(&table[slot]), static_cast<unsigned long long>(entry));
+                atomicAdd(dups, 1ull);
                 return;
             }
         }
@@ -245,20 +256,23 @@ int keys_dispatch(int W, const uint8_t* configs, int64_t n, int nq, uint64_t* ke
 }
 
 template <int W>
-int insert_launch(const uint64_t* keys, int64_t n, uint64_t* table, uint64_t mask, hipStream_t stream) {
+int insert_launch(const uint64_t* keys, int64_t n, uint64_t* table, uint64_t mask, uint64_t* filter, uint64_t fmask,
+                  const uint64_t* cols, int64_t* dups, hipStream_t stream) {
     if (n > 0) {
-        insert_kernel<W><<<blocks_for(n), kBlock, 0, stream>>>(keys, n, table, mask);
+        insert_kernel<W><<<blocks_for(n), kBlock, 0, stream>>>(keys, n, table, mask, filter, fmask, cols,
+                                                                 reinterpret_cast<unsigned long long*>(dups));
         QMB_CHECK_LAUNCH();
     }
     return QMB_OK;
 }
 
-int insert_dispatch(int W, const uint64_t* keys, int64_t n, uint64_t* table, uint64_t mask, hipStream_t stream) {
+int insert_dispatch(int W, const uint64_t* keys, int64_t n, uint64_t* table, uint64_t mask, uint64_t* filter, uint64_t fmask,
+                    const uint64_t* cols, int64_t* dups, hipStream_t stream) {
     switch (W) {
-        case 1: return insert_launch<1>(keys, n, table, mask, stream);
-        case 2: return insert_launch<2>(keys, n, table, mask, stream);
-        case 3: return insert_launch<3>(keys, n, table, mask, stream);
-        default: return insert_launch<4>(keys, n, table, mask, stream);
+        case 1: return insert_launch<1>(keys, n, table, mask, filter, fmask, cols, dups, stream);
+        case 2: return insert_launch<2>(keys, n, table, mask, filter, fmask, cols, dups, stream);
+        case 3: return insert_launch<3>(keys, n, table, mask, filter, fmask, cols, dups, stream);
+        default: return insert_launch<4>(keys, n, table, mask, filter, fmask, cols, dups, stream);
     }
 }
 
@@ -420,8 +434,12 @@ int targets_layout(Carver& cv, int n_qubytes, int64_t n, qmb_targets* out) {
     t.n = n;
     t.capacity = next_pow2(static_cast<uint64_t>(4 * n > 64 ? 4 * n : 64));  // load factor <= 1/4
     t.buffer = cv.base;
+    t.filter_words = next_pow2(static_cast<uint64_t>(n / 4 > 64 ? n / 4 : 64));  // 16 bits per key
     t.keys = cv.take<uint64_t>(static_cast<size_t>(n > 0 ? n : 1) * t.n_words);
     t.table = cv.take<uint64_t>(t.capacity);
+    t.filter = cv.take<uint64_t>(t.filter_words);
+    t.cols = cv.take<uint64_t>(64 * kMaxWords);
+    t.duplicates = cv.take<int64_t>(1);
     t.bytes = cv.off;
     if (out) {
         *out = t;
@@ -440,7 +458,12 @@ int targets_build(int n_qubytes, const uint8_t* configs, int64_t n, void* buffer
     t.bytes = bytes;
     QMB_TRY(keys_dispatch(t.n_words, configs, n, n_qubytes, t.keys, stream));
     QMB_TRY(launch_fill_u64(t.table, kEmptySlot, static_cast<int64_t>(t.capacity), stream));
-    QMB_TRY(insert_dispatch(t.n_words, t.keys, n, t.table, t.capacity - 1, stream));
+    QMB_TRY(launch_fill_u64(t.filter, 0ull, static_cast<int64_t>(t.filter_words), stream));
+    QMB_TRY(launch_fill_u64(reinterpret_cast<uint64_t*>(t.duplicates), 0ull, 1, stream));
+    hash_columns_kernel<<<1, 64 * kMaxWords, 0, stream>>>(t.cols, 64 * kMaxWords);
+    QMB_CHECK_LAUNCH();
+    QMB_TRY(insert_dispatch(t.n_words, t.keys, n, t.table, t.capacity - 1, t.filter, t.filter_words - 1, t.cols, t.duplicates,
+                            stream));
     *out = t;
     return QMB_OK;
 }

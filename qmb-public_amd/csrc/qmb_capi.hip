@@ -4,8 +4,13 @@
 // the caller's workspace out into a compiled term table and a hash index over the config set it
 // needs, enqueues the builders and the kernel on the caller's stream, and returns.  Nothing here
 // allocates device memory or synchronises.
+#include <algorithm>
+#include <array>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <utility>
+#include <vector>
 
 #include "qmb_internal.hpp"
 
@@ -162,6 +167,78 @@ int qmb_apply_record_host(uint64_t rec, int n_qubytes, const uint8_t* config, ui
         }
     }
     return ok ? 1 : 0;
+}
+
+int qmb_terms_hermitian_host(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
+                             int64_t n_terms, double tol, int32_t* hermitian) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    QMB_TRY(check_particle_cut(particle_cut));
+    QMB_TRY(check_ptr(site, n_terms, "site"));
+    QMB_TRY(check_ptr(kind, n_terms, "kind"));
+    QMB_TRY(check_ptr(coef, n_terms, "coef"));
+    if (!hermitian) {
+        return set_error(QMB_ERR_ARGUMENT, "hermitian must be non-null");
+    }
+    // An operator term acts as: configs with (c & m) == v map to c ^ f with sign
+    // (-1)^(popcount(c & p) ^ pconst).  Its adjoint maps (c & m) == v ^ f back with the same sign
+    // function (p excludes the touched sites, f lies within them), so in the key (m, v, f, p) it is
+    // (m, v ^ f, f, p) with the conjugate of coef * (-1)^pconst.
+    using Key = std::array<uint64_t, 4 * kMaxWords>;
+    std::vector<std::pair<Key, std::pair<double, double>>> entries;
+    entries.reserve(static_cast<size_t>(n_terms));
+    double scale = 0.0;
+    for (int64_t t = 0; t < n_terms; ++t) {
+        const CompiledTerm c = compile_term(site + 4 * t, kind + 4 * t, particle_cut, 8 * n_qubytes);
+        if (c.status == kCompileBadSite || c.status == kCompileBadKind) {
+            return set_error(QMB_ERR_ARGUMENT, "term %lld is invalid", static_cast<long long>(t));
+        }
+        if (c.status != kCompileOk) {
+            continue;  // the zero operator
+        }
+        const Masks<kMaxWords> x = expand_rec<kMaxWords>(c.rec, nullptr);
+        Key k{};
+        for (int w = 0; w < kMaxWords; ++w) {
+            k[w] = x.m[w];
+            k[kMaxWords + w] = x.v[w];
+            k[2 * kMaxWords + w] = x.f[w];
+            k[3 * kMaxWords + w] = x.p[w];
+        }
+        const double s = x.pconst ? -1.0 : 1.0;
+        entries.push_back({k, {s * coef[2 * t], s * coef[2 * t + 1]}});
+        scale = std::max(scale, std::hypot(coef[2 * t], coef[2 * t + 1]));
+    }
+    std::sort(entries.begin(), entries.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    std::vector<std::pair<Key, std::pair<double, double>>> merged;
+    for (const auto& e : entries) {
+        if (!merged.empty() && merged.back().first == e.first) {
+            merged.back().second.first += e.second.first;
+            merged.back().second.second += e.second.second;
+        } else {
+            merged.push_back(e);
+        }
+    }
+    const double bound = tol * (scale > 0.0 ? scale : 1.0);
+    int32_t ok = 1;
+    for (const auto& e : merged) {
+        Key adj = e.first;
+        for (int w = 0; w < kMaxWords; ++w) {
+            adj[kMaxWords + w] ^= adj[2 * kMaxWords + w];
+        }
+        const auto it = std::lower_bound(merged.begin(), merged.end(), adj,
+                                         [](const auto& a, const Key& b) { return a.first < b; });
+        double re = 0.0, im = 0.0;
+        if (it != merged.end() && it->first == adj) {
+            re = it->second.first;
+            im = it->second.second;
+        }
+        if (std::hypot(re - e.second.first, im + e.second.second) > bound) {
+            ok = 0;
+            break;
+        }
+    }
+    *hermitian = ok;
+    return QMB_OK;
 }
 
 void qmb_philox4x32_10_host(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {

@@ -265,6 +265,62 @@ __host__ __device__ __forceinline__ bool apply_rec(uint64_t rec, const Cfg<W>& c
     return true;
 }
 
+// A compiled record expanded to word masks: the term succeeds on c iff (c & m) == v for every word;
+// the target is c ^ f and the parity popcount(c & p) ^ pconst (p excludes the touched sites).
+template <int W>
+struct Masks {
+    uint64_t m[W], v[W], f[W], p[W];
+    uint64_t hf;  // index hash of the flip mask f
+    int pconst;
+};
+
+template <int W>
+__host__ __device__ __forceinline__ Masks<W> expand_rec(uint64_t rec, const uint64_t* __restrict__ cols) {
+    Masks<W> x;
+    x.hf = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        x.m[w] = 0;
+        x.v[w] = 0;
+        x.f[w] = 0;
+        x.p[w] = 0;
+    }
+    const uint32_t valid = static_cast<uint32_t>(rec >> 32) & 0xf;
+    const uint32_t req = static_cast<uint32_t>(rec >> 36) & 0xf;
+    const uint32_t fin = static_cast<uint32_t>(rec >> 40) & 0xf;
+    const uint32_t odd = static_cast<uint32_t>(rec >> 44) & 0xf;
+#pragma unroll
+    for (int k = 0; k < kMaxOps; ++k) {
+        if ((valid >> k) & 1u) {
+            const int s = static_cast<int>((rec >> (8 * k)) & 0xff);
+            const uint64_t bit = 1ull << (s & 63);
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                if (w == (s >> 6)) {
+                    x.m[w] |= bit;
+                    x.v[w] |= ((req >> k) & 1u) ? bit : 0ull;
+                    x.f[w] |= (((req ^ fin) >> k) & 1u) ? bit : 0ull;
+                }
+            }
+            if (cols != nullptr && (((req ^ fin) >> k) & 1u)) {
+                x.hf ^= cols[s];
+            }
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                if ((odd >> k) & 1u) {
+                    x.p[w] ^= low_mask_word(s, w);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        x.p[w] &= ~x.m[w];
+    }
+    x.pconst = static_cast<int>((rec >> 48) & 1);
+    return x;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Hashing (MurmurHash3 64-bit finaliser, a bijection on 64-bit words).
 // ---------------------------------------------------------------------------------------------
@@ -277,14 +333,33 @@ __host__ __device__ __forceinline__ uint64_t fmix64(uint64_t k) {
     return k;
 }
 
+// The index hash is linear over GF(2): H(x) = XOR of cols[s] over the occupied sites s, with cols
+// pseudorandom 64-bit vectors.  Random linear maps are a 2-universal family, and linearity gives
+// H(c ^ f) = H(c) ^ H(f): a connection's target hash is the source's hash XOR the term's flip hash.
+__host__ __device__ __forceinline__ uint64_t hash_column(int site) {
+    return fmix64(0x6a09e667f3bcc909ull + 0x9e3779b97f4a7c15ull * static_cast<uint64_t>(site + 1));
+}
+
 template <int W>
-__host__ __device__ __forceinline__ uint64_t hash_cfg(const Cfg<W>& c) {
-    uint64_t h = 0x9e3779b97f4a7c15ull;
+__host__ __device__ __forceinline__ uint64_t hash_cfg(const Cfg<W>& c, const uint64_t* __restrict__ cols) {
+    uint64_t h = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-        h = fmix64(h ^ c.w[w]);
+        uint64_t x = c.w[w];
+        while (x) {
+            h ^= cols[64 * w + __builtin_ctzll(x)];
+            x &= x - 1;
+        }
     }
     return h;
+}
+
+// Blocked Bloom filter over the same hash: one 64-bit word per key (chosen by the high half of the
+// hash, like the table slot) with three bits chosen by the low half.
+__host__ __device__ __forceinline__ uint64_t filter_word(uint64_t h, uint64_t fmask) { return (h >> 32) & fmask; }
+
+__host__ __device__ __forceinline__ uint64_t filter_bits(uint64_t h) {
+    return (1ull << (h & 63)) | (1ull << ((h >> 6) & 63)) | (1ull << ((h >> 12) & 63));
 }
 
 template <int W>

@@ -53,8 +53,8 @@ __device__ __forceinline__ void store_row(uint8_t* __restrict__ row, int nq, con
 // so the probe sequence reaches an empty slot; the loop is additionally capped at the capacity.
 template <int W>
 __device__ __forceinline__ int64_t lookup(const uint64_t* __restrict__ table, uint64_t mask, const uint64_t* __restrict__ keys,
-                                          const Cfg<W>& c) {
-    const uint64_t h = hash_cfg<W>(c);
+                                          const uint64_t* __restrict__ cols, const Cfg<W>& c) {
+    const uint64_t h = hash_cfg<W>(c, cols);
     const uint32_t fp = static_cast<uint32_t>(h);
     uint64_t slot = (h >> 32) & mask;
     for (uint64_t probe = 0; probe <= mask; ++probe) {
@@ -76,6 +76,66 @@ __device__ __forceinline__ int64_t lookup(const uint64_t* __restrict__ table, ui
         slot = (slot + 1) & mask;
     }
     return -1;
+}
+
+// A hash index over a config set as the kernels see it.
+struct HashIndex {
+    const uint64_t* table;
+    const uint64_t* keys;
+    uint64_t mask;
+    const uint64_t* filter;
+    uint64_t fmask;
+    const uint64_t* cols;
+};
+
+// Batched lookups: the first probe of every wanted slot is issued before any is resolved, so a
+// thread keeps U independent loads in flight instead of one dependent chain per connection.
+// Action protocol (hs, ht: index hashes of source and target; ht == hs iff the term flips nothing):
+//   bool pre(i, t, target, parity, hs, ht)       -> true if the target must be looked up
+//   void post(i, t, target, parity, j, distinct) -> j = row of the target in the index, or -1;
+//                                                   distinct = (ht != hs)
+template <int W, int U, typename Action>
+__device__ __forceinline__ void lookup_batch(const HashIndex& ix, Action& act, const int64_t (&i)[U], const uint32_t (&t)[U],
+                                             const Cfg<W> (&o)[U], const int (&par)[U], const bool (&want)[U],
+                                             const uint64_t (&h)[U], const uint64_t (&hs)[U]) {
+    uint64_t f[U];
+    // Filter first: most targets are absent and are answered by one L2-resident word.
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        f[u] = want[u] ? ix.filter[filter_word(h[u], ix.fmask)] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!want[u]) {
+            continue;
+        }
+        const uint64_t bits = filter_bits(h[u]);
+        int64_t j = -1;
+        if ((f[u] & bits) == bits) {
+            const uint32_t fp = static_cast<uint32_t>(h[u]);
+            uint64_t s = (h[u] >> 32) & ix.mask;
+            for (uint64_t probe = 0; probe <= ix.mask; ++probe) {
+                const uint64_t ee = ix.table[s];
+                if (ee == kEmptySlot) {
+                    break;
+                }
+                if (static_cast<uint32_t>(ee) == fp) {
+                    const uint64_t jj = ee >> 32;
+                    bool eq = true;
+#pragma unroll
+                    for (int k = 0; k < W; ++k) {
+                        eq = eq && (ix.keys[jj * W + k] == o[u].w[k]);
+                    }
+                    if (eq) {
+                        j = static_cast<int64_t>(jj);
+                        break;
+                    }
+                }
+                s = (s + 1) & ix.mask;
+            }
+        }
+        act.post(i[u], t[u], o[u], par[u], j, h[u] != hs[u]);
+    }
 }
 
 __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__ a, int64_t n, uint64_t v) {
@@ -104,13 +164,18 @@ struct EnumSource {
 };
 
 constexpr int kEnumBlock = 256;
+// Lookups kept in flight per thread.
+constexpr int kBatch = 2;
 
 // LDS carve-up for a tile of C = 1 << tile_log2 configs.
 struct EnumLds {
-    uint64_t* cfg;    // [C][W]
-    uint64_t* bits;   // [n_bits][C / 64]
+    uint64_t* cfg;    // [C][W] tile configs
+    uint64_t* hc;     // [C] their index hashes
+    uint64_t* bits;   // [n_bits][C / 64] per-site occupancy bitsets
     uint64_t* valid;  // [C / 64]
-    uint16_t* mem;    // [C]
+    uint64_t* mcfg;   // [C][W] members of the current bucket (compacted copies)
+    uint64_t* mh;     // [C] their hashes
+    uint16_t* mem;    // [C] their tile rows
 };
 
 template <int W>
@@ -119,58 +184,13 @@ __device__ __forceinline__ EnumLds carve_lds(uint64_t* smem, int tile_log2, int 
     const int CW = C >> 6;
     EnumLds l;
     l.cfg = smem;
-    l.bits = l.cfg + C * W;
+    l.hc = l.cfg + C * W;
+    l.bits = l.hc + C;
     l.valid = l.bits + n_bits * CW;
-    l.mem = reinterpret_cast<uint16_t*>(l.valid + CW);
+    l.mcfg = l.valid + CW;
+    l.mh = l.mcfg + C * W;
+    l.mem = reinterpret_cast<uint16_t*>(l.mh + C);
     return l;
-}
-
-// A compiled record expanded to word masks: the term succeeds on c iff (c & m) == v for every word;
-// the target is c ^ f and the parity popcount(c & p) ^ pconst (p excludes the touched sites).
-template <int W>
-struct Masks {
-    uint64_t m[W], v[W], f[W], p[W];
-    int pconst;
-};
-
-template <int W>
-__device__ __forceinline__ Masks<W> expand_rec(uint64_t rec) {
-    Masks<W> x;
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-        x.m[w] = 0;
-        x.v[w] = 0;
-        x.f[w] = 0;
-        x.p[w] = 0;
-    }
-    const uint32_t valid = static_cast<uint32_t>(rec >> 32) & 0xf;
-    const uint32_t req = static_cast<uint32_t>(rec >> 36) & 0xf;
-    const uint32_t fin = static_cast<uint32_t>(rec >> 40) & 0xf;
-    const uint32_t odd = static_cast<uint32_t>(rec >> 44) & 0xf;
-#pragma unroll
-    for (int k = 0; k < kMaxOps; ++k) {
-        if ((valid >> k) & 1u) {
-            const int s = static_cast<int>((rec >> (8 * k)) & 0xff);
-            const uint64_t bit = 1ull << (s & 63);
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                if (w == (s >> 6)) {
-                    x.m[w] |= bit;
-                    x.v[w] |= ((req >> k) & 1u) ? bit : 0ull;
-                    x.f[w] |= (((req ^ fin) >> k) & 1u) ? bit : 0ull;
-                }
-                if ((odd >> k) & 1u) {
-                    x.p[w] ^= low_mask_word(s, w);
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-        x.p[w] &= ~x.m[w];
-    }
-    x.pconst = static_cast<int>((rec >> 48) & 1);
-    return x;
 }
 
 template <int W>
@@ -189,10 +209,11 @@ __device__ __forceinline__ bool apply_masks(const Masks<W>& x, const Cfg<W>& c, 
 
 // Processes tile `tile` x bucket group `g` (of `groups`): buckets b with b % groups == g, so the
 // heavy buckets (pairs of low sites that nearly every config occupies) spread over all groups.
-// Action provides on_success(i, t, target, parity) for every term position t that succeeds on
-// config row i.  Every thread of the workgroup must call this (it synchronises the workgroup).
+// Every success goes through the Action protocol of lookup_batch against `ix`.  Every thread of the
+// workgroup must call this (it synchronises the workgroup).
 template <int W, typename Action>
-__device__ void enumerate_tile(const EnumSource& p, int64_t tile, int tile_log2, int g, int groups, uint64_t* smem, Action& act) {
+__device__ void enumerate_tile(const EnumSource& p, const HashIndex& ix, int64_t tile, int tile_log2, int g, int groups, uint64_t* smem,
+                               Action& act) {
     __shared__ int s_count;
     const int C = 1 << tile_log2;
     const int CW = C >> 6;
@@ -218,6 +239,7 @@ __device__ void enumerate_tile(const EnumSource& p, int64_t tile, int tile_log2,
         for (int k = 0; k < W; ++k) {
             L.cfg[r * W + k] = c.w[k];
         }
+        L.hc[r] = hash_cfg<W>(c, ix.cols);
     }
     __syncthreads();
     for (int q = wave; q < CW; q += kEnumBlock / 64) {
@@ -279,8 +301,14 @@ __device__ void enumerate_tile(const EnumSource& p, int64_t tile, int tile_log2,
             }
             int pos = incl - cnt;
             while (m) {
-                const int bit = __builtin_ctzll(m);
-                L.mem[pos++] = static_cast<uint16_t>(lane * 64 + bit);
+                const int r = lane * 64 + __builtin_ctzll(m);
+                L.mem[pos] = static_cast<uint16_t>(r);
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    L.mcfg[pos * W + w] = L.cfg[r * W + w];
+                }
+                L.mh[pos] = L.hc[r];
+                ++pos;
                 m &= m - 1;
             }
             if (lane == 63) {
@@ -299,19 +327,33 @@ __device__ void enumerate_tile(const EnumSource& p, int64_t tile, int tile_log2,
         const int tx = tid & (TX - 1);
         const int ty = tid >> lg;
         for (uint32_t t = ts + tx; t < te; t += TX) {
-            const Masks<W> x = expand_rec<W>(p.recs[t]);
-            for (int q = ty; q < mcount; q += TY) {
-                const int r = L.mem[q];
-                Cfg<W> c;
+            const Masks<W> x = expand_rec<W>(p.recs[t], ix.cols);
+            for (int q0 = ty; q0 < mcount; q0 += TY * kBatch) {
+                int64_t ii[kBatch];
+                uint32_t tt[kBatch];
+                Cfg<W> o[kBatch];
+                int par[kBatch];
+                bool want[kBatch];
+                uint64_t h[kBatch];
+                uint64_t hs[kBatch];
 #pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    c.w[w] = L.cfg[r * W + w];
+                for (int u = 0; u < kBatch; ++u) {
+                    const int q = q0 + u * TY;
+                    const bool in = q < mcount;
+                    const int qq = in ? q : 0;
+                    Cfg<W> c;
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        c.w[w] = L.mcfg[qq * W + w];
+                    }
+                    hs[u] = L.mh[qq];
+                    h[u] = hs[u] ^ x.hf;
+                    tt[u] = t;
+                    const bool ok = apply_masks<W>(x, c, o[u], par[u]) && in;
+                    ii[u] = ok ? i0 + L.mem[qq] : i0;
+                    want[u] = ok && act.pre(ii[u], t, o[u], par[u], hs[u], h[u]);
                 }
-                Cfg<W> o;
-                int par;
-                if (apply_masks<W>(x, c, o, par)) {
-                    act.on_success(i0 + r, t, o, par);
-                }
+                lookup_batch<W, kBatch>(ix, act, ii, tt, o, par, want, h, hs);
             }
         }
         __syncthreads();
@@ -328,11 +370,12 @@ struct DirectLds {
     uint64_t v[kDirectChunk * W];
     uint64_t f[kDirectChunk * W];
     uint64_t p[kDirectChunk * W];
+    uint64_t hf[kDirectChunk];
     int32_t pc[kDirectChunk];
 };
 
 template <int W, typename Action>
-__device__ void enumerate_direct(const EnumSource& p, int64_t block, DirectLds<W>& L, Action& act) {
+__device__ void enumerate_direct(const EnumSource& p, const HashIndex& ix, int64_t block, DirectLds<W>& L, Action& act) {
     const int tid = threadIdx.x;
     const int64_t i = block * kEnumBlock + tid;
     const bool valid = i < p.n_i;
@@ -345,12 +388,13 @@ __device__ void enumerate_direct(const EnumSource& p, int64_t block, DirectLds<W
             c.w[k] = 0;
         }
     }
+    const uint64_t hc = hash_cfg<W>(c, ix.cols);
     const int64_t tb = p.counts[0];
     for (int64_t base = 0; base < tb; base += kDirectChunk) {
         const int cnt = static_cast<int>(tb - base < kDirectChunk ? tb - base : kDirectChunk);
         __syncthreads();
         for (int u = tid; u < cnt; u += kEnumBlock) {
-            const Masks<W> x = expand_rec<W>(p.recs[base + u]);
+            const Masks<W> x = expand_rec<W>(p.recs[base + u], ix.cols);
 #pragma unroll
             for (int w = 0; w < W; ++w) {
                 L.m[u * W + w] = x.m[w];
@@ -359,22 +403,37 @@ __device__ void enumerate_direct(const EnumSource& p, int64_t block, DirectLds<W
                 L.p[u * W + w] = x.p[w];
             }
             L.pc[u] = x.pconst;
+            L.hf[u] = x.hf;
         }
         __syncthreads();
         if (valid) {
-            for (int u = 0; u < cnt; ++u) {
-                bool ok = true;
-                int par = L.pc[u];
-                Cfg<W> o;
+            for (int u0 = 0; u0 < cnt; u0 += kBatch) {
+                int64_t ii[kBatch];
+                uint32_t tt[kBatch];
+                Cfg<W> o[kBatch];
+                int par[kBatch];
+                bool want[kBatch];
+                uint64_t h[kBatch];
+                uint64_t hs[kBatch];
 #pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    ok = ok && ((c.w[w] & L.m[u * W + w]) == L.v[u * W + w]);
-                    o.w[w] = c.w[w] ^ L.f[u * W + w];
-                    par ^= __popcll(c.w[w] & L.p[u * W + w]) & 1;
+                for (int b = 0; b < kBatch; ++b) {
+                    const int u = u0 + b < cnt ? u0 + b : cnt - 1;
+                    bool ok = u0 + b < cnt;
+                    int pr = L.pc[u];
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        ok = ok && ((c.w[w] & L.m[u * W + w]) == L.v[u * W + w]);
+                        o[b].w[w] = c.w[w] ^ L.f[u * W + w];
+                        pr ^= __popcll(c.w[w] & L.p[u * W + w]) & 1;
+                    }
+                    par[b] = pr;
+                    ii[b] = i;
+                    tt[b] = static_cast<uint32_t>(base + u);
+                    hs[b] = hc;
+                    h[b] = hc ^ L.hf[u];
+                    want[b] = ok && act.pre(i, tt[b], o[b], pr, hc, h[b]);
                 }
-                if (ok) {
-                    act.on_success(i, static_cast<uint32_t>(base + u), o, par);
-                }
+                lookup_batch<W, kBatch>(ix, act, ii, tt, o, par, want, h, hs);
             }
         }
     }
@@ -383,7 +442,7 @@ __device__ void enumerate_direct(const EnumSource& p, int64_t block, DirectLds<W
 inline size_t enum_lds_bytes_of(int n_words, int n_bits, int tile_log2) {
     const size_t C = static_cast<size_t>(1) << tile_log2;
     const size_t CW = C / 64;
-    return C * n_words * 8 + static_cast<size_t>(n_bits) * CW * 8 + CW * 8 + C * 2;
+    return 2 * (C * n_words * 8 + C * 8) + static_cast<size_t>(n_bits) * CW * 8 + CW * 8 + C * 2;
 }
 
 template <typename F>

@@ -53,6 +53,9 @@ struct FrParams {
     const uint64_t* xkeys;
     const uint64_t* xtable;
     uint64_t xmask;
+    const uint64_t* xfilter;
+    uint64_t xfmask;
+    const uint64_t* xcols;
     FrState* state;
     unsigned long long* hist;
     uint64_t* sel_c;   // [K][W] selected configs ("above" entries, then the kept candidates)
@@ -81,17 +84,19 @@ struct HistAction {
     uint64_t prefix;
     int shift;
     int width;
-    __device__ HistAction(const FrParams& pp, uint32_t* h, uint64_t pre, int sh, int wd)
-        : p(pp), hist(h), prefix(pre), shift(sh), width(wd) {}
-    __device__ __forceinline__ void on_success(int64_t i, uint32_t t, const Cfg<W>& o, int) {
-        const uint64_t wb = weight_bits(p.coefs[t], p.psi[i]);
-        if (wb == 0 || (wb >> (shift + width)) != prefix) {
-            return;
+    __device__ HistAction(const FrParams& pp, uint32_t* h, uint64_t pre_, int sh, int wd)
+        : p(pp), hist(h), prefix(pre_), shift(sh), width(wd) {}
+    // Only entries still tied with the threshold prefix need the (costly) exclusion lookup.
+    __device__ __forceinline__ bool pre(int64_t i, uint32_t t, const Cfg<W>&, int, uint64_t, uint64_t) {
+        const uint64_t w = weight_bits(p.coefs[t], p.psi[i]);
+        return w != 0 && (w >> (shift + width)) == prefix;
+    }
+    __device__ __forceinline__ void post(int64_t i, uint32_t t, const Cfg<W>&, int, int64_t j, bool) {
+        if (j >= 0) {
+            return;  // excluded
         }
-        if (lookup<W>(p.xtable, p.xmask, p.xkeys, o) >= 0) {
-            return;
-        }
-        atomicAdd(&hist[(wb >> shift) & ((1u << width) - 1u)], 1u);
+        const uint64_t w = weight_bits(p.coefs[t], p.psi[i]);
+        atomicAdd(&hist[(w >> shift) & ((1u << width) - 1u)], 1u);
     }
 };
 
@@ -101,35 +106,34 @@ struct CollectAction {
     uint64_t prefix;
     int shift;
     bool all;
-    __device__ CollectAction(const FrParams& pp, uint64_t pre, int sh, bool a) : p(pp), prefix(pre), shift(sh), all(a) {}
-    __device__ __forceinline__ void on_success(int64_t i, uint32_t t, const Cfg<W>& o, int) {
-        const uint64_t wb = weight_bits(p.coefs[t], p.psi[i]);
-        if (wb == 0) {
-            return;
+    __device__ CollectAction(const FrParams& pp, uint64_t pre_, int sh, bool a) : p(pp), prefix(pre_), shift(sh), all(a) {}
+    __device__ __forceinline__ uint64_t top_of(uint64_t w) const { return all ? 1 : (w >> shift); }
+    __device__ __forceinline__ bool pre(int64_t i, uint32_t t, const Cfg<W>&, int, uint64_t, uint64_t) {
+        const uint64_t w = weight_bits(p.coefs[t], p.psi[i]);
+        return w != 0 && top_of(w) >= (all ? 0 : prefix);
+    }
+    __device__ __forceinline__ void post(int64_t i, uint32_t t, const Cfg<W>& o, int, int64_t j, bool) {
+        if (j >= 0) {
+            return;  // excluded
         }
-        const uint64_t top = all ? 1 : (wb >> shift);
-        const uint64_t pre = all ? 0 : prefix;
-        if (top < pre) {
-            return;
-        }
-        if (lookup<W>(p.xtable, p.xmask, p.xkeys, o) >= 0) {
-            return;
-        }
-        if (top > pre) {
-            const unsigned long long slot = atomicAdd(&p.state->n_above, 1ull);
-            if (slot < static_cast<unsigned long long>(p.k)) {
+        const uint64_t w = weight_bits(p.coefs[t], p.psi[i]);
+        const uint64_t top = top_of(w);
+        const uint64_t pre_ = all ? 0 : prefix;
+        if (top > pre_) {
+            const unsigned long long s = atomicAdd(&p.state->n_above, 1ull);
+            if (s < static_cast<unsigned long long>(p.k)) {
 #pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    p.sel_c[slot * W + w] = o.w[w];
+                for (int w2 = 0; w2 < W; ++w2) {
+                    p.sel_c[s * W + w2] = o.w[w2];
                 }
             }
         } else {
-            const unsigned long long slot = atomicAdd(&p.state->n_cand, 1ull);
-            if (slot < static_cast<unsigned long long>(p.cand_cap)) {
-                p.cand_w[slot] = wb;
+            const unsigned long long s = atomicAdd(&p.state->n_cand, 1ull);
+            if (s < static_cast<unsigned long long>(p.cand_cap)) {
+                p.cand_w[s] = w;
 #pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    p.cand_c[slot * W + w] = o.w[w];
+                for (int w2 = 0; w2 < W; ++w2) {
+                    p.cand_c[s * W + w2] = o.w[w2];
                 }
             }
         }
@@ -138,10 +142,11 @@ struct CollectAction {
 
 template <int W, typename Action>
 __device__ __forceinline__ void fr_enumerate(const FrParams& p, uint64_t* smem, Action& act) {
+    const HashIndex ix{p.xtable, p.xkeys, p.xmask, p.xfilter, p.xfmask, p.xcols};
     if (p.direct) {
-        enumerate_direct<W>(p.src, blockIdx.x, *reinterpret_cast<DirectLds<W>*>(smem), act);
+        enumerate_direct<W>(p.src, ix, blockIdx.x, *reinterpret_cast<DirectLds<W>*>(smem), act);
     } else {
-        enumerate_tile<W>(p.src, blockIdx.x / p.groups, p.tile_log2, blockIdx.x % p.groups, p.groups, smem, act);
+        enumerate_tile<W>(p.src, ix, blockIdx.x / p.groups, p.tile_log2, blockIdx.x % p.groups, p.groups, smem, act);
     }
 }
 
@@ -415,6 +420,9 @@ int find_relative_impl(const qmb_terms* terms, const qmb_targets* exclude, const
     p.xkeys = exclude->keys;
     p.xtable = exclude->table;
     p.xmask = exclude->capacity - 1;
+    p.xfilter = exclude->filter;
+    p.xfmask = exclude->filter_words - 1;
+    p.xcols = exclude->cols;
     p.state = s.state;
     p.hist = s.hist;
     p.sel_c = s.sel_c;

@@ -83,6 +83,7 @@ struct LaunchShape {
     int tile_log2;
     int groups;
     bool direct;
+    bool symmetric;
 };
 
 // Term tables up to this size use the direct strategy by default.

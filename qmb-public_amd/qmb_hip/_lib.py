@@ -55,17 +55,21 @@ class QmbTargets(ctypes.Structure):
         ("n_words", _c_i32),
         ("n", _c_i64),
         ("capacity", _c_u64),
+        ("filter_words", _c_u64),
         ("buffer", _vp),
         ("bytes", _c_size),
         ("keys", _vp),
         ("table", _vp),
+        ("filter", _vp),
+        ("cols", _vp),
+        ("duplicates", _vp),
     ]
 
 
 class QmbLaunch(ctypes.Structure):
     """Mirror of ``struct qmb_launch`` (include/qmb_hip.h)."""
 
-    _fields_ = [("tile", _c_i32), ("groups", _c_i32), ("strategy", _c_i32)]
+    _fields_ = [("tile", _c_i32), ("groups", _c_i32), ("strategy", _c_i32), ("symmetric", _c_i32)]
 
 
 # name -> (restype, argtypes); kept in the order of include/qmb_hip.h
@@ -78,6 +82,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
         [_vp, _vp, _c_int, _c_int, ctypes.POINTER(_c_u64), ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i32)],
     ),
     "qmb_apply_record_host": (_c_int, [_c_u64, _c_int, _vp, _vp, ctypes.POINTER(_c_i32)]),
+    "qmb_terms_hermitian_host": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _c_i64, ctypes.c_double, ctypes.POINTER(_c_i32)]),
     "qmb_philox4x32_10_host": (None, [_vp, _vp, _vp]),
     "qmb_philox_uniform_host": (ctypes.c_double, [_c_u64, _c_u64]),
     "qmb_apply_within_workspace_size": (_c_int, [_c_int, _c_i64, _c_i64, _c_i64, ctypes.POINTER(_c_size)]),

@@ -24,16 +24,35 @@ STRATEGY_TILES = 1
 STRATEGY_DIRECT = 2
 
 
-def _launch(tile: int = 0, groups: int = 0, strategy: int = 0) -> _lib.QmbLaunch | None:
-    if tile == 0 and groups == 0 and strategy == 0:
+def _launch(tile: int = 0, groups: int = 0, strategy: int = 0, symmetric: bool = False) -> _lib.QmbLaunch | None:
+    if tile == 0 and groups == 0 and strategy == 0 and not symmetric:
         return None
-    return _lib.QmbLaunch(tile, groups, strategy)
+    return _lib.QmbLaunch(tile, groups, strategy, 1 if symmetric else 0)
+
+
+def can_use_symmetric(terms: "PreparedTerms", targets: "PreparedTargets", configs_i: torch.Tensor) -> bool:
+    """Hermitian operator applied within the indexed set itself, rows distinct."""
+    return terms.hermitian and targets.indexes(configs_i) and targets.duplicates() == 0
+
+
+def is_hermitian(site: torch.Tensor, kind: torch.Tensor, coef: torch.Tensor, n_qubytes: int, particle_cut: int,
+                 tol: float = 1e-12) -> bool:
+    """Whether sum_t coef_t O_t equals its adjoint (exact test on the compiled terms, host side)."""
+    s = site.detach().cpu().contiguous()
+    k = kind.detach().cpu().contiguous()
+    c = coef.detach().cpu().contiguous()
+    out = ctypes.c_int32(0)
+    lib = _lib.library()
+    _lib.check(lib.qmb_terms_hermitian_host(n_qubytes, particle_cut, _ptr(s), _ptr(k), _ptr(c), s.shape[0], tol, ctypes.byref(out)),
+               "qmb_terms_hermitian_host")
+    return bool(out.value)
 
 
 class PreparedTerms:
     """Compiled, bucketed term table on one device (qmb_terms_build)."""
 
-    def __init__(self, site: torch.Tensor, kind: torch.Tensor, coef: torch.Tensor, n_qubytes: int, particle_cut: int) -> None:
+    def __init__(self, site: torch.Tensor, kind: torch.Tensor, coef: torch.Tensor, n_qubytes: int, particle_cut: int,
+                 check_hermitian: bool = True) -> None:
         device = site.device
         site, kind, coef = site.contiguous(), kind.contiguous(), coef.contiguous()
         self.n_terms = _check_terms(site, kind, coef, device, n_qubytes)
@@ -50,6 +69,7 @@ class PreparedTerms:
         with torch.cuda.device(device):
             _lib.check(lib.qmb_terms_build(n_qubytes, particle_cut, _ptr(site), _ptr(kind), _ptr(coef), self.n_terms, _ptr(self.buffer),
                                            self.buffer.numel(), ctypes.byref(self.desc), _stream(device)), "qmb_terms_build")
+        self.hermitian = is_hermitian(site, kind, coef, n_qubytes, particle_cut) if check_hermitian else False
 
     def counts(self) -> tuple[int, int, int]:
         """(valid terms, diagonal terms, invalid terms) -- synchronises."""
@@ -78,13 +98,27 @@ class PreparedTargets:
         with torch.cuda.device(device):
             _lib.check(lib.qmb_targets_build(n_qubytes, _ptr(configs), self.n, _ptr(self.buffer), self.buffer.numel(),
                                              ctypes.byref(self.desc), _stream(device)), "qmb_targets_build")
+        self._duplicates: int | None = None
+
+    def duplicates(self) -> int:
+        """Rows equal to an earlier row (synchronises once)."""
+        if self._duplicates is None:
+            offset = int(self.desc.duplicates) - self.buffer.data_ptr()
+            self._duplicates = int(self.buffer[offset:offset + 8].view(torch.int64).item())
+        return self._duplicates
+
+    def indexes(self, configs: torch.Tensor) -> bool:
+        """True when `configs` is the very tensor this index was built over."""
+        return configs.data_ptr() == self.configs.data_ptr() and configs.shape == self.configs.shape
 
 
 def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tensor, vec: torch.Tensor, mode: int,
-          out: torch.Tensor | None = None, tile: int = 0, groups: int = 0, zero: bool = True, strategy: int = 0) -> torch.Tensor:
+          out: torch.Tensor | None = None, tile: int = 0, groups: int = 0, zero: bool = True, strategy: int = 0,
+          symmetric: bool | None = None) -> torch.Tensor:
     """mode 0: psi_j = A psi_i (vec = psi_i as float64 [n_i, 2]); mode 1: grad_i = A^H grad_j.
 
     `out` (float64 [n, 2]) is accumulated into when given, after zeroing unless zero=False.
+    symmetric=None uses the symmetric kernel whenever can_use_symmetric() holds.
     """
     configs_i = configs_i.contiguous()
     vec = vec.contiguous()
@@ -95,7 +129,9 @@ def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tenso
         out = torch.zeros((n_out, 2), dtype=torch.float64, device=device)
     elif zero:
         out.zero_()
-    launch = _launch(tile, groups, strategy)
+    if symmetric is None:
+        symmetric = mode != MODE_COUNT and can_use_symmetric(terms, targets, configs_i)
+    launch = _launch(tile, groups, strategy, symmetric)
     lib = _lib.library()
     with torch.cuda.device(device):
         _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.desc), ctypes.byref(targets.desc), _ptr(configs_i), n_i, _ptr(vec), _ptr(out),

@@ -142,3 +142,33 @@ def test_validate_terms_host() -> None:
     bad = ctypes.c_int64(0)
     _lib.check(lib.qmb_validate_terms_host(1, site.ctypes.data, kind.ctypes.data, 3, ctypes.byref(bad)), "validate")
     assert bad.value == 2
+
+
+def test_hermiticity_check() -> None:
+    """The exact adjoint test accepts Hermitian model tables and rejects perturbed ones."""
+    import torch  # pylint: disable=import-outside-toplevel
+
+    from qmb_hip import prepared as pp  # pylint: disable=import-outside-toplevel
+    from qmb_hip import synthetic as sy  # pylint: disable=import-outside-toplevel
+
+    s, k, c = sy.molecular_terms(4, 3)
+    assert pp.is_hermitian(s, k, c, 1, 1)
+    c2 = c.clone()
+    c2[5, 1] = 0.3  # a complex coefficient without its conjugate partner
+    assert not pp.is_hermitian(s, k, c2, 1, 1)
+    # a+_3 a_1 alone is not Hermitian; adding its adjoint a+_1 a_3 (written in any order) makes it so
+    hop = {((3, 1), (1, 0)): 0.5 + 0.25j}
+    assert not pp.is_hermitian(*sy.terms_from_dict(hop), 1, 1)
+    hop[((1, 1), (3, 0))] = 0.5 - 0.25j
+    assert pp.is_hermitian(*sy.terms_from_dict(hop), 1, 1)
+    # (a+_0 a+_2 a_3 a_1)^dagger = a+_1 a+_3 a_2 a_0 = a+_3 a+_1 a_0 a_2 (two swaps): same coefficient
+    pair = {((0, 1), (2, 1), (3, 0), (1, 0)): 0.7, ((3, 1), (1, 1), (0, 0), (2, 0)): 0.7}
+    assert pp.is_hermitian(*sy.terms_from_dict(pair), 1, 1)
+    pair_wrong = {((0, 1), (2, 1), (3, 0), (1, 0)): 0.7, ((3, 1), (1, 1), (0, 0), (2, 0)): -0.7}
+    assert not pp.is_hermitian(*sy.terms_from_dict(pair_wrong), 1, 1)
+    # one swap: (a+_0 a_1)^dagger = a+_1 a_0 = -a_0 a+_1
+    hop2 = {((0, 1), (1, 0)): 0.3, ((0, 0), (1, 1)): -0.3}
+    assert pp.is_hermitian(*sy.terms_from_dict(hop2), 1, 1)
+    assert pp.is_hermitian(*sy.terms_from_dict(sy.ising_dict(3, 2, x=0.5, y=0.2, zh=1.0, yv=0.4)), 1, 2)
+    assert not pp.is_hermitian(*sy.random_terms(10, 50, 1), 2, 1)
+    assert isinstance(torch.zeros(1), torch.Tensor)
