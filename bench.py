@@ -77,6 +77,17 @@ def build_workload(name: str) -> dict[str, object]:
             "sample": min(sample, len(configs)), "desc": desc}
 
 
+def read_traffic(workload: str) -> float | None:
+    """HBM bytes per forward launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/."""
+    path = ROOT / "profiles" / f"apply_kernel_{workload}.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("hbm_bytes_per_forward_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(w: dict[str, object], m_hit_sample: int) -> dict[str, object]:
     """The reference CPU op (oracle/_ref) on the first `sample` configs_i against all of configs_j."""
     import numpy as np  # pylint: disable=import-outside-toplevel
@@ -121,6 +132,8 @@ def main() -> int:
     parser.add_argument("--groups", type=int, default=0)
     parser.add_argument("--no-plain", action="store_true", help="skip the reference-scheme comparator")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    parser.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                        help="also run a plain device-to-device copy of BYTES bytes three times (HBM counter calibration)")
     args = parser.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is described in DESIGN.md.",
@@ -176,6 +189,8 @@ def main() -> int:
 
     # Dominant kernel: the forward enumeration, timed alone with events on the stream it runs on.
     targets = pp.PreparedTargets(configs, nq)
+    symmetric = pp.can_use_symmetric(terms, targets, configs)
+    direct = n_terms <= pp.DIRECT_MAX_TERMS
     m_gen, m_hit = pp.count(terms, targets, configs, args.tile, args.groups)
     out = torch.zeros((n, 2), dtype=torch.float64, device=dev)
     vec = torch.view_as_real(psi).contiguous()
@@ -206,10 +221,15 @@ def main() -> int:
     achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
     log(f"M_gen={m_gen} M_hit={m_hit}; forward kernel {kernel_ms:.3f} ms, adjoint kernel {adjoint_ms:.3f} ms")
 
-    traffic = None
-    tfile = ROOT / "profiles" / f"traffic_{args.workload}.json"
-    if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_forward_launch")
+    traffic = read_traffic(args.workload)
+    if args.copy_calibration > 0:
+        src = torch.empty(args.copy_calibration // 8, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 x {src.numel() * 8} bytes")
+        del src, dst
 
     plain = None
     if not args.no_plain:
@@ -255,9 +275,12 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic (seeded)",
         "config": dict(w["desc"], n_terms=n_terms, n_qubytes=nq, M_gen=m_gen, M_hit=m_hit, valid_terms=valid_terms,
-                       diagonal_terms=diag_terms, tile=args.tile or "auto", groups=args.groups or 8),
+                       diagonal_terms=diag_terms, tile=args.tile or "auto", groups=args.groups or 8,
+                       kernel_path=("symmetric " if symmetric else "scatter/gather ") + ("direct" if direct else "bucketed tiles")),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": traffic, "kernel": f"apply_kernel<W={(nq + 7) // 8},forward>", "kernel_ms": kernel_ms,
+                     "traffic": traffic,
+                     "kernel": f"apply_kernel<W={(nq + 7) // 8},{'symmetric' if symmetric else 'forward'},{'direct' if direct else 'tiles'}>",
+                     "kernel_ms": kernel_ms,
                      "adjoint_kernel_ms": adjoint_ms, "algorithmic_bytes": algo_bytes},
         "cpu_baseline": cpu,
         "plain_kernel": plain,

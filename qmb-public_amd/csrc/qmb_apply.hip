@@ -344,7 +344,10 @@ __global__ __launch_bounds__(kBlock) void single_rel_kernel(const uint8_t* __res
 
 LaunchShape resolve_launch(const qmb_launch* launch, int n_words, int64_t n_i, int64_t n_terms) {
     LaunchShape s{};
-    int tile = 2048 / n_words;
+    // 256-config tiles: the member lists of one bucket stay short enough that the compaction and the
+    // barriers around it are amortised over ~10^2 terms per thread, and a 10^6-config set still gives
+    // ~3900 tiles x 8 groups of work-groups.
+    int tile = 256;
     int groups = 8;
     int strategy = launch ? launch->strategy : 0;
     if (strategy == 0) {
@@ -356,8 +359,8 @@ LaunchShape resolve_launch(const qmb_launch* launch, int n_words, int64_t n_i, i
     if (launch && launch->tile > 0) {
         tile = launch->tile;
     } else {
-        // keep at least ~128 tiles in flight for small inputs (tile >= 256)
-        while (tile > 256 && (n_i + tile - 1) / tile < 128) {
+        // small inputs: keep at least ~128 tiles in flight (tile >= 64)
+        while (tile > 64 && (n_i + tile - 1) / tile < 128) {
             tile >>= 1;
         }
     }

@@ -22,6 +22,8 @@ MODE_COUNT = 2
 STRATEGY_AUTO = 0
 STRATEGY_TILES = 1
 STRATEGY_DIRECT = 2
+# auto picks the direct kernel up to this many terms (kDirectMaxTerms in the library)
+DIRECT_MAX_TERMS = 4096
 
 
 def _launch(tile: int = 0, groups: int = 0, strategy: int = 0, symmetric: bool = False) -> _lib.QmbLaunch | None:
