@@ -178,10 +178,12 @@ typedef struct qmb_launch {
     int32_t groups;   /* bucketed strategy: bucket groups per tile (bucket b goes to group b % groups) */
     int32_t strategy; /* 0 auto (direct for <= 4096 terms), 1 bucketed tiles, 2 direct (every config
                          walks every term from LDS) */
-    int32_t symmetric; /* 1: the caller asserts the operator is Hermitian (see qmb_terms_hermitian_host)
-                          and configs_i is the indexed set itself with no repeated rows; each connected
-                          pair is then looked up once and both A_ji and conj(A_ji) are applied.  Mode 1
-                          then computes the same map as mode 0 (A^H = A). */
+    int32_t symmetric; /* nonzero: the caller asserts the operator is Hermitian (see
+                          qmb_terms_hermitian_host) and configs_i is the indexed set itself with no
+                          repeated rows, so modes 0 and 1 compute the same map (A^H = A).
+                          1: both run as the gather of mode 1 (per-config sums, no atomics per
+                          contribution); 2: each connected pair is looked up once and both A_ji and
+                          conj(A_ji) are scattered with atomics. */
 } qmb_launch;
 
 /* mode 0: forward scatter  out[k] += sum sign*coef*in[i]      (in = psi_i [n_i], out = psi_j [n_j])
@@ -192,6 +194,65 @@ typedef struct qmb_launch {
  * `out` is accumulated into (zero it first).  configs_i are n_qubytes-byte rows. */
 int qmb_apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const uint8_t* configs_i, int64_t n_i,
                        const double* in, double* out, int mode, const qmb_launch* launch, void* stream);
+
+/* ---------------- pair-excitation path (no reference counterpart) ------------------------------
+ * A pair excitation is a term on four distinct sites that annihilates two and creates two, with
+ * no condition on other sites (the two-body integrals of a molecular Hamiltonian).  Instead of
+ * generating every (config, term) target and looking it up, the pair path groups the configs of
+ * one set by "core" (the config with two occupied sites removed): two configs of the set are
+ * connected by a pair excitation exactly when they share a core with disjoint removed pairs, so
+ * only connections whose target is in the set are ever visited.  The remaining terms stay on the
+ * enumeration path (qmb_terms_build_residual).  Applies within one config set (configs_i indexed
+ * by the qmb_targets the cores were built from, rows distinct). */
+
+/* Merged coefficients of the pair-excitation terms, dense over (annihilated pair, created pair):
+ * coef[pair_index(r0,r1) * n_pairs + pair_index(c0,c1)] = sum over terms of (-1)^delta_t coef_t,
+ * pair_index(a<b) = b(b-1)/2 + a over the 8*n_qubytes sites.  n_qubytes <= 16. */
+typedef struct qmb_pairs {
+    int32_t n_qubytes, particle_cut, n_bits, reserved;
+    int64_t n_pairs;
+    void* buffer;
+    size_t bytes;
+    double* coef;      /* [n_pairs * n_pairs][2] */
+    int64_t* counts;   /* device: {pair-excitation terms, 0, 0, 0} */
+} qmb_pairs;
+
+int qmb_pairs_workspace_size(int n_qubytes, size_t* bytes);
+int qmb_pairs_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
+                    int64_t n_terms, void* buffer, size_t bytes, qmb_pairs* out, void* stream);
+
+/* qmb_terms_build without the pair-excitation terms (same layout and buffer size). */
+int qmb_terms_build_residual(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
+                             int64_t n_terms, void* buffer, size_t bytes, qmb_terms* out, void* stream);
+
+/* Core index of the set indexed by a qmb_targets: one entry per (config, occupied site pair),
+ * radix-sorted by the 48-bit hash of the core, and the start of every group of equal hashes.
+ * max_occupied must bound the number of occupied sites of every config (a row with more sets
+ * counts[2], and qmb_apply_pairs then writes NaN into out[0] instead of a wrong result). */
+typedef struct qmb_cores {
+    int32_t n_qubytes, n_words, max_occupied, per_config; /* per_config = C(max_occupied, 2) */
+    int64_t n;
+    int64_t capacity;  /* n * per_config entries */
+    void* buffer;
+    size_t bytes;
+    uint64_t* keys;    /* [capacity] sorted core hashes (unused entries last) */
+    uint64_t* vals;    /* [capacity] row << 16 | site1 << 8 | site0 of the removed pair */
+    uint32_t* starts;  /* [capacity + 1] first entry of each group, then the end */
+    int64_t* counts;   /* device: {entries, groups, overflow, 0} */
+} qmb_cores;
+
+int qmb_cores_workspace_size(int n_qubytes, int64_t n, int32_t max_occupied, size_t* bytes);
+int qmb_cores_build(const qmb_targets* targets, int32_t max_occupied, void* buffer, size_t bytes, qmb_cores* out, void* stream);
+
+/* Adds the pair-excitation part of mode 0 (forward, out = A in) or mode 1 (adjoint,
+ * out = A^H in) on the set of `cores`/`targets` into out (float64 [n][2]).  A gather over the
+ * core groups: one FP64 atomic per (config, group) sum, none per contribution. */
+int qmb_apply_pairs(const qmb_pairs* pairs, const qmb_cores* cores, const qmb_targets* targets, const double* in, double* out,
+                    int mode, void* stream);
+
+/* Host-only pair-excitation test of a compiled record (for unit tests): returns 1 and writes
+ * {r0, r1, c0, c1, delta} when the record is a pair excitation, else 0. */
+int qmb_pair_excitation_host(uint64_t rec, int particle_cut, int32_t out[5]);
 
 #ifdef __cplusplus
 }

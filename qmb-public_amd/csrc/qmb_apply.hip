@@ -39,7 +39,7 @@ struct ApplyParams {
 // (i, j) is handled once, from the end with the smaller index hash, applying both A_ji and
 // A_ij = conj(A_ji); connections seen from the other end are skipped before their lookup.
 template <int W, bool SYM>
-struct ForwardAction {
+struct ForwardAction : ActionHooks {
     const ApplyParams& p;
     __device__ explicit ForwardAction(const ApplyParams& pp) : p(pp) {}
     __device__ __forceinline__ bool pre(int64_t, uint32_t, const Cfg<W>&, int, uint64_t hs, uint64_t ht) {
@@ -63,33 +63,84 @@ struct ForwardAction {
     }
 };
 
-// out[i] += conj(sign * coef) * in[j]: gather from the target's gradient.
+// conj(sign * coef) * in[j]: the adjoint's contribution to source config i from target row j.
+__device__ __forceinline__ double2 adjoint_term(const double2& cf, const double2& g, int par) {
+    double re = cf.x * g.x + cf.y * g.y;
+    double im = cf.x * g.y - cf.y * g.x;
+    return par ? make_double2(-re, -im) : make_double2(re, im);
+}
+
+// out[i] += conj(sign * coef) * in[j], gathered per source config.  Tiled enumeration: the tile's
+// sums live in LDS (ds_add_f64, contended only by the lanes that hit the same config in one step)
+// and reach HBM once per (tile, bucket group) with one FP64 atomic per component, instead of one
+// global atomic per contribution.
 template <int W>
-struct AdjointAction {
+struct AdjointTileAction : ActionHooks {
     const ApplyParams& p;
-    __device__ explicit AdjointAction(const ApplyParams& pp) : p(pp) {}
+    double2* acc;
+    int64_t base;
+    __device__ explicit AdjointTileAction(const ApplyParams& pp) : p(pp), acc(nullptr), base(0) {}
+    __device__ __forceinline__ void tile_begin(int64_t i0, int, double2* a) {
+        acc = a;
+        base = i0;
+        for (int r = threadIdx.x; r < (1 << p.tile_log2); r += kEnumBlock) {
+            a[r] = make_double2(0.0, 0.0);
+        }
+    }
+    __device__ __forceinline__ void tile_end(int64_t i0, int nc, double2* a) {
+        for (int r = threadIdx.x; r < nc; r += kEnumBlock) {
+            const double2 v = a[r];
+            if (v.x != 0.0 || v.y != 0.0) {
+                if (p.groups == 1) {
+                    p.out[i0 + r].x += v.x;
+                    p.out[i0 + r].y += v.y;
+                } else {
+                    unsafeAtomicAdd(&p.out[i0 + r].x, v.x);
+                    unsafeAtomicAdd(&p.out[i0 + r].y, v.y);
+                }
+            }
+        }
+    }
     __device__ __forceinline__ bool pre(int64_t, uint32_t, const Cfg<W>&, int, uint64_t, uint64_t) { return true; }
     __device__ __forceinline__ void post(int64_t i, uint32_t t, const Cfg<W>&, int par, int64_t j, bool) {
         if (j < 0) {
             return;
         }
-        const double2 cf = p.coefs[t];
-        const double2 g = p.in[j];
-        double re = cf.x * g.x + cf.y * g.y;
-        double im = cf.x * g.y - cf.y * g.x;
-        if (par) {
-            re = -re;
-            im = -im;
+        const double2 v = adjoint_term(p.coefs[t], p.in[j], par);
+        double2* a = acc + (i - base);
+        atomicAdd(&a->x, v.x);
+        atomicAdd(&a->y, v.y);
+    }
+};
+
+// The same gather for the direct enumeration, where one thread owns each source config: the sum
+// stays in registers and is added to out[i] once, without atomics.
+template <int W>
+struct AdjointDirectAction : ActionHooks {
+    const ApplyParams& p;
+    double re, im;
+    __device__ explicit AdjointDirectAction(const ApplyParams& pp) : p(pp), re(0.0), im(0.0) {}
+    __device__ __forceinline__ bool pre(int64_t, uint32_t, const Cfg<W>&, int, uint64_t, uint64_t) { return true; }
+    __device__ __forceinline__ void post(int64_t, uint32_t t, const Cfg<W>&, int par, int64_t j, bool) {
+        if (j < 0) {
+            return;
         }
-        unsafeAtomicAdd(&p.out[i].x, re);
-        unsafeAtomicAdd(&p.out[i].y, im);
+        const double2 v = adjoint_term(p.coefs[t], p.in[j], par);
+        re += v.x;
+        im += v.y;
+    }
+    __device__ __forceinline__ void direct_end(int64_t i, bool valid) {
+        if (valid && (re != 0.0 || im != 0.0)) {
+            p.out[i].x += re;
+            p.out[i].y += im;
+        }
     }
 };
 
 // Counts connections (successful applications) and contributions (connections whose target is in
 // the index) for the traffic model; `out` is reinterpreted as two unsigned 64-bit counters.
 template <int W>
-struct CountAction {
+struct CountAction : ActionHooks {
     const ApplyParams& p;
     unsigned long long gen, hit;
     __device__ explicit CountAction(const ApplyParams& pp) : p(pp), gen(0), hit(0) {}
@@ -127,8 +178,13 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(ApplyParams p) {
         ForwardAction<W, false> act(p);
         run(act);
     } else if (MODE == 1) {
-        AdjointAction<W> act(p);
-        run(act);
+        if (DIRECT) {
+            AdjointDirectAction<W> act(p);
+            run(act);
+        } else {
+            AdjointTileAction<W> act(p);
+            run(act);
+        }
     } else if (MODE == 3) {
         ForwardAction<W, true> act(p);
         run(act);
@@ -355,7 +411,7 @@ LaunchShape resolve_launch(const qmb_launch* launch, int n_words, int64_t n_i, i
         strategy = n_terms <= kDirectMaxTerms ? 2 : 1;
     }
     s.direct = strategy == 2;
-    s.symmetric = launch != nullptr && launch->symmetric != 0;
+    s.symmetric = launch != nullptr ? launch->symmetric : 0;
     if (launch && launch->tile > 0) {
         tile = launch->tile;
     } else {
@@ -448,8 +504,9 @@ int apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const uin
             QMB_CHECK_LAUNCH();
             return QMB_OK;
         };
-        // A Hermitian operator on its own index: forward and adjoint are the same map.
-        const int m = (ls.symmetric && mode != 2) ? 3 : mode;
+        // A Hermitian operator on its own index: forward and adjoint are the same map, computed as
+        // the gather (symmetric 1) or as the half-lookup scatter (symmetric 2).
+        const int m = (ls.symmetric != 0 && mode != 2) ? (ls.symmetric == 2 ? 3 : 1) : mode;
         if (ls.direct) {
             if (m == 0) return launch_one(&apply_kernel<W, 0, 1>);
             if (m == 1) return launch_one(&apply_kernel<W, 1, 1>);

@@ -37,16 +37,22 @@ __global__ void zero_f64_kernel(double* __restrict__ p, int64_t n) {
 
 // -------------------------------------------------------------------------------- terms ------
 __global__ void compile_terms_kernel(const int16_t* __restrict__ site, const uint8_t* __restrict__ kind, int64_t n_terms,
-                                     int particle_cut, int n_bits, int64_t n_buckets, uint64_t* __restrict__ rec_tmp,
-                                     uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ dflag,
-                                     int64_t* __restrict__ counts) {
+                                     int particle_cut, int n_bits, int64_t n_buckets, int exclude_pairs,
+                                     uint64_t* __restrict__ rec_tmp, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                     uint32_t* __restrict__ dflag, int64_t* __restrict__ counts) {
     int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (t >= n_terms) {
         return;
     }
     CompiledTerm c = compile_term(site + 4 * t, kind + 4 * t, particle_cut, n_bits);
     vals[t] = static_cast<uint32_t>(t);
-    if (c.status == kCompileOk) {
+    PairExcitation pe;
+    if (c.status == kCompileOk && exclude_pairs && pair_excitation(c.rec, particle_cut, &pe)) {
+        // carried by the pair-excitation table instead
+        keys[t] = static_cast<uint32_t>(n_buckets);
+        rec_tmp[t] = kNeverSucceeds;
+        dflag[t] = 0u;
+    } else if (c.status == kCompileOk) {
         keys[t] = static_cast<uint32_t>(bucket_of(c, n_bits));
         rec_tmp[t] = c.rec;
         dflag[t] = c.diagonal ? 1u : 0u;
@@ -359,7 +365,7 @@ int terms_layout(Carver& cv, int n_qubytes, int64_t n_terms, qmb_terms* out, voi
 }
 
 int terms_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
-                void* buffer, size_t bytes, qmb_terms* out, hipStream_t stream) {
+                void* buffer, size_t bytes, qmb_terms* out, hipStream_t stream, int exclude_pairs) {
     if (particle_cut != 1 && particle_cut != 2) {
         return set_error(QMB_ERR_UNSUPPORTED, "particle_cut must be 1 or 2, got %d", particle_cut);
     }
@@ -389,7 +395,8 @@ int terms_build(int n_qubytes, int particle_cut, const int16_t* site, const uint
     QMB_CHECK_LAUNCH();
     if (n_terms > 0) {
         compile_terms_kernel<<<blocks_for(n_terms), kBlock, 0, stream>>>(site, kind, n_terms, particle_cut, t.n_bits,
-                                                                        t.n_buckets, rec_tmp, keys, vals, dflag, t.counts);
+                                                                        t.n_buckets, exclude_pairs, rec_tmp, keys, vals, dflag,
+                                                                        t.counts);
         QMB_CHECK_LAUNCH();
         size_t need = tmp_bytes;
         QMB_HIP_TRY(rocprim::radix_sort_pairs(tmp, need, keys, skeys, vals, order, static_cast<unsigned>(n_terms), 0,

@@ -303,6 +303,87 @@ int qmb_apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const
     return apply_prepared(terms, targets, configs_i, n_i, in, out, mode, launch, as_stream(stream));
 }
 
+// ---------------------------------------------------------------------------- pair path --
+int qmb_pairs_workspace_size(int n_qubytes, size_t* bytes) {
+    clear_error();
+    if (!bytes) {
+        return set_error(QMB_ERR_ARGUMENT, "bytes must be non-null");
+    }
+    QMB_TRY(check_qubytes(n_qubytes));
+    *bytes = pairs_bytes(n_qubytes);
+    return *bytes ? QMB_OK : QMB_ERR_UNSUPPORTED;
+}
+
+int qmb_pairs_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
+                    void* buffer, size_t bytes, qmb_pairs* out, void* stream) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    QMB_TRY(check_particle_cut(particle_cut));
+    QMB_TRY(check_ptr(site, n_terms, "site"));
+    QMB_TRY(check_ptr(kind, n_terms, "kind"));
+    QMB_TRY(check_ptr(coef, n_terms, "coef"));
+    if (!out || !buffer) {
+        return set_error(QMB_ERR_ARGUMENT, "buffer and out must be non-null");
+    }
+    return pairs_build(n_qubytes, particle_cut, site, kind, coef, n_terms, buffer, bytes, out, as_stream(stream));
+}
+
+int qmb_terms_build_residual(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
+                             int64_t n_terms, void* buffer, size_t bytes, qmb_terms* out, void* stream) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    QMB_TRY(check_particle_cut(particle_cut));
+    QMB_TRY(check_ptr(site, n_terms, "site"));
+    QMB_TRY(check_ptr(kind, n_terms, "kind"));
+    QMB_TRY(check_ptr(coef, n_terms, "coef"));
+    if (!out || !buffer) {
+        return set_error(QMB_ERR_ARGUMENT, "buffer and out must be non-null");
+    }
+    return terms_build(n_qubytes, particle_cut, site, kind, coef, n_terms, buffer, bytes, out, as_stream(stream), 1);
+}
+
+int qmb_cores_workspace_size(int n_qubytes, int64_t n, int32_t max_occupied, size_t* bytes) {
+    clear_error();
+    if (!bytes) {
+        return set_error(QMB_ERR_ARGUMENT, "bytes must be non-null");
+    }
+    *bytes = cores_bytes(n_qubytes, n, max_occupied);
+    return *bytes ? QMB_OK : QMB_ERR_ARGUMENT;
+}
+
+int qmb_cores_build(const qmb_targets* targets, int32_t max_occupied, void* buffer, size_t bytes, qmb_cores* out, void* stream) {
+    clear_error();
+    if (!out || !buffer) {
+        return set_error(QMB_ERR_ARGUMENT, "buffer and out must be non-null");
+    }
+    return cores_build(targets, max_occupied, buffer, bytes, out, as_stream(stream));
+}
+
+int qmb_apply_pairs(const qmb_pairs* pairs, const qmb_cores* cores, const qmb_targets* targets, const double* in, double* out,
+                    int mode, void* stream) {
+    clear_error();
+    if (cores && cores->n > 0 && (!in || !out)) {
+        return set_error(QMB_ERR_ARGUMENT, "in and out must be non-null");
+    }
+    return apply_pairs(pairs, cores, targets, in, out, mode, as_stream(stream));
+}
+
+int qmb_pair_excitation_host(uint64_t rec, int particle_cut, int32_t out[5]) {
+    clear_error();
+    PairExcitation e;
+    if (!pair_excitation(rec, particle_cut, &e)) {
+        return 0;
+    }
+    if (out) {
+        out[0] = e.r0;
+        out[1] = e.r1;
+        out[2] = e.c0;
+        out[3] = e.c1;
+        out[4] = e.delta;
+    }
+    return 1;
+}
+
 // ------------------------------------------------------------------------------ apply_within --
 int qmb_apply_within_workspace_size(int n_qubytes, int64_t n_i, int64_t n_j, int64_t n_terms, size_t* bytes) {
     clear_error();

@@ -193,6 +193,70 @@ __host__ __device__ inline void decode_pair(int64_t k, int* r0, int* r1) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// Pair excitations: terms on four distinct sites that annihilate two (r0 < r1) and create two
+// (c0 < c1), with no condition on any other site.  Every operator order of such a term succeeds
+// exactly on the configs holding r0, r1 and not c0, c1, maps them to the same target, and has the
+// Jordan-Wigner parity  pair_parity(c) ^ delta  with
+//     pair_parity(c) = popcount(c & (L(r0) ^ L(r1) ^ L(c0) ^ L(c1))) mod 2,  L(s) = sites below s,
+// where delta depends on the term only (its parity mask is the same XOR of low masks minus the
+// four sites).  So all such terms with the same (r0, r1, c0, c1) merge into one coefficient
+// sum_t (-1)^delta_t coef_t, applied with sign (-1)^pair_parity(source config).
+// ---------------------------------------------------------------------------------------------
+struct PairExcitation {
+    int r0, r1;  // annihilated sites, r0 < r1
+    int c0, c1;  // created sites, c0 < c1
+    int delta;   // parity offset of this operator order
+};
+
+// Index of the unordered site pair (a < b) among all pairs of n sites: b (b - 1) / 2 + a.
+__host__ __device__ __forceinline__ int64_t pair_index(int a, int b) { return static_cast<int64_t>(b) * (b - 1) / 2 + a; }
+
+__host__ __device__ inline bool pair_excitation(uint64_t rec, int particle_cut, PairExcitation* out) {
+    if ((rec >> 63) & 1ull) {
+        return false;
+    }
+    const uint32_t valid = static_cast<uint32_t>(rec >> 32) & 0xf;
+    if (valid != 0xf) {
+        return false;
+    }
+    const uint32_t req = static_cast<uint32_t>(rec >> 36) & 0xf;
+    const uint32_t fin = static_cast<uint32_t>(rec >> 40) & 0xf;
+    int ann[2], cre[2], na = 0, nc = 0;
+    int sites[4];
+    for (int k = 0; k < 4; ++k) {
+        sites[k] = static_cast<int>((rec >> (8 * k)) & 0xff);
+        for (int q = 0; q < k; ++q) {
+            if (sites[q] == sites[k]) {
+                return false;
+            }
+        }
+        const uint32_t r = (req >> k) & 1u, f = (fin >> k) & 1u;
+        if (r == 1u && f == 0u) {
+            ann[na++ & 1] = sites[k];
+        } else if (r == 0u && f == 1u) {
+            cre[nc++ & 1] = sites[k];
+        } else {
+            return false;
+        }
+    }
+    if (na != 2 || nc != 2) {
+        return false;
+    }
+    PairExcitation e;
+    e.r0 = ann[0] < ann[1] ? ann[0] : ann[1];
+    e.r1 = ann[0] < ann[1] ? ann[1] : ann[0];
+    e.c0 = cre[0] < cre[1] ? cre[0] : cre[1];
+    e.c1 = cre[0] < cre[1] ? cre[1] : cre[0];
+    // pair_parity of the smallest source config {r0, r1}: r0 is below r1, plus the four
+    // comparisons with the created sites; the term's own parity there is its constant bit.
+    const int canon = (1 + (e.r0 < e.c0) + (e.r1 < e.c0) + (e.r0 < e.c1) + (e.r1 < e.c1)) & 1;
+    const int pconst = static_cast<int>((rec >> 48) & 1);
+    e.delta = particle_cut == 1 ? (pconst ^ canon) : 0;
+    *out = e;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------------
 // Config words.
 // ---------------------------------------------------------------------------------------------
 template <int W>

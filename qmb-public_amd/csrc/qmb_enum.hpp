@@ -78,6 +78,18 @@ __device__ __forceinline__ int64_t lookup(const uint64_t* __restrict__ table, ui
     return -1;
 }
 
+// Optional hooks of the Action protocol (no-ops here; an Action overrides what it needs):
+//   tile_begin(i0, nc, acc)  every thread, after the tile of nc configs from row i0 is loaded and
+//                            before the barrier that precedes the first post(); acc is the tile's
+//                            LDS accumulator row per config (double2[C]);
+//   tile_end(i0, nc, acc)    every thread, after the barrier that follows the last post();
+//   direct_end(i, valid)     the direct enumeration's thread for config i, after its last post().
+struct ActionHooks {
+    __device__ __forceinline__ void tile_begin(int64_t, int, double2*) {}
+    __device__ __forceinline__ void tile_end(int64_t, int, double2*) {}
+    __device__ __forceinline__ void direct_end(int64_t, bool) {}
+};
+
 // A hash index over a config set as the kernels see it.
 struct HashIndex {
     const uint64_t* table;
@@ -169,6 +181,7 @@ constexpr int kBatch = 2;
 
 // LDS carve-up for a tile of C = 1 << tile_log2 configs.
 struct EnumLds {
+    double2* acc;     // [C] per-config accumulator for gathering actions
     uint64_t* cfg;    // [C][W] tile configs
     uint64_t* hc;     // [C] their index hashes
     uint64_t* bits;   // [n_bits][C / 64] per-site occupancy bitsets
@@ -183,7 +196,8 @@ __device__ __forceinline__ EnumLds carve_lds(uint64_t* smem, int tile_log2, int 
     const int C = 1 << tile_log2;
     const int CW = C >> 6;
     EnumLds l;
-    l.cfg = smem;
+    l.acc = reinterpret_cast<double2*>(smem);
+    l.cfg = smem + 2 * C;
     l.hc = l.cfg + C * W;
     l.bits = l.hc + C;
     l.valid = l.bits + n_bits * CW;
@@ -241,6 +255,7 @@ __device__ void enumerate_tile(const EnumSource& p, const HashIndex& ix, int64_t
         }
         L.hc[r] = hash_cfg<W>(c, ix.cols);
     }
+    act.tile_begin(i0, nc, L.acc);
     __syncthreads();
     for (int q = wave; q < CW; q += kEnumBlock / 64) {
         const int r = q * 64 + lane;
@@ -358,6 +373,7 @@ __device__ void enumerate_tile(const EnumSource& p, const HashIndex& ix, int64_t
         }
         __syncthreads();
     }
+    act.tile_end(i0, nc, L.acc);
 }
 
 // Direct enumeration for small term tables: each thread owns one config in registers and walks
@@ -437,12 +453,13 @@ __device__ void enumerate_direct(const EnumSource& p, const HashIndex& ix, int64
             }
         }
     }
+    act.direct_end(i, valid);
 }
 
 inline size_t enum_lds_bytes_of(int n_words, int n_bits, int tile_log2) {
     const size_t C = static_cast<size_t>(1) << tile_log2;
     const size_t CW = C / 64;
-    return 2 * (C * n_words * 8 + C * 8) + static_cast<size_t>(n_bits) * CW * 8 + CW * 8 + C * 2;
+    return C * 16 + 2 * (C * n_words * 8 + C * 8) + static_cast<size_t>(n_bits) * CW * 8 + CW * 8 + C * 2;
 }
 
 template <typename F>

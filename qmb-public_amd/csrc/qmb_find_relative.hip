@@ -78,7 +78,7 @@ __device__ __forceinline__ uint64_t weight_bits(const double2 cf, const double2 
 }
 
 template <int W>
-struct HistAction {
+struct HistAction : ActionHooks {
     const FrParams& p;
     uint32_t* hist;
     uint64_t prefix;
@@ -101,7 +101,7 @@ struct HistAction {
 };
 
 template <int W>
-struct CollectAction {
+struct CollectAction : ActionHooks {
     const FrParams& p;
     uint64_t prefix;
     int shift;

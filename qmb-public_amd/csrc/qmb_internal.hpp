@@ -83,7 +83,7 @@ struct LaunchShape {
     int tile_log2;
     int groups;
     bool direct;
-    bool symmetric;
+    int symmetric;  // 0 off, 1 gather, 2 half-lookup scatter
 };
 
 // Term tables up to this size use the direct strategy by default.
@@ -97,8 +97,9 @@ size_t direct_lds_bytes(int n_words);
 // Builders (qmb_build.hip).
 // ---------------------------------------------------------------------------------------------
 int terms_layout(Carver& cv, int n_qubytes, int64_t n_terms, qmb_terms* out, void** scratch, size_t* scratch_bytes);
+// exclude_pairs: leave out the pair-excitation terms (they go to a qmb_pairs table instead).
 int terms_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
-                int64_t n_terms, void* buffer, size_t bytes, qmb_terms* out, hipStream_t stream);
+                int64_t n_terms, void* buffer, size_t bytes, qmb_terms* out, hipStream_t stream, int exclude_pairs = 0);
 int targets_layout(Carver& cv, int n_qubytes, int64_t n, qmb_targets* out);
 int targets_build(int n_qubytes, const uint8_t* configs, int64_t n, void* buffer, size_t bytes, qmb_targets* out,
                   hipStream_t stream);
@@ -134,6 +135,15 @@ int single_relative_prepared(int n_qubytes, int particle_cut, const uint8_t* con
                              const qmb_targets* exclude, uint8_t* out, void* scratch, size_t scratch_bytes,
                              hipStream_t stream);
 size_t single_relative_scratch_bytes(int64_t n_terms);
+
+// Pair-excitation path (qmb_pairs.hip).
+size_t pairs_bytes(int n_qubytes);
+int pairs_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
+                void* buffer, size_t bytes, qmb_pairs* out, hipStream_t stream);
+size_t cores_bytes(int n_qubytes, int64_t n, int max_occupied);
+int cores_build(const qmb_targets* targets, int max_occupied, void* buffer, size_t bytes, qmb_cores* out, hipStream_t stream);
+int apply_pairs(const qmb_pairs* pairs, const qmb_cores* cores, const qmb_targets* targets, const double* in, double* out,
+                int mode, hipStream_t stream);
 
 size_t find_relative_scratch_bytes(int n_qubytes, int64_t count_selected);
 int find_relative_prepared(const qmb_terms* terms, const qmb_targets* exclude, const uint8_t* configs_i, const double* psi_i,

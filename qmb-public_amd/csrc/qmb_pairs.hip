@@ -1,0 +1,525 @@
+// qmb_pairs.hip -- the pair-excitation path of apply_within on gfx950.
+//
+// Two configs a, b of one set are connected by a pair excitation (annihilate the pair Pa of a,
+// create the pair Pb) exactly when a \ Pa == b \ Pb ("core") and Pa, Pb are disjoint.  So instead
+// of generating every (config, term) target and probing the index for it -- the reference's
+// scheme, qmb/_hamiltonian_cuda.cu:86-135, which for a molecular Hamiltonian spends ~90% of its
+// work on double-excitation targets that are not in the set -- every config emits one entry per
+// occupied pair (its cores), the entries are radix-sorted by core hash, and a wave walks each group
+// of equal cores, combining its rows pairwise with the merged coefficient table of qmb_pairs.
+// Only connections whose target is in the set are visited; each output row sums in registers and
+// reaches HBM with one FP64 atomic per group it belongs to.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "qmb_internal.hpp"
+
+namespace qmb {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kKeyBits = 48;
+constexpr uint64_t kKeyMask = (1ull << kKeyBits) - 1ull;
+constexpr uint64_t kKeySentinel = kKeyMask;  // unused entries; sorts after every real key
+constexpr int kMaxPairBytes = 16;            // the dense table is n_pairs^2 entries
+
+inline unsigned blocks_for(int64_t n, int block = kBlock) {
+    int64_t b = (n + block - 1) / block;
+    return static_cast<unsigned>(b < 1 ? 1 : b);
+}
+
+__global__ void zero_counts_kernel(int64_t* counts) {
+    if (threadIdx.x < 4) {
+        counts[threadIdx.x] = 0;
+    }
+}
+
+// ------------------------------------------------------------------------------ coefficients --
+__global__ void pairs_fill_kernel(const int16_t* __restrict__ site, const uint8_t* __restrict__ kind,
+                                  const double2* __restrict__ coef, int64_t n_terms, int particle_cut, int n_bits,
+                                  int64_t n_pairs, double2* __restrict__ table, int64_t* __restrict__ counts) {
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= n_terms) {
+        return;
+    }
+    const CompiledTerm c = compile_term(site + 4 * t, kind + 4 * t, particle_cut, n_bits);
+    PairExcitation e;
+    if (c.status != kCompileOk || !pair_excitation(c.rec, particle_cut, &e)) {
+        return;
+    }
+    const double2 v = coef[t];
+    double2* dst = table + pair_index(e.r0, e.r1) * n_pairs + pair_index(e.c0, e.c1);
+    unsafeAtomicAdd(&dst->x, e.delta ? -v.x : v.x);
+    unsafeAtomicAdd(&dst->y, e.delta ? -v.y : v.y);
+    atomicAdd(reinterpret_cast<unsigned long long*>(&counts[0]), 1ull);
+}
+
+int pairs_layout(Carver& cv, int n_qubytes, qmb_pairs* out) {
+    QMB_TRY(check_qubytes(n_qubytes));
+    if (n_qubytes > kMaxPairBytes) {
+        return set_error(QMB_ERR_UNSUPPORTED, "the pair-excitation table supports n_qubytes <= %d, got %d", kMaxPairBytes,
+                         n_qubytes);
+    }
+    qmb_pairs p{};
+    p.n_qubytes = n_qubytes;
+    p.n_bits = 8 * n_qubytes;
+    p.n_pairs = static_cast<int64_t>(p.n_bits) * (p.n_bits - 1) / 2;
+    p.buffer = cv.base;
+    p.counts = cv.take<int64_t>(4);
+    p.coef = cv.take<double>(2 * static_cast<size_t>(p.n_pairs) * static_cast<size_t>(p.n_pairs));
+    p.bytes = cv.off;
+    if (out) {
+        *out = p;
+    }
+    return QMB_OK;
+}
+
+// ----------------------------------------------------------------------------------- cores --
+template <int W>
+__device__ __forceinline__ Cfg<W> load_key(const uint64_t* __restrict__ keys, int64_t i) {
+    Cfg<W> c;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        c.w[w] = keys[i * W + w];
+    }
+    return c;
+}
+
+template <int W>
+__device__ __forceinline__ void flip_site(Cfg<W>& c, int s) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        if (w == (s >> 6)) {
+            c.w[w] ^= 1ull << (s & 63);
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t core_key(uint64_t h) {
+    const uint64_t k = h & kKeyMask;
+    return k == kKeySentinel ? k - 1 : k;  // a merged hash value only makes its group "mixed"
+}
+
+// One thread per config: every occupied pair (x < y) gives the entry (hash of the core, row).
+template <int W>
+__global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64_t* __restrict__ cols, int64_t n, int per_config,
+                                 uint64_t* __restrict__ okeys, uint64_t* __restrict__ ovals, int64_t* __restrict__ counts) {
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const Cfg<W> c = load_key<W>(keys, i);
+    const uint64_t h = hash_cfg<W>(c, cols);
+    const int64_t base = i * per_config;
+    int64_t k = 0;
+#pragma unroll
+    for (int wx = 0; wx < W; ++wx) {
+        uint64_t mx = c.w[wx];
+        while (mx) {
+            const int x = 64 * wx + __builtin_ctzll(mx);
+            mx &= mx - 1;
+            const uint64_t hx = h ^ cols[x];
+#pragma unroll
+            for (int wy = 0; wy < W; ++wy) {
+                if (wy < wx) {
+                    continue;
+                }
+                uint64_t my = wy == wx ? mx : c.w[wy];
+                while (my) {
+                    const int y = 64 * wy + __builtin_ctzll(my);
+                    my &= my - 1;
+                    if (k < per_config) {
+                        okeys[base + k] = core_key(hx ^ cols[y]);
+                        ovals[base + k] = (static_cast<uint64_t>(i) << 16) | (static_cast<uint64_t>(y) << 8) |
+                                          static_cast<uint64_t>(x);
+                    }
+                    ++k;
+                }
+            }
+        }
+    }
+    if (k > per_config) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&counts[2]), 1ull);
+    }
+    for (; k < per_config; ++k) {
+        okeys[base + k] = kKeySentinel;
+        ovals[base + k] = 0;
+    }
+}
+
+__global__ void core_flag_kernel(const uint64_t* __restrict__ keys, int64_t cap, uint32_t* __restrict__ flag) {
+    const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (e >= cap) {
+        return;
+    }
+    const uint64_t k = keys[e];
+    flag[e] = (k != kKeySentinel && (e == 0 || keys[e - 1] != k)) ? 1u : 0u;
+}
+
+// starts[g] = first entry of group g; starts[G] = number of used entries; counts {entries, G}.
+__global__ void core_start_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flag,
+                                  const uint32_t* __restrict__ gid, int64_t cap, uint32_t* __restrict__ starts,
+                                  int64_t* __restrict__ counts) {
+    const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (e >= cap) {
+        return;
+    }
+    const uint64_t k = keys[e];
+    if (k == kKeySentinel) {
+        return;
+    }
+    if (flag[e]) {
+        starts[gid[e]] = static_cast<uint32_t>(e);
+    }
+    if (e == cap - 1 || keys[e + 1] == kKeySentinel) {
+        const uint32_t groups = gid[e] + flag[e];
+        starts[groups] = static_cast<uint32_t>(e + 1);
+        counts[0] = e + 1;
+        counts[1] = groups;
+    }
+}
+
+int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, qmb_cores* out, void** scratch, size_t* scratch_bytes) {
+    QMB_TRY(check_qubytes(n_qubytes));
+    if (n < 0) {
+        return set_error(QMB_ERR_ARGUMENT, "n must be >= 0");
+    }
+    if (max_occupied < 0 || max_occupied > 8 * n_qubytes) {
+        return set_error(QMB_ERR_ARGUMENT, "max_occupied must be in [0, %d], got %d", 8 * n_qubytes, max_occupied);
+    }
+    qmb_cores c{};
+    c.n_qubytes = n_qubytes;
+    c.n_words = n_words_of(n_qubytes);
+    c.max_occupied = max_occupied;
+    c.per_config = max_occupied * (max_occupied - 1) / 2;
+    c.n = n;
+    c.capacity = n * c.per_config;
+    if (c.capacity >= (1ll << 32) - 1) {
+        return set_error(QMB_ERR_UNSUPPORTED, "core index of %lld entries exceeds 2^32", static_cast<long long>(c.capacity));
+    }
+    const size_t cap = static_cast<size_t>(c.capacity > 0 ? c.capacity : 1);
+    c.buffer = cv.base;
+    c.counts = cv.take<int64_t>(4);
+    c.keys = cv.take<uint64_t>(cap);
+    c.vals = cv.take<uint64_t>(cap);
+    c.starts = cv.take<uint32_t>(cap + 1);
+    size_t tmp = 0;
+    {
+        uint64_t* k = nullptr;
+        uint32_t* u = nullptr;
+        if (rocprim::radix_sort_pairs(nullptr, tmp, k, k, k, k, static_cast<unsigned>(cap), 0, kKeyBits) != hipSuccess) {
+            return set_error(QMB_ERR_HIP, "rocprim radix_sort_pairs size query failed");
+        }
+        size_t scan = 0;
+        if (rocprim::exclusive_scan(nullptr, scan, u, u, 0u, cap, rocprim::plus<uint32_t>()) != hipSuccess) {
+            return set_error(QMB_ERR_HIP, "rocprim exclusive_scan size query failed");
+        }
+        tmp = tmp > scan ? tmp : scan;
+    }
+    const size_t s0 = cv.off;
+    void* s = cv.take_bytes(2 * cap * sizeof(uint64_t) + 2 * cap * sizeof(uint32_t) + tmp + 4096);
+    if (scratch) {
+        *scratch = s;
+    }
+    if (scratch_bytes) {
+        *scratch_bytes = cv.off - s0;
+    }
+    c.bytes = cv.off;
+    if (out) {
+        *out = c;
+    }
+    return QMB_OK;
+}
+
+// ------------------------------------------------------------------------------ group kernel --
+struct PairParams {
+    const double2* coef;    // merged pair-excitation coefficients
+    int64_t n_pairs;
+    int particle_cut;
+    const uint64_t* ckeys;  // config words of the set, [n][W]
+    const uint64_t* vals;   // sorted entries
+    const uint32_t* starts;
+    const int64_t* counts;
+    const double2* in;
+    double2* out;
+};
+
+template <int W>
+__device__ __forceinline__ int pair_parity(const Cfg<W>& c, int a, int b, int x, int y) {
+    int par = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const uint64_t m = low_mask_word(a, w) ^ low_mask_word(b, w) ^ low_mask_word(x, w) ^ low_mask_word(y, w);
+        par ^= __popcll(c.w[w] & m) & 1;
+    }
+    return par;
+}
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int lane) {
+    const uint32_t lo = __shfl(static_cast<uint32_t>(v), lane);
+    const uint32_t hi = __shfl(static_cast<uint32_t>(v >> 32), lane);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ double2 shfl_d2(const double2& v, int lane) {
+    return make_double2(__shfl(v.x, lane), __shfl(v.y, lane));
+}
+
+constexpr int kPairBatch = 4;
+
+// One wave, one group of entries [s, e) (wave-uniform).  Lane = row a; the group's entries b are
+// staged 64 at a time in registers and broadcast with shuffles.
+//   ADJ:  out[a] += sum_b conj(A_ba) in[b],  A_ba = (-1)^pair_parity(c_a) T[Pa][Pb]
+//   !ADJ: out[a] += sum_b A_ab in[b],        A_ab = (-1)^pair_parity(c_b) T[Pb][Pa]
+template <int W, bool ADJ>
+__device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane) {
+    const uint64_t v0 = p.vals[s];
+    Cfg<W> core0 = load_key<W>(p.ckeys, static_cast<int64_t>(v0 >> 16));
+    flip_site<W>(core0, static_cast<int>(v0 & 0xff));
+    flip_site<W>(core0, static_cast<int>((v0 >> 8) & 0xff));
+    const bool fermi = p.particle_cut == 1;
+    for (uint32_t a0 = s; a0 < e; a0 += 64) {
+        const uint32_t a = a0 + lane;
+        const bool act = a < e;
+        const uint64_t va = act ? p.vals[a] : v0;
+        const int64_t ia = static_cast<int64_t>(va >> 16);
+        const int r0 = static_cast<int>(va & 0xff);
+        const int r1 = static_cast<int>((va >> 8) & 0xff);
+        const Cfg<W> ca = load_key<W>(p.ckeys, ia);
+        Cfg<W> core = ca;
+        flip_site<W>(core, r0);
+        flip_site<W>(core, r1);
+        bool same = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            same = same && core.w[w] == core0.w[w];
+        }
+        // Different cores with equal 48-bit hashes share a group: then every pair is checked exactly.
+        const bool mixed = __ballot(act && !same) != 0ull;
+        const int64_t prow = pair_index(r0, r1);
+        double acc_re = 0.0, acc_im = 0.0;
+        for (uint32_t b0 = s; b0 < e; b0 += 64) {
+            const uint32_t bl = b0 + lane;
+            const bool bin = bl < e;
+            const uint64_t vbl = bin ? p.vals[bl] : 0ull;
+            const double2 gbl = bin ? p.in[vbl >> 16] : make_double2(0.0, 0.0);
+            const int nb = static_cast<int>(e - b0 < 64u ? e - b0 : 64u);
+            for (int u = 0; u < nb; u += kPairBatch) {
+                double2 x[kPairBatch], g[kPairBatch];
+                int sg[kPairBatch];
+                bool ok[kPairBatch];
+#pragma unroll
+                for (int k = 0; k < kPairBatch; ++k) {
+                    const int uu = (u + k) & 63;
+                    const uint64_t vb = shfl_u64(vbl, uu);
+                    g[k] = shfl_d2(gbl, uu);
+                    const int q0 = static_cast<int>(vb & 0xff);
+                    const int q1 = static_cast<int>((vb >> 8) & 0xff);
+                    bool o = act && (u + k < nb) && (b0 + static_cast<uint32_t>(u + k) != a) && q0 != r0 && q0 != r1 && q1 != r0 &&
+                             q1 != r1;
+                    if (mixed && o) {
+                        Cfg<W> cb = load_key<W>(p.ckeys, static_cast<int64_t>(vb >> 16));
+                        flip_site<W>(cb, q0);
+                        flip_site<W>(cb, q1);
+#pragma unroll
+                        for (int w = 0; w < W; ++w) {
+                            o = o && cb.w[w] == core.w[w];
+                        }
+                    }
+                    ok[k] = o;
+                    int64_t idx;
+                    if (ADJ) {
+                        idx = prow * p.n_pairs + pair_index(q0, q1);
+                        sg[k] = fermi ? pair_parity<W>(ca, r0, r1, q0, q1) : 0;
+                    } else {
+                        Cfg<W> cb = core;
+                        flip_site<W>(cb, q0);
+                        flip_site<W>(cb, q1);
+                        idx = pair_index(q0, q1) * p.n_pairs + prow;
+                        sg[k] = fermi ? pair_parity<W>(cb, r0, r1, q0, q1) : 0;
+                    }
+                    x[k] = o ? p.coef[idx] : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int k = 0; k < kPairBatch; ++k) {
+                    if (ok[k]) {
+                        const double xr = sg[k] ? -x[k].x : x[k].x;
+                        const double xi = sg[k] ? -x[k].y : x[k].y;
+                        if (ADJ) {  // conj(x) * g
+                            acc_re += xr * g[k].x + xi * g[k].y;
+                            acc_im += xr * g[k].y - xi * g[k].x;
+                        } else {    // x * g
+                            acc_re += xr * g[k].x - xi * g[k].y;
+                            acc_im += xr * g[k].y + xi * g[k].x;
+                        }
+                    }
+                }
+            }
+        }
+        if (act && (acc_re != 0.0 || acc_im != 0.0)) {
+            unsafeAtomicAdd(&p.out[ia].x, acc_re);
+            unsafeAtomicAdd(&p.out[ia].y, acc_im);
+        }
+    }
+}
+
+// Persistent waves over the groups, 64 groups per step: the lanes test their group's size and the
+// wave walks the ones with at least two entries.  Every wave leaves when the group count is done.
+template <int W, bool ADJ>
+__global__ __launch_bounds__(kBlock) void pair_kernel(PairParams p) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6;
+    const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * kBlock) >> 6;
+    if (p.counts[2] != 0) {
+        // a row had more occupied sites than the index was built for
+        if (wave == 0 && lane == 0) {
+            p.out[0] = make_double2(__longlong_as_double(0x7ff8000000000000ll), __longlong_as_double(0x7ff8000000000000ll));
+        }
+        return;
+    }
+    const int64_t groups = p.counts[1];
+    for (int64_t g0 = wave * 64; g0 < groups; g0 += nwaves * 64) {
+        const int64_t gg = g0 + lane;
+        uint32_t gs = 0, ge = 0;
+        if (gg < groups) {
+            gs = p.starts[gg];
+            ge = p.starts[gg + 1];
+        }
+        uint64_t multi = __ballot(gg < groups && ge - gs >= 2u);
+        while (multi) {
+            const int l = __builtin_ctzll(multi);
+            multi &= multi - 1;
+            const uint32_t s = __shfl(gs, l);
+            const uint32_t e = __shfl(ge, l);
+            pair_group<W, ADJ>(p, s, e, lane);
+        }
+    }
+}
+
+}  // namespace
+
+size_t pairs_bytes(int n_qubytes) {
+    Carver cv{nullptr, 0};
+    return pairs_layout(cv, n_qubytes, nullptr) == QMB_OK ? cv.off : 0;
+}
+
+int pairs_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
+                void* buffer, size_t bytes, qmb_pairs* out, hipStream_t stream) {
+    if (particle_cut != 1 && particle_cut != 2) {
+        return set_error(QMB_ERR_UNSUPPORTED, "particle_cut must be 1 or 2, got %d", particle_cut);
+    }
+    Carver cv{static_cast<char*>(buffer), 0};
+    qmb_pairs p{};
+    QMB_TRY(pairs_layout(cv, n_qubytes, &p));
+    if (cv.off > bytes) {
+        return set_error(QMB_ERR_WORKSPACE, "pair table buffer too small: need %zu bytes, have %zu", cv.off, bytes);
+    }
+    p.particle_cut = particle_cut;
+    p.bytes = bytes;
+    zero_counts_kernel<<<1, 64, 0, stream>>>(p.counts);
+    QMB_CHECK_LAUNCH();
+    QMB_TRY(launch_zero_f64(p.coef, 2 * p.n_pairs * p.n_pairs, stream));
+    if (n_terms > 0) {
+        pairs_fill_kernel<<<blocks_for(n_terms), kBlock, 0, stream>>>(site, kind, reinterpret_cast<const double2*>(coef), n_terms,
+                                                                      particle_cut, p.n_bits, p.n_pairs,
+                                                                      reinterpret_cast<double2*>(p.coef), p.counts);
+        QMB_CHECK_LAUNCH();
+    }
+    *out = p;
+    return QMB_OK;
+}
+
+size_t cores_bytes(int n_qubytes, int64_t n, int max_occupied) {
+    Carver cv{nullptr, 0};
+    return cores_layout(cv, n_qubytes, n, max_occupied, nullptr, nullptr, nullptr) == QMB_OK ? cv.off : 0;
+}
+
+int cores_build(const qmb_targets* targets, int max_occupied, void* buffer, size_t bytes, qmb_cores* out, hipStream_t stream) {
+    if (!targets) {
+        return set_error(QMB_ERR_ARGUMENT, "targets must be non-null");
+    }
+    Carver cv{static_cast<char*>(buffer), 0};
+    qmb_cores c{};
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    QMB_TRY(cores_layout(cv, targets->n_qubytes, targets->n, max_occupied, &c, &scratch, &scratch_bytes));
+    if (cv.off > bytes) {
+        return set_error(QMB_ERR_WORKSPACE, "core index buffer too small: need %zu bytes, have %zu", cv.off, bytes);
+    }
+    c.bytes = bytes;
+    zero_counts_kernel<<<1, 64, 0, stream>>>(c.counts);
+    QMB_CHECK_LAUNCH();
+    if (c.capacity > 0) {
+        const size_t cap = static_cast<size_t>(c.capacity);
+        Carver sc{static_cast<char*>(scratch), 0};
+        uint64_t* keys_in = sc.take<uint64_t>(cap);
+        uint64_t* vals_in = sc.take<uint64_t>(cap);
+        uint32_t* flag = sc.take<uint32_t>(cap);
+        uint32_t* gid = sc.take<uint32_t>(cap);
+        size_t tmp_bytes = scratch_bytes > sc.off + 512 ? scratch_bytes - sc.off - 512 : 0;
+        void* tmp = sc.take_bytes(tmp_bytes);
+        const unsigned nb = blocks_for(c.n);
+        switch (c.n_words) {
+            case 1: core_emit_kernel<1><<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, c.per_config, keys_in, vals_in, c.counts); break;
+            case 2: core_emit_kernel<2><<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, c.per_config, keys_in, vals_in, c.counts); break;
+            case 3: core_emit_kernel<3><<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, c.per_config, keys_in, vals_in, c.counts); break;
+            default: core_emit_kernel<4><<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, c.per_config, keys_in, vals_in, c.counts); break;
+        }
+        QMB_CHECK_LAUNCH();
+        size_t need = tmp_bytes;
+        QMB_HIP_TRY(rocprim::radix_sort_pairs(tmp, need, keys_in, c.keys, vals_in, c.vals, static_cast<unsigned>(cap), 0, kKeyBits,
+                                              stream));
+        core_flag_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, c.capacity, flag);
+        QMB_CHECK_LAUNCH();
+        need = tmp_bytes;
+        QMB_HIP_TRY(rocprim::exclusive_scan(tmp, need, flag, gid, 0u, cap, rocprim::plus<uint32_t>(), stream));
+        core_start_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, flag, gid, c.capacity, c.starts, c.counts);
+        QMB_CHECK_LAUNCH();
+    }
+    *out = c;
+    return QMB_OK;
+}
+
+int apply_pairs(const qmb_pairs* pairs, const qmb_cores* cores, const qmb_targets* targets, const double* in, double* out,
+                int mode, hipStream_t stream) {
+    if (!pairs || !cores || !targets) {
+        return set_error(QMB_ERR_ARGUMENT, "pairs, cores and targets must be non-null");
+    }
+    if (pairs->n_qubytes != cores->n_qubytes || cores->n_qubytes != targets->n_qubytes || cores->n != targets->n) {
+        return set_error(QMB_ERR_ARGUMENT, "pairs, cores and targets describe different shapes");
+    }
+    if (mode != 0 && mode != 1) {
+        return set_error(QMB_ERR_ARGUMENT, "mode must be 0 (forward) or 1 (adjoint)");
+    }
+    if (cores->capacity == 0) {
+        return QMB_OK;
+    }
+    PairParams p{};
+    p.coef = reinterpret_cast<const double2*>(pairs->coef);
+    p.n_pairs = pairs->n_pairs;
+    p.particle_cut = pairs->particle_cut;
+    p.ckeys = targets->keys;
+    p.vals = cores->vals;
+    p.starts = cores->starts;
+    p.counts = cores->counts;
+    p.in = reinterpret_cast<const double2*>(in);
+    p.out = reinterpret_cast<double2*>(out);
+    // persistent waves: enough to fill every CU several times over, never more than the groups need
+    const int64_t want = (cores->capacity + 64 * 4 - 1) / (64 * 4);
+    const unsigned grid = static_cast<unsigned>(want < 4096 ? (want < 1 ? 1 : want) : 4096);
+    auto go = [&](auto kernel) -> int {
+        kernel<<<grid, kBlock, 0, stream>>>(p);
+        QMB_CHECK_LAUNCH();
+        return QMB_OK;
+    };
+    if (cores->n_words == 1) {
+        return mode == 1 ? go(&pair_kernel<1, true>) : go(&pair_kernel<1, false>);
+    }
+    if (cores->n_words == 2) {
+        return mode == 1 ? go(&pair_kernel<2, true>) : go(&pair_kernel<2, false>);
+    }
+    return set_error(QMB_ERR_UNSUPPORTED, "pair path supports n_qubytes <= %d", kMaxPairBytes);
+}
+
+}  // namespace qmb

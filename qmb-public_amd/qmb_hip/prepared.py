@@ -24,12 +24,21 @@ STRATEGY_TILES = 1
 STRATEGY_DIRECT = 2
 # auto picks the direct kernel up to this many terms (kDirectMaxTerms in the library)
 DIRECT_MAX_TERMS = 4096
+# the dense pair-excitation table covers configs of up to this many bytes
+PAIR_MAX_QUBYTES = 16
 
 
-def _launch(tile: int = 0, groups: int = 0, strategy: int = 0, symmetric: bool = False) -> _lib.QmbLaunch | None:
-    if tile == 0 and groups == 0 and strategy == 0 and not symmetric:
+# qmb_launch.symmetric values
+SYMMETRIC_OFF = 0
+SYMMETRIC_GATHER = 1   # forward computed as the adjoint gather (A = A^H): no atomics per contribution
+SYMMETRIC_HALF = 2     # each connected pair looked up once, both directions scattered with atomics
+
+
+def _launch(tile: int = 0, groups: int = 0, strategy: int = 0, symmetric: bool | int = False) -> _lib.QmbLaunch | None:
+    sym = SYMMETRIC_GATHER if symmetric is True else int(symmetric)
+    if tile == 0 and groups == 0 and strategy == 0 and sym == 0:
         return None
-    return _lib.QmbLaunch(tile, groups, strategy, 1 if symmetric else 0)
+    return _lib.QmbLaunch(tile, groups, strategy, sym)
 
 
 def can_use_symmetric(terms: "PreparedTerms", targets: "PreparedTargets", configs_i: torch.Tensor) -> bool:
@@ -54,7 +63,7 @@ class PreparedTerms:
     """Compiled, bucketed term table on one device (qmb_terms_build)."""
 
     def __init__(self, site: torch.Tensor, kind: torch.Tensor, coef: torch.Tensor, n_qubytes: int, particle_cut: int,
-                 check_hermitian: bool = True) -> None:
+                 check_hermitian: bool = True, pairs: bool = True) -> None:
         device = site.device
         site, kind, coef = site.contiguous(), kind.contiguous(), coef.contiguous()
         self.n_terms = _check_terms(site, kind, coef, device, n_qubytes)
@@ -72,6 +81,30 @@ class PreparedTerms:
             _lib.check(lib.qmb_terms_build(n_qubytes, particle_cut, _ptr(site), _ptr(kind), _ptr(coef), self.n_terms, _ptr(self.buffer),
                                            self.buffer.numel(), ctypes.byref(self.desc), _stream(device)), "qmb_terms_build")
         self.hermitian = is_hermitian(site, kind, coef, n_qubytes, particle_cut) if check_hermitian else False
+        # Pair-excitation path: merged coefficient table + the table without those terms.
+        self.pairs_desc: _lib.QmbPairs | None = None
+        self.residual_desc: _lib.QmbTerms | None = None
+        self.n_pair_terms = 0
+        if pairs and n_qubytes <= PAIR_MAX_QUBYTES and self.n_terms > 0:
+            _lib.check(lib.qmb_pairs_workspace_size(n_qubytes, ctypes.byref(size)), "qmb_pairs_workspace_size")
+            pbuf = _workspace(size.value, device)
+            pdesc = _lib.QmbPairs()
+            with torch.cuda.device(device):
+                _lib.check(lib.qmb_pairs_build(n_qubytes, particle_cut, _ptr(site), _ptr(kind), _ptr(coef), self.n_terms, _ptr(pbuf),
+                                               pbuf.numel(), ctypes.byref(pdesc), _stream(device)), "qmb_pairs_build")
+            offset = int(pdesc.counts) - pbuf.data_ptr()
+            self.n_pair_terms = int(pbuf[offset:offset + 8].view(torch.int64).item())
+            if self.n_pair_terms > 0:
+                _lib.check(lib.qmb_terms_workspace_size(n_qubytes, self.n_terms, ctypes.byref(size)), "qmb_terms_workspace_size")
+                self.residual_buffer = _workspace(size.value, device)
+                self.residual_desc = _lib.QmbTerms()
+                with torch.cuda.device(device):
+                    _lib.check(lib.qmb_terms_build_residual(n_qubytes, particle_cut, _ptr(site), _ptr(kind), _ptr(coef), self.n_terms,
+                                                            _ptr(self.residual_buffer), self.residual_buffer.numel(),
+                                                            ctypes.byref(self.residual_desc), _stream(device)),
+                               "qmb_terms_build_residual")
+                self.pairs_buffer = pbuf
+                self.pairs_desc = pdesc
 
     def counts(self) -> tuple[int, int, int]:
         """(valid terms, diagonal terms, invalid terms) -- synchronises."""
@@ -101,6 +134,7 @@ class PreparedTargets:
             _lib.check(lib.qmb_targets_build(n_qubytes, _ptr(configs), self.n, _ptr(self.buffer), self.buffer.numel(),
                                              ctypes.byref(self.desc), _stream(device)), "qmb_targets_build")
         self._duplicates: int | None = None
+        self._cores: tuple[torch.Tensor, _lib.QmbCores] | None = None
 
     def duplicates(self) -> int:
         """Rows equal to an earlier row (synchronises once)."""
@@ -113,14 +147,43 @@ class PreparedTargets:
         """True when `configs` is the very tensor this index was built over."""
         return configs.data_ptr() == self.configs.data_ptr() and configs.shape == self.configs.shape
 
+    def max_occupied(self) -> int:
+        """Largest number of occupied sites in a row (synchronises)."""
+        if self.n == 0:
+            return 0
+        table = torch.tensor([bin(v).count("1") for v in range(256)], dtype=torch.int32, device=self.device)
+        return int(table[self.configs.long()].sum(dim=1).max().item())
+
+    def cores(self) -> _lib.QmbCores:
+        """Core index of the set for the pair-excitation path (built on first use)."""
+        if self._cores is None:
+            lib = _lib.library()
+            occ = self.max_occupied()
+            size = ctypes.c_size_t(0)
+            _lib.check(lib.qmb_cores_workspace_size(self.n_qubytes, self.n, occ, ctypes.byref(size)), "qmb_cores_workspace_size")
+            buf = _workspace(size.value, self.device)
+            desc = _lib.QmbCores()
+            with torch.cuda.device(self.device):
+                _lib.check(lib.qmb_cores_build(ctypes.byref(self.desc), occ, _ptr(buf), buf.numel(), ctypes.byref(desc),
+                                               _stream(self.device)), "qmb_cores_build")
+            self._cores = (buf, desc)
+        return self._cores[1]
+
+
+def can_use_pairs(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tensor) -> bool:
+    """The pair-excitation path applies within one distinct-row set indexed by `targets`."""
+    return terms.pairs_desc is not None and targets.indexes(configs_i) and targets.duplicates() == 0
+
 
 def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tensor, vec: torch.Tensor, mode: int,
           out: torch.Tensor | None = None, tile: int = 0, groups: int = 0, zero: bool = True, strategy: int = 0,
-          symmetric: bool | None = None) -> torch.Tensor:
+          symmetric: bool | int | None = None, pairs: bool | None = None) -> torch.Tensor:
     """mode 0: psi_j = A psi_i (vec = psi_i as float64 [n_i, 2]); mode 1: grad_i = A^H grad_j.
 
     `out` (float64 [n, 2]) is accumulated into when given, after zeroing unless zero=False.
     symmetric=None uses the symmetric kernel whenever can_use_symmetric() holds.
+    pairs=None takes the pair-excitation path whenever can_use_pairs() holds: the pair-excitation
+    terms are applied by the core-group kernel and the rest by the enumeration kernel.
     """
     configs_i = configs_i.contiguous()
     vec = vec.contiguous()
@@ -133,11 +196,21 @@ def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tenso
         out.zero_()
     if symmetric is None:
         symmetric = mode != MODE_COUNT and can_use_symmetric(terms, targets, configs_i)
+    use_pairs = mode != MODE_COUNT and (pairs is None or pairs) and can_use_pairs(terms, targets, configs_i)
+    if pairs and not use_pairs:
+        raise RuntimeError("the pair-excitation path needs pair terms and configs_i equal to the indexed, distinct-row set.")
     launch = _launch(tile, groups, strategy, symmetric)
     lib = _lib.library()
+    term_desc = terms.residual_desc if use_pairs else terms.desc
     with torch.cuda.device(device):
-        _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.desc), ctypes.byref(targets.desc), _ptr(configs_i), n_i, _ptr(vec), _ptr(out),
+        _lib.check(lib.qmb_apply_prepared(ctypes.byref(term_desc), ctypes.byref(targets.desc), _ptr(configs_i), n_i, _ptr(vec), _ptr(out),
                                           mode, ctypes.byref(launch) if launch else None, _stream(device)), "qmb_apply_prepared")
+        if use_pairs:
+            cores = targets.cores()
+            # a Hermitian table: forward and adjoint are the same map, computed as the adjoint gather
+            pmode = MODE_ADJOINT if symmetric else mode
+            _lib.check(lib.qmb_apply_pairs(ctypes.byref(terms.pairs_desc), ctypes.byref(cores), ctypes.byref(targets.desc), _ptr(vec),
+                                           _ptr(out), pmode, _stream(device)), "qmb_apply_pairs")
     return out
 
 

@@ -52,7 +52,7 @@ def test_prepared_symmetric_and_general(cuda_device: torch.device, name: str, pc
     targets = pp.PreparedTargets(cfg_t, cfg.shape[1])
     assert terms.hermitian and pp.can_use_symmetric(terms, targets, cfg_t)
     vec = torch.view_as_real(torch.from_numpy(psi).to(cuda_device)).contiguous()
-    for symmetric in (False, True):
+    for symmetric in (pp.SYMMETRIC_OFF, pp.SYMMETRIC_GATHER, pp.SYMMETRIC_HALF):
         for mode in (pp.MODE_FORWARD, pp.MODE_ADJOINT):
             out = pp.apply(terms, targets, cfg_t, vec, mode, strategy=strategy, symmetric=symmetric).cpu().numpy()
             got = out[:, 0] + 1j * out[:, 1]

@@ -172,3 +172,60 @@ def test_hermiticity_check() -> None:
     assert pp.is_hermitian(*sy.terms_from_dict(sy.ising_dict(3, 2, x=0.5, y=0.2, zh=1.0, yv=0.4)), 1, 2)
     assert not pp.is_hermitian(*sy.random_terms(10, 50, 1), 2, 1)
     assert isinstance(torch.zeros(1), torch.Tensor)
+
+
+def _pair_parity(config: np.ndarray, sites: tuple[int, int, int, int]) -> int:
+    """popcount(c & (L(r0) ^ L(r1) ^ L(c0) ^ L(c1))) mod 2 with L(s) = the sites below s."""
+    bits = np.unpackbits(config, bitorder="little").astype(np.int64)
+    return int(sum(bits[:s].sum() for s in sites) & 1)
+
+
+@pytest.mark.parametrize("pc", [1, 2])
+@pytest.mark.parametrize("nq", [1, 2, 7, 16])
+def test_pair_excitation_classifier_matches_oracle(pc: int, nq: int) -> None:
+    """Every operator order of two creations and two annihilations on distinct sites is a pair
+    excitation whose oracle sign is (-1)^(pair_parity(c) ^ delta) on every config it applies to;
+    terms with repeated sites, other kind mixes or fewer ops are not."""
+    lib = _lib.library()
+    rng = np.random.default_rng(7000 + 10 * nq + pc)
+    n_bits = 8 * nq
+    out = (ctypes.c_int32 * 5)()
+    seen_pair = seen_other = 0
+    for _ in range(300):
+        if rng.random() < 0.7 and n_bits >= 4:
+            site = rng.choice(n_bits, size=4, replace=False).astype(np.int16)
+            kind = rng.permutation(np.array([0, 0, 1, 1], dtype=np.uint8))
+        else:
+            n_ops = int(rng.integers(1, 5))
+            site = np.zeros(4, dtype=np.int16)
+            kind = np.full(4, 2, dtype=np.uint8)
+            site[:n_ops] = rng.integers(0, min(n_bits, 6), size=n_ops)
+            kind[:n_ops] = rng.integers(0, 2, size=n_ops)
+        rec, status, _, _ = _compile(site, kind, pc, nq)
+        is_pair = bool(lib.qmb_pair_excitation_host(rec, pc, out))
+        distinct = len(set(site.tolist())) == 4 and np.all(kind < 2)
+        expect = status == 0 and distinct and sorted(kind.tolist()) == [0, 0, 1, 1]
+        assert is_pair == expect, (site, kind)
+        if not is_pair:
+            seen_other += 1
+            continue
+        seen_pair += 1
+        r0, r1, c0, c1, delta = list(out)
+        ann = sorted(int(s) for s, k in zip(site, kind) if k == 0)
+        cre = sorted(int(s) for s, k in zip(site, kind) if k == 1)
+        assert [r0, r1] == ann and [c0, c1] == cre
+        for _ in range(6):
+            cfg = rng.integers(0, 256, size=nq, dtype=np.uint8)
+            bits = np.unpackbits(cfg, bitorder="little")
+            bits[[r0, r1]] = 1
+            bits[[c0, c1]] = 0
+            cfg = np.packbits(bits, bitorder="little")
+            ok, res, par = orc.apply_term(cfg, site, kind, pc)
+            assert ok
+            want = bits.copy()
+            want[[r0, r1]] = 0
+            want[[c0, c1]] = 1
+            np.testing.assert_array_equal(res, np.packbits(want, bitorder="little"))
+            sign = (_pair_parity(cfg, (r0, r1, c0, c1)) ^ delta) if pc == 1 else 0
+            assert par == sign, (site, kind, cfg)
+    assert seen_pair > 50 and seen_other > 20
