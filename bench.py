@@ -24,6 +24,7 @@ launch / its HIP-event time) and the reference CPU op timed on this host on a bo
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import pathlib
@@ -52,19 +53,19 @@ def build_workload(name: str) -> dict[str, object]:
     if name == "molecular52":
         site, kind, coef = sy.molecular_terms(26, 2001, threshold=1e-10)
         configs = sy.excitation_configs(26, 7, 7, 1_000_000, 2002)
-        pc, seeds, sample = 1, (2003, 2004), 1000
+        pc, seeds, sample, occ = 1, (2003, 2004), 1000, 14
         desc = {"workload": "molecular52", "spin_orbitals": 52, "electrons": "7a+7b", "n_configs": 1_000_000,
                 "configs": "HF + singles + doubles + seeded triples/quadruples", "threshold": 1e-10}
     elif name == "hubbard128":
         site, kind, coef = sy.terms_from_dict(sy.hubbard_dict(64, 1, 1.0, 4.0))
         configs = sy.hubbard_chain_configs(64, 4_000_000, 3000)
-        pc, seeds, sample = 1, (3001, 3002), 400_000
+        pc, seeds, sample, occ = 1, (3001, 3002), 400_000, 64
         desc = {"workload": "hubbard128", "sites": 64, "spin_orbitals": 128, "t": 1.0, "U": 4.0, "filling": "half",
                 "n_configs": int(len(configs))}
     elif name == "molecular20":
         site, kind, coef = sy.molecular_terms(10, 1001)
         configs = sy.sector_configs(10, 5, 5, 10_000, 1002)
-        pc, seeds, sample = 1, (1003, 1004), 10_000
+        pc, seeds, sample, occ = 1, (1003, 1004), 10_000, 10
         desc = {"workload": "molecular20", "spin_orbitals": 20, "electrons": "5a+5b", "n_configs": 10_000}
     else:
         raise SystemExit(f"unknown workload {name}")
@@ -74,12 +75,12 @@ def build_workload(name: str) -> dict[str, object]:
     grad = sy.random_psi(len(configs), seeds[1])
     log(f"workload {name}: {len(site)} terms, {len(configs)} configs x {configs.shape[1]} bytes, built in {time.time() - t0:.1f}s")
     return {"site": site, "kind": kind, "coef": coef, "configs": configs, "psi": psi, "grad": grad, "pc": pc,
-            "sample": min(sample, len(configs)), "desc": desc}
+            "sample": min(sample, len(configs)), "desc": desc, "max_occupied": occ}
 
 
 def read_traffic(workload: str) -> float | None:
     """HBM bytes per forward launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/."""
-    path = ROOT / "profiles" / f"apply_kernel_{workload}.md"
+    path = ROOT / "profiles" / f"apply_within_forward_{workload}.md"
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
@@ -143,6 +144,7 @@ def main() -> int:
     import numpy as np  # pylint: disable=import-outside-toplevel
     import torch  # pylint: disable=import-outside-toplevel
 
+    from qmb_hip import _lib  # pylint: disable=import-outside-toplevel
     from qmb_hip import autograd as ag  # pylint: disable=import-outside-toplevel
     from qmb_hip import ops  # pylint: disable=import-outside-toplevel
     from qmb_hip import prepared as pp  # pylint: disable=import-outside-toplevel
@@ -164,8 +166,11 @@ def main() -> int:
     valid_terms, diag_terms, bad_terms = terms.counts()
     assert bad_terms == 0
 
+    occ = int(w["max_occupied"])
+
     def step() -> torch.Tensor:
-        targets = pp.PreparedTargets(configs, nq)
+        # configs are distinct by construction; occ bounds the occupied sites of every row
+        targets = pp.PreparedTargets(configs, nq, distinct=True, max_occupied=occ)
         x = psi.detach().requires_grad_(True)
         y = ag.apply_within(x, configs, terms, targets, args.tile, args.groups)
         y.backward(grad)
@@ -187,39 +192,73 @@ def main() -> int:
     step_ms = wall * 1e3 / args.steps
     log(f"timed {args.steps} steps: {step_ms:.3f} ms/step (events {ev0.elapsed_time(ev1) / args.steps:.3f} ms)")
 
-    # Dominant kernel: the forward enumeration, timed alone with events on the stream it runs on.
-    targets = pp.PreparedTargets(configs, nq)
-    symmetric = pp.can_use_symmetric(terms, targets, configs)
-    direct = n_terms <= pp.DIRECT_MAX_TERMS
+    # The forward application (the dominant work), timed alone with events on the stream it runs on,
+    # then its parts: the pair-excitation kernel and the enumeration kernel over the other terms.
+    targets = pp.PreparedTargets(configs, nq, distinct=True, max_occupied=occ)
     m_gen, m_hit = pp.count(terms, targets, configs, args.tile, args.groups)
-    out = torch.zeros((n, 2), dtype=torch.float64, device=dev)
+    symmetric = pp.can_use_symmetric(terms, targets, configs)
+    use_pairs = pp.can_use_pairs(terms, targets, configs)
+    direct = n_terms <= pp.DIRECT_MAX_TERMS
     vec = torch.view_as_real(psi).contiguous()
-    reps = max(3, min(20, args.steps))
-    k_ms = []
-    for _ in range(reps):
-        out.zero_()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        pp.apply(terms, targets, configs, vec, pp.MODE_FORWARD, out=out, tile=args.tile, groups=args.groups, zero=False)
-        b.record(stream)
-        b.synchronize()
-        k_ms.append(a.elapsed_time(b))
-    kernel_ms = float(np.median(k_ms))
-    adj_ms = []
     gvec = torch.view_as_real(grad).contiguous()
-    gout = torch.zeros((n, 2), dtype=torch.float64, device=dev)
-    for _ in range(reps):
-        gout.zero_()
+    out = torch.zeros((n, 2), dtype=torch.float64, device=dev)
+    reps = max(3, min(20, args.steps))
+    lib = _lib.library()
+
+    def timed(fn: object) -> float:
+        ts = []
+        for _ in range(reps):
+            out.zero_()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            b.synchronize()
+            ts.append(a.elapsed_time(b))
+        return float(np.median(ts))
+
+    kernel_ms = timed(lambda: pp.apply(terms, targets, configs, vec, pp.MODE_FORWARD, out=out, tile=args.tile, groups=args.groups,
+                                       zero=False))
+    adjoint_ms = timed(lambda: pp.apply(terms, targets, configs, gvec, pp.MODE_ADJOINT, out=out, tile=args.tile, groups=args.groups,
+                                        zero=False))
+    parts = None
+    if use_pairs:
+        cores = targets.cores()
+        s_ptr = ctypes.c_void_p(stream.cuda_stream)
+        launch = pp._launch(args.tile, args.groups, 0, symmetric)  # pylint: disable=protected-access
+        pmode = pp.MODE_ADJOINT if symmetric else pp.MODE_FORWARD
+        pair_ms = timed(lambda: lib.qmb_apply_pairs(ctypes.byref(terms.pairs_desc), ctypes.byref(cores), ctypes.byref(targets.desc),
+                                                    ctypes.c_void_p(vec.data_ptr()), ctypes.c_void_p(out.data_ptr()), pmode, s_ptr))
+        residual_ms = timed(lambda: lib.qmb_apply_prepared(ctypes.byref(terms.residual_desc), ctypes.byref(targets.desc),
+                                                           ctypes.c_void_p(configs.data_ptr()), n, ctypes.c_void_p(vec.data_ptr()),
+                                                           ctypes.c_void_p(out.data_ptr()), pp.MODE_FORWARD,
+                                                           ctypes.byref(launch) if launch else None, s_ptr))
+
+        def rebuild() -> None:
+            targets._cores = None  # pylint: disable=protected-access
+            targets.cores()
+
+        cores_ms = timed(rebuild)
+        parts = {"pair_kernel_ms": pair_ms, "enumeration_kernel_ms": residual_ms, "core_index_build_ms": cores_ms,
+                 "pair_excitation_terms": terms.n_pair_terms}
+        log(f"pair kernel {pair_ms:.3f} ms, enumeration kernel (other terms) {residual_ms:.3f} ms, core index {cores_ms:.3f} ms")
+
+    # achievable HBM bandwidth: a plain 1 GiB device copy
+    src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    cps = []
+    for _ in range(5):
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record(stream)
-        pp.apply(terms, targets, configs, gvec, pp.MODE_ADJOINT, out=gout, tile=args.tile, groups=args.groups, zero=False)
+        dst.copy_(src)
         b.record(stream)
         b.synchronize()
-        adj_ms.append(a.elapsed_time(b))
-    adjoint_ms = float(np.median(adj_ms))
+        cps.append(a.elapsed_time(b))
+    copy_gbps = 2 * src.numel() * 8 / (float(np.median(cps)) * 1e-3) / 1e9
+    del src, dst
     algo_bytes = n * (nq + 16) + n * (nq + 16) + n_terms * 28 + m_gen * nq + m_hit * 16
     achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
-    log(f"M_gen={m_gen} M_hit={m_hit}; forward kernel {kernel_ms:.3f} ms, adjoint kernel {adjoint_ms:.3f} ms")
+    log(f"M_gen={m_gen} M_hit={m_hit}; forward {kernel_ms:.3f} ms, adjoint {adjoint_ms:.3f} ms, copy {copy_gbps:.0f} GB/s")
 
     traffic = read_traffic(args.workload)
     if args.copy_calibration > 0:
@@ -276,11 +315,13 @@ def main() -> int:
         "data": "synthetic (seeded)",
         "config": dict(w["desc"], n_terms=n_terms, n_qubytes=nq, M_gen=m_gen, M_hit=m_hit, valid_terms=valid_terms,
                        diagonal_terms=diag_terms, tile=args.tile or "auto", groups=args.groups or 8,
-                       kernel_path=("symmetric " if symmetric else "scatter/gather ") + ("direct" if direct else "bucketed tiles")),
+                       kernel_path=("pair-excitation core groups + " if use_pairs else "")
+                       + ("symmetric " if symmetric else "scatter/gather ") + ("direct" if direct else "bucketed tiles")),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": traffic,
-                     "kernel": f"apply_kernel<W={(nq + 7) // 8},{'symmetric' if symmetric else 'forward'},{'direct' if direct else 'tiles'}>",
-                     "kernel_ms": kernel_ms,
+                     "kernel": ((f"pair_kernel<W={(nq + 7) // 8},{'adjoint' if symmetric else 'forward'}> + " if use_pairs else "")
+                                + f"apply_kernel<W={(nq + 7) // 8},{'gather' if symmetric else 'forward'},{'direct' if direct else 'tiles'}>"),
+                     "kernel_ms": kernel_ms, "parts": parts, "copy_gbps": copy_gbps,
                      "adjoint_kernel_ms": adjoint_ms, "algorithmic_bytes": algo_bytes},
         "cpu_baseline": cpu,
         "plain_kernel": plain,

@@ -116,7 +116,9 @@ class PreparedTerms:
 class PreparedTargets:
     """Hash index over a config set on one device (qmb_targets_build)."""
 
-    def __init__(self, configs: torch.Tensor, n_qubytes: int) -> None:
+    def __init__(self, configs: torch.Tensor, n_qubytes: int, distinct: bool | None = None, max_occupied: int | None = None) -> None:
+        """distinct=True asserts the rows are pairwise different and max_occupied bounds the number of
+        occupied sites per row; either left as None is measured on first use (a device sync)."""
         configs = configs.contiguous()
         if configs.dtype != torch.uint8 or configs.dim() != 2 or configs.size(1) != n_qubytes:
             raise RuntimeError("configs must be a uint8 [n, n_qubytes] tensor.")
@@ -133,7 +135,8 @@ class PreparedTargets:
         with torch.cuda.device(device):
             _lib.check(lib.qmb_targets_build(n_qubytes, _ptr(configs), self.n, _ptr(self.buffer), self.buffer.numel(),
                                              ctypes.byref(self.desc), _stream(device)), "qmb_targets_build")
-        self._duplicates: int | None = None
+        self._duplicates: int | None = 0 if distinct else None
+        self._max_occupied = max_occupied
         self._cores: tuple[torch.Tensor, _lib.QmbCores] | None = None
 
     def duplicates(self) -> int:
@@ -148,11 +151,14 @@ class PreparedTargets:
         return configs.data_ptr() == self.configs.data_ptr() and configs.shape == self.configs.shape
 
     def max_occupied(self) -> int:
-        """Largest number of occupied sites in a row (synchronises)."""
-        if self.n == 0:
-            return 0
-        table = torch.tensor([bin(v).count("1") for v in range(256)], dtype=torch.int32, device=self.device)
-        return int(table[self.configs.long()].sum(dim=1).max().item())
+        """Largest number of occupied sites in a row (synchronises unless given at construction)."""
+        if self._max_occupied is None:
+            if self.n == 0:
+                self._max_occupied = 0
+            else:
+                table = torch.tensor([bin(v).count("1") for v in range(256)], dtype=torch.int32, device=self.device)
+                self._max_occupied = int(table[self.configs.long()].sum(dim=1).max().item())
+        return self._max_occupied
 
     def cores(self) -> _lib.QmbCores:
         """Core index of the set for the pair-excitation path (built on first use)."""
