@@ -221,28 +221,51 @@ int qmb_pairs_workspace_size(int n_qubytes, size_t* bytes);
 int qmb_pairs_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
                     int64_t n_terms, void* buffer, size_t bytes, qmb_pairs* out, void* stream);
 
-/* qmb_terms_build without the pair-excitation terms (same layout and buffer size). */
-int qmb_terms_build_residual(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
-                             int64_t n_terms, void* buffer, size_t bytes, qmb_terms* out, void* stream);
+/* Merged coefficients of the single excitations (annihilate r, create p, conditioned on at most
+ * one other site k): the element on a source config c holding r and not p is
+ *   (-1)^(popcount(c & (L(r) ^ L(p)))) * (a[r * n_bits + p] + sum_{k in c, k != r} kd[(r * n_bits + p) * n_bits + k])
+ * with L(s) = the sites below s.  n_qubytes <= 16. */
+typedef struct qmb_singles {
+    int32_t n_qubytes, particle_cut, n_bits, reserved;
+    void* buffer;
+    size_t bytes;
+    double* a;         /* [n_bits * n_bits][2] */
+    double* kd;        /* [n_bits * n_bits * n_bits][2] */
+    int64_t* counts;   /* device: {single-excitation terms, 0, 0, 0} */
+} qmb_singles;
 
-/* Core index of the set indexed by a qmb_targets: one entry per (config, occupied site pair),
- * radix-sorted by the 48-bit hash of the core, and the start of every group of equal hashes.
- * max_occupied must bound the number of occupied sites of every config (a row with more sets
- * counts[2], and qmb_apply_pairs then writes NaN into out[0] instead of a wrong result). */
+int qmb_singles_workspace_size(int n_qubytes, size_t* bytes);
+int qmb_singles_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
+                      int64_t n_terms, void* buffer, size_t bytes, qmb_singles* out, void* stream);
+
+/* qmb_terms_build without some terms (same layout and buffer size).  exclude is a mask:
+ * 1 pair excitations, 2 single excitations, 4 diagonal terms (from the buckets; the diagonal
+ * records are kept). */
+int qmb_terms_build_residual(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
+                             int64_t n_terms, int32_t exclude, void* buffer, size_t bytes, qmb_terms* out, void* stream);
+
+/* Core index of the set indexed by a qmb_targets: one entry per (config, `order` occupied sites)
+ * -- order 2: every occupied pair (cores of pair excitations), order 1: every occupied site
+ * (cores of single excitations) -- radix-sorted by the 48-bit hash of the core, and the start of
+ * every group of equal hashes.  max_occupied must bound the number of occupied sites of every
+ * config (a row with more sets counts[2], and the apply calls then write NaN into out[0] instead
+ * of a wrong result). */
 typedef struct qmb_cores {
-    int32_t n_qubytes, n_words, max_occupied, per_config; /* per_config = C(max_occupied, 2) */
+    int32_t n_qubytes, n_words, max_occupied, per_config; /* per_config = C(max_occupied, order) */
+    int32_t order, reserved;
     int64_t n;
     int64_t capacity;  /* n * per_config entries */
     void* buffer;
     size_t bytes;
     uint64_t* keys;    /* [capacity] sorted core hashes (unused entries last) */
-    uint64_t* vals;    /* [capacity] row << 16 | site1 << 8 | site0 of the removed pair */
+    uint64_t* vals;    /* [capacity] row << 16 | site1 << 8 | site0 of the removed sites (order 1: site0) */
     uint32_t* starts;  /* [capacity + 1] first entry of each group, then the end */
     int64_t* counts;   /* device: {entries, groups, overflow, 0} */
 } qmb_cores;
 
-int qmb_cores_workspace_size(int n_qubytes, int64_t n, int32_t max_occupied, size_t* bytes);
-int qmb_cores_build(const qmb_targets* targets, int32_t max_occupied, void* buffer, size_t bytes, qmb_cores* out, void* stream);
+int qmb_cores_workspace_size(int n_qubytes, int64_t n, int32_t max_occupied, int32_t order, size_t* bytes);
+int qmb_cores_build(const qmb_targets* targets, int32_t max_occupied, int32_t order, void* buffer, size_t bytes, qmb_cores* out,
+                    void* stream);
 
 /* Adds the pair-excitation part of mode 0 (forward, out = A in) or mode 1 (adjoint,
  * out = A^H in) on the set of `cores`/`targets` into out (float64 [n][2]).  A gather over the
@@ -250,9 +273,21 @@ int qmb_cores_build(const qmb_targets* targets, int32_t max_occupied, void* buff
 int qmb_apply_pairs(const qmb_pairs* pairs, const qmb_cores* cores, const qmb_targets* targets, const double* in, double* out,
                     int mode, void* stream);
 
+/* The single-excitation part (order-1 cores), as qmb_apply_pairs. */
+int qmb_apply_singles(const qmb_singles* singles, const qmb_cores* cores, const qmb_targets* targets, const double* in,
+                      double* out, int mode, void* stream);
+
+/* The diagonal part on the set indexed by `targets`: out[i] += d_i in[i] (mode 0) or
+ * conj(d_i) in[i] (mode 1), d_i = sum of the diagonal terms on row i in term order. */
+int qmb_apply_diagonal(const qmb_terms* terms, const qmb_targets* targets, const double* in, double* out, int mode, void* stream);
+
 /* Host-only pair-excitation test of a compiled record (for unit tests): returns 1 and writes
  * {r0, r1, c0, c1, delta} when the record is a pair excitation, else 0. */
 int qmb_pair_excitation_host(uint64_t rec, int particle_cut, int32_t out[5]);
+
+/* Host-only single-excitation test of a compiled record: returns 1 and writes {r, p, k, kocc, delta}
+ * (k = -1 without a condition) when the record is a single excitation, else 0. */
+int qmb_single_excitation_host(uint64_t rec, int particle_cut, int32_t out[5]);
 
 #ifdef __cplusplus
 }

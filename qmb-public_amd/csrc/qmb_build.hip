@@ -47,8 +47,11 @@ __global__ void compile_terms_kernel(const int16_t* __restrict__ site, const uin
     CompiledTerm c = compile_term(site + 4 * t, kind + 4 * t, particle_cut, n_bits);
     vals[t] = static_cast<uint32_t>(t);
     PairExcitation pe;
-    if (c.status == kCompileOk && exclude_pairs && pair_excitation(c.rec, particle_cut, &pe)) {
-        // carried by the pair-excitation table instead
+    SingleExcitation se;
+    if (c.status == kCompileOk && (((exclude_pairs & kExcludePairs) && pair_excitation(c.rec, particle_cut, &pe)) ||
+                                   ((exclude_pairs & kExcludeSingles) && single_excitation(c.rec, particle_cut, &se)) ||
+                                   ((exclude_pairs & kExcludeDiagonal) && c.diagonal))) {
+        // carried by the pair / single excitation tables or the diagonal kernel instead
         keys[t] = static_cast<uint32_t>(n_buckets);
         rec_tmp[t] = kNeverSucceeds;
         dflag[t] = 0u;

@@ -328,8 +328,18 @@ int qmb_pairs_build(int n_qubytes, int particle_cut, const int16_t* site, const 
     return pairs_build(n_qubytes, particle_cut, site, kind, coef, n_terms, buffer, bytes, out, as_stream(stream));
 }
 
-int qmb_terms_build_residual(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
-                             int64_t n_terms, void* buffer, size_t bytes, qmb_terms* out, void* stream) {
+int qmb_singles_workspace_size(int n_qubytes, size_t* bytes) {
+    clear_error();
+    if (!bytes) {
+        return set_error(QMB_ERR_ARGUMENT, "bytes must be non-null");
+    }
+    QMB_TRY(check_qubytes(n_qubytes));
+    *bytes = singles_bytes(n_qubytes);
+    return *bytes ? QMB_OK : QMB_ERR_UNSUPPORTED;
+}
+
+int qmb_singles_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
+                      int64_t n_terms, void* buffer, size_t bytes, qmb_singles* out, void* stream) {
     clear_error();
     QMB_TRY(check_qubytes(n_qubytes));
     QMB_TRY(check_particle_cut(particle_cut));
@@ -339,24 +349,42 @@ int qmb_terms_build_residual(int n_qubytes, int particle_cut, const int16_t* sit
     if (!out || !buffer) {
         return set_error(QMB_ERR_ARGUMENT, "buffer and out must be non-null");
     }
-    return terms_build(n_qubytes, particle_cut, site, kind, coef, n_terms, buffer, bytes, out, as_stream(stream), 1);
+    return singles_build(n_qubytes, particle_cut, site, kind, coef, n_terms, buffer, bytes, out, as_stream(stream));
 }
 
-int qmb_cores_workspace_size(int n_qubytes, int64_t n, int32_t max_occupied, size_t* bytes) {
+int qmb_terms_build_residual(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
+                             int64_t n_terms, int32_t exclude, void* buffer, size_t bytes, qmb_terms* out, void* stream) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    QMB_TRY(check_particle_cut(particle_cut));
+    QMB_TRY(check_ptr(site, n_terms, "site"));
+    QMB_TRY(check_ptr(kind, n_terms, "kind"));
+    QMB_TRY(check_ptr(coef, n_terms, "coef"));
+    if (!out || !buffer) {
+        return set_error(QMB_ERR_ARGUMENT, "buffer and out must be non-null");
+    }
+    if (exclude < 0 || exclude > (kExcludePairs | kExcludeSingles | kExcludeDiagonal)) {
+        return set_error(QMB_ERR_ARGUMENT, "exclude must be a mask of 1, 2 and 4, got %d", exclude);
+    }
+    return terms_build(n_qubytes, particle_cut, site, kind, coef, n_terms, buffer, bytes, out, as_stream(stream), exclude);
+}
+
+int qmb_cores_workspace_size(int n_qubytes, int64_t n, int32_t max_occupied, int32_t order, size_t* bytes) {
     clear_error();
     if (!bytes) {
         return set_error(QMB_ERR_ARGUMENT, "bytes must be non-null");
     }
-    *bytes = cores_bytes(n_qubytes, n, max_occupied);
+    *bytes = cores_bytes(n_qubytes, n, max_occupied, order);
     return *bytes ? QMB_OK : QMB_ERR_ARGUMENT;
 }
 
-int qmb_cores_build(const qmb_targets* targets, int32_t max_occupied, void* buffer, size_t bytes, qmb_cores* out, void* stream) {
+int qmb_cores_build(const qmb_targets* targets, int32_t max_occupied, int32_t order, void* buffer, size_t bytes, qmb_cores* out,
+                    void* stream) {
     clear_error();
     if (!out || !buffer) {
         return set_error(QMB_ERR_ARGUMENT, "buffer and out must be non-null");
     }
-    return cores_build(targets, max_occupied, buffer, bytes, out, as_stream(stream));
+    return cores_build(targets, max_occupied, order, buffer, bytes, out, as_stream(stream));
 }
 
 int qmb_apply_pairs(const qmb_pairs* pairs, const qmb_cores* cores, const qmb_targets* targets, const double* in, double* out,
@@ -366,6 +394,39 @@ int qmb_apply_pairs(const qmb_pairs* pairs, const qmb_cores* cores, const qmb_ta
         return set_error(QMB_ERR_ARGUMENT, "in and out must be non-null");
     }
     return apply_pairs(pairs, cores, targets, in, out, mode, as_stream(stream));
+}
+
+int qmb_apply_singles(const qmb_singles* singles, const qmb_cores* cores, const qmb_targets* targets, const double* in,
+                      double* out, int mode, void* stream) {
+    clear_error();
+    if (cores && cores->n > 0 && (!in || !out)) {
+        return set_error(QMB_ERR_ARGUMENT, "in and out must be non-null");
+    }
+    return apply_singles(singles, cores, targets, in, out, mode, as_stream(stream));
+}
+
+int qmb_apply_diagonal(const qmb_terms* terms, const qmb_targets* targets, const double* in, double* out, int mode, void* stream) {
+    clear_error();
+    if (targets && targets->n > 0 && (!in || !out)) {
+        return set_error(QMB_ERR_ARGUMENT, "in and out must be non-null");
+    }
+    return apply_diagonal(terms, targets, in, out, mode, as_stream(stream));
+}
+
+int qmb_single_excitation_host(uint64_t rec, int particle_cut, int32_t out[5]) {
+    clear_error();
+    SingleExcitation e;
+    if (!single_excitation(rec, particle_cut, &e)) {
+        return 0;
+    }
+    if (out) {
+        out[0] = e.r;
+        out[1] = e.p;
+        out[2] = e.k;
+        out[3] = e.kocc;
+        out[4] = e.delta;
+    }
+    return 1;
 }
 
 int qmb_pair_excitation_host(uint64_t rec, int particle_cut, int32_t out[5]) {

@@ -256,6 +256,65 @@ __host__ __device__ inline bool pair_excitation(uint64_t rec, int particle_cut, 
     return true;
 }
 
+// Single excitations: terms that annihilate one site r and create one site p, conditioned on at
+// most one other site k (occupied or empty) -- the one-body terms and the number-dressed hops
+// a^+_p n_k a_r of a molecular Hamiltonian.  Their parity is  single_parity(c) ^ delta  with
+//     single_parity(c) = popcount(c & (L(r) ^ L(p))) mod 2,
+// so per (r, p) they merge into a constant part A (valid with k empty) and a per-k change K:
+// the matrix element on a source config c is (-1)^single_parity(c) (A + sum_{k in c} K[k]).
+struct SingleExcitation {
+    int r, p;   // annihilated, created
+    int k;      // conditioned site, or -1
+    int kocc;   // 1: k must be occupied, 0: empty
+    int delta;  // parity offset of this operator order
+};
+
+__host__ __device__ inline bool single_excitation(uint64_t rec, int particle_cut, SingleExcitation* out) {
+    if ((rec >> 63) & 1ull) {
+        return false;
+    }
+    const uint32_t valid = static_cast<uint32_t>(rec >> 32) & 0xf;
+    if (valid != 0x3u && valid != 0x7u) {
+        return false;
+    }
+    const uint32_t req = static_cast<uint32_t>(rec >> 36) & 0xf;
+    const uint32_t fin = static_cast<uint32_t>(rec >> 40) & 0xf;
+    SingleExcitation e{-1, -1, -1, 0, 0};
+    for (int u = 0; u < 4; ++u) {
+        if (!((valid >> u) & 1u)) {
+            continue;
+        }
+        const int s = static_cast<int>((rec >> (8 * u)) & 0xff);
+        const uint32_t r = (req >> u) & 1u, f = (fin >> u) & 1u;
+        if (r == 1u && f == 0u) {
+            if (e.r >= 0) {
+                return false;
+            }
+            e.r = s;
+        } else if (r == 0u && f == 1u) {
+            if (e.p >= 0) {
+                return false;
+            }
+            e.p = s;
+        } else {
+            if (e.k >= 0) {
+                return false;
+            }
+            e.k = s;
+            e.kocc = static_cast<int>(r);
+        }
+    }
+    if (e.r < 0 || e.p < 0) {
+        return false;
+    }
+    // single_parity of the required pattern v = {r} (+ k when it must be occupied)
+    const int canon = (e.r < e.p ? 1 : 0) ^ ((e.k >= 0 && e.kocc && ((e.k < e.r) != (e.k < e.p))) ? 1 : 0);
+    const int pconst = static_cast<int>((rec >> 48) & 1);
+    e.delta = particle_cut == 1 ? (pconst ^ canon) : 0;
+    *out = e;
+    return true;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Config words.
 // ---------------------------------------------------------------------------------------------

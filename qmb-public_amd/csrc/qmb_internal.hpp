@@ -97,7 +97,10 @@ size_t direct_lds_bytes(int n_words);
 // Builders (qmb_build.hip).
 // ---------------------------------------------------------------------------------------------
 int terms_layout(Carver& cv, int n_qubytes, int64_t n_terms, qmb_terms* out, void** scratch, size_t* scratch_bytes);
-// exclude_pairs: leave out the pair-excitation terms (they go to a qmb_pairs table instead).
+// exclude_pairs: terms to leave out (they go to the structured path instead), a mask of:
+constexpr int kExcludePairs = 1;     // pair excitations (qmb_pairs)
+constexpr int kExcludeSingles = 2;   // single excitations (qmb_singles)
+constexpr int kExcludeDiagonal = 4;  // diagonal terms (applied from the diagonal records)
 int terms_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
                 int64_t n_terms, void* buffer, size_t bytes, qmb_terms* out, hipStream_t stream, int exclude_pairs = 0);
 int targets_layout(Carver& cv, int n_qubytes, int64_t n, qmb_targets* out);
@@ -140,10 +143,18 @@ size_t single_relative_scratch_bytes(int64_t n_terms);
 size_t pairs_bytes(int n_qubytes);
 int pairs_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
                 void* buffer, size_t bytes, qmb_pairs* out, hipStream_t stream);
-size_t cores_bytes(int n_qubytes, int64_t n, int max_occupied);
-int cores_build(const qmb_targets* targets, int max_occupied, void* buffer, size_t bytes, qmb_cores* out, hipStream_t stream);
+size_t singles_bytes(int n_qubytes);
+int singles_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
+                  void* buffer, size_t bytes, qmb_singles* out, hipStream_t stream);
+size_t cores_bytes(int n_qubytes, int64_t n, int max_occupied, int order);
+int cores_build(const qmb_targets* targets, int max_occupied, int order, void* buffer, size_t bytes, qmb_cores* out,
+                hipStream_t stream);
 int apply_pairs(const qmb_pairs* pairs, const qmb_cores* cores, const qmb_targets* targets, const double* in, double* out,
                 int mode, hipStream_t stream);
+int apply_singles(const qmb_singles* singles, const qmb_cores* cores, const qmb_targets* targets, const double* in, double* out,
+                  int mode, hipStream_t stream);
+int apply_diagonal(const qmb_terms* terms, const qmb_targets* targets, const double* in, double* out, int mode,
+                   hipStream_t stream);
 
 size_t find_relative_scratch_bytes(int n_qubytes, int64_t count_selected);
 int find_relative_prepared(const qmb_terms* terms, const qmb_targets* exclude, const uint8_t* configs_i, const double* psi_i,

@@ -101,8 +101,9 @@ __device__ __forceinline__ uint64_t core_key(uint64_t h) {
     return k == kKeySentinel ? k - 1 : k;  // a merged hash value only makes its group "mixed"
 }
 
-// One thread per config: every occupied pair (x < y) gives the entry (hash of the core, row).
-template <int W>
+// One thread per config: every occupied pair (x < y) -- ORDER 2 -- or every occupied site x --
+// ORDER 1 -- gives the entry (hash of the config without those sites, row).
+template <int W, int ORDER>
 __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64_t* __restrict__ cols, int64_t n, int per_config,
                                  uint64_t* __restrict__ okeys, uint64_t* __restrict__ ovals, int64_t* __restrict__ counts) {
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
@@ -120,6 +121,14 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
             const int x = 64 * wx + __builtin_ctzll(mx);
             mx &= mx - 1;
             const uint64_t hx = h ^ cols[x];
+            if (ORDER == 1) {
+                if (k < per_config) {
+                    okeys[base + k] = core_key(hx);
+                    ovals[base + k] = (static_cast<uint64_t>(i) << 16) | static_cast<uint64_t>(x);
+                }
+                ++k;
+                continue;
+            }
 #pragma unroll
             for (int wy = 0; wy < W; ++wy) {
                 if (wy < wx) {
@@ -180,10 +189,14 @@ __global__ void core_start_kernel(const uint64_t* __restrict__ keys, const uint3
     }
 }
 
-int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, qmb_cores* out, void** scratch, size_t* scratch_bytes) {
+int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, int order, qmb_cores* out, void** scratch,
+                 size_t* scratch_bytes) {
     QMB_TRY(check_qubytes(n_qubytes));
     if (n < 0) {
         return set_error(QMB_ERR_ARGUMENT, "n must be >= 0");
+    }
+    if (order != 1 && order != 2) {
+        return set_error(QMB_ERR_ARGUMENT, "order must be 1 or 2, got %d", order);
     }
     if (max_occupied < 0 || max_occupied > 8 * n_qubytes) {
         return set_error(QMB_ERR_ARGUMENT, "max_occupied must be in [0, %d], got %d", 8 * n_qubytes, max_occupied);
@@ -192,7 +205,8 @@ int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, qmb_cor
     c.n_qubytes = n_qubytes;
     c.n_words = n_words_of(n_qubytes);
     c.max_occupied = max_occupied;
-    c.per_config = max_occupied * (max_occupied - 1) / 2;
+    c.order = order;
+    c.per_config = order == 2 ? max_occupied * (max_occupied - 1) / 2 : max_occupied;
     c.n = n;
     c.capacity = n * c.per_config;
     if (c.capacity >= (1ll << 32) - 1) {
@@ -397,6 +411,259 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(PairParams p) {
     }
 }
 
+// ------------------------------------------------------------------------- single excitations --
+__global__ void singles_fill_kernel(const int16_t* __restrict__ site, const uint8_t* __restrict__ kind,
+                                    const double2* __restrict__ coef, int64_t n_terms, int particle_cut, int n_bits,
+                                    double2* __restrict__ a, double2* __restrict__ kd, int64_t* __restrict__ counts) {
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= n_terms) {
+        return;
+    }
+    const CompiledTerm c = compile_term(site + 4 * t, kind + 4 * t, particle_cut, n_bits);
+    SingleExcitation e;
+    if (c.status != kCompileOk || !single_excitation(c.rec, particle_cut, &e)) {
+        return;
+    }
+    const double2 v0 = coef[t];
+    const double2 v = e.delta ? make_double2(-v0.x, -v0.y) : v0;
+    const int64_t rp = static_cast<int64_t>(e.r) * n_bits + e.p;
+    if (e.k < 0 || !e.kocc) {
+        // holds whenever k is empty
+        unsafeAtomicAdd(&a[rp].x, v.x);
+        unsafeAtomicAdd(&a[rp].y, v.y);
+    }
+    if (e.k >= 0) {
+        double2* dk = kd + rp * n_bits + e.k;
+        unsafeAtomicAdd(&dk->x, e.kocc ? v.x : -v.x);
+        unsafeAtomicAdd(&dk->y, e.kocc ? v.y : -v.y);
+    }
+    atomicAdd(reinterpret_cast<unsigned long long*>(&counts[0]), 1ull);
+}
+
+int singles_layout(Carver& cv, int n_qubytes, qmb_singles* out) {
+    QMB_TRY(check_qubytes(n_qubytes));
+    if (n_qubytes > kMaxPairBytes) {
+        return set_error(QMB_ERR_UNSUPPORTED, "the single-excitation table supports n_qubytes <= %d, got %d", kMaxPairBytes,
+                         n_qubytes);
+    }
+    qmb_singles g{};
+    g.n_qubytes = n_qubytes;
+    g.n_bits = 8 * n_qubytes;
+    const size_t nb = static_cast<size_t>(g.n_bits);
+    g.buffer = cv.base;
+    g.counts = cv.take<int64_t>(4);
+    g.a = cv.take<double>(2 * nb * nb);
+    g.kd = cv.take<double>(2 * nb * nb * nb);
+    g.bytes = cv.off;
+    if (out) {
+        *out = g;
+    }
+    return QMB_OK;
+}
+
+struct SingleParams {
+    const double2* a;
+    const double2* kd;
+    int n_bits;
+    int particle_cut;
+    const uint64_t* ckeys;
+    const uint64_t* vals;
+    const uint32_t* starts;
+    const int64_t* counts;
+    const double2* in;
+    double2* out;
+};
+
+template <int W>
+__device__ __forceinline__ int single_parity(const Cfg<W>& c, int r, int p) {
+    int par = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        par ^= __popcll(c.w[w] & (low_mask_word(r, w) ^ low_mask_word(p, w))) & 1;
+    }
+    return par;
+}
+
+// Element of the single excitation (annihilate r, create p) on a source config whose other
+// occupied sites are `core`.
+template <int W>
+__device__ __forceinline__ double2 single_element(const SingleParams& p, const Cfg<W>& core, int r, int q) {
+    const int64_t rq = static_cast<int64_t>(r) * p.n_bits + q;
+    double2 x = p.a[rq];
+    const double2* row = p.kd + rq * p.n_bits;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        uint64_t m = core.w[w];
+        while (m) {
+            const int k = 64 * w + __builtin_ctzll(m);
+            m &= m - 1;
+            const double2 d = row[k];
+            x.x += d.x;
+            x.y += d.y;
+        }
+    }
+    return x;
+}
+
+// One wave, one group of order-1 entries [s, e): rows a (config = core + r) against entries b
+// (config = core + q), q != r:
+//   ADJ:  out[a] += conj(M_ba) in[b],  M_ba = (-1)^single_parity(c_a; r, q) (A[r][q] + sum_{k in core} K[r][q][k])
+//   !ADJ: out[a] += M_ab in[b],        M_ab = (-1)^single_parity(c_b; q, r) (A[q][r] + sum_{k in core} K[q][r][k])
+template <int W, bool ADJ>
+__device__ void single_group(const SingleParams& p, uint32_t s, uint32_t e, int lane) {
+    const uint64_t v0 = p.vals[s];
+    Cfg<W> core0 = load_key<W>(p.ckeys, static_cast<int64_t>(v0 >> 16));
+    flip_site<W>(core0, static_cast<int>(v0 & 0xff));
+    const bool fermi = p.particle_cut == 1;
+    for (uint32_t a0 = s; a0 < e; a0 += 64) {
+        const uint32_t a = a0 + lane;
+        const bool act = a < e;
+        const uint64_t va = act ? p.vals[a] : v0;
+        const int64_t ia = static_cast<int64_t>(va >> 16);
+        const int r = static_cast<int>(va & 0xff);
+        const Cfg<W> ca = load_key<W>(p.ckeys, ia);
+        Cfg<W> core = ca;
+        flip_site<W>(core, r);
+        bool same = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            same = same && core.w[w] == core0.w[w];
+        }
+        const bool mixed = __ballot(act && !same) != 0ull;
+        double acc_re = 0.0, acc_im = 0.0;
+        for (uint32_t b0 = s; b0 < e; b0 += 64) {
+            const uint32_t bl = b0 + lane;
+            const bool bin = bl < e;
+            const uint64_t vbl = bin ? p.vals[bl] : 0ull;
+            const double2 gbl = bin ? p.in[vbl >> 16] : make_double2(0.0, 0.0);
+            const int nb = static_cast<int>(e - b0 < 64u ? e - b0 : 64u);
+            for (int u = 0; u < nb; ++u) {
+                const uint64_t vb = shfl_u64(vbl, u);
+                const double2 g = shfl_d2(gbl, u);
+                const int q = static_cast<int>(vb & 0xff);
+                bool ok = act && (b0 + static_cast<uint32_t>(u) != a) && q != r;
+                if (mixed && ok) {
+                    Cfg<W> cb = load_key<W>(p.ckeys, static_cast<int64_t>(vb >> 16));
+                    flip_site<W>(cb, q);
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        ok = ok && cb.w[w] == core.w[w];
+                    }
+                }
+                if (!ok) {
+                    continue;
+                }
+                double2 x;
+                int sg;
+                if (ADJ) {
+                    x = single_element<W>(p, core, r, q);
+                    sg = fermi ? single_parity<W>(ca, r, q) : 0;
+                } else {
+                    Cfg<W> cb = core;
+                    flip_site<W>(cb, q);
+                    x = single_element<W>(p, core, q, r);
+                    sg = fermi ? single_parity<W>(cb, q, r) : 0;
+                }
+                const double xr = sg ? -x.x : x.x;
+                const double xi = sg ? -x.y : x.y;
+                if (ADJ) {
+                    acc_re += xr * g.x + xi * g.y;
+                    acc_im += xr * g.y - xi * g.x;
+                } else {
+                    acc_re += xr * g.x - xi * g.y;
+                    acc_im += xr * g.y + xi * g.x;
+                }
+            }
+        }
+        if (act && (acc_re != 0.0 || acc_im != 0.0)) {
+            unsafeAtomicAdd(&p.out[ia].x, acc_re);
+            unsafeAtomicAdd(&p.out[ia].y, acc_im);
+        }
+    }
+}
+
+template <int W, bool ADJ>
+__global__ __launch_bounds__(kBlock) void single_kernel(SingleParams p) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6;
+    const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * kBlock) >> 6;
+    if (p.counts[2] != 0) {
+        if (wave == 0 && lane == 0) {
+            p.out[0] = make_double2(__longlong_as_double(0x7ff8000000000000ll), __longlong_as_double(0x7ff8000000000000ll));
+        }
+        return;
+    }
+    const int64_t groups = p.counts[1];
+    for (int64_t g0 = wave * 64; g0 < groups; g0 += nwaves * 64) {
+        const int64_t gg = g0 + lane;
+        uint32_t gs = 0, ge = 0;
+        if (gg < groups) {
+            gs = p.starts[gg];
+            ge = p.starts[gg + 1];
+        }
+        uint64_t multi = __ballot(gg < groups && ge - gs >= 2u);
+        while (multi) {
+            const int l = __builtin_ctzll(multi);
+            multi &= multi - 1;
+            const uint32_t s = __shfl(gs, l);
+            const uint32_t e = __shfl(ge, l);
+            single_group<W, ADJ>(p, s, e, lane);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------- diagonal --
+constexpr int kDiagTile = 1024;
+
+// out[i] += d_i in[i] (conj(d_i) for ADJ), d_i = the diagonal terms on row i summed in term order.
+template <int W, bool ADJ>
+__global__ __launch_bounds__(kBlock) void diag_apply_kernel(const uint64_t* __restrict__ ckeys, int64_t n,
+                                                            const uint64_t* __restrict__ drecs, const double2* __restrict__ dcoefs,
+                                                            const int64_t* __restrict__ counts, const double2* __restrict__ in,
+                                                            double2* __restrict__ out) {
+    __shared__ uint64_t s_rec[kDiagTile];
+    __shared__ double2 s_coef[kDiagTile];
+    const int64_t i = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x;
+    const bool valid = i < n;
+    Cfg<W> c;
+    if (valid) {
+        c = load_key<W>(ckeys, i);
+    } else {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            c.w[w] = 0;
+        }
+    }
+    const int64_t td = counts[1];
+    double re = 0.0, im = 0.0;
+    for (int64_t base = 0; base < td; base += kDiagTile) {
+        const int cnt = static_cast<int>(td - base < kDiagTile ? td - base : kDiagTile);
+        __syncthreads();
+        for (int u = threadIdx.x; u < cnt; u += kBlock) {
+            s_rec[u] = drecs[base + u];
+            s_coef[u] = dcoefs[base + u];
+        }
+        __syncthreads();
+        for (int u = 0; u < cnt; ++u) {
+            Cfg<W> o;
+            int par;
+            if (apply_rec<W>(s_rec[u], c, o, par)) {
+                const double2 cf = s_coef[u];
+                re += par ? -cf.x : cf.x;
+                im += par ? -cf.y : cf.y;
+            }
+        }
+    }
+    if (valid && (re != 0.0 || im != 0.0)) {
+        const double2 g = in[i];
+        if (ADJ) {
+            im = -im;
+        }
+        out[i].x += re * g.x - im * g.y;
+        out[i].y += re * g.y + im * g.x;
+    }
+}
+
 }  // namespace
 
 size_t pairs_bytes(int n_qubytes) {
@@ -430,12 +697,13 @@ int pairs_build(int n_qubytes, int particle_cut, const int16_t* site, const uint
     return QMB_OK;
 }
 
-size_t cores_bytes(int n_qubytes, int64_t n, int max_occupied) {
+size_t cores_bytes(int n_qubytes, int64_t n, int max_occupied, int order) {
     Carver cv{nullptr, 0};
-    return cores_layout(cv, n_qubytes, n, max_occupied, nullptr, nullptr, nullptr) == QMB_OK ? cv.off : 0;
+    return cores_layout(cv, n_qubytes, n, max_occupied, order, nullptr, nullptr, nullptr) == QMB_OK ? cv.off : 0;
 }
 
-int cores_build(const qmb_targets* targets, int max_occupied, void* buffer, size_t bytes, qmb_cores* out, hipStream_t stream) {
+int cores_build(const qmb_targets* targets, int max_occupied, int order, void* buffer, size_t bytes, qmb_cores* out,
+                hipStream_t stream) {
     if (!targets) {
         return set_error(QMB_ERR_ARGUMENT, "targets must be non-null");
     }
@@ -443,7 +711,7 @@ int cores_build(const qmb_targets* targets, int max_occupied, void* buffer, size
     qmb_cores c{};
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
-    QMB_TRY(cores_layout(cv, targets->n_qubytes, targets->n, max_occupied, &c, &scratch, &scratch_bytes));
+    QMB_TRY(cores_layout(cv, targets->n_qubytes, targets->n, max_occupied, order, &c, &scratch, &scratch_bytes));
     if (cv.off > bytes) {
         return set_error(QMB_ERR_WORKSPACE, "core index buffer too small: need %zu bytes, have %zu", cv.off, bytes);
     }
@@ -460,11 +728,18 @@ int cores_build(const qmb_targets* targets, int max_occupied, void* buffer, size
         size_t tmp_bytes = scratch_bytes > sc.off + 512 ? scratch_bytes - sc.off - 512 : 0;
         void* tmp = sc.take_bytes(tmp_bytes);
         const unsigned nb = blocks_for(c.n);
-        switch (c.n_words) {
-            case 1: core_emit_kernel<1><<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, c.per_config, keys_in, vals_in, c.counts); break;
-            case 2: core_emit_kernel<2><<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, c.per_config, keys_in, vals_in, c.counts); break;
-            case 3: core_emit_kernel<3><<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, c.per_config, keys_in, vals_in, c.counts); break;
-            default: core_emit_kernel<4><<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, c.per_config, keys_in, vals_in, c.counts); break;
+        auto emit = [&](auto kernel) {
+            kernel<<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, c.per_config, keys_in, vals_in, c.counts);
+        };
+        switch (c.n_words * 2 + (order - 1)) {
+            case 2: emit(&core_emit_kernel<1, 1>); break;
+            case 3: emit(&core_emit_kernel<1, 2>); break;
+            case 4: emit(&core_emit_kernel<2, 1>); break;
+            case 5: emit(&core_emit_kernel<2, 2>); break;
+            case 6: emit(&core_emit_kernel<3, 1>); break;
+            case 7: emit(&core_emit_kernel<3, 2>); break;
+            case 8: emit(&core_emit_kernel<4, 1>); break;
+            default: emit(&core_emit_kernel<4, 2>); break;
         }
         QMB_CHECK_LAUNCH();
         size_t need = tmp_bytes;
@@ -491,6 +766,9 @@ int apply_pairs(const qmb_pairs* pairs, const qmb_cores* cores, const qmb_target
     }
     if (mode != 0 && mode != 1) {
         return set_error(QMB_ERR_ARGUMENT, "mode must be 0 (forward) or 1 (adjoint)");
+    }
+    if (cores->order != 2) {
+        return set_error(QMB_ERR_ARGUMENT, "the pair path needs a core index of order 2");
     }
     if (cores->capacity == 0) {
         return QMB_OK;
@@ -520,6 +798,115 @@ int apply_pairs(const qmb_pairs* pairs, const qmb_cores* cores, const qmb_target
         return mode == 1 ? go(&pair_kernel<2, true>) : go(&pair_kernel<2, false>);
     }
     return set_error(QMB_ERR_UNSUPPORTED, "pair path supports n_qubytes <= %d", kMaxPairBytes);
+}
+
+size_t singles_bytes(int n_qubytes) {
+    Carver cv{nullptr, 0};
+    return singles_layout(cv, n_qubytes, nullptr) == QMB_OK ? cv.off : 0;
+}
+
+int singles_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
+                  void* buffer, size_t bytes, qmb_singles* out, hipStream_t stream) {
+    if (particle_cut != 1 && particle_cut != 2) {
+        return set_error(QMB_ERR_UNSUPPORTED, "particle_cut must be 1 or 2, got %d", particle_cut);
+    }
+    Carver cv{static_cast<char*>(buffer), 0};
+    qmb_singles g{};
+    QMB_TRY(singles_layout(cv, n_qubytes, &g));
+    if (cv.off > bytes) {
+        return set_error(QMB_ERR_WORKSPACE, "single-excitation buffer too small: need %zu bytes, have %zu", cv.off, bytes);
+    }
+    g.particle_cut = particle_cut;
+    g.bytes = bytes;
+    const int64_t nb = g.n_bits;
+    zero_counts_kernel<<<1, 64, 0, stream>>>(g.counts);
+    QMB_CHECK_LAUNCH();
+    QMB_TRY(launch_zero_f64(g.a, 2 * nb * nb, stream));
+    QMB_TRY(launch_zero_f64(g.kd, 2 * nb * nb * nb, stream));
+    if (n_terms > 0) {
+        singles_fill_kernel<<<blocks_for(n_terms), kBlock, 0, stream>>>(site, kind, reinterpret_cast<const double2*>(coef), n_terms,
+                                                                        particle_cut, g.n_bits, reinterpret_cast<double2*>(g.a),
+                                                                        reinterpret_cast<double2*>(g.kd), g.counts);
+        QMB_CHECK_LAUNCH();
+    }
+    *out = g;
+    return QMB_OK;
+}
+
+int apply_singles(const qmb_singles* singles, const qmb_cores* cores, const qmb_targets* targets, const double* in, double* out,
+                  int mode, hipStream_t stream) {
+    if (!singles || !cores || !targets) {
+        return set_error(QMB_ERR_ARGUMENT, "singles, cores and targets must be non-null");
+    }
+    if (singles->n_qubytes != cores->n_qubytes || cores->n_qubytes != targets->n_qubytes || cores->n != targets->n) {
+        return set_error(QMB_ERR_ARGUMENT, "singles, cores and targets describe different shapes");
+    }
+    if (cores->order != 1) {
+        return set_error(QMB_ERR_ARGUMENT, "the single-excitation path needs a core index of order 1");
+    }
+    if (mode != 0 && mode != 1) {
+        return set_error(QMB_ERR_ARGUMENT, "mode must be 0 (forward) or 1 (adjoint)");
+    }
+    if (cores->capacity == 0) {
+        return QMB_OK;
+    }
+    SingleParams p{};
+    p.a = reinterpret_cast<const double2*>(singles->a);
+    p.kd = reinterpret_cast<const double2*>(singles->kd);
+    p.n_bits = singles->n_bits;
+    p.particle_cut = singles->particle_cut;
+    p.ckeys = targets->keys;
+    p.vals = cores->vals;
+    p.starts = cores->starts;
+    p.counts = cores->counts;
+    p.in = reinterpret_cast<const double2*>(in);
+    p.out = reinterpret_cast<double2*>(out);
+    const int64_t want = (cores->capacity + 64 * 4 - 1) / (64 * 4);
+    const unsigned grid = static_cast<unsigned>(want < 4096 ? (want < 1 ? 1 : want) : 4096);
+    auto go = [&](auto kernel) -> int {
+        kernel<<<grid, kBlock, 0, stream>>>(p);
+        QMB_CHECK_LAUNCH();
+        return QMB_OK;
+    };
+    if (cores->n_words == 1) {
+        return mode == 1 ? go(&single_kernel<1, true>) : go(&single_kernel<1, false>);
+    }
+    if (cores->n_words == 2) {
+        return mode == 1 ? go(&single_kernel<2, true>) : go(&single_kernel<2, false>);
+    }
+    return set_error(QMB_ERR_UNSUPPORTED, "single-excitation path supports n_qubytes <= %d", kMaxPairBytes);
+}
+
+int apply_diagonal(const qmb_terms* terms, const qmb_targets* targets, const double* in, double* out, int mode,
+                   hipStream_t stream) {
+    if (!terms || !targets) {
+        return set_error(QMB_ERR_ARGUMENT, "terms and targets must be non-null");
+    }
+    if (terms->n_qubytes != targets->n_qubytes) {
+        return set_error(QMB_ERR_ARGUMENT, "terms built for %d bytes, targets for %d", terms->n_qubytes, targets->n_qubytes);
+    }
+    if (mode != 0 && mode != 1) {
+        return set_error(QMB_ERR_ARGUMENT, "mode must be 0 (forward) or 1 (adjoint)");
+    }
+    if (targets->n == 0 || terms->n_terms == 0) {
+        return QMB_OK;
+    }
+    const unsigned grid = blocks_for(targets->n);
+    const auto* dc = reinterpret_cast<const double2*>(terms->dcoefs);
+    const auto* vi = reinterpret_cast<const double2*>(in);
+    auto* vo = reinterpret_cast<double2*>(out);
+    switch (targets->n_words * 2 + mode) {
+        case 2: diag_apply_kernel<1, false><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
+        case 3: diag_apply_kernel<1, true><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
+        case 4: diag_apply_kernel<2, false><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
+        case 5: diag_apply_kernel<2, true><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
+        case 6: diag_apply_kernel<3, false><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
+        case 7: diag_apply_kernel<3, true><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
+        case 8: diag_apply_kernel<4, false><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
+        default: diag_apply_kernel<4, true><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
+    }
+    QMB_CHECK_LAUNCH();
+    return QMB_OK;
 }
 
 }  // namespace qmb

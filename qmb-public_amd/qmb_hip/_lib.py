@@ -82,6 +82,22 @@ class QmbPairs(ctypes.Structure):
     ]
 
 
+class QmbSingles(ctypes.Structure):
+    """Mirror of ``struct qmb_singles`` (include/qmb_hip.h)."""
+
+    _fields_ = [
+        ("n_qubytes", _c_i32),
+        ("particle_cut", _c_i32),
+        ("n_bits", _c_i32),
+        ("reserved", _c_i32),
+        ("buffer", _vp),
+        ("bytes", _c_size),
+        ("a", _vp),
+        ("kd", _vp),
+        ("counts", _vp),
+    ]
+
+
 class QmbCores(ctypes.Structure):
     """Mirror of ``struct qmb_cores`` (include/qmb_hip.h)."""
 
@@ -90,6 +106,8 @@ class QmbCores(ctypes.Structure):
         ("n_words", _c_i32),
         ("max_occupied", _c_i32),
         ("per_config", _c_i32),
+        ("order", _c_i32),
+        ("reserved", _c_i32),
         ("n", _c_i64),
         ("capacity", _c_i64),
         ("buffer", _vp),
@@ -144,14 +162,22 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     ),
     "qmb_pairs_workspace_size": (_c_int, [_c_int, ctypes.POINTER(_c_size)]),
     "qmb_pairs_build": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _c_i64, _vp, _c_size, ctypes.POINTER(QmbPairs), _vp]),
-    "qmb_terms_build_residual": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _c_i64, _vp, _c_size, ctypes.POINTER(QmbTerms), _vp]),
-    "qmb_cores_workspace_size": (_c_int, [_c_int, _c_i64, _c_i32, ctypes.POINTER(_c_size)]),
-    "qmb_cores_build": (_c_int, [ctypes.POINTER(QmbTargets), _c_i32, _vp, _c_size, ctypes.POINTER(QmbCores), _vp]),
+    "qmb_singles_workspace_size": (_c_int, [_c_int, ctypes.POINTER(_c_size)]),
+    "qmb_singles_build": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _c_i64, _vp, _c_size, ctypes.POINTER(QmbSingles), _vp]),
+    "qmb_terms_build_residual": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _c_i64, _c_i32, _vp, _c_size, ctypes.POINTER(QmbTerms), _vp]),
+    "qmb_cores_workspace_size": (_c_int, [_c_int, _c_i64, _c_i32, _c_i32, ctypes.POINTER(_c_size)]),
+    "qmb_cores_build": (_c_int, [ctypes.POINTER(QmbTargets), _c_i32, _c_i32, _vp, _c_size, ctypes.POINTER(QmbCores), _vp]),
     "qmb_apply_pairs": (
         _c_int,
         [ctypes.POINTER(QmbPairs), ctypes.POINTER(QmbCores), ctypes.POINTER(QmbTargets), _vp, _vp, _c_int, _vp],
     ),
+    "qmb_apply_singles": (
+        _c_int,
+        [ctypes.POINTER(QmbSingles), ctypes.POINTER(QmbCores), ctypes.POINTER(QmbTargets), _vp, _vp, _c_int, _vp],
+    ),
+    "qmb_apply_diagonal": (_c_int, [ctypes.POINTER(QmbTerms), ctypes.POINTER(QmbTargets), _vp, _vp, _c_int, _vp]),
     "qmb_pair_excitation_host": (_c_int, [_c_u64, _c_int, ctypes.POINTER(_c_i32)]),
+    "qmb_single_excitation_host": (_c_int, [_c_u64, _c_int, ctypes.POINTER(_c_i32)]),
 }
 
 

@@ -26,6 +26,16 @@ STRATEGY_DIRECT = 2
 DIRECT_MAX_TERMS = 4096
 # the dense pair-excitation table covers configs of up to this many bytes
 PAIR_MAX_QUBYTES = 16
+# qmb_terms_build_residual exclusion mask
+EXCLUDE_PAIRS = 1
+EXCLUDE_SINGLES = 2
+EXCLUDE_DIAGONAL = 4
+
+
+def _device_count(buffer: torch.Tensor, counts_ptr: int) -> int:
+    """First int64 of a device counter array living inside `buffer` (synchronises)."""
+    offset = int(counts_ptr) - buffer.data_ptr()
+    return int(buffer[offset:offset + 8].view(torch.int64).item())
 
 
 # qmb_launch.symmetric values
@@ -81,10 +91,14 @@ class PreparedTerms:
             _lib.check(lib.qmb_terms_build(n_qubytes, particle_cut, _ptr(site), _ptr(kind), _ptr(coef), self.n_terms, _ptr(self.buffer),
                                            self.buffer.numel(), ctypes.byref(self.desc), _stream(device)), "qmb_terms_build")
         self.hermitian = is_hermitian(site, kind, coef, n_qubytes, particle_cut) if check_hermitian else False
-        # Pair-excitation path: merged coefficient table + the table without those terms.
+        # Structured path (used when the table has pair excitations): merged pair and single
+        # excitation tables, the diagonal records of the full table, and the rest for enumeration.
         self.pairs_desc: _lib.QmbPairs | None = None
+        self.singles_desc: _lib.QmbSingles | None = None
         self.residual_desc: _lib.QmbTerms | None = None
         self.n_pair_terms = 0
+        self.n_single_terms = 0
+        self.n_residual_terms = self.n_terms
         if pairs and n_qubytes <= PAIR_MAX_QUBYTES and self.n_terms > 0:
             _lib.check(lib.qmb_pairs_workspace_size(n_qubytes, ctypes.byref(size)), "qmb_pairs_workspace_size")
             pbuf = _workspace(size.value, device)
@@ -92,19 +106,29 @@ class PreparedTerms:
             with torch.cuda.device(device):
                 _lib.check(lib.qmb_pairs_build(n_qubytes, particle_cut, _ptr(site), _ptr(kind), _ptr(coef), self.n_terms, _ptr(pbuf),
                                                pbuf.numel(), ctypes.byref(pdesc), _stream(device)), "qmb_pairs_build")
-            offset = int(pdesc.counts) - pbuf.data_ptr()
-            self.n_pair_terms = int(pbuf[offset:offset + 8].view(torch.int64).item())
+            self.n_pair_terms = _device_count(pbuf, pdesc.counts)
             if self.n_pair_terms > 0:
+                self.pairs_buffer, self.pairs_desc = pbuf, pdesc
+                _lib.check(lib.qmb_singles_workspace_size(n_qubytes, ctypes.byref(size)), "qmb_singles_workspace_size")
+                sbuf = _workspace(size.value, device)
+                sdesc = _lib.QmbSingles()
+                with torch.cuda.device(device):
+                    _lib.check(lib.qmb_singles_build(n_qubytes, particle_cut, _ptr(site), _ptr(kind), _ptr(coef), self.n_terms,
+                                                     _ptr(sbuf), sbuf.numel(), ctypes.byref(sdesc), _stream(device)), "qmb_singles_build")
+                self.n_single_terms = _device_count(sbuf, sdesc.counts)
+                exclude = EXCLUDE_PAIRS | EXCLUDE_DIAGONAL
+                if self.n_single_terms > 0:
+                    self.singles_buffer, self.singles_desc = sbuf, sdesc
+                    exclude |= EXCLUDE_SINGLES
                 _lib.check(lib.qmb_terms_workspace_size(n_qubytes, self.n_terms, ctypes.byref(size)), "qmb_terms_workspace_size")
                 self.residual_buffer = _workspace(size.value, device)
                 self.residual_desc = _lib.QmbTerms()
                 with torch.cuda.device(device):
                     _lib.check(lib.qmb_terms_build_residual(n_qubytes, particle_cut, _ptr(site), _ptr(kind), _ptr(coef), self.n_terms,
-                                                            _ptr(self.residual_buffer), self.residual_buffer.numel(),
+                                                            exclude, _ptr(self.residual_buffer), self.residual_buffer.numel(),
                                                             ctypes.byref(self.residual_desc), _stream(device)),
                                "qmb_terms_build_residual")
-                self.pairs_buffer = pbuf
-                self.pairs_desc = pdesc
+                self.n_residual_terms = _device_count(self.residual_buffer, self.residual_desc.counts)
 
     def counts(self) -> tuple[int, int, int]:
         """(valid terms, diagonal terms, invalid terms) -- synchronises."""
@@ -137,7 +161,7 @@ class PreparedTargets:
                                              ctypes.byref(self.desc), _stream(device)), "qmb_targets_build")
         self._duplicates: int | None = 0 if distinct else None
         self._max_occupied = max_occupied
-        self._cores: tuple[torch.Tensor, _lib.QmbCores] | None = None
+        self._cores: dict[int, tuple[torch.Tensor, _lib.QmbCores]] = {}
 
     def duplicates(self) -> int:
         """Rows equal to an earlier row (synchronises once)."""
@@ -160,20 +184,21 @@ class PreparedTargets:
                 self._max_occupied = int(table[self.configs.long()].sum(dim=1).max().item())
         return self._max_occupied
 
-    def cores(self) -> _lib.QmbCores:
-        """Core index of the set for the pair-excitation path (built on first use)."""
-        if self._cores is None:
+    def cores(self, order: int = 2) -> _lib.QmbCores:
+        """Core index of the set (order 2: pair-excitation cores, 1: single-excitation cores), built
+        on first use."""
+        if order not in self._cores:
             lib = _lib.library()
             occ = self.max_occupied()
             size = ctypes.c_size_t(0)
-            _lib.check(lib.qmb_cores_workspace_size(self.n_qubytes, self.n, occ, ctypes.byref(size)), "qmb_cores_workspace_size")
+            _lib.check(lib.qmb_cores_workspace_size(self.n_qubytes, self.n, occ, order, ctypes.byref(size)), "qmb_cores_workspace_size")
             buf = _workspace(size.value, self.device)
             desc = _lib.QmbCores()
             with torch.cuda.device(self.device):
-                _lib.check(lib.qmb_cores_build(ctypes.byref(self.desc), occ, _ptr(buf), buf.numel(), ctypes.byref(desc),
+                _lib.check(lib.qmb_cores_build(ctypes.byref(self.desc), occ, order, _ptr(buf), buf.numel(), ctypes.byref(desc),
                                                _stream(self.device)), "qmb_cores_build")
-            self._cores = (buf, desc)
-        return self._cores[1]
+            self._cores[order] = (buf, desc)
+        return self._cores[order][1]
 
 
 def can_use_pairs(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tensor) -> bool:
@@ -207,16 +232,25 @@ def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tenso
         raise RuntimeError("the pair-excitation path needs pair terms and configs_i equal to the indexed, distinct-row set.")
     launch = _launch(tile, groups, strategy, symmetric)
     lib = _lib.library()
-    term_desc = terms.residual_desc if use_pairs else terms.desc
+    stream = _stream(device)
     with torch.cuda.device(device):
-        _lib.check(lib.qmb_apply_prepared(ctypes.byref(term_desc), ctypes.byref(targets.desc), _ptr(configs_i), n_i, _ptr(vec), _ptr(out),
-                                          mode, ctypes.byref(launch) if launch else None, _stream(device)), "qmb_apply_prepared")
-        if use_pairs:
-            cores = targets.cores()
-            # a Hermitian table: forward and adjoint are the same map, computed as the adjoint gather
-            pmode = MODE_ADJOINT if symmetric else mode
-            _lib.check(lib.qmb_apply_pairs(ctypes.byref(terms.pairs_desc), ctypes.byref(cores), ctypes.byref(targets.desc), _ptr(vec),
-                                           _ptr(out), pmode, _stream(device)), "qmb_apply_pairs")
+        if not use_pairs:
+            _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.desc), ctypes.byref(targets.desc), _ptr(configs_i), n_i, _ptr(vec),
+                                              _ptr(out), mode, ctypes.byref(launch) if launch else None, stream), "qmb_apply_prepared")
+            return out
+        # a Hermitian table: forward and adjoint are the same map, computed as the adjoint gather
+        pmode = MODE_ADJOINT if symmetric else mode
+        if terms.n_residual_terms > 0:
+            _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.residual_desc), ctypes.byref(targets.desc), _ptr(configs_i), n_i,
+                                              _ptr(vec), _ptr(out), mode, ctypes.byref(launch) if launch else None, stream),
+                       "qmb_apply_prepared")
+        _lib.check(lib.qmb_apply_diagonal(ctypes.byref(terms.desc), ctypes.byref(targets.desc), _ptr(vec), _ptr(out), pmode, stream),
+                   "qmb_apply_diagonal")
+        if terms.singles_desc is not None:
+            _lib.check(lib.qmb_apply_singles(ctypes.byref(terms.singles_desc), ctypes.byref(targets.cores(1)), ctypes.byref(targets.desc),
+                                             _ptr(vec), _ptr(out), pmode, stream), "qmb_apply_singles")
+        _lib.check(lib.qmb_apply_pairs(ctypes.byref(terms.pairs_desc), ctypes.byref(targets.cores(2)), ctypes.byref(targets.desc),
+                                       _ptr(vec), _ptr(out), pmode, stream), "qmb_apply_pairs")
     return out
 
 

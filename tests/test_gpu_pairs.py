@@ -1,17 +1,21 @@
 """
-GPU checks of the pair-excitation path (merged pair coefficients + core-group gather kernel + the
-residual table on the enumeration kernel) against the oracle and against the enumeration-only path:
+GPU checks of the structured path (merged pair- and single-excitation coefficients applied by the
+core-group gather kernels, the diagonal kernel, and the remaining terms on the enumeration kernel)
+against the oracle and against the enumeration-only path:
 
   - forward through the pair path equals oracle apply_within within 1e-12 relative to each entry's
     sum of |contributions|, for molecular tables (Hermitian, pc = 1), random non-Hermitian tables
     over two config words (pc = 1 and 2) and every symmetric setting;
   - the adjoint through the pair path equals the enumeration-only adjoint (itself checked against
     the dense conjugate transpose elsewhere);
-  - the split is exact: residual terms + merged pair table reproduce the counts of the full table;
+  - the split is exact: every valid term lands in exactly one of the pair table, the single table,
+    the diagonal records or the residual enumeration;
   - the pair path is refused for foreign config sets and chosen automatically otherwise.
 """
 
 from __future__ import annotations
+
+import ctypes
 
 import numpy as np
 import pytest
@@ -19,6 +23,7 @@ import torch
 
 from conftest import sorted_terms
 from oracle import oracle as orc
+from qmb_hip import _lib
 from qmb_hip import autograd as ag
 from qmb_hip import prepared as pp
 from qmb_hip import synthetic as sy
@@ -34,20 +39,31 @@ def _pair_heavy_table(sites: np.ndarray, n_terms: int, seed: int) -> tuple[np.nd
     kind = np.full((n_terms, 4), 2, dtype=np.uint8)
     for t in range(n_terms):
         u = rng.random()
-        if u < 0.6:
+        if u < 0.5:  # pair excitation, any operator order
             site[t] = rng.choice(sites, size=4, replace=False)
             kind[t] = rng.permutation([0, 0, 1, 1])
-        elif u < 0.8:
+        elif u < 0.65:  # hop
             site[t, :2] = rng.choice(sites, size=2, replace=False)
             kind[t, :2] = [1, 0]
-        elif u < 0.9:
+        elif u < 0.75:  # hop dressed by an occupied site: a+_p a+_k a_k a_r
             p, k, r = rng.choice(sites, size=3, replace=False)
             site[t] = [p, k, k, r]
             kind[t] = [1, 1, 0, 0]
-        else:
-            p = rng.choice(sites)
-            site[t, :2] = [p, p]
-            kind[t, :2] = [1, 0]
+        elif u < 0.82:  # hop dressed by an empty site: a+_p a_k a+_k a_r
+            p, k, r = rng.choice(sites, size=3, replace=False)
+            site[t] = [p, k, k, r]
+            kind[t] = [1, 0, 1, 0]
+        elif u < 0.9:  # diagonal: n_p or n_p n_q
+            p, q = rng.choice(sites, size=2, replace=False)
+            if rng.random() < 0.5:
+                site[t, :2] = [p, p]
+                kind[t, :2] = [1, 0]
+            else:
+                site[t] = [p, q, q, p]
+                kind[t] = [1, 1, 0, 0]
+        else:  # neither: a+_p a+_q a_r (changes the particle number)
+            site[t, :3] = rng.choice(sites, size=3, replace=False)
+            kind[t, :3] = [1, 1, 0]
     coef = rng.normal(size=(n_terms, 2))
     return site, kind, coef
 
@@ -74,6 +90,16 @@ CASES = [
     ("random-two-words-fermi", 1, lambda: _two_word_case(1)),
     ("random-two-words-bose2", 2, lambda: _two_word_case(2)),
 ]
+
+
+def _compile(site4: np.ndarray, kind4: np.ndarray, pc: int, nq: int) -> tuple[int, int]:
+    lib = _lib.library()
+    s = np.ascontiguousarray(site4, dtype=np.int16)
+    k = np.ascontiguousarray(kind4, dtype=np.uint8)
+    rec, status, bucket, diag = ctypes.c_uint64(0), ctypes.c_int32(0), ctypes.c_int64(0), ctypes.c_int32(0)
+    _lib.check(lib.qmb_compile_term_host(s.ctypes.data, k.ctypes.data, pc, nq, ctypes.byref(rec), ctypes.byref(status),
+                                         ctypes.byref(bucket), ctypes.byref(diag)), "compile")
+    return rec.value, status.value
 
 
 def _dev(x: np.ndarray, device: torch.device) -> torch.Tensor:
@@ -120,21 +146,27 @@ def test_pair_path_adjoint_matches_enumeration(cuda_device: torch.device, name: 
         assert np.all(err <= 1e-12 * scale), (name, symmetric, err.max())
 
 
-def test_pair_split_is_exact(cuda_device: torch.device) -> None:
-    """Counts of the residual table plus the pair-excitation term count give back the full table."""
-    s, k, c = sorted_terms(*sy.molecular_terms(10, 1001))
-    terms = pp.PreparedTerms(_dev(s, cuda_device), _dev(k, cuda_device), _dev(c, cuda_device), 3, 1)
+@pytest.mark.parametrize("name,pc,case_fn", CASES, ids=[c[0] for c in CASES])
+def test_structured_split_is_exact(cuda_device: torch.device, name: str, pc: int, case_fn: object) -> None:
+    """Every valid term goes to exactly one of: pair table, single table, diagonal records, residual
+    enumeration; the device counts agree with the host classifiers."""
+    (s, k, c), cfg = case_fn()
+    nq = cfg.shape[1]
+    terms = pp.PreparedTerms(_dev(s, cuda_device), _dev(k, cuda_device), _dev(c, cuda_device), nq, pc)
     full_valid, full_diag, _ = terms.counts()
-    off = int(terms.residual_desc.counts) - terms.residual_buffer.data_ptr()
-    res = terms.residual_buffer[off:off + 32].view(torch.int64).tolist()
-    assert res[0] + terms.n_pair_terms == full_valid
-    assert res[1] == full_diag
-    # number of pair-excitation terms from the host classifier
-    n_pairs = 0
+    assert terms.n_residual_terms + terms.n_pair_terms + terms.n_single_terms + full_diag == full_valid, name
+    lib = _lib.library()
+    out = (ctypes.c_int32 * 5)()
+    n_pairs = n_singles = 0
     for row_s, row_k in zip(s, k):
-        distinct = len(set(row_s.tolist())) == 4 and np.all(row_k < 2)
-        n_pairs += int(distinct and sorted(row_k.tolist()) == [0, 0, 1, 1])
-    assert n_pairs == terms.n_pair_terms
+        rec, status = _compile(row_s, row_k, pc, nq)
+        if status != 0:
+            continue
+        n_pairs += lib.qmb_pair_excitation_host(rec, pc, out)
+        n_singles += lib.qmb_single_excitation_host(rec, pc, out)
+    assert (n_pairs, n_singles) == (terms.n_pair_terms, terms.n_single_terms), name
+    if name.startswith("molecular"):
+        assert terms.n_residual_terms == 0  # one-body, two-body and constant terms are all structured
 
 
 def test_pair_path_selection(cuda_device: torch.device) -> None:

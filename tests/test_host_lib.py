@@ -229,3 +229,66 @@ def test_pair_excitation_classifier_matches_oracle(pc: int, nq: int) -> None:
             sign = (_pair_parity(cfg, (r0, r1, c0, c1)) ^ delta) if pc == 1 else 0
             assert par == sign, (site, kind, cfg)
     assert seen_pair > 50 and seen_other > 20
+
+
+def _single_parity(config: np.ndarray, r: int, p: int) -> int:
+    bits = np.unpackbits(config, bitorder="little").astype(np.int64)
+    return int((bits[:r].sum() + bits[:p].sum()) & 1)
+
+
+@pytest.mark.parametrize("pc", [1, 2])
+@pytest.mark.parametrize("nq", [1, 2, 7, 16])
+def test_single_excitation_classifier_matches_oracle(pc: int, nq: int) -> None:
+    """Hops, hops dressed by an occupied or empty site in every operator order, and repeated-op forms
+    are single excitations whose oracle sign is (-1)^(single_parity(c) ^ delta) on every config they
+    apply to; pair excitations, diagonal terms and particle-number changes are not."""
+    lib = _lib.library()
+    rng = np.random.default_rng(9000 + 10 * nq + pc)
+    n_bits = 8 * nq
+    out = (ctypes.c_int32 * 5)()
+    seen = {"single": 0, "other": 0}
+    for _ in range(400):
+        site = np.zeros(4, dtype=np.int16)
+        kind = np.full(4, 2, dtype=np.uint8)
+        u = rng.random()
+        if u < 0.3 and n_bits >= 2:
+            site[:2] = rng.choice(n_bits, size=2, replace=False)
+            kind[:2] = rng.permutation([0, 1])
+        elif u < 0.7 and n_bits >= 3:
+            p, k, r = rng.choice(n_bits, size=3, replace=False)
+            order = rng.permutation(4)
+            site[order] = [p, k, k, r]
+            kind[order] = [1, 1, 0, 0] if rng.random() < 0.5 else [1, 0, 1, 0]
+        else:
+            n_ops = int(rng.integers(1, 5))
+            site[:n_ops] = rng.integers(0, min(n_bits, 5), size=n_ops)
+            kind[:n_ops] = rng.integers(0, 2, size=n_ops)
+        rec, status, _, _ = _compile(site, kind, pc, nq)
+        if status != 0:
+            continue
+        is_single = bool(lib.qmb_single_excitation_host(rec, pc, out))
+        # the host classifier must agree with the action of the term on configs: exactly one site
+        # emptied, one filled, at most one other site looked at
+        used = sorted({int(s) for s, kk in zip(site, kind) if kk < 2})
+        if not is_single:
+            seen["other"] += 1
+            continue
+        seen["single"] += 1
+        r, p, k, kocc, delta = list(out)
+        assert r in used and p in used and r != p and (k == -1 or (k in used and k not in (r, p)))
+        hits = 0
+        for _ in range(24):
+            cfg = rng.integers(0, 256, size=nq, dtype=np.uint8)
+            ok, res, par = orc.apply_term(cfg, site, kind, pc)
+            bits = np.unpackbits(cfg, bitorder="little")
+            expect_ok = bits[r] == 1 and bits[p] == 0 and (k == -1 or bits[k] == kocc)
+            assert ok == expect_ok, (site, kind, cfg)
+            if not ok:
+                continue
+            hits += 1
+            want = bits.copy()
+            want[r], want[p] = 0, 1
+            np.testing.assert_array_equal(res, np.packbits(want, bitorder="little"))
+            sign = (_single_parity(cfg, r, p) ^ delta) if pc == 1 else 0
+            assert par == sign, (site, kind, cfg)
+    assert seen["single"] > 100 and seen["other"] > 20
