@@ -123,12 +123,163 @@ def cpu_baseline(w: dict[str, object], m_hit_sample: int) -> dict[str, object]:
                 "sample": sample_desc + " (oracle/qmb_oracle.c, 1 thread)", "seconds": t}
 
 
+def build_batch_instances(seed0: int = 4000, count: int = 64) -> list[dict[str, object]]:
+    """Seeded molecular instances: spin orbitals uniform over the even numbers in [24, 80], ~30%
+    filling split alpha/beta, |(pq|rs)| < 1e-10 dropped, config-set sizes log-uniform in
+    [1e3, 5e5] (capped by the sector) built as HF + singles + doubles + triples/quadruples."""
+    import math  # pylint: disable=import-outside-toplevel
+
+    import numpy as np  # pylint: disable=import-outside-toplevel
+
+    from qmb_hip import synthetic as sy  # pylint: disable=import-outside-toplevel
+
+    rng = np.random.default_rng(seed0)
+    out = []
+    for k in range(count):
+        n_so = 2 * int(rng.integers(12, 41))
+        n_orb = n_so // 2
+        n_e = int(round(0.3 * n_so))
+        n_a, n_b = (n_e + 1) // 2, n_e // 2
+        n = int(round(math.exp(rng.uniform(math.log(1e3), math.log(5e5)))))
+        n = min(n, math.comb(n_orb, n_a) * math.comb(n_orb, n_b))
+        site, kind, coef = sy.molecular_terms(n_orb, seed0 + k, threshold=1e-10)
+        order = np.argsort(-np.linalg.norm(coef.numpy(), axis=1), kind="stable")
+        try:
+            configs = sy.excitation_configs(n_orb, n_a, n_b, n, seed0 + 1000 + k)
+        except ValueError:
+            configs = sy.sector_configs(n_orb, n_a, n_b, n, seed0 + 1000 + k)
+        out.append({"site": site[order], "kind": kind[order], "coef": coef[order], "configs": configs, "n_so": n_so,
+                    "electrons": (n_a, n_b), "psi": sy.random_psi(len(configs), seed0 + 2000 + k),
+                    "grad": sy.random_psi(len(configs), seed0 + 3000 + k)})
+        if (k + 1) % 8 == 0:
+            log(f"batch64: generated {k + 1}/{count} instances")
+    return out
+
+
+def run_batch(args: argparse.Namespace, dev: object) -> int:
+    """64 molecular instances, forward + adjoint of all of them: one persistent batched launch per
+    direction (graph-captured) against per-instance structured applications."""
+    import numpy as np  # pylint: disable=import-outside-toplevel
+    import torch  # pylint: disable=import-outside-toplevel
+
+    from qmb_hip import batch as qb  # pylint: disable=import-outside-toplevel
+    from qmb_hip import prepared as pp  # pylint: disable=import-outside-toplevel
+
+    t0 = time.time()
+    raw = build_batch_instances()
+    log(f"batch64: generated {len(raw)} instances in {time.time() - t0:.1f}s")
+    instances, psis, grads = [], [], []
+    m_gen = m_hit = 0
+    algo_bytes = 0
+    for w in raw:
+        configs = torch.from_numpy(w["configs"]).to(dev)
+        nq = configs.shape[1]
+        site, kind, coef = (t.contiguous().to(dev) for t in (w["site"], w["kind"], w["coef"]))
+        terms = pp.PreparedTerms(site, kind, coef, nq, 1)
+        targets = pp.PreparedTargets(configs, nq, distinct=True, max_occupied=sum(w["electrons"]))
+        g, h = pp.count(terms, targets, configs)
+        m_gen += g
+        m_hit += h
+        n, t = configs.shape[0], site.shape[0]
+        algo_bytes += 2 * n * (nq + 16) + t * 28 + g * nq + h * 16
+        instances.append((terms, targets))
+        if len(instances) % 8 == 0:
+            log(f"batch64: prepared {len(instances)}/{len(raw)} instances")
+        psis.append(torch.from_numpy(w["psi"]).to(dev))
+        grads.append(torch.from_numpy(w["grad"]).to(dev))
+    torch.cuda.synchronize()
+    log(f"batch64: prepared in {time.time() - t0:.1f}s; M_gen={m_gen} M_hit={m_hit}")
+    b = qb.Batch(instances)
+    b.set_inputs(psis)
+    waves = b.tune()
+    stream = torch.cuda.current_stream(dev)
+
+    def time_it(fn: object, reps: int) -> float:
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        a, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        e.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(e) / reps
+
+    # per-instance: the structured application of each instance, forward then adjoint
+    outs = [torch.zeros((t.n, 2), dtype=torch.float64, device=dev) for _, t in instances]
+    vins = [torch.view_as_real(p).contiguous() for p in psis]
+    gins = [torch.view_as_real(g).contiguous() for g in grads]
+
+    def per_instance() -> None:
+        for (terms, targets), o, v, g in zip(instances, outs, vins, gins):
+            pp.apply(terms, targets, targets.configs, v, pp.MODE_FORWARD, out=o)
+            pp.apply(terms, targets, targets.configs, g, pp.MODE_ADJOINT, out=o)
+
+    def batched() -> None:
+        b.launch(pp.MODE_FORWARD)
+        b.launch(pp.MODE_ADJOINT)
+
+    steps = max(args.steps, 1)
+    eager_inst = time_it(per_instance, steps)
+    eager_batch = time_it(batched, steps)
+    graphs = {}
+    for name, fn in (("per_instance", per_instance), ("batched", batched)):
+        side = torch.cuda.Stream(device=dev)
+        side.wait_stream(stream)
+        with torch.cuda.stream(side):
+            fn()
+        stream.wait_stream(side)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            fn()
+        graphs[name] = g
+    graph_inst = time_it(graphs["per_instance"].replay, steps)
+    graph_batch = time_it(graphs["batched"].replay, steps)
+    fwd_only = time_it(lambda: b.launch(pp.MODE_FORWARD), steps)
+    # check: the batched forward equals the per-instance forward
+    b.launch(pp.MODE_FORWARD)
+    worst = 0.0
+    for k, ((terms, targets), v) in enumerate(zip(instances, vins)):
+        ref = pp.apply(terms, targets, targets.configs, v, pp.MODE_FORWARD)
+        worst = max(worst, float((b.out[b.rows[k][0]:b.rows[k][1]] - ref).abs().max()) / max(float(ref.abs().max()), 1e-300))
+    log(f"batch64: per-instance eager {eager_inst:.2f} ms, graph {graph_inst:.2f} ms; batched eager {eager_batch:.2f} ms, "
+        f"graph {graph_batch:.2f} ms (waves {waves}); max rel diff {worst:.2e}")
+    achieved = algo_bytes / (fwd_only * 1e-3) / 1e9
+    result = {
+        "metric": "apply_within fwd+bwd matrix elements/s",
+        "value": 2 * m_hit / (graph_batch * 1e-3),
+        "unit": "matrix elements/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 2,
+        "ms_per_step": graph_batch,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded)",
+        "config": {"workload": "batch64", "instances": len(raw), "spin_orbitals": [w["n_so"] for w in raw],
+                   "n_configs": [int(len(w["configs"])) for w in raw], "n_terms_total": int(sum(len(w["site"]) for w in raw)),
+                   "M_gen": m_gen, "M_hit": m_hit, "waves": waves},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": None, "kernel": "batch_kernel<adjoint> (one launch, all instances)", "kernel_ms": fwd_only,
+                     "algorithmic_bytes": algo_bytes},
+        "batch": {"batched_graph_ms": graph_batch, "batched_eager_ms": eager_batch, "per_instance_eager_ms": eager_inst,
+                  "per_instance_graph_ms": graph_inst, "speedup_vs_per_instance_eager": eager_inst / graph_batch,
+                  "speedup_vs_per_instance_graph": graph_inst / graph_batch, "max_rel_diff_vs_per_instance": worst},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=10)
     parser.add_argument("--warmup", type=int, default=3)
-    parser.add_argument("--workload", default="molecular52", choices=["molecular52", "hubbard128", "molecular20"])
+    parser.add_argument("--workload", default="molecular52", choices=["molecular52", "hubbard128", "molecular20", "batch64"])
     parser.add_argument("--tile", type=int, default=0)
     parser.add_argument("--groups", type=int, default=0)
     parser.add_argument("--no-plain", action="store_true", help="skip the reference-scheme comparator")
@@ -152,6 +303,8 @@ def main() -> int:
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a HIP GPU")
     dev = torch.device("cuda:0")
+    if args.workload == "batch64":
+        return run_batch(args, dev)
     w = build_workload(args.workload)
     configs = torch.from_numpy(w["configs"]).to(dev)
     nq = configs.shape[1]
@@ -223,25 +376,32 @@ def main() -> int:
                                         zero=False))
     parts = None
     if use_pairs:
-        cores = targets.cores()
+        # the structured path's kernels, each timed alone on the same stream
         s_ptr = ctypes.c_void_p(stream.cuda_stream)
-        launch = pp._launch(args.tile, args.groups, 0, symmetric)  # pylint: disable=protected-access
         pmode = pp.MODE_ADJOINT if symmetric else pp.MODE_FORWARD
-        pair_ms = timed(lambda: lib.qmb_apply_pairs(ctypes.byref(terms.pairs_desc), ctypes.byref(cores), ctypes.byref(targets.desc),
-                                                    ctypes.c_void_p(vec.data_ptr()), ctypes.c_void_p(out.data_ptr()), pmode, s_ptr))
-        residual_ms = timed(lambda: lib.qmb_apply_prepared(ctypes.byref(terms.residual_desc), ctypes.byref(targets.desc),
-                                                           ctypes.c_void_p(configs.data_ptr()), n, ctypes.c_void_p(vec.data_ptr()),
-                                                           ctypes.c_void_p(out.data_ptr()), pp.MODE_FORWARD,
-                                                           ctypes.byref(launch) if launch else None, s_ptr))
+        vin, vout = ctypes.c_void_p(vec.data_ptr()), ctypes.c_void_p(out.data_ptr())
+        cores2, cores1 = targets.cores(2), targets.cores(1)
+        parts = {"pair_excitation_terms": terms.n_pair_terms, "single_excitation_terms": terms.n_single_terms,
+                 "residual_terms": terms.n_residual_terms}
+        parts["pair_kernel_ms"] = timed(lambda: lib.qmb_apply_pairs(ctypes.byref(terms.pairs_desc), ctypes.byref(cores2),
+                                                                    ctypes.byref(targets.desc), vin, vout, pmode, s_ptr))
+        if terms.singles_desc is not None:
+            parts["single_kernel_ms"] = timed(lambda: lib.qmb_apply_singles(ctypes.byref(terms.singles_desc), ctypes.byref(cores1),
+                                                                            ctypes.byref(targets.desc), vin, vout, pmode, s_ptr))
+        parts["diagonal_kernel_ms"] = timed(lambda: lib.qmb_apply_diagonal(ctypes.byref(terms.desc), ctypes.byref(targets.desc), vin,
+                                                                           vout, pmode, s_ptr))
+        if terms.n_residual_terms > 0:
+            launch = pp._launch(args.tile, args.groups, 0, symmetric)  # pylint: disable=protected-access
+            parts["enumeration_kernel_ms"] = timed(lambda: lib.qmb_apply_prepared(
+                ctypes.byref(terms.residual_desc), ctypes.byref(targets.desc), ctypes.c_void_p(configs.data_ptr()), n, vin, vout,
+                pp.MODE_FORWARD, ctypes.byref(launch) if launch else None, s_ptr))
 
-        def rebuild() -> None:
-            targets._cores = None  # pylint: disable=protected-access
-            targets.cores()
+        def rebuild(order: int) -> None:
+            targets._cores.pop(order, None)  # pylint: disable=protected-access
+            targets.cores(order)
 
-        cores_ms = timed(rebuild)
-        parts = {"pair_kernel_ms": pair_ms, "enumeration_kernel_ms": residual_ms, "core_index_build_ms": cores_ms,
-                 "pair_excitation_terms": terms.n_pair_terms}
-        log(f"pair kernel {pair_ms:.3f} ms, enumeration kernel (other terms) {residual_ms:.3f} ms, core index {cores_ms:.3f} ms")
+        parts["core_index_build_ms"] = {"order2": timed(lambda: rebuild(2)), "order1": timed(lambda: rebuild(1))}
+        log(f"structured path parts: {parts}")
 
     # achievable HBM bandwidth: a plain 1 GiB device copy
     src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)

@@ -281,6 +281,41 @@ int qmb_apply_singles(const qmb_singles* singles, const qmb_cores* cores, const 
  * conj(d_i) in[i] (mode 1), d_i = sum of the diagonal terms on row i in term order. */
 int qmb_apply_diagonal(const qmb_terms* terms, const qmb_targets* targets, const double* in, double* out, int mode, void* stream);
 
+/* ---------------- batched persistent launch (no reference counterpart) -------------------------
+ * Many instances -- a term table and a config set each, of any sizes -- applied in ONE persistent
+ * kernel launch: the diagonal, single- and pair-excitation work of every instance is cut into
+ * units of 64 core groups or 64 rows, and persistent waves take the units round-robin (pair units
+ * of all instances first).  For many small instances this replaces a few kernels per instance by
+ * one launch that fills the GPU; the launch is capturable in a hipGraph (no allocation, no sync).
+ * An instance whose table has terms outside those classes (its residual table) needs that part
+ * applied separately. */
+typedef struct qmb_batch_item {
+    const qmb_terms* terms;      /* full table: its diagonal records are applied */
+    const qmb_pairs* pairs;      /* pair-excitation table or NULL */
+    const qmb_singles* singles;  /* single-excitation table or NULL */
+    const qmb_targets* targets;  /* index of the instance's config set (n_qubytes <= 16) */
+    const qmb_cores* cores2;     /* order-2 core index of that set (with pairs) */
+    const qmb_cores* cores1;     /* order-1 core index of that set (with singles) */
+    const double* in;            /* [n][2] */
+    double* out;                 /* [n][2], accumulated into */
+} qmb_batch_item;
+
+typedef struct qmb_batch {
+    int32_t n_items, reserved;
+    int64_t total_units;
+    void* buffer;
+    size_t bytes;
+    void* items;       /* device table of the items */
+    int64_t* offsets;  /* device [3 * n_items + 1]: first unit of (type, item) */
+} qmb_batch;
+
+int qmb_batch_workspace_size(int n_items, size_t* bytes);
+/* Copies the item table to the device; synchronises `stream` once (set-up, not per launch). */
+int qmb_batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t bytes, qmb_batch* out, void* stream);
+/* One persistent launch of `waves` wavefronts (0: 8192) over all units; mode 0 forward, 1 adjoint
+ * (a Hermitian instance gives the same map in both). */
+int qmb_apply_batch(const qmb_batch* batch, int mode, int waves, void* stream);
+
 /* Host-only pair-excitation test of a compiled record (for unit tests): returns 1 and writes
  * {r0, r1, c0, c1, delta} when the record is a pair excitation, else 0. */
 int qmb_pair_excitation_host(uint64_t rec, int particle_cut, int32_t out[5]);

@@ -413,6 +413,28 @@ int qmb_apply_diagonal(const qmb_terms* terms, const qmb_targets* targets, const
     return apply_diagonal(terms, targets, in, out, mode, as_stream(stream));
 }
 
+int qmb_batch_workspace_size(int n_items, size_t* bytes) {
+    clear_error();
+    if (!bytes) {
+        return set_error(QMB_ERR_ARGUMENT, "bytes must be non-null");
+    }
+    *bytes = batch_bytes(n_items);
+    return *bytes ? QMB_OK : QMB_ERR_ARGUMENT;
+}
+
+int qmb_batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t bytes, qmb_batch* out, void* stream) {
+    clear_error();
+    if (!items || !out || !buffer) {
+        return set_error(QMB_ERR_ARGUMENT, "items, buffer and out must be non-null");
+    }
+    return batch_plan(items, n_items, buffer, bytes, out, as_stream(stream));
+}
+
+int qmb_apply_batch(const qmb_batch* batch, int mode, int waves, void* stream) {
+    clear_error();
+    return apply_batch(batch, mode, waves, as_stream(stream));
+}
+
 int qmb_single_excitation_host(uint64_t rec, int particle_cut, int32_t out[5]) {
     clear_error();
     SingleExcitation e;

@@ -155,6 +155,9 @@ int apply_singles(const qmb_singles* singles, const qmb_cores* cores, const qmb_
                   int mode, hipStream_t stream);
 int apply_diagonal(const qmb_terms* terms, const qmb_targets* targets, const double* in, double* out, int mode,
                    hipStream_t stream);
+size_t batch_bytes(int n_items);
+int batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t bytes, qmb_batch* out, hipStream_t stream);
+int apply_batch(const qmb_batch* batch, int mode, int waves, hipStream_t stream);
 
 size_t find_relative_scratch_bytes(int n_qubytes, int64_t count_selected);
 int find_relative_prepared(const qmb_terms* terms, const qmb_targets* exclude, const uint8_t* configs_i, const double* psi_i,

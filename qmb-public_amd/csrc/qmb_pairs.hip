@@ -12,6 +12,8 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
+#include <vector>
+
 #include "qmb_internal.hpp"
 
 namespace qmb {
@@ -378,36 +380,45 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
     }
 }
 
-// Persistent waves over the groups, 64 groups per step: the lanes test their group's size and the
-// wave walks the ones with at least two entries.  Every wave leaves when the group count is done.
+// Walks the groups [g0, g0 + 64) of a core index that exist: the lanes test their group's size
+// and fn(s, e) runs, wave-uniform, on every group with at least two entries.
+template <typename F>
+__device__ __forceinline__ void walk_chunk(const uint32_t* __restrict__ starts, int64_t groups, int64_t g0, int lane, F&& fn) {
+    const int64_t gg = g0 + lane;
+    uint32_t gs = 0, ge = 0;
+    if (gg < groups) {
+        gs = starts[gg];
+        ge = starts[gg + 1];
+    }
+    uint64_t multi = __ballot(gg < groups && ge - gs >= 2u);
+    while (multi) {
+        const int l = __builtin_ctzll(multi);
+        multi &= multi - 1;
+        fn(__shfl(gs, l), __shfl(ge, l));
+    }
+}
+
+__device__ __forceinline__ void mark_overflow(double2* out) {
+    // a row had more occupied sites than the core index was built for
+    out[0] = make_double2(__longlong_as_double(0x7ff8000000000000ll), __longlong_as_double(0x7ff8000000000000ll));
+}
+
+// Persistent waves over the groups, 64 groups per step.  Every wave leaves when the group count
+// is done.
 template <int W, bool ADJ>
 __global__ __launch_bounds__(kBlock) void pair_kernel(PairParams p) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6;
     const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * kBlock) >> 6;
     if (p.counts[2] != 0) {
-        // a row had more occupied sites than the index was built for
         if (wave == 0 && lane == 0) {
-            p.out[0] = make_double2(__longlong_as_double(0x7ff8000000000000ll), __longlong_as_double(0x7ff8000000000000ll));
+            mark_overflow(p.out);
         }
         return;
     }
     const int64_t groups = p.counts[1];
     for (int64_t g0 = wave * 64; g0 < groups; g0 += nwaves * 64) {
-        const int64_t gg = g0 + lane;
-        uint32_t gs = 0, ge = 0;
-        if (gg < groups) {
-            gs = p.starts[gg];
-            ge = p.starts[gg + 1];
-        }
-        uint64_t multi = __ballot(gg < groups && ge - gs >= 2u);
-        while (multi) {
-            const int l = __builtin_ctzll(multi);
-            multi &= multi - 1;
-            const uint32_t s = __shfl(gs, l);
-            const uint32_t e = __shfl(ge, l);
-            pair_group<W, ADJ>(p, s, e, lane);
-        }
+        walk_chunk(p.starts, groups, g0, lane, [&](uint32_t s, uint32_t e) { pair_group<W, ADJ>(p, s, e, lane); });
     }
 }
 
@@ -589,26 +600,13 @@ __global__ __launch_bounds__(kBlock) void single_kernel(SingleParams p) {
     const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * kBlock) >> 6;
     if (p.counts[2] != 0) {
         if (wave == 0 && lane == 0) {
-            p.out[0] = make_double2(__longlong_as_double(0x7ff8000000000000ll), __longlong_as_double(0x7ff8000000000000ll));
+            mark_overflow(p.out);
         }
         return;
     }
     const int64_t groups = p.counts[1];
     for (int64_t g0 = wave * 64; g0 < groups; g0 += nwaves * 64) {
-        const int64_t gg = g0 + lane;
-        uint32_t gs = 0, ge = 0;
-        if (gg < groups) {
-            gs = p.starts[gg];
-            ge = p.starts[gg + 1];
-        }
-        uint64_t multi = __ballot(gg < groups && ge - gs >= 2u);
-        while (multi) {
-            const int l = __builtin_ctzll(multi);
-            multi &= multi - 1;
-            const uint32_t s = __shfl(gs, l);
-            const uint32_t e = __shfl(ge, l);
-            single_group<W, ADJ>(p, s, e, lane);
-        }
+        walk_chunk(p.starts, groups, g0, lane, [&](uint32_t s, uint32_t e) { single_group<W, ADJ>(p, s, e, lane); });
     }
 }
 
@@ -661,6 +659,141 @@ __global__ __launch_bounds__(kBlock) void diag_apply_kernel(const uint64_t* __re
         }
         out[i].x += re * g.x - im * g.y;
         out[i].y += re * g.y + im * g.x;
+    }
+}
+
+// --------------------------------------------------------------------- batched persistent launch --
+// Many instances (term tables + config sets) in one launch: the work of every instance is cut into
+// units -- 64 pair-core groups, 64 single-core groups, 64 diagonal rows -- numbered type-major
+// (all pair units, then singles, then diagonal, so the heaviest work is taken first), and
+// persistent waves take units round-robin until all are done.
+struct BatchItemDev {
+    int32_t n_words, particle_cut, has_pairs, has_singles;
+    int32_t n_bits, reserved;
+    int64_t n;
+    const uint64_t* ckeys;
+    const uint64_t* drecs;
+    const double2* dcoefs;
+    const int64_t* tcounts;
+    const double2* pcoef;
+    int64_t n_pairs;
+    const uint64_t* pvals;
+    const uint32_t* pstarts;
+    const int64_t* pcounts;
+    const double2* sa;
+    const double2* skd;
+    const uint64_t* svals;
+    const uint32_t* sstarts;
+    const int64_t* scounts;
+    const double2* in;
+    double2* out;
+};
+
+constexpr int kBatchMaxItems = 4096;
+
+template <int W, bool ADJ>
+__device__ void diag_rows(const BatchItemDev& it, int64_t r0, int lane) {
+    const int64_t i = r0 + lane;
+    const bool valid = i < it.n;
+    Cfg<W> c;
+    if (valid) {
+        c = load_key<W>(it.ckeys, i);
+    } else {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            c.w[w] = 0;
+        }
+    }
+    const int64_t td = it.tcounts[1];
+    double re = 0.0, im = 0.0;
+    for (int64_t u = 0; u < td; ++u) {
+        Cfg<W> o;
+        int par;
+        if (apply_rec<W>(it.drecs[u], c, o, par)) {
+            const double2 cf = it.dcoefs[u];
+            re += par ? -cf.x : cf.x;
+            im += par ? -cf.y : cf.y;
+        }
+    }
+    if (valid && (re != 0.0 || im != 0.0)) {
+        const double2 g = it.in[i];
+        if (ADJ) {
+            im = -im;
+        }
+        // pair and single units of the same launch add into these rows concurrently
+        unsafeAtomicAdd(&it.out[i].x, re * g.x - im * g.y);
+        unsafeAtomicAdd(&it.out[i].y, re * g.y + im * g.x);
+    }
+}
+
+template <int W, bool ADJ>
+__device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int lane) {
+    if (type == 0) {
+        PairParams p{};
+        p.coef = it.pcoef;
+        p.n_pairs = it.n_pairs;
+        p.particle_cut = it.particle_cut;
+        p.ckeys = it.ckeys;
+        p.vals = it.pvals;
+        p.starts = it.pstarts;
+        p.counts = it.pcounts;
+        p.in = it.in;
+        p.out = it.out;
+        const int64_t groups = it.pcounts[1];
+        if (local * 64 < groups && it.pcounts[2] == 0) {
+            walk_chunk(p.starts, groups, local * 64, lane, [&](uint32_t s, uint32_t e) { pair_group<W, ADJ>(p, s, e, lane); });
+        }
+    } else if (type == 1) {
+        SingleParams p{};
+        p.a = it.sa;
+        p.kd = it.skd;
+        p.n_bits = it.n_bits;
+        p.particle_cut = it.particle_cut;
+        p.ckeys = it.ckeys;
+        p.vals = it.svals;
+        p.starts = it.sstarts;
+        p.counts = it.scounts;
+        p.in = it.in;
+        p.out = it.out;
+        const int64_t groups = it.scounts[1];
+        if (local * 64 < groups && it.scounts[2] == 0) {
+            walk_chunk(p.starts, groups, local * 64, lane, [&](uint32_t s, uint32_t e) { single_group<W, ADJ>(p, s, e, lane); });
+        }
+    } else {
+        diag_rows<W, ADJ>(it, local * 64, lane);
+        if (local == 0 && lane == 0 &&
+            ((it.has_pairs && it.pcounts[2] != 0) || (it.has_singles && it.scounts[2] != 0))) {
+            mark_overflow(it.out);
+        }
+    }
+}
+
+template <bool ADJ>
+__global__ __launch_bounds__(kBlock) void batch_kernel(const BatchItemDev* __restrict__ items, const int64_t* __restrict__ offsets,
+                                                       int n_items, int64_t total) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6;
+    const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * kBlock) >> 6;
+    const int slots = 3 * n_items;
+    for (int64_t u = wave; u < total; u += nwaves) {
+        // last slot whose first unit is <= u (offsets[slots] == total > u)
+        int lo = 0, hi = slots;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (offsets[mid] <= u) {
+                lo = mid;
+            } else {
+                hi = mid;
+            }
+        }
+        const int type = lo / n_items;
+        const BatchItemDev& it = items[lo - type * n_items];
+        const int64_t local = u - offsets[lo];
+        if (it.n_words == 1) {
+            batch_unit<1, ADJ>(it, type, local, lane);
+        } else {
+            batch_unit<2, ADJ>(it, type, local, lane);
+        }
     }
 }
 
@@ -904,6 +1037,123 @@ int apply_diagonal(const qmb_terms* terms, const qmb_targets* targets, const dou
         case 7: diag_apply_kernel<3, true><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
         case 8: diag_apply_kernel<4, false><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
         default: diag_apply_kernel<4, true><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
+    }
+    QMB_CHECK_LAUNCH();
+    return QMB_OK;
+}
+
+int batch_layout(Carver& cv, int n_items, qmb_batch* out) {
+    if (n_items < 1 || n_items > kBatchMaxItems) {
+        return set_error(QMB_ERR_ARGUMENT, "n_items must be in [1, %d], got %d", kBatchMaxItems, n_items);
+    }
+    qmb_batch b{};
+    b.n_items = n_items;
+    b.buffer = cv.base;
+    b.items = cv.take<BatchItemDev>(static_cast<size_t>(n_items));
+    b.offsets = cv.take<int64_t>(3 * static_cast<size_t>(n_items) + 1);
+    b.bytes = cv.off;
+    if (out) {
+        *out = b;
+    }
+    return QMB_OK;
+}
+
+size_t batch_bytes(int n_items) {
+    Carver cv{nullptr, 0};
+    return batch_layout(cv, n_items, nullptr) == QMB_OK ? cv.off : 0;
+}
+
+int batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t bytes, qmb_batch* out, hipStream_t stream) {
+    Carver cv{static_cast<char*>(buffer), 0};
+    qmb_batch b{};
+    QMB_TRY(batch_layout(cv, n_items, &b));
+    if (cv.off > bytes) {
+        return set_error(QMB_ERR_WORKSPACE, "batch buffer too small: need %zu bytes, have %zu", cv.off, bytes);
+    }
+    b.bytes = bytes;
+    std::vector<BatchItemDev> dev(static_cast<size_t>(n_items));
+    std::vector<int64_t> units(3 * static_cast<size_t>(n_items), 0);
+    for (int k = 0; k < n_items; ++k) {
+        const qmb_batch_item& x = items[k];
+        if (!x.terms || !x.targets) {
+            return set_error(QMB_ERR_ARGUMENT, "item %d: terms and targets must be non-null", k);
+        }
+        const qmb_targets& tg = *x.targets;
+        if (x.terms->n_qubytes != tg.n_qubytes || tg.n_words > 2) {
+            return set_error(QMB_ERR_ARGUMENT, "item %d: terms / targets shapes differ or n_qubytes > 16", k);
+        }
+        if (tg.n > 0 && (!x.in || !x.out)) {
+            return set_error(QMB_ERR_ARGUMENT, "item %d: in and out must be non-null", k);
+        }
+        if ((x.pairs && (!x.cores2 || x.cores2->order != 2 || x.cores2->n != tg.n || x.pairs->n_qubytes != tg.n_qubytes)) ||
+            (x.singles && (!x.cores1 || x.cores1->order != 1 || x.cores1->n != tg.n || x.singles->n_qubytes != tg.n_qubytes))) {
+            return set_error(QMB_ERR_ARGUMENT, "item %d: pair / single tables need matching core indexes of order 2 / 1", k);
+        }
+        BatchItemDev d{};
+        d.n_words = tg.n_words;
+        d.particle_cut = x.terms->particle_cut;
+        d.n_bits = 8 * tg.n_qubytes;
+        d.n = tg.n;
+        d.ckeys = tg.keys;
+        d.drecs = x.terms->drecs;
+        d.dcoefs = reinterpret_cast<const double2*>(x.terms->dcoefs);
+        d.tcounts = x.terms->counts;
+        if (x.pairs) {
+            d.has_pairs = 1;
+            d.pcoef = reinterpret_cast<const double2*>(x.pairs->coef);
+            d.n_pairs = x.pairs->n_pairs;
+            d.pvals = x.cores2->vals;
+            d.pstarts = x.cores2->starts;
+            d.pcounts = x.cores2->counts;
+            units[k] = (x.cores2->capacity + 63) / 64;
+        }
+        if (x.singles) {
+            d.has_singles = 1;
+            d.sa = reinterpret_cast<const double2*>(x.singles->a);
+            d.skd = reinterpret_cast<const double2*>(x.singles->kd);
+            d.svals = x.cores1->vals;
+            d.sstarts = x.cores1->starts;
+            d.scounts = x.cores1->counts;
+            units[n_items + k] = (x.cores1->capacity + 63) / 64;
+        }
+        units[2 * n_items + k] = (tg.n + 63) / 64;
+        d.in = reinterpret_cast<const double2*>(x.in);
+        d.out = reinterpret_cast<double2*>(x.out);
+        dev[static_cast<size_t>(k)] = d;
+    }
+    std::vector<int64_t> offsets(units.size() + 1, 0);
+    for (size_t q = 0; q < units.size(); ++q) {
+        offsets[q + 1] = offsets[q] + units[q];
+    }
+    b.total_units = offsets.back();
+    QMB_HIP_TRY(hipMemcpyAsync(b.items, dev.data(), dev.size() * sizeof(BatchItemDev), hipMemcpyHostToDevice, stream));
+    QMB_HIP_TRY(hipMemcpyAsync(b.offsets, offsets.data(), offsets.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+    // the host tables above go out of scope on return
+    QMB_HIP_TRY(hipStreamSynchronize(stream));
+    *out = b;
+    return QMB_OK;
+}
+
+int apply_batch(const qmb_batch* batch, int mode, int waves, hipStream_t stream) {
+    if (!batch) {
+        return set_error(QMB_ERR_ARGUMENT, "batch must be non-null");
+    }
+    if (mode != 0 && mode != 1) {
+        return set_error(QMB_ERR_ARGUMENT, "mode must be 0 (forward) or 1 (adjoint)");
+    }
+    if (batch->total_units == 0) {
+        return QMB_OK;
+    }
+    int64_t w = waves > 0 ? waves : 8192;
+    w = w < 4 ? 4 : w;
+    int64_t blocks = (w + 3) / 4;
+    const int64_t need = (batch->total_units + 3) / 4;
+    blocks = blocks < need ? blocks : need;
+    const auto* items = static_cast<const BatchItemDev*>(batch->items);
+    if (mode == 1) {
+        batch_kernel<true><<<static_cast<unsigned>(blocks), kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
+    } else {
+        batch_kernel<false><<<static_cast<unsigned>(blocks), kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
     }
     QMB_CHECK_LAUNCH();
     return QMB_OK;

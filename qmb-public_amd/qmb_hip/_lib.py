@@ -119,6 +119,35 @@ class QmbCores(ctypes.Structure):
     ]
 
 
+class QmbBatchItem(ctypes.Structure):
+    """Mirror of ``struct qmb_batch_item`` (include/qmb_hip.h)."""
+
+    _fields_ = [
+        ("terms", ctypes.POINTER(QmbTerms)),
+        ("pairs", ctypes.POINTER(QmbPairs)),
+        ("singles", ctypes.POINTER(QmbSingles)),
+        ("targets", ctypes.POINTER(QmbTargets)),
+        ("cores2", ctypes.POINTER(QmbCores)),
+        ("cores1", ctypes.POINTER(QmbCores)),
+        ("in_", _vp),
+        ("out", _vp),
+    ]
+
+
+class QmbBatch(ctypes.Structure):
+    """Mirror of ``struct qmb_batch`` (include/qmb_hip.h)."""
+
+    _fields_ = [
+        ("n_items", _c_i32),
+        ("reserved", _c_i32),
+        ("total_units", _c_i64),
+        ("buffer", _vp),
+        ("bytes", _c_size),
+        ("items", _vp),
+        ("offsets", _vp),
+    ]
+
+
 class QmbLaunch(ctypes.Structure):
     """Mirror of ``struct qmb_launch`` (include/qmb_hip.h)."""
 
@@ -176,6 +205,9 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
         [ctypes.POINTER(QmbSingles), ctypes.POINTER(QmbCores), ctypes.POINTER(QmbTargets), _vp, _vp, _c_int, _vp],
     ),
     "qmb_apply_diagonal": (_c_int, [ctypes.POINTER(QmbTerms), ctypes.POINTER(QmbTargets), _vp, _vp, _c_int, _vp]),
+    "qmb_batch_workspace_size": (_c_int, [_c_int, ctypes.POINTER(_c_size)]),
+    "qmb_batch_plan": (_c_int, [ctypes.POINTER(QmbBatchItem), _c_int, _vp, _c_size, ctypes.POINTER(QmbBatch), _vp]),
+    "qmb_apply_batch": (_c_int, [ctypes.POINTER(QmbBatch), _c_int, _c_int, _vp]),
     "qmb_pair_excitation_host": (_c_int, [_c_u64, _c_int, ctypes.POINTER(_c_i32)]),
     "qmb_single_excitation_host": (_c_int, [_c_u64, _c_int, ctypes.POINTER(_c_i32)]),
 }

@@ -1,0 +1,125 @@
+"""
+Batched application of many Hamiltonians to many config sets in one persistent launch.
+
+Each instance is a PreparedTerms (with the structured tables: pair and single excitations,
+diagonal records) and a PreparedTargets over its distinct config set.  The inputs and outputs of all
+instances are packed into two device buffers, so the plan (qmb_batch_plan) is built once and every
+launch (qmb_apply_batch) is a single persistent kernel over all instances' work; the whole step
+can be captured in a hipGraph (torch.cuda.CUDAGraph) and replayed.  The number of persistent
+waves is picked by the autotuner.  Instances whose tables keep terms outside the structured
+classes get those applied by one enumeration launch each after the batch.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import typing
+
+import torch
+
+from . import _lib
+from . import prepared as pp
+from .autotune import TUNER
+from .ops import _ptr, _stream, _workspace
+
+WAVE_CANDIDATES = (1024, 2048, 4096, 8192, 16384, 32768)
+
+
+class Batch:
+    """Packed inputs/outputs + the device plan for a list of (terms, targets) instances."""
+
+    def __init__(self, instances: typing.Sequence[tuple[pp.PreparedTerms, pp.PreparedTargets]]) -> None:
+        if not instances:
+            raise ValueError("a batch needs at least one instance")
+        device = instances[0][1].device
+        self.device = device
+        self.instances = list(instances)
+        self.rows = []
+        off = 0
+        for terms, targets in self.instances:
+            if targets.device != device or terms.device != device:
+                raise ValueError("all instances must live on one device")
+            if terms.pairs_desc is None:
+                raise ValueError("batched instances need the structured tables (a table with pair excitations, n_qubytes <= 16)")
+            if targets.duplicates() != 0:
+                raise ValueError("batched config sets must have distinct rows")
+            targets.cores(2)
+            if terms.singles_desc is not None:
+                targets.cores(1)
+            self.rows.append((off, off + targets.n))
+            off += targets.n
+        self.n_rows = off
+        self.inp = torch.zeros((max(off, 1), 2), dtype=torch.float64, device=device)
+        self.out = torch.zeros((max(off, 1), 2), dtype=torch.float64, device=device)
+        self.hermitian = all(t.hermitian for t, _ in self.instances)
+        n = len(self.instances)
+        self._items = (_lib.QmbBatchItem * n)()
+        for k, (terms, targets) in enumerate(self.instances):
+            lo = self.rows[k][0]
+            it = self._items[k]
+            it.terms = ctypes.pointer(terms.desc)
+            it.pairs = ctypes.pointer(terms.pairs_desc)
+            it.singles = ctypes.pointer(terms.singles_desc) if terms.singles_desc is not None else None
+            it.targets = ctypes.pointer(targets.desc)
+            it.cores2 = ctypes.pointer(targets.cores(2))
+            it.cores1 = ctypes.pointer(targets.cores(1)) if terms.singles_desc is not None else None
+            it.in_ = self.inp.data_ptr() + 16 * lo
+            it.out = self.out.data_ptr() + 16 * lo
+        lib = _lib.library()
+        size = ctypes.c_size_t(0)
+        _lib.check(lib.qmb_batch_workspace_size(n, ctypes.byref(size)), "qmb_batch_workspace_size")
+        self.buffer = _workspace(size.value, device)
+        self.plan = _lib.QmbBatch()
+        with torch.cuda.device(device):
+            _lib.check(lib.qmb_batch_plan(self._items, n, _ptr(self.buffer), self.buffer.numel(), ctypes.byref(self.plan),
+                                          _stream(device)), "qmb_batch_plan")
+        self.waves = 0
+        self.residual = [k for k, (t, _) in enumerate(self.instances) if t.n_residual_terms > 0]
+
+    # ------------------------------------------------------------------ inputs and outputs
+    def set_inputs(self, psis: typing.Sequence[torch.Tensor]) -> None:
+        """Copies each instance's complex128 vector into the packed input buffer."""
+        if len(psis) != len(self.instances):
+            raise ValueError("one vector per instance")
+        for (lo, hi), psi in zip(self.rows, psis):
+            if psi.dtype != torch.complex128 or psi.shape != (hi - lo,):
+                raise RuntimeError("each vector must be complex128 with one entry per config of its instance")
+            self.inp[lo:hi].copy_(torch.view_as_real(psi.resolve_conj()))
+
+    def output(self, k: int) -> torch.Tensor:
+        """Instance k's result as a complex128 view of the packed output buffer."""
+        lo, hi = self.rows[k]
+        return torch.view_as_complex(self.out[lo:hi])
+
+    def input(self, k: int) -> torch.Tensor:
+        lo, hi = self.rows[k]
+        return torch.view_as_complex(self.inp[lo:hi])
+
+    # ------------------------------------------------------------------------------ launch
+    def launch(self, mode: int, waves: int | None = None) -> None:
+        """out = A in (mode 0) or A^H in (mode 1) for every instance: one persistent launch plus one
+        enumeration launch per instance with residual terms.  Enqueue-only (graph capturable)."""
+        self.out.zero_()
+        lib = _lib.library()
+        stream = _stream(self.device)
+        pmode = pp.MODE_ADJOINT if self.hermitian else mode
+        w = self.waves if waves is None else waves
+        with torch.cuda.device(self.device):
+            _lib.check(lib.qmb_apply_batch(ctypes.byref(self.plan), pmode, w, stream), "qmb_apply_batch")
+            for k in self.residual:
+                terms, targets = self.instances[k]
+                lo, _ = self.rows[k]
+                launch = pp._launch(0, 0, 0, self.hermitian)  # pylint: disable=protected-access
+                _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.residual_desc), ctypes.byref(targets.desc), _ptr(targets.configs),
+                                                  targets.n, ctypes.c_void_p(self.inp.data_ptr() + 16 * lo),
+                                                  ctypes.c_void_p(self.out.data_ptr() + 16 * lo), mode,
+                                                  ctypes.byref(launch) if launch else None, stream), "qmb_apply_prepared")
+
+    def tune(self, retune: bool = False) -> int:
+        """Picks the number of persistent waves with the autotuner (runs the batch on the current
+        inputs; overwrites the outputs)."""
+        key = f"batch:n={len(self.instances)}:rows={self.n_rows}:units={self.plan.total_units}"
+        best = TUNER.best(key, [{"waves": w} for w in WAVE_CANDIDATES], lambda c: self.launch(pp.MODE_FORWARD, c["waves"]),
+                          retune=retune)
+        self.waves = int(best["waves"])
+        return self.waves
