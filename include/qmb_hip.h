@@ -246,8 +246,8 @@ int qmb_terms_build_residual(int n_qubytes, int particle_cut, const int16_t* sit
 
 /* Core index of the set indexed by a qmb_targets: one entry per (config, `order` occupied sites)
  * -- order 2: every occupied pair (cores of pair excitations), order 1: every occupied site
- * (cores of single excitations) -- radix-sorted by the 48-bit hash of the core, and the start of
- * every group of equal hashes.  max_occupied must bound the number of occupied sites of every
+ * (cores of single excitations) -- radix-sorted by a 36-bit hash of the core and, within a group
+ * of equal hashes, by the removed pair (site), and the start of every group.  max_occupied must bound the number of occupied sites of every
  * config (a row with more sets counts[2], and the apply calls then write NaN into out[0] instead
  * of a wrong result). */
 typedef struct qmb_cores {
@@ -257,7 +257,7 @@ typedef struct qmb_cores {
     int64_t capacity;  /* n * per_config entries */
     void* buffer;
     size_t bytes;
-    uint64_t* keys;    /* [capacity] sorted core hashes (unused entries last) */
+    uint64_t* keys;    /* [capacity] sorted keys: core hash << 20 | removed pair index (unused entries last) */
     uint64_t* vals;    /* [capacity] row << 16 | site1 << 8 | site0 of the removed sites (order 1: site0) */
     uint32_t* starts;  /* [capacity + 1] first entry of each group, then the end */
     int64_t* counts;   /* device: {entries, groups, overflow, 0} */

@@ -21,9 +21,12 @@ namespace qmb {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kKeyBits = 48;
-constexpr uint64_t kKeyMask = (1ull << kKeyBits) - 1ull;
-constexpr uint64_t kKeySentinel = kKeyMask;  // unused entries; sorts after every real key
+// Sort key of a core entry: 36 bits of the core hash above 20 bits of the removed pair (or site),
+// so a group of equal cores is a run of equal top bits and, inside it, its entries ascend by pair
+// -- the rows of a group then read the coefficient table along rows in order.
+constexpr int kKeyBits = 56;
+constexpr int kSubBits = 20;
+constexpr uint64_t kKeySentinel = (1ull << kKeyBits) - 1ull;  // unused entries; sorts after every real key
 constexpr int kMaxPairBytes = 16;            // the dense table is n_pairs^2 entries
 
 inline unsigned blocks_for(int64_t n, int block = kBlock) {
@@ -98,10 +101,14 @@ __device__ __forceinline__ void flip_site(Cfg<W>& c, int s) {
     }
 }
 
-__device__ __forceinline__ uint64_t core_key(uint64_t h) {
-    const uint64_t k = h & kKeyMask;
-    return k == kKeySentinel ? k - 1 : k;  // a merged hash value only makes its group "mixed"
+__device__ __forceinline__ uint64_t core_key(uint64_t h, uint64_t sub) {
+    constexpr uint64_t kTop = (1ull << (kKeyBits - kSubBits)) - 1ull;
+    uint64_t top = h & kTop;
+    top = top == kTop ? top - 1 : top;  // a merged hash value only makes its group "mixed"
+    return (top << kSubBits) | sub;
 }
+
+__device__ __forceinline__ uint64_t group_of(uint64_t key) { return key >> kSubBits; }
 
 // One thread per config: every occupied pair (x < y) -- ORDER 2 -- or every occupied site x --
 // ORDER 1 -- gives the entry (hash of the config without those sites, row).
@@ -125,7 +132,7 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
             const uint64_t hx = h ^ cols[x];
             if (ORDER == 1) {
                 if (k < per_config) {
-                    okeys[base + k] = core_key(hx);
+                    okeys[base + k] = core_key(hx, static_cast<uint64_t>(x));
                     ovals[base + k] = (static_cast<uint64_t>(i) << 16) | static_cast<uint64_t>(x);
                 }
                 ++k;
@@ -141,7 +148,7 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
                     const int y = 64 * wy + __builtin_ctzll(my);
                     my &= my - 1;
                     if (k < per_config) {
-                        okeys[base + k] = core_key(hx ^ cols[y]);
+                        okeys[base + k] = core_key(hx ^ cols[y], static_cast<uint64_t>(pair_index(x, y)));
                         ovals[base + k] = (static_cast<uint64_t>(i) << 16) | (static_cast<uint64_t>(y) << 8) |
                                           static_cast<uint64_t>(x);
                     }
@@ -165,7 +172,7 @@ __global__ void core_flag_kernel(const uint64_t* __restrict__ keys, int64_t cap,
         return;
     }
     const uint64_t k = keys[e];
-    flag[e] = (k != kKeySentinel && (e == 0 || keys[e - 1] != k)) ? 1u : 0u;
+    flag[e] = (k != kKeySentinel && (e == 0 || group_of(keys[e - 1]) != group_of(k))) ? 1u : 0u;
 }
 
 // starts[g] = first entry of group g; starts[G] = number of used entries; counts {entries, G}.
