@@ -387,21 +387,115 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
     }
 }
 
-// Walks the groups [g0, g0 + 64) of a core index that exist: the lanes test their group's size
-// and fn(s, e) runs, wave-uniform, on every group with at least two entries.
-template <typename F>
-__device__ __forceinline__ void walk_chunk(const uint32_t* __restrict__ starts, int64_t groups, int64_t g0, int lane, F&& fn) {
+// Groups of 2..8 entries, one size class at a time (gp = 8, 4, 2 lanes per row): one lane per
+// (row a, entry b) of a gp x gp block, 64 / gp^2 groups per pass.  Each lane checks its pair exactly
+// (cores compared through a shuffle from the lane holding row b), the rows are summed over b with a
+// butterfly inside the block, and the first lane of each row adds the sum.  `mask` holds the lanes
+// of the 64-group chunk whose group is in this class; (gs_l, ge_l) is that lane's group.
+template <int W, bool ADJ>
+__device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_l, uint32_t ge_l, int gp, int lane) {
+    const int lg = gp == 8 ? 3 : (gp == 4 ? 2 : 1);
+    const int bs = gp * gp;
+    const int per = 64 / bs;
+    const int blk = lane >> (2 * lg);
+    const int sub = lane & (bs - 1);
+    const int ai = sub >> lg;
+    const int bi = sub & (gp - 1);
+    const bool fermi = p.particle_cut == 1;
+    while (mask) {
+        uint32_t s = 0, e = 0;
+        for (int j = 0; j < per && mask != 0ull; ++j) {
+            const int l = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const uint32_t sj = __builtin_amdgcn_readlane(gs_l, l);
+            const uint32_t ej = __builtin_amdgcn_readlane(ge_l, l);
+            if (blk == j) {
+                s = sj;
+                e = ej;
+            }
+        }
+        const uint32_t g = e - s;
+        const bool rowa = static_cast<uint32_t>(ai) < g;
+        const bool rowb = static_cast<uint32_t>(bi) < g;
+        const uint64_t va = rowa ? p.vals[s + ai] : 0ull;
+        const uint64_t vb = rowb ? p.vals[s + bi] : 0ull;
+        const int64_t ia = static_cast<int64_t>(va >> 16);
+        const int r0 = static_cast<int>(va & 0xff), r1 = static_cast<int>((va >> 8) & 0xff);
+        const int q0 = static_cast<int>(vb & 0xff), q1 = static_cast<int>((vb >> 8) & 0xff);
+        Cfg<W> ca;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            ca.w[w] = rowa ? p.ckeys[ia * W + w] : 0ull;
+        }
+        Cfg<W> core = ca;
+        if (rowa) {
+            flip_site<W>(core, r0);
+            flip_site<W>(core, r1);
+        }
+        // the core of row b: computed by the lanes whose row is b
+        const int src = (blk << (2 * lg)) + (bi << lg);
+        bool same = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            same = same && shfl_u64(core.w[w], src) == core.w[w];
+        }
+        const bool ok = rowa && rowb && ai != bi && same && q0 != r0 && q0 != r1 && q1 != r0 && q1 != r1;
+        double acc_re = 0.0, acc_im = 0.0;
+        if (ok) {
+            const double2 gv = p.in[vb >> 16];
+            int64_t idx;
+            int sg;
+            if (ADJ) {
+                idx = pair_index(r0, r1) * p.n_pairs + pair_index(q0, q1);
+                sg = fermi ? pair_parity<W>(ca, r0, r1, q0, q1) : 0;
+            } else {
+                Cfg<W> cb = core;
+                flip_site<W>(cb, q0);
+                flip_site<W>(cb, q1);
+                idx = pair_index(q0, q1) * p.n_pairs + pair_index(r0, r1);
+                sg = fermi ? pair_parity<W>(cb, r0, r1, q0, q1) : 0;
+            }
+            const double2 x = p.coef[idx];
+            const double xr = sg ? -x.x : x.x;
+            const double xi = sg ? -x.y : x.y;
+            if (ADJ) {
+                acc_re = xr * gv.x + xi * gv.y;
+                acc_im = xr * gv.y - xi * gv.x;
+            } else {
+                acc_re = xr * gv.x - xi * gv.y;
+                acc_im = xr * gv.y + xi * gv.x;
+            }
+        }
+        for (int m = 1; m < gp; m <<= 1) {
+            acc_re += __shfl_xor(acc_re, m);
+            acc_im += __shfl_xor(acc_im, m);
+        }
+        if (bi == 0 && rowa && (acc_re != 0.0 || acc_im != 0.0)) {
+            unsafeAtomicAdd(&p.out[ia].x, acc_re);
+            unsafeAtomicAdd(&p.out[ia].y, acc_im);
+        }
+    }
+}
+
+// The pair work of the groups [g0, g0 + 64): groups of up to 8 entries through the tiny-group
+// classes, larger ones a row block at a time (pair_group).
+template <int W, bool ADJ>
+__device__ __forceinline__ void pair_chunk(const PairParams& p, int64_t groups, int64_t g0, int lane) {
     const int64_t gg = g0 + lane;
     uint32_t gs = 0, ge = 0;
     if (gg < groups) {
-        gs = starts[gg];
-        ge = starts[gg + 1];
+        gs = p.starts[gg];
+        ge = p.starts[gg + 1];
     }
-    uint64_t multi = __ballot(gg < groups && ge - gs >= 2u);
-    while (multi) {
-        const int l = __builtin_ctzll(multi);
-        multi &= multi - 1;
-        fn(__shfl(gs, l), __shfl(ge, l));
+    const uint32_t sz = ge - gs;
+    pair_tiny_class<W, ADJ>(p, __ballot(sz > 4u && sz <= 8u), gs, ge, 8, lane);
+    pair_tiny_class<W, ADJ>(p, __ballot(sz > 2u && sz <= 4u), gs, ge, 4, lane);
+    pair_tiny_class<W, ADJ>(p, __ballot(sz == 2u), gs, ge, 2, lane);
+    uint64_t big = __ballot(sz > 8u);
+    while (big) {
+        const int l = __builtin_ctzll(big);
+        big &= big - 1;
+        pair_group<W, ADJ>(p, __shfl(gs, l), __shfl(ge, l), lane);
     }
 }
 
@@ -425,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(PairParams p) {
     }
     const int64_t groups = p.counts[1];
     for (int64_t g0 = wave * 64; g0 < groups; g0 += nwaves * 64) {
-        walk_chunk(p.starts, groups, g0, lane, [&](uint32_t s, uint32_t e) { pair_group<W, ADJ>(p, s, e, lane); });
+        pair_chunk<W, ADJ>(p, groups, g0, lane);
     }
 }
 
@@ -600,6 +694,108 @@ __device__ void single_group(const SingleParams& p, uint32_t s, uint32_t e, int 
     }
 }
 
+// Single-excitation groups of 2..8 entries: the lane-per-(row, entry) scheme of pair_tiny_class.
+template <int W, bool ADJ>
+__device__ void single_tiny_class(const SingleParams& p, uint64_t mask, uint32_t gs_l, uint32_t ge_l, int gp, int lane) {
+    const int lg = gp == 8 ? 3 : (gp == 4 ? 2 : 1);
+    const int bs = gp * gp;
+    const int per = 64 / bs;
+    const int blk = lane >> (2 * lg);
+    const int sub = lane & (bs - 1);
+    const int ai = sub >> lg;
+    const int bi = sub & (gp - 1);
+    const bool fermi = p.particle_cut == 1;
+    while (mask) {
+        uint32_t s = 0, e = 0;
+        for (int j = 0; j < per && mask != 0ull; ++j) {
+            const int l = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const uint32_t sj = __builtin_amdgcn_readlane(gs_l, l);
+            const uint32_t ej = __builtin_amdgcn_readlane(ge_l, l);
+            if (blk == j) {
+                s = sj;
+                e = ej;
+            }
+        }
+        const uint32_t g = e - s;
+        const bool rowa = static_cast<uint32_t>(ai) < g;
+        const bool rowb = static_cast<uint32_t>(bi) < g;
+        const uint64_t va = rowa ? p.vals[s + ai] : 0ull;
+        const uint64_t vb = rowb ? p.vals[s + bi] : 0ull;
+        const int64_t ia = static_cast<int64_t>(va >> 16);
+        const int r = static_cast<int>(va & 0xff);
+        const int q = static_cast<int>(vb & 0xff);
+        Cfg<W> ca;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            ca.w[w] = rowa ? p.ckeys[ia * W + w] : 0ull;
+        }
+        Cfg<W> core = ca;
+        if (rowa) {
+            flip_site<W>(core, r);
+        }
+        const int src = (blk << (2 * lg)) + (bi << lg);
+        bool same = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            same = same && shfl_u64(core.w[w], src) == core.w[w];
+        }
+        const bool ok = rowa && rowb && ai != bi && same && q != r;
+        double acc_re = 0.0, acc_im = 0.0;
+        if (ok) {
+            const double2 gv = p.in[vb >> 16];
+            double2 x;
+            int sg;
+            if (ADJ) {
+                x = single_element<W>(p, core, r, q);
+                sg = fermi ? single_parity<W>(ca, r, q) : 0;
+            } else {
+                Cfg<W> cb = core;
+                flip_site<W>(cb, q);
+                x = single_element<W>(p, core, q, r);
+                sg = fermi ? single_parity<W>(cb, q, r) : 0;
+            }
+            const double xr = sg ? -x.x : x.x;
+            const double xi = sg ? -x.y : x.y;
+            if (ADJ) {
+                acc_re = xr * gv.x + xi * gv.y;
+                acc_im = xr * gv.y - xi * gv.x;
+            } else {
+                acc_re = xr * gv.x - xi * gv.y;
+                acc_im = xr * gv.y + xi * gv.x;
+            }
+        }
+        for (int m = 1; m < gp; m <<= 1) {
+            acc_re += __shfl_xor(acc_re, m);
+            acc_im += __shfl_xor(acc_im, m);
+        }
+        if (bi == 0 && rowa && (acc_re != 0.0 || acc_im != 0.0)) {
+            unsafeAtomicAdd(&p.out[ia].x, acc_re);
+            unsafeAtomicAdd(&p.out[ia].y, acc_im);
+        }
+    }
+}
+
+template <int W, bool ADJ>
+__device__ __forceinline__ void single_chunk(const SingleParams& p, int64_t groups, int64_t g0, int lane) {
+    const int64_t gg = g0 + lane;
+    uint32_t gs = 0, ge = 0;
+    if (gg < groups) {
+        gs = p.starts[gg];
+        ge = p.starts[gg + 1];
+    }
+    const uint32_t sz = ge - gs;
+    single_tiny_class<W, ADJ>(p, __ballot(sz > 4u && sz <= 8u), gs, ge, 8, lane);
+    single_tiny_class<W, ADJ>(p, __ballot(sz > 2u && sz <= 4u), gs, ge, 4, lane);
+    single_tiny_class<W, ADJ>(p, __ballot(sz == 2u), gs, ge, 2, lane);
+    uint64_t big = __ballot(sz > 8u);
+    while (big) {
+        const int l = __builtin_ctzll(big);
+        big &= big - 1;
+        single_group<W, ADJ>(p, __shfl(gs, l), __shfl(ge, l), lane);
+    }
+}
+
 template <int W, bool ADJ>
 __global__ __launch_bounds__(kBlock) void single_kernel(SingleParams p) {
     const int lane = threadIdx.x & 63;
@@ -613,7 +809,7 @@ __global__ __launch_bounds__(kBlock) void single_kernel(SingleParams p) {
     }
     const int64_t groups = p.counts[1];
     for (int64_t g0 = wave * 64; g0 < groups; g0 += nwaves * 64) {
-        walk_chunk(p.starts, groups, g0, lane, [&](uint32_t s, uint32_t e) { single_group<W, ADJ>(p, s, e, lane); });
+        single_chunk<W, ADJ>(p, groups, g0, lane);
     }
 }
 
@@ -748,7 +944,7 @@ __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int 
         p.out = it.out;
         const int64_t groups = it.pcounts[1];
         if (local * 64 < groups && it.pcounts[2] == 0) {
-            walk_chunk(p.starts, groups, local * 64, lane, [&](uint32_t s, uint32_t e) { pair_group<W, ADJ>(p, s, e, lane); });
+            pair_chunk<W, ADJ>(p, groups, local * 64, lane);
         }
     } else if (type == 1) {
         SingleParams p{};
@@ -764,7 +960,7 @@ __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int 
         p.out = it.out;
         const int64_t groups = it.scounts[1];
         if (local * 64 < groups && it.scounts[2] == 0) {
-            walk_chunk(p.starts, groups, local * 64, lane, [&](uint32_t s, uint32_t e) { single_group<W, ADJ>(p, s, e, lane); });
+            single_chunk<W, ADJ>(p, groups, local * 64, lane);
         }
     } else {
         diag_rows<W, ADJ>(it, local * 64, lane);
