@@ -388,8 +388,8 @@ def main() -> int:
         if terms.singles_desc is not None:
             parts["single_kernel_ms"] = timed(lambda: lib.qmb_apply_singles(ctypes.byref(terms.singles_desc), ctypes.byref(cores1),
                                                                             ctypes.byref(targets.desc), vin, vout, pmode, s_ptr))
-        parts["diagonal_kernel_ms"] = timed(lambda: lib.qmb_apply_diagonal(ctypes.byref(terms.desc), ctypes.byref(targets.desc), vin,
-                                                                           vout, pmode, s_ptr))
+        parts["diagonal_kernel_ms"] = timed(lambda: lib.qmb_apply_diagonal(ctypes.byref(terms.pairs_desc), ctypes.byref(targets.desc),
+                                                                           vin, vout, pmode, s_ptr))
         if terms.n_residual_terms > 0:
             launch = pp._launch(args.tile, args.groups, 0, symmetric)  # pylint: disable=protected-access
             parts["enumeration_kernel_ms"] = timed(lambda: lib.qmb_apply_prepared(

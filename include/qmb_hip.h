@@ -207,14 +207,18 @@ int qmb_apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const
 
 /* Merged coefficients of the pair-excitation terms, dense over (annihilated pair, created pair):
  * coef[pair_index(r0,r1) * n_pairs + pair_index(c0,c1)] = sum over terms of (-1)^delta_t coef_t,
- * pair_index(a<b) = b(b-1)/2 + a over the 8*n_qubytes sites.  n_qubytes <= 16. */
+ * pair_index(a<b) = b(b-1)/2 + a over the 8*n_qubytes sites.  n_qubytes <= 16.
+ * Also the diagonal terms, folded exactly (a diagonal term conditions at most two sites and has a
+ * constant sign) into diag = {constant, per site p, per site pair (p, q)}: the diagonal element of
+ * a config c is diag[0] + sum_{p in c} diag[1 + p] + sum_{p<q in c} diag[1 + n_bits + pair_index(p, q)]. */
 typedef struct qmb_pairs {
     int32_t n_qubytes, particle_cut, n_bits, reserved;
     int64_t n_pairs;
     void* buffer;
     size_t bytes;
     double* coef;      /* [n_pairs * n_pairs][2] */
-    int64_t* counts;   /* device: {pair-excitation terms, 0, 0, 0} */
+    double* diag;      /* [1 + n_bits + n_pairs][2] */
+    int64_t* counts;   /* device: {pair-excitation terms, diagonal terms, 0, 0} */
 } qmb_pairs;
 
 int qmb_pairs_workspace_size(int n_qubytes, size_t* bytes);
@@ -278,8 +282,8 @@ int qmb_apply_singles(const qmb_singles* singles, const qmb_cores* cores, const 
                       double* out, int mode, void* stream);
 
 /* The diagonal part on the set indexed by `targets`: out[i] += d_i in[i] (mode 0) or
- * conj(d_i) in[i] (mode 1), d_i = sum of the diagonal terms on row i in term order. */
-int qmb_apply_diagonal(const qmb_terms* terms, const qmb_targets* targets, const double* in, double* out, int mode, void* stream);
+ * conj(d_i) in[i] (mode 1), d_i from the folded diagonal table of `pairs`. */
+int qmb_apply_diagonal(const qmb_pairs* pairs, const qmb_targets* targets, const double* in, double* out, int mode, void* stream);
 
 /* ---------------- batched persistent launch (no reference counterpart) -------------------------
  * Many instances -- a term table and a config set each, of any sizes -- applied in ONE persistent
@@ -290,8 +294,8 @@ int qmb_apply_diagonal(const qmb_terms* terms, const qmb_targets* targets, const
  * An instance whose table has terms outside those classes (its residual table) needs that part
  * applied separately. */
 typedef struct qmb_batch_item {
-    const qmb_terms* terms;      /* full table: its diagonal records are applied */
-    const qmb_pairs* pairs;      /* pair-excitation table or NULL */
+    const qmb_terms* terms;      /* the full table (kept for the caller's residual part; unused here) */
+    const qmb_pairs* pairs;      /* pair-excitation and diagonal tables (required) */
     const qmb_singles* singles;  /* single-excitation table or NULL */
     const qmb_targets* targets;  /* index of the instance's config set (n_qubytes <= 16) */
     const qmb_cores* cores2;     /* order-2 core index of that set (with pairs) */

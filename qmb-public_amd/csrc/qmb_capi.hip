@@ -405,12 +405,12 @@ int qmb_apply_singles(const qmb_singles* singles, const qmb_cores* cores, const 
     return apply_singles(singles, cores, targets, in, out, mode, as_stream(stream));
 }
 
-int qmb_apply_diagonal(const qmb_terms* terms, const qmb_targets* targets, const double* in, double* out, int mode, void* stream) {
+int qmb_apply_diagonal(const qmb_pairs* pairs, const qmb_targets* targets, const double* in, double* out, int mode, void* stream) {
     clear_error();
     if (targets && targets->n > 0 && (!in || !out)) {
         return set_error(QMB_ERR_ARGUMENT, "in and out must be non-null");
     }
-    return apply_diagonal(terms, targets, in, out, mode, as_stream(stream));
+    return apply_diagonal(pairs, targets, in, out, mode, as_stream(stream));
 }
 
 int qmb_batch_workspace_size(int n_items, size_t* bytes) {

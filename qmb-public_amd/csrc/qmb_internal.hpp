@@ -153,7 +153,7 @@ int apply_pairs(const qmb_pairs* pairs, const qmb_cores* cores, const qmb_target
                 int mode, hipStream_t stream);
 int apply_singles(const qmb_singles* singles, const qmb_cores* cores, const qmb_targets* targets, const double* in, double* out,
                   int mode, hipStream_t stream);
-int apply_diagonal(const qmb_terms* terms, const qmb_targets* targets, const double* in, double* out, int mode,
+int apply_diagonal(const qmb_pairs* pairs, const qmb_targets* targets, const double* in, double* out, int mode,
                    hipStream_t stream);
 size_t batch_bytes(int n_items);
 int batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t bytes, qmb_batch* out, hipStream_t stream);

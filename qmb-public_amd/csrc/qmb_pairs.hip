@@ -41,14 +41,75 @@ __global__ void zero_counts_kernel(int64_t* counts) {
 }
 
 // ------------------------------------------------------------------------------ coefficients --
+// A diagonal term needs every site it touches back in its first state, so each of its sites carries
+// an even number of its (at most four) operators: it conditions at most two sites, and its sign is
+// the constant (-1)^pconst.  Its indicator prod_s (v_s ? n_s : 1 - n_s) expands into a constant,
+// per-site and per-site-pair part: diag[0], diag[1 + p], diag[1 + n_bits + pair_index(p, q)].
+__device__ __forceinline__ void add2(double2* d, double sgn, const double2& v) {
+    unsafeAtomicAdd(&d->x, sgn * v.x);
+    unsafeAtomicAdd(&d->y, sgn * v.y);
+}
+
+__device__ void fold_diagonal(uint64_t rec, int particle_cut, int n_bits, const double2& coef, double2* diag) {
+    const uint32_t valid = static_cast<uint32_t>(rec >> 32) & 0xf;
+    const uint32_t req = static_cast<uint32_t>(rec >> 36) & 0xf;
+    const int pconst = particle_cut == 1 ? static_cast<int>((rec >> 48) & 1) : 0;
+    const double2 v = pconst ? make_double2(-coef.x, -coef.y) : coef;
+    int sites[2], occ[2], ns = 0;
+    for (int u = 0; u < 4; ++u) {
+        if ((valid >> u) & 1u) {
+            if (ns < 2) {
+                sites[ns] = static_cast<int>((rec >> (8 * u)) & 0xff);
+                occ[ns] = static_cast<int>((req >> u) & 1u);
+            }
+            ++ns;
+        }
+    }
+    double2* e1 = diag + 1;
+    double2* e2 = diag + 1 + n_bits;
+    if (ns == 0) {
+        add2(diag, 1.0, v);
+    } else if (ns == 1) {
+        if (occ[0]) {
+            add2(e1 + sites[0], 1.0, v);
+        } else {
+            add2(diag, 1.0, v);
+            add2(e1 + sites[0], -1.0, v);
+        }
+    } else if (ns == 2) {
+        const int p = sites[0] < sites[1] ? sites[0] : sites[1];
+        const int q = sites[0] < sites[1] ? sites[1] : sites[0];
+        const int op = sites[0] < sites[1] ? occ[0] : occ[1];
+        const int oq = sites[0] < sites[1] ? occ[1] : occ[0];
+        double2* pq = e2 + pair_index(p, q);
+        // (op ? n_p : 1 - n_p) (oq ? n_q : 1 - n_q)
+        add2(pq, (op == oq) ? 1.0 : -1.0, v);
+        if (!oq) {
+            add2(e1 + p, op ? 1.0 : -1.0, v);
+        }
+        if (!op) {
+            add2(e1 + q, oq ? 1.0 : -1.0, v);
+        }
+        if (!op && !oq) {
+            add2(diag, 1.0, v);
+        }
+    }
+}
+
 __global__ void pairs_fill_kernel(const int16_t* __restrict__ site, const uint8_t* __restrict__ kind,
                                   const double2* __restrict__ coef, int64_t n_terms, int particle_cut, int n_bits,
-                                  int64_t n_pairs, double2* __restrict__ table, int64_t* __restrict__ counts) {
+                                  int64_t n_pairs, double2* __restrict__ table, double2* __restrict__ diag,
+                                  int64_t* __restrict__ counts) {
     const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (t >= n_terms) {
         return;
     }
     const CompiledTerm c = compile_term(site + 4 * t, kind + 4 * t, particle_cut, n_bits);
+    if (c.status == kCompileOk && c.diagonal) {
+        fold_diagonal(c.rec, particle_cut, n_bits, coef[t], diag);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&counts[1]), 1ull);
+        return;
+    }
     PairExcitation e;
     if (c.status != kCompileOk || !pair_excitation(c.rec, particle_cut, &e)) {
         return;
@@ -73,6 +134,7 @@ int pairs_layout(Carver& cv, int n_qubytes, qmb_pairs* out) {
     p.buffer = cv.base;
     p.counts = cv.take<int64_t>(4);
     p.coef = cv.take<double>(2 * static_cast<size_t>(p.n_pairs) * static_cast<size_t>(p.n_pairs));
+    p.diag = cv.take<double>(2 * (1 + static_cast<size_t>(p.n_bits) + static_cast<size_t>(p.n_pairs)));
     p.bytes = cv.off;
     if (out) {
         *out = p;
@@ -814,54 +876,55 @@ __global__ __launch_bounds__(kBlock) void single_kernel(SingleParams p) {
 }
 
 // ------------------------------------------------------------------------------- diagonal --
-constexpr int kDiagTile = 1024;
-
-// out[i] += d_i in[i] (conj(d_i) for ADJ), d_i = the diagonal terms on row i summed in term order.
-template <int W, bool ADJ>
-__global__ __launch_bounds__(kBlock) void diag_apply_kernel(const uint64_t* __restrict__ ckeys, int64_t n,
-                                                            const uint64_t* __restrict__ drecs, const double2* __restrict__ dcoefs,
-                                                            const int64_t* __restrict__ counts, const double2* __restrict__ in,
-                                                            double2* __restrict__ out) {
-    __shared__ uint64_t s_rec[kDiagTile];
-    __shared__ double2 s_coef[kDiagTile];
-    const int64_t i = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x;
-    const bool valid = i < n;
-    Cfg<W> c;
-    if (valid) {
-        c = load_key<W>(ckeys, i);
-    } else {
+// d(c) = diag[0] + sum_{p in c} diag[1 + p] + sum_{p < q in c} diag[1 + n_bits + pair_index(p, q)]
+template <int W>
+__device__ __forceinline__ double2 diag_value(const Cfg<W>& c, const double2* __restrict__ diag, int n_bits) {
+    double2 d = diag[0];
+    const double2* e1 = diag + 1;
+    const double2* e2 = diag + 1 + n_bits;
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-            c.w[w] = 0;
-        }
-    }
-    const int64_t td = counts[1];
-    double re = 0.0, im = 0.0;
-    for (int64_t base = 0; base < td; base += kDiagTile) {
-        const int cnt = static_cast<int>(td - base < kDiagTile ? td - base : kDiagTile);
-        __syncthreads();
-        for (int u = threadIdx.x; u < cnt; u += kBlock) {
-            s_rec[u] = drecs[base + u];
-            s_coef[u] = dcoefs[base + u];
-        }
-        __syncthreads();
-        for (int u = 0; u < cnt; ++u) {
-            Cfg<W> o;
-            int par;
-            if (apply_rec<W>(s_rec[u], c, o, par)) {
-                const double2 cf = s_coef[u];
-                re += par ? -cf.x : cf.x;
-                im += par ? -cf.y : cf.y;
+    for (int wp = 0; wp < W; ++wp) {
+        uint64_t mp = c.w[wp];
+        while (mp) {
+            const int p = 64 * wp + __builtin_ctzll(mp);
+            mp &= mp - 1;
+            const double2 a = e1[p];
+            d.x += a.x;
+            d.y += a.y;
+#pragma unroll
+            for (int wq = 0; wq < W; ++wq) {
+                if (wq < wp) {
+                    continue;
+                }
+                uint64_t mq = wq == wp ? mp : c.w[wq];
+                while (mq) {
+                    const int q = 64 * wq + __builtin_ctzll(mq);
+                    mq &= mq - 1;
+                    const double2 b = e2[pair_index(p, q)];
+                    d.x += b.x;
+                    d.y += b.y;
+                }
             }
         }
     }
-    if (valid && (re != 0.0 || im != 0.0)) {
+    return d;
+}
+
+// out[i] += d_i in[i] (conj(d_i) for ADJ)
+template <int W, bool ADJ>
+__global__ __launch_bounds__(kBlock) void diag_apply_kernel(const uint64_t* __restrict__ ckeys, int64_t n,
+                                                            const double2* __restrict__ diag, int n_bits,
+                                                            const double2* __restrict__ in, double2* __restrict__ out) {
+    const int64_t i = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const double2 d = diag_value<W>(load_key<W>(ckeys, i), diag, n_bits);
+    const double im = ADJ ? -d.y : d.y;
+    if (d.x != 0.0 || im != 0.0) {
         const double2 g = in[i];
-        if (ADJ) {
-            im = -im;
-        }
-        out[i].x += re * g.x - im * g.y;
-        out[i].y += re * g.y + im * g.x;
+        out[i].x += d.x * g.x - im * g.y;
+        out[i].y += d.x * g.y + im * g.x;
     }
 }
 
@@ -875,9 +938,7 @@ struct BatchItemDev {
     int32_t n_bits, reserved;
     int64_t n;
     const uint64_t* ckeys;
-    const uint64_t* drecs;
-    const double2* dcoefs;
-    const int64_t* tcounts;
+    const double2* diag;
     const double2* pcoef;
     int64_t n_pairs;
     const uint64_t* pvals;
@@ -897,35 +958,16 @@ constexpr int kBatchMaxItems = 4096;
 template <int W, bool ADJ>
 __device__ void diag_rows(const BatchItemDev& it, int64_t r0, int lane) {
     const int64_t i = r0 + lane;
-    const bool valid = i < it.n;
-    Cfg<W> c;
-    if (valid) {
-        c = load_key<W>(it.ckeys, i);
-    } else {
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-            c.w[w] = 0;
-        }
+    if (i >= it.n) {
+        return;
     }
-    const int64_t td = it.tcounts[1];
-    double re = 0.0, im = 0.0;
-    for (int64_t u = 0; u < td; ++u) {
-        Cfg<W> o;
-        int par;
-        if (apply_rec<W>(it.drecs[u], c, o, par)) {
-            const double2 cf = it.dcoefs[u];
-            re += par ? -cf.x : cf.x;
-            im += par ? -cf.y : cf.y;
-        }
-    }
-    if (valid && (re != 0.0 || im != 0.0)) {
+    const double2 d = diag_value<W>(load_key<W>(it.ckeys, i), it.diag, it.n_bits);
+    const double im = ADJ ? -d.y : d.y;
+    if (d.x != 0.0 || im != 0.0) {
         const double2 g = it.in[i];
-        if (ADJ) {
-            im = -im;
-        }
         // pair and single units of the same launch add into these rows concurrently
-        unsafeAtomicAdd(&it.out[i].x, re * g.x - im * g.y);
-        unsafeAtomicAdd(&it.out[i].y, re * g.y + im * g.x);
+        unsafeAtomicAdd(&it.out[i].x, d.x * g.x - im * g.y);
+        unsafeAtomicAdd(&it.out[i].y, d.x * g.y + im * g.x);
     }
 }
 
@@ -1023,10 +1065,12 @@ int pairs_build(int n_qubytes, int particle_cut, const int16_t* site, const uint
     zero_counts_kernel<<<1, 64, 0, stream>>>(p.counts);
     QMB_CHECK_LAUNCH();
     QMB_TRY(launch_zero_f64(p.coef, 2 * p.n_pairs * p.n_pairs, stream));
+    QMB_TRY(launch_zero_f64(p.diag, 2 * (1 + p.n_bits + p.n_pairs), stream));
     if (n_terms > 0) {
         pairs_fill_kernel<<<blocks_for(n_terms), kBlock, 0, stream>>>(site, kind, reinterpret_cast<const double2*>(coef), n_terms,
                                                                       particle_cut, p.n_bits, p.n_pairs,
-                                                                      reinterpret_cast<double2*>(p.coef), p.counts);
+                                                                      reinterpret_cast<double2*>(p.coef),
+                                                                      reinterpret_cast<double2*>(p.diag), p.counts);
         QMB_CHECK_LAUNCH();
     }
     *out = p;
@@ -1213,33 +1257,30 @@ int apply_singles(const qmb_singles* singles, const qmb_cores* cores, const qmb_
     return set_error(QMB_ERR_UNSUPPORTED, "single-excitation path supports n_qubytes <= %d", kMaxPairBytes);
 }
 
-int apply_diagonal(const qmb_terms* terms, const qmb_targets* targets, const double* in, double* out, int mode,
+int apply_diagonal(const qmb_pairs* pairs, const qmb_targets* targets, const double* in, double* out, int mode,
                    hipStream_t stream) {
-    if (!terms || !targets) {
-        return set_error(QMB_ERR_ARGUMENT, "terms and targets must be non-null");
+    if (!pairs || !targets) {
+        return set_error(QMB_ERR_ARGUMENT, "pairs and targets must be non-null");
     }
-    if (terms->n_qubytes != targets->n_qubytes) {
-        return set_error(QMB_ERR_ARGUMENT, "terms built for %d bytes, targets for %d", terms->n_qubytes, targets->n_qubytes);
+    if (pairs->n_qubytes != targets->n_qubytes) {
+        return set_error(QMB_ERR_ARGUMENT, "pairs built for %d bytes, targets for %d", pairs->n_qubytes, targets->n_qubytes);
     }
     if (mode != 0 && mode != 1) {
         return set_error(QMB_ERR_ARGUMENT, "mode must be 0 (forward) or 1 (adjoint)");
     }
-    if (targets->n == 0 || terms->n_terms == 0) {
+    if (targets->n == 0) {
         return QMB_OK;
     }
     const unsigned grid = blocks_for(targets->n);
-    const auto* dc = reinterpret_cast<const double2*>(terms->dcoefs);
+    const auto* dg = reinterpret_cast<const double2*>(pairs->diag);
     const auto* vi = reinterpret_cast<const double2*>(in);
     auto* vo = reinterpret_cast<double2*>(out);
+    const int nb = pairs->n_bits;
     switch (targets->n_words * 2 + mode) {
-        case 2: diag_apply_kernel<1, false><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
-        case 3: diag_apply_kernel<1, true><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
-        case 4: diag_apply_kernel<2, false><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
-        case 5: diag_apply_kernel<2, true><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
-        case 6: diag_apply_kernel<3, false><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
-        case 7: diag_apply_kernel<3, true><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
-        case 8: diag_apply_kernel<4, false><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
-        default: diag_apply_kernel<4, true><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo); break;
+        case 2: diag_apply_kernel<1, false><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, dg, nb, vi, vo); break;
+        case 3: diag_apply_kernel<1, true><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, dg, nb, vi, vo); break;
+        case 4: diag_apply_kernel<2, false><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, dg, nb, vi, vo); break;
+        default: diag_apply_kernel<2, true><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, dg, nb, vi, vo); break;
     }
     QMB_CHECK_LAUNCH();
     return QMB_OK;
@@ -1278,12 +1319,12 @@ int batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t by
     std::vector<int64_t> units(3 * static_cast<size_t>(n_items), 0);
     for (int k = 0; k < n_items; ++k) {
         const qmb_batch_item& x = items[k];
-        if (!x.terms || !x.targets) {
-            return set_error(QMB_ERR_ARGUMENT, "item %d: terms and targets must be non-null", k);
+        if (!x.pairs || !x.targets) {
+            return set_error(QMB_ERR_ARGUMENT, "item %d: pairs and targets must be non-null", k);
         }
         const qmb_targets& tg = *x.targets;
-        if (x.terms->n_qubytes != tg.n_qubytes || tg.n_words > 2) {
-            return set_error(QMB_ERR_ARGUMENT, "item %d: terms / targets shapes differ or n_qubytes > 16", k);
+        if (x.pairs->n_qubytes != tg.n_qubytes || tg.n_words > 2) {
+            return set_error(QMB_ERR_ARGUMENT, "item %d: pairs / targets shapes differ or n_qubytes > 16", k);
         }
         if (tg.n > 0 && (!x.in || !x.out)) {
             return set_error(QMB_ERR_ARGUMENT, "item %d: in and out must be non-null", k);
@@ -1294,13 +1335,11 @@ int batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t by
         }
         BatchItemDev d{};
         d.n_words = tg.n_words;
-        d.particle_cut = x.terms->particle_cut;
+        d.particle_cut = x.pairs->particle_cut;
         d.n_bits = 8 * tg.n_qubytes;
         d.n = tg.n;
         d.ckeys = tg.keys;
-        d.drecs = x.terms->drecs;
-        d.dcoefs = reinterpret_cast<const double2*>(x.terms->dcoefs);
-        d.tcounts = x.terms->counts;
+        d.diag = reinterpret_cast<const double2*>(x.pairs->diag);
         if (x.pairs) {
             d.has_pairs = 1;
             d.pcoef = reinterpret_cast<const double2*>(x.pairs->coef);

@@ -78,6 +78,7 @@ class QmbPairs(ctypes.Structure):
         ("buffer", _vp),
         ("bytes", _c_size),
         ("coef", _vp),
+        ("diag", _vp),
         ("counts", _vp),
     ]
 
@@ -204,7 +205,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
         _c_int,
         [ctypes.POINTER(QmbSingles), ctypes.POINTER(QmbCores), ctypes.POINTER(QmbTargets), _vp, _vp, _c_int, _vp],
     ),
-    "qmb_apply_diagonal": (_c_int, [ctypes.POINTER(QmbTerms), ctypes.POINTER(QmbTargets), _vp, _vp, _c_int, _vp]),
+    "qmb_apply_diagonal": (_c_int, [ctypes.POINTER(QmbPairs), ctypes.POINTER(QmbTargets), _vp, _vp, _c_int, _vp]),
     "qmb_batch_workspace_size": (_c_int, [_c_int, ctypes.POINTER(_c_size)]),
     "qmb_batch_plan": (_c_int, [ctypes.POINTER(QmbBatchItem), _c_int, _vp, _c_size, ctypes.POINTER(QmbBatch), _vp]),
     "qmb_apply_batch": (_c_int, [ctypes.POINTER(QmbBatch), _c_int, _c_int, _vp]),

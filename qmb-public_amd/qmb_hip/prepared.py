@@ -244,8 +244,8 @@ def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tenso
             _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.residual_desc), ctypes.byref(targets.desc), _ptr(configs_i), n_i,
                                               _ptr(vec), _ptr(out), mode, ctypes.byref(launch) if launch else None, stream),
                        "qmb_apply_prepared")
-        _lib.check(lib.qmb_apply_diagonal(ctypes.byref(terms.desc), ctypes.byref(targets.desc), _ptr(vec), _ptr(out), pmode, stream),
-                   "qmb_apply_diagonal")
+        _lib.check(lib.qmb_apply_diagonal(ctypes.byref(terms.pairs_desc), ctypes.byref(targets.desc), _ptr(vec), _ptr(out), pmode,
+                                          stream), "qmb_apply_diagonal")
         if terms.singles_desc is not None:
             _lib.check(lib.qmb_apply_singles(ctypes.byref(terms.singles_desc), ctypes.byref(targets.cores(1)), ctypes.byref(targets.desc),
                                              _ptr(vec), _ptr(out), pmode, stream), "qmb_apply_singles")

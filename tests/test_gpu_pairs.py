@@ -155,6 +155,8 @@ def test_structured_split_is_exact(cuda_device: torch.device, name: str, pc: int
     terms = pp.PreparedTerms(_dev(s, cuda_device), _dev(k, cuda_device), _dev(c, cuda_device), nq, pc)
     full_valid, full_diag, _ = terms.counts()
     assert terms.n_residual_terms + terms.n_pair_terms + terms.n_single_terms + full_diag == full_valid, name
+    off = int(terms.pairs_desc.counts) - terms.pairs_buffer.data_ptr()
+    assert int(terms.pairs_buffer[off + 8:off + 16].view(torch.int64).item()) == full_diag  # every diagonal term folded
     lib = _lib.library()
     out = (ctypes.c_int32 * 5)()
     n_pairs = n_singles = 0
