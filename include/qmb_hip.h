@@ -206,8 +206,10 @@ int qmb_apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const
  * by the qmb_targets the cores were built from, rows distinct). */
 
 /* Merged coefficients of the pair-excitation terms, dense over (annihilated pair, created pair):
- * coef[pair_index(r0,r1) * n_pairs + pair_index(c0,c1)] = sum over terms of (-1)^delta_t coef_t,
- * pair_index(a<b) = b(b-1)/2 + a over the 8*n_qubytes sites.  n_qubytes <= 16.
+ * coef[pair_slot(r0,r1) * n_pairs + pair_slot(c0,c1)] = sum over terms of (-1)^delta_t coef_t.
+ * pair_slot numbers the pairs (a<b) of the n_bits = 8*n_qubytes sites in three blocks by site
+ * parity: even-even pairs (index (b/2)((b/2)-1)/2 + a/2), then odd-odd, then even-odd
+ * (2 C(n_bits/2, 2) + (even site / 2) (n_bits / 2) + odd site / 2).  n_qubytes <= 16.
  * Also the diagonal terms, folded exactly (a diagonal term conditions at most two sites and has a
  * constant sign) into diag = {constant, per site p, per site pair (p, q)}: the diagonal element of
  * a config c is diag[0] + sum_{p in c} diag[1 + p] + sum_{p<q in c} diag[1 + n_bits + pair_index(p, q)]. */
@@ -250,8 +252,9 @@ int qmb_terms_build_residual(int n_qubytes, int particle_cut, const int16_t* sit
 
 /* Core index of the set indexed by a qmb_targets: one entry per (config, `order` occupied sites)
  * -- order 2: every occupied pair (cores of pair excitations), order 1: every occupied site
- * (cores of single excitations) -- radix-sorted by a 36-bit hash of the core and, within a group
- * of equal hashes, by the removed pair (site), and the start of every group.  max_occupied must bound the number of occupied sites of every
+ * (cores of single excitations) -- radix-sorted by a class of the core (even sites mod 3), a
+ * 34-bit hash of the core and, within a group of equal hashes, the removed pair (site); and the
+ * start of every group.  max_occupied must bound the number of occupied sites of every
  * config (a row with more sets counts[2], and the apply calls then write NaN into out[0] instead
  * of a wrong result). */
 typedef struct qmb_cores {
@@ -261,7 +264,8 @@ typedef struct qmb_cores {
     int64_t capacity;  /* n * per_config entries */
     void* buffer;
     size_t bytes;
-    uint64_t* keys;    /* [capacity] sorted keys: core hash << 20 | removed pair index (unused entries last) */
+    uint64_t* keys;    /* [capacity] sorted keys: (even sites of the core mod 3) << 54 | 34-bit core hash << 20 |
+                          removed pair slot (order 1: removed site); unused entries last */
     uint64_t* vals;    /* [capacity] row << 16 | site1 << 8 | site0 of the removed sites (order 1: site0) */
     uint32_t* starts;  /* [capacity + 1] first entry of each group, then the end */
     int64_t* counts;   /* device: {entries, groups, overflow, 0} */

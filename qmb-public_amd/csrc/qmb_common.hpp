@@ -211,6 +211,21 @@ struct PairExcitation {
 // Index of the unordered site pair (a < b) among all pairs of n sites: b (b - 1) / 2 + a.
 __host__ __device__ __forceinline__ int64_t pair_index(int a, int b) { return static_cast<int64_t>(b) * (b - 1) / 2 + a; }
 
+// The same pairs of n_bits sites (n_bits even) in blocks by site parity: even-even pairs, then
+// odd-odd, then even-odd.  With sites interleaved by spin (even = alpha, odd = beta) the blocks are
+// the alpha-alpha, beta-beta and alpha-beta pairs, so the coefficients a spin-conserving excitation
+// can meet sit in one block.  A permutation of pair_index, so any table is merged the same way.
+__host__ __device__ __forceinline__ int64_t pair_slot(int a, int b, int n_bits) {
+    const int64_t half = n_bits >> 1;
+    const int64_t same = half * (half - 1) / 2;
+    if (((a ^ b) & 1) == 0) {
+        return ((a & 1) ? same : 0) + pair_index(a >> 1, b >> 1);
+    }
+    const int ev = (a & 1) ? b : a;
+    const int od = (a & 1) ? a : b;
+    return 2 * same + static_cast<int64_t>(ev >> 1) * half + (od >> 1);
+}
+
 __host__ __device__ inline bool pair_excitation(uint64_t rec, int particle_cut, PairExcitation* out) {
     if ((rec >> 63) & 1ull) {
         return false;

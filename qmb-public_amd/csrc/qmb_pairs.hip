@@ -21,11 +21,15 @@ namespace qmb {
 namespace {
 
 constexpr int kBlock = 256;
-// Sort key of a core entry: 36 bits of the core hash above 20 bits of the removed pair (or site),
-// so a group of equal cores is a run of equal top bits and, inside it, its entries ascend by pair
-// -- the rows of a group then read the coefficient table along rows in order.
+// Sort key of a core entry: 2 bits of core class, 34 bits of the core hash, 20 bits of the removed
+// pair's slot (or the removed site).  The core class is the number of even sites of the core mod 3,
+// a function of the core, so a group never splits.  In a set where every config has the same
+// numbers of even and odd occupied sites (spin sectors) it tells the pair class of all the group's
+// entries, so groups come in class order -- waves running side by side read one block of the
+// coefficient table -- and inside a group the entries ascend by slot, reading table rows in order.
 constexpr int kKeyBits = 56;
 constexpr int kSubBits = 20;
+constexpr int kHashBits = 34;
 constexpr uint64_t kKeySentinel = (1ull << kKeyBits) - 1ull;  // unused entries; sorts after every real key
 constexpr int kMaxPairBytes = 16;            // the dense table is n_pairs^2 entries
 
@@ -115,7 +119,7 @@ __global__ void pairs_fill_kernel(const int16_t* __restrict__ site, const uint8_
         return;
     }
     const double2 v = coef[t];
-    double2* dst = table + pair_index(e.r0, e.r1) * n_pairs + pair_index(e.c0, e.c1);
+    double2* dst = table + pair_slot(e.r0, e.r1, n_bits) * n_pairs + pair_slot(e.c0, e.c1, n_bits);
     unsafeAtomicAdd(&dst->x, e.delta ? -v.x : v.x);
     unsafeAtomicAdd(&dst->y, e.delta ? -v.y : v.y);
     atomicAdd(reinterpret_cast<unsigned long long*>(&counts[0]), 1ull);
@@ -163,9 +167,9 @@ __device__ __forceinline__ void flip_site(Cfg<W>& c, int s) {
     }
 }
 
-__device__ __forceinline__ uint64_t core_key(uint64_t h, uint64_t sub) {
+__device__ __forceinline__ uint64_t core_key(uint64_t h, uint64_t cls, uint64_t sub) {
     constexpr uint64_t kTop = (1ull << (kKeyBits - kSubBits)) - 1ull;
-    uint64_t top = h & kTop;
+    uint64_t top = (cls << kHashBits) | (h & ((1ull << kHashBits) - 1ull));
     top = top == kTop ? top - 1 : top;  // a merged hash value only makes its group "mixed"
     return (top << kSubBits) | sub;
 }
@@ -175,8 +179,9 @@ __device__ __forceinline__ uint64_t group_of(uint64_t key) { return key >> kSubB
 // One thread per config: every occupied pair (x < y) -- ORDER 2 -- or every occupied site x --
 // ORDER 1 -- gives the entry (hash of the config without those sites, row).
 template <int W, int ORDER>
-__global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64_t* __restrict__ cols, int64_t n, int per_config,
-                                 uint64_t* __restrict__ okeys, uint64_t* __restrict__ ovals, int64_t* __restrict__ counts) {
+__global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64_t* __restrict__ cols, int64_t n, int n_bits,
+                                 int per_config, uint64_t* __restrict__ okeys, uint64_t* __restrict__ ovals,
+                                 int64_t* __restrict__ counts) {
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i >= n) {
         return;
@@ -184,6 +189,11 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
     const Cfg<W> c = load_key<W>(keys, i);
     const uint64_t h = hash_cfg<W>(c, cols);
     const int64_t base = i * per_config;
+    int even = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        even += __popcll(c.w[w] & 0x5555555555555555ull);
+    }
     int64_t k = 0;
 #pragma unroll
     for (int wx = 0; wx < W; ++wx) {
@@ -194,7 +204,7 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
             const uint64_t hx = h ^ cols[x];
             if (ORDER == 1) {
                 if (k < per_config) {
-                    okeys[base + k] = core_key(hx, static_cast<uint64_t>(x));
+                    okeys[base + k] = core_key(hx, 0, static_cast<uint64_t>(x));
                     ovals[base + k] = (static_cast<uint64_t>(i) << 16) | static_cast<uint64_t>(x);
                 }
                 ++k;
@@ -210,7 +220,9 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
                     const int y = 64 * wy + __builtin_ctzll(my);
                     my &= my - 1;
                     if (k < per_config) {
-                        okeys[base + k] = core_key(hx ^ cols[y], static_cast<uint64_t>(pair_index(x, y)));
+                        const int core_even = even - ((x & 1) ? 0 : 1) - ((y & 1) ? 0 : 1);
+                        okeys[base + k] = core_key(hx ^ cols[y], static_cast<uint64_t>(core_even % 3),
+                                                   static_cast<uint64_t>(pair_slot(x, y, n_bits)));
                         ovals[base + k] = (static_cast<uint64_t>(i) << 16) | (static_cast<uint64_t>(y) << 8) |
                                           static_cast<uint64_t>(x);
                     }
@@ -319,8 +331,9 @@ int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, int ord
 
 // ------------------------------------------------------------------------------ group kernel --
 struct PairParams {
-    const double2* coef;    // merged pair-excitation coefficients
+    const double2* coef;    // merged pair-excitation coefficients, [pair_slot][pair_slot]
     int64_t n_pairs;
+    int n_bits;
     int particle_cut;
     const uint64_t* ckeys;  // config words of the set, [n][W]
     const uint64_t* vals;   // sorted entries
@@ -330,16 +343,57 @@ struct PairParams {
     double2* out;
 };
 
+// Sites in [a, b), a <= b: L(a) ^ L(b) with L(s) the sites below s.
 template <int W>
-__device__ __forceinline__ int pair_parity(const Cfg<W>& c, int a, int b, int x, int y) {
+__device__ __forceinline__ Cfg<W> range_mask(int a, int b) {
+    Cfg<W> m;
+    if (W == 1) {
+        m.w[0] = (1ull << b) - (1ull << a);  // sites of a one-word config are < 64
+    } else {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const int lo = a - 64 * w, hi = b - 64 * w;
+            const uint64_t mh = hi <= 0 ? 0ull : (hi >= 64 ? ~0ull : (1ull << hi) - 1ull);
+            const uint64_t ml = lo <= 0 ? 0ull : (lo >= 64 ? ~0ull : (1ull << lo) - 1ull);
+            m.w[w] = mh & ~ml;
+        }
+    }
+    return m;
+}
+
+template <int W>
+__device__ __forceinline__ int masked_parity(const Cfg<W>& c, const Cfg<W>& m) {
     int par = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-        const uint64_t m = low_mask_word(a, w) ^ low_mask_word(b, w) ^ low_mask_word(x, w) ^ low_mask_word(y, w);
-        par ^= __popcll(c.w[w] & m) & 1;
+        par ^= __popcll(c.w[w] & m.w[w]);
     }
-    return par;
+    return par & 1;
 }
+
+// pair_parity(c) = popcount(c & (L(a) ^ L(b) ^ L(x) ^ L(y))) mod 2 for a < b, x < y.
+template <int W>
+__device__ __forceinline__ int pair_parity(const Cfg<W>& c, int a, int b, int x, int y) {
+    const Cfg<W> m1 = range_mask<W>(a, b);
+    const Cfg<W> m2 = range_mask<W>(x, y);
+    int par = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        par ^= __popcll(c.w[w] & (m1.w[w] ^ m2.w[w]));
+    }
+    return par & 1;
+}
+
+// One staged group entry of pair_group, shared by the wave through LDS.
+template <int W>
+struct PairStage {
+    double2 g;        // in[row]
+    uint64_t m[W];    // sites in [q0, q1)
+    uint32_t packed;  // q0 | q1 << 8 | pair_slot(q0, q1) << 16
+    uint32_t row;
+};
+constexpr int kWavesPerBlock = kBlock / 64;
+constexpr size_t kPairStageBytes = kWavesPerBlock * 64 * sizeof(PairStage<2>);
 
 __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int lane) {
     const uint32_t lo = __shfl(static_cast<uint32_t>(v), lane);
@@ -354,11 +408,14 @@ __device__ __forceinline__ double2 shfl_d2(const double2& v, int lane) {
 constexpr int kPairBatch = 4;
 
 // One wave, one group of entries [s, e) (wave-uniform).  Lane = row a; the group's entries b are
-// staged 64 at a time in registers and broadcast with shuffles.
+// staged 64 at a time in this wave's LDS slots (`stage`, with their pair index, site-range mask and
+// input amplitude) and read back as broadcasts.
 //   ADJ:  out[a] += sum_b conj(A_ba) in[b],  A_ba = (-1)^pair_parity(c_a) T[Pa][Pb]
 //   !ADJ: out[a] += sum_b A_ab in[b],        A_ab = (-1)^pair_parity(c_b) T[Pb][Pa]
+// with pair_parity(c) = popcount(c & ([r0, r1) ^ [q0, q1))), split into a per-row part and a
+// per-entry part.
 template <int W, bool ADJ>
-__device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane) {
+__device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane, PairStage<W>* stage) {
     const uint64_t v0 = p.vals[s];
     Cfg<W> core0 = load_key<W>(p.ckeys, static_cast<int64_t>(v0 >> 16));
     flip_site<W>(core0, static_cast<int>(v0 & 0xff));
@@ -380,15 +437,41 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
         for (int w = 0; w < W; ++w) {
             same = same && core.w[w] == core0.w[w];
         }
-        // Different cores with equal 48-bit hashes share a group: then every pair is checked exactly.
+        // Different cores with equal hashes share a group: then every pair is checked exactly.
         const bool mixed = __ballot(act && !same) != 0ull;
-        const int64_t prow = pair_index(r0, r1);
+        const int64_t prow = pair_slot(r0, r1, p.n_bits);
+        const Cfg<W> mr = range_mask<W>(r0, r1);
+        const int base = fermi ? masked_parity<W>(ADJ ? ca : core, mr) : 0;
         double acc_re = 0.0, acc_im = 0.0;
         for (uint32_t b0 = s; b0 < e; b0 += 64) {
             const uint32_t bl = b0 + lane;
-            const bool bin = bl < e;
-            const uint64_t vbl = bin ? p.vals[bl] : 0ull;
-            const double2 gbl = bin ? p.in[vbl >> 16] : make_double2(0.0, 0.0);
+            {
+                PairStage<W> st;
+                if (bl < e) {
+                    const uint64_t vb = p.vals[bl];
+                    const int q0 = static_cast<int>(vb & 0xff);
+                    const int q1 = static_cast<int>((vb >> 8) & 0xff);
+                    st.row = static_cast<uint32_t>(vb >> 16);
+                    st.g = p.in[st.row];
+                    const Cfg<W> mq = range_mask<W>(q0, q1);
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        st.m[w] = mq.w[w];
+                    }
+                    st.packed = static_cast<uint32_t>(q0) | (static_cast<uint32_t>(q1) << 8) |
+                                (static_cast<uint32_t>(pair_slot(q0, q1, p.n_bits)) << 16);
+                } else {
+                    st.row = 0;
+                    st.g = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        st.m[w] = 0;
+                    }
+                    st.packed = 0;
+                }
+                stage[lane] = st;
+            }
+            __builtin_amdgcn_wave_barrier();
             const int nb = static_cast<int>(e - b0 < 64u ? e - b0 : 64u);
             for (int u = 0; u < nb; u += kPairBatch) {
                 double2 x[kPairBatch], g[kPairBatch];
@@ -397,14 +480,16 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
 #pragma unroll
                 for (int k = 0; k < kPairBatch; ++k) {
                     const int uu = (u + k) & 63;
-                    const uint64_t vb = shfl_u64(vbl, uu);
-                    g[k] = shfl_d2(gbl, uu);
-                    const int q0 = static_cast<int>(vb & 0xff);
-                    const int q1 = static_cast<int>((vb >> 8) & 0xff);
+                    const PairStage<W>& st = stage[uu];
+                    const uint32_t pk = st.packed;
+                    const int q0 = static_cast<int>(pk & 0xff);
+                    const int q1 = static_cast<int>((pk >> 8) & 0xff);
+                    const int64_t pq = static_cast<int64_t>(pk >> 16);
+                    g[k] = st.g;
                     bool o = act && (u + k < nb) && (b0 + static_cast<uint32_t>(u + k) != a) && q0 != r0 && q0 != r1 && q1 != r0 &&
                              q1 != r1;
                     if (mixed && o) {
-                        Cfg<W> cb = load_key<W>(p.ckeys, static_cast<int64_t>(vb >> 16));
+                        Cfg<W> cb = load_key<W>(p.ckeys, static_cast<int64_t>(st.row));
                         flip_site<W>(cb, q0);
                         flip_site<W>(cb, q1);
 #pragma unroll
@@ -413,16 +498,21 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
                         }
                     }
                     ok[k] = o;
+                    Cfg<W> mq;
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        mq.w[w] = st.m[w];
+                    }
                     int64_t idx;
                     if (ADJ) {
-                        idx = prow * p.n_pairs + pair_index(q0, q1);
-                        sg[k] = fermi ? pair_parity<W>(ca, r0, r1, q0, q1) : 0;
+                        idx = prow * p.n_pairs + pq;
+                        sg[k] = fermi ? base ^ masked_parity<W>(ca, mq) : 0;
                     } else {
-                        Cfg<W> cb = core;
-                        flip_site<W>(cb, q0);
-                        flip_site<W>(cb, q1);
-                        idx = pair_index(q0, q1) * p.n_pairs + prow;
-                        sg[k] = fermi ? pair_parity<W>(cb, r0, r1, q0, q1) : 0;
+                        // c_b = core + {q0, q1}: q0 lies in [q0, q1), q1 does not
+                        idx = pq * p.n_pairs + prow;
+                        const int in0 = (q0 >= r0 && q0 < r1) ? 1 : 0;
+                        const int in1 = (q1 >= r0 && q1 < r1) ? 1 : 0;
+                        sg[k] = fermi ? (base ^ masked_parity<W>(core, mq) ^ in0 ^ in1 ^ 1) : 0;
                     }
                     x[k] = o ? p.coef[idx] : make_double2(0.0, 0.0);
                 }
@@ -441,6 +531,7 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
                     }
                 }
             }
+            __builtin_amdgcn_wave_barrier();
         }
         if (act && (acc_re != 0.0 || acc_im != 0.0)) {
             unsafeAtomicAdd(&p.out[ia].x, acc_re);
@@ -508,13 +599,13 @@ __device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_
             int64_t idx;
             int sg;
             if (ADJ) {
-                idx = pair_index(r0, r1) * p.n_pairs + pair_index(q0, q1);
+                idx = pair_slot(r0, r1, p.n_bits) * p.n_pairs + pair_slot(q0, q1, p.n_bits);
                 sg = fermi ? pair_parity<W>(ca, r0, r1, q0, q1) : 0;
             } else {
                 Cfg<W> cb = core;
                 flip_site<W>(cb, q0);
                 flip_site<W>(cb, q1);
-                idx = pair_index(q0, q1) * p.n_pairs + pair_index(r0, r1);
+                idx = pair_slot(q0, q1, p.n_bits) * p.n_pairs + pair_slot(r0, r1, p.n_bits);
                 sg = fermi ? pair_parity<W>(cb, r0, r1, q0, q1) : 0;
             }
             const double2 x = p.coef[idx];
@@ -542,7 +633,7 @@ __device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_
 // The pair work of the groups [g0, g0 + 64): groups of up to 8 entries through the tiny-group
 // classes, larger ones a row block at a time (pair_group).
 template <int W, bool ADJ>
-__device__ __forceinline__ void pair_chunk(const PairParams& p, int64_t groups, int64_t g0, int lane) {
+__device__ __forceinline__ void pair_chunk(const PairParams& p, int64_t groups, int64_t g0, int lane, char* stage_bytes) {
     const int64_t gg = g0 + lane;
     uint32_t gs = 0, ge = 0;
     if (gg < groups) {
@@ -554,10 +645,11 @@ __device__ __forceinline__ void pair_chunk(const PairParams& p, int64_t groups, 
     pair_tiny_class<W, ADJ>(p, __ballot(sz > 2u && sz <= 4u), gs, ge, 4, lane);
     pair_tiny_class<W, ADJ>(p, __ballot(sz == 2u), gs, ge, 2, lane);
     uint64_t big = __ballot(sz > 8u);
+    PairStage<W>* stage = reinterpret_cast<PairStage<W>*>(stage_bytes) + 64 * (threadIdx.x >> 6);
     while (big) {
         const int l = __builtin_ctzll(big);
         big &= big - 1;
-        pair_group<W, ADJ>(p, __shfl(gs, l), __shfl(ge, l), lane);
+        pair_group<W, ADJ>(p, __shfl(gs, l), __shfl(ge, l), lane, stage);
     }
 }
 
@@ -579,9 +671,10 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(PairParams p) {
         }
         return;
     }
+    __shared__ __align__(16) char stage[kPairStageBytes];
     const int64_t groups = p.counts[1];
     for (int64_t g0 = wave * 64; g0 < groups; g0 += nwaves * 64) {
-        pair_chunk<W, ADJ>(p, groups, g0, lane);
+        pair_chunk<W, ADJ>(p, groups, g0, lane, stage);
     }
 }
 
@@ -650,12 +743,7 @@ struct SingleParams {
 
 template <int W>
 __device__ __forceinline__ int single_parity(const Cfg<W>& c, int r, int p) {
-    int par = 0;
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-        par ^= __popcll(c.w[w] & (low_mask_word(r, w) ^ low_mask_word(p, w))) & 1;
-    }
-    return par;
+    return masked_parity<W>(c, range_mask<W>(r < p ? r : p, r < p ? p : r));
 }
 
 // Element of the single excitation (annihilate r, create p) on a source config whose other
@@ -972,11 +1060,12 @@ __device__ void diag_rows(const BatchItemDev& it, int64_t r0, int lane) {
 }
 
 template <int W, bool ADJ>
-__device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int lane) {
+__device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int lane, char* stage) {
     if (type == 0) {
         PairParams p{};
         p.coef = it.pcoef;
         p.n_pairs = it.n_pairs;
+        p.n_bits = it.n_bits;
         p.particle_cut = it.particle_cut;
         p.ckeys = it.ckeys;
         p.vals = it.pvals;
@@ -986,7 +1075,7 @@ __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int 
         p.out = it.out;
         const int64_t groups = it.pcounts[1];
         if (local * 64 < groups && it.pcounts[2] == 0) {
-            pair_chunk<W, ADJ>(p, groups, local * 64, lane);
+            pair_chunk<W, ADJ>(p, groups, local * 64, lane, stage);
         }
     } else if (type == 1) {
         SingleParams p{};
@@ -1019,6 +1108,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchItemDev* __res
     const int lane = threadIdx.x & 63;
     const int64_t wave = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6;
     const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * kBlock) >> 6;
+    __shared__ __align__(16) char stage[kPairStageBytes];
     const int slots = 3 * n_items;
     for (int64_t u = wave; u < total; u += nwaves) {
         // last slot whose first unit is <= u (offsets[slots] == total > u)
@@ -1035,9 +1125,9 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchItemDev* __res
         const BatchItemDev& it = items[lo - type * n_items];
         const int64_t local = u - offsets[lo];
         if (it.n_words == 1) {
-            batch_unit<1, ADJ>(it, type, local, lane);
+            batch_unit<1, ADJ>(it, type, local, lane, stage);
         } else {
-            batch_unit<2, ADJ>(it, type, local, lane);
+            batch_unit<2, ADJ>(it, type, local, lane, stage);
         }
     }
 }
@@ -1109,7 +1199,8 @@ int cores_build(const qmb_targets* targets, int max_occupied, int order, void* b
         void* tmp = sc.take_bytes(tmp_bytes);
         const unsigned nb = blocks_for(c.n);
         auto emit = [&](auto kernel) {
-            kernel<<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, c.per_config, keys_in, vals_in, c.counts);
+            kernel<<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, 8 * c.n_qubytes, c.per_config, keys_in, vals_in,
+                                              c.counts);
         };
         switch (c.n_words * 2 + (order - 1)) {
             case 2: emit(&core_emit_kernel<1, 1>); break;
@@ -1156,6 +1247,7 @@ int apply_pairs(const qmb_pairs* pairs, const qmb_cores* cores, const qmb_target
     PairParams p{};
     p.coef = reinterpret_cast<const double2*>(pairs->coef);
     p.n_pairs = pairs->n_pairs;
+    p.n_bits = pairs->n_bits;
     p.particle_cut = pairs->particle_cut;
     p.ckeys = targets->keys;
     p.vals = cores->vals;
