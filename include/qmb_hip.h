@@ -214,13 +214,16 @@ int qmb_apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const
  * constant sign) into diag = {constant, per site p, per site pair (p, q)}: the diagonal element of
  * a config c is diag[0] + sum_{p in c} diag[1 + p] + sum_{p<q in c} diag[1 + n_bits + pair_index(p, q)]. */
 typedef struct qmb_pairs {
-    int32_t n_qubytes, particle_cut, n_bits, reserved;
+    int32_t n_qubytes, particle_cut, n_bits;
+    int32_t real;      /* 1: the kernels read coef_re (set by qmb_pairs_set_real) */
     int64_t n_pairs;
     void* buffer;
     size_t bytes;
     double* coef;      /* [n_pairs * n_pairs][2] */
+    double* coef_re;   /* [n_pairs * n_pairs] real parts, filled by qmb_pairs_set_real */
     double* diag;      /* [1 + n_bits + n_pairs][2] */
-    int64_t* counts;   /* device: {pair-excitation terms, diagonal terms, 0, 0} */
+    int64_t* counts;   /* device: {pair-excitation terms, diagonal terms, entries with a nonzero imaginary
+                          part (after qmb_pairs_set_real), 0} */
 } qmb_pairs;
 
 int qmb_pairs_workspace_size(int n_qubytes, size_t* bytes);
@@ -243,6 +246,12 @@ typedef struct qmb_singles {
 int qmb_singles_workspace_size(int n_qubytes, size_t* bytes);
 int qmb_singles_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
                       int64_t n_terms, void* buffer, size_t bytes, qmb_singles* out, void* stream);
+
+/* For tables whose coefficients are all real: copies the real parts to coef_re and makes the
+ * kernels read those (half the bytes per coefficient).  real = 0 switches back to the complex
+ * table.  Counts the entries with a nonzero imaginary part into counts[2]; the caller asserts
+ * there are none. */
+int qmb_pairs_set_real(qmb_pairs* pairs, int32_t real, void* stream);
 
 /* qmb_terms_build without some terms (same layout and buffer size).  exclude is a mask:
  * 1 pair excitations, 2 single excitations, 4 diagonal terms (from the buckets; the diagonal

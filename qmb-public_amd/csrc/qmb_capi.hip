@@ -328,6 +328,11 @@ int qmb_pairs_build(int n_qubytes, int particle_cut, const int16_t* site, const 
     return pairs_build(n_qubytes, particle_cut, site, kind, coef, n_terms, buffer, bytes, out, as_stream(stream));
 }
 
+int qmb_pairs_set_real(qmb_pairs* pairs, int32_t real, void* stream) {
+    clear_error();
+    return pairs_set_real(pairs, real, as_stream(stream));
+}
+
 int qmb_singles_workspace_size(int n_qubytes, size_t* bytes) {
     clear_error();
     if (!bytes) {

@@ -138,12 +138,26 @@ int pairs_layout(Carver& cv, int n_qubytes, qmb_pairs* out) {
     p.buffer = cv.base;
     p.counts = cv.take<int64_t>(4);
     p.coef = cv.take<double>(2 * static_cast<size_t>(p.n_pairs) * static_cast<size_t>(p.n_pairs));
+    p.coef_re = cv.take<double>(static_cast<size_t>(p.n_pairs) * static_cast<size_t>(p.n_pairs));
     p.diag = cv.take<double>(2 * (1 + static_cast<size_t>(p.n_bits) + static_cast<size_t>(p.n_pairs)));
     p.bytes = cv.off;
     if (out) {
         *out = p;
     }
     return QMB_OK;
+}
+
+// Real parts of the merged coefficients; counts[2] += entries with a nonzero imaginary part.
+__global__ void pairs_real_kernel(const double2* __restrict__ coef, int64_t n, double* __restrict__ re, int64_t* __restrict__ counts) {
+    const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (k >= n) {
+        return;
+    }
+    const double2 v = coef[k];
+    re[k] = v.x;
+    if (v.y != 0.0) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&counts[2]), 1ull);
+    }
 }
 
 // ----------------------------------------------------------------------------------- cores --
@@ -332,6 +346,7 @@ int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, int ord
 // ------------------------------------------------------------------------------ group kernel --
 struct PairParams {
     const double2* coef;    // merged pair-excitation coefficients, [pair_slot][pair_slot]
+    const double* coef_re;  // their real parts when every coefficient is real, else null
     int64_t n_pairs;
     int n_bits;
     int particle_cut;
@@ -514,7 +529,7 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
                         const int in1 = (q1 >= r0 && q1 < r1) ? 1 : 0;
                         sg[k] = fermi ? (base ^ masked_parity<W>(core, mq) ^ in0 ^ in1 ^ 1) : 0;
                     }
-                    x[k] = o ? p.coef[idx] : make_double2(0.0, 0.0);
+                    x[k] = !o ? make_double2(0.0, 0.0) : (p.coef_re ? make_double2(p.coef_re[idx], 0.0) : p.coef[idx]);
                 }
 #pragma unroll
                 for (int k = 0; k < kPairBatch; ++k) {
@@ -608,7 +623,7 @@ __device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_
                 idx = pair_slot(q0, q1, p.n_bits) * p.n_pairs + pair_slot(r0, r1, p.n_bits);
                 sg = fermi ? pair_parity<W>(cb, r0, r1, q0, q1) : 0;
             }
-            const double2 x = p.coef[idx];
+            const double2 x = p.coef_re ? make_double2(p.coef_re[idx], 0.0) : p.coef[idx];
             const double xr = sg ? -x.x : x.x;
             const double xi = sg ? -x.y : x.y;
             if (ADJ) {
@@ -1028,6 +1043,7 @@ struct BatchItemDev {
     const uint64_t* ckeys;
     const double2* diag;
     const double2* pcoef;
+    const double* pcoef_re;
     int64_t n_pairs;
     const uint64_t* pvals;
     const uint32_t* pstarts;
@@ -1064,6 +1080,7 @@ __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int 
     if (type == 0) {
         PairParams p{};
         p.coef = it.pcoef;
+        p.coef_re = it.pcoef_re;
         p.n_pairs = it.n_pairs;
         p.n_bits = it.n_bits;
         p.particle_cut = it.particle_cut;
@@ -1167,6 +1184,21 @@ int pairs_build(int n_qubytes, int particle_cut, const int16_t* site, const uint
     return QMB_OK;
 }
 
+int pairs_set_real(qmb_pairs* pairs, int real, hipStream_t stream) {
+    if (!pairs) {
+        return set_error(QMB_ERR_ARGUMENT, "pairs must be non-null");
+    }
+    pairs->real = 0;
+    if (real) {
+        const int64_t n = pairs->n_pairs * pairs->n_pairs;
+        pairs_real_kernel<<<blocks_for(n), kBlock, 0, stream>>>(reinterpret_cast<const double2*>(pairs->coef), n, pairs->coef_re,
+                                                                pairs->counts);
+        QMB_CHECK_LAUNCH();
+        pairs->real = 1;
+    }
+    return QMB_OK;
+}
+
 size_t cores_bytes(int n_qubytes, int64_t n, int max_occupied, int order) {
     Carver cv{nullptr, 0};
     return cores_layout(cv, n_qubytes, n, max_occupied, order, nullptr, nullptr, nullptr) == QMB_OK ? cv.off : 0;
@@ -1246,6 +1278,7 @@ int apply_pairs(const qmb_pairs* pairs, const qmb_cores* cores, const qmb_target
     }
     PairParams p{};
     p.coef = reinterpret_cast<const double2*>(pairs->coef);
+    p.coef_re = pairs->real ? pairs->coef_re : nullptr;
     p.n_pairs = pairs->n_pairs;
     p.n_bits = pairs->n_bits;
     p.particle_cut = pairs->particle_cut;
@@ -1435,6 +1468,7 @@ int batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t by
         if (x.pairs) {
             d.has_pairs = 1;
             d.pcoef = reinterpret_cast<const double2*>(x.pairs->coef);
+            d.pcoef_re = x.pairs->real ? x.pairs->coef_re : nullptr;
             d.n_pairs = x.pairs->n_pairs;
             d.pvals = x.cores2->vals;
             d.pstarts = x.cores2->starts;

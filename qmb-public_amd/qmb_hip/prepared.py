@@ -109,6 +109,10 @@ class PreparedTerms:
             self.n_pair_terms = _device_count(pbuf, pdesc.counts)
             if self.n_pair_terms > 0:
                 self.pairs_buffer, self.pairs_desc = pbuf, pdesc
+                if bool((coef[:, 1] == 0).all().item()):
+                    # real table: the pair kernels read 8-byte coefficients
+                    with torch.cuda.device(device):
+                        _lib.check(lib.qmb_pairs_set_real(ctypes.byref(pdesc), 1, _stream(device)), "qmb_pairs_set_real")
                 _lib.check(lib.qmb_singles_workspace_size(n_qubytes, ctypes.byref(size)), "qmb_singles_workspace_size")
                 sbuf = _workspace(size.value, device)
                 sdesc = _lib.QmbSingles()

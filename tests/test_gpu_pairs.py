@@ -157,6 +157,10 @@ def test_structured_split_is_exact(cuda_device: torch.device, name: str, pc: int
     assert terms.n_residual_terms + terms.n_pair_terms + terms.n_single_terms + full_diag == full_valid, name
     off = int(terms.pairs_desc.counts) - terms.pairs_buffer.data_ptr()
     assert int(terms.pairs_buffer[off + 8:off + 16].view(torch.int64).item()) == full_diag  # every diagonal term folded
+    # real tables switch the pair kernels to 8-byte coefficients, and no merged entry has an imaginary part
+    assert terms.pairs_desc.real == int(np.all(np.asarray(c)[:, 1] == 0)), name
+    if terms.pairs_desc.real:
+        assert int(terms.pairs_buffer[off + 16:off + 24].view(torch.int64).item()) == 0
     lib = _lib.library()
     out = (ctypes.c_int32 * 5)()
     n_pairs = n_singles = 0
