@@ -409,6 +409,7 @@ struct PairStage {
 };
 constexpr int kWavesPerBlock = kBlock / 64;
 constexpr size_t kPairStageBytes = kWavesPerBlock * 64 * sizeof(PairStage<2>);
+static_assert(sizeof(PairStage<1>) <= sizeof(PairStage<2>) && sizeof(PairStage<2>) % 16 == 0, "stage slot layout");
 
 __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int lane) {
     const uint32_t lo = __shfl(static_cast<uint32_t>(v), lane);
@@ -420,7 +421,10 @@ __device__ __forceinline__ double2 shfl_d2(const double2& v, int lane) {
     return make_double2(__shfl(v.x, lane), __shfl(v.y, lane));
 }
 
+// Coefficient gathers kept in flight per lane in pair_group: 4 in the pair kernel; 2 inside the
+// batched kernel, whose other paths already hold many registers (fewer registers, more waves).
 constexpr int kPairBatch = 4;
+constexpr int kPairBatchInBatch = 2;
 
 // One wave, one group of entries [s, e) (wave-uniform).  Lane = row a; the group's entries b are
 // staged 64 at a time in this wave's LDS slots (`stage`, with their pair index, site-range mask and
@@ -429,7 +433,7 @@ constexpr int kPairBatch = 4;
 //   !ADJ: out[a] += sum_b A_ab in[b],        A_ab = (-1)^pair_parity(c_b) T[Pb][Pa]
 // with pair_parity(c) = popcount(c & ([r0, r1) ^ [q0, q1))), split into a per-row part and a
 // per-entry part.
-template <int W, bool ADJ>
+template <int W, bool ADJ, int B>
 __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane, PairStage<W>* stage) {
     const uint64_t v0 = p.vals[s];
     Cfg<W> core0 = load_key<W>(p.ckeys, static_cast<int64_t>(v0 >> 16));
@@ -488,12 +492,12 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
             }
             __builtin_amdgcn_wave_barrier();
             const int nb = static_cast<int>(e - b0 < 64u ? e - b0 : 64u);
-            for (int u = 0; u < nb; u += kPairBatch) {
-                double2 x[kPairBatch], g[kPairBatch];
-                int sg[kPairBatch];
-                bool ok[kPairBatch];
+            for (int u = 0; u < nb; u += B) {
+                double2 x[B], g[B];
+                int sg[B];
+                bool ok[B];
 #pragma unroll
-                for (int k = 0; k < kPairBatch; ++k) {
+                for (int k = 0; k < B; ++k) {
                     const int uu = (u + k) & 63;
                     const PairStage<W>& st = stage[uu];
                     const uint32_t pk = st.packed;
@@ -532,7 +536,7 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
                     x[k] = !o ? make_double2(0.0, 0.0) : (p.coef_re ? make_double2(p.coef_re[idx], 0.0) : p.coef[idx]);
                 }
 #pragma unroll
-                for (int k = 0; k < kPairBatch; ++k) {
+                for (int k = 0; k < B; ++k) {
                     if (ok[k]) {
                         const double xr = sg[k] ? -x[k].x : x[k].x;
                         const double xi = sg[k] ? -x[k].y : x[k].y;
@@ -647,7 +651,7 @@ __device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_
 
 // The pair work of the groups [g0, g0 + 64): groups of up to 8 entries through the tiny-group
 // classes, larger ones a row block at a time (pair_group).
-template <int W, bool ADJ>
+template <int W, bool ADJ, int B>
 __device__ __forceinline__ void pair_chunk(const PairParams& p, int64_t groups, int64_t g0, int lane, char* stage_bytes) {
     const int64_t gg = g0 + lane;
     uint32_t gs = 0, ge = 0;
@@ -660,11 +664,12 @@ __device__ __forceinline__ void pair_chunk(const PairParams& p, int64_t groups, 
     pair_tiny_class<W, ADJ>(p, __ballot(sz > 2u && sz <= 4u), gs, ge, 4, lane);
     pair_tiny_class<W, ADJ>(p, __ballot(sz == 2u), gs, ge, 2, lane);
     uint64_t big = __ballot(sz > 8u);
-    PairStage<W>* stage = reinterpret_cast<PairStage<W>*>(stage_bytes) + 64 * (threadIdx.x >> 6);
+    // each wave owns 64 slots of the largest stage size: waves of one block may run different W
+    PairStage<W>* stage = reinterpret_cast<PairStage<W>*>(stage_bytes + (threadIdx.x >> 6) * 64 * sizeof(PairStage<2>));
     while (big) {
         const int l = __builtin_ctzll(big);
         big &= big - 1;
-        pair_group<W, ADJ>(p, __shfl(gs, l), __shfl(ge, l), lane, stage);
+        pair_group<W, ADJ, B>(p, __shfl(gs, l), __shfl(ge, l), lane, stage);
     }
 }
 
@@ -689,7 +694,7 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(PairParams p) {
     __shared__ __align__(16) char stage[kPairStageBytes];
     const int64_t groups = p.counts[1];
     for (int64_t g0 = wave * 64; g0 < groups; g0 += nwaves * 64) {
-        pair_chunk<W, ADJ>(p, groups, g0, lane, stage);
+        pair_chunk<W, ADJ, kPairBatch>(p, groups, g0, lane, stage);
     }
 }
 
@@ -1092,7 +1097,7 @@ __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int 
         p.out = it.out;
         const int64_t groups = it.pcounts[1];
         if (local * 64 < groups && it.pcounts[2] == 0) {
-            pair_chunk<W, ADJ>(p, groups, local * 64, lane, stage);
+            pair_chunk<W, ADJ, kPairBatchInBatch>(p, groups, local * 64, lane, stage);
         }
     } else if (type == 1) {
         SingleParams p{};

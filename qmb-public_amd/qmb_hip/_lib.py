@@ -13,7 +13,7 @@ import os
 import pathlib
 
 _HERE = pathlib.Path(__file__).resolve().parent
-LIBRARY_PATH = pathlib.Path(os.environ.get("QMB_HIP_LIBRARY", str(_HERE / "libqmb_hip.so")))
+LIBRARY_PATH = pathlib.Path(os.environ.get("QMB_HIP_LIBRARY") or str(_HERE / "libqmb_hip.so"))
 
 QMB_OK = 0
 

@@ -110,6 +110,30 @@ def test_batch_non_hermitian_with_residual(cuda_device: torch.device) -> None:
         np.testing.assert_array_less(np.abs(b.output(idx).cpu().numpy() - ref), 1e-12 * _scale(ref, 10.0, psis[idx]) + 1e-300)
 
 
+def test_batch_mixed_word_counts(cuda_device: torch.device) -> None:
+    """Instances of one and two config words in one launch (waves of one block then run both), with
+    groups large enough for the staged path: a regression test for per-wave staging slots."""
+    s1, k1, c1 = sorted_terms(*sy.molecular_terms(10, 71))
+    cfg1 = sy.excitation_configs(10, 4, 4, 3000, 72)
+    s2, k2, c2 = sorted_terms(*sy.molecular_terms(34, 73, threshold=1e-6))
+    cfg2 = sy.excitation_configs(34, 3, 3, 4000, 74)
+    assert cfg1.shape[1] <= 8 < cfg2.shape[1]
+    instances = []
+    for s, k, c, cfg in ((s1, k1, c1, cfg1), (s2, k2, c2, cfg2), (s1, k1, c1, cfg1)):
+        nq = cfg.shape[1]
+        terms = pp.PreparedTerms(_dev(s, cuda_device), _dev(k, cuda_device), _dev(c, cuda_device), nq, 1)
+        instances.append((terms, pp.PreparedTargets(_dev(cfg, cuda_device), nq)))
+    b = qb.Batch(instances)
+    psis = [torch.from_numpy(sy.random_psi(t.n, 400 + k)).to(cuda_device) for k, (_, t) in enumerate(instances)]
+    b.set_inputs(psis)
+    for waves in (64, 4096):
+        b.launch(pp.MODE_FORWARD, waves=waves)
+        for k, (terms, targets) in enumerate(instances):
+            ref = _reference(terms, targets, psis[k], pp.MODE_FORWARD)
+            got = b.output(k).cpu().numpy()
+            np.testing.assert_array_less(np.abs(got - ref), 1e-12 * _scale(ref, 10.0, psis[k]) + 1e-300)
+
+
 def test_batch_graph_replays_match_eager(cuda_device: torch.device) -> None:
     specs = [(6, 3, 3, 300, 31), (8, 4, 3, 1000, 32), (4, 2, 2, 36, 33)]
     instances = [_molecular(*spec, cuda_device) for spec in specs]
