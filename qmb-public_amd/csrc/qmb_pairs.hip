@@ -191,7 +191,8 @@ __device__ __forceinline__ uint64_t core_key(uint64_t h, uint64_t cls, uint64_t 
 __device__ __forceinline__ uint64_t group_of(uint64_t key) { return key >> kSubBits; }
 
 // One thread per config: every occupied pair (x < y) -- ORDER 2 -- or every occupied site x --
-// ORDER 1 -- gives the entry (hash of the config without those sites, row).
+// ORDER 1 -- gives the entry (hash of the config without those sites, row); unused slots of a row
+// with fewer occupied sites get the sentinel key.
 template <int W, int ORDER>
 __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64_t* __restrict__ cols, int64_t n, int n_bits,
                                  int per_config, uint64_t* __restrict__ okeys, uint64_t* __restrict__ ovals,
@@ -202,7 +203,8 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
     }
     const Cfg<W> c = load_key<W>(keys, i);
     const uint64_t h = hash_cfg<W>(c, cols);
-    const int64_t base = i * per_config;
+    // entry k of row i sits at k * n + i: the lanes of a wave store to consecutive addresses (the
+    // sort that follows does not care about the input order)
     int even = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
@@ -218,8 +220,8 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
             const uint64_t hx = h ^ cols[x];
             if (ORDER == 1) {
                 if (k < per_config) {
-                    okeys[base + k] = core_key(hx, 0, static_cast<uint64_t>(x));
-                    ovals[base + k] = (static_cast<uint64_t>(i) << 16) | static_cast<uint64_t>(x);
+                    okeys[k * n + i] = core_key(hx, 0, static_cast<uint64_t>(x));
+                    ovals[k * n + i] = (static_cast<uint64_t>(i) << 16) | static_cast<uint64_t>(x);
                 }
                 ++k;
                 continue;
@@ -235,9 +237,9 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
                     my &= my - 1;
                     if (k < per_config) {
                         const int core_even = even - ((x & 1) ? 0 : 1) - ((y & 1) ? 0 : 1);
-                        okeys[base + k] = core_key(hx ^ cols[y], static_cast<uint64_t>(core_even % 3),
+                        okeys[k * n + i] = core_key(hx ^ cols[y], static_cast<uint64_t>(core_even % 3),
                                                    static_cast<uint64_t>(pair_slot(x, y, n_bits)));
-                        ovals[base + k] = (static_cast<uint64_t>(i) << 16) | (static_cast<uint64_t>(y) << 8) |
+                        ovals[k * n + i] = (static_cast<uint64_t>(i) << 16) | (static_cast<uint64_t>(y) << 8) |
                                           static_cast<uint64_t>(x);
                     }
                     ++k;
@@ -249,8 +251,8 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
         atomicAdd(reinterpret_cast<unsigned long long*>(&counts[2]), 1ull);
     }
     for (; k < per_config; ++k) {
-        okeys[base + k] = kKeySentinel;
-        ovals[base + k] = 0;
+        okeys[k * n + i] = kKeySentinel;
+        ovals[k * n + i] = 0;
     }
 }
 

@@ -292,3 +292,47 @@ def test_single_excitation_classifier_matches_oracle(pc: int, nq: int) -> None:
             sign = (_single_parity(cfg, r, p) ^ delta) if pc == 1 else 0
             assert par == sign, (site, kind, cfg)
     assert seen["single"] > 100 and seen["other"] > 20
+
+
+def test_batch_plan_rejects_bad_items_without_touching_the_device() -> None:
+    """qmb_batch_plan validates every item on the host before any copy; these calls never reach HIP."""
+    lib = _lib.library()
+    size = ctypes.c_size_t(0)
+    assert lib.qmb_batch_workspace_size(0, ctypes.byref(size)) != 0
+    assert b"n_items" in lib.qmb_last_error()
+    assert lib.qmb_batch_workspace_size(3, ctypes.byref(size)) == 0 and size.value > 0
+    items = (_lib.QmbBatchItem * 2)()
+    plan = _lib.QmbBatch()
+    buf = ctypes.create_string_buffer(size.value)
+    # item 0 without its pair table
+    assert lib.qmb_batch_plan(items, 2, ctypes.cast(buf, ctypes.c_void_p), size.value, ctypes.byref(plan), None) != 0
+    assert b"item 0" in lib.qmb_last_error()
+    # a pair table whose core index is missing
+    pairs = _lib.QmbPairs(n_qubytes=1, particle_cut=1, n_bits=8)
+    targets = _lib.QmbTargets(n_qubytes=1, n_words=1, n=0)
+    items[0].pairs = ctypes.pointer(pairs)
+    items[0].targets = ctypes.pointer(targets)
+    assert lib.qmb_batch_plan(items, 1, ctypes.cast(buf, ctypes.c_void_p), size.value, ctypes.byref(plan), None) != 0
+    assert b"core" in lib.qmb_last_error()
+    # a buffer that is too small
+    assert lib.qmb_batch_plan(items, 2, ctypes.cast(buf, ctypes.c_void_p), 16, ctypes.byref(plan), None) != 0
+    assert b"too small" in lib.qmb_last_error()
+    # mode checks of the apply entry points happen before any launch
+    assert lib.qmb_apply_batch(None, 0, 0, None) != 0
+
+
+def test_structured_builders_reject_bad_arguments() -> None:
+    lib = _lib.library()
+    size = ctypes.c_size_t(0)
+    assert lib.qmb_pairs_workspace_size(17, ctypes.byref(size)) != 0  # the dense pair table stops at 16 bytes
+    assert lib.qmb_pairs_workspace_size(7, ctypes.byref(size)) == 0
+    n_pairs = 56 * 55 // 2
+    assert size.value >= 16 * n_pairs * n_pairs
+    assert lib.qmb_singles_workspace_size(7, ctypes.byref(size)) == 0 and size.value >= 16 * 56 ** 3
+    assert lib.qmb_cores_workspace_size(7, 1000, 14, 3, ctypes.byref(size)) != 0  # order must be 1 or 2
+    assert lib.qmb_cores_workspace_size(7, 1000, 57, 2, ctypes.byref(size)) != 0  # more occupied sites than sites
+    assert b"max_occupied" in lib.qmb_last_error()
+    buf = ctypes.create_string_buffer(64)
+    terms = _lib.QmbTerms()
+    assert lib.qmb_terms_build_residual(7, 1, None, None, None, 0, 8, ctypes.cast(buf, ctypes.c_void_p), 64,
+                                        ctypes.byref(terms), None) != 0  # exclude mask out of range
