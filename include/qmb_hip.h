@@ -315,10 +315,13 @@ typedef struct qmb_batch_item {
     const qmb_cores* cores1;     /* order-1 core index of that set (with singles) */
     const double* in;            /* [n][2] */
     double* out;                 /* [n][2], accumulated into */
+    const double* in2;           /* two-vector batches: a second input [n][2] (NULL otherwise) */
+    double* out2;                /* and its output, accumulated into */
 } qmb_batch_item;
 
 typedef struct qmb_batch {
-    int32_t n_items, reserved;
+    int32_t n_items;
+    int32_t dual;      /* 1: every item has in2/out2 */
     int64_t total_units;
     void* buffer;
     size_t bytes;
@@ -330,7 +333,10 @@ int qmb_batch_workspace_size(int n_items, size_t* bytes);
 /* Copies the item table to the device; synchronises `stream` once (set-up, not per launch). */
 int qmb_batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t bytes, qmb_batch* out, void* stream);
 /* One persistent launch of `waves` wavefronts (0: 8192) over all units; mode 0 forward, 1 adjoint
- * (a Hermitian instance gives the same map in both). */
+ * (a Hermitian instance gives the same map in both).  A two-vector batch (dual) computes
+ * out = A^H in and out2 = A^H in2 in the same pass (mode 1 only), sharing every group walk and
+ * coefficient read: for Hermitian instances that is the forward and the backward of a training
+ * step in one launch. */
 int qmb_apply_batch(const qmb_batch* batch, int mode, int waves, void* stream);
 
 /* Host-only pair-excitation test of a compiled record (for unit tests): returns 1 and writes

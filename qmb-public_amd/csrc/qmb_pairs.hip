@@ -358,6 +358,8 @@ struct PairParams {
     const int64_t* counts;
     const double2* in;
     double2* out;
+    const double2* in2;  // second input/output of a two-vector pass (NV = 2), else null
+    double2* out2;
 };
 
 // Sites in [a, b), a <= b: L(a) ^ L(b) with L(s) the sites below s.
@@ -405,6 +407,7 @@ __device__ __forceinline__ int pair_parity(const Cfg<W>& c, int a, int b, int x,
 template <int W>
 struct PairStage {
     double2 g;        // in[row]
+    double2 g2;       // in2[row] of a two-vector pass
     uint64_t m[W];    // sites in [q0, q1)
     uint32_t packed;  // q0 | q1 << 8 | pair_slot(q0, q1) << 16
     uint32_t row;
@@ -435,7 +438,7 @@ constexpr int kPairBatchInBatch = 2;
 //   !ADJ: out[a] += sum_b A_ab in[b],        A_ab = (-1)^pair_parity(c_b) T[Pb][Pa]
 // with pair_parity(c) = popcount(c & ([r0, r1) ^ [q0, q1))), split into a per-row part and a
 // per-entry part.
-template <int W, bool ADJ, int B>
+template <int W, bool ADJ, int B, int NV>
 __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane, PairStage<W>* stage) {
     const uint64_t v0 = p.vals[s];
     Cfg<W> core0 = load_key<W>(p.ckeys, static_cast<int64_t>(v0 >> 16));
@@ -463,7 +466,7 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
         const int64_t prow = pair_slot(r0, r1, p.n_bits);
         const Cfg<W> mr = range_mask<W>(r0, r1);
         const int base = fermi ? masked_parity<W>(ADJ ? ca : core, mr) : 0;
-        double acc_re = 0.0, acc_im = 0.0;
+        double acc_re[NV] = {}, acc_im[NV] = {};
         for (uint32_t b0 = s; b0 < e; b0 += 64) {
             const uint32_t bl = b0 + lane;
             {
@@ -474,6 +477,9 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
                     const int q1 = static_cast<int>((vb >> 8) & 0xff);
                     st.row = static_cast<uint32_t>(vb >> 16);
                     st.g = p.in[st.row];
+                    if (NV == 2) {
+                        st.g2 = p.in2[st.row];
+                    }
                     const Cfg<W> mq = range_mask<W>(q0, q1);
 #pragma unroll
                     for (int w = 0; w < W; ++w) {
@@ -484,6 +490,7 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
                 } else {
                     st.row = 0;
                     st.g = make_double2(0.0, 0.0);
+                    st.g2 = make_double2(0.0, 0.0);
 #pragma unroll
                     for (int w = 0; w < W; ++w) {
                         st.m[w] = 0;
@@ -495,7 +502,7 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
             __builtin_amdgcn_wave_barrier();
             const int nb = static_cast<int>(e - b0 < 64u ? e - b0 : 64u);
             for (int u = 0; u < nb; u += B) {
-                double2 x[B], g[B];
+                double2 x[B], g[B], g2[B];
                 int sg[B];
                 bool ok[B];
 #pragma unroll
@@ -507,6 +514,7 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
                     const int q1 = static_cast<int>((pk >> 8) & 0xff);
                     const int64_t pq = static_cast<int64_t>(pk >> 16);
                     g[k] = st.g;
+                    g2[k] = NV == 2 ? st.g2 : make_double2(0.0, 0.0);
                     bool o = act && (u + k < nb) && (b0 + static_cast<uint32_t>(u + k) != a) && q0 != r0 && q0 != r1 && q1 != r0 &&
                              q1 != r1;
                     if (mixed && o) {
@@ -542,21 +550,29 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
                     if (ok[k]) {
                         const double xr = sg[k] ? -x[k].x : x[k].x;
                         const double xi = sg[k] ? -x[k].y : x[k].y;
-                        if (ADJ) {  // conj(x) * g
-                            acc_re += xr * g[k].x + xi * g[k].y;
-                            acc_im += xr * g[k].y - xi * g[k].x;
-                        } else {    // x * g
-                            acc_re += xr * g[k].x - xi * g[k].y;
-                            acc_im += xr * g[k].y + xi * g[k].x;
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) {
+                            const double2 gg = v == 0 ? g[k] : g2[k];
+                            if (ADJ) {  // conj(x) * g
+                                acc_re[v] += xr * gg.x + xi * gg.y;
+                                acc_im[v] += xr * gg.y - xi * gg.x;
+                            } else {    // x * g
+                                acc_re[v] += xr * gg.x - xi * gg.y;
+                                acc_im[v] += xr * gg.y + xi * gg.x;
+                            }
                         }
                     }
                 }
             }
             __builtin_amdgcn_wave_barrier();
         }
-        if (act && (acc_re != 0.0 || acc_im != 0.0)) {
-            unsafeAtomicAdd(&p.out[ia].x, acc_re);
-            unsafeAtomicAdd(&p.out[ia].y, acc_im);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            if (act && (acc_re[v] != 0.0 || acc_im[v] != 0.0)) {
+                double2* o = v == 0 ? p.out : p.out2;
+                unsafeAtomicAdd(&o[ia].x, acc_re[v]);
+                unsafeAtomicAdd(&o[ia].y, acc_im[v]);
+            }
         }
     }
 }
@@ -566,7 +582,7 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
 // (cores compared through a shuffle from the lane holding row b), the rows are summed over b with a
 // butterfly inside the block, and the first lane of each row adds the sum.  `mask` holds the lanes
 // of the 64-group chunk whose group is in this class; (gs_l, ge_l) is that lane's group.
-template <int W, bool ADJ>
+template <int W, bool ADJ, int NV>
 __device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_l, uint32_t ge_l, int gp, int lane) {
     const int lg = gp == 8 ? 3 : (gp == 4 ? 2 : 1);
     const int bs = gp * gp;
@@ -614,9 +630,8 @@ __device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_
             same = same && shfl_u64(core.w[w], src) == core.w[w];
         }
         const bool ok = rowa && rowb && ai != bi && same && q0 != r0 && q0 != r1 && q1 != r0 && q1 != r1;
-        double acc_re = 0.0, acc_im = 0.0;
+        double acc_re[NV] = {}, acc_im[NV] = {};
         if (ok) {
-            const double2 gv = p.in[vb >> 16];
             int64_t idx;
             int sg;
             if (ADJ) {
@@ -632,28 +647,36 @@ __device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_
             const double2 x = p.coef_re ? make_double2(p.coef_re[idx], 0.0) : p.coef[idx];
             const double xr = sg ? -x.x : x.x;
             const double xi = sg ? -x.y : x.y;
-            if (ADJ) {
-                acc_re = xr * gv.x + xi * gv.y;
-                acc_im = xr * gv.y - xi * gv.x;
-            } else {
-                acc_re = xr * gv.x - xi * gv.y;
-                acc_im = xr * gv.y + xi * gv.x;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double2 gv = (v == 0 ? p.in : p.in2)[vb >> 16];
+                if (ADJ) {
+                    acc_re[v] = xr * gv.x + xi * gv.y;
+                    acc_im[v] = xr * gv.y - xi * gv.x;
+                } else {
+                    acc_re[v] = xr * gv.x - xi * gv.y;
+                    acc_im[v] = xr * gv.y + xi * gv.x;
+                }
             }
         }
-        for (int m = 1; m < gp; m <<= 1) {
-            acc_re += __shfl_xor(acc_re, m);
-            acc_im += __shfl_xor(acc_im, m);
-        }
-        if (bi == 0 && rowa && (acc_re != 0.0 || acc_im != 0.0)) {
-            unsafeAtomicAdd(&p.out[ia].x, acc_re);
-            unsafeAtomicAdd(&p.out[ia].y, acc_im);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            for (int m = 1; m < gp; m <<= 1) {
+                acc_re[v] += __shfl_xor(acc_re[v], m);
+                acc_im[v] += __shfl_xor(acc_im[v], m);
+            }
+            if (bi == 0 && rowa && (acc_re[v] != 0.0 || acc_im[v] != 0.0)) {
+                double2* o = v == 0 ? p.out : p.out2;
+                unsafeAtomicAdd(&o[ia].x, acc_re[v]);
+                unsafeAtomicAdd(&o[ia].y, acc_im[v]);
+            }
         }
     }
 }
 
 // The pair work of the groups [g0, g0 + 64): groups of up to 8 entries through the tiny-group
 // classes, larger ones a row block at a time (pair_group).
-template <int W, bool ADJ, int B>
+template <int W, bool ADJ, int B, int NV>
 __device__ __forceinline__ void pair_chunk(const PairParams& p, int64_t groups, int64_t g0, int lane, char* stage_bytes) {
     const int64_t gg = g0 + lane;
     uint32_t gs = 0, ge = 0;
@@ -662,16 +685,16 @@ __device__ __forceinline__ void pair_chunk(const PairParams& p, int64_t groups, 
         ge = p.starts[gg + 1];
     }
     const uint32_t sz = ge - gs;
-    pair_tiny_class<W, ADJ>(p, __ballot(sz > 4u && sz <= 8u), gs, ge, 8, lane);
-    pair_tiny_class<W, ADJ>(p, __ballot(sz > 2u && sz <= 4u), gs, ge, 4, lane);
-    pair_tiny_class<W, ADJ>(p, __ballot(sz == 2u), gs, ge, 2, lane);
+    pair_tiny_class<W, ADJ, NV>(p, __ballot(sz > 4u && sz <= 8u), gs, ge, 8, lane);
+    pair_tiny_class<W, ADJ, NV>(p, __ballot(sz > 2u && sz <= 4u), gs, ge, 4, lane);
+    pair_tiny_class<W, ADJ, NV>(p, __ballot(sz == 2u), gs, ge, 2, lane);
     uint64_t big = __ballot(sz > 8u);
     // each wave owns 64 slots of the largest stage size: waves of one block may run different W
     PairStage<W>* stage = reinterpret_cast<PairStage<W>*>(stage_bytes + (threadIdx.x >> 6) * 64 * sizeof(PairStage<2>));
     while (big) {
         const int l = __builtin_ctzll(big);
         big &= big - 1;
-        pair_group<W, ADJ, B>(p, __shfl(gs, l), __shfl(ge, l), lane, stage);
+        pair_group<W, ADJ, B, NV>(p, __shfl(gs, l), __shfl(ge, l), lane, stage);
     }
 }
 
@@ -696,7 +719,7 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(PairParams p) {
     __shared__ __align__(16) char stage[kPairStageBytes];
     const int64_t groups = p.counts[1];
     for (int64_t g0 = wave * 64; g0 < groups; g0 += nwaves * 64) {
-        pair_chunk<W, ADJ, kPairBatch>(p, groups, g0, lane, stage);
+        pair_chunk<W, ADJ, kPairBatch, 1>(p, groups, g0, lane, stage);
     }
 }
 
@@ -761,6 +784,8 @@ struct SingleParams {
     const int64_t* counts;
     const double2* in;
     double2* out;
+    const double2* in2;  // second input/output of a two-vector pass (NV = 2), else null
+    double2* out2;
 };
 
 template <int W>
@@ -793,7 +818,7 @@ __device__ __forceinline__ double2 single_element(const SingleParams& p, const C
 // (config = core + q), q != r:
 //   ADJ:  out[a] += conj(M_ba) in[b],  M_ba = (-1)^single_parity(c_a; r, q) (A[r][q] + sum_{k in core} K[r][q][k])
 //   !ADJ: out[a] += M_ab in[b],        M_ab = (-1)^single_parity(c_b; q, r) (A[q][r] + sum_{k in core} K[q][r][k])
-template <int W, bool ADJ>
+template <int W, bool ADJ, int NV>
 __device__ void single_group(const SingleParams& p, uint32_t s, uint32_t e, int lane) {
     const uint64_t v0 = p.vals[s];
     Cfg<W> core0 = load_key<W>(p.ckeys, static_cast<int64_t>(v0 >> 16));
@@ -814,16 +839,18 @@ __device__ void single_group(const SingleParams& p, uint32_t s, uint32_t e, int 
             same = same && core.w[w] == core0.w[w];
         }
         const bool mixed = __ballot(act && !same) != 0ull;
-        double acc_re = 0.0, acc_im = 0.0;
+        double acc_re[NV] = {}, acc_im[NV] = {};
         for (uint32_t b0 = s; b0 < e; b0 += 64) {
             const uint32_t bl = b0 + lane;
             const bool bin = bl < e;
             const uint64_t vbl = bin ? p.vals[bl] : 0ull;
             const double2 gbl = bin ? p.in[vbl >> 16] : make_double2(0.0, 0.0);
+            const double2 gbl2 = (NV == 2 && bin) ? p.in2[vbl >> 16] : make_double2(0.0, 0.0);
             const int nb = static_cast<int>(e - b0 < 64u ? e - b0 : 64u);
             for (int u = 0; u < nb; ++u) {
                 const uint64_t vb = shfl_u64(vbl, u);
                 const double2 g = shfl_d2(gbl, u);
+                const double2 g2 = NV == 2 ? shfl_d2(gbl2, u) : make_double2(0.0, 0.0);
                 const int q = static_cast<int>(vb & 0xff);
                 bool ok = act && (b0 + static_cast<uint32_t>(u) != a) && q != r;
                 if (mixed && ok) {
@@ -850,24 +877,32 @@ __device__ void single_group(const SingleParams& p, uint32_t s, uint32_t e, int 
                 }
                 const double xr = sg ? -x.x : x.x;
                 const double xi = sg ? -x.y : x.y;
-                if (ADJ) {
-                    acc_re += xr * g.x + xi * g.y;
-                    acc_im += xr * g.y - xi * g.x;
-                } else {
-                    acc_re += xr * g.x - xi * g.y;
-                    acc_im += xr * g.y + xi * g.x;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    const double2 gg = v == 0 ? g : g2;
+                    if (ADJ) {
+                        acc_re[v] += xr * gg.x + xi * gg.y;
+                        acc_im[v] += xr * gg.y - xi * gg.x;
+                    } else {
+                        acc_re[v] += xr * gg.x - xi * gg.y;
+                        acc_im[v] += xr * gg.y + xi * gg.x;
+                    }
                 }
             }
         }
-        if (act && (acc_re != 0.0 || acc_im != 0.0)) {
-            unsafeAtomicAdd(&p.out[ia].x, acc_re);
-            unsafeAtomicAdd(&p.out[ia].y, acc_im);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            if (act && (acc_re[v] != 0.0 || acc_im[v] != 0.0)) {
+                double2* o = v == 0 ? p.out : p.out2;
+                unsafeAtomicAdd(&o[ia].x, acc_re[v]);
+                unsafeAtomicAdd(&o[ia].y, acc_im[v]);
+            }
         }
     }
 }
 
 // Single-excitation groups of 2..8 entries: the lane-per-(row, entry) scheme of pair_tiny_class.
-template <int W, bool ADJ>
+template <int W, bool ADJ, int NV>
 __device__ void single_tiny_class(const SingleParams& p, uint64_t mask, uint32_t gs_l, uint32_t ge_l, int gp, int lane) {
     const int lg = gp == 8 ? 3 : (gp == 4 ? 2 : 1);
     const int bs = gp * gp;
@@ -913,9 +948,8 @@ __device__ void single_tiny_class(const SingleParams& p, uint64_t mask, uint32_t
             same = same && shfl_u64(core.w[w], src) == core.w[w];
         }
         const bool ok = rowa && rowb && ai != bi && same && q != r;
-        double acc_re = 0.0, acc_im = 0.0;
+        double acc_re[NV] = {}, acc_im[NV] = {};
         if (ok) {
-            const double2 gv = p.in[vb >> 16];
             double2 x;
             int sg;
             if (ADJ) {
@@ -929,26 +963,34 @@ __device__ void single_tiny_class(const SingleParams& p, uint64_t mask, uint32_t
             }
             const double xr = sg ? -x.x : x.x;
             const double xi = sg ? -x.y : x.y;
-            if (ADJ) {
-                acc_re = xr * gv.x + xi * gv.y;
-                acc_im = xr * gv.y - xi * gv.x;
-            } else {
-                acc_re = xr * gv.x - xi * gv.y;
-                acc_im = xr * gv.y + xi * gv.x;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double2 gv = (v == 0 ? p.in : p.in2)[vb >> 16];
+                if (ADJ) {
+                    acc_re[v] = xr * gv.x + xi * gv.y;
+                    acc_im[v] = xr * gv.y - xi * gv.x;
+                } else {
+                    acc_re[v] = xr * gv.x - xi * gv.y;
+                    acc_im[v] = xr * gv.y + xi * gv.x;
+                }
             }
         }
-        for (int m = 1; m < gp; m <<= 1) {
-            acc_re += __shfl_xor(acc_re, m);
-            acc_im += __shfl_xor(acc_im, m);
-        }
-        if (bi == 0 && rowa && (acc_re != 0.0 || acc_im != 0.0)) {
-            unsafeAtomicAdd(&p.out[ia].x, acc_re);
-            unsafeAtomicAdd(&p.out[ia].y, acc_im);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            for (int m = 1; m < gp; m <<= 1) {
+                acc_re[v] += __shfl_xor(acc_re[v], m);
+                acc_im[v] += __shfl_xor(acc_im[v], m);
+            }
+            if (bi == 0 && rowa && (acc_re[v] != 0.0 || acc_im[v] != 0.0)) {
+                double2* o = v == 0 ? p.out : p.out2;
+                unsafeAtomicAdd(&o[ia].x, acc_re[v]);
+                unsafeAtomicAdd(&o[ia].y, acc_im[v]);
+            }
         }
     }
 }
 
-template <int W, bool ADJ>
+template <int W, bool ADJ, int NV>
 __device__ __forceinline__ void single_chunk(const SingleParams& p, int64_t groups, int64_t g0, int lane) {
     const int64_t gg = g0 + lane;
     uint32_t gs = 0, ge = 0;
@@ -957,14 +999,14 @@ __device__ __forceinline__ void single_chunk(const SingleParams& p, int64_t grou
         ge = p.starts[gg + 1];
     }
     const uint32_t sz = ge - gs;
-    single_tiny_class<W, ADJ>(p, __ballot(sz > 4u && sz <= 8u), gs, ge, 8, lane);
-    single_tiny_class<W, ADJ>(p, __ballot(sz > 2u && sz <= 4u), gs, ge, 4, lane);
-    single_tiny_class<W, ADJ>(p, __ballot(sz == 2u), gs, ge, 2, lane);
+    single_tiny_class<W, ADJ, NV>(p, __ballot(sz > 4u && sz <= 8u), gs, ge, 8, lane);
+    single_tiny_class<W, ADJ, NV>(p, __ballot(sz > 2u && sz <= 4u), gs, ge, 4, lane);
+    single_tiny_class<W, ADJ, NV>(p, __ballot(sz == 2u), gs, ge, 2, lane);
     uint64_t big = __ballot(sz > 8u);
     while (big) {
         const int l = __builtin_ctzll(big);
         big &= big - 1;
-        single_group<W, ADJ>(p, __shfl(gs, l), __shfl(ge, l), lane);
+        single_group<W, ADJ, NV>(p, __shfl(gs, l), __shfl(ge, l), lane);
     }
 }
 
@@ -981,7 +1023,7 @@ __global__ __launch_bounds__(kBlock) void single_kernel(SingleParams p) {
     }
     const int64_t groups = p.counts[1];
     for (int64_t g0 = wave * 64; g0 < groups; g0 += nwaves * 64) {
-        single_chunk<W, ADJ>(p, groups, g0, lane);
+        single_chunk<W, ADJ, 1>(p, groups, g0, lane);
     }
 }
 
@@ -1062,11 +1104,13 @@ struct BatchItemDev {
     const int64_t* scounts;
     const double2* in;
     double2* out;
+    const double2* in2;  // two-vector batches: the second input and output
+    double2* out2;
 };
 
 constexpr int kBatchMaxItems = 4096;
 
-template <int W, bool ADJ>
+template <int W, bool ADJ, int NV>
 __device__ void diag_rows(const BatchItemDev& it, int64_t r0, int lane) {
     const int64_t i = r0 + lane;
     if (i >= it.n) {
@@ -1075,14 +1119,18 @@ __device__ void diag_rows(const BatchItemDev& it, int64_t r0, int lane) {
     const double2 d = diag_value<W>(load_key<W>(it.ckeys, i), it.diag, it.n_bits);
     const double im = ADJ ? -d.y : d.y;
     if (d.x != 0.0 || im != 0.0) {
-        const double2 g = it.in[i];
-        // pair and single units of the same launch add into these rows concurrently
-        unsafeAtomicAdd(&it.out[i].x, d.x * g.x - im * g.y);
-        unsafeAtomicAdd(&it.out[i].y, d.x * g.y + im * g.x);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double2 g = (v == 0 ? it.in : it.in2)[i];
+            double2* o = v == 0 ? it.out : it.out2;
+            // pair and single units of the same launch add into these rows concurrently
+            unsafeAtomicAdd(&o[i].x, d.x * g.x - im * g.y);
+            unsafeAtomicAdd(&o[i].y, d.x * g.y + im * g.x);
+        }
     }
 }
 
-template <int W, bool ADJ>
+template <int W, bool ADJ, int NV>
 __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int lane, char* stage) {
     if (type == 0) {
         PairParams p{};
@@ -1097,9 +1145,11 @@ __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int 
         p.counts = it.pcounts;
         p.in = it.in;
         p.out = it.out;
+        p.in2 = it.in2;
+        p.out2 = it.out2;
         const int64_t groups = it.pcounts[1];
         if (local * 64 < groups && it.pcounts[2] == 0) {
-            pair_chunk<W, ADJ, kPairBatchInBatch>(p, groups, local * 64, lane, stage);
+            pair_chunk<W, ADJ, kPairBatchInBatch, NV>(p, groups, local * 64, lane, stage);
         }
     } else if (type == 1) {
         SingleParams p{};
@@ -1113,20 +1163,25 @@ __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int 
         p.counts = it.scounts;
         p.in = it.in;
         p.out = it.out;
+        p.in2 = it.in2;
+        p.out2 = it.out2;
         const int64_t groups = it.scounts[1];
         if (local * 64 < groups && it.scounts[2] == 0) {
-            single_chunk<W, ADJ>(p, groups, local * 64, lane);
+            single_chunk<W, ADJ, NV>(p, groups, local * 64, lane);
         }
     } else {
-        diag_rows<W, ADJ>(it, local * 64, lane);
+        diag_rows<W, ADJ, NV>(it, local * 64, lane);
         if (local == 0 && lane == 0 &&
             ((it.has_pairs && it.pcounts[2] != 0) || (it.has_singles && it.scounts[2] != 0))) {
             mark_overflow(it.out);
+            if (NV == 2) {
+                mark_overflow(it.out2);
+            }
         }
     }
 }
 
-template <bool ADJ>
+template <bool ADJ, int NV>
 __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchItemDev* __restrict__ items, const int64_t* __restrict__ offsets,
                                                        int n_items, int64_t total) {
     const int lane = threadIdx.x & 63;
@@ -1149,9 +1204,9 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchItemDev* __res
         const BatchItemDev& it = items[lo - type * n_items];
         const int64_t local = u - offsets[lo];
         if (it.n_words == 1) {
-            batch_unit<1, ADJ>(it, type, local, lane, stage);
+            batch_unit<1, ADJ, NV>(it, type, local, lane, stage);
         } else {
-            batch_unit<2, ADJ>(it, type, local, lane, stage);
+            batch_unit<2, ADJ, NV>(it, type, local, lane, stage);
         }
     }
 }
@@ -1448,6 +1503,7 @@ int batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t by
     }
     b.bytes = bytes;
     std::vector<BatchItemDev> dev(static_cast<size_t>(n_items));
+    int dual = 0;
     std::vector<int64_t> units(3 * static_cast<size_t>(n_items), 0);
     for (int k = 0; k < n_items; ++k) {
         const qmb_batch_item& x = items[k];
@@ -1494,6 +1550,13 @@ int batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t by
         units[2 * n_items + k] = (tg.n + 63) / 64;
         d.in = reinterpret_cast<const double2*>(x.in);
         d.out = reinterpret_cast<double2*>(x.out);
+        d.in2 = reinterpret_cast<const double2*>(x.in2);
+        d.out2 = reinterpret_cast<double2*>(x.out2);
+        const int two = (x.in2 != nullptr && x.out2 != nullptr) ? 1 : 0;
+        if ((x.in2 != nullptr) != (x.out2 != nullptr) || (k > 0 && two != dual)) {
+            return set_error(QMB_ERR_ARGUMENT, "item %d: in2 and out2 must be given together, for every item or none", k);
+        }
+        dual = two;
         dev[static_cast<size_t>(k)] = d;
     }
     std::vector<int64_t> offsets(units.size() + 1, 0);
@@ -1501,6 +1564,7 @@ int batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t by
         offsets[q + 1] = offsets[q] + units[q];
     }
     b.total_units = offsets.back();
+    b.dual = dual;
     QMB_HIP_TRY(hipMemcpyAsync(b.items, dev.data(), dev.size() * sizeof(BatchItemDev), hipMemcpyHostToDevice, stream));
     QMB_HIP_TRY(hipMemcpyAsync(b.offsets, offsets.data(), offsets.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
     // the host tables above go out of scope on return
@@ -1525,10 +1589,16 @@ int apply_batch(const qmb_batch* batch, int mode, int waves, hipStream_t stream)
     const int64_t need = (batch->total_units + 3) / 4;
     blocks = blocks < need ? blocks : need;
     const auto* items = static_cast<const BatchItemDev*>(batch->items);
-    if (mode == 1) {
-        batch_kernel<true><<<static_cast<unsigned>(blocks), kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
+    const unsigned g = static_cast<unsigned>(blocks);
+    if (batch->dual) {
+        if (mode != 1) {
+            return set_error(QMB_ERR_ARGUMENT, "a two-vector batch applies the adjoint gather (mode 1) to both vectors");
+        }
+        batch_kernel<true, 2><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
+    } else if (mode == 1) {
+        batch_kernel<true, 1><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
     } else {
-        batch_kernel<false><<<static_cast<unsigned>(blocks), kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
+        batch_kernel<false, 1><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
     }
     QMB_CHECK_LAUNCH();
     return QMB_OK;

@@ -133,6 +133,8 @@ class QmbBatchItem(ctypes.Structure):
         ("cores1", ctypes.POINTER(QmbCores)),
         ("in_", _vp),
         ("out", _vp),
+        ("in2", _vp),
+        ("out2", _vp),
     ]
 
 
@@ -141,7 +143,7 @@ class QmbBatch(ctypes.Structure):
 
     _fields_ = [
         ("n_items", _c_i32),
-        ("reserved", _c_i32),
+        ("dual", _c_i32),
         ("total_units", _c_i64),
         ("buffer", _vp),
         ("bytes", _c_size),

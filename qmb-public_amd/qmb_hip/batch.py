@@ -28,7 +28,10 @@ WAVE_CANDIDATES = (1024, 2048, 4096, 8192, 16384, 32768)
 class Batch:
     """Packed inputs/outputs + the device plan for a list of (terms, targets) instances."""
 
-    def __init__(self, instances: typing.Sequence[tuple[pp.PreparedTerms, pp.PreparedTargets]]) -> None:
+    def __init__(self, instances: typing.Sequence[tuple[pp.PreparedTerms, pp.PreparedTargets]], two_vectors: bool = False) -> None:
+        """two_vectors=True (Hermitian instances only) adds a second packed input/output, so that
+        launch_both() applies every instance to both in one pass: the forward and the backward of
+        a training step."""
         if not instances:
             raise ValueError("a batch needs at least one instance")
         device = instances[0][1].device
@@ -52,6 +55,12 @@ class Batch:
         self.inp = torch.zeros((max(off, 1), 2), dtype=torch.float64, device=device)
         self.out = torch.zeros((max(off, 1), 2), dtype=torch.float64, device=device)
         self.hermitian = all(t.hermitian for t, _ in self.instances)
+        self.two_vectors = two_vectors
+        if two_vectors:
+            if not self.hermitian:
+                raise ValueError("a two-vector batch needs Hermitian instances (both vectors take the same map)")
+            self.inp2 = torch.zeros_like(self.inp)
+            self.out2 = torch.zeros_like(self.out)
         n = len(self.instances)
         self._items = (_lib.QmbBatchItem * n)()
         for k, (terms, targets) in enumerate(self.instances):
@@ -65,6 +74,9 @@ class Batch:
             it.cores1 = ctypes.pointer(targets.cores(1)) if terms.singles_desc is not None else None
             it.in_ = self.inp.data_ptr() + 16 * lo
             it.out = self.out.data_ptr() + 16 * lo
+            if two_vectors:
+                it.in2 = self.inp2.data_ptr() + 16 * lo
+                it.out2 = self.out2.data_ptr() + 16 * lo
         lib = _lib.library()
         size = ctypes.c_size_t(0)
         _lib.check(lib.qmb_batch_workspace_size(n, ctypes.byref(size)), "qmb_batch_workspace_size")
@@ -77,28 +89,61 @@ class Batch:
         self.residual = [k for k, (t, _) in enumerate(self.instances) if t.n_residual_terms > 0]
 
     # ------------------------------------------------------------------ inputs and outputs
-    def set_inputs(self, psis: typing.Sequence[torch.Tensor]) -> None:
-        """Copies each instance's complex128 vector into the packed input buffer."""
-        if len(psis) != len(self.instances):
-            raise ValueError("one vector per instance")
-        for (lo, hi), psi in zip(self.rows, psis):
-            if psi.dtype != torch.complex128 or psi.shape != (hi - lo,):
-                raise RuntimeError("each vector must be complex128 with one entry per config of its instance")
-            self.inp[lo:hi].copy_(torch.view_as_real(psi.resolve_conj()))
+    def set_inputs(self, psis: typing.Sequence[torch.Tensor], second: typing.Sequence[torch.Tensor] | None = None) -> None:
+        """Copies each instance's complex128 vector into the packed input buffer (and, for a
+        two-vector batch, the second vectors into the second buffer)."""
+        for buf, vecs in ((self.inp, psis), (getattr(self, "inp2", None), second)):
+            if vecs is None:
+                continue
+            if buf is None:
+                raise ValueError("second vectors need a two-vector batch")
+            if len(vecs) != len(self.instances):
+                raise ValueError("one vector per instance")
+            for (lo, hi), psi in zip(self.rows, vecs):
+                if psi.dtype != torch.complex128 or psi.shape != (hi - lo,):
+                    raise RuntimeError("each vector must be complex128 with one entry per config of its instance")
+                buf[lo:hi].copy_(torch.view_as_real(psi.resolve_conj()))
 
-    def output(self, k: int) -> torch.Tensor:
-        """Instance k's result as a complex128 view of the packed output buffer."""
+    def output(self, k: int, second: bool = False) -> torch.Tensor:
+        """Instance k's result as a complex128 view of the packed output buffer (or the second)."""
         lo, hi = self.rows[k]
-        return torch.view_as_complex(self.out[lo:hi])
+        return torch.view_as_complex((self.out2 if second else self.out)[lo:hi])
 
     def input(self, k: int) -> torch.Tensor:
         lo, hi = self.rows[k]
         return torch.view_as_complex(self.inp[lo:hi])
 
     # ------------------------------------------------------------------------------ launch
+    def launch_both(self, waves: int | None = None) -> None:
+        """out = H in and out2 = H in2 for every (Hermitian) instance in ONE persistent launch, plus
+        the residual enumeration per instance that needs it.  Enqueue-only (graph capturable)."""
+        if not self.two_vectors:
+            raise RuntimeError("launch_both needs a batch built with two_vectors=True")
+        self.out.zero_()
+        self.out2.zero_()
+        lib = _lib.library()
+        stream = _stream(self.device)
+        w = self.waves if waves is None else waves
+        with torch.cuda.device(self.device):
+            _lib.check(lib.qmb_apply_batch(ctypes.byref(self.plan), pp.MODE_ADJOINT, w, stream), "qmb_apply_batch")
+            for k in self.residual:
+                for inp, out in ((self.inp, self.out), (self.inp2, self.out2)):
+                    self._residual(k, pp.MODE_ADJOINT, inp, out, stream)
+
+    def _residual(self, k: int, mode: int, inp: torch.Tensor, out: torch.Tensor, stream: ctypes.c_void_p) -> None:
+        terms, targets = self.instances[k]
+        lo, _ = self.rows[k]
+        launch = pp._launch(0, 0, 0, self.hermitian)  # pylint: disable=protected-access
+        _lib.check(_lib.library().qmb_apply_prepared(ctypes.byref(terms.residual_desc), ctypes.byref(targets.desc), _ptr(targets.configs),
+                                                     targets.n, ctypes.c_void_p(inp.data_ptr() + 16 * lo),
+                                                     ctypes.c_void_p(out.data_ptr() + 16 * lo), mode,
+                                                     ctypes.byref(launch) if launch else None, stream), "qmb_apply_prepared")
+
     def launch(self, mode: int, waves: int | None = None) -> None:
         """out = A in (mode 0) or A^H in (mode 1) for every instance: one persistent launch plus one
         enumeration launch per instance with residual terms.  Enqueue-only (graph capturable)."""
+        if self.two_vectors:
+            raise RuntimeError("a two-vector batch runs with launch_both()")
         self.out.zero_()
         lib = _lib.library()
         stream = _stream(self.device)
@@ -107,19 +152,13 @@ class Batch:
         with torch.cuda.device(self.device):
             _lib.check(lib.qmb_apply_batch(ctypes.byref(self.plan), pmode, w, stream), "qmb_apply_batch")
             for k in self.residual:
-                terms, targets = self.instances[k]
-                lo, _ = self.rows[k]
-                launch = pp._launch(0, 0, 0, self.hermitian)  # pylint: disable=protected-access
-                _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.residual_desc), ctypes.byref(targets.desc), _ptr(targets.configs),
-                                                  targets.n, ctypes.c_void_p(self.inp.data_ptr() + 16 * lo),
-                                                  ctypes.c_void_p(self.out.data_ptr() + 16 * lo), mode,
-                                                  ctypes.byref(launch) if launch else None, stream), "qmb_apply_prepared")
+                self._residual(k, mode, self.inp, self.out, stream)
 
     def tune(self, retune: bool = False) -> int:
         """Picks the number of persistent waves with the autotuner (runs the batch on the current
         inputs; overwrites the outputs)."""
-        key = f"batch:n={len(self.instances)}:rows={self.n_rows}:units={self.plan.total_units}"
-        best = TUNER.best(key, [{"waves": w} for w in WAVE_CANDIDATES], lambda c: self.launch(pp.MODE_FORWARD, c["waves"]),
-                          retune=retune)
+        key = f"batch:n={len(self.instances)}:rows={self.n_rows}:units={self.plan.total_units}:two={int(self.two_vectors)}"
+        run = (lambda c: self.launch_both(c["waves"])) if self.two_vectors else (lambda c: self.launch(pp.MODE_FORWARD, c["waves"]))
+        best = TUNER.best(key, [{"waves": w} for w in WAVE_CANDIDATES], run, retune=retune)
         self.waves = int(best["waves"])
         return self.waves

@@ -9,6 +9,8 @@ qmb_hip.batch.Batch):
     apart and adds the residual enumeration per instance;
   - a hipGraph of the batched forward + adjoint, replayed several times with eager work between
     replays, reproduces the eager result on every replay;
+  - a two-vector batch applies every instance to two vectors in one launch (forward and backward of
+    a training step for Hermitian instances), equal to the per-instance results, graph replays too;
   - the autotuner picks a wave count among its candidates without changing the result.
 """
 
@@ -132,6 +134,49 @@ def test_batch_mixed_word_counts(cuda_device: torch.device) -> None:
             ref = _reference(terms, targets, psis[k], pp.MODE_FORWARD)
             got = b.output(k).cpu().numpy()
             np.testing.assert_array_less(np.abs(got - ref), 1e-12 * _scale(ref, 10.0, psis[k]) + 1e-300)
+
+
+def test_batch_two_vectors_in_one_launch(cuda_device: torch.device) -> None:
+    """A two-vector batch gives, for both vectors, what each instance's own application gives;
+    one and two config words mixed in the batch, and its graph replays match."""
+    s1, k1, c1 = sorted_terms(*sy.molecular_terms(10, 81))
+    cfg1 = sy.excitation_configs(10, 4, 4, 2500, 82)
+    s2, k2, c2 = sorted_terms(*sy.molecular_terms(34, 83, threshold=1e-6))
+    cfg2 = sy.excitation_configs(34, 3, 3, 3000, 84)
+    instances = []
+    for s, k, c, cfg in ((s1, k1, c1, cfg1), (s2, k2, c2, cfg2)):
+        nq = cfg.shape[1]
+        terms = pp.PreparedTerms(_dev(s, cuda_device), _dev(k, cuda_device), _dev(c, cuda_device), nq, 1)
+        instances.append((terms, pp.PreparedTargets(_dev(cfg, cuda_device), nq)))
+    b = qb.Batch(instances, two_vectors=True)
+    psis = [torch.from_numpy(sy.random_psi(t.n, 500 + k)).to(cuda_device) for k, (_, t) in enumerate(instances)]
+    grads = [torch.from_numpy(sy.random_psi(t.n, 600 + k)).to(cuda_device) for k, (_, t) in enumerate(instances)]
+    b.set_inputs(psis, grads)
+    b.launch_both()
+    eager = (b.out.clone(), b.out2.clone())
+    for k, (terms, targets) in enumerate(instances):
+        for second, vec in ((False, psis[k]), (True, grads[k])):
+            ref = _reference(terms, targets, vec, pp.MODE_FORWARD)
+            got = b.output(k, second).cpu().numpy()
+            np.testing.assert_array_less(np.abs(got - ref), 1e-12 * _scale(ref, 10.0, vec) + 1e-300)
+    with pytest.raises(RuntimeError):
+        b.launch(pp.MODE_FORWARD)
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream(device=cuda_device)
+    side.wait_stream(torch.cuda.current_stream(cuda_device))
+    with torch.cuda.stream(side):
+        b.launch_both()
+    torch.cuda.current_stream(cuda_device).wait_stream(side)
+    with torch.cuda.graph(graph):
+        b.launch_both()
+    for _ in range(3):
+        b.out.fill_(float("nan"))
+        b.out2.fill_(float("nan"))
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        for got, want in zip((b.out, b.out2), eager):
+            assert (got - want).abs().max().item() <= 1e-12 * want.abs().max().item()
 
 
 def test_batch_graph_replays_match_eager(cuda_device: torch.device) -> None:
