@@ -16,6 +16,8 @@ Workloads (--workload):
   hubbard128:            open 1-D Hubbard chain, 64 sites (128 spin orbitals), t=1, U=4, half
                          filling, 4,000,000 configs (random seeds + single hops), configs_j = configs_i.
   molecular20:           20 spin orbitals, 5a+5b, 10,000 configs uniform in the sector.
+  batch64:               64 seeded molecular instances, forward + backward in one persistent launch.
+  lanczos52:             Lanczos iterations on the molecular52 basis (application + vector step).
 
 Prints one JSON line with the metric, a roofline entry for the forward kernel (algorithmic bytes per
 launch / its HIP-event time) and the reference CPU op timed on this host on a bounded sample.
@@ -287,12 +289,104 @@ def run_batch(args: argparse.Namespace, dev: object) -> int:
     return 0
 
 
+def run_lanczos(args: argparse.Namespace, dev: object) -> int:
+    """Lanczos on the molecular52 basis (qmb/haar.py:107-142): every iteration is one application
+    through the structured path plus the fused vector step (alpha, three-term update, norm,
+    normalisation) and one 16-byte read-back; timed per iteration, with its parts timed alone and
+    the reference's unfused tensor arithmetic for the same vector work beside it."""
+    import numpy as np  # pylint: disable=import-outside-toplevel
+    import torch  # pylint: disable=import-outside-toplevel
+
+    from qmb_hip import Hamiltonian  # pylint: disable=import-outside-toplevel
+    from qmb_hip import krylov as kr  # pylint: disable=import-outside-toplevel
+    from qmb_hip import prepared as pp  # pylint: disable=import-outside-toplevel
+
+    w = build_workload("molecular52")
+    configs = torch.from_numpy(w["configs"]).to(dev)
+    ham = Hamiltonian(tuple(t.contiguous().to(dev) for t in (w["site"], w["kind"], w["coef"])), kind="fermi")
+    psi = torch.from_numpy(w["psi"]).to(dev)
+    t0 = time.perf_counter()
+    op = kr.WithinOperator(ham, configs)
+    op(psi)  # builds the core indexes
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    m_gen, m_hit = pp.count(op.terms, op.targets, configs)
+    stream = torch.cuda.current_stream(dev)
+    steps = max(args.steps, 2)
+    gen = kr.lanczos(op, psi, threshold=1e-12)
+    for _ in range(max(args.warmup, 1)):
+        next(gen)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        alpha, beta, v = next(gen)
+    torch.cuda.synchronize()
+    iter_ms = (time.perf_counter() - t0) * 1e3 / steps
+    energy, _ = kr.eigh_tridiagonal_lowest(alpha, beta, v, dev)
+
+    def timed(fn: object, reps: int = 10) -> float:
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    vk, vprev = v[-1], v[-2]
+    wvec = op(vk)
+    fused = kr.LanczosStep(dev)
+    scratch = wvec.clone()
+
+    def fused_step() -> None:
+        scratch.copy_(wvec)
+        fused(vk, vprev, 0.5, scratch)
+
+    def unfused_step() -> None:
+        # the reference's arithmetic between two applications (qmb/haar.py:130-138)
+        a = (wvec.conj() @ vk).real
+        x = wvec - a * vk - 0.5 * vprev
+        nrm = torch.linalg.norm(x)  # pylint: disable=not-callable
+        _ = bool(nrm < 1e-12)
+        _ = x / nrm
+
+    copy_ms = timed(lambda: scratch.copy_(wvec))
+    apply_ms = timed(lambda: op(vk))
+    fused_ms = timed(fused_step) - copy_ms
+    unfused_ms = timed(unfused_step)
+    log(f"lanczos52: {iter_ms:.3f} ms/iteration (application {apply_ms:.3f} ms, fused vector step {fused_ms:.3f} ms, "
+        f"reference tensor arithmetic {unfused_ms:.3f} ms); lowest Ritz value after {len(alpha)} iterations {float(energy):.10f}")
+    result = {
+        "metric": "Lanczos iterations/s",
+        "value": 1e3 / iter_ms,
+        "unit": "iterations/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": max(args.warmup, 1),
+        "ms_per_step": iter_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded)",
+        "config": {"workload": "lanczos52", "basis": w["desc"], "M_gen": m_gen, "M_hit": m_hit},
+        "lanczos": {"application_ms": apply_ms, "fused_vector_step_ms": fused_ms, "reference_tensor_arithmetic_ms": unfused_ms,
+                    "setup_s": setup_s, "iterations": len(alpha), "lowest_ritz_value": float(energy),
+                    "matrix_elements_per_s": m_hit / (iter_ms * 1e-3)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=10)
     parser.add_argument("--warmup", type=int, default=3)
-    parser.add_argument("--workload", default="molecular52", choices=["molecular52", "hubbard128", "molecular20", "batch64"])
+    parser.add_argument("--workload", default="molecular52", choices=["molecular52", "hubbard128", "molecular20", "batch64", "lanczos52"])
     parser.add_argument("--tile", type=int, default=0)
     parser.add_argument("--groups", type=int, default=0)
     parser.add_argument("--no-plain", action="store_true", help="skip the reference-scheme comparator")
@@ -318,6 +412,8 @@ def main() -> int:
     dev = torch.device("cuda:0")
     if args.workload == "batch64":
         return run_batch(args, dev)
+    if args.workload == "lanczos52":
+        return run_lanczos(args, dev)
     w = build_workload(args.workload)
     configs = torch.from_numpy(w["configs"]).to(dev)
     nq = configs.shape[1]

@@ -339,6 +339,19 @@ int qmb_batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_
  * step in one launch. */
 int qmb_apply_batch(const qmb_batch* batch, int mode, int waves, void* stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Lanczos vector step.  Replaces the tensor arithmetic between two applications in the Krylov
+ * loops of qmb/haar.py:123-142 (_DynamicLanczos._run) and qmb/rldiag.py:23-44 (lanczos_energy):
+ *     alpha = Re(w^H v);  w -= alpha v + beta v_prev;  |w|;  v_next = w / |w|
+ * on complex128 vectors stored as float64 [n][2].  On entry w = H v; v_prev may be null (the first
+ * step: no beta term).  Writes scalars[0] = alpha and scalars[1] = |w| (device doubles), updates w
+ * in place and writes v_next (zeros when |w| = 0).  Deterministic: fixed-order reductions.  The
+ * workspace is a fixed 16 KiB of partial sums; calls that share one workspace must be ordered.
+ * ------------------------------------------------------------------------------------------- */
+int qmb_lanczos_workspace_size(size_t* bytes);
+int qmb_lanczos_step(int64_t n, const double* v, const double* v_prev, double beta, double* w, double* v_next, double* scalars,
+                     void* workspace, size_t bytes, void* stream);
+
 /* Host-only pair-excitation test of a compiled record (for unit tests): returns 1 and writes
  * {r0, r1, c0, c1, delta} when the record is a pair excitation, else 0. */
 int qmb_pair_excitation_host(uint64_t rec, int particle_cut, int32_t out[5]);

@@ -440,6 +440,21 @@ int qmb_apply_batch(const qmb_batch* batch, int mode, int waves, void* stream) {
     return apply_batch(batch, mode, waves, as_stream(stream));
 }
 
+int qmb_lanczos_workspace_size(size_t* bytes) {
+    clear_error();
+    if (!bytes) {
+        return set_error(QMB_ERR_ARGUMENT, "bytes must be non-null");
+    }
+    *bytes = lanczos_bytes();
+    return QMB_OK;
+}
+
+int qmb_lanczos_step(int64_t n, const double* v, const double* v_prev, double beta, double* w, double* v_next, double* scalars,
+                     void* workspace, size_t bytes, void* stream) {
+    clear_error();
+    return lanczos_step(n, v, v_prev, beta, w, v_next, scalars, workspace, bytes, as_stream(stream));
+}
+
 int qmb_single_excitation_host(uint64_t rec, int particle_cut, int32_t out[5]) {
     clear_error();
     SingleExcitation e;

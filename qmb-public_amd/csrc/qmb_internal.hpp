@@ -160,6 +160,10 @@ size_t batch_bytes(int n_items);
 int batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t bytes, qmb_batch* out, hipStream_t stream);
 int apply_batch(const qmb_batch* batch, int mode, int waves, hipStream_t stream);
 
+size_t lanczos_bytes();
+int lanczos_step(int64_t n, const double* v, const double* v_prev, double beta, double* w, double* v_next, double* scalars,
+                 void* workspace, size_t bytes, hipStream_t stream);
+
 size_t find_relative_scratch_bytes(int n_qubytes, int64_t count_selected);
 int find_relative_prepared(const qmb_terms* terms, const qmb_targets* exclude, const uint8_t* configs_i, const double* psi_i,
                            int64_t n_i, int64_t count_selected, uint8_t* out, int64_t* out_rows, void* scratch,

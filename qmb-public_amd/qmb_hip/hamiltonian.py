@@ -18,6 +18,7 @@ import typing
 import torch
 
 from . import ops
+from .prepared import PreparedTerms
 from .synthetic import terms_from_dict
 
 
@@ -85,6 +86,18 @@ class Hamiltonian:
         self.site = self.site.to(device=device).contiguous()
         self.kind = self.kind.to(device=device).contiguous()
         self.coef = self.coef.to(device=device).contiguous()
+
+    def prepared_terms(self, device: torch.device, n_qubytes: int) -> PreparedTerms:
+        """The compiled term table (buckets, structured pair/single/diagonal tables) for configs of
+        n_qubytes bytes on `device`, built on first use and kept while site/kind/coef stay the same
+        tensors.  Repeated applications on one config set (Krylov loops) start from it."""
+        self._prepare_data(device)
+        key = (str(device), n_qubytes, self.site.data_ptr(), self.kind.data_ptr(), self.coef.data_ptr(), self.site.size(0))
+        cached = getattr(self, "_prepared", None)
+        if cached is None or cached[0] != key:
+            cached = (key, PreparedTerms(self.site, self.kind, self.coef, n_qubytes, self.particle_cut))
+            self._prepared = cached  # pylint: disable=attribute-defined-outside-init
+        return cached[1]
 
     def apply_within(self, configs_i: torch.Tensor, psi_i: torch.Tensor, configs_j: torch.Tensor) -> torch.Tensor:
         """
