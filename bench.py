@@ -18,6 +18,7 @@ Workloads (--workload):
   molecular20:           20 spin orbitals, 5a+5b, 10,000 configs uniform in the sector.
   batch64:               64 seeded molecular instances, forward + backward in one persistent launch.
   lanczos52:             Lanczos iterations on the molecular52 basis (application + vector step).
+  pt2_52:                the perturbation step (E0, find_relative, application into the external set).
 
 Prints one JSON line with the metric, a roofline entry for the forward kernel (algorithmic bytes per
 launch / its HIP-event time) and the reference CPU op timed on this host on a bounded sample.
@@ -381,12 +382,72 @@ def run_lanczos(args: argparse.Namespace, dev: object) -> int:
     return 0
 
 
+def run_pt2(args: argparse.Namespace, dev: object) -> int:
+    """The perturbation step (qmb/pert.py:37-63) on the molecular52 basis: E0 on the set,
+    find_relative with count = len(configs), the application into the selected external set and its
+    diagonal, each timed with events."""
+    import torch  # pylint: disable=import-outside-toplevel
+
+    from qmb_hip import Hamiltonian  # pylint: disable=import-outside-toplevel
+    from qmb_hip import krylov as kr  # pylint: disable=import-outside-toplevel
+    from qmb_hip import pt2  # pylint: disable=import-outside-toplevel
+
+    w = build_workload("molecular52")
+    configs = torch.from_numpy(w["configs"]).to(dev)
+    ham = Hamiltonian(tuple(t.contiguous().to(dev) for t in (w["site"], w["kind"], w["coef"])), kind="fermi")
+    psi = torch.from_numpy(w["psi"]).to(dev)
+    stream = torch.cuda.current_stream(dev)
+    for _ in range(max(args.warmup, 1)):
+        res = pt2.perturbation(ham, configs, psi)
+    torch.cuda.synchronize()
+    steps = max(args.steps, 1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = pt2.perturbation(ham, configs, psi)
+    torch.cuda.synchronize()
+    step_ms = (time.perf_counter() - t0) * 1e3 / steps
+
+    def timed(fn: object) -> float:
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b)
+
+    op = kr.WithinOperator(ham, configs)
+    parts = {"energy0_ms": timed(lambda: op(psi)),
+             "find_relative_ms": timed(lambda: ham.find_relative(configs, psi, configs.size(0), configs)),
+             "apply_into_external_ms": timed(lambda: pt2._apply_foreign(ham, configs, psi, res.other)),  # pylint: disable=protected-access
+             "diagonal_term_ms": timed(lambda: ham.diagonal_term(res.other))}
+    log(f"pt2_52: {step_ms:.1f} ms per step, parts {parts}; E0 {res.energy0:.10f} E2 {res.energy2:.10f} over {res.other.size(0)} configs")
+    result = {
+        "metric": "perturbation steps/s",
+        "value": 1e3 / step_ms,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": max(args.warmup, 1),
+        "ms_per_step": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded)",
+        "config": {"workload": "pt2_52", "basis": w["desc"], "count": int(configs.size(0)), "external_configs": int(res.other.size(0))},
+        "pt2": dict(parts, energy0=res.energy0, energy2=res.energy2),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=10)
     parser.add_argument("--warmup", type=int, default=3)
-    parser.add_argument("--workload", default="molecular52", choices=["molecular52", "hubbard128", "molecular20", "batch64", "lanczos52"])
+    parser.add_argument("--workload", default="molecular52", choices=["molecular52", "hubbard128", "molecular20", "batch64", "lanczos52", "pt2_52"])
     parser.add_argument("--tile", type=int, default=0)
     parser.add_argument("--groups", type=int, default=0)
     parser.add_argument("--no-plain", action="store_true", help="skip the reference-scheme comparator")
@@ -414,6 +475,8 @@ def main() -> int:
         return run_batch(args, dev)
     if args.workload == "lanczos52":
         return run_lanczos(args, dev)
+    if args.workload == "pt2_52":
+        return run_pt2(args, dev)
     w = build_workload(args.workload)
     configs = torch.from_numpy(w["configs"]).to(dev)
     nq = configs.shape[1]
