@@ -19,6 +19,7 @@ Workloads (--workload):
   batch64:               64 seeded molecular instances, forward + backward in one persistent launch.
   lanczos52:             Lanczos iterations on the molecular52 basis (application + vector step).
   pt2_52:                the perturbation step (E0, find_relative, application into the external set).
+  sampler52:             the config producer: unique sampling of 1M configs from a 26-site network.
 
 Prints one JSON line with the metric, a roofline entry for the forward kernel (algorithmic bytes per
 launch / its HIP-event time) and the reference CPU op timed on this host on a bounded sample.
@@ -442,12 +443,66 @@ def run_pt2(args: argparse.Namespace, dev: object) -> int:
     return 0
 
 
+def run_sampler(args: argparse.Namespace, dev: object) -> int:
+    """The config producer on molecular52's shape (qmb/mlp.py:61-314): 26 sites, 7 + 7 electrons,
+    one hidden layer of 512, float32; generate_unique of 1,000,000 distinct configs with their psi,
+    and the forward alone on them."""
+    import torch  # pylint: disable=import-outside-toplevel
+
+    from qmb_hip import mlp  # pylint: disable=import-outside-toplevel
+
+    torch.manual_seed(5001)
+    net = mlp.WaveFunctionElectronUpDown(double_sites=52, physical_dim=2, is_complex=True, spin_up=7, spin_down=7, hidden_size=(512,),
+                                         ordering=+1).to(dev)
+    batch = 1_000_000
+    for _ in range(max(args.warmup, 1)):
+        configs, psi, _, _ = net.generate_unique(batch, seed=1)
+    torch.cuda.synchronize()
+    steps = max(args.steps, 1)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        configs, psi, _, _ = net.generate_unique(batch, seed=2 + k)
+    torch.cuda.synchronize()
+    gen_ms = (time.perf_counter() - t0) * 1e3 / steps
+    stream = torch.cuda.current_stream(dev)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.no_grad():
+        net(configs)
+        a.record(stream)
+        for _ in range(steps):
+            net(configs)
+        b.record(stream)
+    torch.cuda.synchronize()
+    fwd_ms = a.elapsed_time(b) / steps
+    log(f"sampler52: generate_unique {configs.size(0)} configs in {gen_ms:.1f} ms; forward on them {fwd_ms:.2f} ms")
+    result = {
+        "metric": "unique configs sampled/s",
+        "value": configs.size(0) / (gen_ms * 1e-3),
+        "unit": "configs/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": max(args.warmup, 1),
+        "ms_per_step": gen_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded random-init network)",
+        "config": {"workload": "sampler52", "sites": 26, "electrons": "7a+7b", "hidden": [512], "batch": batch,
+                   "sampled": int(configs.size(0))},
+        "sampler": {"generate_unique_ms": gen_ms, "forward_ms": fwd_ms, "forward_configs_per_s": configs.size(0) / (fwd_ms * 1e-3)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=10)
     parser.add_argument("--warmup", type=int, default=3)
-    parser.add_argument("--workload", default="molecular52", choices=["molecular52", "hubbard128", "molecular20", "batch64", "lanczos52", "pt2_52"])
+    parser.add_argument("--workload", default="molecular52", choices=["molecular52", "hubbard128", "molecular20", "batch64", "lanczos52", "pt2_52", "sampler52"])
     parser.add_argument("--tile", type=int, default=0)
     parser.add_argument("--groups", type=int, default=0)
     parser.add_argument("--no-plain", action="store_true", help="skip the reference-scheme comparator")
@@ -477,6 +532,8 @@ def main() -> int:
         return run_lanczos(args, dev)
     if args.workload == "pt2_52":
         return run_pt2(args, dev)
+    if args.workload == "sampler52":
+        return run_sampler(args, dev)
     w = build_workload(args.workload)
     configs = torch.from_numpy(w["configs"]).to(dev)
     nq = configs.shape[1]

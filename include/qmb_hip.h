@@ -352,6 +352,43 @@ int qmb_lanczos_workspace_size(size_t* bytes);
 int qmb_lanczos_step(int64_t n, const double* v, const double* v_prev, double beta, double* w, double* v_next, double* scalars,
                      void* workspace, size_t bytes, void* stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Autoregressive electron up/down network (qmb/mlp.py:61-314, WaveFunctionElectronUpDown): the
+ * element-wise and bit-level parts of its forward and of its unique sampler; the GEMMs of the
+ * network run in the caller's BLAS.  dtype 0 = float32, 1 = float64 (the network's dtype).
+ * Sites are taken in sampling order: site k is physical site ordering[k] (config bits
+ * 2 ordering[k], 2 ordering[k] + 1 = spin up, spin down).  A sampled prefix is a bit string of two
+ * bits per site in sampling order, in `words` uint64 words per row.  Logits are site-major and
+ * row-minor: [4][n] per sampling step, [S][4][n] for the forward; child / entry c = 2 up + down.
+ * ------------------------------------------------------------------------------------------- */
+/* One sampling step at `site` (qmb/mlp.py:247-270): masks the children a prefix can no longer
+ * take, normalises the logits (sum exp(2 x) = 1), and writes for child c of row r, at c n + r,
+ * its log-probability (unperturbed[r] + 2 x[c]) and its Gumbel-perturbed value conditioned on the
+ * parent's perturbed[r].  The uniforms come from `uniform` ([4][n], values in (0, 1)) when given,
+ * else from Philox4x32-10 under `seed`. */
+int qmb_sampler_children(int dtype, int64_t n, int site, int n_sites, int spin_up, int spin_down, const uint64_t* prefix, int words,
+                         const void* logits, const void* unperturbed, const void* perturbed, const void* uniform, uint64_t seed,
+                         void* child_unperturbed, void* child_perturbed, void* stream);
+/* Keeps the children index[0..n_sel) (index = c n_parent + r): their prefixes (parent's plus the
+ * child's two bits at `site`) and their two log-probabilities (qmb/mlp.py:272-287). */
+int qmb_sampler_select(int dtype, int64_t n_sel, const int64_t* index, int64_t n_parent, int site, const uint64_t* prefix, int words,
+                       const void* child_unperturbed, const void* child_perturbed, uint64_t* out_prefix, void* out_unperturbed,
+                       void* out_perturbed, void* stream);
+/* The network input of a sampling step: the first 2 site prefix bits as 0/1 values, [2 site][n]. */
+int qmb_sampler_prefix_inputs(int dtype, int64_t n, int site, const uint64_t* prefix, int words, void* out, void* stream);
+/* Full prefixes -> packed configs of n_qubytes bytes (qmb/mlp.py:289-295, ordering a permutation). */
+int qmb_sampler_prefix_configs(int64_t n, int n_sites, const uint64_t* prefix, int words, const int64_t* ordering, int n_qubytes,
+                               uint8_t* out, void* stream);
+/* The network input of the forward (qmb/mlp.py:177-181): config bits in sampling order, [2 S][n]. */
+int qmb_network_inputs(int dtype, int64_t n, int n_sites, const uint8_t* configs, int n_qubytes, const int64_t* ordering_reversed,
+                       void* out, void* stream);
+/* The forward's log-amplitude (qmb/mlp.py:183-199): out[r] = sum over sites (in order, double) of
+ * the selected normalised logit.  With grad/grad_logits (both or neither) it writes instead
+ * grad_logits[k][c][r] = (delta(c, selected) - exp(2 x[c])) grad[r], the gradient of that sum. */
+int qmb_network_amplitude(int dtype, int64_t n, int n_sites, int spin_up, int spin_down, const uint8_t* configs, int n_qubytes,
+                          const int64_t* ordering_reversed, const void* logits, double* out, const double* grad, void* grad_logits,
+                          void* stream);
+
 /* Host-only pair-excitation test of a compiled record (for unit tests): returns 1 and writes
  * {r0, r1, c0, c1, delta} when the record is a pair excitation, else 0. */
 int qmb_pair_excitation_host(uint64_t rec, int particle_cut, int32_t out[5]);

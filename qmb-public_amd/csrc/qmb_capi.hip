@@ -440,6 +440,47 @@ int qmb_apply_batch(const qmb_batch* batch, int mode, int waves, void* stream) {
     return apply_batch(batch, mode, waves, as_stream(stream));
 }
 
+int qmb_sampler_children(int dtype, int64_t n, int site, int n_sites, int spin_up, int spin_down, const uint64_t* prefix, int words,
+                         const void* logits, const void* unperturbed, const void* perturbed, const void* uniform, uint64_t seed,
+                         void* child_unperturbed, void* child_perturbed, void* stream) {
+    clear_error();
+    return sampler_children(dtype, n, site, n_sites, spin_up, spin_down, prefix, words, logits, unperturbed, perturbed, uniform, seed,
+                            child_unperturbed, child_perturbed, as_stream(stream));
+}
+
+int qmb_sampler_select(int dtype, int64_t n_sel, const int64_t* index, int64_t n_parent, int site, const uint64_t* prefix, int words,
+                       const void* child_unperturbed, const void* child_perturbed, uint64_t* out_prefix, void* out_unperturbed,
+                       void* out_perturbed, void* stream) {
+    clear_error();
+    return sampler_select(dtype, n_sel, index, n_parent, site, prefix, words, child_unperturbed, child_perturbed, out_prefix,
+                          out_unperturbed, out_perturbed, as_stream(stream));
+}
+
+int qmb_sampler_prefix_inputs(int dtype, int64_t n, int site, const uint64_t* prefix, int words, void* out, void* stream) {
+    clear_error();
+    return sampler_prefix_inputs(dtype, n, site, prefix, words, out, as_stream(stream));
+}
+
+int qmb_sampler_prefix_configs(int64_t n, int n_sites, const uint64_t* prefix, int words, const int64_t* ordering, int n_qubytes,
+                               uint8_t* out, void* stream) {
+    clear_error();
+    return sampler_prefix_configs(n, n_sites, prefix, words, ordering, n_qubytes, out, as_stream(stream));
+}
+
+int qmb_network_inputs(int dtype, int64_t n, int n_sites, const uint8_t* configs, int n_qubytes, const int64_t* ordering_reversed,
+                       void* out, void* stream) {
+    clear_error();
+    return network_inputs(dtype, n, n_sites, configs, n_qubytes, ordering_reversed, out, as_stream(stream));
+}
+
+int qmb_network_amplitude(int dtype, int64_t n, int n_sites, int spin_up, int spin_down, const uint8_t* configs, int n_qubytes,
+                          const int64_t* ordering_reversed, const void* logits, double* out, const double* grad, void* grad_logits,
+                          void* stream) {
+    clear_error();
+    return network_amplitude(dtype, n, n_sites, spin_up, spin_down, configs, n_qubytes, ordering_reversed, logits, out, grad, grad_logits,
+                             as_stream(stream));
+}
+
 int qmb_lanczos_workspace_size(size_t* bytes) {
     clear_error();
     if (!bytes) {

@@ -160,6 +160,21 @@ size_t batch_bytes(int n_items);
 int batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t bytes, qmb_batch* out, hipStream_t stream);
 int apply_batch(const qmb_batch* batch, int mode, int waves, hipStream_t stream);
 
+int sampler_children(int dtype, int64_t n, int site, int n_sites, int spin_up, int spin_down, const uint64_t* prefix, int words,
+                     const void* logits, const void* unperturbed, const void* perturbed, const void* uniform, uint64_t seed,
+                     void* child_unperturbed, void* child_perturbed, hipStream_t stream);
+int sampler_select(int dtype, int64_t n_sel, const int64_t* index, int64_t n_parent, int site, const uint64_t* prefix, int words,
+                   const void* child_unperturbed, const void* child_perturbed, uint64_t* out_prefix, void* out_unperturbed,
+                   void* out_perturbed, hipStream_t stream);
+int sampler_prefix_inputs(int dtype, int64_t n, int site, const uint64_t* prefix, int words, void* out, hipStream_t stream);
+int sampler_prefix_configs(int64_t n, int n_sites, const uint64_t* prefix, int words, const int64_t* ordering, int n_qubytes,
+                           uint8_t* out, hipStream_t stream);
+int network_inputs(int dtype, int64_t n, int n_sites, const uint8_t* configs, int n_qubytes, const int64_t* ordering_reversed, void* out,
+                   hipStream_t stream);
+int network_amplitude(int dtype, int64_t n, int n_sites, int spin_up, int spin_down, const uint8_t* configs, int n_qubytes,
+                      const int64_t* ordering_reversed, const void* logits, double* out, const double* grad, void* grad_logits,
+                      hipStream_t stream);
+
 size_t lanczos_bytes();
 int lanczos_step(int64_t n, const double* v, const double* v_prev, double beta, double* w, double* v_next, double* scalars,
                  void* workspace, size_t bytes, hipStream_t stream);
