@@ -455,26 +455,59 @@ def run_sampler(args: argparse.Namespace, dev: object) -> int:
     net = mlp.WaveFunctionElectronUpDown(double_sites=52, physical_dim=2, is_complex=True, spin_up=7, spin_down=7, hidden_size=(512,),
                                          ordering=+1).to(dev)
     batch = 1_000_000
-    for _ in range(max(args.warmup, 1)):
-        configs, psi, _, _ = net.generate_unique(batch, seed=1)
-    torch.cuda.synchronize()
     steps = max(args.steps, 1)
-    t0 = time.perf_counter()
-    for k in range(steps):
-        configs, psi, _, _ = net.generate_unique(batch, seed=2 + k)
-    torch.cuda.synchronize()
-    gen_ms = (time.perf_counter() - t0) * 1e3 / steps
-    stream = torch.cuda.current_stream(dev)
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     with torch.no_grad():
-        net(configs)
-        a.record(stream)
-        for _ in range(steps):
-            net(configs)
-        b.record(stream)
-    torch.cuda.synchronize()
-    fwd_ms = a.elapsed_time(b) / steps
-    log(f"sampler52: generate_unique {configs.size(0)} configs in {gen_ms:.1f} ms; forward on them {fwd_ms:.2f} ms")
+        for _ in range(max(args.warmup, 1)):
+            configs, psi, _, _ = net.generate_unique(batch, seed=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            configs, psi, _, _ = net.generate_unique(batch, seed=2 + k)
+        torch.cuda.synchronize()
+        gen_ms = (time.perf_counter() - t0) * 1e3 / steps
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(fn: object) -> float:
+        with torch.no_grad():
+            fn()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            for _ in range(steps):
+                fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
+    def per_site_forward(x: torch.Tensor) -> torch.Tensor:
+        """The reference's forward arithmetic (qmb/mlp.py:169-205): one small network per site."""
+        from qmb_hip import bitspack  # pylint: disable=import-outside-toplevel
+
+        n = x.shape[0]
+        xs = bitspack.unpack_int(x, size=1, last_dim=net.double_sites).view([n, net.sites, 2])
+        xs = torch.index_select(xs, 1, net.ordering_reversed)
+        xf = xs.to(torch.float32)
+        number = torch.zeros((n, 2), dtype=torch.int64, device=x.device)
+        total = torch.zeros(n, dtype=torch.float64, device=x.device)
+        rows = torch.arange(n, device=x.device)
+        for i, m in enumerate(net.amplitude):
+            d = m(xf[:, :i].reshape([n, 2 * i])).view([n, 2, 2])
+            ok_up = torch.stack([i - number[:, 0] < net.sites - net.spin_up, number[:, 0] < net.spin_up], dim=-1).unsqueeze(-1)
+            ok_dn = torch.stack([i - number[:, 1] < net.sites - net.spin_down, number[:, 1] < net.spin_down], dim=-1).unsqueeze(-2)
+            d = d + torch.where(torch.logical_and(ok_up, ok_dn), 0, -torch.inf)
+            d = d - ((2 * d).exp().sum(dim=[-2, -1]).log() / 2)[:, None, None]
+            xi = xs[:, i, :].to(torch.int64)
+            total = total + d[rows, xi[:, 0], xi[:, 1]].double()
+            number += xi
+        phase = net.phase(xf.view([n, net.double_sites])).view([n])
+        return torch.view_as_complex(torch.stack([total, phase.double()], dim=-1)).exp()
+
+    fwd_ms = timed(lambda: net(configs))
+    net.fused = False
+    gemm_ms = timed(lambda: net(configs))
+    net.fused = None
+    per_site_ms = timed(lambda: per_site_forward(configs))
+    log(f"sampler52: generate_unique {configs.size(0)} configs in {gen_ms:.1f} ms; forward on them: fused {fwd_ms:.2f} ms, "
+        f"stacked GEMMs {gemm_ms:.2f} ms, per-site networks (the reference's arithmetic) {per_site_ms:.2f} ms")
     result = {
         "metric": "unique configs sampled/s",
         "value": configs.size(0) / (gen_ms * 1e-3),
@@ -490,7 +523,9 @@ def run_sampler(args: argparse.Namespace, dev: object) -> int:
         "data": "synthetic (seeded random-init network)",
         "config": {"workload": "sampler52", "sites": 26, "electrons": "7a+7b", "hidden": [512], "batch": batch,
                    "sampled": int(configs.size(0))},
-        "sampler": {"generate_unique_ms": gen_ms, "forward_ms": fwd_ms, "forward_configs_per_s": configs.size(0) / (fwd_ms * 1e-3)},
+        "sampler": {"generate_unique_ms": gen_ms, "forward_fused_ms": fwd_ms, "forward_stacked_gemm_ms": gemm_ms,
+                    "forward_per_site_ms": per_site_ms, "forward_configs_per_s": configs.size(0) / (fwd_ms * 1e-3),
+                    "forward_speedup_vs_per_site": per_site_ms / fwd_ms},
         "cpu_baseline": None,
     }
     print(json.dumps(result), flush=True)

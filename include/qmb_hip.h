@@ -389,6 +389,24 @@ int qmb_network_amplitude(int dtype, int64_t n, int n_sites, int spin_up, int sp
                           const int64_t* ordering_reversed, const void* logits, double* out, const double* grad, void* grad_logits,
                           void* stream);
 
+/* Fused one-hidden-layer float32 networks (hidden width 64, 128, 256 or 512; 2 n_sites hidden
+ * columns of float32 must fit LDS): returns 1 when the two calls below support (n_sites, hidden).
+ * Weights, float32, row-major: w1 = the site networks' first-layer weights transposed and
+ * concatenated (site i: [2 i][hidden] at row i (i - 1)), b1 [S][hidden], w2 [S][4][hidden],
+ * b2 [S][4]; the phase network wp1 [2 S][hidden] (transposed), bp1 [hidden], wp2 [hidden], bp2 [1]. */
+int qmb_network_fused_supported(int n_sites, int hidden);
+/* The whole forward (qmb/mlp.py:169-205) without materialising activations: amplitude[r] = the
+ * double log-amplitude, phase[r] = the phase network's output. */
+int qmb_network_forward_fused(int64_t n, int n_sites, int hidden, int spin_up, int spin_down, const uint8_t* configs, int n_qubytes,
+                              const int64_t* ordering_reversed, const float* w1, const float* b1, const float* w2, const float* b2,
+                              const float* wp1, const float* bp1, const float* wp2, const float* bp2, double* amplitude, float* phase,
+                              void* stream);
+/* qmb_sampler_children with the site network evaluated in the same kernel (prefix words <= 2). */
+int qmb_sampler_step_fused(int64_t n, int site, int n_sites, int hidden, int spin_up, int spin_down, const uint64_t* prefix, int words,
+                           const float* w1, const float* b1, const float* w2, const float* b2, const float* unperturbed,
+                           const float* perturbed, const float* uniform, uint64_t seed, float* child_unperturbed, float* child_perturbed,
+                           void* stream);
+
 /* Host-only pair-excitation test of a compiled record (for unit tests): returns 1 and writes
  * {r0, r1, c0, c1, delta} when the record is a pair excitation, else 0. */
 int qmb_pair_excitation_host(uint64_t rec, int particle_cut, int32_t out[5]);

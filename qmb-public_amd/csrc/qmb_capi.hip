@@ -481,6 +481,26 @@ int qmb_network_amplitude(int dtype, int64_t n, int n_sites, int spin_up, int sp
                              as_stream(stream));
 }
 
+int qmb_network_fused_supported(int n_sites, int hidden) { return network_fused_supported(n_sites, hidden); }
+
+int qmb_network_forward_fused(int64_t n, int n_sites, int hidden, int spin_up, int spin_down, const uint8_t* configs, int n_qubytes,
+                              const int64_t* ordering_reversed, const float* w1, const float* b1, const float* w2, const float* b2,
+                              const float* wp1, const float* bp1, const float* wp2, const float* bp2, double* amplitude, float* phase,
+                              void* stream) {
+    clear_error();
+    return network_forward_fused(n, n_sites, hidden, spin_up, spin_down, configs, n_qubytes, ordering_reversed, w1, b1, w2, b2, wp1, bp1, wp2,
+                                 bp2, amplitude, phase, as_stream(stream));
+}
+
+int qmb_sampler_step_fused(int64_t n, int site, int n_sites, int hidden, int spin_up, int spin_down, const uint64_t* prefix, int words,
+                           const float* w1, const float* b1, const float* w2, const float* b2, const float* unperturbed,
+                           const float* perturbed, const float* uniform, uint64_t seed, float* child_unperturbed, float* child_perturbed,
+                           void* stream) {
+    clear_error();
+    return sampler_step_fused(n, site, n_sites, hidden, spin_up, spin_down, prefix, words, w1, b1, w2, b2, unperturbed, perturbed, uniform,
+                              seed, child_unperturbed, child_perturbed, as_stream(stream));
+}
+
 int qmb_lanczos_workspace_size(size_t* bytes) {
     clear_error();
     if (!bytes) {

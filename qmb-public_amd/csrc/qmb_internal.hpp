@@ -175,6 +175,15 @@ int network_amplitude(int dtype, int64_t n, int n_sites, int spin_up, int spin_d
                       const int64_t* ordering_reversed, const void* logits, double* out, const double* grad, void* grad_logits,
                       hipStream_t stream);
 
+int network_fused_supported(int n_sites, int hidden);
+int network_forward_fused(int64_t n, int n_sites, int hidden, int spin_up, int spin_down, const uint8_t* configs, int n_qubytes,
+                          const int64_t* ordering_reversed, const float* w1, const float* b1, const float* w2, const float* b2,
+                          const float* wp1, const float* bp1, const float* wp2, const float* bp2, double* amplitude, float* phase,
+                          hipStream_t stream);
+int sampler_step_fused(int64_t n, int site, int n_sites, int hidden, int spin_up, int spin_down, const uint64_t* prefix, int words,
+                       const float* w1, const float* b1, const float* w2, const float* b2, const float* unperturbed, const float* perturbed,
+                       const float* uniform, uint64_t seed, float* child_unperturbed, float* child_perturbed, hipStream_t stream);
+
 size_t lanczos_bytes();
 int lanczos_step(int64_t n, const double* v, const double* v_prev, double beta, double* w, double* v_next, double* scalars,
                  void* workspace, size_t bytes, hipStream_t stream);

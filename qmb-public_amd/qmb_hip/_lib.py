@@ -219,6 +219,9 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "qmb_sampler_prefix_configs": (_c_int, [_c_i64, _c_int, _vp, _c_int, _vp, _c_int, _vp, _vp]),
     "qmb_network_inputs": (_c_int, [_c_int, _c_i64, _c_int, _vp, _c_int, _vp, _vp, _vp]),
     "qmb_network_amplitude": (_c_int, [_c_int, _c_i64, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "qmb_network_fused_supported": (_c_int, [_c_int, _c_int]),
+    "qmb_network_forward_fused": (_c_int, [_c_i64, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "qmb_sampler_step_fused": (_c_int, [_c_i64, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_u64, _vp, _vp, _vp]),
     "qmb_lanczos_workspace_size": (_c_int, [ctypes.POINTER(_c_size)]),
     "qmb_lanczos_step": (_c_int, [_c_i64, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp, _c_size, _vp]),
     "qmb_pair_excitation_host": (_c_int, [_c_u64, _c_int, ctypes.POINTER(_c_i32)]),

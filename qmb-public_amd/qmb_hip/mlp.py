@@ -13,6 +13,11 @@ What runs differs:
     [S][4][B] logits to the log-amplitude (qmb_network_amplitude: particle-number masks,
     normalisation, selection, double-precision sum over sites) read and write coalesced rows.  The
     log-amplitude has a hand-written backward (the same kernel in gradient mode).
+  - one hidden layer in float32 (width 64 to 512), without gradients: qmb_network_forward_fused
+    runs the whole forward in one kernel.  With 0/1 inputs the first layer is a sum of weight
+    columns over the set bits, so it is a few vector adds per row; the hidden activations stay in
+    registers, the site weights in LDS, and nothing but the configs is read from HBM.  The sampler
+    uses the same fusion per step (qmb_sampler_step_fused: site network plus children).
   - generate_unique (Gumbel-top-k without replacement, arXiv:2408.07625): prefixes live as two bits
     per site in uint64 words; per site one GEMM chain evaluates that site's network on the kept
     prefixes, one kernel (qmb_sampler_children) masks, normalises and perturbs the four children of
@@ -156,6 +161,49 @@ class WaveFunctionElectronUpDown(torch.nn.Module):
         self.register_buffer("ordering_reversed", torch.scatter(torch.zeros(self.sites, dtype=torch.int64), 0, order,
                                                                 torch.arange(self.sites, dtype=torch.int64)))
         self.dummy_param = torch.nn.Parameter(torch.empty(0))
+        # None: use the fused kernels whenever they apply; False: never (the GEMM path)
+        self.fused: bool | None = None
+        self._fused_cache: tuple[tuple[object, ...], tuple[torch.Tensor, ...]] | None = None
+
+    # ------------------------------------------------------------------------------ fused path
+    def _fused_ok(self) -> bool:
+        """The fused float32 kernels cover one hidden layer of width 64, 128, 256 or 512 whose
+        first-layer columns fit LDS."""
+        if self.fused is False or self.dummy_param.dtype != torch.float32 or len(self.hidden_size) != 1:
+            return False
+        return bool(_lib.library().qmb_network_fused_supported(self.sites, self.hidden_size[0]))
+
+    def _fused_weights(self) -> tuple[torch.Tensor, ...]:
+        """(w1, b1, w2, b2, wp1, bp1, wp2, bp2) in the fused kernels' layout, rebuilt when a
+        parameter changes."""
+        params = list(self.parameters())
+        key = tuple((p.data_ptr(), p._version) for p in params)  # pylint: disable=protected-access
+        if self._fused_cache is not None and self._fused_cache[0] == key:
+            return self._fused_cache[1]
+        with torch.no_grad():
+            per = [m.layers() for m in self.amplitude]
+            dev = self.dummy_param.device
+            h = self.hidden_size[0]
+            w1 = torch.cat([per[i][0][0].t() for i in range(1, self.sites)]) if self.sites > 1 else torch.zeros((1, h), device=dev)
+            b1 = torch.stack([layers[0][1] for layers in per])
+            w2 = torch.stack([layers[1][0] for layers in per])
+            b2 = torch.stack([layers[1][1] for layers in per])
+            (pw1, pb1), (pw2, pb2) = self.phase.layers()
+            assert pw1 is not None and pw2 is not None
+            packed = tuple(t.detach().float().contiguous() for t in (w1, b1, w2, b2, pw1.t(), pb1, pw2.reshape(-1), pb2))
+        self._fused_cache = (key, packed)
+        return packed
+
+    def _forward_fused(self, x: torch.Tensor) -> torch.Tensor:
+        n = x.size(0)
+        amplitude = torch.empty(n, dtype=torch.float64, device=x.device)
+        phase = torch.empty(n, dtype=torch.float32, device=x.device)
+        w = self._fused_weights()
+        with torch.cuda.device(x.device):
+            _lib.check(_lib.library().qmb_network_forward_fused(
+                n, self.sites, self.hidden_size[0], self.spin_up, self.spin_down, _ptr(x), x.size(1), _ptr(self.ordering_reversed),
+                *[_ptr(t) for t in w], _ptr(amplitude), _ptr(phase), _stream(x.device)), "qmb_network_forward_fused")
+        return torch.view_as_complex(torch.stack([amplitude, phase.double()], dim=-1)).exp()
 
     # ------------------------------------------------------------------------------ network
     def _stacked(self) -> tuple[torch.Tensor, torch.Tensor, list[tuple[torch.Tensor, torch.Tensor]]]:
@@ -205,6 +253,9 @@ class WaveFunctionElectronUpDown(torch.nn.Module):
         if x.dtype != torch.uint8 or x.dim() != 2 or x.size(1) * 8 < self.double_sites or not x.is_cuda:
             raise RuntimeError("x must be a uint8 [batch, n_qubytes] tensor on a HIP device")
         x = x.contiguous()
+        needs_grad = torch.is_grad_enabled() and any(p.requires_grad for p in self.parameters())
+        if not needs_grad and self._fused_ok():
+            return self._forward_fused(x)
         n = x.size(0)
         xt = self._inputs(x)
         amplitude = _LogAmplitude.apply(self._amplitude_logits(xt), x, self)
@@ -241,12 +292,11 @@ class WaveFunctionElectronUpDown(torch.nn.Module):
         prefix = torch.zeros((1, words), dtype=torch.int64, device=device)
         unperturbed = torch.zeros(1, dtype=dtype, device=device)
         perturbed = torch.zeros(1, dtype=dtype, device=device)
+        fused = self._fused_ok()
         with torch.cuda.device(device), torch.no_grad():
+            fw = self._fused_weights() if fused else None
             for site in range(self.sites):
                 n = prefix.size(0)
-                xt = torch.empty((2 * site, n), dtype=dtype, device=device)
-                _lib.check(lib.qmb_sampler_prefix_inputs(code, n, site, _ptr(prefix), words, _ptr(xt), stream), "qmb_sampler_prefix_inputs")
-                logits = self._site_logits(site, xt, n, block_num)
                 u = None
                 if uniform is not None:
                     u = uniform(site, n).to(device=device, dtype=dtype).contiguous()
@@ -254,9 +304,20 @@ class WaveFunctionElectronUpDown(torch.nn.Module):
                         raise RuntimeError("uniform(site, n) must return a [4, n] tensor")
                 child_unperturbed = torch.empty(4 * n, dtype=dtype, device=device)
                 child_perturbed = torch.empty(4 * n, dtype=dtype, device=device)
-                _lib.check(lib.qmb_sampler_children(code, n, site, self.sites, self.spin_up, self.spin_down, _ptr(prefix), words, _ptr(logits),
-                                                    _ptr(unperturbed), _ptr(perturbed), _ptr(u), ctypes.c_uint64(seed),
-                                                    _ptr(child_unperturbed), _ptr(child_perturbed), stream), "qmb_sampler_children")
+                if fw is not None:
+                    # the site network and the children in one kernel
+                    _lib.check(lib.qmb_sampler_step_fused(n, site, self.sites, self.hidden_size[0], self.spin_up, self.spin_down, _ptr(prefix),
+                                                          words, *[_ptr(t) for t in fw[:4]], _ptr(unperturbed), _ptr(perturbed), _ptr(u),
+                                                          ctypes.c_uint64(seed), _ptr(child_unperturbed), _ptr(child_perturbed), stream),
+                               "qmb_sampler_step_fused")
+                else:
+                    xt = torch.empty((2 * site, n), dtype=dtype, device=device)
+                    _lib.check(lib.qmb_sampler_prefix_inputs(code, n, site, _ptr(prefix), words, _ptr(xt), stream),
+                               "qmb_sampler_prefix_inputs")
+                    logits = self._site_logits(site, xt, n, block_num)
+                    _lib.check(lib.qmb_sampler_children(code, n, site, self.sites, self.spin_up, self.spin_down, _ptr(prefix), words,
+                                                        _ptr(logits), _ptr(unperturbed), _ptr(perturbed), _ptr(u), ctypes.c_uint64(seed),
+                                                        _ptr(child_unperturbed), _ptr(child_perturbed), stream), "qmb_sampler_children")
                 values, index = torch.sort(child_perturbed, descending=True)
                 values, index = values[:batch_size], index[:batch_size]
                 index = index[torch.isfinite(values)].contiguous()
