@@ -719,7 +719,6 @@ int qmb_single_relative(int n_qubytes, int particle_cut, const uint8_t* configs,
 int qmb_find_relative_workspace_size(int n_qubytes, int64_t n_i, int64_t n_exclude, int64_t n_terms, int64_t count_selected,
                                      size_t* bytes) {
     clear_error();
-    (void)n_i;
     QMB_TRY(check_qubytes(n_qubytes));
     if (!bytes) {
         return set_error(QMB_ERR_ARGUMENT, "bytes must be non-null");
@@ -728,7 +727,7 @@ int qmb_find_relative_workspace_size(int n_qubytes, int64_t n_i, int64_t n_exclu
         return set_error(QMB_ERR_ARGUMENT, "count_selected must be positive");
     }
     *bytes = terms_bytes(n_qubytes, n_terms) + targets_bytes(n_qubytes, n_exclude) +
-             find_relative_scratch_bytes(n_qubytes, count_selected) + 768;
+             find_relative_scratch_bytes(n_qubytes, n_i, count_selected, n_terms) + 768;
     return QMB_OK;
 }
 
@@ -754,7 +753,7 @@ int qmb_find_relative(int n_qubytes, int particle_cut, const uint8_t* configs_i,
     hipStream_t s = as_stream(stream);
     const size_t tb = terms_bytes(n_qubytes, n_terms);
     const size_t gb = targets_bytes(n_qubytes, n_exclude);
-    const size_t fb = find_relative_scratch_bytes(n_qubytes, count_selected);
+    const size_t fb = find_relative_scratch_bytes(n_qubytes, n_i, count_selected, n_terms);
     Regions r{static_cast<char*>(workspace), 0, workspace_bytes};
     qmb_terms terms{};
     qmb_targets exclude{};

@@ -13,6 +13,15 @@
 // the threshold prefix and the tied candidates; the candidates are ordered by (weight desc, config
 // asc) and the first K - above are kept.  All control lives in device memory, so the whole call is
 // enqueued without host synchronisation.
+//
+// Large selections first try a sampled threshold, which needs one full pass instead of three: the
+// contributions of an evenly spaced sample of n_s rows (with their exclusion lookups) are sorted,
+// and the sample weights at ranks f K -+ 6 sqrt(f K) (f = n_s / n_i) bracket the K-th weight of the
+// whole set with overwhelming probability.  One pass then collects every entry above the upper
+// bound and every entry between the bounds as candidates.  The result is exact when fewer than K
+// entries lie above and at least K lie above the lower bound, which the device checks; otherwise
+// the state is reset and the digit-by-digit passes run as before.  Either way the candidates are
+// finished by the same sort, so ties resolve identically.
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
@@ -44,6 +53,13 @@ struct FrState {
     unsigned long long n_above;
     unsigned long long n_cand;
     int64_t cand_needed;
+    uint64_t lo;   // collect: candidates are lo <= w < hi, entries with w >= hi are selected outright
+    uint64_t hi;
+    unsigned long long n_sample;
+    int32_t fast_try;   // the sampled bounds are set
+    int32_t collected;  // a collect pass produced the final selection
+    int32_t failed;     // the sampled attempt did not bracket the K-th weight
+    int32_t reserved;
 };
 
 struct FrParams {
@@ -61,6 +77,8 @@ struct FrParams {
     uint64_t* sel_c;   // [K][W] selected configs ("above" entries, then the kept candidates)
     uint64_t* cand_w;  // [cap]
     uint64_t* cand_c;  // [cap][W]
+    uint64_t* sample_w;  // [sample_cap] weights of the sampled rows' contributions
+    int64_t sample_cap;
     int64_t k;
     int64_t cand_cap;
     int tile_log2;
@@ -100,26 +118,60 @@ struct HistAction : ActionHooks {
     }
 };
 
+// Appends to a device counter from the active lanes with one atomic per wave: returns this lane's slot.
+__device__ __forceinline__ unsigned long long wave_append(unsigned long long* counter) {
+    const uint64_t active = __ballot(1);
+    const int leader = __ffsll(static_cast<long long>(active)) - 1;
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    unsigned long long base = 0;
+    if (lane == leader) {
+        base = atomicAdd(counter, static_cast<unsigned long long>(__popcll(active)));
+    }
+    base = __shfl(base, leader);
+    return base + static_cast<unsigned long long>(__popcll(active & ((1ull << lane) - 1ull)));
+}
+
+__device__ __forceinline__ bool sample_full(const FrParams& p) {
+    return *reinterpret_cast<const volatile unsigned long long*>(&p.state->n_sample) > static_cast<unsigned long long>(p.sample_cap);
+}
+
+// The sampled rows' contributions: every nonzero weight whose target is not excluded.  Threads
+// that start after the buffer has overflowed (the attempt is then abandoned) skip their lookups.
+template <int W>
+struct SampleAction : ActionHooks {
+    const FrParams& p;
+    bool full;
+    __device__ explicit SampleAction(const FrParams& pp) : p(pp), full(sample_full(pp)) {}
+    __device__ __forceinline__ bool pre(int64_t i, uint32_t t, const Cfg<W>&, int, uint64_t, uint64_t) {
+        return !full && weight_bits(p.coefs[t], p.psi[i]) != 0;
+    }
+    __device__ __forceinline__ void post(int64_t i, uint32_t t, const Cfg<W>&, int, int64_t j, bool) {
+        if (j >= 0) {
+            return;  // excluded
+        }
+        const unsigned long long s = wave_append(&p.state->n_sample);
+        if (s < static_cast<unsigned long long>(p.sample_cap)) {
+            p.sample_w[s] = weight_bits(p.coefs[t], p.psi[i]);
+        }
+    }
+};
+
 template <int W>
 struct CollectAction : ActionHooks {
     const FrParams& p;
-    uint64_t prefix;
-    int shift;
-    bool all;
-    __device__ CollectAction(const FrParams& pp, uint64_t pre_, int sh, bool a) : p(pp), prefix(pre_), shift(sh), all(a) {}
-    __device__ __forceinline__ uint64_t top_of(uint64_t w) const { return all ? 1 : (w >> shift); }
+    uint64_t lo;
+    uint64_t hi;
+    __device__ CollectAction(const FrParams& pp, uint64_t lo_, uint64_t hi_) : p(pp), lo(lo_), hi(hi_) {}
     __device__ __forceinline__ bool pre(int64_t i, uint32_t t, const Cfg<W>&, int, uint64_t, uint64_t) {
         const uint64_t w = weight_bits(p.coefs[t], p.psi[i]);
-        return w != 0 && top_of(w) >= (all ? 0 : prefix);
+        return w != 0 && w >= lo;
     }
     __device__ __forceinline__ void post(int64_t i, uint32_t t, const Cfg<W>& o, int, int64_t j, bool) {
         if (j >= 0) {
             return;  // excluded
         }
         const uint64_t w = weight_bits(p.coefs[t], p.psi[i]);
-        const uint64_t top = top_of(w);
-        const uint64_t pre_ = all ? 0 : prefix;
-        if (top > pre_) {
+        if (w >= hi) {
             const unsigned long long s = atomicAdd(&p.state->n_above, 1ull);
             if (s < static_cast<unsigned long long>(p.k)) {
 #pragma unroll
@@ -175,11 +227,90 @@ __global__ __launch_bounds__(kEnumBlock) void fr_hist_kernel(FrParams p) {
     }
 }
 
+// phase 0: the sampled attempt (runs when its bounds are set); phase 1: the digit-by-digit result
+// (runs unless the sampled attempt already produced the selection).
 template <int W>
-__global__ __launch_bounds__(kEnumBlock) void fr_collect_kernel(FrParams p) {
+__global__ __launch_bounds__(kEnumBlock) void fr_collect_kernel(FrParams p, int phase) {
     extern __shared__ uint64_t smem[];
-    CollectAction<W> act(p, p.state->prefix, p.state->shift, p.state->all != 0);
+    if (p.state->collected || (phase == 0 && !p.state->fast_try)) {
+        return;  // uniform: every thread reads the same flags
+    }
+    CollectAction<W> act(p, p.state->lo, p.state->hi);
     fr_enumerate<W>(p, smem, act);
+}
+
+template <int W>
+__global__ __launch_bounds__(kEnumBlock) void fr_sample_kernel(FrParams p) {
+    extern __shared__ uint64_t smem[];
+    SampleAction<W> act(p);
+    fr_enumerate<W>(p, smem, act);
+}
+
+// Bounds from the sorted (descending) sample weights; single thread.
+__global__ void fr_bounds_kernel(FrState* st, const uint64_t* __restrict__ sorted, int64_t sample_cap, double fraction, int64_t k) {
+    if (threadIdx.x != 0) {
+        return;
+    }
+    st->fast_try = 0;
+    if (st->n_sample > static_cast<unsigned long long>(sample_cap)) {
+        return;  // the sample overflowed its buffer: take the exact passes
+    }
+    const int64_t count = static_cast<int64_t>(st->n_sample);
+    const double e = fraction * static_cast<double>(k);
+    const double sd = sqrt(e);
+    const int64_t k_hi = static_cast<int64_t>(floor(e - 6.0 * sd)) - 1;
+    const int64_t k_lo = static_cast<int64_t>(ceil(e + 6.0 * sd)) + 1;
+    st->hi = k_hi < 0 ? ~0ull : (k_hi < count ? sorted[k_hi] + 1ull : 1ull);
+    st->lo = k_lo < count ? sorted[k_lo] : 1ull;
+    if (st->lo < 1ull) {
+        st->lo = 1ull;
+    }
+    if (st->hi < st->lo) {
+        st->hi = st->lo;
+    }
+    st->fast_try = 1;
+}
+
+// Accepts the sampled attempt when the bounds bracket the K-th weight, else resets for the exact
+// passes; single thread.
+__global__ void fr_verify_kernel(FrState* st, int64_t k, int64_t cand_cap) {
+    if (threadIdx.x != 0 || !st->fast_try) {
+        return;
+    }
+    const unsigned long long kk = static_cast<unsigned long long>(k);
+    // lo = 1 collects every nonzero entry, so fewer than K of them is a complete selection too
+    const bool reached = st->n_above + st->n_cand >= kk || st->lo <= 1ull;
+    if (st->n_above < kk && reached && st->n_cand <= static_cast<unsigned long long>(cand_cap)) {
+        st->collected = 1;
+        st->done = 1;
+        st->cand_needed = static_cast<int64_t>(kk - st->n_above);
+    } else {
+        st->failed = 1;
+        st->n_above = 0;
+        st->n_cand = 0;
+    }
+}
+
+// After a failed sampled attempt the candidate buffer must start empty again.
+__global__ void fr_clear_failed_kernel(const FrState* __restrict__ st, uint64_t* __restrict__ cand_w, int64_t cap) {
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i < cap && st->failed) {
+        cand_w[i] = 0ull;
+    }
+}
+
+// Every sample-th row of configs_i and psi_i, compacted.
+__global__ void fr_gather_sample_kernel(const uint8_t* __restrict__ configs, const double2* __restrict__ psi, int64_t n_i, int nq,
+                                        int64_t n_s, uint8_t* __restrict__ sample_configs, double2* __restrict__ sample_psi) {
+    const int64_t j = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (j >= n_s) {
+        return;
+    }
+    const int64_t r = j * n_i / n_s;
+    for (int b = 0; b < nq; ++b) {
+        sample_configs[j * nq + b] = configs[r * nq + b];
+    }
+    sample_psi[j] = psi[r];
 }
 
 __global__ void fr_init_kernel(FrState* st, unsigned long long* hist, int64_t k) {
@@ -196,6 +327,13 @@ __global__ void fr_init_kernel(FrState* st, unsigned long long* hist, int64_t k)
         st->n_above = 0;
         st->n_cand = 0;
         st->cand_needed = 0;
+        st->lo = 1;
+        st->hi = 1;
+        st->n_sample = 0;
+        st->fast_try = 0;
+        st->collected = 0;
+        st->failed = 0;
+        st->reserved = 0;
     }
 }
 
@@ -216,6 +354,8 @@ __global__ void fr_select_kernel(FrState* st, unsigned long long* hist, int64_t 
         st->all = 1;
         st->done = 1;
         st->cand_needed = 0;
+        st->lo = 1;  // every nonzero entry is selected outright
+        st->hi = 1;
     } else {
         unsigned long long cum = 0;
         int d = bins - 1;
@@ -231,6 +371,8 @@ __global__ void fr_select_kernel(FrState* st, unsigned long long* hist, int64_t 
         if (hist[d] <= static_cast<unsigned long long>(cand_cap) || level == kLevels - 1) {
             st->done = 1;
             st->cand_needed = st->k_rem;
+            st->lo = st->prefix << st->shift;         // the tied class
+            st->hi = (st->prefix + 1ull) << st->shift;  // everything above it
         } else {
             st->level = level + 1;
         }
@@ -346,7 +488,36 @@ int64_t cand_capacity(int64_t k) {
     return cap < (1 << 20) ? (1 << 20) : cap;
 }
 
+// The sampled attempt: n_s rows (0 = no attempt) with f K >= 64 sampled ranks, and a weight
+// buffer of at most 2^25 entries (n_s n_terms bounds the contributions of the sample).  At most
+// 4096 rows are sampled, so an attempt whose sample overflows the buffer wastes little.
+struct FrSamplePlan {
+    int64_t n_s;
+    int64_t cap;
+};
+
+FrSamplePlan fr_sample_plan(int64_t n_i, int64_t k, int64_t n_terms) {
+    FrSamplePlan sp{0, 0};
+    if (n_i < 4096 || k <= 0 || n_terms <= 0) {
+        return sp;
+    }
+    int64_t n_s = (64 * n_i + k - 1) / k;
+    n_s = n_s < 512 ? 512 : n_s;
+    if (n_s > n_i / 4 || n_s > 4096) {
+        return sp;  // a sample this large saves little over the exact passes
+    }
+    const int64_t limit = int64_t{1} << 25;
+    sp.n_s = n_s;
+    sp.cap = n_s > limit / n_terms ? limit : n_s * n_terms;
+    return sp;
+}
+
 struct FrScratch {
+    FrSamplePlan plan;
+    uint8_t* sample_configs;
+    double2* sample_psi;
+    uint64_t* sample_w;
+    uint64_t* sample_alt;
     FrState* state;
     unsigned long long* hist;
     uint64_t* sel_c;
@@ -362,10 +533,15 @@ struct FrScratch {
     size_t tmp_bytes;
 };
 
-FrScratch fr_layout(Carver& cv, int W, int64_t k) {
+FrScratch fr_layout(Carver& cv, int W, int n_qubytes, int64_t n_i, int64_t k, int64_t n_terms) {
     const int64_t cap = cand_capacity(k);
     const size_t M = static_cast<size_t>(cap > k ? cap : k);
     FrScratch s{};
+    s.plan = fr_sample_plan(n_i, k, n_terms);
+    s.sample_configs = cv.take<uint8_t>(static_cast<size_t>(s.plan.n_s) * n_qubytes);
+    s.sample_psi = cv.take<double2>(static_cast<size_t>(s.plan.n_s));
+    s.sample_w = cv.take<uint64_t>(static_cast<size_t>(s.plan.cap));
+    s.sample_alt = cv.take<uint64_t>(static_cast<size_t>(s.plan.cap));
     s.state = cv.take<FrState>(1);
     s.hist = cv.take<unsigned long long>(kHistBins);
     s.sel_c = cv.take<uint64_t>(static_cast<size_t>(k) * W);
@@ -377,12 +553,16 @@ FrScratch fr_layout(Carver& cv, int W, int64_t k) {
     s.key_b = cv.take<uint64_t>(M);
     s.flag = cv.take<uint32_t>(static_cast<size_t>(k));
     s.pos = cv.take<uint32_t>(static_cast<size_t>(k));
-    size_t sort_bytes = 0, sort_desc_bytes = 0, scan_bytes = 0;
+    size_t sort_bytes = 0, sort_desc_bytes = 0, scan_bytes = 0, sample_bytes = 0;
     (void)rocprim::radix_sort_pairs(nullptr, sort_bytes, s.key_a, s.key_b, s.perm_a, s.perm_b, static_cast<unsigned>(M));
     (void)rocprim::radix_sort_pairs_desc(nullptr, sort_desc_bytes, s.key_a, s.key_b, s.perm_a, s.perm_b, static_cast<unsigned>(M));
     (void)rocprim::exclusive_scan(nullptr, scan_bytes, s.flag, s.pos, 0u, static_cast<size_t>(k), rocprim::plus<uint32_t>());
+    if (s.plan.n_s > 0) {
+        (void)rocprim::radix_sort_keys_desc(nullptr, sample_bytes, s.sample_w, s.sample_alt, static_cast<unsigned>(s.plan.cap));
+    }
     size_t t = sort_bytes > sort_desc_bytes ? sort_bytes : sort_desc_bytes;
     t = t > scan_bytes ? t : scan_bytes;
+    t = t > sample_bytes ? t : sample_bytes;
     s.tmp = cv.take_bytes(t);
     s.tmp_bytes = t;
     return s;
@@ -407,7 +587,7 @@ int find_relative_impl(const qmb_terms* terms, const qmb_targets* exclude, const
                        int64_t n_i, int64_t k, uint8_t* out, int64_t* out_rows, void* scratch, size_t scratch_bytes,
                        hipStream_t stream) {
     Carver cv{static_cast<char*>(scratch), 0};
-    FrScratch s = fr_layout(cv, W, k);
+    FrScratch s = fr_layout(cv, W, terms->n_qubytes, n_i, k, terms->n_terms);
     if (cv.off > scratch_bytes) {
         return set_error(QMB_ERR_WORKSPACE, "find_relative scratch too small: need %zu, have %zu", cv.off, scratch_bytes);
     }
@@ -428,6 +608,8 @@ int find_relative_impl(const qmb_terms* terms, const qmb_targets* exclude, const
     p.sel_c = s.sel_c;
     p.cand_w = s.cand_w;
     p.cand_c = s.cand_c;
+    p.sample_w = s.sample_w;
+    p.sample_cap = s.plan.cap;
     p.k = k;
     p.cand_cap = cap;
     p.tile_log2 = ls.tile_log2;
@@ -445,13 +627,47 @@ int find_relative_impl(const qmb_terms* terms, const qmb_targets* exclude, const
                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
         QMB_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&fr_collect_kernel<W>),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        if (s.plan.n_s > 0) {
+            // the sampled attempt: sample contributions, bounds, one collect pass, check
+            const int64_t n_s = s.plan.n_s;
+            fr_gather_sample_kernel<<<blocks_for(n_s), 256, 0, stream>>>(configs_i, p.psi, n_i, terms->n_qubytes, n_s, s.sample_configs,
+                                                                         s.sample_psi);
+            QMB_CHECK_LAUNCH();
+            QMB_TRY(launch_fill_u64(s.sample_w, 0ull, s.plan.cap, stream));
+            FrParams ps = p;
+            ps.src = make_source(terms, s.sample_configs, n_s);
+            ps.psi = s.sample_psi;
+            const LaunchShape lss = resolve_launch(nullptr, W, n_s, terms->n_terms);
+            const int64_t tiles_s = (n_s + (1ll << lss.tile_log2) - 1) >> lss.tile_log2;
+            const int64_t grid_s = lss.direct ? (n_s + kEnumBlock - 1) / kEnumBlock : tiles_s * lss.groups;
+            const size_t lds_s = lss.direct ? direct_lds_bytes(W) : enum_lds_bytes_of(W, terms->n_bits, lss.tile_log2);
+            ps.tile_log2 = lss.tile_log2;
+            ps.groups = lss.groups;
+            ps.direct = lss.direct ? 1 : 0;
+            QMB_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&fr_sample_kernel<W>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_s > lds ? lds_s : lds)));
+            fr_sample_kernel<W><<<static_cast<unsigned>(grid_s), kEnumBlock, lds_s, stream>>>(ps);
+            QMB_CHECK_LAUNCH();
+            size_t need_s = s.tmp_bytes;
+            QMB_HIP_TRY(rocprim::radix_sort_keys_desc(s.tmp, need_s, s.sample_w, s.sample_alt, static_cast<unsigned>(s.plan.cap), 0, 64,
+                                                      stream));
+            fr_bounds_kernel<<<1, 64, 0, stream>>>(s.state, s.sample_alt, s.plan.cap,
+                                                   static_cast<double>(n_s) / static_cast<double>(n_i), k);
+            QMB_CHECK_LAUNCH();
+            fr_collect_kernel<W><<<static_cast<unsigned>(grid), kEnumBlock, lds, stream>>>(p, 0);
+            QMB_CHECK_LAUNCH();
+            fr_verify_kernel<<<1, 64, 0, stream>>>(s.state, k, cap);
+            QMB_CHECK_LAUNCH();
+            fr_clear_failed_kernel<<<blocks_for(cap), 256, 0, stream>>>(s.state, s.cand_w, cap);
+            QMB_CHECK_LAUNCH();
+        }
         for (int level = 0; level < kLevels; ++level) {
             fr_hist_kernel<W><<<static_cast<unsigned>(grid), kEnumBlock, lds, stream>>>(p);
             QMB_CHECK_LAUNCH();
             fr_select_kernel<<<1, 64, 0, stream>>>(s.state, s.hist, cap);
             QMB_CHECK_LAUNCH();
         }
-        fr_collect_kernel<W><<<static_cast<unsigned>(grid), kEnumBlock, lds, stream>>>(p);
+        fr_collect_kernel<W><<<static_cast<unsigned>(grid), kEnumBlock, lds, stream>>>(p, 1);
         QMB_CHECK_LAUNCH();
     }
     // Candidates: order by (weight desc, config asc), keep the first cand_needed.
@@ -484,9 +700,9 @@ int find_relative_impl(const qmb_terms* terms, const qmb_targets* exclude, const
 
 }  // namespace
 
-size_t find_relative_scratch_bytes(int n_qubytes, int64_t count_selected) {
+size_t find_relative_scratch_bytes(int n_qubytes, int64_t n_i, int64_t count_selected, int64_t n_terms) {
     Carver cv;
-    (void)fr_layout(cv, n_words_of(n_qubytes), count_selected);
+    (void)fr_layout(cv, n_words_of(n_qubytes), n_qubytes, n_i, count_selected, n_terms);
     return cv.off + 256;
 }
 

@@ -188,7 +188,7 @@ size_t lanczos_bytes();
 int lanczos_step(int64_t n, const double* v, const double* v_prev, double beta, double* w, double* v_next, double* scalars,
                  void* workspace, size_t bytes, hipStream_t stream);
 
-size_t find_relative_scratch_bytes(int n_qubytes, int64_t count_selected);
+size_t find_relative_scratch_bytes(int n_qubytes, int64_t n_i, int64_t count_selected, int64_t n_terms);
 int find_relative_prepared(const qmb_terms* terms, const qmb_targets* exclude, const uint8_t* configs_i, const double* psi_i,
                            int64_t n_i, int64_t count_selected, uint8_t* out, int64_t* out_rows, void* scratch,
                            size_t scratch_bytes, hipStream_t stream);

@@ -68,6 +68,13 @@ FR_CASES = [
     ("ising-bose2", 2, lambda: tuple(x.numpy() for x in sy.terms_from_dict(sy.ising_dict(4, 2, x=0.8, zh=1.0, zv=0.4))),
      lambda: sy.sort_unique_configs(np.random.default_rng(6).integers(0, 256, size=(60, 1), dtype=np.uint8)), 40),
     ("k-exceeds-candidates", 1, lambda: sorted_terms(*sy.molecular_terms(4, 11)), lambda: sy.sector_configs(4, 2, 2, 5, 3), 100000),
+    # at least 4096 rows and K >= 256: the sampled threshold runs first (and is checked on the device)
+    ("molecular-20so-sampled", 1, lambda: sorted_terms(*sy.molecular_terms(10, 1001)), lambda: sy.sector_configs(10, 5, 5, 8192, 31),
+     4000),
+    ("hubbard-4x2-sampled-ties", 1, lambda: tuple(x.numpy() for x in sy.terms_from_dict(sy.hubbard_dict(4, 2, 1.0, 4.0))),
+     lambda: sy.sector_configs(8, 4, 4, 4500, 32), 300),
+    ("hubbard-4x2-sampled-all", 1, lambda: tuple(x.numpy() for x in sy.terms_from_dict(sy.hubbard_dict(4, 2, 1.0, 4.0))),
+     lambda: sy.sector_configs(8, 4, 4, 4500, 33), 1_000_000),
 ]
 
 
