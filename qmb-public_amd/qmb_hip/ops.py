@@ -16,12 +16,20 @@ with respect to psi_i through apply_within_adjoint.
 
 Each impl validates its arguments with the reference's messages (qmb/_hamiltonian_cuda.cu:186-232),
 takes scratch memory from PyTorch's caching allocator and enqueues the C ABI call on the current
-stream.  Like the reference, find_relative synchronises to learn its output size; apply_within,
-diagonal_term and single_relative do not.
+stream.  Like the reference, find_relative synchronises to learn its output size; diagonal_term and
+single_relative do not.
+
+apply_within (and its adjoint) on one config set -- configs_j the very tensor configs_i, rows
+distinct -- takes the structured path: the term table is compiled once per (site, kind, coef)
+tensors and kept (a small cache keyed by the tensors and their versions), the set's core indexes
+are built per call, and the pair, single and diagonal kernels apply it.  That path synchronises
+once per call to size the core index.  Other calls run the one-shot enumeration kernel
+(qmb_apply_within), as stateless as the reference's op.
 """
 
 from __future__ import annotations
 
+import collections
 import ctypes
 import functools
 import threading
@@ -33,6 +41,9 @@ from . import _lib
 _REGISTRY: dict[str, torch.library.Library] = {}
 _REGISTRY_LOCK = threading.Lock()
 _VALIDATED: dict[tuple[int, int, int, int, int, int], bool] = {}
+_PREPARED: "collections.OrderedDict[tuple[object, ...], object]" = collections.OrderedDict()
+_PREPARED_MAX = 4
+_PREPARED_LOCK = threading.Lock()
 
 
 def _ptr(t: torch.Tensor | None) -> ctypes.c_void_p:
@@ -92,6 +103,48 @@ def _check_terms(site: torch.Tensor, kind: torch.Tensor, coef: torch.Tensor, dev
 
 
 # ----------------------------------------------------------------------------------- impls -----
+def _prepared_terms(nq: int, pc: int, site: torch.Tensor, kind: torch.Tensor, coef: torch.Tensor) -> object:
+    """The compiled table of these term tensors, kept while they are (the cache holds them, so their
+    storage cannot be reused by other tensors meanwhile); an in-place change bumps a version and
+    rebuilds."""
+    from . import prepared as pp  # pylint: disable=import-outside-toplevel
+
+    key = (str(site.device), nq, pc, site.data_ptr(), kind.data_ptr(), coef.data_ptr(), site.size(0), site._version, kind._version,
+           coef._version)  # pylint: disable=protected-access
+    with _PREPARED_LOCK:
+        terms = _PREPARED.get(key)
+        if terms is not None:
+            _PREPARED.move_to_end(key)
+            return terms
+    terms = pp.PreparedTerms(site, kind, coef, nq, pc)
+    with _PREPARED_LOCK:
+        _PREPARED[key] = terms
+        while len(_PREPARED) > _PREPARED_MAX:
+            _PREPARED.popitem(last=False)
+    return terms
+
+
+def _within_structured(nq: int, pc: int, configs: torch.Tensor, vec: torch.Tensor, site: torch.Tensor, kind: torch.Tensor,
+                       coef: torch.Tensor, mode: int) -> torch.Tensor | None:
+    """apply_within (mode 0) or its adjoint (mode 1) within one config set through the structured
+    path, or None when that path does not apply (no pair excitations, repeated rows, nq > 16)."""
+    from . import prepared as pp  # pylint: disable=import-outside-toplevel
+
+    if configs.size(0) == 0 or nq > pp.PAIR_MAX_QUBYTES:
+        return None
+    terms = _prepared_terms(nq, pc, site, kind, coef)
+    if terms.pairs_desc is None:  # type: ignore[attr-defined]
+        return None
+    targets = pp.PreparedTargets(configs, nq)
+    if not pp.can_use_pairs(terms, targets, configs):  # type: ignore[arg-type]
+        return None
+    return pp.apply(terms, targets, configs, vec, mode)  # type: ignore[arg-type]
+
+
+def _same_set(configs_i: torch.Tensor, configs_j: torch.Tensor) -> bool:
+    return configs_i.data_ptr() == configs_j.data_ptr() and configs_i.shape == configs_j.shape and configs_i.is_contiguous()
+
+
 def _apply_within(nq: int, pc: int, configs_i: torch.Tensor, psi_i: torch.Tensor, configs_j: torch.Tensor,
                   site: torch.Tensor, kind: torch.Tensor, coef: torch.Tensor) -> torch.Tensor:
     device = configs_i.device
@@ -102,6 +155,10 @@ def _apply_within(nq: int, pc: int, configs_i: torch.Tensor, psi_i: torch.Tensor
     _check_tensor(psi_i, "psi", device, torch.float64, 2, n_i)
     _check_tensor(configs_j, "result_configs", device, torch.uint8, nq, n_j)
     n_terms = _check_terms(site, kind, coef, device, nq)
+    if _same_set(configs_i, configs_j):
+        out_s = _within_structured(nq, pc, configs_i, psi_i.contiguous(), site, kind, coef, 0)
+        if out_s is not None:
+            return out_s
     lib = _lib.library()
     with torch.cuda.device(device):
         ws_bytes = _size_query(lib.qmb_apply_within_workspace_size, nq, n_i, n_j, n_terms)
@@ -125,6 +182,10 @@ def _apply_within_adjoint(nq: int, pc: int, configs_i: torch.Tensor, grad_j: tor
     _check_tensor(configs_j, "result_configs", device, torch.uint8, nq, n_j)
     _check_tensor(grad_j, "grad", device, torch.float64, 2, n_j)
     n_terms = _check_terms(site, kind, coef, device, nq)
+    if _same_set(configs_i, configs_j):
+        out_s = _within_structured(nq, pc, configs_i, grad_j.contiguous(), site, kind, coef, 1)
+        if out_s is not None:
+            return out_s
     lib = _lib.library()
     with torch.cuda.device(device):
         ws_bytes = _size_query(lib.qmb_apply_within_workspace_size, nq, n_i, n_j, n_terms)

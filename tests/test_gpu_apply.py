@@ -86,6 +86,11 @@ def test_apply_within_matches_oracle(cuda_device: torch.device, name: str, pc: i
     _assert_close(got, want, asum, name)
     plain = _complex(ns.apply_within_plain(*args))
     _assert_close(plain, want, asum, name + " (plain kernel)")
+    if cfg_i.shape == cfg_j.shape and np.array_equal(cfg_i, cfg_j):
+        # one tensor for both sets: the structured path when the table has pair excitations
+        same = _to(cfg_i, cuda_device)
+        got_same = _complex(ns.apply_within(same, psi_t, same, *args[3:]))
+        _assert_close(got_same, want, asum, name + " (one config set)")
 
 
 @pytest.mark.parametrize("name,pc,terms,cfg_i,cfg_j", CASES, ids=[c[0] for c in CASES])
@@ -171,3 +176,31 @@ def test_argument_checks(cuda_device: torch.device) -> None:
     kb[0, 0] = 1
     with pytest.raises(RuntimeError, match="site outside"):
         ns.apply_within(cfg, psi, cfg, bad, kb, ct)
+
+
+def test_op_structured_path_cache(cuda_device: torch.device) -> None:
+    """apply_within on one config tensor keeps the compiled table per term tensors, rebuilds it after
+    an in-place change of the coefficients, and differentiates through the same path."""
+    s, k, c = sorted_terms(*sy.molecular_terms(4, 11))
+    cfg = sy.sector_configs(4, 2, 2, 36, 14)
+    psi = sy.random_psi(36, 15)
+    ns = ops.load(1, 1)
+    cfg_t = _to(cfg, cuda_device)
+    st, kt, ct = _to(s, cuda_device), _to(k, cuda_device), _to(c.copy(), cuda_device)
+    psi_t = torch.view_as_real(torch.from_numpy(psi)).contiguous().to(cuda_device)
+    ops._PREPARED.clear()  # pylint: disable=protected-access
+    first = ns.apply_within(cfg_t, psi_t, cfg_t, st, kt, ct)
+    assert len(ops._PREPARED) == 1  # pylint: disable=protected-access
+    second = ns.apply_within(cfg_t, psi_t, cfg_t, st, kt, ct)
+    assert len(ops._PREPARED) == 1 and torch.equal(first, second)  # pylint: disable=protected-access
+    want, asum, _ = orc.apply_within(1, cfg, psi, cfg, s, k, 2 * c)
+    ct.mul_(2.0)
+    doubled = _complex(ns.apply_within(cfg_t, psi_t, cfg_t, st, kt, ct))
+    assert len(ops._PREPARED) == 2  # pylint: disable=protected-access
+    _assert_close(doubled, want, 2 * asum, "after an in-place change")
+    # the gradient through the op equals the enumeration adjoint on a copy of the set
+    x = psi_t.clone().requires_grad_(True)
+    g = torch.view_as_real(torch.from_numpy(sy.random_psi(36, 16))).contiguous().to(cuda_device)
+    (ns.apply_within(cfg_t, x, cfg_t, st, kt, ct) * g).sum().backward()
+    ref = ns.apply_within_adjoint(cfg_t, g, cfg_t.clone(), st, kt, ct)
+    torch.testing.assert_close(x.grad, ref, rtol=0, atol=1e-12 * float(ref.abs().max()))
