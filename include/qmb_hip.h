@@ -169,6 +169,10 @@ typedef struct qmb_targets {
 } qmb_targets;
 
 int qmb_targets_workspace_size(int n_qubytes, int64_t n, size_t* bytes);
+/* rows[i] = the row of configs[i] (n_qubytes bytes) in the indexed set (the lowest one when the set
+ * repeats it), or -1.  No counterpart in the reference: it is how an application across two
+ * config sets is mapped onto one set (see qmb_hip.prepared.apply_across). */
+int qmb_targets_lookup(const qmb_targets* targets, const uint8_t* configs, int64_t n, int64_t* rows, void* stream);
 int qmb_targets_build(int n_qubytes, const uint8_t* configs, int64_t n, void* buffer, size_t bytes, qmb_targets* out,
                       void* stream);
 

@@ -612,4 +612,33 @@ int single_relative_prepared(int n_qubytes, int particle_cut, const uint8_t* con
     });
 }
 
+// Row of each config in an indexed set, or -1 (a thread per config).
+template <int W>
+__global__ void targets_lookup_kernel(const uint64_t* __restrict__ table, uint64_t mask, const uint64_t* __restrict__ keys,
+                                      const uint64_t* __restrict__ cols, const uint8_t* __restrict__ configs, int nq, int64_t n,
+                                      int64_t* __restrict__ rows) {
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    rows[i] = lookup<W>(table, mask, keys, cols, load_row<W>(configs + i * nq, nq));
+}
+
+int targets_lookup(const qmb_targets* targets, const uint8_t* configs, int64_t n, int64_t* rows, hipStream_t stream) {
+    if (!targets || n < 0 || (n > 0 && (!configs || !rows))) {
+        return set_error(QMB_ERR_ARGUMENT, "targets, configs and rows must be non-null");
+    }
+    if (n == 0) {
+        return QMB_OK;
+    }
+    return dispatch_words(targets->n_words, [&](auto wc) -> int {
+        constexpr int W = decltype(wc)::value;
+        const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
+        targets_lookup_kernel<W><<<blocks, 256, 0, stream>>>(targets->table, targets->capacity - 1, targets->keys, targets->cols, configs,
+                                                              targets->n_qubytes, n, rows);
+        QMB_CHECK_LAUNCH();
+        return QMB_OK;
+    });
+}
+
 }  // namespace qmb

@@ -501,6 +501,11 @@ int qmb_sampler_step_fused(int64_t n, int site, int n_sites, int hidden, int spi
                               seed, child_unperturbed, child_perturbed, as_stream(stream));
 }
 
+int qmb_targets_lookup(const qmb_targets* targets, const uint8_t* configs, int64_t n, int64_t* rows, void* stream) {
+    clear_error();
+    return targets_lookup(targets, configs, n, rows, as_stream(stream));
+}
+
 int qmb_lanczos_workspace_size(size_t* bytes) {
     clear_error();
     if (!bytes) {

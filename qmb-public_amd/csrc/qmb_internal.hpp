@@ -184,6 +184,8 @@ int sampler_step_fused(int64_t n, int site, int n_sites, int hidden, int spin_up
                        const float* w1, const float* b1, const float* w2, const float* b2, const float* unperturbed, const float* perturbed,
                        const float* uniform, uint64_t seed, float* child_unperturbed, float* child_perturbed, hipStream_t stream);
 
+int targets_lookup(const qmb_targets* targets, const uint8_t* configs, int64_t n, int64_t* rows, hipStream_t stream);
+
 size_t lanczos_bytes();
 int lanczos_step(int64_t n, const double* v, const double* v_prev, double beta, double* w, double* v_next, double* scalars,
                  void* workspace, size_t bytes, hipStream_t stream);

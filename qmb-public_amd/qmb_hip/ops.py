@@ -19,12 +19,13 @@ takes scratch memory from PyTorch's caching allocator and enqueues the C ABI cal
 stream.  Like the reference, find_relative synchronises to learn its output size; diagonal_term and
 single_relative do not.
 
-apply_within (and its adjoint) on one config set -- configs_j the very tensor configs_i, rows
-distinct -- takes the structured path: the term table is compiled once per (site, kind, coef)
-tensors and kept (a small cache keyed by the tensors and their versions), the set's core indexes
-are built per call, and the pair, single and diagonal kernels apply it.  That path synchronises
-once per call to size the core index.  Other calls run the one-shot enumeration kernel
-(qmb_apply_within), as stateless as the reference's op.
+apply_within (and its adjoint) takes the structured path when the table has pair excitations:
+the term table is compiled once per (site, kind, coef) tensors and kept (a small cache keyed by
+the tensors and their versions), and the pair, single and diagonal kernels apply it within one
+config set -- configs_i itself when configs_j is the very same tensor, else configs_j joined with
+the rows of configs_i it lacks (prepared.apply_across).  That path synchronises to size the set's
+core index.  Calls it does not cover (repeated target rows, no pair excitations, nq > 16) run the
+one-shot enumeration kernel (qmb_apply_within), as stateless as the reference's op.
 """
 
 from __future__ import annotations
@@ -145,6 +146,20 @@ def _same_set(configs_i: torch.Tensor, configs_j: torch.Tensor) -> bool:
     return configs_i.data_ptr() == configs_j.data_ptr() and configs_i.shape == configs_j.shape and configs_i.is_contiguous()
 
 
+def _structured(nq: int, pc: int, configs_i: torch.Tensor, vec: torch.Tensor, configs_j: torch.Tensor, site: torch.Tensor,
+                kind: torch.Tensor, coef: torch.Tensor, mode: int) -> torch.Tensor | None:
+    """apply_within (mode 0) or its adjoint (mode 1) through the structured path: within one set
+    when configs_j is configs_i, else across the two sets; None when it does not apply."""
+    from . import prepared as pp  # pylint: disable=import-outside-toplevel
+
+    if _same_set(configs_i, configs_j):
+        return _within_structured(nq, pc, configs_i, vec.contiguous(), site, kind, coef, mode)
+    if configs_i.size(0) == 0 or configs_j.size(0) == 0 or nq > pp.PAIR_MAX_QUBYTES:
+        return None
+    terms = _prepared_terms(nq, pc, site, kind, coef)
+    return pp.apply_across(terms, configs_i, vec, configs_j, mode)  # type: ignore[arg-type]
+
+
 def _apply_within(nq: int, pc: int, configs_i: torch.Tensor, psi_i: torch.Tensor, configs_j: torch.Tensor,
                   site: torch.Tensor, kind: torch.Tensor, coef: torch.Tensor) -> torch.Tensor:
     device = configs_i.device
@@ -155,10 +170,9 @@ def _apply_within(nq: int, pc: int, configs_i: torch.Tensor, psi_i: torch.Tensor
     _check_tensor(psi_i, "psi", device, torch.float64, 2, n_i)
     _check_tensor(configs_j, "result_configs", device, torch.uint8, nq, n_j)
     n_terms = _check_terms(site, kind, coef, device, nq)
-    if _same_set(configs_i, configs_j):
-        out_s = _within_structured(nq, pc, configs_i, psi_i.contiguous(), site, kind, coef, 0)
-        if out_s is not None:
-            return out_s
+    out_s = _structured(nq, pc, configs_i, psi_i, configs_j, site, kind, coef, 0)
+    if out_s is not None:
+        return out_s
     lib = _lib.library()
     with torch.cuda.device(device):
         ws_bytes = _size_query(lib.qmb_apply_within_workspace_size, nq, n_i, n_j, n_terms)
@@ -182,10 +196,9 @@ def _apply_within_adjoint(nq: int, pc: int, configs_i: torch.Tensor, grad_j: tor
     _check_tensor(configs_j, "result_configs", device, torch.uint8, nq, n_j)
     _check_tensor(grad_j, "grad", device, torch.float64, 2, n_j)
     n_terms = _check_terms(site, kind, coef, device, nq)
-    if _same_set(configs_i, configs_j):
-        out_s = _within_structured(nq, pc, configs_i, grad_j.contiguous(), site, kind, coef, 1)
-        if out_s is not None:
-            return out_s
+    out_s = _structured(nq, pc, configs_i, grad_j, configs_j, site, kind, coef, 1)
+    if out_s is not None:
+        return out_s
     lib = _lib.library()
     with torch.cuda.device(device):
         ws_bytes = _size_query(lib.qmb_apply_within_workspace_size, nq, n_i, n_j, n_terms)

@@ -258,6 +258,51 @@ def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tenso
     return out
 
 
+def apply_across(terms: PreparedTerms, configs_i: torch.Tensor, vec: torch.Tensor, configs_j: torch.Tensor, mode: int,
+                 max_occupied: int | None = None) -> torch.Tensor | None:
+    """The structured path between two config sets: mode 0 returns A psi_i on the rows of configs_j
+    (vec = psi_i, float64 [n_i, 2]); mode 1 returns A^H g on the rows of configs_i (vec = g on
+    configs_j).  Both are computed within one set that holds both -- configs_j followed by the rows
+    of configs_i that configs_j lacks -- with the input placed on its rows (repeated rows of
+    configs_i add up) and the output read from them.  That is exact: the element between two rows
+    of the joint set is the element between the two configs.  Returns None when the path does not
+    apply (no pair excitations, configs_j repeats a row, or a missing row of configs_i repeats),
+    leaving the caller to the enumeration kernel.  Synchronises to size the joint set."""
+    if terms.pairs_desc is None or mode not in (MODE_FORWARD, MODE_ADJOINT):
+        return None
+    n_i, n_j = configs_i.size(0), configs_j.size(0)
+    if n_i == 0 or n_j == 0:
+        return None
+    nq = terms.n_qubytes
+    device = configs_i.device
+    configs_i = configs_i.contiguous()
+    configs_j = configs_j.contiguous()
+    tj = PreparedTargets(configs_j, nq, max_occupied=max_occupied)
+    if tj.duplicates() != 0:
+        return None
+    rows = torch.empty(n_i, dtype=torch.int64, device=device)
+    with torch.cuda.device(device):
+        _lib.check(_lib.library().qmb_targets_lookup(ctypes.byref(tj.desc), _ptr(configs_i), n_i, _ptr(rows), _stream(device)),
+                   "qmb_targets_lookup")
+    missing = rows < 0
+    n_miss = int(missing.sum().item())
+    if n_miss:
+        union = torch.cat([configs_j, configs_i[missing]])
+        pos = rows.clone()
+        pos[missing] = torch.arange(n_j, n_j + n_miss, device=device)
+        targets = PreparedTargets(union, nq, max_occupied=max_occupied)
+        if targets.duplicates() != 0:
+            return None
+    else:
+        union, pos, targets = configs_j, rows, tj
+    uvec = torch.zeros((union.size(0), 2), dtype=torch.float64, device=device)
+    if mode == MODE_FORWARD:
+        uvec.index_add_(0, pos, vec.contiguous())
+        return apply(terms, targets, union, uvec, MODE_FORWARD)[:n_j]
+    uvec[:n_j] = vec
+    return apply(terms, targets, union, uvec, MODE_ADJOINT)[pos]
+
+
 def count(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tensor, tile: int = 0, groups: int = 0,
           strategy: int = 0) -> tuple[int, int]:
     """(M_gen, M_hit): successful term applications and those whose target is indexed -- synchronises."""

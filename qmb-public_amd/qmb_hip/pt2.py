@@ -6,8 +6,8 @@ From a variational state psi on configs:
     other = find_relative(configs, psi, count, configs)        (count = len(configs) in the reference)
     E2 = sum_a |<a|H|psi>|^2 / <psi|psi> / (E0 - H_aa)          over a in other
 The three operator calls are the library's: E0 through the prepared operator on the config set
-(structured path), <a|H|psi> through the prepared term table with a hash index over `other`
-(enumeration kernel, scatter into the foreign set), H_aa through diagonal_term.  The term table is
+(structured path), <a|H|psi> through the structured path on configs joined with `other`
+(prepared.apply_across), H_aa through diagonal_term.  The term table is
 compiled once per Hamiltonian and shared by the three.  The reference's loop that would double
 `count` runs its body once (it breaks unconditionally), so one find_relative call is made here too.
 """
@@ -44,9 +44,12 @@ def _apply_foreign(model: object, configs: torch.Tensor, psi: torch.Tensor, targ
         return model.apply_within(configs, psi, targets_configs)  # type: ignore[attr-defined]
     nq = configs.size(1)
     terms = ham.prepared_terms(configs.device, nq)
-    # find_relative returns distinct rows
-    targets = pp.PreparedTargets(targets_configs, nq, distinct=True)
-    out = pp.apply(terms, targets, configs.contiguous(), torch.view_as_real(psi.resolve_conj().contiguous()), pp.MODE_FORWARD, pairs=False)
+    vec = torch.view_as_real(psi.resolve_conj().contiguous())
+    out = pp.apply_across(terms, configs.contiguous(), vec, targets_configs, pp.MODE_FORWARD)
+    if out is None:
+        # find_relative returns distinct rows
+        targets = pp.PreparedTargets(targets_configs, nq, distinct=True)
+        out = pp.apply(terms, targets, configs.contiguous(), vec, pp.MODE_FORWARD, pairs=False)
     return torch.view_as_complex(out)
 
 
