@@ -20,6 +20,9 @@ Workloads (--workload):
   lanczos52:             Lanczos iterations on the molecular52 basis (application + vector step).
   pt2_52:                the perturbation step (E0, find_relative, application into the external set).
   sampler52:             the config producer: unique sampling of 1M configs from a 26-site network.
+  vmc76:                 one VMC training step: 76 spin orbitals, 13a+13b, 3.06M terms, 2M configs,
+                         psi from the electron up/down network (float32), energy fwd+bwd through the
+                         adjoint, network backward, Adam step.
 
 Prints one JSON line with the metric, a roofline entry for the forward kernel (algorithmic bytes per
 launch / its HIP-event time) and the reference CPU op timed on this host on a bounded sample.
@@ -443,6 +446,191 @@ def run_pt2(args: argparse.Namespace, dev: object) -> int:
     return 0
 
 
+def vmc_energy_reference(terms: tuple[object, object, object], configs: object, psi: object) -> dict[str, object]:
+    """The reference CPU op (oracle/_ref, else the oracle) applied on a small config set:
+    <psi|H|psi> / <psi|psi> there and the time it took (1 thread)."""
+    import numpy as np  # pylint: disable=import-outside-toplevel
+
+    nq = configs.shape[1]
+    ref = ROOT / "oracle" / "_ref" / f"qmb_ref_{nq}_1.so"
+    site, kind, coef = (np.ascontiguousarray(t.numpy()) for t in terms)
+    with tempfile.TemporaryDirectory() as d:
+        inp = pathlib.Path(d) / "in.npz"
+        out = pathlib.Path(d) / "out.npz"
+        np.savez(inp, configs_i=configs, psi_i=psi, configs_j=configs, site=site, kind=kind, coef=coef, particle_cut=1)
+        hpsi, seconds, kind_name = None, 0.0, "port"
+        if ref.exists():
+            env = dict(os.environ, OMP_NUM_THREADS="1")
+            proc = subprocess.run([sys.executable, str(ROOT / "oracle" / "ref_runner.py"), "apply_within", str(inp), str(out),
+                                   "--repeat", "1", "--threads", "1"], env=env, capture_output=True, text=True, check=False)
+            if proc.returncode == 0:
+                data = np.load(out)
+                hpsi, seconds, kind_name = data["result"], float(data["times"][0]), "reference"
+            else:
+                log(f"reference op failed: {proc.stderr[-500:]}")
+        if hpsi is None:
+            from oracle import oracle as orc  # pylint: disable=import-outside-toplevel
+
+            t0 = time.perf_counter()
+            hpsi, _, _ = orc.apply_within(1, configs, psi, configs, site, kind, coef)
+            seconds = time.perf_counter() - t0
+    energy = float(np.vdot(psi, hpsi).real / np.vdot(psi, psi).real)
+    return {"energy": energy, "seconds": seconds, "kind": kind_name}
+
+
+def run_vmc(args: argparse.Namespace, dev: object) -> int:
+    """One VMC training step (qmb/vmc.py:104-118) at 76 spin orbitals, 13a+13b, |(pq|rs)| < 1e-8
+    dropped (3.06M terms), over 2M configs: the electron up/down network in float32 (each site network
+    and the phase network 4 layers of width 512 by default), psi cast to complex128, the energy
+    <psi|H|psi>/<psi|psi> forward and backward through the adjoint kernel, the network's backward
+    (in row chunks) and an Adam step.  The config set and its index are the cycle's, built once per
+    cycle of local steps, outside the timed steps (reported as setup)."""
+    import numpy as np  # pylint: disable=import-outside-toplevel
+    import torch  # pylint: disable=import-outside-toplevel
+
+    from qmb_hip import Hamiltonian  # pylint: disable=import-outside-toplevel
+    from qmb_hip import mlp  # pylint: disable=import-outside-toplevel
+    from qmb_hip import prepared as pp  # pylint: disable=import-outside-toplevel
+    from qmb_hip import synthetic as sy  # pylint: disable=import-outside-toplevel
+    from qmb_hip import vmc  # pylint: disable=import-outside-toplevel
+
+    n_orbit, n_e, n_configs = 38, 13, 2_000_000
+    hidden = tuple(int(h) for h in args.hidden.split(",")) if args.hidden else (512, 512, 512)
+    t0 = time.time()
+    site, kind, coef = sy.molecular_terms(n_orbit, 5001, threshold=1e-8)
+    order = np.argsort(-np.linalg.norm(coef.numpy(), axis=1), kind="stable")
+    site, kind, coef = site[order], kind[order], coef[order]
+    configs_np = sy.excitation_configs(n_orbit, n_e, n_e, n_configs, 5002)
+    log(f"vmc76: {len(site)} terms, {len(configs_np)} configs built in {time.time() - t0:.1f}s")
+    ham = Hamiltonian(tuple(t.contiguous().to(dev) for t in (site, kind, coef)), kind="fermi")
+    configs = torch.from_numpy(configs_np).to(dev)
+    torch.manual_seed(5003)
+    net = mlp.WaveFunctionElectronUpDown(double_sites=2 * n_orbit, physical_dim=2, is_complex=True, spin_up=n_e, spin_down=n_e,
+                                         hidden_size=hidden, ordering=+1).to(dev)
+    opt = torch.optim.Adam(net.parameters(), lr=1e-3)
+    chunk = args.chunk
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(fn: object) -> float:
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b)
+
+    holder: dict[str, object] = {}
+
+    def setup() -> None:
+        es = vmc.EnergySet(ham, configs, max_occupied=2 * n_e)
+        es.targets.cores(2)
+        es.targets.cores(1)
+        holder["set"] = es
+
+    setup_ms = timed(setup)
+    energy_set = holder["set"]
+    energies: list[float] = []
+
+    def step() -> None:
+        opt.zero_grad()
+        energies.append(vmc.energy_backward(net, energy_set, chunk))
+        opt.step()
+
+    for i in range(max(args.warmup, 1)):
+        step()
+        log(f"warmup {i + 1}: energy {energies[-1]:.10f}")
+    torch.cuda.synchronize()
+    steps = max(args.steps, 1)
+    t1 = time.perf_counter()
+    for i in range(steps):
+        step()
+        log(f"step {i + 1}/{steps}: energy {energies[-1]:.10f}")
+    torch.cuda.synchronize()
+    step_ms = (time.perf_counter() - t1) * 1e3 / steps
+
+    # the parts of a step, each timed alone with events
+    psi_leaf = vmc.network_forward(net, configs, chunk).requires_grad_(True)
+
+    def energy_fwd_bwd() -> None:
+        psi_leaf.grad = None
+        energy_set.energy(psi_leaf).backward()
+
+    parts = {"network_forward_ms": timed(lambda: vmc.network_forward(net, configs, chunk)), "energy_forward_backward_ms": timed(energy_fwd_bwd)}
+    grad_psi = psi_leaf.grad
+    net.zero_grad()
+    parts["network_backward_ms"] = timed(lambda: vmc.network_backward(net, configs, grad_psi, chunk))
+    terms, targets = energy_set.terms, energy_set.targets
+    vec = torch.view_as_real(psi_leaf.detach()).contiguous()
+    out = torch.zeros_like(vec)
+    fwd = []
+    for _ in range(5):
+        fwd.append(timed(lambda: pp.apply(terms, targets, configs, vec, pp.MODE_FORWARD, out=out)))
+    forward_ms = float(np.median(fwd))
+    m_gen, m_hit = pp.count(terms, targets, configs)
+    nq = configs.size(1)
+    algo_bytes = 2 * n_configs * (nq + 16) + len(site) * 28 + m_gen * nq + m_hit * 16
+    achieved = algo_bytes / (forward_ms * 1e-3) / 1e9
+    log(f"vmc76: {step_ms:.1f} ms/step, setup {setup_ms:.1f} ms, parts {parts}, H forward {forward_ms:.2f} ms, M_hit {m_hit}")
+
+    sampler_ms = None
+    if args.sample:
+        with torch.no_grad():  # psi of the sample feeds find_relative only
+            sampler_ms = timed(lambda: net.generate_unique(n_configs, max(1, n_configs // chunk), seed=5004))
+        log(f"generate_unique({n_configs}): {sampler_ms:.1f} ms")
+
+    cpu = None
+    if not args.no_cpu:
+        # energy error on a set near the reference determinant: HF + 255 seeded singles/doubles of the set
+        occ = sy.unpack_occupation(configs_np, 2 * n_orbit)
+        hf = np.zeros(2 * n_orbit, dtype=np.uint8)
+        hf[0:2 * n_e:2] = 1
+        hf[1:2 * n_e:2] = 1
+        level = (occ != hf).sum(axis=1) // 2
+        rows_hf = np.flatnonzero(level == 0)
+        rows_sd = np.flatnonzero((level >= 1) & (level <= 2))
+        rows = np.sort(np.concatenate([rows_hf, np.random.default_rng(5005).choice(rows_sd, 255, replace=False)]))
+        sub = configs_np[rows]
+        sub_t = torch.from_numpy(sub).to(dev)
+        psi_sub = vmc.network_forward(net, sub_t, chunk)
+        e_gpu = float(vmc.EnergySet(ham, sub_t, max_occupied=2 * n_e).energy(psi_sub))
+        ref = vmc_energy_reference((site, kind, coef), sub, psi_sub.cpu().numpy())
+        m_sub = pp.count(terms, pp.PreparedTargets(sub_t, nq, distinct=True), sub_t)[1]
+        cpu = {"value": m_sub / ref["seconds"], "unit": "matrix elements/s", "cores": 1, "kind": ref["kind"],
+               "sample": f"apply_within on {len(sub)} configs (HF + 255 seeded singles/doubles of the set) into themselves, "
+                         + ("qmb/_hamiltonian_cpu.cpp built from source, -O3 -ffast-math, 1 thread" if ref["kind"] == "reference"
+                            else "oracle/qmb_oracle.c, 1 thread"),
+               "seconds": ref["seconds"], "energy_cpu": ref["energy"], "energy_gpu": e_gpu,
+               "energy_error": abs(e_gpu - ref["energy"])}
+        log(f"energy on the sample: GPU {e_gpu:.15f} CPU {ref['energy']:.15f} ({ref['seconds']:.1f} s)")
+
+    result = {
+        "metric": "VMC training step time",
+        "value": step_ms,
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": max(args.warmup, 1),
+        "ms_per_step": step_ms,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 network, f64 Hamiltonian",
+        "data": "synthetic (seeded Hamiltonian and configs, seeded network init)",
+        "config": {"workload": "vmc76", "spin_orbitals": 2 * n_orbit, "electrons": f"{n_e}a+{n_e}b", "threshold": 1e-8,
+                   "n_terms": len(site), "n_configs": n_configs, "configs": "HF + singles + doubles + seeded triples/quadruples",
+                   "network": "WaveFunctionElectronUpDown", "hidden": list(hidden), "chunk": chunk, "optimizer": "Adam(lr=1e-3)",
+                   "M_gen": m_gen, "M_hit": m_hit},
+        "vmc": dict(parts, setup_ms=setup_ms, energies=energies, h_forward_ms=forward_ms, generate_unique_ms=sampler_ms,
+                    h_elements_per_s=2 * m_hit / (parts["energy_forward_backward_ms"] * 1e-3)),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": None, "kernel": "H forward (pair_kernel<W=2> + single_kernel<W=2> + diagonal)", "kernel_ms": forward_ms,
+                     "algorithmic_bytes": algo_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def run_sampler(args: argparse.Namespace, dev: object) -> int:
     """The config producer on molecular52's shape (qmb/mlp.py:61-314): 26 sites, 7 + 7 electrons,
     one hidden layer of 512, float32; generate_unique of 1,000,000 distinct configs with their psi,
@@ -537,7 +725,10 @@ def main() -> int:
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=10)
     parser.add_argument("--warmup", type=int, default=3)
-    parser.add_argument("--workload", default="molecular52", choices=["molecular52", "hubbard128", "molecular20", "batch64", "lanczos52", "pt2_52", "sampler52"])
+    parser.add_argument("--workload", default="molecular52", choices=["molecular52", "hubbard128", "molecular20", "batch64", "lanczos52", "pt2_52", "sampler52", "vmc76"])
+    parser.add_argument("--hidden", default="", help="vmc76: network hidden widths, comma separated (default 512,512,512)")
+    parser.add_argument("--chunk", type=int, default=65536, help="vmc76: rows per network backward chunk")
+    parser.add_argument("--sample", action="store_true", help="vmc76: also time generate_unique of the 2M configs")
     parser.add_argument("--tile", type=int, default=0)
     parser.add_argument("--groups", type=int, default=0)
     parser.add_argument("--no-plain", action="store_true", help="skip the reference-scheme comparator")
@@ -569,6 +760,8 @@ def main() -> int:
         return run_pt2(args, dev)
     if args.workload == "sampler52":
         return run_sampler(args, dev)
+    if args.workload == "vmc76":
+        return run_vmc(args, dev)
     w = build_workload(args.workload)
     configs = torch.from_numpy(w["configs"]).to(dev)
     nq = configs.shape[1]
