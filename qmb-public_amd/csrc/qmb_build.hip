@@ -444,7 +444,12 @@ int targets_layout(Carver& cv, int n_qubytes, int64_t n, qmb_targets* out) {
     t.n = n;
     t.capacity = next_pow2(static_cast<uint64_t>(4 * n > 64 ? 4 * n : 64));  // load factor <= 1/4
     t.buffer = cv.base;
-    t.filter_words = next_pow2(static_cast<uint64_t>(n / 4 > 64 ? n / 4 : 64));  // 16 bits per key
+    // 16 bits per key up to 2 MB of filter (~0.5% false positives), then 8 bits per key (~3%): a
+    // filter that stays in an XCD's 4 MB L2 answers more of the probes than a sparser one
+    // that spills (4M configs: 8 MB -> 4 MB measured 3% faster over the whole lookup)
+    constexpr uint64_t kFilterWords16 = 1ull << 18;
+    const uint64_t w16 = next_pow2(static_cast<uint64_t>(n / 4 > 64 ? n / 4 : 64));
+    t.filter_words = w16 <= kFilterWords16 ? w16 : std::max(kFilterWords16, next_pow2(static_cast<uint64_t>(n / 8)));
     t.keys = cv.take<uint64_t>(static_cast<size_t>(n > 0 ? n : 1) * t.n_words);
     t.table = cv.take<uint64_t>(t.capacity);
     t.filter = cv.take<uint64_t>(t.filter_words);

@@ -100,8 +100,10 @@ struct HashIndex {
     const uint64_t* cols;
 };
 
-// Batched lookups: the first probe of every wanted slot is issued before any is resolved, so a
-// thread keeps U independent loads in flight instead of one dependent chain per connection.
+// Batched lookups in phases, so a thread keeps U independent loads in flight at each step instead
+// of one dependent chain per connection: the filter words of every wanted target, then the first
+// table slot of every target that passed the filter, then the stored keys of every slot whose
+// fingerprint matched.  The rare longer probe sequences (load factor <= 1/4) finish one by one.
 // Action protocol (hs, ht: index hashes of source and target; ht == hs iff the term flips nothing):
 //   bool pre(i, t, target, parity, hs, ht)       -> true if the target must be looked up
 //   void post(i, t, target, parity, j, distinct) -> j = row of the target in the index, or -1;
@@ -110,43 +112,69 @@ template <int W, int U, typename Action>
 __device__ __forceinline__ void lookup_batch(const HashIndex& ix, Action& act, const int64_t (&i)[U], const uint32_t (&t)[U],
                                              const Cfg<W> (&o)[U], const int (&par)[U], const bool (&want)[U],
                                              const uint64_t (&h)[U], const uint64_t (&hs)[U]) {
-    uint64_t f[U];
-    // Filter first: most targets are absent and are answered by one L2-resident word.
+    uint64_t e[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        f[u] = want[u] ? ix.filter[filter_word(h[u], ix.fmask)] : 0ull;
+        e[u] = want[u] ? ix.filter[filter_word(h[u], ix.fmask)] : 0ull;
     }
+    // e[u]: the first table slot of a target that passed the filter, else kEmptySlot
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        if (!want[u]) {
-            continue;
-        }
         const uint64_t bits = filter_bits(h[u]);
-        int64_t j = -1;
-        if ((f[u] & bits) == bits) {
+        const bool pass = want[u] && (e[u] & bits) == bits;
+        e[u] = pass ? ix.table[(h[u] >> 32) & ix.mask] : kEmptySlot;
+    }
+    uint64_t key[U][W];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const bool fp = e[u] != kEmptySlot && static_cast<uint32_t>(e[u]) == static_cast<uint32_t>(h[u]);
+        const uint64_t jj = fp ? e[u] >> 32 : 0ull;
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            key[u][k] = fp ? ix.keys[jj * W + k] : ~o[u].w[k];
+        }
+    }
+    int64_t j[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        bool eq = true;
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            eq = eq && key[u][k] == o[u].w[k];
+        }
+        j[u] = -1;
+        if (eq) {
+            j[u] = static_cast<int64_t>(e[u] >> 32);
+        } else if (e[u] != kEmptySlot) {
+            // the first slot held another key: continue the linear probe from the next slot
             const uint32_t fp = static_cast<uint32_t>(h[u]);
-            uint64_t s = (h[u] >> 32) & ix.mask;
-            for (uint64_t probe = 0; probe <= ix.mask; ++probe) {
+            uint64_t s = (((h[u] >> 32) & ix.mask) + 1) & ix.mask;
+            for (uint64_t probe = 1; probe <= ix.mask; ++probe) {
                 const uint64_t ee = ix.table[s];
                 if (ee == kEmptySlot) {
                     break;
                 }
                 if (static_cast<uint32_t>(ee) == fp) {
                     const uint64_t jj = ee >> 32;
-                    bool eq = true;
+                    bool same = true;
 #pragma unroll
                     for (int k = 0; k < W; ++k) {
-                        eq = eq && (ix.keys[jj * W + k] == o[u].w[k]);
+                        same = same && (ix.keys[jj * W + k] == o[u].w[k]);
                     }
-                    if (eq) {
-                        j = static_cast<int64_t>(jj);
+                    if (same) {
+                        j[u] = static_cast<int64_t>(jj);
                         break;
                     }
                 }
                 s = (s + 1) & ix.mask;
             }
         }
-        act.post(i[u], t[u], o[u], par[u], j, h[u] != hs[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (want[u]) {
+            act.post(i[u], t[u], o[u], par[u], j[u], h[u] != hs[u]);
+        }
     }
 }
 
@@ -379,6 +407,8 @@ __device__ void enumerate_tile(const EnumSource& p, const HashIndex& ix, int64_t
 // Direct enumeration for small term tables: each thread owns one config in registers and walks
 // every term, the terms staged through LDS (expanded to masks) in chunks shared by the workgroup.
 constexpr int kDirectChunk = 256;
+// Lookups kept in flight per thread by the direct enumeration.
+constexpr int kDirectBatch = 4;
 
 template <int W>
 struct DirectLds {
@@ -423,33 +453,49 @@ __device__ void enumerate_direct(const EnumSource& p, const HashIndex& ix, int64
         }
         __syncthreads();
         if (valid) {
-            for (int u0 = 0; u0 < cnt; u0 += kBatch) {
-                int64_t ii[kBatch];
-                uint32_t tt[kBatch];
-                Cfg<W> o[kBatch];
-                int par[kBatch];
-                bool want[kBatch];
-                uint64_t h[kBatch];
-                uint64_t hs[kBatch];
-#pragma unroll
-                for (int b = 0; b < kBatch; ++b) {
-                    const int u = u0 + b < cnt ? u0 + b : cnt - 1;
-                    bool ok = u0 + b < cnt;
-                    int pr = L.pc[u];
+            // Per 32 terms: a mask of those that apply to this config, then lookups of kDirectBatch of
+            // them at a time (in term order), so the loads in flight follow the successful terms
+            // rather than the terms tried.
+            for (int t0 = 0; t0 < cnt; t0 += 32) {
+                const int lim = cnt - t0 < 32 ? cnt - t0 : 32;
+                uint32_t todo = 0;
+                for (int k = 0; k < lim; ++k) {
+                    const int u = t0 + k;
+                    bool ok = true;
 #pragma unroll
                     for (int w = 0; w < W; ++w) {
                         ok = ok && ((c.w[w] & L.m[u * W + w]) == L.v[u * W + w]);
-                        o[b].w[w] = c.w[w] ^ L.f[u * W + w];
-                        pr ^= __popcll(c.w[w] & L.p[u * W + w]) & 1;
                     }
-                    par[b] = pr;
-                    ii[b] = i;
-                    tt[b] = static_cast<uint32_t>(base + u);
-                    hs[b] = hc;
-                    h[b] = hc ^ L.hf[u];
-                    want[b] = ok && act.pre(i, tt[b], o[b], pr, hc, h[b]);
+                    todo |= ok ? (1u << k) : 0u;
                 }
-                lookup_batch<W, kBatch>(ix, act, ii, tt, o, par, want, h, hs);
+                while (todo != 0u) {
+                    int64_t ii[kDirectBatch];
+                    uint32_t tt[kDirectBatch];
+                    Cfg<W> o[kDirectBatch];
+                    int par[kDirectBatch];
+                    bool want[kDirectBatch];
+                    uint64_t h[kDirectBatch];
+                    uint64_t hs[kDirectBatch];
+#pragma unroll
+                    for (int b = 0; b < kDirectBatch; ++b) {
+                        const bool have = todo != 0u;
+                        const int u = t0 + (have ? __builtin_ctz(todo) : 0);
+                        todo = have ? todo & (todo - 1u) : 0u;
+                        int pr = L.pc[u];
+#pragma unroll
+                        for (int w = 0; w < W; ++w) {
+                            o[b].w[w] = c.w[w] ^ L.f[u * W + w];
+                            pr ^= __popcll(c.w[w] & L.p[u * W + w]) & 1;
+                        }
+                        par[b] = pr;
+                        ii[b] = i;
+                        tt[b] = static_cast<uint32_t>(base + u);
+                        hs[b] = hc;
+                        h[b] = hc ^ L.hf[u];
+                        want[b] = have && act.pre(i, tt[b], o[b], pr, hc, h[b]);
+                    }
+                    lookup_batch<W, kDirectBatch>(ix, act, ii, tt, o, par, want, h, hs);
+                }
             }
         }
     }
