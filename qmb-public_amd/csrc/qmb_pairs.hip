@@ -1187,8 +1187,17 @@ template <bool ADJ, int NV>
 __global__ __launch_bounds__(kBlock, 4) void batch_kernel(const BatchItemDev* __restrict__ items, const int64_t* __restrict__ offsets,
                                                        int n_items, int64_t total) {
     const int lane = threadIdx.x & 63;
-    const int64_t wave = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6;
-    const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * kBlock) >> 6;
+    constexpr int kXcds = 8;
+    constexpr int64_t kWavesPerBlock = kBlock / 64;
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+    // Workgroups go to the 8 XCDs round-robin by blockIdx.x.  When the grid divides evenly, each XCD
+    // takes a contiguous eighth of every window of nwaves units, so its L2 holds the coefficient rows
+    // of that eighth instead of rows from the whole window.
+    int64_t wave = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    if (gridDim.x % kXcds == 0) {
+        wave = static_cast<int64_t>(blockIdx.x % kXcds) * (nwaves / kXcds) + static_cast<int64_t>(blockIdx.x / kXcds) * kWavesPerBlock +
+               (threadIdx.x >> 6);
+    }
     __shared__ __align__(16) char stage[kPairStageBytes];
     const int slots = 3 * n_items;
     for (int64_t u = wave; u < total; u += nwaves) {
