@@ -38,6 +38,24 @@ inline unsigned blocks_for(int64_t n, int block = kBlock) {
     return static_cast<unsigned>(b < 1 ? 1 : b);
 }
 
+// Index of the calling wave among the grid's persistent waves, ordered so that each XCD holds a
+// contiguous eighth of [0, waves).  Workgroups go to the 8 XCDs round-robin by blockIdx.x, so with
+// a grid that divides evenly, waves walking units w, w + waves, ... keep each XCD's L2 on the
+// coefficient rows of an eighth of every window of units instead of rows from the whole window.
+// Measured: the batch kernel gains (138.9 -> 134 ms on 64 molecular instances); the single-set
+// pair and single kernels lose (8.07 -> 8.46 ms at 1M configs), so they keep the plain order.
+__device__ __forceinline__ int64_t xcd_wave() {
+    constexpr int kXcds = 8;
+    constexpr int64_t kWavesPerBlock = kBlock / 64;
+    const int64_t in_block = threadIdx.x >> 6;
+    if (gridDim.x % kXcds != 0) {
+        return static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + in_block;
+    }
+    const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+    return static_cast<int64_t>(blockIdx.x % kXcds) * (waves / kXcds) + static_cast<int64_t>(blockIdx.x / kXcds) * kWavesPerBlock +
+           in_block;
+}
+
 __global__ void zero_counts_kernel(int64_t* counts) {
     if (threadIdx.x < 4) {
         counts[threadIdx.x] = 0;
@@ -1187,17 +1205,8 @@ template <bool ADJ, int NV>
 __global__ __launch_bounds__(kBlock, 4) void batch_kernel(const BatchItemDev* __restrict__ items, const int64_t* __restrict__ offsets,
                                                        int n_items, int64_t total) {
     const int lane = threadIdx.x & 63;
-    constexpr int kXcds = 8;
-    constexpr int64_t kWavesPerBlock = kBlock / 64;
-    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
-    // Workgroups go to the 8 XCDs round-robin by blockIdx.x.  When the grid divides evenly, each XCD
-    // takes a contiguous eighth of every window of nwaves units, so its L2 holds the coefficient rows
-    // of that eighth instead of rows from the whole window.
-    int64_t wave = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
-    if (gridDim.x % kXcds == 0) {
-        wave = static_cast<int64_t>(blockIdx.x % kXcds) * (nwaves / kXcds) + static_cast<int64_t>(blockIdx.x / kXcds) * kWavesPerBlock +
-               (threadIdx.x >> 6);
-    }
+    const int64_t wave = xcd_wave();
+    const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * kBlock) >> 6;
     __shared__ __align__(16) char stage[kPairStageBytes];
     const int slots = 3 * n_items;
     for (int64_t u = wave; u < total; u += nwaves) {
