@@ -7,7 +7,10 @@ the differentiable apply_within:
   - M_gen / M_hit from the count kernel equal the oracle's counts exactly;
   - autograd backward equals the dense conjugate transpose (non-Hermitian table);
   - the energy <psi|H|psi>/<psi|psi> gradient against central finite differences (step 1e-6) on
-    random psi entries, and against PyTorch autograd through a dense matrix built by the oracle.
+    random psi entries, and against PyTorch autograd through a dense matrix built by the oracle;
+  - the same central differences at the differentiation workload's full size: the open 64-site
+    Hubbard chain (128 spin orbitals, 16-byte configs, t = 1, U = 4, half filling), 4M configs,
+    64 random psi entries, real and imaginary parts.
 """
 
 from __future__ import annotations
@@ -145,6 +148,42 @@ def test_energy_gradient_central_differences(cuda_device: torch.device) -> None:
             # PyTorch's complex gradient of a real loss is dL/dRe + i dL/dIm
             an = grad[idx].real if part == "re" else grad[idx].imag
             assert abs(fd - an) < 1e-7 * max(1.0, abs(fd)), (idx, part, fd, an)
+
+
+def test_energy_gradient_central_differences_hubbard128(cuda_device: torch.device) -> None:
+    """dE/dpsi from the adjoint kernel against central differences of the energy (forward kernel),
+    step 1e-6, on 64 random entries of a 4M-config psi over the 128-spin-orbital chain."""
+    s, k, c = (x.contiguous().to(cuda_device) for x in sy.terms_from_dict(sy.hubbard_dict(64, 1, 1.0, 4.0)))
+    cfg_t = torch.from_numpy(sy.hubbard_chain_configs(64, 4_000_000, 3000)).to(cuda_device)
+    n = cfg_t.size(0)
+    assert n == 4_000_000 and cfg_t.size(1) == 16
+    terms = pp.PreparedTerms(s, k, c, 16, 1)
+    targets = pp.PreparedTargets(cfg_t, 16, distinct=True, max_occupied=64)
+    psi0 = torch.from_numpy(sy.random_psi(n, 3001)).to(cuda_device)
+    psi = psi0.clone().requires_grad_(True)
+    ag.energy(psi, cfg_t, terms, targets).backward()
+    grad = psi.grad.cpu().numpy()
+
+    def energy(p: torch.Tensor) -> float:
+        with torch.no_grad():
+            return float(ag.energy(p, cfg_t, terms, targets))
+
+    h = 1e-6
+    # The gradient entries are ~2e-4 here and each energy (sums over 4M rows) carries a few 1e-14 of
+    # rounding, so a difference quotient is good to ~2e-8 absolute; a wrong sign, conjugation or
+    # dropped term moves an entry by O(|entry|).
+    fds, ans = [], []
+    for idx in np.random.default_rng(3002).choice(n, size=64, replace=False):
+        for part, unit in (("re", 1.0), ("im", 1j)):
+            p_plus, p_minus = psi0.clone(), psi0.clone()
+            p_plus[idx] += h * unit
+            p_minus[idx] -= h * unit
+            fd = (energy(p_plus) - energy(p_minus)) / (2 * h)
+            an = grad[idx].real if part == "re" else grad[idx].imag
+            assert abs(fd - an) <= 1e-4 * abs(an) + 1e-7, (int(idx), part, fd, an)
+            fds.append(fd)
+            ans.append(an)
+    assert np.linalg.norm(np.array(fds) - np.array(ans)) <= 1e-3 * np.linalg.norm(ans)
 
 
 def test_energy_gradient_matches_dense_autograd(cuda_device: torch.device) -> None:
