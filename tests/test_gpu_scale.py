@@ -9,6 +9,8 @@ dropped (669,579 terms), 1,000,000 configs (HF + singles + doubles + seeded trip
   - The structured path (pair / single core groups + folded diagonal) equals the enumeration path
     (hash lookups of every generated target) on every row, forward and adjoint.
   - <phi, H psi> = conj(<psi, H phi>) over the whole set.
+  - find_relative(K = 100,000, exclude = the set): the output properties of the reference
+    (qmb/_hamiltonian_cpu.cpp:442-546) and nesting of the K = 50,000 selection.
 """
 
 from __future__ import annotations
@@ -76,3 +78,27 @@ def test_molecular52_hermitian_identity(molecular52: dict[str, object]) -> None:
     lhs = np.vdot(phi, _apply(w, psi, pp.MODE_FORWARD))
     rhs = np.conj(np.vdot(psi, _apply(w, phi, pp.MODE_FORWARD)))
     assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
+
+
+def test_molecular52_find_relative_properties(molecular52: dict[str, object]) -> None:
+    """find_relative(K = 100,000, exclude = the set) at full size: at most K rows, distinct, sorted by
+    bytes, none in the exclude set, all in the 7a+7b sector; the rows kept for K = 50,000 are among
+    those kept for K = 100,000 (top-50k entries are top-100k entries; a boundary tie class may swap a
+    few rows)."""
+    from qmb_hip import ops  # pylint: disable=import-outside-toplevel
+
+    w = molecular52
+    dev = w["device"]
+    s, k, c = (torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in w["terms_np"])
+    psi = torch.view_as_real(torch.from_numpy(w["psi"]).to(dev)).contiguous()
+    ns = ops.load(7, 1)
+    big = ns.find_relative(w["cfg_t"], psi, 100_000, s, k, c, w["cfg_t"]).cpu().numpy()
+    small = ns.find_relative(w["cfg_t"], psi, 50_000, s, k, c, w["cfg_t"]).cpu().numpy()
+    for rows, kk in ((big, 100_000), (small, 50_000)):
+        assert 0.9 * kk <= len(rows) <= kk
+        keys = [r.tobytes() for r in rows]
+        assert keys == sorted(set(keys))
+        assert not set(keys) & {r.tobytes() for r in w["cfg"]}
+        occ = sy.unpack_occupation(rows, 52)
+        assert (occ[:, 0::2].sum(axis=1) == 7).all() and (occ[:, 1::2].sum(axis=1) == 7).all()
+    assert len({r.tobytes() for r in small} - {r.tobytes() for r in big}) <= 64
