@@ -239,12 +239,16 @@ int qmb_pairs_build(int n_qubytes, int particle_cut, const int16_t* site, const 
  *   (-1)^(popcount(c & (L(r) ^ L(p)))) * (a[r * n_bits + p] + sum_{k in c, k != r} kd[(r * n_bits + p) * n_bits + k])
  * with L(s) = the sites below s.  n_qubytes <= 16. */
 typedef struct qmb_singles {
-    int32_t n_qubytes, particle_cut, n_bits, reserved;
+    int32_t n_qubytes, particle_cut, n_bits;
+    int32_t real;      /* 1: the kernels read a_re / kd_re (set by qmb_singles_set_real) */
     void* buffer;
     size_t bytes;
     double* a;         /* [n_bits * n_bits][2] */
     double* kd;        /* [n_bits * n_bits * n_bits][2] */
-    int64_t* counts;   /* device: {single-excitation terms, 0, 0, 0} */
+    int64_t* counts;   /* device: {single-excitation terms, 0, entries with a nonzero imaginary part
+                          (after qmb_singles_set_real), 0} */
+    double* a_re;      /* [n_bits * n_bits] real parts, filled by qmb_singles_set_real */
+    double* kd_re;     /* [n_bits * n_bits * n_bits] */
 } qmb_singles;
 
 int qmb_singles_workspace_size(int n_qubytes, size_t* bytes);
@@ -256,6 +260,8 @@ int qmb_singles_build(int n_qubytes, int particle_cut, const int16_t* site, cons
  * table.  Counts the entries with a nonzero imaginary part into counts[2]; the caller asserts
  * there are none. */
 int qmb_pairs_set_real(qmb_pairs* pairs, int32_t real, void* stream);
+/* The same for the single-excitation table (a and kd).  No reference counterpart. */
+int qmb_singles_set_real(qmb_singles* singles, int32_t real, void* stream);
 
 /* qmb_terms_build without some terms (same layout and buffer size).  exclude is a mask:
  * 1 pair excitations, 2 single excitations, 4 diagonal terms (from the buckets; the diagonal

@@ -333,6 +333,11 @@ int qmb_pairs_set_real(qmb_pairs* pairs, int32_t real, void* stream) {
     return pairs_set_real(pairs, real, as_stream(stream));
 }
 
+int qmb_singles_set_real(qmb_singles* singles, int32_t real, void* stream) {
+    clear_error();
+    return singles_set_real(singles, real, as_stream(stream));
+}
+
 int qmb_singles_workspace_size(int n_qubytes, size_t* bytes) {
     clear_error();
     if (!bytes) {

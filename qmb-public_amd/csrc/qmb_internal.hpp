@@ -144,6 +144,7 @@ size_t pairs_bytes(int n_qubytes);
 int pairs_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
                 void* buffer, size_t bytes, qmb_pairs* out, hipStream_t stream);
 int pairs_set_real(qmb_pairs* pairs, int real, hipStream_t stream);
+int singles_set_real(qmb_singles* singles, int real, hipStream_t stream);
 size_t singles_bytes(int n_qubytes);
 int singles_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
                   void* buffer, size_t bytes, qmb_singles* out, hipStream_t stream);

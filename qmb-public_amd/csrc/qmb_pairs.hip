@@ -784,6 +784,8 @@ int singles_layout(Carver& cv, int n_qubytes, qmb_singles* out) {
     g.counts = cv.take<int64_t>(4);
     g.a = cv.take<double>(2 * nb * nb);
     g.kd = cv.take<double>(2 * nb * nb * nb);
+    g.a_re = cv.take<double>(nb * nb);
+    g.kd_re = cv.take<double>(nb * nb * nb);
     g.bytes = cv.off;
     if (out) {
         *out = g;
@@ -794,6 +796,8 @@ int singles_layout(Carver& cv, int n_qubytes, qmb_singles* out) {
 struct SingleParams {
     const double2* a;
     const double2* kd;
+    const double* a_re;   // their real parts when every coefficient is real, else null
+    const double* kd_re;
     int n_bits;
     int particle_cut;
     const uint64_t* ckeys;
@@ -816,6 +820,21 @@ __device__ __forceinline__ int single_parity(const Cfg<W>& c, int r, int p) {
 template <int W>
 __device__ __forceinline__ double2 single_element(const SingleParams& p, const Cfg<W>& core, int r, int q) {
     const int64_t rq = static_cast<int64_t>(r) * p.n_bits + q;
+    if (p.kd_re) {
+        // real table: 8-byte reads, the sum kept in one register
+        double xr = p.a_re[rq];
+        const double* row_re = p.kd_re + rq * p.n_bits;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            uint64_t m = core.w[w];
+            while (m) {
+                const int k = 64 * w + __builtin_ctzll(m);
+                m &= m - 1;
+                xr += row_re[k];
+            }
+        }
+        return make_double2(xr, 0.0);
+    }
     double2 x = p.a[rq];
     const double2* row = p.kd + rq * p.n_bits;
 #pragma unroll
@@ -1117,6 +1136,8 @@ struct BatchItemDev {
     const int64_t* pcounts;
     const double2* sa;
     const double2* skd;
+    const double* sa_re;
+    const double* skd_re;
     const uint64_t* svals;
     const uint32_t* sstarts;
     const int64_t* scounts;
@@ -1173,6 +1194,8 @@ __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int 
         SingleParams p{};
         p.a = it.sa;
         p.kd = it.skd;
+        p.a_re = it.sa_re;
+        p.kd_re = it.skd_re;
         p.n_bits = it.n_bits;
         p.particle_cut = it.particle_cut;
         p.ckeys = it.ckeys;
@@ -1263,6 +1286,24 @@ int pairs_build(int n_qubytes, int particle_cut, const int16_t* site, const uint
         QMB_CHECK_LAUNCH();
     }
     *out = p;
+    return QMB_OK;
+}
+
+int singles_set_real(qmb_singles* singles, int real, hipStream_t stream) {
+    if (!singles) {
+        return set_error(QMB_ERR_ARGUMENT, "singles must be non-null");
+    }
+    singles->real = 0;
+    if (real) {
+        const int64_t nb = singles->n_bits;
+        pairs_real_kernel<<<blocks_for(nb * nb), kBlock, 0, stream>>>(reinterpret_cast<const double2*>(singles->a), nb * nb,
+                                                                      singles->a_re, singles->counts);
+        QMB_CHECK_LAUNCH();
+        pairs_real_kernel<<<blocks_for(nb * nb * nb), kBlock, 0, stream>>>(reinterpret_cast<const double2*>(singles->kd),
+                                                                           nb * nb * nb, singles->kd_re, singles->counts);
+        QMB_CHECK_LAUNCH();
+        singles->real = 1;
+    }
     return QMB_OK;
 }
 
@@ -1440,6 +1481,8 @@ int apply_singles(const qmb_singles* singles, const qmb_cores* cores, const qmb_
     SingleParams p{};
     p.a = reinterpret_cast<const double2*>(singles->a);
     p.kd = reinterpret_cast<const double2*>(singles->kd);
+    p.a_re = singles->real ? singles->a_re : nullptr;
+    p.kd_re = singles->real ? singles->kd_re : nullptr;
     p.n_bits = singles->n_bits;
     p.particle_cut = singles->particle_cut;
     p.ckeys = targets->keys;
@@ -1562,6 +1605,8 @@ int batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t by
             d.has_singles = 1;
             d.sa = reinterpret_cast<const double2*>(x.singles->a);
             d.skd = reinterpret_cast<const double2*>(x.singles->kd);
+            d.sa_re = x.singles->real ? x.singles->a_re : nullptr;
+            d.skd_re = x.singles->real ? x.singles->kd_re : nullptr;
             d.svals = x.cores1->vals;
             d.sstarts = x.cores1->starts;
             d.scounts = x.cores1->counts;

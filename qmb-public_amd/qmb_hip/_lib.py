@@ -91,12 +91,14 @@ class QmbSingles(ctypes.Structure):
         ("n_qubytes", _c_i32),
         ("particle_cut", _c_i32),
         ("n_bits", _c_i32),
-        ("reserved", _c_i32),
+        ("real", _c_i32),
         ("buffer", _vp),
         ("bytes", _c_size),
         ("a", _vp),
         ("kd", _vp),
         ("counts", _vp),
+        ("a_re", _vp),
+        ("kd_re", _vp),
     ]
 
 
@@ -197,6 +199,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "qmb_pairs_workspace_size": (_c_int, [_c_int, ctypes.POINTER(_c_size)]),
     "qmb_pairs_build": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _c_i64, _vp, _c_size, ctypes.POINTER(QmbPairs), _vp]),
     "qmb_pairs_set_real": (_c_int, [ctypes.POINTER(QmbPairs), _c_i32, _vp]),
+    "qmb_singles_set_real": (_c_int, [ctypes.POINTER(QmbSingles), _c_i32, _vp]),
     "qmb_singles_workspace_size": (_c_int, [_c_int, ctypes.POINTER(_c_size)]),
     "qmb_singles_build": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _c_i64, _vp, _c_size, ctypes.POINTER(QmbSingles), _vp]),
     "qmb_terms_build_residual": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _c_i64, _c_i32, _vp, _c_size, ctypes.POINTER(QmbTerms), _vp]),

@@ -120,6 +120,10 @@ class PreparedTerms:
                     _lib.check(lib.qmb_singles_build(n_qubytes, particle_cut, _ptr(site), _ptr(kind), _ptr(coef), self.n_terms,
                                                      _ptr(sbuf), sbuf.numel(), ctypes.byref(sdesc), _stream(device)), "qmb_singles_build")
                 self.n_single_terms = _device_count(sbuf, sdesc.counts)
+                if self.n_single_terms > 0 and pdesc.real:
+                    # real table: the single-excitation kernels read 8-byte coefficients too
+                    with torch.cuda.device(device):
+                        _lib.check(lib.qmb_singles_set_real(ctypes.byref(sdesc), 1, _stream(device)), "qmb_singles_set_real")
                 exclude = EXCLUDE_PAIRS | EXCLUDE_DIAGONAL
                 if self.n_single_terms > 0:
                     self.singles_buffer, self.singles_desc = sbuf, sdesc
