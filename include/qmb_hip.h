@@ -278,13 +278,14 @@ int qmb_terms_build_residual(int n_qubytes, int particle_cut, const int16_t* sit
  * of a wrong result). */
 typedef struct qmb_cores {
     int32_t n_qubytes, n_words, max_occupied, per_config; /* per_config = C(max_occupied, order) */
-    int32_t order, reserved;
+    int32_t order, key_bits;  /* key_bits: width of the sorted keys (set by qmb_cores_build) */
     int64_t n;
     int64_t capacity;  /* n * per_config entries */
     void* buffer;
     size_t bytes;
-    uint64_t* keys;    /* [capacity] sorted keys: (even sites of the core mod 3) << 54 | 34-bit core hash << 20 |
-                          removed pair slot (order 1: removed site); unused entries last */
+    uint64_t* keys;    /* [capacity] sorted keys: (even sites of the core mod 3) << (34 + s) | 34-bit core hash << s |
+                          removed pair slot (order 1: removed site) in the low s = key_bits - 36 bits (the
+                          fewest that hold n_pairs, or n_bits); unused entries last (all key_bits ones) */
     uint64_t* vals;    /* [capacity] row << 16 | site1 << 8 | site0 of the removed sites (order 1: site0) */
     uint32_t* starts;  /* [capacity + 1] first entry of each group, then the end */
     int64_t* counts;   /* device: {entries, groups, overflow, 0} */

@@ -27,10 +27,20 @@ constexpr int kBlock = 256;
 // numbers of even and odd occupied sites (spin sectors) it tells the pair class of all the group's
 // entries, so groups come in class order -- waves running side by side read one block of the
 // coefficient table -- and inside a group the entries ascend by slot, reading table rows in order.
-constexpr int kKeyBits = 56;
-constexpr int kSubBits = 20;
+constexpr int kKeyBits = 56;   // the widest key: 2 + kHashBits + 20 sub bits
 constexpr int kHashBits = 34;
-constexpr uint64_t kKeySentinel = (1ull << kKeyBits) - 1ull;  // unused entries; sorts after every real key
+// The sub field holds the removed pair's slot (order 2, < n_pairs) or the removed site (order 1,
+// < n_bits), in as few bits as that needs: the radix sort then runs over 2 + 34 + sub_bits bits
+// (47 at 56 sites instead of 56), one pass fewer.
+inline int core_sub_bits(int n_bits, int order) {
+    const int64_t range = order == 2 ? static_cast<int64_t>(n_bits) * (n_bits - 1) / 2 : n_bits;
+    int b = 1;
+    while ((1ll << b) < range) {
+        ++b;
+    }
+    return b;
+}
+__host__ __device__ __forceinline__ uint64_t key_sentinel(int key_bits) { return (1ull << key_bits) - 1ull; }  // sorts last
 constexpr int kMaxPairBytes = 16;            // the dense table is n_pairs^2 entries
 
 inline unsigned blocks_for(int64_t n, int block = kBlock) {
@@ -199,21 +209,21 @@ __device__ __forceinline__ void flip_site(Cfg<W>& c, int s) {
     }
 }
 
-__device__ __forceinline__ uint64_t core_key(uint64_t h, uint64_t cls, uint64_t sub) {
-    constexpr uint64_t kTop = (1ull << (kKeyBits - kSubBits)) - 1ull;
+__device__ __forceinline__ uint64_t core_key(uint64_t h, uint64_t cls, uint64_t sub, int sub_bits) {
+    constexpr uint64_t kTop = (1ull << (2 + kHashBits)) - 1ull;
     uint64_t top = (cls << kHashBits) | (h & ((1ull << kHashBits) - 1ull));
     top = top == kTop ? top - 1 : top;  // a merged hash value only makes its group "mixed"
-    return (top << kSubBits) | sub;
+    return (top << sub_bits) | sub;
 }
 
-__device__ __forceinline__ uint64_t group_of(uint64_t key) { return key >> kSubBits; }
+__device__ __forceinline__ uint64_t group_of(uint64_t key, int sub_bits) { return key >> sub_bits; }
 
 // One thread per config: every occupied pair (x < y) -- ORDER 2 -- or every occupied site x --
 // ORDER 1 -- gives the entry (hash of the config without those sites, row); unused slots of a row
 // with fewer occupied sites get the sentinel key.
 template <int W, int ORDER>
 __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64_t* __restrict__ cols, int64_t n, int n_bits,
-                                 int per_config, uint64_t* __restrict__ okeys, uint64_t* __restrict__ ovals,
+                                 int per_config, int sub_bits, uint64_t* __restrict__ okeys, uint64_t* __restrict__ ovals,
                                  int64_t* __restrict__ counts) {
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i >= n) {
@@ -238,7 +248,7 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
             const uint64_t hx = h ^ cols[x];
             if (ORDER == 1) {
                 if (k < per_config) {
-                    okeys[k * n + i] = core_key(hx, 0, static_cast<uint64_t>(x));
+                    okeys[k * n + i] = core_key(hx, 0, static_cast<uint64_t>(x), sub_bits);
                     ovals[k * n + i] = (static_cast<uint64_t>(i) << 16) | static_cast<uint64_t>(x);
                 }
                 ++k;
@@ -256,7 +266,7 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
                     if (k < per_config) {
                         const int core_even = even - ((x & 1) ? 0 : 1) - ((y & 1) ? 0 : 1);
                         okeys[k * n + i] = core_key(hx ^ cols[y], static_cast<uint64_t>(core_even % 3),
-                                                   static_cast<uint64_t>(pair_slot(x, y, n_bits)));
+                                                   static_cast<uint64_t>(pair_slot(x, y, n_bits)), sub_bits);
                         ovals[k * n + i] = (static_cast<uint64_t>(i) << 16) | (static_cast<uint64_t>(y) << 8) |
                                           static_cast<uint64_t>(x);
                     }
@@ -269,36 +279,37 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
         atomicAdd(reinterpret_cast<unsigned long long*>(&counts[2]), 1ull);
     }
     for (; k < per_config; ++k) {
-        okeys[k * n + i] = kKeySentinel;
+        okeys[k * n + i] = key_sentinel(2 + kHashBits + sub_bits);
         ovals[k * n + i] = 0;
     }
 }
 
-__global__ void core_flag_kernel(const uint64_t* __restrict__ keys, int64_t cap, uint32_t* __restrict__ flag) {
+__global__ void core_flag_kernel(const uint64_t* __restrict__ keys, int64_t cap, int sub_bits, uint32_t* __restrict__ flag) {
     const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (e >= cap) {
         return;
     }
     const uint64_t k = keys[e];
-    flag[e] = (k != kKeySentinel && (e == 0 || group_of(keys[e - 1]) != group_of(k))) ? 1u : 0u;
+    flag[e] = (k != key_sentinel(2 + kHashBits + sub_bits) && (e == 0 || group_of(keys[e - 1], sub_bits) != group_of(k, sub_bits))) ? 1u
+                                                                                                                                   : 0u;
 }
 
 // starts[g] = first entry of group g; starts[G] = number of used entries; counts {entries, G}.
 __global__ void core_start_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flag,
-                                  const uint32_t* __restrict__ gid, int64_t cap, uint32_t* __restrict__ starts,
+                                  const uint32_t* __restrict__ gid, int64_t cap, uint64_t sentinel, uint32_t* __restrict__ starts,
                                   int64_t* __restrict__ counts) {
     const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (e >= cap) {
         return;
     }
     const uint64_t k = keys[e];
-    if (k == kKeySentinel) {
+    if (k == sentinel) {
         return;
     }
     if (flag[e]) {
         starts[gid[e]] = static_cast<uint32_t>(e);
     }
-    if (e == cap - 1 || keys[e + 1] == kKeySentinel) {
+    if (e == cap - 1 || keys[e + 1] == sentinel) {
         const uint32_t groups = gid[e] + flag[e];
         starts[groups] = static_cast<uint32_t>(e + 1);
         counts[0] = e + 1;
@@ -1352,10 +1363,12 @@ int cores_build(const qmb_targets* targets, int max_occupied, int order, void* b
         uint32_t* gid = sc.take<uint32_t>(cap);
         size_t tmp_bytes = scratch_bytes > sc.off + 512 ? scratch_bytes - sc.off - 512 : 0;
         void* tmp = sc.take_bytes(tmp_bytes);
+        const int sub_bits = core_sub_bits(8 * c.n_qubytes, order);
+        const int key_bits = 2 + kHashBits + sub_bits;
         const unsigned nb = blocks_for(c.n);
         auto emit = [&](auto kernel) {
-            kernel<<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, 8 * c.n_qubytes, c.per_config, keys_in, vals_in,
-                                              c.counts);
+            kernel<<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, 8 * c.n_qubytes, c.per_config, sub_bits, keys_in,
+                                              vals_in, c.counts);
         };
         switch (c.n_words * 2 + (order - 1)) {
             case 2: emit(&core_emit_kernel<1, 1>); break;
@@ -1369,13 +1382,15 @@ int cores_build(const qmb_targets* targets, int max_occupied, int order, void* b
         }
         QMB_CHECK_LAUNCH();
         size_t need = tmp_bytes;
-        QMB_HIP_TRY(rocprim::radix_sort_pairs(tmp, need, keys_in, c.keys, vals_in, c.vals, static_cast<unsigned>(cap), 0, kKeyBits,
+        QMB_HIP_TRY(rocprim::radix_sort_pairs(tmp, need, keys_in, c.keys, vals_in, c.vals, static_cast<unsigned>(cap), 0, key_bits,
                                               stream));
-        core_flag_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, c.capacity, flag);
+        core_flag_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, c.capacity, sub_bits, flag);
         QMB_CHECK_LAUNCH();
         need = tmp_bytes;
         QMB_HIP_TRY(rocprim::exclusive_scan(tmp, need, flag, gid, 0u, cap, rocprim::plus<uint32_t>(), stream));
-        core_start_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, flag, gid, c.capacity, c.starts, c.counts);
+        core_start_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, flag, gid, c.capacity, key_sentinel(key_bits), c.starts,
+                                                                           c.counts);
+        c.key_bits = key_bits;
         QMB_CHECK_LAUNCH();
     }
     *out = c;

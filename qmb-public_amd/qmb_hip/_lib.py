@@ -111,7 +111,7 @@ class QmbCores(ctypes.Structure):
         ("max_occupied", _c_i32),
         ("per_config", _c_i32),
         ("order", _c_i32),
-        ("reserved", _c_i32),
+        ("key_bits", _c_i32),
         ("n", _c_i64),
         ("capacity", _c_i64),
         ("buffer", _vp),
