@@ -32,6 +32,7 @@ struct ApplyParams {
     const uint64_t* cols;
     int tile_log2;
     int groups;
+    int self_rows;  // symmetric launches: configs_i is the indexed set, rows distinct
 };
 
 // out[j] += sign * coef * in[i]: scatter with hardware FP64 atomics.
@@ -166,7 +167,7 @@ struct CountAction : ActionHooks {
 template <int W, int MODE, int DIRECT>
 __global__ __launch_bounds__(kBlock) void apply_kernel(ApplyParams p) {
     extern __shared__ uint64_t smem[];
-    const HashIndex ix{p.table, p.tkeys, p.mask, p.filter, p.fmask, p.cols};
+    const HashIndex ix{p.table, p.tkeys, p.mask, p.filter, p.fmask, p.cols, p.self_rows != 0};
     auto run = [&](auto& act) {
         if (DIRECT) {
             enumerate_direct<W>(p.src, ix, blockIdx.x, *reinterpret_cast<DirectLds<W>*>(smem), act);
@@ -479,6 +480,7 @@ int apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const uin
     p.cols = targets->cols;
     p.tile_log2 = ls.tile_log2;
     p.groups = ls.groups;
+    p.self_rows = ls.symmetric != 0 ? 1 : 0;
     int64_t grid;
     size_t lds;
     if (ls.direct) {

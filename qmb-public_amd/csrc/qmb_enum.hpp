@@ -98,12 +98,17 @@ struct HashIndex {
     const uint64_t* filter;
     uint64_t fmask;
     const uint64_t* cols;
+    // The index was built from the source configs themselves, rows distinct: a term that flips no
+    // site targets the source's own row, which is answered without a lookup.
+    bool self_rows;
 };
 
 // Batched lookups in phases, so a thread keeps U independent loads in flight at each step instead
 // of one dependent chain per connection: the filter words of every wanted target, then the first
 // table slot of every target that passed the filter, then the stored keys of every slot whose
 // fingerprint matched.  The rare longer probe sequences (load factor <= 1/4) finish one by one.
+// stay[u]: the term flips no site and the index holds the source rows (ix.self_rows): the target is
+// row i[u] itself and nothing is loaded.
 // Action protocol (hs, ht: index hashes of source and target; ht == hs iff the term flips nothing):
 //   bool pre(i, t, target, parity, hs, ht)       -> true if the target must be looked up
 //   void post(i, t, target, parity, j, distinct) -> j = row of the target in the index, or -1;
@@ -111,17 +116,17 @@ struct HashIndex {
 template <int W, int U, typename Action>
 __device__ __forceinline__ void lookup_batch(const HashIndex& ix, Action& act, const int64_t (&i)[U], const uint32_t (&t)[U],
                                              const Cfg<W> (&o)[U], const int (&par)[U], const bool (&want)[U],
-                                             const uint64_t (&h)[U], const uint64_t (&hs)[U]) {
+                                             const bool (&stay)[U], const uint64_t (&h)[U], const uint64_t (&hs)[U]) {
     uint64_t e[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        e[u] = want[u] ? ix.filter[filter_word(h[u], ix.fmask)] : 0ull;
+        e[u] = (want[u] && !stay[u]) ? ix.filter[filter_word(h[u], ix.fmask)] : 0ull;
     }
     // e[u]: the first table slot of a target that passed the filter, else kEmptySlot
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const uint64_t bits = filter_bits(h[u]);
-        const bool pass = want[u] && (e[u] & bits) == bits;
+        const bool pass = want[u] && !stay[u] && (e[u] & bits) == bits;
         e[u] = pass ? ix.table[(h[u] >> 32) & ix.mask] : kEmptySlot;
     }
     uint64_t key[U][W];
@@ -143,7 +148,9 @@ __device__ __forceinline__ void lookup_batch(const HashIndex& ix, Action& act, c
             eq = eq && key[u][k] == o[u].w[k];
         }
         j[u] = -1;
-        if (eq) {
+        if (stay[u]) {
+            j[u] = i[u];
+        } else if (eq) {
             j[u] = static_cast<int64_t>(e[u] >> 32);
         } else if (e[u] != kEmptySlot) {
             // the first slot held another key: continue the linear probe from the next slot
@@ -371,12 +378,18 @@ __device__ void enumerate_tile(const EnumSource& p, const HashIndex& ix, int64_t
         const int ty = tid >> lg;
         for (uint32_t t = ts + tx; t < te; t += TX) {
             const Masks<W> x = expand_rec<W>(p.recs[t], ix.cols);
+            bool no_flip = true;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                no_flip = no_flip && x.f[w] == 0ull;
+            }
             for (int q0 = ty; q0 < mcount; q0 += TY * kBatch) {
                 int64_t ii[kBatch];
                 uint32_t tt[kBatch];
                 Cfg<W> o[kBatch];
                 int par[kBatch];
                 bool want[kBatch];
+                bool stay[kBatch];
                 uint64_t h[kBatch];
                 uint64_t hs[kBatch];
 #pragma unroll
@@ -395,8 +408,9 @@ __device__ void enumerate_tile(const EnumSource& p, const HashIndex& ix, int64_t
                     const bool ok = apply_masks<W>(x, c, o[u], par[u]) && in;
                     ii[u] = ok ? i0 + L.mem[qq] : i0;
                     want[u] = ok && act.pre(ii[u], t, o[u], par[u], hs[u], h[u]);
+                    stay[u] = ix.self_rows && no_flip;
                 }
-                lookup_batch<W, kBatch>(ix, act, ii, tt, o, par, want, h, hs);
+                lookup_batch<W, kBatch>(ix, act, ii, tt, o, par, want, stay, h, hs);
             }
         }
         __syncthreads();
@@ -474,6 +488,7 @@ __device__ void enumerate_direct(const EnumSource& p, const HashIndex& ix, int64
                     Cfg<W> o[kDirectBatch];
                     int par[kDirectBatch];
                     bool want[kDirectBatch];
+                    bool stay[kDirectBatch];
                     uint64_t h[kDirectBatch];
                     uint64_t hs[kDirectBatch];
 #pragma unroll
@@ -482,11 +497,14 @@ __device__ void enumerate_direct(const EnumSource& p, const HashIndex& ix, int64
                         const int u = t0 + (have ? __builtin_ctz(todo) : 0);
                         todo = have ? todo & (todo - 1u) : 0u;
                         int pr = L.pc[u];
+                        bool no_flip = true;
 #pragma unroll
                         for (int w = 0; w < W; ++w) {
                             o[b].w[w] = c.w[w] ^ L.f[u * W + w];
                             pr ^= __popcll(c.w[w] & L.p[u * W + w]) & 1;
+                            no_flip = no_flip && L.f[u * W + w] == 0ull;
                         }
+                        stay[b] = ix.self_rows && no_flip;
                         par[b] = pr;
                         ii[b] = i;
                         tt[b] = static_cast<uint32_t>(base + u);
@@ -494,7 +512,7 @@ __device__ void enumerate_direct(const EnumSource& p, const HashIndex& ix, int64
                         h[b] = hc ^ L.hf[u];
                         want[b] = have && act.pre(i, tt[b], o[b], pr, hc, h[b]);
                     }
-                    lookup_batch<W, kDirectBatch>(ix, act, ii, tt, o, par, want, h, hs);
+                    lookup_batch<W, kDirectBatch>(ix, act, ii, tt, o, par, want, stay, h, hs);
                 }
             }
         }

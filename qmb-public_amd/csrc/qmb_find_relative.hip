@@ -194,7 +194,7 @@ struct CollectAction : ActionHooks {
 
 template <int W, typename Action>
 __device__ __forceinline__ void fr_enumerate(const FrParams& p, uint64_t* smem, Action& act) {
-    const HashIndex ix{p.xtable, p.xkeys, p.xmask, p.xfilter, p.xfmask, p.xcols};
+    const HashIndex ix{p.xtable, p.xkeys, p.xmask, p.xfilter, p.xfmask, p.xcols, false};
     if (p.direct) {
         enumerate_direct<W>(p.src, ix, blockIdx.x, *reinterpret_cast<DirectLds<W>*>(smem), act);
     } else {
