@@ -421,8 +421,9 @@ __device__ void enumerate_tile(const EnumSource& p, const HashIndex& ix, int64_t
 // Direct enumeration for small term tables: each thread owns one config in registers and walks
 // every term, the terms staged through LDS (expanded to masks) in chunks shared by the workgroup.
 constexpr int kDirectChunk = 256;
-// Lookups kept in flight per thread by the direct enumeration.
-constexpr int kDirectBatch = 4;
+// Lookups kept in flight per thread by the direct enumeration: 2 keeps the kernel at 60 VGPRs and 8
+// waves per SIMD, which hides more latency than 3 (74 VGPRs, 6 waves) or 4 (92 VGPRs, 5 waves).
+constexpr int kDirectBatch = 2;
 
 template <int W>
 struct DirectLds {
