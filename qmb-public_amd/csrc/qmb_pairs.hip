@@ -826,6 +826,39 @@ __device__ __forceinline__ int single_parity(const Cfg<W>& c, int r, int p) {
     return masked_parity<W>(c, range_mask<W>(r < p ? r : p, r < p ? p : r));
 }
 
+// Sum of row[k] over the set bits k of c, ascending k.  Up to 8 indices are taken from the mask before
+// their loads are issued, so the loads of one element overlap instead of each waiting for the last
+// add (same order of additions, same result as a bit-by-bit walk).  row must hold at least
+// 64 * (W - 1) + 1 entries (a lane without an index reads row[64 w] and drops it).
+__device__ __forceinline__ double zero_like(double) { return 0.0; }
+__device__ __forceinline__ double2 zero_like(double2) { return make_double2(0.0, 0.0); }
+__device__ __forceinline__ double add_to(double a, double b) { return a + b; }
+__device__ __forceinline__ double2 add_to(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+
+template <int W, typename T>
+__device__ __forceinline__ T masked_row_sum(const T* __restrict__ row, const Cfg<W>& c, T acc) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        uint64_t m = c.w[w];
+        while (m) {
+            T v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const bool have = m != 0ull;
+                const int k = 64 * w + (have ? __builtin_ctzll(m) : 0);
+                m = have ? m & (m - 1ull) : 0ull;
+                const T x = row[k];
+                v[u] = have ? x : zero_like(x);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                acc = add_to(acc, v[u]);
+            }
+        }
+    }
+    return acc;
+}
+
 // Element of the single excitation (annihilate r, create p) on a source config whose other
 // occupied sites are `core`.
 template <int W>
@@ -833,33 +866,9 @@ __device__ __forceinline__ double2 single_element(const SingleParams& p, const C
     const int64_t rq = static_cast<int64_t>(r) * p.n_bits + q;
     if (p.kd_re) {
         // real table: 8-byte reads, the sum kept in one register
-        double xr = p.a_re[rq];
-        const double* row_re = p.kd_re + rq * p.n_bits;
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-            uint64_t m = core.w[w];
-            while (m) {
-                const int k = 64 * w + __builtin_ctzll(m);
-                m &= m - 1;
-                xr += row_re[k];
-            }
-        }
-        return make_double2(xr, 0.0);
+        return make_double2(masked_row_sum<W, double>(p.kd_re + rq * p.n_bits, core, p.a_re[rq]), 0.0);
     }
-    double2 x = p.a[rq];
-    const double2* row = p.kd + rq * p.n_bits;
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-        uint64_t m = core.w[w];
-        while (m) {
-            const int k = 64 * w + __builtin_ctzll(m);
-            m &= m - 1;
-            const double2 d = row[k];
-            x.x += d.x;
-            x.y += d.y;
-        }
-    }
-    return x;
+    return masked_row_sum<W, double2>(p.kd + rq * p.n_bits, core, p.a[rq]);
 }
 
 // One wave, one group of order-1 entries [s, e): rows a (config = core + r) against entries b
