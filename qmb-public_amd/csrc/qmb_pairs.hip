@@ -641,6 +641,20 @@ __device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_
         const int64_t ia = static_cast<int64_t>(va >> 16);
         const int r0 = static_cast<int>(va & 0xff), r1 = static_cast<int>((va >> 8) & 0xff);
         const int q0 = static_cast<int>(vb & 0xff), q1 = static_cast<int>((vb >> 8) & 0xff);
+        // Everything that needs only the entries: the coefficient and the input row are loaded now,
+        // beside the stored configs, and dropped below when the cores differ (entry 0 of the
+        // table and row 0 of the input are always there).
+        const bool pre = rowa && rowb && ai != bi && q0 != r0 && q0 != r1 && q1 != r0 && q1 != r1;
+        const int64_t idx = !pre ? 0
+                                 : (ADJ ? pair_slot(r0, r1, p.n_bits) * p.n_pairs + pair_slot(q0, q1, p.n_bits)
+                                        : pair_slot(q0, q1, p.n_bits) * p.n_pairs + pair_slot(r0, r1, p.n_bits));
+        const double2 x = p.coef_re ? make_double2(p.coef_re[idx], 0.0) : p.coef[idx];
+        const int64_t ib = pre ? static_cast<int64_t>(vb >> 16) : 0;
+        double2 gv[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            gv[v] = (v == 0 ? p.in : p.in2)[ib];
+        }
         Cfg<W> ca;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
@@ -658,33 +672,28 @@ __device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_
         for (int w = 0; w < W; ++w) {
             same = same && shfl_u64(core.w[w], src) == core.w[w];
         }
-        const bool ok = rowa && rowb && ai != bi && same && q0 != r0 && q0 != r1 && q1 != r0 && q1 != r1;
+        const bool ok = pre && same;
         double acc_re[NV] = {}, acc_im[NV] = {};
         if (ok) {
-            int64_t idx;
             int sg;
             if (ADJ) {
-                idx = pair_slot(r0, r1, p.n_bits) * p.n_pairs + pair_slot(q0, q1, p.n_bits);
                 sg = fermi ? pair_parity<W>(ca, r0, r1, q0, q1) : 0;
             } else {
                 Cfg<W> cb = core;
                 flip_site<W>(cb, q0);
                 flip_site<W>(cb, q1);
-                idx = pair_slot(q0, q1, p.n_bits) * p.n_pairs + pair_slot(r0, r1, p.n_bits);
                 sg = fermi ? pair_parity<W>(cb, r0, r1, q0, q1) : 0;
             }
-            const double2 x = p.coef_re ? make_double2(p.coef_re[idx], 0.0) : p.coef[idx];
             const double xr = sg ? -x.x : x.x;
             const double xi = sg ? -x.y : x.y;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                const double2 gv = (v == 0 ? p.in : p.in2)[vb >> 16];
                 if (ADJ) {
-                    acc_re[v] = xr * gv.x + xi * gv.y;
-                    acc_im[v] = xr * gv.y - xi * gv.x;
+                    acc_re[v] = xr * gv[v].x + xi * gv[v].y;
+                    acc_im[v] = xr * gv[v].y - xi * gv[v].x;
                 } else {
-                    acc_re[v] = xr * gv.x - xi * gv.y;
-                    acc_im[v] = xr * gv.y + xi * gv.x;
+                    acc_re[v] = xr * gv[v].x - xi * gv[v].y;
+                    acc_im[v] = xr * gv[v].y + xi * gv[v].x;
                 }
             }
         }
