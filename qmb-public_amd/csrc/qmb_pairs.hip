@@ -998,6 +998,14 @@ __device__ void single_tiny_class(const SingleParams& p, uint64_t mask, uint32_t
         const int64_t ia = static_cast<int64_t>(va >> 16);
         const int r = static_cast<int>(va & 0xff);
         const int q = static_cast<int>(vb & 0xff);
+        // the input row needs only the entry: loaded beside the stored configs (row 0 when unused)
+        const bool pre = rowa && rowb && ai != bi && q != r;
+        const int64_t ib = pre ? static_cast<int64_t>(vb >> 16) : 0;
+        double2 gv[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            gv[v] = (v == 0 ? p.in : p.in2)[ib];
+        }
         Cfg<W> ca;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
@@ -1013,7 +1021,7 @@ __device__ void single_tiny_class(const SingleParams& p, uint64_t mask, uint32_t
         for (int w = 0; w < W; ++w) {
             same = same && shfl_u64(core.w[w], src) == core.w[w];
         }
-        const bool ok = rowa && rowb && ai != bi && same && q != r;
+        const bool ok = pre && same;
         double acc_re[NV] = {}, acc_im[NV] = {};
         if (ok) {
             double2 x;
@@ -1031,13 +1039,12 @@ __device__ void single_tiny_class(const SingleParams& p, uint64_t mask, uint32_t
             const double xi = sg ? -x.y : x.y;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                const double2 gv = (v == 0 ? p.in : p.in2)[vb >> 16];
                 if (ADJ) {
-                    acc_re[v] = xr * gv.x + xi * gv.y;
-                    acc_im[v] = xr * gv.y - xi * gv.x;
+                    acc_re[v] = xr * gv[v].x + xi * gv[v].y;
+                    acc_im[v] = xr * gv[v].y - xi * gv[v].x;
                 } else {
-                    acc_re[v] = xr * gv.x - xi * gv.y;
-                    acc_im[v] = xr * gv.y + xi * gv.x;
+                    acc_re[v] = xr * gv[v].x - xi * gv[v].y;
+                    acc_im[v] = xr * gv[v].y + xi * gv[v].x;
                 }
             }
         }
