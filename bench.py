@@ -737,8 +737,8 @@ def main() -> int:
                         help="also run a plain device-to-device copy of BYTES bytes three times (HBM counter calibration)")
     args = parser.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); sharding across GPUs is a design, not a "
-              "measured path, in this build.", file=sys.stderr)
+        print(f"bench.py measures one GPU (asked for {args.gpus}); the kernels run on one GPU per process and are not "
+              "sharded across GPUs.", file=sys.stderr)
         return 2
 
     import numpy as np  # pylint: disable=import-outside-toplevel
