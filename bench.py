@@ -3,10 +3,12 @@
 apply_within forward + adjoint throughput on one MI355X, in matrix elements per second.
 
 A step is one differentiable application of a seeded synthetic Hamiltonian to a seeded config set:
-build the hash index over configs_j, forward psi_j = H psi_i (scatter kernel), then the adjoint
-grad_i = H^H g for a random upstream gradient g (gather kernel) through torch.autograd.  The term
-table is compiled once outside the timed region (it belongs to the Hamiltonian, like the reference's
-one-off _prepare_data).  A matrix element is a contribution: a (config, term) pair whose application
+build the index over the set (hash index, plus the core indexes of the structured path when the
+table has pair excitations), forward psi_j = H psi_i, then the adjoint grad_i = H^H g for a random
+upstream gradient g through torch.autograd.  Molecular tables run the structured path (pair / single
+core groups, folded diagonal); tables without pair excitations run the enumeration with hash
+lookups, as a gather when the operator is Hermitian on its own set.  The term table is compiled once
+outside the timed region (it belongs to the Hamiltonian, like the reference's one-off _prepare_data).  A matrix element is a contribution: a (config, term) pair whose application
 succeeds and whose target is in configs_j; a step computes M_hit of them forward and M_hit backward.
 
 Workloads (--workload):
