@@ -103,17 +103,46 @@ struct HashIndex {
     bool self_rows;
 };
 
-// Batched lookups in phases, so a thread keeps U independent loads in flight at each step instead
-// of one dependent chain per connection: the filter words of every wanted target, then the first
-// table slot of every target that passed the filter, then the stored keys of every slot whose
-// fingerprint matched.  The rare longer probe sequences (load factor <= 1/4) finish one by one.
-// stay[u]: the term flips no site and the index holds the source rows (ix.self_rows): the target is
-// row i[u] itself and nothing is loaded.
-// Action protocol (hs, ht: index hashes of source and target; ht == hs iff the term flips nothing):
+// Linear probe for target o (index hash h) from table slot `slot` until its key or an empty slot.
+// Bounded by the capacity: the table is at most a quarter full.
+template <int W>
+__device__ __forceinline__ int64_t lookup_from(const HashIndex& ix, const Cfg<W>& o, uint64_t h, uint64_t slot) {
+    const uint32_t fp = static_cast<uint32_t>(h);
+    for (uint64_t probe = 0; probe <= ix.mask; ++probe) {
+        const uint64_t ee = ix.table[slot];
+        if (ee == kEmptySlot) {
+            return -1;
+        }
+        if (static_cast<uint32_t>(ee) == fp) {
+            const uint64_t jj = ee >> 32;
+            bool same = true;
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                same = same && (ix.keys[jj * W + k] == o.w[k]);
+            }
+            if (same) {
+                return static_cast<int64_t>(jj);
+            }
+        }
+        slot = (slot + 1) & ix.mask;
+    }
+    return -1;
+}
+
+// Batched lookups of U targets.  Action protocol (hs, ht: index hashes of source and target; ht == hs
+// iff the term flips nothing):
 //   bool pre(i, t, target, parity, hs, ht)       -> true if the target must be looked up
 //   void post(i, t, target, parity, j, distinct) -> j = row of the target in the index, or -1;
 //                                                   distinct = (ht != hs)
-template <int W, int U, typename Action>
+// stay[u]: the term flips no site and the index holds the source rows (ix.self_rows): the target is
+// row i[u] itself and nothing is loaded.
+// The filter words of all U targets are loaded first.  PHASED then also loads the first table slot
+// of every target that passed, then the stored keys of every slot whose fingerprint matched, and
+// calls post() last; otherwise each target's probe chain and post() run in turn.  The direct
+// enumeration (few lookups per term row, 2 in flight at 8 waves) gains from the phases (hubbard128
+// forward 5.25 -> 4.2 ms with them); the tile enumeration, whose posts do more work between lookups,
+// loses (find_relative at 1M configs 73.6 -> 90.7 ms), so it keeps the chains.
+template <int W, int U, bool PHASED, typename Action>
 __device__ __forceinline__ void lookup_batch(const HashIndex& ix, Action& act, const int64_t (&i)[U], const uint32_t (&t)[U],
                                              const Cfg<W> (&o)[U], const int (&par)[U], const bool (&want)[U],
                                              const bool (&stay)[U], const uint64_t (&h)[U], const uint64_t (&hs)[U]) {
@@ -121,6 +150,23 @@ __device__ __forceinline__ void lookup_batch(const HashIndex& ix, Action& act, c
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         e[u] = (want[u] && !stay[u]) ? ix.filter[filter_word(h[u], ix.fmask)] : 0ull;
+    }
+    if (!PHASED) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (!want[u]) {
+                continue;
+            }
+            int64_t j = -1;
+            const uint64_t bits = filter_bits(h[u]);
+            if (stay[u]) {
+                j = i[u];
+            } else if ((e[u] & bits) == bits) {
+                j = lookup_from<W>(ix, o[u], h[u], (h[u] >> 32) & ix.mask);
+            }
+            act.post(i[u], t[u], o[u], par[u], j, h[u] != hs[u]);
+        }
+        return;
     }
     // e[u]: the first table slot of a target that passed the filter, else kEmptySlot
 #pragma unroll
@@ -154,27 +200,7 @@ __device__ __forceinline__ void lookup_batch(const HashIndex& ix, Action& act, c
             j[u] = static_cast<int64_t>(e[u] >> 32);
         } else if (e[u] != kEmptySlot) {
             // the first slot held another key: continue the linear probe from the next slot
-            const uint32_t fp = static_cast<uint32_t>(h[u]);
-            uint64_t s = (((h[u] >> 32) & ix.mask) + 1) & ix.mask;
-            for (uint64_t probe = 1; probe <= ix.mask; ++probe) {
-                const uint64_t ee = ix.table[s];
-                if (ee == kEmptySlot) {
-                    break;
-                }
-                if (static_cast<uint32_t>(ee) == fp) {
-                    const uint64_t jj = ee >> 32;
-                    bool same = true;
-#pragma unroll
-                    for (int k = 0; k < W; ++k) {
-                        same = same && (ix.keys[jj * W + k] == o[u].w[k]);
-                    }
-                    if (same) {
-                        j[u] = static_cast<int64_t>(jj);
-                        break;
-                    }
-                }
-                s = (s + 1) & ix.mask;
-            }
+            j[u] = lookup_from<W>(ix, o[u], h[u], (((h[u] >> 32) & ix.mask) + 1) & ix.mask);
         }
     }
 #pragma unroll
@@ -410,7 +436,7 @@ __device__ void enumerate_tile(const EnumSource& p, const HashIndex& ix, int64_t
                     want[u] = ok && act.pre(ii[u], t, o[u], par[u], hs[u], h[u]);
                     stay[u] = ix.self_rows && no_flip;
                 }
-                lookup_batch<W, kBatch>(ix, act, ii, tt, o, par, want, stay, h, hs);
+                lookup_batch<W, kBatch, false>(ix, act, ii, tt, o, par, want, stay, h, hs);
             }
         }
         __syncthreads();
@@ -513,7 +539,7 @@ __device__ void enumerate_direct(const EnumSource& p, const HashIndex& ix, int64
                         h[b] = hc ^ L.hf[u];
                         want[b] = have && act.pre(i, tt[b], o[b], pr, hc, h[b]);
                     }
-                    lookup_batch<W, kDirectBatch>(ix, act, ii, tt, o, par, want, stay, h, hs);
+                    lookup_batch<W, kDirectBatch, true>(ix, act, ii, tt, o, par, want, stay, h, hs);
                 }
             }
         }
