@@ -606,6 +606,149 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
     }
 }
 
+// Groups of 9..H entries (H = 16 or 32), 64 / H of them per pass: lane l serves row l % H of group
+// l / H.  A group's entries are its rows, so each lane stages its own entry in its LDS slot and the
+// rows of a group walk the slots of that group, as in pair_group.  Packing keeps the lanes busy
+// where one group per wave would leave most of them idle (a 20-entry group uses 20 of 64).
+template <int W, bool ADJ, int B, int NV, int H>
+__device__ void pair_group_packed(const PairParams& p, uint64_t mask, uint32_t gs_l, uint32_t ge_l, int lane, PairStage<W>* stage) {
+    constexpr int kPer = 64 / H;
+    const int gi = lane / H;
+    const int ri = lane % H;
+    const bool fermi = p.particle_cut == 1;
+    while (mask) {
+        uint32_t s = 0, e = 0;
+        int nmax = 0;
+        for (int j = 0; j < kPer && mask != 0ull; ++j) {
+            const int l = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const uint32_t sj = __builtin_amdgcn_readlane(gs_l, l);
+            const uint32_t ej = __builtin_amdgcn_readlane(ge_l, l);
+            nmax = static_cast<int>(ej - sj) > nmax ? static_cast<int>(ej - sj) : nmax;
+            if (gi == j) {
+                s = sj;
+                e = ej;
+            }
+        }
+        const int g = static_cast<int>(e - s);  // 0 for a slot without a group
+        const bool act = ri < g;
+        const uint32_t a = s + static_cast<uint32_t>(ri);
+        const uint64_t va = act ? p.vals[a] : 0ull;
+        const int64_t ia = static_cast<int64_t>(va >> 16);
+        const int r0 = static_cast<int>(va & 0xff);
+        const int r1 = static_cast<int>((va >> 8) & 0xff);
+        Cfg<W> ca;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            ca.w[w] = act ? p.ckeys[ia * W + w] : 0ull;
+        }
+        Cfg<W> core = ca;
+        if (act) {
+            flip_site<W>(core, r0);
+            flip_site<W>(core, r1);
+        }
+        // this lane's entry, staged for the rows of its group
+        {
+            PairStage<W> st;
+            st.row = static_cast<uint32_t>(ia);
+            st.g = act ? p.in[ia] : make_double2(0.0, 0.0);
+            st.g2 = (NV == 2 && act) ? p.in2[ia] : make_double2(0.0, 0.0);
+            const Cfg<W> mq = range_mask<W>(r0, r1);
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                st.m[w] = act ? mq.w[w] : 0ull;
+            }
+            st.packed = act ? (static_cast<uint32_t>(r0) | (static_cast<uint32_t>(r1) << 8) |
+                               (static_cast<uint32_t>(pair_slot(r0, r1, p.n_bits)) << 16))
+                            : 0u;
+            stage[lane] = st;
+        }
+        // Different cores with equal hashes share a group: then every pair is checked exactly.
+        bool same = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            same = same && shfl_u64(core.w[w], gi * H) == core.w[w];
+        }
+        const bool mixed = __ballot(act && !same) != 0ull;
+        const int64_t prow = pair_slot(r0, r1, p.n_bits);
+        const Cfg<W> mr = range_mask<W>(r0, r1);
+        const int base = fermi ? masked_parity<W>(ADJ ? ca : core, mr) : 0;
+        double acc_re[NV] = {}, acc_im[NV] = {};
+        __builtin_amdgcn_wave_barrier();
+        for (int u = 0; u < nmax; u += B) {
+            double2 x[B], gv[B], gv2[B];
+            int sg[B];
+            bool ok[B];
+#pragma unroll
+            for (int k = 0; k < B; ++k) {
+                const int uu = u + k;
+                const PairStage<W>& st = stage[gi * H + (uu & (H - 1))];
+                const uint32_t pk = st.packed;
+                const int q0 = static_cast<int>(pk & 0xff);
+                const int q1 = static_cast<int>((pk >> 8) & 0xff);
+                const int64_t pq = static_cast<int64_t>(pk >> 16);
+                gv[k] = st.g;
+                gv2[k] = NV == 2 ? st.g2 : make_double2(0.0, 0.0);
+                bool o = act && uu < g && uu != ri && q0 != r0 && q0 != r1 && q1 != r0 && q1 != r1;
+                if (mixed && o) {
+                    Cfg<W> cb = load_key<W>(p.ckeys, static_cast<int64_t>(st.row));
+                    flip_site<W>(cb, q0);
+                    flip_site<W>(cb, q1);
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        o = o && cb.w[w] == core.w[w];
+                    }
+                }
+                ok[k] = o;
+                Cfg<W> mq;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    mq.w[w] = st.m[w];
+                }
+                int64_t idx;
+                if (ADJ) {
+                    idx = prow * p.n_pairs + pq;
+                    sg[k] = fermi ? base ^ masked_parity<W>(ca, mq) : 0;
+                } else {
+                    // c_b = core + {q0, q1}: q0 lies in [q0, q1), q1 does not
+                    idx = pq * p.n_pairs + prow;
+                    const int in0 = (q0 >= r0 && q0 < r1) ? 1 : 0;
+                    const int in1 = (q1 >= r0 && q1 < r1) ? 1 : 0;
+                    sg[k] = fermi ? (base ^ masked_parity<W>(core, mq) ^ in0 ^ in1 ^ 1) : 0;
+                }
+                x[k] = !o ? make_double2(0.0, 0.0) : (p.coef_re ? make_double2(p.coef_re[idx], 0.0) : p.coef[idx]);
+            }
+#pragma unroll
+            for (int k = 0; k < B; ++k) {
+                if (ok[k]) {
+                    const double xr = sg[k] ? -x[k].x : x[k].x;
+                    const double xi = sg[k] ? -x[k].y : x[k].y;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        const double2 gg = v == 0 ? gv[k] : gv2[k];
+                        if (ADJ) {  // conj(x) * g
+                            acc_re[v] += xr * gg.x + xi * gg.y;
+                            acc_im[v] += xr * gg.y - xi * gg.x;
+                        } else {    // x * g
+                            acc_re[v] += xr * gg.x - xi * gg.y;
+                            acc_im[v] += xr * gg.y + xi * gg.x;
+                        }
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            if (act && (acc_re[v] != 0.0 || acc_im[v] != 0.0)) {
+                double2* o = v == 0 ? p.out : p.out2;
+                unsafeAtomicAdd(&o[ia].x, acc_re[v]);
+                unsafeAtomicAdd(&o[ia].y, acc_im[v]);
+            }
+        }
+    }
+}
+
 // Groups of 2..8 entries, one size class at a time (gp = 8, 4, 2 lanes per row): one lane per
 // (row a, entry b) of a gp x gp block, 64 / gp^2 groups per pass.  Each lane checks its pair exactly
 // (cores compared through a shuffle from the lane holding row b), the rows are summed over b with a
@@ -714,7 +857,7 @@ __device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_
 
 // The pair work of the groups [g0, g0 + 64): groups of up to 8 entries through the tiny-group
 // classes, larger ones a row block at a time (pair_group).
-template <int W, bool ADJ, int B, int NV>
+template <int W, bool ADJ, int B, int NV, bool PACK = true>
 __device__ __forceinline__ void pair_chunk(const PairParams& p, int64_t groups, int64_t g0, int lane, char* stage_bytes) {
     const int64_t gg = g0 + lane;
     uint32_t gs = 0, ge = 0;
@@ -726,9 +869,15 @@ __device__ __forceinline__ void pair_chunk(const PairParams& p, int64_t groups, 
     pair_tiny_class<W, ADJ, NV>(p, __ballot(sz > 4u && sz <= 8u), gs, ge, 8, lane);
     pair_tiny_class<W, ADJ, NV>(p, __ballot(sz > 2u && sz <= 4u), gs, ge, 4, lane);
     pair_tiny_class<W, ADJ, NV>(p, __ballot(sz == 2u), gs, ge, 2, lane);
-    uint64_t big = __ballot(sz > 8u);
     // each wave owns 64 slots of the largest stage size: waves of one block may run different W
     PairStage<W>* stage = reinterpret_cast<PairStage<W>*>(stage_bytes + (threadIdx.x >> 6) * 64 * sizeof(PairStage<2>));
+    // PACK: groups of 9..32 entries several to a wave (the batch kernel, at its register limit,
+    // keeps them one per wave)
+    if (PACK) {
+        pair_group_packed<W, ADJ, B, NV, 16>(p, __ballot(sz > 8u && sz <= 16u), gs, ge, lane, stage);
+        pair_group_packed<W, ADJ, B, NV, 32>(p, __ballot(sz > 16u && sz <= 32u), gs, ge, lane, stage);
+    }
+    uint64_t big = __ballot(sz > (PACK ? 32u : 8u));
     while (big) {
         const int l = __builtin_ctzll(big);
         big &= big - 1;
@@ -969,6 +1118,106 @@ __device__ void single_group(const SingleParams& p, uint32_t s, uint32_t e, int 
     }
 }
 
+// Single-excitation groups of 9..H entries (H = 16 or 32), 64 / H of them per pass: lane l serves
+// row l % H of group l / H; a group's entries are its rows, read from the lanes of the group by
+// shuffles (as single_group reads its staged entries), so several groups share a wave.
+template <int W, bool ADJ, int NV, int CH, int H>
+__device__ void single_group_packed(const SingleParams& p, uint64_t mask, uint32_t gs_l, uint32_t ge_l, int lane) {
+    constexpr int kPer = 64 / H;
+    const int gi = lane / H;
+    const int ri = lane % H;
+    const bool fermi = p.particle_cut == 1;
+    while (mask) {
+        uint32_t s = 0, e = 0;
+        int nmax = 0;
+        for (int j = 0; j < kPer && mask != 0ull; ++j) {
+            const int l = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const uint32_t sj = __builtin_amdgcn_readlane(gs_l, l);
+            const uint32_t ej = __builtin_amdgcn_readlane(ge_l, l);
+            nmax = static_cast<int>(ej - sj) > nmax ? static_cast<int>(ej - sj) : nmax;
+            if (gi == j) {
+                s = sj;
+                e = ej;
+            }
+        }
+        const int g = static_cast<int>(e - s);  // 0 for a slot without a group
+        const bool act = ri < g;
+        const uint64_t va = act ? p.vals[s + static_cast<uint32_t>(ri)] : 0ull;
+        const int64_t ia = static_cast<int64_t>(va >> 16);
+        const int r = static_cast<int>(va & 0xff);
+        Cfg<W> ca;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            ca.w[w] = act ? p.ckeys[ia * W + w] : 0ull;
+        }
+        Cfg<W> core = ca;
+        if (act) {
+            flip_site<W>(core, r);
+        }
+        bool same = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            same = same && shfl_u64(core.w[w], gi * H) == core.w[w];
+        }
+        const bool mixed = __ballot(act && !same) != 0ull;
+        const double2 gin = act ? p.in[ia] : make_double2(0.0, 0.0);
+        const double2 gin2 = (NV == 2 && act) ? p.in2[ia] : make_double2(0.0, 0.0);
+        double acc_re[NV] = {}, acc_im[NV] = {};
+        for (int u = 0; u < nmax; ++u) {
+            const int src = gi * H + (u & (H - 1));
+            const uint64_t vb = shfl_u64(va, src);
+            const double2 gb = shfl_d2(gin, src);
+            const double2 gb2 = NV == 2 ? shfl_d2(gin2, src) : make_double2(0.0, 0.0);
+            const int q = static_cast<int>(vb & 0xff);
+            bool ok = act && u < g && u != ri && q != r;
+            if (mixed && ok) {
+                Cfg<W> cb = load_key<W>(p.ckeys, static_cast<int64_t>(vb >> 16));
+                flip_site<W>(cb, q);
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    ok = ok && cb.w[w] == core.w[w];
+                }
+            }
+            if (!ok) {
+                continue;
+            }
+            double2 x;
+            int sg;
+            if (ADJ) {
+                x = single_element<W, CH>(p, core, r, q);
+                sg = fermi ? single_parity<W>(ca, r, q) : 0;
+            } else {
+                Cfg<W> cb = core;
+                flip_site<W>(cb, q);
+                x = single_element<W, CH>(p, core, q, r);
+                sg = fermi ? single_parity<W>(cb, q, r) : 0;
+            }
+            const double xr = sg ? -x.x : x.x;
+            const double xi = sg ? -x.y : x.y;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double2 gg = v == 0 ? gb : gb2;
+                if (ADJ) {
+                    acc_re[v] += xr * gg.x + xi * gg.y;
+                    acc_im[v] += xr * gg.y - xi * gg.x;
+                } else {
+                    acc_re[v] += xr * gg.x - xi * gg.y;
+                    acc_im[v] += xr * gg.y + xi * gg.x;
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            if (act && (acc_re[v] != 0.0 || acc_im[v] != 0.0)) {
+                double2* o = v == 0 ? p.out : p.out2;
+                unsafeAtomicAdd(&o[ia].x, acc_re[v]);
+                unsafeAtomicAdd(&o[ia].y, acc_im[v]);
+            }
+        }
+    }
+}
+
 // Single-excitation groups of 2..8 entries: the lane-per-(row, entry) scheme of pair_tiny_class.
 template <int W, bool ADJ, int NV, int CH>
 __device__ void single_tiny_class(const SingleParams& p, uint64_t mask, uint32_t gs_l, uint32_t ge_l, int gp, int lane) {
@@ -1065,7 +1314,7 @@ __device__ void single_tiny_class(const SingleParams& p, uint64_t mask, uint32_t
     }
 }
 
-template <int W, bool ADJ, int NV, int CH>
+template <int W, bool ADJ, int NV, int CH, bool PACK = true>
 __device__ __forceinline__ void single_chunk(const SingleParams& p, int64_t groups, int64_t g0, int lane) {
     const int64_t gg = g0 + lane;
     uint32_t gs = 0, ge = 0;
@@ -1077,7 +1326,12 @@ __device__ __forceinline__ void single_chunk(const SingleParams& p, int64_t grou
     single_tiny_class<W, ADJ, NV, CH>(p, __ballot(sz > 4u && sz <= 8u), gs, ge, 8, lane);
     single_tiny_class<W, ADJ, NV, CH>(p, __ballot(sz > 2u && sz <= 4u), gs, ge, 4, lane);
     single_tiny_class<W, ADJ, NV, CH>(p, __ballot(sz == 2u), gs, ge, 2, lane);
-    uint64_t big = __ballot(sz > 8u);
+    // PACK: groups of 9..32 entries several to a wave (not in the batch kernel: register limit)
+    if (PACK) {
+        single_group_packed<W, ADJ, NV, CH, 16>(p, __ballot(sz > 8u && sz <= 16u), gs, ge, lane);
+        single_group_packed<W, ADJ, NV, CH, 32>(p, __ballot(sz > 16u && sz <= 32u), gs, ge, lane);
+    }
+    uint64_t big = __ballot(sz > (PACK ? 32u : 8u));
     while (big) {
         const int l = __builtin_ctzll(big);
         big &= big - 1;
@@ -1236,7 +1490,7 @@ __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int 
         p.out2 = it.out2;
         const int64_t groups = it.pcounts[1];
         if (local * 64 < groups && it.pcounts[2] == 0) {
-            pair_chunk<W, ADJ, kPairBatchInBatch, NV>(p, groups, local * 64, lane, stage);
+            pair_chunk<W, ADJ, kPairBatchInBatch, NV, false>(p, groups, local * 64, lane, stage);
         }
     } else if (type == 1) {
         SingleParams p{};
@@ -1256,7 +1510,7 @@ __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int 
         p.out2 = it.out2;
         const int64_t groups = it.scounts[1];
         if (local * 64 < groups && it.scounts[2] == 0) {
-            single_chunk<W, ADJ, NV, 4>(p, groups, local * 64, lane);
+            single_chunk<W, ADJ, NV, 4, false>(p, groups, local * 64, lane);
         }
     } else {
         diag_rows<W, ADJ, NV>(it, local * 64, lane);
