@@ -139,9 +139,10 @@ __device__ __forceinline__ int64_t lookup_from(const HashIndex& ix, const Cfg<W>
 // The filter words of all U targets are loaded first.  PHASED then also loads the first table slot
 // of every target that passed, then the stored keys of every slot whose fingerprint matched, and
 // calls post() last; otherwise each target's probe chain and post() run in turn.  The direct
-// enumeration (few lookups per term row, 2 in flight at 8 waves) gains from the phases (hubbard128
-// forward 5.25 -> 4.2 ms with them); the tile enumeration, whose posts do more work between lookups,
-// loses (find_relative at 1M configs 73.6 -> 90.7 ms), so it keeps the chains.
+// enumeration (few lookups per term row, 2 in flight at 8 waves) gains from the phases (a 64-site
+// Hubbard chain with 4M configs: forward 5.25 -> 4.2 ms with them); the tile enumeration, whose posts
+// do more work between lookups, loses (find_relative at 1M configs 73.6 -> 90.7 ms), so it keeps the
+// chains.
 template <int W, int U, bool PHASED, typename Action>
 __device__ __forceinline__ void lookup_batch(const HashIndex& ix, Action& act, const int64_t (&i)[U], const uint32_t (&t)[U],
                                              const Cfg<W> (&o)[U], const int (&par)[U], const bool (&want)[U],
