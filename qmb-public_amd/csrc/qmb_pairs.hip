@@ -474,6 +474,26 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
     flip_site<W>(core0, static_cast<int>(v0 & 0xff));
     flip_site<W>(core0, static_cast<int>((v0 >> 8) & 0xff));
     const bool fermi = p.particle_cut == 1;
+    // Different cores with equal hashes share a group: then every pair is checked exactly.  The test
+    // covers all entries of the group, not one 64-row block: a block whose rows all hold the first
+    // core still meets entries of the other cores in the other blocks.
+    bool mixed = false;
+    for (uint32_t m0 = s; m0 < e; m0 += 64) {
+        const uint32_t ml = m0 + lane;
+        bool diff = false;
+        if (ml < e) {
+            const uint64_t vm = p.vals[ml];
+            Cfg<W> cm = load_key<W>(p.ckeys, static_cast<int64_t>(vm >> 16));
+            flip_site<W>(cm, static_cast<int>(vm & 0xff));
+            flip_site<W>(cm, static_cast<int>((vm >> 8) & 0xff));
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                diff = diff || cm.w[w] != core0.w[w];
+            }
+        }
+        const bool any_diff = __ballot(diff) != 0ull;
+        mixed = mixed || any_diff;
+    }
     for (uint32_t a0 = s; a0 < e; a0 += 64) {
         const uint32_t a = a0 + lane;
         const bool act = a < e;
@@ -485,13 +505,6 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
         Cfg<W> core = ca;
         flip_site<W>(core, r0);
         flip_site<W>(core, r1);
-        bool same = true;
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-            same = same && core.w[w] == core0.w[w];
-        }
-        // Different cores with equal hashes share a group: then every pair is checked exactly.
-        const bool mixed = __ballot(act && !same) != 0ull;
         const int64_t prow = pair_slot(r0, r1, p.n_bits);
         const Cfg<W> mr = range_mask<W>(r0, r1);
         const int base = fermi ? masked_parity<W>(ADJ ? ca : core, mr) : 0;
@@ -667,7 +680,8 @@ __device__ void pair_group_packed(const PairParams& p, uint64_t mask, uint32_t g
         bool same = true;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            same = same && shfl_u64(core.w[w], gi * H) == core.w[w];
+            const bool eq = shfl_u64(core.w[w], gi * H) == core.w[w];
+            same = same & eq;  // every lane runs every shuffle (a short-circuit would leave lanes out)
         }
         const bool mixed = __ballot(act && !same) != 0ull;
         const int64_t prow = pair_slot(r0, r1, p.n_bits);
@@ -813,7 +827,8 @@ __device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_
         bool same = true;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            same = same && shfl_u64(core.w[w], src) == core.w[w];
+            const bool eq = shfl_u64(core.w[w], src) == core.w[w];
+            same = same & eq;  // every lane runs every shuffle (a short-circuit would leave lanes out)
         }
         const bool ok = pre && same;
         double acc_re[NV] = {}, acc_im[NV] = {};
@@ -1041,6 +1056,23 @@ __device__ void single_group(const SingleParams& p, uint32_t s, uint32_t e, int 
     Cfg<W> core0 = load_key<W>(p.ckeys, static_cast<int64_t>(v0 >> 16));
     flip_site<W>(core0, static_cast<int>(v0 & 0xff));
     const bool fermi = p.particle_cut == 1;
+    // merged cores: the test covers every entry of the group (see pair_group)
+    bool mixed = false;
+    for (uint32_t m0 = s; m0 < e; m0 += 64) {
+        const uint32_t ml = m0 + lane;
+        bool diff = false;
+        if (ml < e) {
+            const uint64_t vm = p.vals[ml];
+            Cfg<W> cm = load_key<W>(p.ckeys, static_cast<int64_t>(vm >> 16));
+            flip_site<W>(cm, static_cast<int>(vm & 0xff));
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                diff = diff || cm.w[w] != core0.w[w];
+            }
+        }
+        const bool any_diff = __ballot(diff) != 0ull;
+        mixed = mixed || any_diff;
+    }
     for (uint32_t a0 = s; a0 < e; a0 += 64) {
         const uint32_t a = a0 + lane;
         const bool act = a < e;
@@ -1050,12 +1082,6 @@ __device__ void single_group(const SingleParams& p, uint32_t s, uint32_t e, int 
         const Cfg<W> ca = load_key<W>(p.ckeys, ia);
         Cfg<W> core = ca;
         flip_site<W>(core, r);
-        bool same = true;
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-            same = same && core.w[w] == core0.w[w];
-        }
-        const bool mixed = __ballot(act && !same) != 0ull;
         double acc_re[NV] = {}, acc_im[NV] = {};
         for (uint32_t b0 = s; b0 < e; b0 += 64) {
             const uint32_t bl = b0 + lane;
@@ -1158,7 +1184,8 @@ __device__ void single_group_packed(const SingleParams& p, uint64_t mask, uint32
         bool same = true;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            same = same && shfl_u64(core.w[w], gi * H) == core.w[w];
+            const bool eq = shfl_u64(core.w[w], gi * H) == core.w[w];
+            same = same & eq;  // every lane runs every shuffle (a short-circuit would leave lanes out)
         }
         const bool mixed = __ballot(act && !same) != 0ull;
         const double2 gin = act ? p.in[ia] : make_double2(0.0, 0.0);
@@ -1270,7 +1297,8 @@ __device__ void single_tiny_class(const SingleParams& p, uint64_t mask, uint32_t
         bool same = true;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            same = same && shfl_u64(core.w[w], src) == core.w[w];
+            const bool eq = shfl_u64(core.w[w], src) == core.w[w];
+            same = same & eq;  // every lane runs every shuffle (a short-circuit would leave lanes out)
         }
         const bool ok = pre && same;
         double acc_re[NV] = {}, acc_im[NV] = {};
