@@ -113,23 +113,40 @@ def cpu_baseline(w: dict[str, object], m_hit_sample: int) -> dict[str, object]:
         out = pathlib.Path(d) / "out.npz"
         np.savez(inp, configs_i=configs[:n], psi_i=w["psi"][:n], configs_j=configs, site=w["site"].numpy(), kind=w["kind"].numpy(),
                  coef=w["coef"].numpy(), particle_cut=pc)
+        from oracle import oracle as orc  # pylint: disable=import-outside-toplevel
+
+        args = (pc, configs[:n], w["psi"][:n], configs, w["site"].numpy(), w["kind"].numpy(), w["coef"].numpy())
+        result = None
         if ref.exists():
             env = dict(os.environ, OMP_NUM_THREADS="1")
             proc = subprocess.run([sys.executable, str(ROOT / "oracle" / "ref_runner.py"), "apply_within", str(inp), str(out),
                                    "--repeat", "1", "--threads", "1"], env=env, capture_output=True, text=True, check=False)
             if proc.returncode == 0:
                 t = float(np.load(out)["times"][0])
-                return {"value": m_hit_sample / t, "unit": "matrix elements/s", "cores": 1, "kind": "reference",
-                        "sample": sample_desc + " (qmb/_hamiltonian_cpu.cpp built from source, -O3 -ffast-math, 1 thread)",
-                        "seconds": t, "host_cpu": platform.processor() or platform.machine()}
-            log(f"reference op failed: {proc.stderr[-500:]}")
-        from oracle import oracle as orc  # pylint: disable=import-outside-toplevel
-
+                result = {"value": m_hit_sample / t, "unit": "matrix elements/s", "cores": 1, "kind": "reference",
+                          "sample": sample_desc + " (qmb/_hamiltonian_cpu.cpp built from source, -O3 -ffast-math, 1 thread)",
+                          "seconds": t, "host_cpu": platform.processor() or platform.machine()}
+            else:
+                log(f"reference op failed: {proc.stderr[-500:]}")
+        if result is None:
+            t0 = time.perf_counter()
+            orc.apply_within(*args)
+            t = time.perf_counter() - t0
+            result = {"value": m_hit_sample / t, "unit": "matrix elements/s", "cores": 1, "kind": "port",
+                      "sample": sample_desc + " (oracle/qmb_oracle.c, 1 thread)", "seconds": t}
+        # The same sample over the host cores this process may use (at most 16, one GPU's share of
+        # the host): the port's slices of configs_i on threads, reported beside the serial figure.
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
         t0 = time.perf_counter()
-        orc.apply_within(pc, configs[:n], w["psi"][:n], configs, w["site"].numpy(), w["kind"].numpy(), w["coef"].numpy())
+        _, (_, hit) = orc.apply_within_threads(*args, threads)
         t = time.perf_counter() - t0
-        return {"value": m_hit_sample / t, "unit": "matrix elements/s", "cores": 1, "kind": "port",
-                "sample": sample_desc + " (oracle/qmb_oracle.c, 1 thread)", "seconds": t}
+        if hit != m_hit_sample:
+            log(f"WARNING: the threaded CPU port found {hit} contributions in the sample, the GPU count {m_hit_sample}")
+        result["all_cores"] = {"value": hit / t, "cores": threads, "kind": "port", "seconds": t, "m_hit": hit,
+                               "sample": "the same sample, oracle/qmb_oracle.c apply_within_threads (configs_i split over threads)"}
+        log(f"CPU baseline: {result['kind']} 1 thread {result['value']:.3e} elements/s; "
+            f"port on {threads} threads {hit / t:.3e} elements/s")
+        return result
 
 
 def build_batch_instances(seed0: int = 4000, count: int = 64) -> list[dict[str, object]]:

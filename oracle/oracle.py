@@ -100,6 +100,27 @@ def apply_within(particle_cut: int, configs_i: np.ndarray, psi_i: object, config
     return out[:, 0] + 1j * out[:, 1], asum, (int(counts[0]), int(counts[1]))
 
 
+def apply_within_threads(particle_cut: int, configs_i: np.ndarray, psi_i: object, configs_j: np.ndarray, site: object,
+                         kind: object, coef: object, threads: int) -> tuple[np.ndarray, tuple[int, int]]:
+    """apply_within with configs_i split over `threads` host threads (timing only; no reference
+    counterpart).  Returns (psi_j complex128 [n_j], (M_gen, M_hit))."""
+    L = lib()
+    fn = L.oracle_apply_within_threads
+    fn.argtypes = [_int, _int, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _int, _vp, _vp]
+    fn.restype = _int
+    ci = _c(configs_i, np.uint8)
+    cj = _c(configs_j, np.uint8)
+    nq = ci.shape[1] if ci.ndim == 2 and ci.shape[0] else cj.shape[1]
+    pi = _psi(psi_i)
+    s, k, c = _terms(site, kind, coef)
+    out = np.zeros((len(cj), 2), dtype=np.float64)
+    counts = np.zeros(2, dtype=np.int64)
+    rc = fn(nq, particle_cut, _p(ci), _p(pi), len(ci), _p(cj), len(cj), _p(s), _p(k), _p(c), len(s), int(threads), _p(out), _p(counts))
+    if rc != 0:
+        raise ValueError(f"oracle_apply_within_threads failed ({rc})")
+    return out[:, 0] + 1j * out[:, 1], (int(counts[0]), int(counts[1]))
+
+
 def find_relative(particle_cut: int, configs_i: np.ndarray, psi_i: object, count_selected: int, site: object, kind: object,
                   coef: object, configs_exclude: np.ndarray | None = None) -> tuple[np.ndarray, float]:
     """Returns (unique selected configs sorted by bytes, K-th entry weight of the final heap)."""

@@ -26,9 +26,12 @@
  *                               and the dense JW diagonal.
  *   single_relative           : brute force with injected keys (arg-max definition), dense JW
  *                               connectivity.
+ *   apply_within_threads      : (timing only, no reference counterpart) equal to apply_within:
+ *                               bit for bit on one thread, to rounding and with equal counts on more.
  *
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, no fast-math, so results are portable).
  */
+#include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -201,6 +204,126 @@ int oracle_apply_within(int nq, int particle_cut, const uint8_t* configs_i, cons
     free(acc);
     free(aacc);
     return 0;
+}
+
+/*
+ * apply_within over several host threads: not a restatement of the reference (whose CPU op is
+ * serial) but the same algorithm spread over cores, timed beside the serial one.  Thread k takes
+ * configs_i rows [k n_i / T, (k + 1) n_i / T) through the term-outer loop into a private
+ * accumulator; the accumulators are then summed in thread order, so a run is repeatable for a
+ * given thread count.  With one thread it adds in the serial order; with more, each target's sum
+ * is regrouped by slice (equal to the serial result up to rounding).  counts as in apply_within.
+ */
+typedef struct {
+    int nq, particle_cut;
+    const uint8_t* configs_i;
+    const double* psi_i;
+    int64_t i0, i1;
+    const uint8_t* sorted;
+    int64_t n_j;
+    const int16_t* site;
+    const uint8_t* kind;
+    const double* coef;
+    int64_t n_terms;
+    double* acc;
+    int64_t m_gen, m_hit;
+} apply_slice;
+
+static void* apply_slice_run(void* arg) {
+    apply_slice* a = (apply_slice*)arg;
+    uint8_t cur[MAX_NQ];
+    for (int64_t t = 0; t < a->n_terms; ++t) {
+        for (int64_t i = a->i0; i < a->i1; ++i) {
+            memcpy(cur, a->configs_i + i * a->nq, (size_t)a->nq);
+            int parity = 0;
+            if (!oracle_apply_term(cur, a->site + t * MAX_OP, a->kind + t * MAX_OP, a->particle_cut, &parity)) {
+                continue;
+            }
+            ++a->m_gen;
+            int64_t mid = bsearch_sorted(a->sorted, a->n_j, a->nq, cur);
+            if (mid < 0) {
+                continue;
+            }
+            ++a->m_hit;
+            double sign = parity ? -1.0 : 1.0;
+            double c0 = a->coef[2 * t], c1 = a->coef[2 * t + 1];
+            double p0 = a->psi_i[2 * i], p1 = a->psi_i[2 * i + 1];
+            a->acc[2 * mid] += sign * (c0 * p0 - c1 * p1);
+            a->acc[2 * mid + 1] += sign * (c0 * p1 + c1 * p0);
+        }
+    }
+    return NULL;
+}
+
+int oracle_apply_within_threads(int nq, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i,
+                                const uint8_t* configs_j, int64_t n_j, const int16_t* site, const uint8_t* kind,
+                                const double* coef, int64_t n_terms, int threads, double* psi_j, int64_t* counts) {
+    if (nq <= 0 || nq > MAX_NQ || threads < 1 || threads > 1024) {
+        return 1;
+    }
+    int64_t* order = sorted_order(configs_j, n_j, nq);
+    uint8_t* sorted = gather_rows(configs_j, order, n_j, nq);
+    const size_t width = (size_t)(n_j > 0 ? n_j : 1) * 2;
+    apply_slice* job = (apply_slice*)calloc((size_t)threads, sizeof(apply_slice));
+    pthread_t* tid = (pthread_t*)calloc((size_t)threads, sizeof(pthread_t));
+    int rc = 0;
+    int started = 0;
+    for (int k = 0; k < threads; ++k) {
+        apply_slice* a = &job[k];
+        a->nq = nq;
+        a->particle_cut = particle_cut;
+        a->configs_i = configs_i;
+        a->psi_i = psi_i;
+        a->i0 = n_i * k / threads;
+        a->i1 = n_i * (k + 1) / threads;
+        a->sorted = sorted;
+        a->n_j = n_j;
+        a->site = site;
+        a->kind = kind;
+        a->coef = coef;
+        a->n_terms = n_terms;
+        a->acc = (double*)calloc(width, sizeof(double));
+        if (!a->acc) {
+            rc = 2;
+            break;
+        }
+        if (pthread_create(&tid[k], NULL, apply_slice_run, a) != 0) {
+            rc = 3;
+            break;
+        }
+        ++started;
+    }
+    for (int k = 0; k < started; ++k) {
+        pthread_join(tid[k], NULL);
+    }
+    if (rc == 0) {
+        int64_t m_gen = 0, m_hit = 0;
+        for (int64_t m = 0; m < n_j; ++m) {
+            double re = 0.0, im = 0.0;
+            for (int k = 0; k < threads; ++k) {
+                re += job[k].acc[2 * m];
+                im += job[k].acc[2 * m + 1];
+            }
+            psi_j[2 * order[m]] = re;
+            psi_j[2 * order[m] + 1] = im;
+        }
+        for (int k = 0; k < threads; ++k) {
+            m_gen += job[k].m_gen;
+            m_hit += job[k].m_hit;
+        }
+        if (counts) {
+            counts[0] = m_gen;
+            counts[1] = m_hit;
+        }
+    }
+    for (int k = 0; k < threads; ++k) {
+        free(job[k].acc);
+    }
+    free(job);
+    free(tid);
+    free(order);
+    free(sorted);
+    return rc;
 }
 
 /* ---------------------------------------------------------------------------------------------

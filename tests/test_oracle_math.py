@@ -161,6 +161,29 @@ def test_linearity() -> None:
     np.testing.assert_allclose(fab, 2.5 * fa - 1j * fb, atol=1e-12)
 
 
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_threaded_apply_within_equals_serial(threads: int) -> None:
+    """The multi-threaded timing variant equals the serial restatement: same M_gen / M_hit, the
+    same values bit for bit on one thread and to rounding (regrouped sums) on more; configs_i is
+    a different set from configs_j, with rows outside it, and smaller than some thread counts'
+    slices would suggest (empty slices included)."""
+    s, k, c = sorted_terms(*sy.molecular_terms(8, 21))
+    cfg_j = sy.excitation_configs(8, 3, 3, 400, 22)
+    cfg_i = np.concatenate([cfg_j[::7], sy.excitation_configs(8, 3, 3, 60, 23)])
+    psi = sy.random_psi(len(cfg_i), 24)
+    want, _, want_counts = orc.apply_within(1, cfg_i, psi, cfg_j, s, k, c)
+    got, got_counts = orc.apply_within_threads(1, cfg_i, psi, cfg_j, s, k, c, threads)
+    assert got_counts == want_counts and want_counts[1] > 0
+    if threads == 1:
+        assert np.array_equal(got, want)
+    else:
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * float(np.abs(want).max()))
+    few, few_counts = orc.apply_within_threads(1, cfg_i[:2], psi[:2], cfg_j, s, k, c, threads)
+    ref, _, ref_counts = orc.apply_within(1, cfg_i[:2], psi[:2], cfg_j, s, k, c)
+    assert few_counts == ref_counts
+    np.testing.assert_allclose(few, ref, rtol=0, atol=1e-12 * max(float(np.abs(ref).max()), 1.0))
+
+
 def test_diagonal_term_identity_and_dense() -> None:
     s, k, c = (x.numpy() for x in sy.random_terms(7, 150, 9))
     h = dense_hamiltonian(7, s, k, c, True)
