@@ -98,6 +98,17 @@ def read_traffic(workload: str) -> float | None:
     return None
 
 
+def host_cpu_model() -> str:
+    """The host CPU's model name from /proc/cpuinfo (the machine type where that is unreadable)."""
+    try:
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.machine()
+
+
 def cpu_baseline(w: dict[str, object], m_hit_sample: int) -> dict[str, object]:
     """The reference CPU op (oracle/_ref) on the first `sample` configs_i against all of configs_j."""
     import numpy as np  # pylint: disable=import-outside-toplevel
@@ -125,7 +136,7 @@ def cpu_baseline(w: dict[str, object], m_hit_sample: int) -> dict[str, object]:
                 t = float(np.load(out)["times"][0])
                 result = {"value": m_hit_sample / t, "unit": "matrix elements/s", "cores": 1, "kind": "reference",
                           "sample": sample_desc + " (qmb/_hamiltonian_cpu.cpp built from source, -O3 -ffast-math, 1 thread)",
-                          "seconds": t, "host_cpu": platform.processor() or platform.machine()}
+                          "seconds": t}
             else:
                 log(f"reference op failed: {proc.stderr[-500:]}")
         if result is None:
@@ -142,6 +153,8 @@ def cpu_baseline(w: dict[str, object], m_hit_sample: int) -> dict[str, object]:
         t = time.perf_counter() - t0
         if hit != m_hit_sample:
             log(f"WARNING: the threaded CPU port found {hit} contributions in the sample, the GPU count {m_hit_sample}")
+        result["host_cpu"] = host_cpu_model()
+        result["host_cores_allowed"] = len(os.sched_getaffinity(0))
         result["all_cores"] = {"value": hit / t, "cores": threads, "kind": "port", "seconds": t, "m_hit": hit,
                                "sample": "the same sample, oracle/qmb_oracle.c apply_within_threads (configs_i split over threads)"}
         log(f"CPU baseline: {result['kind']} 1 thread {result['value']:.3e} elements/s; "
