@@ -1,5 +1,5 @@
 """
-Source rules of the HIP kernels that no GPU run can be relied on to show.
+Source rules of the HIP kernels and their host code that no GPU run can be relied on to show.
 
 A cross-lane operation (shuffle, ballot, readlane, DPP move) reads the registers of other lanes, so
 every lane it reads from must execute it.  On the right of `&&` or `||` it runs only on the lanes
@@ -47,3 +47,27 @@ def test_no_cross_lane_operation_behind_a_short_circuit() -> None:
             if short_circuited(stmt):
                 bad.append(f"{path.name}: {' '.join(stmt.split())[:160]}")
     assert not bad, "\n".join(bad)
+
+
+PACKAGE = CSRC.parent
+
+
+def test_hip_code_is_built_without_packed_fp32() -> None:
+    """Every HIP compile line turns the packed-FP32 instructions off (some of their forms have
+    given wrong results without an error); the kernels write no such instruction by hand."""
+    flags = (CSRC / "Makefile").read_text()
+    assert "-Xclang -target-feature -Xclang -packed-fp32-ops" in flags
+    for path in sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.hpp")):
+        assert not re.search(r"v_pk_(add|mul|fma)_f32", path.read_text()), path.name
+
+
+def test_no_priority_streams_memset_nodes_or_host_registration() -> None:
+    """Streams keep the default priority; device memory is cleared by fill kernels, never by a
+    memset node a graph capture would record; host memory is allocated pinned, never registered
+    after the fact."""
+    banned = re.compile(r"hipStreamCreateWithPriority|hipMemsetAsync|hipMemset\b|hipHostRegister|"
+                        r"is_high_priority_stream|Stream\(priority")
+    files = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.hpp")) + sorted((PACKAGE / "qmb_hip").glob("*.py"))
+    assert len(files) >= 15
+    bad = [f"{p.name}: {m.group(0)}" for p in files for m in banned.finditer(p.read_text())]
+    assert not bad, bad
