@@ -241,7 +241,8 @@ def terms_to_dict(site: np.ndarray, kind: np.ndarray, coef: np.ndarray) -> TermD
 def fcidump_terms(file_name: pathlib.Path | str) -> tuple[tuple[int, int, int], TermArrays]:
     """(NORB, NELEC, MS2) and the normal-ordered terms as (site, kind, coef) arrays in dict order."""
     meta, energy_0, h, g = read_integrals(file_name)
-    assert h is not None and g is not None
+    if h is None or g is None:
+        raise ValueError(f"{file_name} holds no one- and two-electron integrals")
     terms, _ = normal_ordered_terms(energy_0, h, g)
     return meta, terms
 

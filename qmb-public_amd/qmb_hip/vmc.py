@@ -87,7 +87,8 @@ def energy_backward(network: torch.nn.Module, energy_set: EnergySet, chunk: int 
     psi = network_forward(network, energy_set.configs, chunk).requires_grad_(True)
     energy = energy_set.energy(psi)
     energy.backward()
-    assert psi.grad is not None
+    if psi.grad is None:
+        raise RuntimeError("the energy did not reach psi: no gradient to pull back through the network")
     network_backward(network, energy_set.configs, psi.grad, chunk)
     return float(energy.detach())
 
