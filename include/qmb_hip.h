@@ -271,11 +271,12 @@ int qmb_terms_build_residual(int n_qubytes, int particle_cut, const int16_t* sit
 
 /* Core index of the set indexed by a qmb_targets: one entry per (config, `order` occupied sites)
  * -- order 2: every occupied pair (cores of pair excitations), order 1: every occupied site
- * (cores of single excitations) -- radix-sorted by a class of the core (even sites mod 3), a
- * 34-bit hash of the core and, within a group of equal hashes, the removed pair (site); and the
- * start of every group.  max_occupied must bound the number of occupied sites of every
- * config (a row with more sets counts[2], and the apply calls then write NaN into out[0] instead
- * of a wrong result). */
+ * (cores of single excitations) -- radix-sorted by a class of the core (order 2: even sites of
+ * the core mod 3; order 1: 0), a 34-bit hash of the core and, within a group of equal keys, the
+ * removed pair (site); and the start of every group.  Different cores with equal class and hash
+ * share a group; the apply calls compare cores exactly there.  max_occupied must bound the number
+ * of occupied sites of every config (a row with more sets counts[2], and the apply calls then
+ * write NaN into out[0] instead of a wrong result). */
 typedef struct qmb_cores {
     int32_t n_qubytes, n_words, max_occupied, per_config; /* per_config = C(max_occupied, order) */
     int32_t order, key_bits;  /* key_bits: width of the sorted keys (set by qmb_cores_build) */
@@ -283,7 +284,7 @@ typedef struct qmb_cores {
     int64_t capacity;  /* n * per_config entries */
     void* buffer;
     size_t bytes;
-    uint64_t* keys;    /* [capacity] sorted keys: (even sites of the core mod 3) << (34 + s) | 34-bit core hash << s |
+    uint64_t* keys;    /* [capacity] sorted keys: core class << (34 + s) | 34-bit core hash << s |
                           removed pair slot (order 1: removed site) in the low s = key_bits - 36 bits (the
                           fewest that hold n_pairs, or n_bits); unused entries last (all key_bits ones) */
     uint64_t* vals;    /* [capacity] row << 16 | site1 << 8 | site0 of the removed sites (order 1: site0) */
