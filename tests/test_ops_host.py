@@ -1,10 +1,15 @@
 """
 The op layer on a host without a GPU: `ops.load` registers the reference's namespace with its four
-schemas word for word (qmb/_hamiltonian.cpp:83-88), and a call on CPU tensors fails loudly -- the
-ops have a GPU kernel only and no CPU fallback.
+schemas word for word (qmb/_hamiltonian.cpp:83-88), a call on CPU tensors fails loudly -- the ops
+have a GPU kernel only and no CPU fallback -- and without the HIP library loading the ops raises.
 """
 
 from __future__ import annotations
+
+import os
+import pathlib
+import subprocess
+import sys
 
 import pytest
 import torch
@@ -50,3 +55,15 @@ def test_load_rejects_unsupported_shapes() -> None:
         ops.load(0, 1)
     with pytest.raises(ValueError, match="n_qubytes"):
         ops.load(33, 1)
+
+
+def test_missing_library_fails_loudly(tmp_path: pathlib.Path) -> None:
+    """With the HIP library absent, loading the ops raises instead of falling back to any other
+    path (run in a fresh process: the library is loaded once per process)."""
+    root = pathlib.Path(__file__).resolve().parent.parent
+    env = dict(os.environ, QMB_HIP_LIBRARY=str(tmp_path / "absent.so"),
+               PYTHONPATH=os.pathsep.join([str(root / "qmb-public_amd"), os.environ.get("PYTHONPATH", "")]))
+    code = "from qmb_hip import ops\nops.load(7, 1)\n"
+    proc = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120, check=False)
+    assert proc.returncode != 0
+    assert "ImportError" in proc.stderr and "no non-HIP fallback" in proc.stderr, proc.stderr[-2000:]
