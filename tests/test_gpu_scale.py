@@ -106,10 +106,11 @@ def test_molecular52_find_relative_properties(molecular52: dict[str, object]) ->
 
 def test_two_word_structured_matches_enumeration(cuda_device: torch.device) -> None:
     """Configs of two words (76 spin orbitals, 13a+13b, 3.06M terms, 200,000 configs): the
-    structured path equals the enumeration path on every row, forward and adjoint.  The set's
-    65M order-2 entries hold thousands of groups where cores with equal 34-bit hashes meet
-    (7,586 of 28.2M groups), many with an entry of the other core first and cores with sites in
-    the second word: the exact core comparison must hold there for every word."""
+    structured path equals the enumeration path on every row, forward and adjoint, and the oracle
+    on six random rows.  The set's 65M order-2 entries hold thousands of groups where cores with
+    equal 34-bit hashes meet (7,586 of 28.2M groups), many with an entry of the other core first
+    and cores with sites in the second word: the exact core comparison must hold there for every
+    word."""
     s, k, c = sorted_terms(*sy.molecular_terms(38, 5001, threshold=1e-8))
     cfg = sy.excitation_configs(38, 13, 13, 200_000, 5002)
     st, kt, ct = (torch.from_numpy(np.ascontiguousarray(x)).to(cuda_device) for x in (s, k, c))
@@ -118,9 +119,19 @@ def test_two_word_structured_matches_enumeration(cuda_device: torch.device) -> N
     targets = pp.PreparedTargets(cfg_t, 10, distinct=True, max_occupied=26)
     assert pp.can_use_pairs(terms, targets, cfg_t)
     w = {"terms": terms, "targets": targets, "cfg_t": cfg_t, "device": cuda_device}
+    results = []
     for seed, mode in ((5003, pp.MODE_FORWARD), (5004, pp.MODE_ADJOINT)):
         vec = sy.random_psi(len(cfg), seed)
         structured = _apply(w, vec, mode)
         enumerated = _apply(w, vec, mode, pairs=False)
         scale = float(np.abs(enumerated).max())
         assert float(np.abs(structured - enumerated).max()) <= 1e-12 * scale
+        results.append((structured, vec))
+    # six rows against the oracle's column of H (the table is Hermitian: row j = conj(column j))
+    for j in np.random.default_rng(5005).choice(len(cfg), size=6, replace=False):
+        column, asum, (_, m_hit) = orc.apply_within(1, cfg[j:j + 1], np.ones(1, dtype=np.complex128), cfg, s, k, c)
+        assert m_hit > 1
+        for got, vec in results:
+            want = np.vdot(column, vec)
+            bound = 1e-12 * float(asum @ np.abs(vec))
+            assert abs(got[j] - want) <= bound, (int(j), got[j], want, bound)
