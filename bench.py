@@ -359,7 +359,7 @@ def run_lanczos(args: argparse.Namespace, dev: object) -> int:
         alpha, beta, v = next(gen)
     torch.cuda.synchronize()
     iter_ms = (time.perf_counter() - t0) * 1e3 / steps
-    energy, _ = kr.eigh_tridiagonal_lowest(alpha, beta, v, dev)
+    energy, _ = kr.lowest_ritz_pair(alpha, beta, v, dev)
 
     def timed(fn: object, reps: int = 10) -> float:
         fn()

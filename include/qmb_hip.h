@@ -419,6 +419,28 @@ int qmb_sampler_step_fused(int64_t n, int site, int n_sites, int hidden, int spi
                            const float* perturbed, const float* uniform, uint64_t seed, float* child_unperturbed, float* child_perturbed,
                            void* stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Operators on HOST memory for CPU tensors (host threads, no device work).  These take host
+ * pointers, allocate host memory for their indexes and return when the result is written.
+ * threads = host threads to use (<= 0: all hardware threads).
+ * ------------------------------------------------------------------------------------------- */
+/* apply_within on host memory: psi_j in configs_j's row order, as qmb_apply_within.
+ * Replaces apply_within_interface, qmb/_hamiltonian_cpu.cpp:164-257 (registered for the CPU key at
+ * :558-561). */
+int qmb_cpu_apply_within(int n_qubytes, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i,
+                         const uint8_t* configs_j, int64_t n_j, const int16_t* site, const uint8_t* kind, const double* coef,
+                         int64_t n_terms, double* psi_j, int threads);
+/* The adjoint on host memory, as qmb_apply_within_adjoint (no reference counterpart: the reference
+ * never differentiates the path). */
+int qmb_cpu_apply_within_adjoint(int n_qubytes, int particle_cut, const uint8_t* configs_i, int64_t n_i, const uint8_t* configs_j,
+                                 const double* grad_j, int64_t n_j, const int16_t* site, const uint8_t* kind, const double* coef,
+                                 int64_t n_terms, double* grad_i, int threads);
+/* find_relative on host memory, as qmb_find_relative; *out_rows is a HOST int64.
+ * Replaces find_relative_interface, qmb/_hamiltonian_cpu.cpp:442-546 (CPU key, :558-561). */
+int qmb_cpu_find_relative(int n_qubytes, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i,
+                          int64_t count_selected, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
+                          const uint8_t* configs_exclude, int64_t n_exclude, uint8_t* out, int64_t* out_rows, int threads);
+
 /* Host-only pair-excitation test of a compiled record (for unit tests): returns 1 and writes
  * {r0, r1, c0, c1, delta} when the record is a pair excitation, else 0. */
 int qmb_pair_excitation_host(uint64_t rec, int particle_cut, int32_t out[5]);

@@ -228,6 +228,12 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "qmb_sampler_step_fused": (_c_int, [_c_i64, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_u64, _vp, _vp, _vp]),
     "qmb_lanczos_workspace_size": (_c_int, [ctypes.POINTER(_c_size)]),
     "qmb_lanczos_step": (_c_int, [_c_i64, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp, _c_size, _vp]),
+    "qmb_cpu_apply_within": (_c_int, [_c_int, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _c_int]),
+    "qmb_cpu_apply_within_adjoint": (_c_int, [_c_int, _c_int, _vp, _c_i64, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _c_int]),
+    "qmb_cpu_find_relative": (
+        _c_int,
+        [_c_int, _c_int, _vp, _vp, _c_i64, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _c_i64, _vp, ctypes.POINTER(_c_i64), _c_int],
+    ),
     "qmb_pair_excitation_host": (_c_int, [_c_u64, _c_int, ctypes.POINTER(_c_i32)]),
     "qmb_single_excitation_host": (_c_int, [_c_u64, _c_int, ctypes.POINTER(_c_i32)]),
 }

@@ -31,10 +31,10 @@ class _Declaration:
 
 
 class Hamiltonian:
-    """
-    The Hamiltonian type, which stores the Hamiltonian and processes iteration over each term in the
-    Hamiltonian for given configurations (qmb/hamiltonian.py:12-15).
-    """
+    """A second-quantized operator sum_t coef_t O_t held as a term table (site int16[T, 4],
+    kind uint8[T, 4], coef float64[T, 2]), with the four operations on bit-packed configurations:
+    apply_within, find_relative, diagonal_term and single_relative (the class of
+    qmb/hamiltonian.py:12-206, same constructor, attributes and methods)."""
 
     _hamiltonian_module: dict[tuple[str, int, int], object] = {}
 
@@ -90,9 +90,12 @@ class Hamiltonian:
     def prepared_terms(self, device: torch.device, n_qubytes: int) -> PreparedTerms:
         """The compiled term table (buckets, structured pair/single/diagonal tables) for configs of
         n_qubytes bytes on `device`, built on first use and kept while site/kind/coef stay the same
-        tensors.  Repeated applications on one config set (Krylov loops) start from it."""
+        tensors with the same contents (their in-place versions are part of the key).  Repeated
+        applications on one config set (Krylov loops) start from it."""
         self._prepare_data(device)
-        key = (str(device), n_qubytes, self.site.data_ptr(), self.kind.data_ptr(), self.coef.data_ptr(), self.site.size(0))
+        # versions too: an in-place change of the table (ham.coef.mul_(2)) must rebuild it
+        key = (str(device), n_qubytes, self.site.data_ptr(), self.kind.data_ptr(), self.coef.data_ptr(), self.site.size(0),
+               self.site._version, self.kind._version, self.coef._version)  # pylint: disable=protected-access
         cached = getattr(self, "_prepared", None)
         if cached is None or cached[0] != key:
             cached = (key, PreparedTerms(self.site, self.kind, self.coef, n_qubytes, self.particle_cut))

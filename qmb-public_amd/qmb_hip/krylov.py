@@ -26,10 +26,10 @@ from __future__ import annotations
 
 import ctypes
 import dataclasses
-import functools
 import logging
 import typing
 
+import numpy
 import scipy.linalg
 import torch
 
@@ -101,17 +101,22 @@ class LanczosStep:
         return alpha, norm, v_next
 
 
-def eigh_tridiagonal_lowest(alpha: typing.Sequence[torch.Tensor], beta: typing.Sequence[torch.Tensor],
-                            v: typing.Sequence[torch.Tensor], device: torch.device) -> tuple[torch.Tensor, torch.Tensor]:
-    """Lowest Ritz pair of the Lanczos tridiagonal (qmb/haar.py:168-183): energy as a tensor and
-    sum_k y_k v_k on `device`."""
+def lowest_ritz_pair(alpha: typing.Sequence[torch.Tensor], beta: typing.Sequence[torch.Tensor],
+                     vectors: typing.Sequence[torch.Tensor], device: torch.device) -> tuple[torch.Tensor, torch.Tensor]:
+    """Lowest eigenpair of the Lanczos tridiagonal T (diagonal alpha, off-diagonal beta), lifted to
+    the basis: (theta, sum_k y_k vectors[k]) on `device`, theta the smallest eigenvalue of T and y
+    its unit eigenvector.  With a single alpha T is 1 x 1, so the pair is (alpha_0, vectors[0]).
+    The eigenproblem goes to LAPACK's bisection driver (stebz), which computes only the one
+    eigenvalue asked for, in O(len(alpha)) memory."""
     if len(beta) == 0:
-        return alpha[0], v[0]
-    vals, vecs = scipy.linalg.eigh_tridiagonal(torch.stack(list(alpha), dim=0).cpu().numpy(), torch.stack(list(beta), dim=0).cpu().numpy(),
-                                               lapack_driver="stebz", select="i", select_range=(0, 0))
-    energy = torch.as_tensor(vals[0])
-    result = functools.reduce(torch.add, (float(weight[0]) * vector.to(device=device) for weight, vector in zip(vecs, v)))
-    return energy, result
+        return alpha[0], vectors[0]
+    diagonal = numpy.fromiter((float(a) for a in alpha), dtype=numpy.float64, count=len(alpha))
+    off_diagonal = numpy.fromiter((float(b) for b in beta), dtype=numpy.float64, count=len(beta))
+    theta, y = scipy.linalg.eigh_tridiagonal(diagonal, off_diagonal, select="i", select_range=(0, 0), lapack_driver="stebz")
+    ritz = torch.zeros(vectors[0].shape, dtype=vectors[0].dtype, device=device)
+    for weight, vector in zip(y[:, 0].tolist(), vectors):
+        ritz.add_(vector.to(device=device), alpha=weight)
+    return torch.tensor(float(theta[0]), dtype=torch.float64), ritz
 
 
 def lanczos(operator: typing.Callable[[torch.Tensor], torch.Tensor], psi: torch.Tensor, threshold: float,
@@ -144,10 +149,24 @@ def lanczos(operator: typing.Callable[[torch.Tensor], torch.Tensor], psi: torch.
 
 @dataclasses.dataclass
 class DynamicLanczos:
-    """Dynamic Lanczos on a growing basis (qmb/haar.py:21-105): `run()` yields (energy, configs, psi)
-    with the reference's four modes (count_extend == 0: plain Lanczos; first_extend: grow the basis
-    before iterating; single_extend: grow once after a full run; otherwise grow by find_relative
-    after every step)."""
+    """Lanczos on a basis that can grow between runs (the driver of qmb/haar.py:21-183, same
+    fields and yields).  Every run of the recurrence starts from the stored start vector `psi`,
+    zero-padded onto the current basis, and `run()` yields (energy, basis, Ritz vector) tuples:
+
+      count_extend == 0   one run on the fixed basis; a tuple after each of its first step + 1
+                          applications;
+      first_extend        `step` growth rounds first -- each seeds find_relative with the
+                          count_extend largest-|amplitude| rows of a running amplitude vector
+                          (first the start vector, then H applied from the previous basis onto
+                          the grown one) -- then as count_extend == 0;
+      single_extend       as count_extend == 0, one growth seeded by the whole basis with
+                          amplitudes sqrt(sum_k |v_k|^2) over the run's Krylov vectors, then as
+                          count_extend == 0 again;
+      otherwise           for s = 1 .. step + 1: a run of s applications, one tuple from its last
+                          state, then (except after the last) a growth seeded by the whole basis
+                          with the run's last Krylov vector.
+
+    A growth appends find_relative(seeds, amplitudes, count_extend, exclude = basis) to the basis."""
 
     # pylint: disable=too-many-instance-attributes
 
@@ -170,69 +189,79 @@ class DynamicLanczos:
             raise RuntimeError("psi must be complex128 with one entry per config")
 
     def _within(self) -> WithinOperator:
-        """The operator on the current basis, rebuilt when the basis changes."""
+        """H on the current basis; a new basis gets a new operator (and new indexes)."""
         if self._operator is None or self._operator.configs is not self.configs:
             self.configs = self.configs.contiguous()
             self._operator = WithinOperator(self.model, self.configs)
         return self._operator
 
-    def _extend(self, psi: torch.Tensor, basic_configs: torch.Tensor | None = None) -> None:
-        """qmb/haar.py:39-50: append the find_relative selection from basic_configs (default the
-        basis) to the basis and zero-pad psi."""
-        if basic_configs is None:
-            basic_configs = self.configs
-        if not psi.is_complex():
-            # weights only: find_relative ranks |coef * psi|^2, the same for a real psi
-            psi = psi.to(torch.complex128)
-        count_core = len(self.configs)
-        logging.info("Extending basis from %d configurations", count_core)
-        self.configs = torch.cat([self.configs, self.model.find_relative(basic_configs, psi, self.count_extend, self.configs)])
-        count_selected = len(self.configs)
-        self.psi = torch.nn.functional.pad(self.psi, (0, count_selected - count_core))
-        logging.info("Basis extended from %d to %d", count_core, count_selected)
+    def _grow(self, amplitudes: torch.Tensor, seeds: torch.Tensor | None = None) -> None:
+        """Appends find_relative(seeds (default: the basis), amplitudes, count_extend, basis) to the
+        basis and zeros for the new rows to the start vector.  find_relative ranks |coef * amplitude|^2,
+        so a real amplitude vector is taken as complex."""
+        origin = self.configs if seeds is None else seeds
+        if not amplitudes.is_complex():
+            amplitudes = amplitudes.to(torch.complex128)
+        before = self.configs.size(0)
+        found = self.model.find_relative(origin, amplitudes, self.count_extend, self.configs)
+        self.configs = torch.cat((self.configs, found))
+        self.psi = torch.cat((self.psi, self.psi.new_zeros(self.configs.size(0) - before)))
+        logging.info("Lanczos basis grown from %d to %d configurations", before, self.configs.size(0))
 
-    def _run(self) -> typing.Iterator[tuple[list[torch.Tensor], list[torch.Tensor], list[torch.Tensor]]]:
-        return lanczos(self._within(), self.psi, self.threshold, self.keep_on_device)
+    def _states(self, applications: int) -> typing.Iterator[tuple[list[torch.Tensor], list[torch.Tensor], list[torch.Tensor]]]:
+        """The recurrence's (alpha, beta, v) after each of its first `applications` applications
+        (fewer if it stops on the threshold), from the start vector on the current basis."""
+        if applications < 1:
+            return
+        for done, state in enumerate(lanczos(self._within(), self.psi, self.threshold, self.keep_on_device), start=1):
+            yield state
+            if done == applications:
+                return
 
-    def _eigh_tridiagonal(self, alpha: list[torch.Tensor], beta: list[torch.Tensor], v: list[torch.Tensor]) -> tuple[torch.Tensor, torch.Tensor]:
-        return eigh_tridiagonal_lowest(alpha, beta, v, self.configs.device)
+    def _ritz(self, state: tuple[list[torch.Tensor], list[torch.Tensor], list[torch.Tensor]]) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+        alpha, beta, v = state
+        energy, vector = lowest_ritz_pair(alpha, beta, v, self.configs.device)
+        return energy, self.configs, vector
+
+    def _grow_ahead(self) -> None:
+        """The first_extend rounds: grow from the largest amplitudes, then carry the amplitudes onto
+        the grown basis through H."""
+        running = self.psi
+        for _ in range(self.step):
+            weight = torch.view_as_real(running).square().sum(dim=-1)
+            top = torch.argsort(weight, descending=True)[:self.count_extend]
+            previous = self.configs
+            self._grow(running[top], previous[top])
+            running = self.model.apply_within(previous, running, self.configs)
 
     def run(self) -> typing.Iterator[tuple[torch.Tensor, torch.Tensor, torch.Tensor]]:
-        """qmb/haar.py:52-105."""
-        alpha: list[torch.Tensor]
-        beta: list[torch.Tensor]
-        v: list[torch.Tensor]
-        if self.count_extend == 0:
-            for _, (alpha, beta, v) in zip(range(1 + self.step), self._run()):
-                energy, psi = self._eigh_tridiagonal(alpha, beta, v)
-                yield energy, self.configs, psi
-        elif self.first_extend:
-            psi = self.psi
-            for _ in range(self.step):
-                selected = (psi.conj() * psi).real.argsort(descending=True)[:self.count_extend]
-                configs = self.configs
-                self._extend(psi[selected], self.configs[selected])
-                psi = self.model.apply_within(configs, psi, self.configs)
-            for _, (alpha, beta, v) in zip(range(1 + self.step), self._run()):
-                energy, psi = self._eigh_tridiagonal(alpha, beta, v)
-                yield energy, self.configs, psi
-        elif self.single_extend:
-            for _, (alpha, beta, v) in zip(range(1 + self.step), self._run()):
-                energy, psi = self._eigh_tridiagonal(alpha, beta, v)
-                yield energy, self.configs, psi
-            v_sum = functools.reduce(torch.add, ((vi.conj() * vi).real.cpu() for vi in v)).sqrt().to(device=self.configs.device)
-            self._extend(v_sum)
-            for _, (alpha, beta, v) in zip(range(1 + self.step), self._run()):
-                energy, psi = self._eigh_tridiagonal(alpha, beta, v)
-                yield energy, self.configs, psi
-        else:
-            for step in range(1 + self.step):
-                for _, (alpha, beta, v) in zip(range(1 + step), self._run()):
-                    pass
-                energy, psi = self._eigh_tridiagonal(alpha, beta, v)
-                yield energy, self.configs, psi
-                if step != self.step:
-                    self._extend(v[-1])
+        """Yields (energy, basis, Ritz vector) as described in the class docstring."""
+        full = self.step + 1
+        if self.count_extend == 0 or self.first_extend or self.single_extend:
+            if self.count_extend != 0 and self.first_extend:
+                self._grow_ahead()
+            vectors: list[torch.Tensor] = []
+            for state in self._states(full):
+                vectors = state[2]
+                yield self._ritz(state)
+            if self.count_extend == 0 or self.first_extend:
+                return
+            # single_extend: one growth from the root-sum-square of every Krylov vector of the run
+            spread = torch.zeros(self.configs.size(0), dtype=torch.float64, device=self.configs.device)
+            for vector in vectors:
+                spread.add_(vector.to(device=spread.device).abs().square())
+            self._grow(spread.sqrt_())
+            for state in self._states(full):
+                yield self._ritz(state)
+            return
+        for applications in range(1, full + 1):
+            last = None
+            for last in self._states(applications):
+                pass
+            assert last is not None
+            yield self._ritz(last)
+            if applications < full:
+                self._grow(last[2][-1])
 
 
 def lanczos_energy(model: object, configs: torch.Tensor, step: int, threshold: float,
@@ -248,5 +277,5 @@ def lanczos_energy(model: object, configs: torch.Tensor, step: int, threshold: f
             break
     if len(beta) == 0:
         return float(alpha[0]), v[0]
-    energy, result = eigh_tridiagonal_lowest(alpha, beta, v, configs.device)
+    energy, result = lowest_ritz_pair(alpha, beta, v, configs.device)
     return float(energy), result

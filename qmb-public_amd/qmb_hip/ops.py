@@ -9,8 +9,9 @@ Registers, per (n_qubytes, particle_cut), the reference's op namespace and schem
     torch.ops.qmb_hamiltonian_{nq}_{pc}.diagonal_term(configs, site, kind, coef)
     torch.ops.qmb_hamiltonian_{nq}_{pc}.single_relative(configs, site, kind, coef)
 
-with implementations on the GPU dispatch key only (HIP tensors report device type "cuda" in
-PyTorch-ROCm).  Extra ops with no reference counterpart live in the same namespace:
+with implementations on the GPU dispatch key (HIP tensors report device type "cuda" in
+PyTorch-ROCm) and, for apply_within, its adjoint and find_relative, on the CPU key (the library's
+host operators, as the reference registers those two for CPU).  Extra ops with no reference counterpart live in the same namespace:
 apply_within_adjoint, apply_within_plain, single_relative_keyed.  apply_within is differentiable
 with respect to psi_i through apply_within_adjoint.
 
@@ -74,8 +75,12 @@ def _check(cond: bool, message: str) -> None:
 
 def _check_tensor(t: torch.Tensor, name: str, device: torch.device, dtype: torch.dtype, cols: int | None,
                   rows: int | None = None, rows_name: str = "batch size") -> None:
-    """Checks of qmb/_hamiltonian_cuda.cu:186-232 (device, contiguity, dtype, rank, widths)."""
-    _check(t.device.type == "cuda", f"{name} must be on CUDA.")
+    """Checks of qmb/_hamiltonian_cuda.cu:186-232 (CPU: qmb/_hamiltonian_cpu.cpp:178-224): device,
+    contiguity, dtype, rank, widths."""
+    if device.type == "cpu":
+        _check(t.device.type == "cpu", f"{name} must be on CPU.")
+    else:
+        _check(t.device.type == "cuda", f"{name} must be on CUDA.")
     _check(t.device == device, f"{name} must be on the same device as others.")
     _check(t.is_contiguous(), f"{name} must be contiguous.")
     _check(t.dtype == dtype, f"{name} must be {str(dtype).replace('torch.', '')}.")
@@ -320,6 +325,68 @@ def _single_relative_seeded(nq: int, pc: int, configs: torch.Tensor, site: torch
     return _single_relative_impl(nq, pc, configs, site, kind, coef, seed, None)
 
 
+# -------------------------------------------------------------------------------- CPU impls -----
+# The CPU dispatch key's implementations (qmb/_hamiltonian_cpu.cpp:558-561 registers apply_within
+# and find_relative for CPU): the library's host operators on the tensors' host memory, on torch's
+# intra-op thread count.  They return when the result is written, like the reference's.
+def _host_threads() -> int:
+    return max(1, torch.get_num_threads())
+
+
+def _apply_within_cpu(nq: int, pc: int, configs_i: torch.Tensor, psi_i: torch.Tensor, configs_j: torch.Tensor, site: torch.Tensor,
+                      kind: torch.Tensor, coef: torch.Tensor) -> torch.Tensor:
+    device = configs_i.device
+    _check(device.type == "cpu", "configs must be on CPU.")
+    n_i = configs_i.size(0)
+    n_j = configs_j.size(0)
+    _check_tensor(configs_i, "configs", device, torch.uint8, nq, n_i)
+    _check_tensor(psi_i, "psi", device, torch.float64, 2, n_i)
+    _check_tensor(configs_j, "result_configs", device, torch.uint8, nq, n_j)
+    n_terms = _check_terms(site, kind, coef, device, nq)
+    out = torch.empty((n_j, 2), dtype=torch.float64)
+    _lib.check(_lib.library().qmb_cpu_apply_within(nq, pc, _ptr(configs_i), _ptr(psi_i), n_i, _ptr(configs_j), n_j, _ptr(site),
+                                                   _ptr(kind), _ptr(coef), n_terms, _ptr(out), _host_threads()),
+               "qmb_cpu_apply_within")
+    return out
+
+
+def _apply_within_adjoint_cpu(nq: int, pc: int, configs_i: torch.Tensor, grad_j: torch.Tensor, configs_j: torch.Tensor,
+                              site: torch.Tensor, kind: torch.Tensor, coef: torch.Tensor) -> torch.Tensor:
+    device = configs_i.device
+    _check(device.type == "cpu", "configs must be on CPU.")
+    n_i = configs_i.size(0)
+    n_j = configs_j.size(0)
+    _check_tensor(configs_i, "configs", device, torch.uint8, nq, n_i)
+    _check_tensor(configs_j, "result_configs", device, torch.uint8, nq, n_j)
+    _check_tensor(grad_j, "grad", device, torch.float64, 2, n_j)
+    n_terms = _check_terms(site, kind, coef, device, nq)
+    out = torch.empty((n_i, 2), dtype=torch.float64)
+    _lib.check(_lib.library().qmb_cpu_apply_within_adjoint(nq, pc, _ptr(configs_i), n_i, _ptr(configs_j), _ptr(grad_j), n_j,
+                                                           _ptr(site), _ptr(kind), _ptr(coef), n_terms, _ptr(out), _host_threads()),
+               "qmb_cpu_apply_within_adjoint")
+    return out
+
+
+def _find_relative_cpu(nq: int, pc: int, configs_i: torch.Tensor, psi_i: torch.Tensor, count_selected: int, site: torch.Tensor,
+                       kind: torch.Tensor, coef: torch.Tensor, configs_exclude: torch.Tensor) -> torch.Tensor:
+    device = configs_i.device
+    _check(device.type == "cpu", "configs must be on CPU.")
+    n_i = configs_i.size(0)
+    n_x = configs_exclude.size(0)
+    _check_tensor(configs_i, "configs", device, torch.uint8, nq, n_i)
+    _check_tensor(psi_i, "psi", device, torch.float64, 2, n_i)
+    _check_tensor(configs_exclude, "configs", device, torch.uint8, nq, n_x)
+    n_terms = _check_terms(site, kind, coef, device, nq)
+    _check(count_selected >= 0, "count_selected must be non-negative.")
+    out = torch.empty((count_selected, nq), dtype=torch.uint8)
+    rows = ctypes.c_int64(0)
+    _lib.check(_lib.library().qmb_cpu_find_relative(nq, pc, _ptr(configs_i), _ptr(psi_i), n_i, count_selected, _ptr(site), _ptr(kind),
+                                                    _ptr(coef), n_terms, _ptr(configs_exclude), n_x, _ptr(out), ctypes.byref(rows),
+                                                    _host_threads()),
+               "qmb_cpu_find_relative")
+    return out[:rows.value].clone()
+
+
 # ---------------------------------------------------------------------------- registration -----
 SCHEMAS = {
     "apply_within": "apply_within(Tensor configs_i, Tensor psi_i, Tensor configs_j, Tensor site, Tensor kind, Tensor coef) -> Tensor",
@@ -333,6 +400,13 @@ SCHEMAS = {
                            "Tensor coef) -> Tensor"),
     "single_relative_keyed": "single_relative_keyed(Tensor configs, Tensor site, Tensor kind, Tensor coef, Tensor keys) -> Tensor",
     "single_relative_seeded": "single_relative_seeded(Tensor configs, Tensor site, Tensor kind, Tensor coef, int seed) -> Tensor",
+}
+
+# impls on the CPU dispatch key (the others raise there, as the reference's do)
+CPU_IMPLS = {
+    "apply_within": _apply_within_cpu,
+    "find_relative": _find_relative_cpu,
+    "apply_within_adjoint": _apply_within_adjoint_cpu,
 }
 
 IMPLS = {
@@ -366,6 +440,8 @@ def load(n_qubytes: int, particle_cut: int) -> object:
             for op, schema in SCHEMAS.items():
                 lib.define(schema)
                 lib.impl(op, functools.partial(IMPLS[op], n_qubytes, particle_cut), "CUDA")
+                if op in CPU_IMPLS:
+                    lib.impl(op, functools.partial(CPU_IMPLS[op], n_qubytes, particle_cut), "CPU")
             _REGISTRY[name] = lib
             ns = getattr(torch.ops, name)
 

@@ -169,6 +169,9 @@ class PreparedTargets:
                                              ctypes.byref(self.desc), _stream(device)), "qmb_targets_build")
         self._duplicates: int | None = 0 if distinct else None
         self._max_occupied = max_occupied
+        # a bound from the caller is checked against the rows once per core index (a measured one
+        # holds by construction)
+        self._occupied_hint = max_occupied is not None
         self._cores: dict[int, tuple[torch.Tensor, _lib.QmbCores]] = {}
 
     def duplicates(self) -> int:
@@ -205,8 +208,18 @@ class PreparedTargets:
             with torch.cuda.device(self.device):
                 _lib.check(lib.qmb_cores_build(ctypes.byref(self.desc), occ, order, _ptr(buf), buf.numel(), ctypes.byref(desc),
                                                _stream(self.device)), "qmb_cores_build")
+            if self._occupied_hint:
+                check_core_overflow(_device_count(buf, desc.counts + 16), occ)
             self._cores[order] = (buf, desc)
         return self._cores[order][1]
+
+
+def check_core_overflow(overflow: int, max_occupied: int) -> None:
+    """Raises when a core index found rows with more occupied sites than the bound it was built
+    for (the kernels would leave those rows' pair and single terms out)."""
+    if overflow != 0:
+        raise RuntimeError(f"{overflow} configs have more than max_occupied={max_occupied} occupied sites; "
+                           "pass a larger max_occupied or None to measure it")
 
 
 def can_use_pairs(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tensor) -> bool:

@@ -119,7 +119,12 @@ def vmc_cycle(model: object, network: torch.nn.Module, optimizer: torch.optim.Op
     """One cycle of qmb/vmc.py:80-125: sample, (optionally) extend with find_relative, then
     `local_step` optimizer steps.  rayleigh=True steps on the full Rayleigh quotient over the sampled
     set (energy_backward, the adjoint path); False on the reference's closure over (configs_src,
-    configs_dst).  Logging, tensorboard and checkpoints stay with the caller (`callback(step, energy)`)."""
+    configs_dst).  relative_count applies to the closure only: with rayleigh=True a nonzero
+    relative_count raises ValueError.  Logging, tensorboard and checkpoints stay with the caller
+    (`callback(step, energy)`)."""
+    if rayleigh and relative_count > 0:
+        raise ValueError("relative_count extends the closure's target set (rayleigh=False); the Rayleigh-quotient step "
+                         "(rayleigh=True) works on the sampled set alone")
     generate = getattr(network, "generate_unique")
     with torch.no_grad():  # psi_i feeds find_relative only; the losses evaluate the network again
         configs_i, psi_i, _, _ = generate(sampling_count, seed=seed) if seed is not None else generate(sampling_count)

@@ -204,3 +204,23 @@ def test_op_structured_path_cache(cuda_device: torch.device) -> None:
     (ns.apply_within(cfg_t, x, cfg_t, st, kt, ct) * g).sum().backward()
     ref = ns.apply_within_adjoint(cfg_t, g, cfg_t.clone(), st, kt, ct)
     torch.testing.assert_close(x.grad, ref, rtol=0, atol=1e-12 * float(ref.abs().max()))
+
+
+def test_within_operator_sees_in_place_coefficient_change(cuda_device: torch.device) -> None:
+    """Hamiltonian.prepared_terms (behind the Krylov WithinOperator) keys its compiled table on the
+    term tensors' versions too: after ham.coef.mul_(2) a new operator applies the doubled table."""
+    from qmb_hip import Hamiltonian  # pylint: disable=import-outside-toplevel
+    from qmb_hip import krylov as kr  # pylint: disable=import-outside-toplevel
+
+    s, k, c = sorted_terms(*sy.molecular_terms(4, 11))
+    ham = Hamiltonian((torch.from_numpy(s), torch.from_numpy(k), torch.from_numpy(c.copy())), kind="fermi")
+    cfg = sy.sector_configs(4, 2, 2, 36, 14)
+    psi = sy.random_psi(36, 15)
+    cfg_t = _to(cfg, cuda_device)
+    psi_c = torch.from_numpy(psi).to(cuda_device)
+    before = kr.WithinOperator(ham, cfg_t)(psi_c).cpu().numpy()
+    want, asum, _ = orc.apply_within(1, cfg, psi, cfg, s, k, c)
+    _assert_close(before, want, asum, "before the change")
+    ham.coef.mul_(2.0)
+    after = kr.WithinOperator(ham, cfg_t)(psi_c).cpu().numpy()
+    _assert_close(after, 2 * want, 2 * asum, "after ham.coef.mul_(2)")

@@ -237,3 +237,18 @@ def test_apply_across_matches_oracle(cuda_device: torch.device, case: str) -> No
     got_a = adj.cpu().numpy()
     scale = np.abs(ref).sum(axis=1) + np.abs(c).sum(axis=1).max() * np.abs(g).sum()
     assert np.all(np.abs(got_a - ref).max(axis=1) <= 1e-12 * scale), case
+
+
+def test_too_low_max_occupied_raises(cuda_device: torch.device) -> None:
+    """A max_occupied bound below a row's occupancy is an error from the core index build, not a
+    NaN row or missing pair terms."""
+    s, k, c = sorted_terms(*sy.molecular_terms(6, 21))
+    cfg = torch.from_numpy(sy.sector_configs(6, 3, 3, 120, 22)).to(cuda_device)
+    terms = pp.PreparedTerms(torch.from_numpy(s).to(cuda_device), torch.from_numpy(k).to(cuda_device),
+                             torch.from_numpy(c).to(cuda_device), 2, 1)
+    vec = torch.zeros((cfg.size(0), 2), dtype=torch.float64, device=cuda_device)
+    low = pp.PreparedTargets(cfg, 2, distinct=True, max_occupied=5)
+    with pytest.raises(RuntimeError, match="max_occupied=5"):
+        pp.apply(terms, low, cfg, vec, pp.MODE_FORWARD)
+    exact = pp.PreparedTargets(cfg, 2, distinct=True, max_occupied=6)
+    assert torch.isfinite(pp.apply(terms, exact, cfg, vec, pp.MODE_FORWARD)).all()

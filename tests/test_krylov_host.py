@@ -29,13 +29,13 @@ def test_tridiagonal_lowest_pair() -> None:
     t = np.diag(a) + np.diag(b, 1) + np.diag(b, -1)
     vals, vecs = np.linalg.eigh(t)
     basis = [torch.from_numpy(rng.normal(size=n) + 1j * rng.normal(size=n)) for _ in range(m)]
-    energy, psi = kr.eigh_tridiagonal_lowest([torch.tensor(x) for x in a], [torch.tensor(x) for x in b], basis, torch.device("cpu"))
+    energy, psi = kr.lowest_ritz_pair([torch.tensor(x) for x in a], [torch.tensor(x) for x in b], basis, torch.device("cpu"))
     assert abs(float(energy) - vals[0]) <= 1e-12 * np.abs(vals).max()
     want = sum(vecs[k, 0] * basis[k].numpy() for k in range(m))
     # eigenvectors are defined up to sign
     sign = np.sign(np.vdot(want, psi.numpy()).real)
     np.testing.assert_allclose(sign * psi.numpy(), want, atol=1e-10)
-    e1, p1 = kr.eigh_tridiagonal_lowest([torch.tensor(1.5)], [], basis[:1], torch.device("cpu"))
+    e1, p1 = kr.lowest_ritz_pair([torch.tensor(1.5)], [], basis[:1], torch.device("cpu"))
     assert float(e1) == 1.5 and p1 is basis[0]
 
 

@@ -1,7 +1,7 @@
 """
 The op layer on a host without a GPU: `ops.load` registers the reference's namespace with its four
-schemas word for word (qmb/_hamiltonian.cpp:83-88), a call on CPU tensors fails loudly -- the ops
-have a GPU kernel only and no CPU fallback -- and without the HIP library loading the ops raises.
+schemas word for word (qmb/_hamiltonian.cpp:83-88), the GPU-only ops raise on CPU tensors, and
+without the HIP library loading the ops raises.
 """
 
 from __future__ import annotations
@@ -35,17 +35,19 @@ def test_namespace_and_reference_schemas(nq: int, pc: int) -> None:
         assert str(getattr(ns, name).default._schema) == f"qmb_hamiltonian_{nq}_{pc}::{schema}"  # pylint: disable=protected-access
 
 
-def test_cpu_tensors_have_no_kernel() -> None:
+def test_gpu_only_ops_have_no_cpu_kernel() -> None:
+    """diagonal_term and single_relative exist for the GPU only, as in the reference
+    (qmb/_hamiltonian_cuda.cu:1008-1013); apply_within and find_relative run on CPU tensors
+    (tests/test_cpu_ops.py)."""
     ns = ops.load(1, 1)
     configs = torch.tensor([[0b011], [0b101]], dtype=torch.uint8)
-    psi = torch.zeros((2, 2), dtype=torch.float64)
     site = torch.tensor([[0, 1, 0, 0]], dtype=torch.int16)
     kind = torch.tensor([[1, 0, 2, 2]], dtype=torch.uint8)
     coef = torch.tensor([[1.0, 0.0]], dtype=torch.float64)
     with pytest.raises(NotImplementedError):
-        ns.apply_within(configs, psi, configs, site, kind, coef)
-    with pytest.raises(NotImplementedError):
         ns.diagonal_term(configs, site, kind, coef)
+    with pytest.raises(NotImplementedError):
+        ns.single_relative(configs, site, kind, coef)
 
 
 def test_load_rejects_unsupported_shapes() -> None:
