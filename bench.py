@@ -163,36 +163,16 @@ def cpu_baseline(w: dict[str, object], m_hit_sample: int) -> dict[str, object]:
 
 
 def build_batch_instances(seed0: int = 4000, count: int = 64) -> list[dict[str, object]]:
-    """Seeded molecular instances: spin orbitals uniform over the even numbers in [24, 80], ~30%
-    filling split alpha/beta, |(pq|rs)| < 1e-10 dropped, config-set sizes log-uniform in
-    [1e3, 5e5] (capped by the sector) built as HF + singles + doubles + triples/quadruples."""
-    import math  # pylint: disable=import-outside-toplevel
-
-    import numpy as np  # pylint: disable=import-outside-toplevel
-
+    """The batch64 workload (qmb_hip.synthetic.batch_instances), built on up to 16 host threads."""
     from qmb_hip import synthetic as sy  # pylint: disable=import-outside-toplevel
 
-    rng = np.random.default_rng(seed0)
-    out = []
-    for k in range(count):
-        n_so = 2 * int(rng.integers(12, 41))
-        n_orb = n_so // 2
-        n_e = int(round(0.3 * n_so))
-        n_a, n_b = (n_e + 1) // 2, n_e // 2
-        n = int(round(math.exp(rng.uniform(math.log(1e3), math.log(5e5)))))
-        n = min(n, math.comb(n_orb, n_a) * math.comb(n_orb, n_b))
-        site, kind, coef = sy.molecular_terms(n_orb, seed0 + k, threshold=1e-10)
-        order = np.argsort(-np.linalg.norm(coef.numpy(), axis=1), kind="stable")
-        try:
-            configs = sy.excitation_configs(n_orb, n_a, n_b, n, seed0 + 1000 + k)
-        except ValueError:
-            configs = sy.sector_configs(n_orb, n_a, n_b, n, seed0 + 1000 + k)
-        out.append({"site": site[order], "kind": kind[order], "coef": coef[order], "configs": configs, "n_so": n_so,
-                    "electrons": (n_a, n_b), "psi": sy.random_psi(len(configs), seed0 + 2000 + k),
-                    "grad": sy.random_psi(len(configs), seed0 + 3000 + k)})
-        if (k + 1) % 8 == 0:
-            log(f"batch64: generated {k + 1}/{count} instances")
-    return out
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+
+    def progress(done: int, total: int) -> None:
+        if done % 8 == 0:
+            log(f"batch64: generated {done}/{total} instances")
+
+    return sy.batch_instances(seed0, count, threads, progress)
 
 
 def run_batch(args: argparse.Namespace, dev: object) -> int:

@@ -157,6 +157,24 @@ def single_relative(particle_cut: int, configs: np.ndarray, site: object, kind: 
     return out
 
 
+def best_weights(particle_cut: int, configs_i: np.ndarray, psi_i: object, site: object, kind: object, coef: object,
+                 configs_t: np.ndarray) -> np.ndarray:
+    """Per row of configs_t, the largest single-contribution weight |sign coef psi|^2 landing on it
+    (-1 where none does)."""
+    L = lib()
+    fn = L.oracle_best_weights
+    fn.argtypes = [_int, _int, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _i64, _vp]
+    fn.restype = _int
+    ci = _c(configs_i, np.uint8)
+    ct = _c(configs_t, np.uint8)
+    pi = _psi(psi_i)
+    s, k, c = _terms(site, kind, coef)
+    best = np.zeros(len(ct), dtype=np.float64)
+    if fn(ct.shape[1], particle_cut, _p(ci), _p(pi), len(ci), _p(s), _p(k), _p(c), len(s), _p(ct), len(ct), _p(best)) != 0:
+        raise ValueError("oracle_best_weights rejected its arguments")
+    return best
+
+
 def count_connections(particle_cut: int, configs_i: np.ndarray, configs_j: np.ndarray, site: object, kind: object) -> tuple[int, int]:
     ci = _c(configs_i, np.uint8)
     cj = _c(configs_j, np.uint8)

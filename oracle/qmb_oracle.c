@@ -28,6 +28,11 @@
  *                               connectivity.
  *   apply_within_threads      : (timing only, no reference counterpart) equal to apply_within:
  *                               bit for bit on one thread, to rounding and with equal counts on more.
+ *   best_weights              : the per-target maximum of a brute-force enumeration of every
+ *                               contribution's weight (tests/brute.py) on the golden cases.
+ *   count_connections         : the closed-form M_gen per config of normal-ordered molecular
+ *                               tables, 1 + sum_s (n_s(n - n_s) + n_s) + sum_s C(n_s,2) C(n - n_s + 2, 2)
+ *                               + n_a n_b (n - n_a + 1)(n - n_b + 1), and M_hit = the in-set count.
  *
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, no fast-math, so results are portable).
  */
@@ -517,6 +522,52 @@ int oracle_single_relative(int nq, int particle_cut, const uint8_t* configs, int
                 found = 1;
                 best = keys[t];
                 memcpy(out + i * nq, cur, (size_t)nq);
+            }
+        }
+    }
+    free(order);
+    free(sorted);
+    return 0;
+}
+
+/*
+ * The largest single-contribution weight |sign * coef_t * psi_i|^2 (the quantity find_relative ranks,
+ * qmb/_hamiltonian_cpu.cpp:389-392) reaching each row of configs_t, over every (term, config_i) pair
+ * whose application lands there; best[k] = -1 where none does.  Same loops as find_relative with
+ * a max per target instead of the heap.  Used to decide whether a config that two find_relative
+ * results disagree on sits in the tie class at the K-th weight.
+ */
+int oracle_best_weights(int nq, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i, const int16_t* site,
+                        const uint8_t* kind, const double* coef, int64_t n_terms, const uint8_t* configs_t, int64_t n_t, double* best) {
+    if (nq <= 0 || nq > MAX_NQ) {
+        return 1;
+    }
+    int64_t* order = sorted_order(configs_t, n_t, nq);
+    uint8_t* sorted = gather_rows(configs_t, order, n_t, nq);
+    for (int64_t k = 0; k < n_t; ++k) {
+        best[k] = -1.0;
+    }
+    uint8_t cur[MAX_NQ];
+    for (int64_t t = 0; t < n_terms; ++t) {
+        for (int64_t i = 0; i < n_i; ++i) {
+            memcpy(cur, configs_i + i * nq, (size_t)nq);
+            int parity = 0;
+            if (!oracle_apply_term(cur, site + t * MAX_OP, kind + t * MAX_OP, particle_cut, &parity)) {
+                continue;
+            }
+            int64_t pos = bsearch_sorted(sorted, n_t, nq, cur);
+            if (pos < 0) {
+                continue;
+            }
+            double sign = parity ? -1.0 : 1.0;
+            double c0 = coef[2 * t], c1 = coef[2 * t + 1];
+            double p0 = psi_i[2 * i], p1 = psi_i[2 * i + 1];
+            double re = sign * (c0 * p0 - c1 * p1);
+            double im = sign * (c0 * p1 + c1 * p0);
+            double w = re * re + im * im;
+            int64_t row = order[pos];
+            if (w > best[row]) {
+                best[row] = w;
             }
         }
     }

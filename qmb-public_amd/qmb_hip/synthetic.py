@@ -48,10 +48,19 @@ def sort_unique_configs(configs: np.ndarray) -> np.ndarray:
     configs = np.ascontiguousarray(configs, dtype=np.uint8)
     if len(configs) == 0:
         return configs
-    nq = configs.shape[1]
-    v = configs.view(np.dtype((np.void, nq))).ravel()
-    u = np.unique(v)
-    return u.view(np.uint8).reshape(-1, nq).copy()
+    n, nq = configs.shape
+    w = (nq + 7) // 8
+    padded = np.zeros((n, 8 * w), dtype=np.uint8)
+    padded[:, :nq] = configs
+    keys = padded.view(">u8").reshape(n, w).astype(np.uint64)  # big-endian: byte 0 most significant
+    if w == 1:
+        u = np.unique(keys[:, 0]).astype(">u8")
+        return u.view(np.uint8).reshape(-1, 8)[:, :nq].copy()
+    order = np.lexsort(keys.T[::-1])  # word 0 is the primary key
+    sk = keys[order]
+    keep = np.ones(n, dtype=bool)
+    keep[1:] = (sk[1:] != sk[:-1]).any(axis=1)
+    return configs[order[keep]].copy()
 
 
 def random_psi(n: int, seed: int) -> np.ndarray:
@@ -348,26 +357,31 @@ def excitation_configs(n_orbit: int, n_alpha: int, n_beta: int, count: int, seed
     vir_b = np.arange(n_beta, n_orbit)
     rows = [hf[None, :]]
 
-    def excite(base: np.ndarray, holes: list[int], parts: list[int]) -> np.ndarray:
-        c = base.copy()
-        c[holes] = 0
-        c[parts] = 1
+    def excite(holes: np.ndarray, parts: np.ndarray) -> np.ndarray:
+        """hf with, per row, the sites holes[r] emptied and parts[r] filled."""
+        c = np.tile(hf, (len(holes), 1))
+        r = np.arange(len(holes))[:, None]
+        c[r, holes] = 0
+        c[r, parts] = 1
         return c
 
-    import itertools
+    def pairs(x: np.ndarray) -> np.ndarray:
+        i, j = np.triu_indices(len(x), 1)
+        return np.stack([x[i], x[j]], axis=1)
+
+    def product(a: np.ndarray, b: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        """Every (row of a, row of b) combination, a-major."""
+        return np.repeat(a, len(b), axis=0), np.tile(b, (len(a), 1))
 
     for spin, occ, vir in ((0, occ_a, vir_a), (1, occ_b, vir_b)):
-        for i in occ:
-            for a in vir:
-                rows.append(excite(hf, [2 * i + spin], [2 * a + spin])[None, :])
-        for i, j in itertools.combinations(occ, 2):
-            for a, b in itertools.combinations(vir, 2):
-                rows.append(excite(hf, [2 * i + spin, 2 * j + spin], [2 * a + spin, 2 * b + spin])[None, :])
-    for i in occ_a:
-        for a in vir_a:
-            for j in occ_b:
-                for b in vir_b:
-                    rows.append(excite(hf, [2 * i, 2 * j + 1], [2 * a, 2 * b + 1])[None, :])
+        h1, p1 = product((2 * occ + spin)[:, None], (2 * vir + spin)[:, None])
+        rows.append(excite(h1, p1))
+        h2, p2 = product(2 * pairs(occ) + spin, 2 * pairs(vir) + spin)
+        rows.append(excite(h2, p2))
+    ia, ja = product((2 * occ_a)[:, None], (2 * vir_a)[:, None])
+    ib, jb = product((2 * occ_b + 1)[:, None], (2 * vir_b + 1)[:, None])
+    holes_ab, parts_ab = product(np.concatenate([ia, ja], axis=1), np.concatenate([ib, jb], axis=1))
+    rows.append(excite(np.stack([holes_ab[:, 0], parts_ab[:, 0]], axis=1), np.stack([holes_ab[:, 1], parts_ab[:, 1]], axis=1)))
     base = sort_unique_configs(pack_occupation(np.concatenate(rows)))
     if len(base) >= count:
         keep = np.sort(rng.choice(len(base), size=count, replace=False))
@@ -433,3 +447,45 @@ def hubbard_chain_configs(n_site: int, count: int, seed: int, n_seed_configs: in
         return allc
     keep = np.sort(rng.choice(len(allc), size=count, replace=False))
     return allc[keep]
+
+
+def batch_instances(seed0: int = 4000, count: int = 64, threads: int = 8,
+                    progress: typing.Callable[[int, int], None] | None = None) -> list[dict[str, object]]:
+    """Seeded molecular instances of uneven sizes: spin orbitals uniform over the even numbers in
+    [24, 80], ~30% filling split alpha/beta, |(pq|rs)| < 1e-10 dropped, config-set sizes
+    log-uniform in [1e3, 5e5] (capped by the sector) built as HF + singles + doubles +
+    triples/quadruples.  Instance k uses seeds seed0 + k (terms), + 1000 + k (configs), + 2000 + k
+    (psi) and + 3000 + k (a second vector); its terms are ordered by |coef| descending (stable).
+    The shapes are drawn first from one generator, then the instances are built on `threads`
+    host threads (the result does not depend on the thread count)."""
+    import concurrent.futures  # pylint: disable=import-outside-toplevel
+
+    rng = np.random.default_rng(seed0)
+    specs = []
+    for k in range(count):
+        n_so = 2 * int(rng.integers(12, 41))
+        n_orb = n_so // 2
+        n_e = int(round(0.3 * n_so))
+        n_a, n_b = (n_e + 1) // 2, n_e // 2
+        n = int(round(math.exp(rng.uniform(math.log(1e3), math.log(5e5)))))
+        specs.append((k, n_orb, n_a, n_b, min(n, math.comb(n_orb, n_a) * math.comb(n_orb, n_b))))
+    done = [0]
+
+    def make(spec: tuple[int, int, int, int, int]) -> dict[str, object]:
+        k, n_orb, n_a, n_b, n = spec
+        site, kind, coef = molecular_terms(n_orb, seed0 + k, threshold=1e-10)
+        order = np.argsort(-np.linalg.norm(coef.numpy(), axis=1), kind="stable")
+        try:
+            configs = excitation_configs(n_orb, n_a, n_b, n, seed0 + 1000 + k)
+        except ValueError:
+            configs = sector_configs(n_orb, n_a, n_b, n, seed0 + 1000 + k)
+        item = {"site": site[order], "kind": kind[order], "coef": coef[order], "configs": configs, "n_so": 2 * n_orb,
+                "electrons": (n_a, n_b), "psi": random_psi(len(configs), seed0 + 2000 + k),
+                "grad": random_psi(len(configs), seed0 + 3000 + k)}
+        done[0] += 1
+        if progress is not None:
+            progress(done[0], count)
+        return item
+
+    with concurrent.futures.ThreadPoolExecutor(max(1, threads)) as pool:
+        return list(pool.map(make, specs))

@@ -72,3 +72,19 @@ def test_find_relative_brute_force(name: str) -> None:
     allowed = strict | {key for w, key in entries if w == kth and w > 0}
     want = {bytes(r) for r in d["result"]}
     assert strict <= want <= allowed, name
+
+
+@pytest.mark.parametrize("name", FIND)
+def test_oracle_best_weights_brute_force(name: str) -> None:
+    """best_weights equals the per-target maximum over a brute-force enumeration of every
+    contribution (exclusion off: the targets are the selected configs and a few excluded ones)."""
+    d = load("fr", name)
+    pc = int(d["particle_cut"])
+    targets = np.concatenate([d["result"], d["configs_exclude"][:5]])
+    got = orc.best_weights(pc, d["configs_i"], d["psi_i"], d["site"], d["kind"], d["coef"], targets)
+    no_exclude = np.zeros((0, targets.shape[1]), dtype=np.uint8)
+    best: dict[bytes, float] = {}
+    for w, key in _entries(pc, d["configs_i"], d["psi_i"], no_exclude, d["site"], d["kind"], d["coef"]):
+        best[key] = max(best.get(key, -1.0), w)
+    want = np.array([best.get(bytes(r), -1.0) for r in targets])
+    assert np.array_equal(got, want), name
