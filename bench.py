@@ -62,13 +62,13 @@ def build_workload(name: str) -> dict[str, object]:
     if name == "molecular52":
         site, kind, coef = sy.molecular_terms(26, 2001, threshold=1e-10)
         configs = sy.excitation_configs(26, 7, 7, 1_000_000, 2002)
-        pc, seeds, sample, occ = 1, (2003, 2004), 1000, 14
+        pc, seeds, sample, occ = 1, (2003, 2004), 300, 14
         desc = {"workload": "molecular52", "spin_orbitals": 52, "electrons": "7a+7b", "n_configs": 1_000_000,
                 "configs": "HF + singles + doubles + seeded triples/quadruples", "threshold": 1e-10}
     elif name == "hubbard128":
         site, kind, coef = sy.terms_from_dict(sy.hubbard_dict(64, 1, 1.0, 4.0))
         configs = sy.hubbard_chain_configs(64, 4_000_000, 3000)
-        pc, seeds, sample, occ = 1, (3001, 3002), 400_000, 64
+        pc, seeds, sample, occ = 1, (3001, 3002), 200_000, 64
         desc = {"workload": "hubbard128", "sites": 64, "spin_orbitals": 128, "t": 1.0, "U": 4.0, "filling": "half",
                 "n_configs": int(len(configs))}
     elif name == "molecular20":
@@ -87,15 +87,67 @@ def build_workload(name: str) -> dict[str, object]:
             "sample": min(sample, len(configs)), "desc": desc, "max_occupied": occ}
 
 
-def read_traffic(workload: str) -> float | None:
-    """HBM bytes per forward launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/."""
-    path = ROOT / "profiles" / f"apply_within_forward_{workload}.md"
+def source_sha() -> str:
+    """SHA-256 (first 16 hex digits) of the library's sources: a profile's counters describe the
+    build they were taken on only."""
+    import hashlib  # pylint: disable=import-outside-toplevel
+
+    h = hashlib.sha256()
+    csrc = ROOT / "qmb-public_amd" / "csrc"
+    for path in sorted(list(csrc.glob("*.hip")) + list(csrc.glob("*.hpp")) + [ROOT / "include" / "qmb_hip.h"]):
+        h.update(path.name.encode())
+        h.update(path.read_bytes())
+    return h.hexdigest()[:16]
+
+
+def read_traffic(workload: str, kernel: str) -> tuple[float | None, float | None, str]:
+    """(HBM bytes per launch, L2 hit rate) of `kernel` from profiles/<kernel>_<workload>.md, taken
+    only when that summary was made on the current sources (its `source_sha:` line)."""
+    path = ROOT / "profiles" / f"{kernel}_{workload}.md"
     if not path.exists():
-        return None
+        return None, None, "no profile"
+    traffic = hit = sha = None
     for line in path.read_text().splitlines():
-        if line.startswith("hbm_bytes_per_forward_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+        if line.startswith("source_sha:"):
+            sha = line.split(":", 1)[1].strip()
+        elif line.startswith("hbm_bytes_per_launch:"):
+            traffic = float(line.split(":", 1)[1])
+        elif line.startswith("l2_hit_rate:"):
+            hit = float(line.split(":", 1)[1])
+    if sha != source_sha():
+        return None, None, f"profile made on other sources ({sha})"
+    return traffic, hit, str(path.relative_to(ROOT))
+
+
+def structured_model(pp: object, terms: object, targets: object, nq: int, n: int) -> dict[str, object]:
+    """Bytes the structured application must move at least once (synchronises to size the core
+    indexes).  Per kernel launch:
+      pair_kernel:   4 per group (its start) + per entry of a group of >= 2 entries: the entry (8 key
+                     + 8 value), its row's config words (8 W) and amplitude (16); the coefficient table
+                     once (8 or 16 per entry of n_pairs^2); the output once (16 per row);
+      single_kernel: the same over the order-1 index, with the single-excitation tables (n_bits^2 +
+                     n_bits^3 entries) once;
+      diag_apply:    per row its config words, amplitude and output (8 W + 32); the folded table once.
+    Per core index build: every entry emitted (16 written), sorted (16 read + 16 written), its group
+    start found (4) -- the sort's extra passes and the hash index are not counted."""
+    words = (nq + 7) // 8
+    per_entry = 16 + 8 * words + 16
+    out: dict[str, object] = {}
+    coef_b = 8 if terms.pairs_desc.real else 16
+    n_bits = 8 * nq
+    st2 = pp.core_stats(targets, 2)
+    out["pair_kernel"] = dict(st2, bytes=4 * st2["groups"] + per_entry * st2["multi_entries"] + coef_b * terms.pairs_desc.n_pairs ** 2
+                              + 16 * n)
+    index_bytes = 36 * st2["entries"] + 4 * st2["groups"]
+    if terms.singles_desc is not None:
+        st1 = pp.core_stats(targets, 1)
+        sb = 8 if terms.singles_desc.real else 16
+        out["single_kernel"] = dict(st1, bytes=4 * st1["groups"] + per_entry * st1["multi_entries"]
+                                    + sb * (n_bits ** 2 + n_bits ** 3) + 16 * n)
+        index_bytes += 36 * st1["entries"] + 4 * st1["groups"]
+    out["diag_apply_kernel"] = {"bytes": n * (8 * words + 32) + 16 * (1 + n_bits + terms.pairs_desc.n_pairs)}
+    out["core_index_build"] = {"bytes": index_bytes}
+    return out
 
 
 def host_cpu_model() -> str:
@@ -109,48 +161,69 @@ def host_cpu_model() -> str:
     return platform.machine()
 
 
-def cpu_baseline(w: dict[str, object], m_hit_sample: int) -> dict[str, object]:
-    """The reference CPU op (oracle/_ref) on the first `sample` configs_i against all of configs_j."""
+def sample_rows(n_total: int, n_sample: int, seed: int = 7001) -> "object":
+    """A seeded random sample of configs_i rows (sorted indices), the CPU baseline's workload."""
+    import numpy as np  # pylint: disable=import-outside-toplevel
+
+    if n_sample >= n_total:
+        return np.arange(n_total)
+    return np.sort(np.random.default_rng(seed).choice(n_total, size=n_sample, replace=False))
+
+
+def cpu_baseline(w: dict[str, object], rows: object, m_hit_sample: int, m_hit_per_row_set: float) -> dict[str, object]:
+    """The reference CPU op (oracle/_ref) applying a seeded random sample of configs_i rows against
+    all of configs_j: one warm-up run, then the median of five (std::chrono-like wall time of each
+    op call).  Beside it, the same sample through the C port split over the host threads."""
     import numpy as np  # pylint: disable=import-outside-toplevel
 
     configs = w["configs"]
-    n = int(w["sample"])
+    sub, sub_psi = configs[rows], w["psi"][rows]
+    n = len(sub)
     nq = configs.shape[1]
     pc = int(w["pc"])
     ref = ROOT / "oracle" / "_ref" / f"qmb_ref_{nq}_{pc}.so"
-    sample_desc = f"apply_within of the first {n} of {len(configs)} configs_i against all {len(configs)} configs_j"
+    sample_desc = (f"apply_within of {n} seeded random configs_i rows against all {len(configs)} configs_j "
+                   f"({m_hit_sample / n:.0f} contributions per sampled row, {m_hit_per_row_set:.0f} per row over the set); "
+                   "1 warm-up + median of 5")
     with tempfile.TemporaryDirectory() as d:
         inp = pathlib.Path(d) / "in.npz"
         out = pathlib.Path(d) / "out.npz"
-        np.savez(inp, configs_i=configs[:n], psi_i=w["psi"][:n], configs_j=configs, site=w["site"].numpy(), kind=w["kind"].numpy(),
+        np.savez(inp, configs_i=sub, psi_i=sub_psi, configs_j=configs, site=w["site"].numpy(), kind=w["kind"].numpy(),
                  coef=w["coef"].numpy(), particle_cut=pc)
         from oracle import oracle as orc  # pylint: disable=import-outside-toplevel
 
-        args = (pc, configs[:n], w["psi"][:n], configs, w["site"].numpy(), w["kind"].numpy(), w["coef"].numpy())
+        args = (pc, sub, sub_psi, configs, w["site"].numpy(), w["kind"].numpy(), w["coef"].numpy())
         result = None
         if ref.exists():
             env = dict(os.environ, OMP_NUM_THREADS="1")
             proc = subprocess.run([sys.executable, str(ROOT / "oracle" / "ref_runner.py"), "apply_within", str(inp), str(out),
-                                   "--repeat", "1", "--threads", "1"], env=env, capture_output=True, text=True, check=False)
+                                   "--repeat", "6", "--threads", "1"], env=env, capture_output=True, text=True, check=False)
             if proc.returncode == 0:
-                t = float(np.load(out)["times"][0])
+                times = np.load(out)["times"]
+                t = float(np.median(times[1:]))
                 result = {"value": m_hit_sample / t, "unit": "matrix elements/s", "cores": 1, "kind": "reference",
                           "sample": sample_desc + " (qmb/_hamiltonian_cpu.cpp built from source, -O3 -ffast-math, 1 thread)",
-                          "seconds": t}
+                          "seconds": t, "runs_s": [float(x) for x in times]}
             else:
                 log(f"reference op failed: {proc.stderr[-500:]}")
         if result is None:
-            t0 = time.perf_counter()
-            orc.apply_within(*args)
-            t = time.perf_counter() - t0
+            times = []
+            for _ in range(6):
+                t0 = time.perf_counter()
+                orc.apply_within(*args)
+                times.append(time.perf_counter() - t0)
+            t = float(np.median(times[1:]))
             result = {"value": m_hit_sample / t, "unit": "matrix elements/s", "cores": 1, "kind": "port",
-                      "sample": sample_desc + " (oracle/qmb_oracle.c, 1 thread)", "seconds": t}
+                      "sample": sample_desc + " (oracle/qmb_oracle.c, 1 thread)", "seconds": t, "runs_s": times}
         # The same sample over the host cores this process may use (at most 16, one GPU's share of
         # the host): the port's slices of configs_i on threads, reported beside the serial figure.
         threads = max(1, min(16, len(os.sched_getaffinity(0))))
-        t0 = time.perf_counter()
-        _, (_, hit) = orc.apply_within_threads(*args, threads)
-        t = time.perf_counter() - t0
+        times, hit = [], 0
+        for _ in range(6):
+            t0 = time.perf_counter()
+            _, (_, hit) = orc.apply_within_threads(*args, threads)
+            times.append(time.perf_counter() - t0)
+        t = float(np.median(times[1:]))
         if hit != m_hit_sample:
             log(f"WARNING: the threaded CPU port found {hit} contributions in the sample, the GPU count {m_hit_sample}")
         result["host_cpu"] = host_cpu_model()
@@ -747,7 +820,13 @@ def main() -> int:
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     parser.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                         help="also run a plain device-to-device copy of BYTES bytes three times (HBM counter calibration)")
+    parser.add_argument("--gather-calibration", type=int, default=0, metavar="ROWS",
+                        help="also gather ROWS random 64-byte rows of a 4 GiB table three times (random-request counter calibration)")
+    parser.add_argument("--source-sha", action="store_true", help="print the library sources' hash (profiles record it) and exit")
     args = parser.parse_args()
+    if args.source_sha:
+        print(source_sha())
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU (asked for {args.gpus}); the kernels run on one GPU per process and are not "
               "sharded across GPUs.", file=sys.stderr)
@@ -885,11 +964,37 @@ def main() -> int:
         cps.append(a.elapsed_time(b))
     copy_gbps = 2 * src.numel() * 8 / (float(np.median(cps)) * 1e-3) / 1e9
     del src, dst
-    algo_bytes = n * (nq + 16) + n * (nq + 16) + n_terms * 28 + m_gen * nq + m_hit * 16
-    achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
+    # The survey's lookup model (one key read per generated connection): the bound of the
+    # enumeration path, reported for the structured path only as a labelled comparison.
+    lookup_bytes = n * (nq + 16) + n * (nq + 16) + n_terms * 28 + m_gen * nq + m_hit * 16
     log(f"M_gen={m_gen} M_hit={m_hit}; forward {kernel_ms:.3f} ms, adjoint {adjoint_ms:.3f} ms, copy {copy_gbps:.0f} GB/s")
+    if use_pairs:
+        model = structured_model(pp, terms, targets, nq, n)
+        kname = "pair_kernel"
+        k_bytes, k_ms = model["pair_kernel"]["bytes"], parts["pair_kernel_ms"]
+        label = f"pair_kernel<{(nq + 7) // 8}, {'true' if symmetric else 'false'}>"
+        app_bytes = sum(v["bytes"] for k, v in model.items() if k != "core_index_build")
+        step_bytes = model["core_index_build"]["bytes"] + 2 * app_bytes
+        model_desc = ("structured path, bytes moved at least once per launch: group starts, the entries of groups of >= 2 "
+                      "entries with their rows' config words and amplitudes, the coefficient table, the output")
+    else:
+        model = None
+        kname = "apply_kernel"
+        k_bytes, k_ms = lookup_bytes, kernel_ms
+        label = f"apply_kernel<{(nq + 7) // 8}, {1 if symmetric else 0}, {1 if direct else 0}>"
+        app_bytes = lookup_bytes
+        step_bytes = 2 * lookup_bytes
+        model_desc = "enumeration path: configs and amplitudes in and out, the term table, one key read per connection, one 16-byte accumulate per contribution"
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9
+    step_gbps = step_bytes / (step_ms * 1e-3) / 1e9
+    if achieved > HBM_PEAK_GBPS or step_gbps > HBM_PEAK_GBPS:
+        print(f"bench.py: refusing to report: the bytes model gives {achieved:.0f} GB/s for {label} and {step_gbps:.0f} GB/s per "
+              f"step, above the {HBM_PEAK_GBPS:.0f} GB/s peak; the model does not bound this path", file=sys.stderr)
+        return 3
+    traffic, l2_hit, traffic_src = read_traffic(args.workload, kname)
+    log(f"roofline: {label} {k_bytes:.4g} B in {k_ms:.3f} ms = {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBPS:.3f} of peak); "
+        f"step {step_bytes:.4g} B = {step_gbps:.0f} GB/s; traffic {traffic} ({traffic_src})")
 
-    traffic = read_traffic(args.workload)
     if args.copy_calibration > 0:
         src = torch.empty(args.copy_calibration // 8, dtype=torch.float64, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
@@ -898,6 +1003,17 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"copy calibration: 3 x {src.numel() * 8} bytes")
         del src, dst
+
+    if args.gather_calibration > 0:
+        # random 64-byte rows of a table far above the 256 MB last-level cache: each row is one
+        # 64-byte HBM read, so FETCH_SIZE over ROWS x (64 + 8 index bytes) calibrates random requests
+        table = torch.empty(((4 << 30) // 64, 8), dtype=torch.float64, device=dev).fill_(1.0)
+        idx = torch.randint(0, table.size(0), (args.gather_calibration,), device=dev, generator=torch.Generator(device=dev).manual_seed(9))
+        for _ in range(3):
+            table.index_select(0, idx)
+        torch.cuda.synchronize()
+        log(f"gather calibration: 3 x {args.gather_calibration} random 64-byte rows of a {table.numel() * 8} byte table")
+        del table, idx
 
     plain = None
     if not args.no_plain:
@@ -921,10 +1037,9 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        s_n = int(w["sample"])
-        sample_targets = targets
-        g_s, h_s = pp.count(terms, sample_targets, configs[:s_n])
-        cpu = cpu_baseline(w, h_s)
+        rows = sample_rows(n, int(w["sample"]))
+        _, h_s = pp.count(terms, targets, configs[torch.from_numpy(rows).to(dev)])
+        cpu = cpu_baseline(w, rows, h_s, m_hit / n)
         cpu["m_hit_sample"] = h_s
         log(f"cpu baseline: {cpu['value']:.3e} elements/s over {cpu['seconds']:.2f}s ({cpu['kind']})")
 
@@ -947,11 +1062,15 @@ def main() -> int:
                        kernel_path=("pair-excitation core groups + " if use_pairs else "")
                        + ("symmetric " if symmetric else "scatter/gather ") + ("direct" if direct else "bucketed tiles")),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": traffic,
-                     "kernel": ((f"pair_kernel<W={(nq + 7) // 8},{'adjoint' if symmetric else 'forward'}> + " if use_pairs else "")
-                                + f"apply_kernel<W={(nq + 7) // 8},{'gather' if symmetric else 'forward'},{'direct' if direct else 'tiles'}>"),
-                     "kernel_ms": kernel_ms, "parts": parts, "copy_gbps": copy_gbps,
-                     "adjoint_kernel_ms": adjoint_ms, "algorithmic_bytes": algo_bytes},
+                     "traffic": traffic, "traffic_source": traffic_src, "l2_hit_rate": l2_hit,
+                     "kernel": label, "kernel_ms": k_ms, "algorithmic_bytes": k_bytes, "model": model_desc,
+                     "model_parts": model, "parts": parts, "copy_gbps": copy_gbps,
+                     "application": {"forward_ms": kernel_ms, "adjoint_ms": adjoint_ms, "bytes": app_bytes,
+                                     "gbps": app_bytes / (kernel_ms * 1e-3) / 1e9},
+                     "step": {"bytes": step_bytes, "gbps": step_gbps},
+                     "lookup_model": {"what": "one key read per generated connection (the enumeration path's bound; the "
+                                              "structured path performs no such lookups, so this is not its roofline)",
+                                      "bytes": lookup_bytes, "gbps_equivalent": lookup_bytes / (kernel_ms * 1e-3) / 1e9}},
         "cpu_baseline": cpu,
         "plain_kernel": plain,
     }

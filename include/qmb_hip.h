@@ -72,7 +72,19 @@ double qmb_philox_uniform_host(uint64_t seed, uint64_t index);
  * sign * coef[t] * psi_i[i]; psi_j is written in configs_j's own row order.
  * Replaces apply_within_interface, qmb/_hamiltonian_cuda.cu:171-274 (CPU: _hamiltonian_cpu.cpp:164-257),
  * bound by qmb/hamiltonian.py:104-131. */
+/* Path: when configs_i and configs_j are the same buffer (same pointer, same row count) and the
+ * workspace is at least qmb_apply_within_structured_workspace_size for the set, the call reads four
+ * counters back from the device (one synchronisation of `stream`, as the reference's op
+ * synchronises after every call) and, when the table has pair excitations, the rows are distinct and
+ * the workspace holds the core indexes for the set's most occupied row, applies the structured path
+ * (pair and single excitations by core groups, the folded diagonal, the remaining terms enumerated).
+ * Otherwise -- two config sets, or a workspace of qmb_apply_within_workspace_size -- it enumerates
+ * every connection with hash lookups and does not synchronise.  Both give the same results (to
+ * FP64 summation order). */
 int qmb_apply_within_workspace_size(int n_qubytes, int64_t n_i, int64_t n_j, int64_t n_terms, size_t* bytes);
+/* Workspace for the structured path on one set of n configs whose rows hold at most max_occupied
+ * occupied sites (e.g. the particle number); n_qubytes <= 16.  No reference counterpart. */
+int qmb_apply_within_structured_workspace_size(int n_qubytes, int64_t n, int64_t n_terms, int32_t max_occupied, size_t* bytes);
 int qmb_apply_within(int n_qubytes, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i,
                      const uint8_t* configs_j, int64_t n_j, const int16_t* site, const uint8_t* kind, const double* coef,
                      int64_t n_terms, double* psi_j, void* workspace, size_t workspace_bytes, void* stream);

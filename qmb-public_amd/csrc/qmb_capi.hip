@@ -570,6 +570,169 @@ int qmb_apply_within_workspace_size(int n_qubytes, int64_t n_i, int64_t n_j, int
     return QMB_OK;
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------- structured path behind the C ABI --
+// qmb_apply_within / qmb_apply_within_adjoint on ONE config set (configs_i and configs_j the same
+// buffer) take the structured path -- pair and single excitations applied by walking core groups,
+// diagonal terms from the folded table, the rest enumerated -- when the table has pair excitations,
+// the rows are distinct and the workspace holds the core indexes (qmb_apply_within_structured_
+// workspace_size).  Choosing needs four numbers from the device (pair terms, rows repeated, the
+// most occupied sites in a row, coefficients with an imaginary part); they are read back with
+// one synchronisation of the stream, as the reference's op synchronises after every call.
+namespace qmb {
+namespace {
+
+struct StructuredLayout {
+    size_t terms, targets, pairs, singles, residual, cores2, cores1, total;
+};
+
+StructuredLayout structured_layout(int nq, int64_t n, int64_t n_terms, int max_occupied) {
+    StructuredLayout l{};
+    l.terms = terms_bytes(nq, n_terms);
+    l.targets = targets_bytes(nq, n);
+    l.pairs = pairs_bytes(nq);
+    l.singles = singles_bytes(nq);
+    l.residual = terms_bytes(nq, n_terms);
+    l.cores2 = cores_bytes(nq, n, max_occupied, 2);
+    l.cores1 = cores_bytes(nq, n, max_occupied, 1);
+    l.total = l.terms + l.targets + l.pairs + l.singles + l.residual + l.cores2 + l.cores1 + 8 * 256 + 1024;
+    return l;
+}
+
+// plan[0] pair terms, [1] single terms, [2] repeated rows, [3] most occupied sites in a row,
+// [4] coefficients with a nonzero imaginary part, [5] terms outside the three structured classes
+__global__ void plan_zero_kernel(int64_t* plan) {
+    if (threadIdx.x < 8) {
+        plan[threadIdx.x] = 0;
+    }
+}
+
+__global__ void plan_scan_kernel(const uint8_t* __restrict__ configs, int64_t n, int nq, const double* __restrict__ coef,
+                                 int64_t n_terms, int64_t* __restrict__ plan) {
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    int occ = 0;
+    if (i < n) {
+        const uint8_t* row = configs + i * nq;
+        for (int b = 0; b < nq; ++b) {
+            occ += __popc(static_cast<unsigned>(row[b]));
+        }
+    }
+    // one atomic per wave for the maximum
+    for (int off = 32; off > 0; off >>= 1) {
+        occ = max(occ, __shfl_xor(occ, off));
+    }
+    if ((threadIdx.x & 63) == 0 && occ > 0) {
+        atomicMax(reinterpret_cast<unsigned long long*>(&plan[3]), static_cast<unsigned long long>(occ));
+    }
+    const unsigned long long imag = (i < n_terms && coef[2 * i + 1] != 0.0) ? 1ull : 0ull;
+    const unsigned long long waves = __popcll(__ballot(imag != 0ull));
+    if ((threadIdx.x & 63) == 0 && waves > 0) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&plan[4]), waves);
+    }
+}
+
+__global__ void plan_gather_kernel(const int64_t* __restrict__ pair_counts, const int64_t* __restrict__ single_counts,
+                                   const int64_t* __restrict__ duplicates, const int64_t* __restrict__ residual_counts,
+                                   int64_t* __restrict__ plan) {
+    if (threadIdx.x == 0) {
+        plan[0] = pair_counts[0];
+        plan[1] = single_counts[0];
+        plan[2] = duplicates[0];
+        plan[5] = residual_counts[0];
+    }
+}
+
+// 0: applied through the structured path; 1: not applicable (the caller enumerates); else an error.
+int apply_structured(int nq, int pc, const uint8_t* configs, int64_t n, const int16_t* site, const uint8_t* kind,
+                     const double* coef, int64_t n_terms, const double* in, double* out, int mode, void* workspace,
+                     size_t workspace_bytes, hipStream_t s) {
+    // the smallest layout (two occupied sites) decides whether to look at all
+    if (nq > kPairMaxQubytes || n <= 0 || n_terms <= 0 || workspace_bytes < structured_layout(nq, n, n_terms, 2).total) {
+        return 1;
+    }
+    const StructuredLayout lo = structured_layout(nq, n, n_terms, 2);
+    Regions r{static_cast<char*>(workspace), 0, workspace_bytes};
+    void* terms_buf = r.next(lo.terms);
+    qmb_targets targets{};
+    QMB_TRY(targets_build(nq, configs, n, r.next(lo.targets), lo.targets, &targets, s));
+    qmb_pairs pairs{};
+    QMB_TRY(pairs_build(nq, pc, site, kind, coef, n_terms, r.next(lo.pairs), lo.pairs, &pairs, s));
+    qmb_singles singles{};
+    QMB_TRY(singles_build(nq, pc, site, kind, coef, n_terms, r.next(lo.singles), lo.singles, &singles, s));
+    // the terms outside the three classes (none for a molecular table); excluding single
+    // excitations is a no-op when the table has none
+    qmb_terms residual{};
+    QMB_TRY(terms_build(nq, pc, site, kind, coef, n_terms, r.next(lo.residual), lo.residual, &residual, s,
+                        kExcludePairs | kExcludeDiagonal | kExcludeSingles));
+    int64_t* plan = static_cast<int64_t*>(r.next(8 * sizeof(int64_t)));
+    const size_t cores_off = r.off;
+    plan_zero_kernel<<<1, 64, 0, s>>>(plan);
+    QMB_CHECK_LAUNCH();
+    const int64_t reach = std::max(n, n_terms);
+    plan_scan_kernel<<<static_cast<unsigned>((reach + 255) / 256), 256, 0, s>>>(configs, n, nq, coef, n_terms, plan);
+    QMB_CHECK_LAUNCH();
+    plan_gather_kernel<<<1, 64, 0, s>>>(pairs.counts, singles.counts, targets.duplicates, residual.counts, plan);
+    QMB_CHECK_LAUNCH();
+    int64_t host[8] = {};
+    QMB_HIP_TRY(hipMemcpyAsync(host, plan, 6 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    QMB_HIP_TRY(hipStreamSynchronize(s));
+    const int64_t n_pair = host[0], n_single = host[1], repeated = host[2], occ = host[3], n_imag = host[4], n_residual = host[5];
+    if (n_pair == 0 || repeated != 0 || occ < 2) {
+        return 1;
+    }
+    const StructuredLayout full = structured_layout(nq, n, n_terms, static_cast<int>(occ));
+    if (workspace_bytes < full.total) {
+        return 1;
+    }
+    (void)terms_buf;
+    if (n_imag == 0) {
+        QMB_TRY(pairs_set_real(&pairs, 1, s));
+        if (n_single > 0) {
+            QMB_TRY(singles_set_real(&singles, 1, s));
+        }
+    }
+    r.off = cores_off;
+    qmb_cores cores2{};
+    QMB_TRY(cores_build(&targets, static_cast<int>(occ), 2, r.next(full.cores2), full.cores2, &cores2, s));
+    qmb_cores cores1{};
+    if (n_single > 0) {
+        QMB_TRY(cores_build(&targets, static_cast<int>(occ), 1, r.next(full.cores1), full.cores1, &cores1, s));
+    }
+    QMB_TRY(launch_zero_f64(out, 2 * n, s));
+    QMB_TRY(apply_diagonal(&pairs, &targets, in, out, mode, s));
+    if (n_single > 0) {
+        QMB_TRY(apply_singles(&singles, &cores1, &targets, in, out, mode, s));
+    }
+    QMB_TRY(apply_pairs(&pairs, &cores2, &targets, in, out, mode, s));
+    if (n_residual > 0) {
+        QMB_TRY(apply_prepared(&residual, &targets, configs, n, in, out, mode, nullptr, s));
+    }
+    return 0;
+}
+
+}  // namespace
+}  // namespace qmb
+
+extern "C" {
+
+int qmb_apply_within_structured_workspace_size(int n_qubytes, int64_t n, int64_t n_terms, int32_t max_occupied, size_t* bytes) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    if (!bytes) {
+        return set_error(QMB_ERR_ARGUMENT, "bytes must be non-null");
+    }
+    if (n < 0 || n_terms < 0 || max_occupied < 0 || max_occupied > 8 * n_qubytes) {
+        return set_error(QMB_ERR_ARGUMENT, "n, n_terms must be non-negative and max_occupied in [0, %d]", 8 * n_qubytes);
+    }
+    if (n_qubytes > kPairMaxQubytes) {
+        return set_error(QMB_ERR_UNSUPPORTED, "the structured path covers configs of up to %d bytes", kPairMaxQubytes);
+    }
+    *bytes = std::max(structured_layout(n_qubytes, n, n_terms, std::max(2, static_cast<int>(max_occupied))).total,
+                      terms_bytes(n_qubytes, n_terms) + targets_bytes(n_qubytes, n) + 512);
+    return QMB_OK;
+}
+
 static int apply_oneshot(int n_qubytes, int particle_cut, const uint8_t* configs_i, int64_t n_i, const uint8_t* configs_j,
                          int64_t n_j, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
                          const double* in, double* out, int mode, void* workspace, size_t workspace_bytes, void* stream) {
@@ -584,6 +747,13 @@ static int apply_oneshot(int n_qubytes, int particle_cut, const uint8_t* configs
     const size_t gb = targets_bytes(n_qubytes, n_j);
     QMB_TRY(need_workspace(tb + gb + 512, workspace_bytes, workspace));
     hipStream_t s = as_stream(stream);
+    if (configs_i == configs_j && n_i == n_j) {
+        const int rc = apply_structured(n_qubytes, particle_cut, configs_i, n_i, site, kind, coef, n_terms, in, out, mode, workspace,
+                                        workspace_bytes, s);
+        if (rc != 1) {
+            return rc;
+        }
+    }
     const int64_t n_out = mode == 0 ? n_j : n_i;
     QMB_TRY(launch_zero_f64(out, 2 * n_out, s));
     if (n_i == 0 || n_j == 0 || n_terms == 0) {

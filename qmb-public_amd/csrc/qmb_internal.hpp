@@ -139,7 +139,9 @@ int single_relative_prepared(int n_qubytes, int particle_cut, const uint8_t* con
                              hipStream_t stream);
 size_t single_relative_scratch_bytes(int64_t n_terms);
 
-// Pair-excitation path (qmb_pairs.hip).
+// Pair-excitation path (qmb_pairs.hip).  The dense pair and single tables cover configs of up to
+// kPairMaxQubytes bytes (128 sites).
+constexpr int kPairMaxQubytes = 16;  // the same bound as the pair tables' layouts
 size_t pairs_bytes(int n_qubytes);
 int pairs_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
                 void* buffer, size_t bytes, qmb_pairs* out, hipStream_t stream);

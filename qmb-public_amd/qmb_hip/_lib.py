@@ -16,6 +16,10 @@ _HERE = pathlib.Path(__file__).resolve().parent
 LIBRARY_PATH = pathlib.Path(os.environ.get("QMB_HIP_LIBRARY") or str(_HERE / "libqmb_hip.so"))
 
 QMB_OK = 0
+QMB_ERR_ARGUMENT = 1
+QMB_ERR_HIP = 2
+QMB_ERR_WORKSPACE = 3
+QMB_ERR_UNSUPPORTED = 4
 
 _c_int = ctypes.c_int
 _c_i32 = ctypes.c_int32
@@ -174,6 +178,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "qmb_philox4x32_10_host": (None, [_vp, _vp, _vp]),
     "qmb_philox_uniform_host": (ctypes.c_double, [_c_u64, _c_u64]),
     "qmb_apply_within_workspace_size": (_c_int, [_c_int, _c_i64, _c_i64, _c_i64, ctypes.POINTER(_c_size)]),
+    "qmb_apply_within_structured_workspace_size": (_c_int, [_c_int, _c_i64, _c_i64, _c_i32, ctypes.POINTER(_c_size)]),
     "qmb_apply_within": (_c_int, [_c_int, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _vp, _c_size, _vp]),
     "qmb_find_relative_workspace_size": (_c_int, [_c_int, _c_i64, _c_i64, _c_i64, _c_i64, ctypes.POINTER(_c_size)]),
     "qmb_find_relative": (

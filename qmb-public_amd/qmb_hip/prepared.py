@@ -214,6 +214,22 @@ class PreparedTargets:
         return self._cores[order][1]
 
 
+def core_stats(targets: PreparedTargets, order: int) -> dict[str, int]:
+    """Sizes of a set's core index (synchronises): entries, groups, and the entries and groups of the
+    groups holding two or more entries (the only ones the group kernels walk)."""
+    cores = targets.cores(order)
+    buf = targets._cores[order][0]  # pylint: disable=protected-access
+    base = buf.data_ptr()
+    counts = buf[int(cores.counts) - base:int(cores.counts) - base + 16].view(torch.int64).tolist()
+    entries, groups = int(counts[0]), int(counts[1])
+    off = int(cores.starts) - base
+    starts = buf[off:off + 4 * (groups + 1)].view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    sizes = starts[1:] - starts[:-1]
+    multi = sizes > 1
+    return {"entries": entries, "groups": groups, "multi_entries": int(sizes[multi].sum().item()),
+            "multi_groups": int(multi.sum().item()), "largest": int(sizes.max().item()) if groups else 0}
+
+
 def check_core_overflow(overflow: int, max_occupied: int) -> None:
     """Raises when a core index found rows with more occupied sites than the bound it was built
     for (the kernels would leave those rows' pair and single terms out)."""
