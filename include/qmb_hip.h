@@ -322,6 +322,15 @@ int qmb_apply_singles(const qmb_singles* singles, const qmb_cores* cores, const 
  * conj(d_i) in[i] (mode 1), d_i from the folded diagonal table of `pairs`. */
 int qmb_apply_diagonal(const qmb_pairs* pairs, const qmb_targets* targets, const double* in, double* out, int mode, void* stream);
 
+/* The diagonal, single- and pair-excitation parts of A^H on two vectors at once, for one set:
+ * out += A^H in and out2 += A^H in2, each kernel walking its groups once for both (for a Hermitian
+ * table, the forward and the backward of a training step).  singles/cores1 may be NULL for a
+ * table without single excitations.  The per-instance form of a two-vector batch.  No reference
+ * counterpart. */
+int qmb_apply_structured_dual(const qmb_pairs* pairs, const qmb_singles* singles, const qmb_cores* cores2, const qmb_cores* cores1,
+                              const qmb_targets* targets, const double* in, double* out, const double* in2, double* out2,
+                              void* stream);
+
 /* ---------------- batched persistent launch (no reference counterpart) -------------------------
  * Many instances -- a term table and a config set each, of any sizes -- applied in ONE persistent
  * kernel launch: the diagonal, single- and pair-excitation work of every instance is cut into

@@ -907,7 +907,7 @@ __device__ __forceinline__ void mark_overflow(double2* out) {
 
 // Persistent waves over the groups, 64 groups per step.  Every wave leaves when the group count
 // is done.
-template <int W, bool ADJ>
+template <int W, bool ADJ, int NV = 1>
 __global__ __launch_bounds__(kBlock) void pair_kernel(PairParams p) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6;
@@ -921,7 +921,8 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(PairParams p) {
     __shared__ __align__(16) char stage[kPairStageBytes];
     const int64_t groups = p.counts[1];
     for (int64_t g0 = wave * 64; g0 < groups; g0 += nwaves * 64) {
-        pair_chunk<W, ADJ, kPairBatch, 1>(p, groups, g0, lane, stage);
+        // two vectors hold twice the accumulators: fewer coefficient gathers in flight per lane
+        pair_chunk<W, ADJ, NV == 1 ? kPairBatch : kPairBatchInBatch, NV>(p, groups, g0, lane, stage);
     }
 }
 
@@ -1367,7 +1368,7 @@ __device__ __forceinline__ void single_chunk(const SingleParams& p, int64_t grou
     }
 }
 
-template <int W, bool ADJ>
+template <int W, bool ADJ, int NV = 1>
 __global__ __launch_bounds__(kBlock) void single_kernel(SingleParams p) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6;
@@ -1380,7 +1381,7 @@ __global__ __launch_bounds__(kBlock) void single_kernel(SingleParams p) {
     }
     const int64_t groups = p.counts[1];
     for (int64_t g0 = wave * 64; g0 < groups; g0 += nwaves * 64) {
-        single_chunk<W, ADJ, 1, 8>(p, groups, g0, lane);
+        single_chunk<W, ADJ, NV, 8>(p, groups, g0, lane);
     }
 }
 
@@ -1444,6 +1445,28 @@ __global__ __launch_bounds__(kBlock) void diag_apply_kernel(const uint64_t* __re
         const double2 g = in[i];
         out[i].x += d.x * g.x - im * g.y;
         out[i].y += d.x * g.y + im * g.x;
+    }
+}
+
+// The same diagonal on two vectors in one pass (the adjoint, conj(d)): the config words and the
+// folded table are read once for both.
+template <int W>
+__global__ __launch_bounds__(kBlock) void diag_apply2_kernel(const uint64_t* __restrict__ ckeys, int64_t n,
+                                                             const double2* __restrict__ diag, int n_bits,
+                                                             const double2* __restrict__ in, double2* __restrict__ out,
+                                                             const double2* __restrict__ in2, double2* __restrict__ out2) {
+    const int64_t i = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const double2 d = diag_value<W, 8>(load_key<W>(ckeys, i), diag, n_bits);
+    if (d.x != 0.0 || d.y != 0.0) {
+        const double2 g = in[i];
+        const double2 g2 = in2[i];
+        out[i].x += d.x * g.x + d.y * g.y;
+        out[i].y += d.x * g.y - d.y * g.x;
+        out2[i].x += d.x * g2.x + d.y * g2.y;
+        out2[i].y += d.x * g2.y - d.y * g2.x;
     }
 }
 
@@ -1867,6 +1890,86 @@ int apply_diagonal(const qmb_pairs* pairs, const qmb_targets* targets, const dou
         default: diag_apply_kernel<2, true><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, dg, nb, vi, vo); break;
     }
     QMB_CHECK_LAUNCH();
+    return QMB_OK;
+}
+
+int apply_structured_dual(const qmb_pairs* pairs, const qmb_singles* singles, const qmb_cores* cores2, const qmb_cores* cores1,
+                          const qmb_targets* targets, const double* in, double* out, const double* in2, double* out2,
+                          hipStream_t stream) {
+    if (!pairs || !cores2 || !targets || !in || !out || !in2 || !out2 || (singles && !cores1)) {
+        return set_error(QMB_ERR_ARGUMENT, "pairs, cores2, targets, the four vectors (and cores1 with singles) must be non-null");
+    }
+    if (pairs->n_qubytes != targets->n_qubytes || cores2->n != targets->n || cores2->order != 2 ||
+        (singles && (cores1->n != targets->n || cores1->order != 1 || singles->n_qubytes != targets->n_qubytes))) {
+        return set_error(QMB_ERR_ARGUMENT, "pairs, singles, cores and targets describe different shapes");
+    }
+    if (targets->n == 0) {
+        return QMB_OK;
+    }
+    const int W = targets->n_words;
+    if (W != 1 && W != 2) {
+        return set_error(QMB_ERR_UNSUPPORTED, "the structured path supports n_qubytes <= %d", kMaxPairBytes);
+    }
+    const auto* vi = reinterpret_cast<const double2*>(in);
+    const auto* vi2 = reinterpret_cast<const double2*>(in2);
+    auto* vo = reinterpret_cast<double2*>(out);
+    auto* vo2 = reinterpret_cast<double2*>(out2);
+    const auto* dg = reinterpret_cast<const double2*>(pairs->diag);
+    if (W == 1) {
+        diag_apply2_kernel<1><<<blocks_for(targets->n), kBlock, 0, stream>>>(targets->keys, targets->n, dg, pairs->n_bits, vi, vo, vi2, vo2);
+    } else {
+        diag_apply2_kernel<2><<<blocks_for(targets->n), kBlock, 0, stream>>>(targets->keys, targets->n, dg, pairs->n_bits, vi, vo, vi2, vo2);
+    }
+    QMB_CHECK_LAUNCH();
+    if (singles && cores1->capacity > 0) {
+        SingleParams p{};
+        p.a = reinterpret_cast<const double2*>(singles->a);
+        p.kd = reinterpret_cast<const double2*>(singles->kd);
+        p.a_re = singles->real ? singles->a_re : nullptr;
+        p.kd_re = singles->real ? singles->kd_re : nullptr;
+        p.n_bits = singles->n_bits;
+        p.particle_cut = singles->particle_cut;
+        p.ckeys = targets->keys;
+        p.vals = cores1->vals;
+        p.starts = cores1->starts;
+        p.counts = cores1->counts;
+        p.in = vi;
+        p.out = vo;
+        p.in2 = vi2;
+        p.out2 = vo2;
+        const int64_t want = (cores1->capacity + 64 * 4 - 1) / (64 * 4);
+        const unsigned grid = static_cast<unsigned>(want < 4096 ? (want < 1 ? 1 : want) : 4096);
+        if (W == 1) {
+            single_kernel<1, true, 2><<<grid, kBlock, 0, stream>>>(p);
+        } else {
+            single_kernel<2, true, 2><<<grid, kBlock, 0, stream>>>(p);
+        }
+        QMB_CHECK_LAUNCH();
+    }
+    if (cores2->capacity > 0) {
+        PairParams p{};
+        p.coef = reinterpret_cast<const double2*>(pairs->coef);
+        p.coef_re = pairs->real ? pairs->coef_re : nullptr;
+        p.n_pairs = pairs->n_pairs;
+        p.n_bits = pairs->n_bits;
+        p.particle_cut = pairs->particle_cut;
+        p.ckeys = targets->keys;
+        p.vals = cores2->vals;
+        p.starts = cores2->starts;
+        p.counts = cores2->counts;
+        p.in = vi;
+        p.out = vo;
+        p.in2 = vi2;
+        p.out2 = vo2;
+        const int64_t want = (cores2->capacity + 64 * 4 - 1) / (64 * 4);
+        const unsigned grid = static_cast<unsigned>(want < 4096 ? (want < 1 ? 1 : want) : 4096);
+        if (W == 1) {
+            pair_kernel<1, true, 2><<<grid, kBlock, 0, stream>>>(p);
+        } else {
+            pair_kernel<2, true, 2><<<grid, kBlock, 0, stream>>>(p);
+        }
+        QMB_CHECK_LAUNCH();
+    }
     return QMB_OK;
 }
 

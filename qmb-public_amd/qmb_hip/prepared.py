@@ -291,6 +291,24 @@ def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tenso
     return out
 
 
+def apply_dual(terms: PreparedTerms, targets: PreparedTargets, vec: torch.Tensor, vec2: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    """A^H vec and A^H vec2 on the indexed set in one pass per kernel (qmb_apply_structured_dual): for a
+    Hermitian table the forward and the backward of a training step.  Needs the structured path
+    (pair excitations, distinct rows) and no residual terms."""
+    if not can_use_pairs(terms, targets, targets.configs) or terms.n_residual_terms > 0:
+        raise RuntimeError("apply_dual needs the structured path: pair excitations, distinct rows and no residual terms.")
+    vec, vec2 = vec.contiguous(), vec2.contiguous()
+    out = torch.zeros((targets.n, 2), dtype=torch.float64, device=targets.device)
+    out2 = torch.zeros_like(out)
+    singles = ctypes.byref(terms.singles_desc) if terms.singles_desc is not None else None
+    cores1 = ctypes.byref(targets.cores(1)) if terms.singles_desc is not None else None
+    with torch.cuda.device(targets.device):
+        _lib.check(_lib.library().qmb_apply_structured_dual(ctypes.byref(terms.pairs_desc), singles, ctypes.byref(targets.cores(2)), cores1,
+                                                            ctypes.byref(targets.desc), _ptr(vec), _ptr(out), _ptr(vec2), _ptr(out2),
+                                                            _stream(targets.device)), "qmb_apply_structured_dual")
+    return out, out2
+
+
 def apply_across(terms: PreparedTerms, configs_i: torch.Tensor, vec: torch.Tensor, configs_j: torch.Tensor, mode: int,
                  max_occupied: int | None = None) -> torch.Tensor | None:
     """The structured path between two config sets: mode 0 returns A psi_i on the rows of configs_j

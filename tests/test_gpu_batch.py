@@ -215,3 +215,22 @@ def test_batch_autotune_keeps_result(cuda_device: torch.device) -> None:
     assert waves in qb.WAVE_CANDIDATES
     b.launch(pp.MODE_FORWARD)
     assert (b.out - first).abs().max().item() <= 1e-12 * first.abs().max().item()
+
+
+def test_per_instance_two_vector_matches_two_applications(cuda_device: torch.device) -> None:
+    """pp.apply_dual (qmb_apply_structured_dual: diagonal, single and pair kernels each applying A^H
+    to two vectors in one pass) equals two separate adjoint applications, one- and two-word sets."""
+    for n_orb, n_e, count, seed in ((8, 3, 800, 31), (34, 5, 3000, 33)):
+        s, k, c = sorted_terms(*sy.molecular_terms(n_orb, seed))
+        cfg = sy.excitation_configs(n_orb, n_e, n_e, count, seed + 3)  # HF + singles + doubles: many connections
+        nq = cfg.shape[1]
+        assert nq == (2 * n_orb + 7) // 8
+        terms = pp.PreparedTerms(_dev(s, cuda_device), _dev(k, cuda_device), _dev(c, cuda_device), nq, 1)
+        targets = pp.PreparedTargets(_dev(cfg, cuda_device), nq, distinct=True)
+        x = torch.from_numpy(sy.random_psi(targets.n, seed + 1)).to(cuda_device)
+        y = torch.from_numpy(sy.random_psi(targets.n, seed + 2)).to(cuda_device)
+        d1, d2 = pp.apply_dual(terms, targets, torch.view_as_real(x), torch.view_as_real(y))
+        for got, vec in ((d1, x), (d2, y)):
+            ref = _reference(terms, targets, vec, pp.MODE_ADJOINT)
+            diff = np.abs(torch.view_as_complex(got).cpu().numpy() - ref)
+            np.testing.assert_array_less(diff, 1e-12 * _scale(ref, 10.0, vec) + 1e-300)
