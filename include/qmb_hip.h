@@ -134,6 +134,20 @@ int qmb_apply_within_adjoint(int n_qubytes, int particle_cut, const uint8_t* con
                              const uint8_t* kind, const double* coef, int64_t n_terms, double* grad_i, void* workspace,
                              size_t workspace_bytes, void* stream);
 
+/* apply_within without a workspace, for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same arguments and results as qmb_apply_within, minus the workspace: the term table
+ * and the target index are built in a fixed device array of qmb_static_scratch_bytes() (64 MiB)
+ * that the library holds on every device it runs on until the process ends.  Calls whose
+ * term table and target index need more (about 50 bytes per term plus 40 per target config)
+ * fail with QMB_ERR_WORKSPACE.  The call allocates nothing and enqueues only.  All calls on one
+ * device share that array: calls that overlap in time on one device (different streams, or
+ * threads) race and must be ordered by the caller. */
+size_t qmb_static_scratch_bytes(void);
+int qmb_apply_within_noalloc(int n_qubytes, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i,
+                             const uint8_t* configs_j, int64_t n_j, const int16_t* site, const uint8_t* kind, const double* coef,
+                             int64_t n_terms, double* psi_j, void* stream);
+
 /* The reference's own scheme on the GPU, kept as the comparator: one thread per (term, config),
  * a lexicographic sort of configs_j, a binary search per connection and FP64 atomics
  * (qmb/_hamiltonian_cuda.cu:86-169 restated).  Same results as qmb_apply_within. */

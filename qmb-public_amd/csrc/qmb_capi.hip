@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -792,6 +793,72 @@ int qmb_apply_within_adjoint(int n_qubytes, int particle_cut, const uint8_t* con
     QMB_TRY(check_ptr(grad_i, n_i, "grad_i"));
     return apply_oneshot(n_qubytes, particle_cut, configs_i, n_i, configs_j, n_j, site, kind, coef, n_terms, grad_j, grad_i, 1,
                          workspace, workspace_bytes, stream);
+}
+
+// --------------------------------------------------------------------- no-allocation variant --
+// qmb_apply_within for callers that cannot allocate: the term table and the target index go into a
+// file-scope device array instead of a caller workspace.  Each device holds its own copy of the
+// array for the life of the process; its address on a device is looked up once, with that device
+// current, and only that address is handed to kernels and copies.
+}  // extern "C"
+
+namespace qmb {
+namespace {
+constexpr size_t kStaticScratchBytes = size_t{64} << 20;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int static_scratch(void** out) {
+    int dev = 0;
+    QMB_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        return set_error(QMB_ERR_UNSUPPORTED, "device %d is beyond the %d devices the static scratch serves", dev, kMaxDevices);
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* ptr = nullptr;
+        QMB_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (ptr == nullptr) {
+            return set_error(QMB_ERR_HIP, "hipGetSymbolAddress returned no address for the static scratch");
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return QMB_OK;
+}
+}  // namespace
+}  // namespace qmb
+
+extern "C" {
+
+size_t qmb_static_scratch_bytes(void) { return kStaticScratchBytes; }
+
+int qmb_apply_within_noalloc(int n_qubytes, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i,
+                             const uint8_t* configs_j, int64_t n_j, const int16_t* site, const uint8_t* kind, const double* coef,
+                             int64_t n_terms, double* psi_j, void* stream) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    QMB_TRY(check_particle_cut(particle_cut));
+    QMB_TRY(check_ptr(psi_i, n_i, "psi_i"));
+    QMB_TRY(check_ptr(psi_j, n_j, "psi_j"));
+    QMB_TRY(check_ptr(configs_i, n_i, "configs_i"));
+    QMB_TRY(check_ptr(configs_j, n_j, "configs_j"));
+    if (n_terms < 0) {
+        return set_error(QMB_ERR_ARGUMENT, "n_terms must be non-negative");
+    }
+    const size_t need = terms_bytes(n_qubytes, n_terms) + targets_bytes(n_qubytes, n_j) + 512;
+    if (need > kStaticScratchBytes) {
+        return set_error(QMB_ERR_WORKSPACE,
+                         "qmb_apply_within_noalloc: %lld terms and %lld target configs need %zu bytes of scratch, above its "
+                         "fixed %zu bytes; use qmb_apply_within with a workspace",
+                         static_cast<long long>(n_terms), static_cast<long long>(n_j), need, kStaticScratchBytes);
+    }
+    void* ptr = nullptr;
+    QMB_TRY(static_scratch(&ptr));
+    return apply_oneshot(n_qubytes, particle_cut, configs_i, n_i, configs_j, n_j, site, kind, coef, n_terms, psi_i, psi_j, 0, ptr,
+                         kStaticScratchBytes, stream);
 }
 
 // ------------------------------------------------------------------------------------ plain --

@@ -190,6 +190,8 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "qmb_single_relative_workspace_size": (_c_int, [_c_int, _c_i64, _c_i64, ctypes.POINTER(_c_size)]),
     "qmb_single_relative": (_c_int, [_c_int, _c_int, _vp, _c_i64, _vp, _vp, _vp, _c_i64, _c_u64, _vp, _vp, _vp, _c_size, _vp]),
     "qmb_apply_within_adjoint": (_c_int, [_c_int, _c_int, _vp, _c_i64, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _vp, _c_size, _vp]),
+    "qmb_static_scratch_bytes": (_c_size, []),
+    "qmb_apply_within_noalloc": (_c_int, [_c_int, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _vp]),
     "qmb_apply_within_plain_workspace_size": (_c_int, [_c_int, _c_i64, ctypes.POINTER(_c_size)]),
     "qmb_apply_within_plain": (_c_int, [_c_int, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _vp, _c_size, _vp]),
     "qmb_terms_workspace_size": (_c_int, [_c_int, _c_i64, ctypes.POINTER(_c_size)]),
