@@ -954,7 +954,7 @@ def main() -> int:
     if use_pairs:
         # the structured path's kernels, each timed alone on the same stream
         s_ptr = ctypes.c_void_p(stream.cuda_stream)
-        pmode = pp.MODE_ADJOINT if symmetric else pp.MODE_FORWARD
+        pmode = pp.HERMITIAN_MODE if symmetric else pp.MODE_FORWARD
         vin, vout = ctypes.c_void_p(vec.data_ptr()), ctypes.c_void_p(out.data_ptr())
         cores2, cores1 = targets.cores(2), targets.cores(1)
         parts = {"pair_excitation_terms": terms.n_pair_terms, "single_excitation_terms": terms.n_single_terms,
@@ -979,6 +979,23 @@ def main() -> int:
         parts["core_index_build_ms"] = {"order2": timed(lambda: rebuild(2)), "order1": timed(lambda: rebuild(1))}
         log(f"structured path parts: {parts}")
 
+        # The C-ABI drop-in (qmb_apply_within with one config buffer and the structured workspace):
+        # one call builds the term tables and core indexes in the workspace, reads the plan
+        # counters back and applies; timed whole, tables and indexes included.
+        ws_size = ctypes.c_size_t(0)
+        _lib.check(lib.qmb_apply_within_structured_workspace_size(nq, n, n_terms, occ, ctypes.byref(ws_size)), "structured size")
+        ws = torch.empty(ws_size.value, dtype=torch.uint8, device=dev)
+        cfg_p = ctypes.c_void_p(configs.data_ptr())
+        term_p = [ctypes.c_void_p(t.data_ptr()) for t in (site, kind, coef)]
+
+        def c_abi_call() -> None:
+            _lib.check(lib.qmb_apply_within(nq, pc, cfg_p, vin, n, cfg_p, n, *term_p, n_terms, vout, ctypes.c_void_p(ws.data_ptr()),
+                                            ws_size.value, s_ptr), "qmb_apply_within")
+
+        parts["c_abi_structured_call_ms"] = timed(c_abi_call)
+        parts["c_abi_workspace_bytes"] = ws_size.value
+        del ws
+
     # achievable HBM bandwidth: a plain 1 GiB device copy
     src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)
     dst = torch.empty_like(src)
@@ -1000,7 +1017,7 @@ def main() -> int:
         model = structured_model(pp, terms, targets, nq, n)
         kname = "pair_kernel"
         k_bytes, k_ms = model["pair_kernel"]["bytes"], parts["pair_kernel_ms"]
-        label = f"pair_kernel<{(nq + 7) // 8}, {'true' if symmetric else 'false'}>"
+        label = f"pair_kernel<{(nq + 7) // 8}, {'true' if pmode == pp.MODE_ADJOINT else 'false'}, 1>"
         app_bytes = sum(v["bytes"] for k, v in model.items() if k != "core_index_build")
         step_bytes = model["core_index_build"]["bytes"] + 2 * app_bytes
         model_desc = ("structured path, bytes moved at least once per launch: group starts, the entries of groups of >= 2 "

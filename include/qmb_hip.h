@@ -336,13 +336,13 @@ int qmb_apply_singles(const qmb_singles* singles, const qmb_cores* cores, const 
  * conj(d_i) in[i] (mode 1), d_i from the folded diagonal table of `pairs`. */
 int qmb_apply_diagonal(const qmb_pairs* pairs, const qmb_targets* targets, const double* in, double* out, int mode, void* stream);
 
-/* The diagonal, single- and pair-excitation parts of A^H on two vectors at once, for one set:
- * out += A^H in and out2 += A^H in2, each kernel walking its groups once for both (for a Hermitian
- * table, the forward and the backward of a training step).  singles/cores1 may be NULL for a
- * table without single excitations.  The per-instance form of a two-vector batch.  No reference
- * counterpart. */
+/* The diagonal, single- and pair-excitation parts of A (mode 0) or A^H (mode 1) on two vectors at
+ * once, for one set: out += A in and out2 += A in2, each kernel walking its groups once for both
+ * (for a Hermitian table, the forward and the backward of a training step).  singles/cores1 may be
+ * NULL for a table without single excitations.  The per-instance form of a two-vector batch.  No
+ * reference counterpart. */
 int qmb_apply_structured_dual(const qmb_pairs* pairs, const qmb_singles* singles, const qmb_cores* cores2, const qmb_cores* cores1,
-                              const qmb_targets* targets, const double* in, double* out, const double* in2, double* out2,
+                              const qmb_targets* targets, const double* in, double* out, const double* in2, double* out2, int mode,
                               void* stream);
 
 /* ---------------- batched persistent launch (no reference counterpart) -------------------------
@@ -381,9 +381,9 @@ int qmb_batch_workspace_size(int n_items, size_t* bytes);
 int qmb_batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t bytes, qmb_batch* out, void* stream);
 /* One persistent launch of `waves` wavefronts (0: 8192) over all units; mode 0 forward, 1 adjoint
  * (a Hermitian instance gives the same map in both).  A two-vector batch (dual) computes
- * out = A^H in and out2 = A^H in2 in the same pass (mode 1 only), sharing every group walk and
- * coefficient read: for Hermitian instances that is the forward and the backward of a training
- * step in one launch. */
+ * out = A in and out2 = A in2 (mode 0; A^H with mode 1) in the same pass, sharing every group walk
+ * and coefficient read: for Hermitian instances that is the forward and the backward of a
+ * training step in one launch. */
 int qmb_apply_batch(const qmb_batch* batch, int mode, int waves, void* stream);
 
 /* ---------------------------------------------------------------------------------------------

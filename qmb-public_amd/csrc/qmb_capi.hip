@@ -425,10 +425,10 @@ int qmb_apply_diagonal(const qmb_pairs* pairs, const qmb_targets* targets, const
 }
 
 int qmb_apply_structured_dual(const qmb_pairs* pairs, const qmb_singles* singles, const qmb_cores* cores2, const qmb_cores* cores1,
-                              const qmb_targets* targets, const double* in, double* out, const double* in2, double* out2,
+                              const qmb_targets* targets, const double* in, double* out, const double* in2, double* out2, int mode,
                               void* stream) {
     clear_error();
-    return apply_structured_dual(pairs, singles, cores2, cores1, targets, in, out, in2, out2, as_stream(stream));
+    return apply_structured_dual(pairs, singles, cores2, cores1, targets, in, out, in2, out2, mode, as_stream(stream));
 }
 
 int qmb_batch_workspace_size(int n_items, size_t* bytes) {

@@ -159,10 +159,11 @@ int apply_singles(const qmb_singles* singles, const qmb_cores* cores, const qmb_
                   int mode, hipStream_t stream);
 int apply_diagonal(const qmb_pairs* pairs, const qmb_targets* targets, const double* in, double* out, int mode,
                    hipStream_t stream);
-// Diagonal, single- and pair-excitation parts of A^H applied to two vectors in one pass each
-// (the forward and backward of a Hermitian training step); accumulates into out and out2.
+// Diagonal, single- and pair-excitation parts of A (mode 0) or A^H (mode 1) applied to two vectors
+// in one pass each (the forward and backward of a Hermitian training step); accumulates into out
+// and out2.
 int apply_structured_dual(const qmb_pairs* pairs, const qmb_singles* singles, const qmb_cores* cores2, const qmb_cores* cores1,
-                          const qmb_targets* targets, const double* in, double* out, const double* in2, double* out2,
+                          const qmb_targets* targets, const double* in, double* out, const double* in2, double* out2, int mode,
                           hipStream_t stream);
 size_t batch_bytes(int n_items);
 int batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t bytes, qmb_batch* out, hipStream_t stream);

@@ -432,14 +432,19 @@ __device__ __forceinline__ int pair_parity(const Cfg<W>& c, int a, int b, int x,
     return par & 1;
 }
 
-// One staged group entry of pair_group, shared by the wave through LDS.
+// One staged group entry (b: a config of the group, removed sites q0 < q1), shared by the wave
+// through LDS: what every row a of the group needs of it, so the per-pair work is a few mask
+// operations, one coefficient load and the multiply-adds.
 template <int W>
 struct PairStage {
-    double2 g;        // in[row]
-    double2 g2;       // in2[row] of a two-vector pass
-    uint64_t m[W];    // sites in [q0, q1)
-    uint32_t packed;  // q0 | q1 << 8 | pair_slot(q0, q1) << 16
-    uint32_t row;
+    double2 g;          // in[row]
+    uint64_t bits[W];   // the sites q0 and q1 (all ones for an empty slot: it meets every row)
+    uint32_t part;      // the entry's part of the coefficient index: pair_slot(q0, q1) (ADJ), times n_pairs (!ADJ)
+    uint32_t ebit;      // parity of the group core's sites in [q0, q1)
+    uint64_t m[W];      // sites in [q0, q1)
+    uint32_t row;       // config row of the entry
+    uint32_t pad;
+    double2 g2;         // in2[row] of a two-vector pass
 };
 constexpr int kWavesPerBlock = kBlock / 64;
 constexpr size_t kPairStageBytes = kWavesPerBlock * 64 * sizeof(PairStage<2>);
@@ -460,20 +465,157 @@ __device__ __forceinline__ double2 shfl_d2(const double2& v, int lane) {
 constexpr int kPairBatch = 4;
 constexpr int kPairBatchInBatch = 2;
 
+template <int W>
+__device__ __forceinline__ Cfg<W> site_bits(int a, int b) {
+    Cfg<W> m;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        m.w[w] = ((a >> 6) == w ? (1ull << (a & 63)) : 0ull) | ((b >> 6) == w ? (1ull << (b & 63)) : 0ull);
+    }
+    return m;
+}
+
+// Stages entry b (value vb) of a group whose first core is core0.  An inactive slot gets all-ones
+// site bits, so every row sees it as overlapping and skips it.
+template <int W, bool ADJ, int NV>
+__device__ __forceinline__ PairStage<W> stage_entry(const PairParams& p, bool act, uint64_t vb, const Cfg<W>& core0, bool fermi) {
+    PairStage<W> st;
+    if (act) {
+        const int q0 = static_cast<int>(vb & 0xff);
+        const int q1 = static_cast<int>((vb >> 8) & 0xff);
+        st.row = static_cast<uint32_t>(vb >> 16);
+        st.g = p.in[st.row];
+        st.g2 = NV == 2 ? p.in2[st.row] : make_double2(0.0, 0.0);
+        const Cfg<W> mq = range_mask<W>(q0, q1);
+        const Cfg<W> bq = site_bits<W>(q0, q1);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            st.m[w] = mq.w[w];
+            st.bits[w] = bq.w[w];
+        }
+        const int64_t slot = pair_slot(q0, q1, p.n_bits);
+        st.part = static_cast<uint32_t>(ADJ ? slot : slot * p.n_pairs);
+        st.ebit = fermi ? static_cast<uint32_t>(masked_parity<W>(core0, mq)) : 0u;
+    } else {
+        st.row = 0;
+        st.g = make_double2(0.0, 0.0);
+        st.g2 = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            st.m[w] = 0;
+            st.bits[w] = ~0ull;
+        }
+        st.part = 0;
+        st.ebit = 0;
+    }
+    st.pad = 0;
+    return st;
+}
+
+// What row a (removed sites r0 < r1, core `core`) keeps while it walks the staged entries.
+template <int W>
+struct PairRow {
+    Cfg<W> core;   // the row's config without r0, r1
+    Cfg<W> bits;   // r0 and r1
+    Cfg<W> mr;     // sites in [r0, r1)
+    int64_t part;  // the row's part of the coefficient index
+    int rbit;      // parity(core & [r0, r1)) ^ 1
+};
+
+// Contribution of staged entry st to row ra, or false when the pair is not a connection: the
+// removed pairs overlap (which includes the row itself), or -- in a group where different cores
+// share the hash (mixed) -- the entry's core differs.  Signs (fermions):
+//   ADJ:  A_ba = (-1)^(rbit ^ ebit ^ parity(bits_a & [q0, q1))) T[Pa][Pb], applied conjugated
+//   !ADJ: A_ab = (-1)^(rbit ^ ebit ^ parity(bits_b & [r0, r1))) T[Pb][Pa]
+// with ebit taken from the row's own core in a mixed group.
+template <int W, bool ADJ, bool REAL>
+__device__ __forceinline__ bool pair_term(const PairParams& p, const PairRow<W>& ra, const PairStage<W>& st, bool mixed, int fermi,
+                                          bool en, double2& x) {
+    uint64_t meet = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        meet |= ra.bits.w[w] & st.bits[w];
+    }
+    bool ok = en && meet == 0ull;
+    int ebit = static_cast<int>(st.ebit);
+    if (mixed) {
+        if (ok) {
+            const Cfg<W> cb = load_key<W>(p.ckeys, static_cast<int64_t>(st.row));
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                ok = ok && ((cb.w[w] ^ st.bits[w]) == ra.core.w[w]);
+            }
+        }
+        Cfg<W> mq;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            mq.w[w] = st.m[w];
+        }
+        ebit = masked_parity<W>(ra.core, mq);
+    }
+    // only the lanes with a connection load a coefficient
+    x = make_double2(0.0, 0.0);
+    if (ok) {
+        const int64_t idx = ra.part + static_cast<int64_t>(st.part);
+        x = REAL ? make_double2(p.coef_re[idx], 0.0) : p.coef[idx];
+    }
+    int c = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        c += __popcll(ADJ ? (ra.bits.w[w] & st.m[w]) : (st.bits[w] & ra.mr.w[w]));
+    }
+    // (-1)^parity as the sign bit, applied without a branch (fermi = 0 for hard-core bosons)
+    const uint64_t sign = static_cast<uint64_t>((ra.rbit ^ ebit ^ c) & fermi) << 63;
+    x.x = __longlong_as_double(__double_as_longlong(x.x) ^ static_cast<long long>(sign));
+    if (!REAL) {
+        x.y = __longlong_as_double(__double_as_longlong(x.y) ^ static_cast<long long>(sign));
+    }
+    return ok;
+}
+
+template <int W>
+__device__ __forceinline__ PairRow<W> pair_row(const PairParams& p, const Cfg<W>& ca, int r0, int r1, bool adj, bool fermi) {
+    PairRow<W> ra;
+    ra.core = ca;
+    flip_site<W>(ra.core, r0);
+    flip_site<W>(ra.core, r1);
+    ra.bits = site_bits<W>(r0, r1);
+    ra.mr = range_mask<W>(r0, r1);
+    const int64_t slot = pair_slot(r0, r1, p.n_bits);
+    ra.part = adj ? slot * p.n_pairs : slot;
+    ra.rbit = fermi ? (masked_parity<W>(ra.core, ra.mr) ^ 1) : 0;
+    return ra;
+}
+
+// acc += x g (!ADJ) or conj(x) g (ADJ); a real table skips the imaginary part of x.
+template <bool ADJ, bool REAL>
+__device__ __forceinline__ void pair_madd(const double2& x, const double2& g, double& re, double& im) {
+    if (REAL) {
+        re = __fma_rn(x.x, g.x, re);
+        im = __fma_rn(x.x, g.y, im);
+    } else if (ADJ) {
+        re = __fma_rn(x.y, g.y, __fma_rn(x.x, g.x, re));
+        im = __fma_rn(-x.y, g.x, __fma_rn(x.x, g.y, im));
+    } else {
+        re = __fma_rn(-x.y, g.y, __fma_rn(x.x, g.x, re));
+        im = __fma_rn(x.y, g.x, __fma_rn(x.x, g.y, im));
+    }
+}
+
 // One wave, one group of entries [s, e) (wave-uniform).  Lane = row a; the group's entries b are
-// staged 64 at a time in this wave's LDS slots (`stage`, with their pair index, site-range mask and
-// input amplitude) and read back as broadcasts.
-//   ADJ:  out[a] += sum_b conj(A_ba) in[b],  A_ba = (-1)^pair_parity(c_a) T[Pa][Pb]
-//   !ADJ: out[a] += sum_b A_ab in[b],        A_ab = (-1)^pair_parity(c_b) T[Pb][Pa]
-// with pair_parity(c) = popcount(c & ([r0, r1) ^ [q0, q1))), split into a per-row part and a
-// per-entry part.
-template <int W, bool ADJ, int B, int NV>
+// staged 64 at a time in this wave's LDS slots and read back as broadcasts.
+//   ADJ:  out[a] += sum_b conj(A_ba) in[b]
+//   !ADJ: out[a] += sum_b A_ab in[b]
+// !ADJ reads T[Pb][Pa]: one table row per entry, the rows' columns ascending across the lanes
+// (entries ascend by pair slot inside a group), so a coefficient load touches few cache lines.
+template <int W, bool ADJ, int B, int NV, bool REAL>
 __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane, PairStage<W>* stage) {
     const uint64_t v0 = p.vals[s];
     Cfg<W> core0 = load_key<W>(p.ckeys, static_cast<int64_t>(v0 >> 16));
     flip_site<W>(core0, static_cast<int>(v0 & 0xff));
     flip_site<W>(core0, static_cast<int>((v0 >> 8) & 0xff));
     const bool fermi = p.particle_cut == 1;
+    const int fmask = fermi ? 1 : 0;
     // Different cores with equal hashes share a group: then every pair is checked exactly.  The test
     // covers all entries of the group, not one 64-row block: a block whose rows all hold the first
     // core still meets entries of the other cores in the other blocks.
@@ -499,110 +641,34 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
         const bool act = a < e;
         const uint64_t va = act ? p.vals[a] : v0;
         const int64_t ia = static_cast<int64_t>(va >> 16);
-        const int r0 = static_cast<int>(va & 0xff);
-        const int r1 = static_cast<int>((va >> 8) & 0xff);
-        const Cfg<W> ca = load_key<W>(p.ckeys, ia);
-        Cfg<W> core = ca;
-        flip_site<W>(core, r0);
-        flip_site<W>(core, r1);
-        const int64_t prow = pair_slot(r0, r1, p.n_bits);
-        const Cfg<W> mr = range_mask<W>(r0, r1);
-        const int base = fermi ? masked_parity<W>(ADJ ? ca : core, mr) : 0;
+        const PairRow<W> ra = pair_row<W>(p, load_key<W>(p.ckeys, ia), static_cast<int>(va & 0xff), static_cast<int>((va >> 8) & 0xff),
+                                          ADJ, fermi);
         double acc_re[NV] = {}, acc_im[NV] = {};
         for (uint32_t b0 = s; b0 < e; b0 += 64) {
             const uint32_t bl = b0 + lane;
-            {
-                PairStage<W> st;
-                if (bl < e) {
-                    const uint64_t vb = p.vals[bl];
-                    const int q0 = static_cast<int>(vb & 0xff);
-                    const int q1 = static_cast<int>((vb >> 8) & 0xff);
-                    st.row = static_cast<uint32_t>(vb >> 16);
-                    st.g = p.in[st.row];
-                    if (NV == 2) {
-                        st.g2 = p.in2[st.row];
-                    }
-                    const Cfg<W> mq = range_mask<W>(q0, q1);
-#pragma unroll
-                    for (int w = 0; w < W; ++w) {
-                        st.m[w] = mq.w[w];
-                    }
-                    st.packed = static_cast<uint32_t>(q0) | (static_cast<uint32_t>(q1) << 8) |
-                                (static_cast<uint32_t>(pair_slot(q0, q1, p.n_bits)) << 16);
-                } else {
-                    st.row = 0;
-                    st.g = make_double2(0.0, 0.0);
-                    st.g2 = make_double2(0.0, 0.0);
-#pragma unroll
-                    for (int w = 0; w < W; ++w) {
-                        st.m[w] = 0;
-                    }
-                    st.packed = 0;
-                }
-                stage[lane] = st;
-            }
+            stage[lane] = stage_entry<W, ADJ, NV>(p, bl < e, bl < e ? p.vals[bl] : 0ull, core0, fermi);
             __builtin_amdgcn_wave_barrier();
             const int nb = static_cast<int>(e - b0 < 64u ? e - b0 : 64u);
             for (int u = 0; u < nb; u += B) {
                 double2 x[B], g[B], g2[B];
-                int sg[B];
                 bool ok[B];
 #pragma unroll
                 for (int k = 0; k < B; ++k) {
-                    const int uu = (u + k) & 63;
-                    const PairStage<W>& st = stage[uu];
-                    const uint32_t pk = st.packed;
-                    const int q0 = static_cast<int>(pk & 0xff);
-                    const int q1 = static_cast<int>((pk >> 8) & 0xff);
-                    const int64_t pq = static_cast<int64_t>(pk >> 16);
+                    const PairStage<W>& st = stage[(u + k) & 63];
                     g[k] = st.g;
                     g2[k] = NV == 2 ? st.g2 : make_double2(0.0, 0.0);
-                    bool o = act && (u + k < nb) && (b0 + static_cast<uint32_t>(u + k) != a) && q0 != r0 && q0 != r1 && q1 != r0 &&
-                             q1 != r1;
-                    if (mixed && o) {
-                        Cfg<W> cb = load_key<W>(p.ckeys, static_cast<int64_t>(st.row));
-                        flip_site<W>(cb, q0);
-                        flip_site<W>(cb, q1);
-#pragma unroll
-                        for (int w = 0; w < W; ++w) {
-                            o = o && cb.w[w] == core.w[w];
-                        }
-                    }
-                    ok[k] = o;
-                    Cfg<W> mq;
-#pragma unroll
-                    for (int w = 0; w < W; ++w) {
-                        mq.w[w] = st.m[w];
-                    }
-                    int64_t idx;
-                    if (ADJ) {
-                        idx = prow * p.n_pairs + pq;
-                        sg[k] = fermi ? base ^ masked_parity<W>(ca, mq) : 0;
-                    } else {
-                        // c_b = core + {q0, q1}: q0 lies in [q0, q1), q1 does not
-                        idx = pq * p.n_pairs + prow;
-                        const int in0 = (q0 >= r0 && q0 < r1) ? 1 : 0;
-                        const int in1 = (q1 >= r0 && q1 < r1) ? 1 : 0;
-                        sg[k] = fermi ? (base ^ masked_parity<W>(core, mq) ^ in0 ^ in1 ^ 1) : 0;
-                    }
-                    x[k] = !o ? make_double2(0.0, 0.0) : (p.coef_re ? make_double2(p.coef_re[idx], 0.0) : p.coef[idx]);
+                    // slots past nb hold entries of the previous chunk or nothing: bounded by u + k < nb
+                    ok[k] = pair_term<W, ADJ, REAL>(p, ra, st, mixed, fmask, act && (u + k < nb), x[k]);
                 }
 #pragma unroll
                 for (int k = 0; k < B; ++k) {
-                    if (ok[k]) {
-                        const double xr = sg[k] ? -x[k].x : x[k].x;
-                        const double xi = sg[k] ? -x[k].y : x[k].y;
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) {
-                            const double2 gg = v == 0 ? g[k] : g2[k];
-                            if (ADJ) {  // conj(x) * g
-                                acc_re[v] += xr * gg.x + xi * gg.y;
-                                acc_im[v] += xr * gg.y - xi * gg.x;
-                            } else {    // x * g
-                                acc_re[v] += xr * gg.x - xi * gg.y;
-                                acc_im[v] += xr * gg.y + xi * gg.x;
-                            }
-                        }
+                    // x[k] is zero where there is no connection, so the multiply-add needs no select
+                    // (a non-finite input amplitude then also reaches the rows of its group it is
+                    // not connected to)
+                    (void)ok[k];
+                    pair_madd<ADJ, REAL>(x[k], g[k], acc_re[0], acc_im[0]);
+                    if (NV == 2) {
+                        pair_madd<ADJ, REAL>(x[k], g2[k], acc_re[NV - 1], acc_im[NV - 1]);
                     }
                 }
             }
@@ -623,12 +689,13 @@ __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane
 // l / H.  A group's entries are its rows, so each lane stages its own entry in its LDS slot and the
 // rows of a group walk the slots of that group, as in pair_group.  Packing keeps the lanes busy
 // where one group per wave would leave most of them idle (a 20-entry group uses 20 of 64).
-template <int W, bool ADJ, int B, int NV, int H>
+template <int W, bool ADJ, int B, int NV, int H, bool REAL>
 __device__ void pair_group_packed(const PairParams& p, uint64_t mask, uint32_t gs_l, uint32_t ge_l, int lane, PairStage<W>* stage) {
     constexpr int kPer = 64 / H;
     const int gi = lane / H;
     const int ri = lane % H;
     const bool fermi = p.particle_cut == 1;
+    const int fmask = fermi ? 1 : 0;
     while (mask) {
         uint32_t s = 0, e = 0;
         int nmax = 0;
@@ -655,99 +722,37 @@ __device__ void pair_group_packed(const PairParams& p, uint64_t mask, uint32_t g
         for (int w = 0; w < W; ++w) {
             ca.w[w] = act ? p.ckeys[ia * W + w] : 0ull;
         }
-        Cfg<W> core = ca;
-        if (act) {
-            flip_site<W>(core, r0);
-            flip_site<W>(core, r1);
-        }
-        // this lane's entry, staged for the rows of its group
-        {
-            PairStage<W> st;
-            st.row = static_cast<uint32_t>(ia);
-            st.g = act ? p.in[ia] : make_double2(0.0, 0.0);
-            st.g2 = (NV == 2 && act) ? p.in2[ia] : make_double2(0.0, 0.0);
-            const Cfg<W> mq = range_mask<W>(r0, r1);
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                st.m[w] = act ? mq.w[w] : 0ull;
-            }
-            st.packed = act ? (static_cast<uint32_t>(r0) | (static_cast<uint32_t>(r1) << 8) |
-                               (static_cast<uint32_t>(pair_slot(r0, r1, p.n_bits)) << 16))
-                            : 0u;
-            stage[lane] = st;
-        }
-        // Different cores with equal hashes share a group: then every pair is checked exactly.
+        const PairRow<W> ra = pair_row<W>(p, ca, r0, r1, ADJ, fermi);
+        // the group's first core, from the lane holding its first row (every lane runs every shuffle)
+        Cfg<W> core0;
         bool same = true;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            const bool eq = shfl_u64(core.w[w], gi * H) == core.w[w];
-            same = same & eq;  // every lane runs every shuffle (a short-circuit would leave lanes out)
+            core0.w[w] = shfl_u64(ra.core.w[w], gi * H);
+            same = same & (core0.w[w] == ra.core.w[w]);
         }
         const bool mixed = __ballot(act && !same) != 0ull;
-        const int64_t prow = pair_slot(r0, r1, p.n_bits);
-        const Cfg<W> mr = range_mask<W>(r0, r1);
-        const int base = fermi ? masked_parity<W>(ADJ ? ca : core, mr) : 0;
+        // this lane's entry (its own row), staged for the rows of its group
+        stage[lane] = stage_entry<W, ADJ, NV>(p, act, va, core0, fermi);
         double acc_re[NV] = {}, acc_im[NV] = {};
         __builtin_amdgcn_wave_barrier();
         for (int u = 0; u < nmax; u += B) {
             double2 x[B], gv[B], gv2[B];
-            int sg[B];
             bool ok[B];
 #pragma unroll
             for (int k = 0; k < B; ++k) {
                 const int uu = u + k;
                 const PairStage<W>& st = stage[gi * H + (uu & (H - 1))];
-                const uint32_t pk = st.packed;
-                const int q0 = static_cast<int>(pk & 0xff);
-                const int q1 = static_cast<int>((pk >> 8) & 0xff);
-                const int64_t pq = static_cast<int64_t>(pk >> 16);
                 gv[k] = st.g;
                 gv2[k] = NV == 2 ? st.g2 : make_double2(0.0, 0.0);
-                bool o = act && uu < g && uu != ri && q0 != r0 && q0 != r1 && q1 != r0 && q1 != r1;
-                if (mixed && o) {
-                    Cfg<W> cb = load_key<W>(p.ckeys, static_cast<int64_t>(st.row));
-                    flip_site<W>(cb, q0);
-                    flip_site<W>(cb, q1);
-#pragma unroll
-                    for (int w = 0; w < W; ++w) {
-                        o = o && cb.w[w] == core.w[w];
-                    }
-                }
-                ok[k] = o;
-                Cfg<W> mq;
-#pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    mq.w[w] = st.m[w];
-                }
-                int64_t idx;
-                if (ADJ) {
-                    idx = prow * p.n_pairs + pq;
-                    sg[k] = fermi ? base ^ masked_parity<W>(ca, mq) : 0;
-                } else {
-                    // c_b = core + {q0, q1}: q0 lies in [q0, q1), q1 does not
-                    idx = pq * p.n_pairs + prow;
-                    const int in0 = (q0 >= r0 && q0 < r1) ? 1 : 0;
-                    const int in1 = (q1 >= r0 && q1 < r1) ? 1 : 0;
-                    sg[k] = fermi ? (base ^ masked_parity<W>(core, mq) ^ in0 ^ in1 ^ 1) : 0;
-                }
-                x[k] = !o ? make_double2(0.0, 0.0) : (p.coef_re ? make_double2(p.coef_re[idx], 0.0) : p.coef[idx]);
+                ok[k] = pair_term<W, ADJ, REAL>(p, ra, st, mixed, fmask, act && uu < g, x[k]);
             }
 #pragma unroll
             for (int k = 0; k < B; ++k) {
-                if (ok[k]) {
-                    const double xr = sg[k] ? -x[k].x : x[k].x;
-                    const double xi = sg[k] ? -x[k].y : x[k].y;
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) {
-                        const double2 gg = v == 0 ? gv[k] : gv2[k];
-                        if (ADJ) {  // conj(x) * g
-                            acc_re[v] += xr * gg.x + xi * gg.y;
-                            acc_im[v] += xr * gg.y - xi * gg.x;
-                        } else {    // x * g
-                            acc_re[v] += xr * gg.x - xi * gg.y;
-                            acc_im[v] += xr * gg.y + xi * gg.x;
-                        }
-                    }
+                (void)ok[k];  // x[k] is zero where there is no connection
+                pair_madd<ADJ, REAL>(x[k], gv[k], acc_re[0], acc_im[0]);
+                if (NV == 2) {
+                    pair_madd<ADJ, REAL>(x[k], gv2[k], acc_re[NV - 1], acc_im[NV - 1]);
                 }
             }
         }
@@ -870,6 +875,23 @@ __device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_
     }
 }
 
+// The groups above 8 entries of a 64-group chunk: packed several to a wave up to 32 entries (PACK),
+// then one wave per group.
+template <int W, bool ADJ, int B, int NV, bool PACK, bool REAL>
+__device__ __forceinline__ void pair_chunk_groups(const PairParams& p, uint32_t gs, uint32_t ge, uint32_t sz, int lane,
+                                                  PairStage<W>* stage) {
+    if (PACK) {
+        pair_group_packed<W, ADJ, B, NV, 16, REAL>(p, __ballot(sz > 8u && sz <= 16u), gs, ge, lane, stage);
+        pair_group_packed<W, ADJ, B, NV, 32, REAL>(p, __ballot(sz > 16u && sz <= 32u), gs, ge, lane, stage);
+    }
+    uint64_t big = __ballot(sz > (PACK ? 32u : 8u));
+    while (big) {
+        const int l = __builtin_ctzll(big);
+        big &= big - 1;
+        pair_group<W, ADJ, B, NV, REAL>(p, __shfl(gs, l), __shfl(ge, l), lane, stage);
+    }
+}
+
 // The pair work of the groups [g0, g0 + 64): groups of up to 8 entries through the tiny-group
 // classes, larger ones a row block at a time (pair_group).
 template <int W, bool ADJ, int B, int NV, bool PACK = true>
@@ -888,15 +910,11 @@ __device__ __forceinline__ void pair_chunk(const PairParams& p, int64_t groups, 
     PairStage<W>* stage = reinterpret_cast<PairStage<W>*>(stage_bytes + (threadIdx.x >> 6) * 64 * sizeof(PairStage<2>));
     // PACK: groups of 9..32 entries several to a wave (the batch kernel, at its register limit,
     // keeps them one per wave)
-    if (PACK) {
-        pair_group_packed<W, ADJ, B, NV, 16>(p, __ballot(sz > 8u && sz <= 16u), gs, ge, lane, stage);
-        pair_group_packed<W, ADJ, B, NV, 32>(p, __ballot(sz > 16u && sz <= 32u), gs, ge, lane, stage);
-    }
-    uint64_t big = __ballot(sz > (PACK ? 32u : 8u));
-    while (big) {
-        const int l = __builtin_ctzll(big);
-        big &= big - 1;
-        pair_group<W, ADJ, B, NV>(p, __shfl(gs, l), __shfl(ge, l), lane, stage);
+    // a real table (wave-uniform) takes the loops specialised for 8-byte coefficients
+    if (p.coef_re != nullptr) {
+        pair_chunk_groups<W, ADJ, B, NV, PACK, true>(p, gs, ge, sz, lane, stage);
+    } else {
+        pair_chunk_groups<W, ADJ, B, NV, PACK, false>(p, gs, ge, sz, lane, stage);
     }
 }
 
@@ -1448,9 +1466,9 @@ __global__ __launch_bounds__(kBlock) void diag_apply_kernel(const uint64_t* __re
     }
 }
 
-// The same diagonal on two vectors in one pass (the adjoint, conj(d)): the config words and the
-// folded table are read once for both.
-template <int W>
+// The same diagonal on two vectors in one pass (d, or conj(d) for the adjoint): the config words
+// and the folded table are read once for both.
+template <int W, bool ADJ>
 __global__ __launch_bounds__(kBlock) void diag_apply2_kernel(const uint64_t* __restrict__ ckeys, int64_t n,
                                                              const double2* __restrict__ diag, int n_bits,
                                                              const double2* __restrict__ in, double2* __restrict__ out,
@@ -1460,13 +1478,14 @@ __global__ __launch_bounds__(kBlock) void diag_apply2_kernel(const uint64_t* __r
         return;
     }
     const double2 d = diag_value<W, 8>(load_key<W>(ckeys, i), diag, n_bits);
-    if (d.x != 0.0 || d.y != 0.0) {
+    const double im = ADJ ? -d.y : d.y;
+    if (d.x != 0.0 || im != 0.0) {
         const double2 g = in[i];
         const double2 g2 = in2[i];
-        out[i].x += d.x * g.x + d.y * g.y;
-        out[i].y += d.x * g.y - d.y * g.x;
-        out2[i].x += d.x * g2.x + d.y * g2.y;
-        out2[i].y += d.x * g2.y - d.y * g2.x;
+        out[i].x += d.x * g.x - im * g.y;
+        out[i].y += d.x * g.y + im * g.x;
+        out2[i].x += d.x * g2.x - im * g2.y;
+        out2[i].y += d.x * g2.y + im * g2.x;
     }
 }
 
@@ -1894,8 +1913,11 @@ int apply_diagonal(const qmb_pairs* pairs, const qmb_targets* targets, const dou
 }
 
 int apply_structured_dual(const qmb_pairs* pairs, const qmb_singles* singles, const qmb_cores* cores2, const qmb_cores* cores1,
-                          const qmb_targets* targets, const double* in, double* out, const double* in2, double* out2,
+                          const qmb_targets* targets, const double* in, double* out, const double* in2, double* out2, int mode,
                           hipStream_t stream) {
+    if (mode != 0 && mode != 1) {
+        return set_error(QMB_ERR_ARGUMENT, "mode must be 0 (forward) or 1 (adjoint)");
+    }
     if (!pairs || !cores2 || !targets || !in || !out || !in2 || !out2 || (singles && !cores1)) {
         return set_error(QMB_ERR_ARGUMENT, "pairs, cores2, targets, the four vectors (and cores1 with singles) must be non-null");
     }
@@ -1915,10 +1937,17 @@ int apply_structured_dual(const qmb_pairs* pairs, const qmb_singles* singles, co
     auto* vo = reinterpret_cast<double2*>(out);
     auto* vo2 = reinterpret_cast<double2*>(out2);
     const auto* dg = reinterpret_cast<const double2*>(pairs->diag);
+    const unsigned dgrid = blocks_for(targets->n);
     if (W == 1) {
-        diag_apply2_kernel<1><<<blocks_for(targets->n), kBlock, 0, stream>>>(targets->keys, targets->n, dg, pairs->n_bits, vi, vo, vi2, vo2);
+        if (mode == 1) {
+            diag_apply2_kernel<1, true><<<dgrid, kBlock, 0, stream>>>(targets->keys, targets->n, dg, pairs->n_bits, vi, vo, vi2, vo2);
+        } else {
+            diag_apply2_kernel<1, false><<<dgrid, kBlock, 0, stream>>>(targets->keys, targets->n, dg, pairs->n_bits, vi, vo, vi2, vo2);
+        }
+    } else if (mode == 1) {
+        diag_apply2_kernel<2, true><<<dgrid, kBlock, 0, stream>>>(targets->keys, targets->n, dg, pairs->n_bits, vi, vo, vi2, vo2);
     } else {
-        diag_apply2_kernel<2><<<blocks_for(targets->n), kBlock, 0, stream>>>(targets->keys, targets->n, dg, pairs->n_bits, vi, vo, vi2, vo2);
+        diag_apply2_kernel<2, false><<<dgrid, kBlock, 0, stream>>>(targets->keys, targets->n, dg, pairs->n_bits, vi, vo, vi2, vo2);
     }
     QMB_CHECK_LAUNCH();
     if (singles && cores1->capacity > 0) {
@@ -1940,9 +1969,9 @@ int apply_structured_dual(const qmb_pairs* pairs, const qmb_singles* singles, co
         const int64_t want = (cores1->capacity + 64 * 4 - 1) / (64 * 4);
         const unsigned grid = static_cast<unsigned>(want < 4096 ? (want < 1 ? 1 : want) : 4096);
         if (W == 1) {
-            single_kernel<1, true, 2><<<grid, kBlock, 0, stream>>>(p);
+            mode == 1 ? single_kernel<1, true, 2><<<grid, kBlock, 0, stream>>>(p) : single_kernel<1, false, 2><<<grid, kBlock, 0, stream>>>(p);
         } else {
-            single_kernel<2, true, 2><<<grid, kBlock, 0, stream>>>(p);
+            mode == 1 ? single_kernel<2, true, 2><<<grid, kBlock, 0, stream>>>(p) : single_kernel<2, false, 2><<<grid, kBlock, 0, stream>>>(p);
         }
         QMB_CHECK_LAUNCH();
     }
@@ -1964,9 +1993,9 @@ int apply_structured_dual(const qmb_pairs* pairs, const qmb_singles* singles, co
         const int64_t want = (cores2->capacity + 64 * 4 - 1) / (64 * 4);
         const unsigned grid = static_cast<unsigned>(want < 4096 ? (want < 1 ? 1 : want) : 4096);
         if (W == 1) {
-            pair_kernel<1, true, 2><<<grid, kBlock, 0, stream>>>(p);
+            mode == 1 ? pair_kernel<1, true, 2><<<grid, kBlock, 0, stream>>>(p) : pair_kernel<1, false, 2><<<grid, kBlock, 0, stream>>>(p);
         } else {
-            pair_kernel<2, true, 2><<<grid, kBlock, 0, stream>>>(p);
+            mode == 1 ? pair_kernel<2, true, 2><<<grid, kBlock, 0, stream>>>(p) : pair_kernel<2, false, 2><<<grid, kBlock, 0, stream>>>(p);
         }
         QMB_CHECK_LAUNCH();
     }
@@ -2093,10 +2122,11 @@ int apply_batch(const qmb_batch* batch, int mode, int waves, hipStream_t stream)
     const auto* items = static_cast<const BatchItemDev*>(batch->items);
     const unsigned g = static_cast<unsigned>(blocks);
     if (batch->dual) {
-        if (mode != 1) {
-            return set_error(QMB_ERR_ARGUMENT, "a two-vector batch applies the adjoint gather (mode 1) to both vectors");
+        if (mode == 1) {
+            batch_kernel<true, 2><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
+        } else {
+            batch_kernel<false, 2><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
         }
-        batch_kernel<true, 2><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
     } else if (mode == 1) {
         batch_kernel<true, 1><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
     } else {

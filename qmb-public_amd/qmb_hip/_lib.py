@@ -224,7 +224,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "qmb_apply_structured_dual": (
         _c_int,
         [ctypes.POINTER(QmbPairs), ctypes.POINTER(QmbSingles), ctypes.POINTER(QmbCores), ctypes.POINTER(QmbCores),
-         ctypes.POINTER(QmbTargets), _vp, _vp, _vp, _vp, _vp],
+         ctypes.POINTER(QmbTargets), _vp, _vp, _vp, _vp, _c_int, _vp],
     ),
     "qmb_batch_workspace_size": (_c_int, [_c_int, ctypes.POINTER(_c_size)]),
     "qmb_batch_plan": (_c_int, [ctypes.POINTER(QmbBatchItem), _c_int, _vp, _c_size, ctypes.POINTER(QmbBatch), _vp]),

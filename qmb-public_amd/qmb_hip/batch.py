@@ -125,10 +125,10 @@ class Batch:
         stream = _stream(self.device)
         w = self.waves if waves is None else waves
         with torch.cuda.device(self.device):
-            _lib.check(lib.qmb_apply_batch(ctypes.byref(self.plan), pp.MODE_ADJOINT, w, stream), "qmb_apply_batch")
+            _lib.check(lib.qmb_apply_batch(ctypes.byref(self.plan), pp.HERMITIAN_MODE, w, stream), "qmb_apply_batch")
             for k in self.residual:
                 for inp, out in ((self.inp, self.out), (self.inp2, self.out2)):
-                    self._residual(k, pp.MODE_ADJOINT, inp, out, stream)
+                    self._residual(k, pp.HERMITIAN_MODE, inp, out, stream)
 
     def _residual(self, k: int, mode: int, inp: torch.Tensor, out: torch.Tensor, stream: ctypes.c_void_p) -> None:
         terms, targets = self.instances[k]
@@ -147,7 +147,7 @@ class Batch:
         self.out.zero_()
         lib = _lib.library()
         stream = _stream(self.device)
-        pmode = pp.MODE_ADJOINT if self.hermitian else mode
+        pmode = pp.HERMITIAN_MODE if self.hermitian else mode
         w = self.waves if waves is None else waves
         with torch.cuda.device(self.device):
             _lib.check(lib.qmb_apply_batch(ctypes.byref(self.plan), pmode, w, stream), "qmb_apply_batch")

@@ -38,6 +38,11 @@ def _device_count(buffer: torch.Tensor, counts_ptr: int) -> int:
     return int(buffer[offset:offset + 8].view(torch.int64).item())
 
 
+# A Hermitian table gives the same map in both modes; its structured kernels run in the forward
+# orientation, where a coefficient load reads one table row with the group's columns ascending
+# across the lanes (the adjoint orientation reads one column of many rows).
+HERMITIAN_MODE = MODE_FORWARD
+
 # qmb_launch.symmetric values
 SYMMETRIC_OFF = 0
 SYMMETRIC_GATHER = 1   # forward computed as the adjoint gather (A = A^H): no atomics per contribution
@@ -275,8 +280,8 @@ def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tenso
             _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.desc), ctypes.byref(targets.desc), _ptr(configs_i), n_i, _ptr(vec),
                                               _ptr(out), mode, ctypes.byref(launch) if launch else None, stream), "qmb_apply_prepared")
             return out
-        # a Hermitian table: forward and adjoint are the same map, computed as the adjoint gather
-        pmode = MODE_ADJOINT if symmetric else mode
+        # a Hermitian table: forward and adjoint are the same map, applied in the faster orientation
+        pmode = HERMITIAN_MODE if symmetric else mode
         if terms.n_residual_terms > 0:
             _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.residual_desc), ctypes.byref(targets.desc), _ptr(configs_i), n_i,
                                               _ptr(vec), _ptr(out), mode, ctypes.byref(launch) if launch else None, stream),
@@ -291,10 +296,14 @@ def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tenso
     return out
 
 
-def apply_dual(terms: PreparedTerms, targets: PreparedTargets, vec: torch.Tensor, vec2: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
-    """A^H vec and A^H vec2 on the indexed set in one pass per kernel (qmb_apply_structured_dual): for a
-    Hermitian table the forward and the backward of a training step.  Needs the structured path
-    (pair excitations, distinct rows) and no residual terms."""
+def apply_dual(terms: PreparedTerms, targets: PreparedTargets, vec: torch.Tensor, vec2: torch.Tensor,
+               mode: int = MODE_ADJOINT) -> tuple[torch.Tensor, torch.Tensor]:
+    """A^H vec and A^H vec2 (mode 1; A with mode 0) on the indexed set in one pass per kernel
+    (qmb_apply_structured_dual): for a Hermitian table the forward and the backward of a training
+    step, then run in HERMITIAN_MODE.  Needs the structured path (pair excitations, distinct rows)
+    and no residual terms."""
+    if terms.hermitian:
+        mode = HERMITIAN_MODE
     if not can_use_pairs(terms, targets, targets.configs) or terms.n_residual_terms > 0:
         raise RuntimeError("apply_dual needs the structured path: pair excitations, distinct rows and no residual terms.")
     vec, vec2 = vec.contiguous(), vec2.contiguous()
@@ -305,7 +314,7 @@ def apply_dual(terms: PreparedTerms, targets: PreparedTargets, vec: torch.Tensor
     with torch.cuda.device(targets.device):
         _lib.check(_lib.library().qmb_apply_structured_dual(ctypes.byref(terms.pairs_desc), singles, ctypes.byref(targets.cores(2)), cores1,
                                                             ctypes.byref(targets.desc), _ptr(vec), _ptr(out), _ptr(vec2), _ptr(out2),
-                                                            _stream(targets.device)), "qmb_apply_structured_dual")
+                                                            mode, _stream(targets.device)), "qmb_apply_structured_dual")
     return out, out2
 
 
