@@ -305,14 +305,14 @@ int qmb_terms_build_residual(int n_qubytes, int particle_cut, const int16_t* sit
  * write NaN into out[0] instead of a wrong result). */
 typedef struct qmb_cores {
     int32_t n_qubytes, n_words, max_occupied, per_config; /* per_config = C(max_occupied, order) */
-    int32_t order, key_bits;  /* key_bits: width of the sorted keys (set by qmb_cores_build) */
+    int32_t order, key_bits;  /* key_bits: width of the sorted keys, 32 (set by qmb_cores_build) */
     int64_t n;
     int64_t capacity;  /* n * per_config entries */
     void* buffer;
     size_t bytes;
-    uint64_t* keys;    /* [capacity] sorted keys: core class << (34 + s) | 34-bit core hash << s |
-                          removed pair slot (order 1: removed site) in the low s = key_bits - 36 bits (the
-                          fewest that hold n_pairs, or n_bits); unused entries last (all key_bits ones) */
+    uint32_t* keys;    /* [capacity] sorted keys: core class (even sites of the core mod 3) * 0x55555555 + the
+                          64-bit core hash mod 0x55555555; a group is a run of equal keys; unused entries last
+                          (0xFFFFFFFF) */
     uint64_t* vals;    /* [capacity] row << 16 | site1 << 8 | site0 of the removed sites (order 1: site0) */
     uint32_t* starts;  /* [capacity + 1] first entry of each group, then the end */
     int64_t* counts;   /* device: {entries, groups, overflow, 0} */

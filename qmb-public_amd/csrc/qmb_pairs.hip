@@ -21,26 +21,20 @@ namespace qmb {
 namespace {
 
 constexpr int kBlock = 256;
-// Sort key of a core entry: 2 bits of core class, 34 bits of the core hash, 20 bits of the removed
-// pair's slot (or the removed site).  The core class is the number of even sites of the core mod 3,
-// a function of the core, so a group never splits.  In a set where every config has the same
-// numbers of even and odd occupied sites (spin sectors) it tells the pair class of all the group's
-// entries, so groups come in class order -- waves running side by side read one block of the
-// coefficient table -- and inside a group the entries ascend by slot, reading table rows in order.
-constexpr int kKeyBits = 56;   // the widest key: 2 + kHashBits + 20 sub bits
-constexpr int kHashBits = 34;
-// The sub field holds the removed pair's slot (order 2, < n_pairs) or the removed site (order 1,
-// < n_bits), in as few bits as that needs: the radix sort then runs over 2 + 34 + sub_bits bits
-// (47 at 56 sites instead of 56), one pass fewer.
-inline int core_sub_bits(int n_bits, int order) {
-    const int64_t range = order == 2 ? static_cast<int64_t>(n_bits) * (n_bits - 1) / 2 : n_bits;
-    int b = 1;
-    while ((1ll << b) < range) {
-        ++b;
-    }
-    return b;
-}
-__host__ __device__ __forceinline__ uint64_t key_sentinel(int key_bits) { return (1ull << key_bits) - 1ull; }  // sorts last
+// Sort key of a core entry, 32 bits: the core class (the number of even sites of the core mod 3)
+// times kHashRange, plus the 64-bit core hash mod kHashRange.  The class is a function of the core,
+// so a group never splits; in a set where every config has the same numbers of even and odd
+// occupied sites (spin sectors) it tells the pair class of all the group's entries, so groups come
+// in class order and waves running side by side read one block of the coefficient table.  Cores
+// with equal keys share a group ("mixed"), where the kernels compare cores exactly; the modulus
+// breaks the hash's linearity, so such merges are chance events (about 1 group in 100 at 12M cores
+// per class) with no structure.  Unused entries get kKeySentinel and sort last.  A 32-bit key
+// sorts in four 8-bit radix passes, with the packed (row, removed sites) value riding along; the
+// sort is stable, so inside a group the entries keep the order they were emitted in: by the rank
+// of the removed pair (order 1: site) among the row's own, then by row.
+constexpr uint32_t kHashRange = 0x55555555u;  // 3 * kHashRange = 2^32 - 1
+constexpr uint32_t kKeySentinel = 0xFFFFFFFFu;
+constexpr int kKeyBits = 32;
 constexpr int kMaxPairBytes = 16;            // the dense table is n_pairs^2 entries
 
 inline unsigned blocks_for(int64_t n, int block = kBlock) {
@@ -209,68 +203,99 @@ __device__ __forceinline__ void flip_site(Cfg<W>& c, int s) {
     }
 }
 
-__device__ __forceinline__ uint64_t core_key(uint64_t h, uint64_t cls, uint64_t sub, int sub_bits) {
-    constexpr uint64_t kTop = (1ull << (2 + kHashBits)) - 1ull;
-    uint64_t top = (cls << kHashBits) | (h & ((1ull << kHashBits) - 1ull));
-    top = top == kTop ? top - 1 : top;  // a merged hash value only makes its group "mixed"
-    return (top << sub_bits) | sub;
+__device__ __forceinline__ uint32_t core_key(uint64_t h, uint32_t cls) {
+    return cls * kHashRange + static_cast<uint32_t>(h % kHashRange);
 }
 
-__device__ __forceinline__ uint64_t group_of(uint64_t key, int sub_bits) { return key >> sub_bits; }
-
 // One thread per config: every occupied pair (x < y) -- ORDER 2 -- or every occupied site x --
-// ORDER 1 -- gives the entry (hash of the config without those sites, row); unused slots of a row
-// with fewer occupied sites get the sentinel key.
+// ORDER 1 -- gives the entry (key of the config without those sites, row << 16 | y << 8 | x); unused
+// slots of a row with fewer occupied sites get the sentinel key.  Entry k of row i sits at k * n + i,
+// so the lanes of a wave store to consecutive addresses.  A row's pairs are emitted in pair_slot
+// order (even-even pairs, then odd-odd, each by the higher site first; then even-odd, by the even
+// site first) and its sites in ascending order.  Rows of one group share the core, so a pair's rank
+// among its row's pairs grows with its slot (ties where no core site lies between two slots), and
+// the stable sort leaves each group's entries close to ascending slots: the lanes of a wave then
+// read nearby columns of a coefficient table row.
 template <int W, int ORDER>
 __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64_t* __restrict__ cols, int64_t n, int n_bits,
-                                 int per_config, int sub_bits, uint64_t* __restrict__ okeys, uint64_t* __restrict__ ovals,
+                                 int per_config, uint32_t* __restrict__ okeys, uint64_t* __restrict__ ovals,
                                  int64_t* __restrict__ counts) {
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i >= n) {
         return;
     }
+    constexpr uint64_t kEven = 0x5555555555555555ull;
     const Cfg<W> c = load_key<W>(keys, i);
     const uint64_t h = hash_cfg<W>(c, cols);
-    // entry k of row i sits at k * n + i: the lanes of a wave store to consecutive addresses (the
-    // sort that follows does not care about the input order)
     int even = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-        even += __popcll(c.w[w] & 0x5555555555555555ull);
+        even += __popcll(c.w[w] & kEven);
     }
     int64_t k = 0;
+    auto emit = [&](uint64_t key, uint64_t val) {
+        if (k < per_config) {
+            okeys[k * n + i] = static_cast<uint32_t>(key);
+            ovals[k * n + i] = (static_cast<uint64_t>(i) << 16) | val;
+        }
+        ++k;
+    };
+    if (ORDER == 1) {
 #pragma unroll
-    for (int wx = 0; wx < W; ++wx) {
-        uint64_t mx = c.w[wx];
-        while (mx) {
-            const int x = 64 * wx + __builtin_ctzll(mx);
-            mx &= mx - 1;
-            const uint64_t hx = h ^ cols[x];
-            if (ORDER == 1) {
-                if (k < per_config) {
-                    okeys[k * n + i] = core_key(hx, 0, static_cast<uint64_t>(x), sub_bits);
-                    ovals[k * n + i] = (static_cast<uint64_t>(i) << 16) | static_cast<uint64_t>(x);
-                }
-                ++k;
-                continue;
+        for (int wx = 0; wx < W; ++wx) {
+            uint64_t mx = c.w[wx];
+            while (mx) {
+                const int x = 64 * wx + __builtin_ctzll(mx);
+                mx &= mx - 1;
+                emit(core_key(h ^ cols[x], 0u), static_cast<uint64_t>(x));
             }
+        }
+    } else {
+        // same-parity pairs: the core loses two even sites (parity 0) or none (parity 1)
+        for (int par = 0; par < 2; ++par) {
+            const uint64_t pm = par ? ~kEven : kEven;
+            const uint32_t cls = static_cast<uint32_t>((even - (par ? 0 : 2)) % 3);
 #pragma unroll
             for (int wy = 0; wy < W; ++wy) {
-                if (wy < wx) {
-                    continue;
-                }
-                uint64_t my = wy == wx ? mx : c.w[wy];
+                uint64_t my = c.w[wy] & pm;
                 while (my) {
                     const int y = 64 * wy + __builtin_ctzll(my);
                     my &= my - 1;
-                    if (k < per_config) {
-                        const int core_even = even - ((x & 1) ? 0 : 1) - ((y & 1) ? 0 : 1);
-                        okeys[k * n + i] = core_key(hx ^ cols[y], static_cast<uint64_t>(core_even % 3),
-                                                   static_cast<uint64_t>(pair_slot(x, y, n_bits)), sub_bits);
-                        ovals[k * n + i] = (static_cast<uint64_t>(i) << 16) | (static_cast<uint64_t>(y) << 8) |
-                                          static_cast<uint64_t>(x);
+                    const uint64_t hy = h ^ cols[y];
+#pragma unroll
+                    for (int wx = 0; wx < W; ++wx) {
+                        if (wx > wy) {
+                            continue;
+                        }
+                        uint64_t mx = c.w[wx] & pm & (wx == wy ? (1ull << (y & 63)) - 1ull : ~0ull);
+                        while (mx) {
+                            const int x = 64 * wx + __builtin_ctzll(mx);
+                            mx &= mx - 1;
+                            emit(core_key(hy ^ cols[x], cls), (static_cast<uint64_t>(y) << 8) | static_cast<uint64_t>(x));
+                        }
                     }
-                    ++k;
+                }
+            }
+        }
+        // even-odd pairs: the core loses one even site
+        const uint32_t cls = static_cast<uint32_t>((even - 1) % 3);
+#pragma unroll
+        for (int we = 0; we < W; ++we) {
+            uint64_t me = c.w[we] & kEven;
+            while (me) {
+                const int ev = 64 * we + __builtin_ctzll(me);
+                me &= me - 1;
+                const uint64_t he = h ^ cols[ev];
+#pragma unroll
+                for (int wo = 0; wo < W; ++wo) {
+                    uint64_t mo = c.w[wo] & ~kEven;
+                    while (mo) {
+                        const int od = 64 * wo + __builtin_ctzll(mo);
+                        mo &= mo - 1;
+                        const int x = ev < od ? ev : od;
+                        const int y = ev < od ? od : ev;
+                        emit(core_key(he ^ cols[od], cls), (static_cast<uint64_t>(y) << 8) | static_cast<uint64_t>(x));
+                    }
                 }
             }
         }
@@ -279,37 +304,35 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
         atomicAdd(reinterpret_cast<unsigned long long*>(&counts[2]), 1ull);
     }
     for (; k < per_config; ++k) {
-        okeys[k * n + i] = key_sentinel(2 + kHashBits + sub_bits);
+        okeys[k * n + i] = kKeySentinel;
         ovals[k * n + i] = 0;
     }
 }
 
-__global__ void core_flag_kernel(const uint64_t* __restrict__ keys, int64_t cap, int sub_bits, uint32_t* __restrict__ flag) {
+__global__ void core_flag_kernel(const uint32_t* __restrict__ keys, int64_t cap, uint32_t* __restrict__ flag) {
     const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (e >= cap) {
         return;
     }
-    const uint64_t k = keys[e];
-    flag[e] = (k != key_sentinel(2 + kHashBits + sub_bits) && (e == 0 || group_of(keys[e - 1], sub_bits) != group_of(k, sub_bits))) ? 1u
-                                                                                                                                   : 0u;
+    const uint32_t k = keys[e];
+    flag[e] = (k != kKeySentinel && (e == 0 || keys[e - 1] != k)) ? 1u : 0u;
 }
 
 // starts[g] = first entry of group g; starts[G] = number of used entries; counts {entries, G}.
-__global__ void core_start_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flag,
-                                  const uint32_t* __restrict__ gid, int64_t cap, uint64_t sentinel, uint32_t* __restrict__ starts,
-                                  int64_t* __restrict__ counts) {
+__global__ void core_start_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ flag,
+                                  const uint32_t* __restrict__ gid, int64_t cap, uint32_t* __restrict__ starts, int64_t* __restrict__ counts) {
     const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (e >= cap) {
         return;
     }
-    const uint64_t k = keys[e];
-    if (k == sentinel) {
+    const uint32_t k = keys[e];
+    if (k == kKeySentinel) {
         return;
     }
     if (flag[e]) {
         starts[gid[e]] = static_cast<uint32_t>(e);
     }
-    if (e == cap - 1 || keys[e + 1] == sentinel) {
+    if (e == cap - 1 || keys[e + 1] == kKeySentinel) {
         const uint32_t groups = gid[e] + flag[e];
         starts[groups] = static_cast<uint32_t>(e + 1);
         counts[0] = e + 1;
@@ -343,14 +366,14 @@ int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, int ord
     const size_t cap = static_cast<size_t>(c.capacity > 0 ? c.capacity : 1);
     c.buffer = cv.base;
     c.counts = cv.take<int64_t>(4);
-    c.keys = cv.take<uint64_t>(cap);
+    c.keys = cv.take<uint32_t>(cap);
     c.vals = cv.take<uint64_t>(cap);
     c.starts = cv.take<uint32_t>(cap + 1);
     size_t tmp = 0;
     {
-        uint64_t* k = nullptr;
+        uint64_t* v = nullptr;
         uint32_t* u = nullptr;
-        if (rocprim::radix_sort_pairs(nullptr, tmp, k, k, k, k, static_cast<unsigned>(cap), 0, kKeyBits) != hipSuccess) {
+        if (rocprim::radix_sort_pairs(nullptr, tmp, u, u, v, v, static_cast<unsigned>(cap), 0, kKeyBits) != hipSuccess) {
             return set_error(QMB_ERR_HIP, "rocprim radix_sort_pairs size query failed");
         }
         size_t scan = 0;
@@ -360,7 +383,7 @@ int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, int ord
         tmp = tmp > scan ? tmp : scan;
     }
     const size_t s0 = cv.off;
-    void* s = cv.take_bytes(2 * cap * sizeof(uint64_t) + 2 * cap * sizeof(uint32_t) + tmp + 4096);
+    void* s = cv.take_bytes(cap * sizeof(uint64_t) + 3 * cap * sizeof(uint32_t) + tmp + 4096);
     if (scratch) {
         *scratch = s;
     }
@@ -606,8 +629,7 @@ __device__ __forceinline__ void pair_madd(const double2& x, const double2& g, do
 // staged 64 at a time in this wave's LDS slots and read back as broadcasts.
 //   ADJ:  out[a] += sum_b conj(A_ba) in[b]
 //   !ADJ: out[a] += sum_b A_ab in[b]
-// !ADJ reads T[Pb][Pa]: one table row per entry, the rows' columns ascending across the lanes
-// (entries ascend by pair slot inside a group), so a coefficient load touches few cache lines.
+// !ADJ reads T[Pb][Pa]: one table row per entry, at the columns of the lanes' rows.
 template <int W, bool ADJ, int B, int NV, bool REAL>
 __device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane, PairStage<W>* stage) {
     const uint64_t v0 = p.vals[s];
@@ -1718,18 +1740,16 @@ int cores_build(const qmb_targets* targets, int max_occupied, int order, void* b
     if (c.capacity > 0) {
         const size_t cap = static_cast<size_t>(c.capacity);
         Carver sc{static_cast<char*>(scratch), 0};
-        uint64_t* keys_in = sc.take<uint64_t>(cap);
+        uint32_t* keys_in = sc.take<uint32_t>(cap);
         uint64_t* vals_in = sc.take<uint64_t>(cap);
         uint32_t* flag = sc.take<uint32_t>(cap);
         uint32_t* gid = sc.take<uint32_t>(cap);
         size_t tmp_bytes = scratch_bytes > sc.off + 512 ? scratch_bytes - sc.off - 512 : 0;
         void* tmp = sc.take_bytes(tmp_bytes);
-        const int sub_bits = core_sub_bits(8 * c.n_qubytes, order);
-        const int key_bits = 2 + kHashBits + sub_bits;
         const unsigned nb = blocks_for(c.n);
         auto emit = [&](auto kernel) {
-            kernel<<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, 8 * c.n_qubytes, c.per_config, sub_bits, keys_in,
-                                              vals_in, c.counts);
+            kernel<<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, 8 * c.n_qubytes, c.per_config, keys_in, vals_in,
+                                              c.counts);
         };
         switch (c.n_words * 2 + (order - 1)) {
             case 2: emit(&core_emit_kernel<1, 1>); break;
@@ -1743,15 +1763,14 @@ int cores_build(const qmb_targets* targets, int max_occupied, int order, void* b
         }
         QMB_CHECK_LAUNCH();
         size_t need = tmp_bytes;
-        QMB_HIP_TRY(rocprim::radix_sort_pairs(tmp, need, keys_in, c.keys, vals_in, c.vals, static_cast<unsigned>(cap), 0, key_bits,
+        QMB_HIP_TRY(rocprim::radix_sort_pairs(tmp, need, keys_in, c.keys, vals_in, c.vals, static_cast<unsigned>(cap), 0, kKeyBits,
                                               stream));
-        core_flag_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, c.capacity, sub_bits, flag);
+        core_flag_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, c.capacity, flag);
         QMB_CHECK_LAUNCH();
         need = tmp_bytes;
         QMB_HIP_TRY(rocprim::exclusive_scan(tmp, need, flag, gid, 0u, cap, rocprim::plus<uint32_t>(), stream));
-        core_start_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, flag, gid, c.capacity, key_sentinel(key_bits), c.starts,
-                                                                           c.counts);
-        c.key_bits = key_bits;
+        core_start_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, flag, gid, c.capacity, c.starts, c.counts);
+        c.key_bits = kKeyBits;
         QMB_CHECK_LAUNCH();
     }
     *out = c;
