@@ -122,31 +122,58 @@ def read_traffic(workload: str, kernel: str) -> tuple[float | None, float | None
 def structured_model(pp: object, terms: object, targets: object, nq: int, n: int) -> dict[str, object]:
     """Bytes the structured application must move at least once (synchronises to size the core
     indexes).  Per kernel launch:
-      pair_kernel:   4 per group (its start) + per entry of a group of >= 2 entries: the entry (8 key
-                     + 8 value), its row's config words (8 W) and amplitude (16); the coefficient table
-                     once (8 or 16 per entry of n_pairs^2); the output once (16 per row);
+      pair_kernel:   4 per group (its start) + per entry of a group of >= 2 entries: the entry (8,
+                     row and removed sites), its row's config words (8 W) and amplitude (16); the
+                     coefficient table once (8 or 16 per entry of n_pairs^2); the output once (16 per
+                     row);
       single_kernel: the same over the order-1 index, with the single-excitation tables (n_bits^2 +
                      n_bits^3 entries) once;
       diag_apply:    per row its config words, amplitude and output (8 W + 32); the folded table once.
-    Per core index build: every entry emitted (16 written), sorted (16 read + 16 written), its group
-    start found (4) -- the sort's extra passes and the hash index are not counted."""
+    Per core index build: every entry emitted (12 written: 4 key + 8 value), sorted (12 read + 12
+    written), its group start found (4) -- the sort's extra passes and the hash index are not
+    counted."""
     words = (nq + 7) // 8
-    per_entry = 16 + 8 * words + 16
+    per_entry = 8 + 8 * words + 16
     out: dict[str, object] = {}
     coef_b = 8 if terms.pairs_desc.real else 16
     n_bits = 8 * nq
     st2 = pp.core_stats(targets, 2)
     out["pair_kernel"] = dict(st2, bytes=4 * st2["groups"] + per_entry * st2["multi_entries"] + coef_b * terms.pairs_desc.n_pairs ** 2
                               + 16 * n)
-    index_bytes = 36 * st2["entries"] + 4 * st2["groups"]
+    index_bytes = 40 * st2["entries"] + 4 * st2["groups"]
     if terms.singles_desc is not None:
         st1 = pp.core_stats(targets, 1)
         sb = 8 if terms.singles_desc.real else 16
         out["single_kernel"] = dict(st1, bytes=4 * st1["groups"] + per_entry * st1["multi_entries"]
                                     + sb * (n_bits ** 2 + n_bits ** 3) + 16 * n)
-        index_bytes += 36 * st1["entries"] + 4 * st1["groups"]
+        index_bytes += 40 * st1["entries"] + 4 * st1["groups"]
     out["diag_apply_kernel"] = {"bytes": n * (8 * words + 32) + 16 * (1 + n_bits + terms.pairs_desc.n_pairs)}
     out["core_index_build"] = {"bytes": index_bytes}
+    return out
+
+
+def hop_model(terms: object, targets: object, nq: int, n: int) -> dict[str, object]:
+    """Bytes the hop path must move at least once (reads the strings indexes' sizes).  Per launch:
+      hop_kernel (one parity): per entry of a group of >= 2 rows its group range (8), row (4) and
+                     config words (8 W), its row's amplitude (16) and output, read and written (32);
+                     the single-excitation table (n_bits^2 entries) once;
+      diag_terms_kernel: per row its config words, amplitude and output (8 W + 32); the diagonal
+                     records once.
+    Per strings index build (one parity): per row its config words read (8 W), key and row emitted
+    (8), sorted (8 read + 8 written), group and start found (8); per entry of a group of >= 2 rows
+    its compacted row, range and config words written (12 + 8 W)."""
+    words = (nq + 7) // 8
+    n_bits = 8 * nq
+    sb = 8 if terms.singles_desc.real else 16
+    out: dict[str, object] = {}
+    build = 0
+    for parity in (0, 1):
+        st = targets.strings_stats(parity)
+        out[f"hop_kernel_{parity}"] = dict(st, bytes=st["multi_entries"] * (8 + 4 + 8 * words + 16 + 32) + sb * n_bits ** 2)
+        build += n * (8 * words + 8 + 16 + 8) + st["multi_entries"] * (12 + 8 * words)
+    valid, diagonal, _ = terms.counts()
+    out["diag_terms_kernel"] = {"bytes": n * (8 * words + 32) + 24 * diagonal}
+    out["strings_build"] = {"bytes": build}
     return out
 
 
@@ -951,6 +978,26 @@ def main() -> int:
     adjoint_ms = timed(lambda: pp.apply(terms, targets, configs, gvec, pp.MODE_ADJOINT, out=out, tile=args.tile, groups=args.groups,
                                         zero=False))
     parts = None
+    use_hops = not use_pairs and pp.can_use_hops(terms, targets, configs)
+    if use_hops:
+        # the hop path's kernels, each timed alone on the same stream
+        s_ptr = ctypes.c_void_p(stream.cuda_stream)
+        vin, vout = ctypes.c_void_p(vec.data_ptr()), ctypes.c_void_p(out.data_ptr())
+        parts = {"hop_terms": terms.n_hop_terms, "residual_terms": terms.n_residual_terms,
+                 "strings": [targets.strings_stats(parity) for parity in (0, 1)]}
+        parts["hop_kernel_ms"] = [timed(lambda parity=parity: lib.qmb_apply_hops(
+            ctypes.byref(terms.singles_desc), ctypes.byref(targets.strings(parity)), ctypes.byref(targets.desc), vin, vout,
+            pp.MODE_FORWARD, s_ptr)) for parity in (0, 1)]
+        parts["diag_terms_kernel_ms"] = timed(lambda: lib.qmb_apply_diagonal_terms(ctypes.byref(terms.desc), ctypes.byref(targets.desc),
+                                                                                   vin, vout, pp.MODE_FORWARD, s_ptr))
+
+        def rebuild_strings() -> None:
+            targets._strings.clear()  # pylint: disable=protected-access
+            targets.strings(0)
+            targets.strings(1)
+
+        parts["strings_build_ms"] = timed(rebuild_strings)
+        log(f"hop path parts: {parts}")
     if use_pairs:
         # the structured path's kernels, each timed alone on the same stream
         s_ptr = ctypes.c_void_p(stream.cuda_stream)
@@ -1022,6 +1069,15 @@ def main() -> int:
         step_bytes = model["core_index_build"]["bytes"] + 2 * app_bytes
         model_desc = ("structured path, bytes moved at least once per launch: group starts, the entries of groups of >= 2 "
                       "entries with their rows' config words and amplitudes, the coefficient table, the output")
+    elif use_hops:
+        model = hop_model(terms, targets, nq, n)
+        kname = "hop_kernel"
+        k_bytes, k_ms = model["hop_kernel_0"]["bytes"], parts["hop_kernel_ms"][0]
+        label = f"hop_kernel<{(nq + 7) // 8}, false>"
+        app_bytes = sum(v["bytes"] for k, v in model.items() if k != "strings_build")
+        step_bytes = model["strings_build"]["bytes"] + 2 * app_bytes
+        model_desc = ("hop path, bytes moved at least once per launch: the entries of groups of >= 2 rows with their rows' "
+                      "config words, amplitudes and outputs, the single-excitation table")
     else:
         model = None
         kname = "apply_kernel"
@@ -1104,7 +1160,7 @@ def main() -> int:
         "data": "synthetic (seeded)",
         "config": dict(w["desc"], n_terms=n_terms, n_qubytes=nq, M_gen=m_gen, M_hit=m_hit, valid_terms=valid_terms,
                        diagonal_terms=diag_terms, tile=args.tile or "auto", groups=args.groups or 8,
-                       kernel_path=("pair-excitation core groups + " if use_pairs else "")
+                       kernel_path=("pair-excitation core groups + " if use_pairs else "hop strings groups + " if use_hops else "")
                        + ("symmetric " if symmetric else "scatter/gather ") + ("direct" if direct else "bucketed tiles")),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": traffic, "traffic_source": traffic_src, "l2_hit_rate": l2_hit,

@@ -271,8 +271,8 @@ typedef struct qmb_singles {
     size_t bytes;
     double* a;         /* [n_bits * n_bits][2] */
     double* kd;        /* [n_bits * n_bits * n_bits][2] */
-    int64_t* counts;   /* device: {single-excitation terms, 0, entries with a nonzero imaginary part
-                          (after qmb_singles_set_real), 0} */
+    int64_t* counts;   /* device: {single-excitation terms, those with a conditioned site (c+_p n_k c_q),
+                          entries with a nonzero imaginary part (after qmb_singles_set_real), 0} */
     double* a_re;      /* [n_bits * n_bits] real parts, filled by qmb_singles_set_real */
     double* kd_re;     /* [n_bits * n_bits * n_bits] */
 } qmb_singles;
@@ -291,7 +291,7 @@ int qmb_singles_set_real(qmb_singles* singles, int32_t real, void* stream);
 
 /* qmb_terms_build without some terms (same layout and buffer size).  exclude is a mask:
  * 1 pair excitations, 2 single excitations, 4 diagonal terms (from the buckets; the diagonal
- * records are kept). */
+ * records are kept), 8 single excitations between two sites of one parity (the hop path). */
 int qmb_terms_build_residual(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
                              int64_t n_terms, int32_t exclude, void* buffer, size_t bytes, qmb_terms* out, void* stream);
 
@@ -335,6 +335,52 @@ int qmb_apply_singles(const qmb_singles* singles, const qmb_cores* cores, const 
 /* The diagonal part on the set indexed by `targets`: out[i] += d_i in[i] (mode 0) or
  * conj(d_i) in[i] (mode 1), d_i from the folded diagonal table of `pairs`. */
 int qmb_apply_diagonal(const qmb_pairs* pairs, const qmb_targets* targets, const double* in, double* out, int mode, void* stream);
+
+/* Hop path, for tables whose off-diagonal terms are single excitations between two sites of one
+ * parity (spin-conserving hops when sites interleave spin, even = up, odd = down: the Hubbard model
+ * of qmb/hubbard.py).  No reference counterpart: the reference enumerates every
+ * term on every config and binary-searches the result (qmb/_hamiltonian_cuda.cu:86-169); here a
+ * hop connects two rows of the set only if they hold the same string on the other parity's sites,
+ * so the rows are grouped by that string and the rows of a group are tested pairwise.
+ *
+ * The strings index of parity s (0: hops between even sites, 1: between odd sites): one entry per
+ * row, radix-sorted by a 32-bit key of the row's string on the sites of parity 1 - s (its core
+ * hash mod 2^32 - 1; rows of different strings with equal keys share a group and are told apart
+ * per pair). */
+typedef struct qmb_strings {
+    int32_t n_qubytes, n_words, parity, pad;
+    int64_t n;
+    void* buffer;
+    size_t bytes;
+    uint32_t* keys;    /* [n] sorted keys */
+    uint32_t* rows;    /* [n] row of each entry */
+    uint32_t* group;   /* [n] group of each entry */
+    uint32_t* starts;  /* [n + 1] first entry of each group, then the end */
+    /* The entries of the groups of two or more rows (the only ones qmb_apply_hops walks), in index
+     * order: */
+    uint32_t* mrows;   /* [n] their rows */
+    uint32_t* mrange;  /* [n][2] their group as [first, end) of this order */
+    uint64_t* mcfg;    /* [n][n_words] their config words */
+    int64_t* counts;   /* device: {entries, groups, sum of size^2 over the groups of two or more rows (the
+                          pairs qmb_apply_hops tests), largest group, entries of those groups, 0, 0, 0} */
+} qmb_strings;
+
+int qmb_strings_workspace_size(int n_qubytes, int64_t n, size_t* bytes);
+int qmb_strings_build(const qmb_targets* targets, int32_t parity, void* buffer, size_t bytes, qmb_strings* out, void* stream);
+
+/* Adds the hops of parity strings->parity of mode 0 (out = A in) or mode 1 (out = A^H in) on the
+ * set: for each pair of rows of a group that differ in two sites of that parity, one occupied in
+ * each, the single-excitation element of `singles` between them (with its conditioned-site terms)
+ * and the fermionic sign (particle_cut 1).  Each row is written by one lane, without atomics. */
+int qmb_apply_hops(const qmb_singles* singles, const qmb_strings* strings, const qmb_targets* targets, const double* in,
+                   double* out, int mode, void* stream);
+
+/* The diagonal terms of `terms` (its diagonal records) on the set indexed by `targets`: out[i] +=
+ * d_i in[i] (mode 0) or conj(d_i) in[i] (mode 1), d_i summed over the diagonal terms as
+ * qmb_diagonal_term does (qmb/_hamiltonian_cuda.cu:677-708).  For tables with few diagonal terms,
+ * where qmb_apply_diagonal's folded table costs a read per occupied pair. */
+int qmb_apply_diagonal_terms(const qmb_terms* terms, const qmb_targets* targets, const double* in, double* out, int mode,
+                             void* stream);
 
 /* The diagonal, single- and pair-excitation parts of A (mode 0) or A^H (mode 1) on two vectors at
  * once, for one set: out += A in and out2 += A in2, each kernel walking its groups once for both

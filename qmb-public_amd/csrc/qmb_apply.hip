@@ -340,6 +340,114 @@ __global__ __launch_bounds__(kBlock) void diag_kernel(const uint8_t* __restrict_
     }
 }
 
+// The diagonal terms applied on an indexed set: out[i] += d_i in[i] (mode 0) or conj(d_i) in[i]
+// (mode 1), d_i the sum over the diagonal terms, in their original order, of the coefficients of
+// those that succeed on config i (with their parity), as diag_kernel sums them.  Each tile of
+// records is expanded to word masks in LDS once per block (succeeds iff (c & m) == v, parity
+// popcount(c & p) folded with the record's constant into the coefficient's sign), so a row pays a
+// few word operations per term.  For tables with few diagonal terms (a Hubbard U per site), where
+// the folded pair table's sum over occupied pairs would read O(occupied^2) entries per row.
+constexpr int kDiagMaskTile = 256;
+
+template <int W>
+struct DiagMask {
+    uint64_t m[W], v[W], p[W];
+    double2 coef;  // sign of the record's constant parity folded in
+};
+
+template <int W>
+__device__ __forceinline__ void stage_diag_masks(const uint64_t* __restrict__ drecs, const double2* __restrict__ dcoefs, int64_t base,
+                                                 int cnt, DiagMask<W>* s_mask) {
+    for (int u = threadIdx.x; u < cnt; u += kBlock) {
+        const Masks<W> x = expand_rec<W>(drecs[base + u], nullptr);
+        DiagMask<W> d;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            d.m[w] = x.m[w];
+            d.v[w] = x.v[w];
+            d.p[w] = x.p[w];
+        }
+        const double2 cf = dcoefs[base + u];
+        d.coef = x.pconst ? make_double2(-cf.x, -cf.y) : cf;
+        s_mask[u] = d;
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void sum_diag_masks(const Cfg<W>& c, const DiagMask<W>* s_mask, int cnt, double& re, double& im) {
+    for (int u = 0; u < cnt; ++u) {
+        const DiagMask<W>& d = s_mask[u];
+        bool ok = true;
+        int par = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            ok = ok && (c.w[w] & d.m[w]) == d.v[w];
+            par ^= __popcll(c.w[w] & d.p[w]);
+        }
+        if (ok) {
+            re += (par & 1) ? -d.coef.x : d.coef.x;
+            im += (par & 1) ? -d.coef.y : d.coef.y;
+        }
+    }
+}
+
+template <bool ADJ>
+__device__ __forceinline__ void add_diag(int64_t i, double re, double im, const double2* __restrict__ in, double2* __restrict__ out) {
+    if (re != 0.0 || im != 0.0) {
+        const double dim = ADJ ? -im : im;
+        const double2 g = in[i];
+        out[i].x += re * g.x - dim * g.y;
+        out[i].y += re * g.y + dim * g.x;
+    }
+}
+
+// Grid-stride over rows.  Up to kDiagMaskTile diagonal terms are staged once per block; more are
+// walked a tile at a time per block of rows.
+template <int W, bool ADJ>
+__global__ __launch_bounds__(kBlock) void diag_terms_kernel(const uint64_t* __restrict__ keys, int64_t n,
+                                                           const uint64_t* __restrict__ drecs, const double2* __restrict__ dcoefs,
+                                                           const int64_t* __restrict__ counts, const double2* __restrict__ in,
+                                                           double2* __restrict__ out) {
+    __shared__ DiagMask<W> s_mask[kDiagMaskTile];
+    const int64_t td = counts[1];
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    if (td <= kDiagMaskTile) {
+        stage_diag_masks<W>(drecs, dcoefs, 0, static_cast<int>(td), s_mask);
+        __syncthreads();
+        for (int64_t i = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x; i < n; i += stride) {
+            Cfg<W> c;
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                c.w[k] = keys[i * W + k];
+            }
+            double re = 0.0, im = 0.0;
+            sum_diag_masks<W>(c, s_mask, static_cast<int>(td), re, im);
+            add_diag<ADJ>(i, re, im, in, out);
+        }
+        return;
+    }
+    for (int64_t i0 = blockIdx.x * static_cast<int64_t>(kBlock); i0 < n; i0 += stride) {
+        const int64_t i = i0 + threadIdx.x;
+        const bool valid = i < n;
+        Cfg<W> c;
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            c.w[k] = valid ? keys[i * W + k] : 0ull;
+        }
+        double re = 0.0, im = 0.0;
+        for (int64_t base = 0; base < td; base += kDiagMaskTile) {
+            const int cnt = static_cast<int>(td - base < kDiagMaskTile ? td - base : kDiagMaskTile);
+            __syncthreads();
+            stage_diag_masks<W>(drecs, dcoefs, base, cnt, s_mask);
+            __syncthreads();
+            sum_diag_masks<W>(c, s_mask, cnt, re, im);
+        }
+        if (valid) {
+            add_diag<ADJ>(i, re, im, in, out);
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // single_relative: per-term Efraimidis-Spirakis keys, terms sorted by key (descending, ties by
 // index), then each config takes the first term that succeeds with a result outside the input set.
@@ -555,6 +663,36 @@ int diagonal_prepared(const qmb_terms* terms, const uint8_t* configs, int64_t n,
         diag_kernel<W><<<blocks_for(n), kBlock, 0, stream>>>(configs, n, terms->n_qubytes, terms->drecs,
                                                              reinterpret_cast<const double2*>(terms->dcoefs), terms->counts,
                                                              reinterpret_cast<double2*>(out));
+        QMB_CHECK_LAUNCH();
+        return QMB_OK;
+    });
+}
+
+int apply_diagonal_terms(const qmb_terms* terms, const qmb_targets* targets, const double* in, double* out, int mode,
+                         hipStream_t stream) {
+    if (!terms || !targets || !in || !out) {
+        return set_error(QMB_ERR_ARGUMENT, "terms, targets, in and out must be non-null");
+    }
+    if (terms->n_qubytes != targets->n_qubytes) {
+        return set_error(QMB_ERR_ARGUMENT, "terms built for %d bytes, targets for %d", terms->n_qubytes, targets->n_qubytes);
+    }
+    if (mode != 0 && mode != 1) {
+        return set_error(QMB_ERR_ARGUMENT, "mode must be 0 (forward) or 1 (adjoint)");
+    }
+    if (targets->n == 0) {
+        return QMB_OK;
+    }
+    return dispatch_words(terms->n_words, [&](auto wc) -> int {
+        constexpr int W = decltype(wc)::value;
+        const auto* dc = reinterpret_cast<const double2*>(terms->dcoefs);
+        const auto* vi = reinterpret_cast<const double2*>(in);
+        auto* vo = reinterpret_cast<double2*>(out);
+        const unsigned grid = blocks_for(targets->n) < 2048u ? blocks_for(targets->n) : 2048u;
+        if (mode == 1) {
+            diag_terms_kernel<W, true><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo);
+        } else {
+            diag_terms_kernel<W, false><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo);
+        }
         QMB_CHECK_LAUNCH();
         return QMB_OK;
     });

@@ -50,7 +50,9 @@ __global__ void compile_terms_kernel(const int16_t* __restrict__ site, const uin
     SingleExcitation se;
     if (c.status == kCompileOk && (((exclude_pairs & kExcludePairs) && pair_excitation(c.rec, particle_cut, &pe)) ||
                                    ((exclude_pairs & kExcludeSingles) && single_excitation(c.rec, particle_cut, &se)) ||
-                                   ((exclude_pairs & kExcludeDiagonal) && c.diagonal))) {
+                                   ((exclude_pairs & kExcludeDiagonal) && c.diagonal) ||
+                                   ((exclude_pairs & kExcludeHops) && single_excitation(c.rec, particle_cut, &se) &&
+                                    ((se.r ^ se.p) & 1) == 0))) {
         // carried by the pair / single excitation tables or the diagonal kernel instead
         keys[t] = static_cast<uint32_t>(n_buckets);
         rec_tmp[t] = kNeverSucceeds;

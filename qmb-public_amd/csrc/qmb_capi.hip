@@ -374,8 +374,8 @@ int qmb_terms_build_residual(int n_qubytes, int particle_cut, const int16_t* sit
     if (!out || !buffer) {
         return set_error(QMB_ERR_ARGUMENT, "buffer and out must be non-null");
     }
-    if (exclude < 0 || exclude > (kExcludePairs | kExcludeSingles | kExcludeDiagonal)) {
-        return set_error(QMB_ERR_ARGUMENT, "exclude must be a mask of 1, 2 and 4, got %d", exclude);
+    if (exclude < 0 || exclude > (kExcludePairs | kExcludeSingles | kExcludeDiagonal | kExcludeHops)) {
+        return set_error(QMB_ERR_ARGUMENT, "exclude must be a mask of 1, 2, 4 and 8, got %d", exclude);
     }
     return terms_build(n_qubytes, particle_cut, site, kind, coef, n_terms, buffer, bytes, out, as_stream(stream), exclude);
 }
@@ -422,6 +422,35 @@ int qmb_apply_diagonal(const qmb_pairs* pairs, const qmb_targets* targets, const
         return set_error(QMB_ERR_ARGUMENT, "in and out must be non-null");
     }
     return apply_diagonal(pairs, targets, in, out, mode, as_stream(stream));
+}
+
+int qmb_strings_workspace_size(int n_qubytes, int64_t n, size_t* bytes) {
+    clear_error();
+    if (!bytes) {
+        return set_error(QMB_ERR_ARGUMENT, "bytes must be non-null");
+    }
+    *bytes = strings_bytes(n_qubytes, n);
+    return *bytes ? QMB_OK : QMB_ERR_ARGUMENT;
+}
+
+int qmb_strings_build(const qmb_targets* targets, int32_t parity, void* buffer, size_t bytes, qmb_strings* out, void* stream) {
+    clear_error();
+    if (!buffer) {
+        return set_error(QMB_ERR_ARGUMENT, "buffer must be non-null");
+    }
+    return strings_build(targets, parity, buffer, bytes, out, as_stream(stream));
+}
+
+int qmb_apply_hops(const qmb_singles* singles, const qmb_strings* strings, const qmb_targets* targets, const double* in, double* out,
+                   int mode, void* stream) {
+    clear_error();
+    return apply_hops(singles, strings, targets, in, out, mode, as_stream(stream));
+}
+
+int qmb_apply_diagonal_terms(const qmb_terms* terms, const qmb_targets* targets, const double* in, double* out, int mode,
+                             void* stream) {
+    clear_error();
+    return apply_diagonal_terms(terms, targets, in, out, mode, as_stream(stream));
 }
 
 int qmb_apply_structured_dual(const qmb_pairs* pairs, const qmb_singles* singles, const qmb_cores* cores2, const qmb_cores* cores1,

@@ -101,6 +101,7 @@ int terms_layout(Carver& cv, int n_qubytes, int64_t n_terms, qmb_terms* out, voi
 constexpr int kExcludePairs = 1;     // pair excitations (qmb_pairs)
 constexpr int kExcludeSingles = 2;   // single excitations (qmb_singles)
 constexpr int kExcludeDiagonal = 4;  // diagonal terms (applied from the diagonal records)
+constexpr int kExcludeHops = 8;      // single excitations between two sites of one parity (qmb_apply_hops)
 int terms_build(int n_qubytes, int particle_cut, const int16_t* site, const uint8_t* kind, const double* coef,
                 int64_t n_terms, void* buffer, size_t bytes, qmb_terms* out, hipStream_t stream, int exclude_pairs = 0);
 int targets_layout(Carver& cv, int n_qubytes, int64_t n, qmb_targets* out);
@@ -132,6 +133,8 @@ int apply_plain(int n_qubytes, int particle_cut, const uint8_t* configs_i, const
                 const double* coef, int64_t n_terms, double* psi_j, hipStream_t stream);
 
 int diagonal_prepared(const qmb_terms* terms, const uint8_t* configs, int64_t n, double* out, hipStream_t stream);
+int apply_diagonal_terms(const qmb_terms* terms, const qmb_targets* targets, const double* in, double* out, int mode,
+                         hipStream_t stream);
 
 int single_relative_prepared(int n_qubytes, int particle_cut, const uint8_t* configs, int64_t n, const int16_t* site,
                              const uint8_t* kind, const double* coef, int64_t n_terms, uint64_t seed, const double* keys,
@@ -159,6 +162,10 @@ int apply_singles(const qmb_singles* singles, const qmb_cores* cores, const qmb_
                   int mode, hipStream_t stream);
 int apply_diagonal(const qmb_pairs* pairs, const qmb_targets* targets, const double* in, double* out, int mode,
                    hipStream_t stream);
+size_t strings_bytes(int n_qubytes, int64_t n);
+int strings_build(const qmb_targets* targets, int parity, void* buffer, size_t bytes, qmb_strings* out, hipStream_t stream);
+int apply_hops(const qmb_singles* singles, const qmb_strings* strings, const qmb_targets* targets, const double* in, double* out,
+               int mode, hipStream_t stream);
 // Diagonal, single- and pair-excitation parts of A (mode 0) or A^H (mode 1) applied to two vectors
 // in one pass each (the forward and backward of a Hermitian training step); accumulates into out
 // and out2.

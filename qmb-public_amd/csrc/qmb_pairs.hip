@@ -991,6 +991,7 @@ __global__ void singles_fill_kernel(const int16_t* __restrict__ site, const uint
         double2* dk = kd + rp * n_bits + e.k;
         unsafeAtomicAdd(&dk->x, e.kocc ? v.x : -v.x);
         unsafeAtomicAdd(&dk->y, e.kocc ? v.y : -v.y);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&counts[1]), 1ull);
     }
     atomicAdd(reinterpret_cast<unsigned long long*>(&counts[0]), 1ull);
 }
@@ -1422,6 +1423,303 @@ __global__ __launch_bounds__(kBlock) void single_kernel(SingleParams p) {
     const int64_t groups = p.counts[1];
     for (int64_t g0 = wave * 64; g0 < groups; g0 += nwaves * 64) {
         single_chunk<W, ADJ, NV, 8>(p, groups, g0, lane);
+    }
+}
+
+// ----------------------------------------------------------------------------------- hops --
+// A single excitation between two sites of one parity (a spin-conserving hop when sites interleave
+// spin: even = up, odd = down) leaves the string on the other parity's sites unchanged, so the two
+// configs it connects hold equal strings there.  The strings index of parity s groups a set's rows
+// by their string on the sites of parity 1 - s: one entry per row, sorted by a 32-bit key of that
+// string (its hash mod 2^32 - 1).  Inside a group two rows are connected by such a hop iff their
+// configs differ in exactly two sites, both of parity s, one occupied in each; rows of different
+// strings merged by equal keys fail that test.  Building it sorts n keys, where the order-1 core
+// index sorts n per occupied site.
+constexpr uint64_t kEvenSites = 0x5555555555555555ull;
+
+__host__ __device__ __forceinline__ uint64_t parity_sites(int parity) { return parity ? ~kEvenSites : kEvenSites; }
+
+template <int W>
+__global__ void strings_emit_kernel(const uint64_t* __restrict__ keys, const uint64_t* __restrict__ cols, int64_t n, int parity,
+                                    uint32_t* __restrict__ okeys, uint32_t* __restrict__ orows) {
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    Cfg<W> c = load_key<W>(keys, i);
+    const uint64_t other = parity_sites(parity ^ 1);
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        c.w[w] &= other;
+    }
+    // below 2^32 - 1: the key that no entry holds marks nothing here, every row has its entry
+    okeys[i] = static_cast<uint32_t>(hash_cfg<W>(c, cols) % 0xFFFFFFFFull);
+    orows[i] = static_cast<uint32_t>(i);
+}
+
+// group[e] holds the inclusive count of group starts up to entry e; turned into the entry's group.
+__global__ void strings_start_kernel(const uint32_t* __restrict__ flag, uint32_t* __restrict__ group, int64_t n,
+                                     uint32_t* __restrict__ starts, int64_t* __restrict__ counts) {
+    const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (e >= n) {
+        return;
+    }
+    const uint32_t g = group[e] - 1u;
+    group[e] = g;
+    if (flag[e]) {
+        starts[g] = static_cast<uint32_t>(e);
+    }
+    if (e == n - 1) {
+        starts[g + 1] = static_cast<uint32_t>(n);
+        counts[0] = n;
+        counts[1] = static_cast<int64_t>(g) + 1;
+    }
+}
+
+// counts[2] = sum of size^2 over the groups of two or more rows (the pairs the hop kernel tests),
+// counts[3] = the largest group: a grid-stride pass, one atomic per block.
+__global__ __launch_bounds__(kBlock) void strings_stats_kernel(const uint32_t* __restrict__ starts, int64_t* __restrict__ counts) {
+    __shared__ unsigned long long s_sq[kBlock / 64], s_mx[kBlock / 64];
+    const int64_t groups = counts[1];
+    unsigned long long sq = 0ull, mx = 0ull;
+    for (int64_t g = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x; g < groups; g += static_cast<int64_t>(gridDim.x) * kBlock) {
+        const unsigned long long sz = starts[g + 1] - starts[g];
+        sq += sz >= 2ull ? sz * sz : 0ull;
+        mx = sz > mx ? sz : mx;
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        sq += __shfl_down(sq, d);
+        const unsigned long long o = __shfl_down(mx, d);
+        mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_sq[threadIdx.x >> 6] = sq;
+        s_mx[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; ++w) {
+            sq += s_sq[w];
+            mx = s_mx[w] > mx ? s_mx[w] : mx;
+        }
+        if (mx != 0ull) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(&counts[2]), sq);
+            atomicMax(reinterpret_cast<unsigned long long*>(&counts[3]), mx);
+        }
+    }
+}
+
+// mflag[e] = 1 when entry e's group holds two or more rows.
+__global__ void strings_multi_kernel(const uint32_t* __restrict__ group, const uint32_t* __restrict__ starts, int64_t n,
+                                     uint32_t* __restrict__ mflag) {
+    const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (e >= n) {
+        return;
+    }
+    const uint32_t g = group[e];
+    mflag[e] = starts[g + 1] - starts[g] >= 2u ? 1u : 0u;
+}
+
+// The entries of groups of two or more rows, in index order, with their config words gathered and
+// each entry's group as a range of the compacted order (mpos: exclusive scan of mflag);
+// counts[4] = their number.
+template <int W>
+__global__ void strings_compact_kernel(const uint64_t* __restrict__ ckeys, const uint32_t* __restrict__ rows,
+                                       const uint32_t* __restrict__ group, const uint32_t* __restrict__ starts,
+                                       const uint32_t* __restrict__ mflag, const uint32_t* __restrict__ mpos, int64_t n,
+                                       uint32_t* __restrict__ mrows, uint2* __restrict__ mrange, uint64_t* __restrict__ mcfg,
+                                       int64_t* __restrict__ counts) {
+    const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (e >= n) {
+        return;
+    }
+    if (e == n - 1) {
+        counts[4] = static_cast<int64_t>(mpos[e]) + mflag[e];
+    }
+    if (!mflag[e]) {
+        return;
+    }
+    const uint32_t p = mpos[e];
+    const uint32_t g = group[e];
+    const uint32_t lo = mpos[starts[g]];
+    const uint32_t row = rows[e];
+    mrows[p] = row;
+    mrange[p] = make_uint2(lo, lo + (starts[g + 1] - starts[g]));
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        mcfg[static_cast<int64_t>(p) * W + w] = ckeys[static_cast<int64_t>(row) * W + w];
+    }
+}
+
+template <int W>
+struct HopStage {
+    uint64_t c[W];  // config of the entry's row
+    double2 g;      // in[row]
+};
+
+struct HopParams {
+    SingleParams s;          // the single-excitation tables, in, out and the set's config words
+    const int64_t* scounts;  // the tables' counts: [1] = terms with a conditioned site
+    uint64_t sites;          // the hop parity's sites, per word
+    const uint32_t* mrows;   // the compacted entries (groups of two or more rows): row,
+    const uint2* mrange;     // group as a range of the compacted order,
+    const uint64_t* mcfg;    // config words
+    const int64_t* counts;   // [4] = number of compacted entries
+};
+
+// One wave per 64 consecutive compacted entries (rows of groups of two or more): lane = row a.  The
+// wave walks the entries of the union of its rows' groups (consecutive in the compacted order),
+// staged 64 at a time in LDS (config words read in order, amplitudes gathered) and read back as
+// broadcasts, and tests each pair.  A connected pair (a few per row) queues its table index, sign
+// and stage slot; the queue is drained once per staged chunk (or when a lane's queue is full), so
+// the coefficient loads of a chunk are in flight together instead of one load latency per step.
+// Tables with conditioned-site terms take the element at once (it sums a row of K over the core).
+// Each row has one entry, so its sum is added to out[a] without atomics.
+//   ADJ:  out[a] += conj(M_ba) in[b],  M_ba = (-1)^single_parity(c_a; r, q) (A[r][q] + sum_{k in core} K[r][q][k])
+//   !ADJ: out[a] += M_ab in[b],        M_ab = (-1)^single_parity(c_b; q, r) (A[q][r] + sum_{k in core} K[q][r][k])
+// with r the site of a, q the site of b (c_a = core + r, c_b = core + q), as in single_group.
+constexpr int kHopQueue = 8;
+
+template <bool ADJ>
+__device__ __forceinline__ void hop_madd(double xr, double xi, const double2& g, double& re, double& im) {
+    if (ADJ) {
+        re += xr * g.x + xi * g.y;
+        im += xr * g.y - xi * g.x;
+    } else {
+        re += xr * g.x - xi * g.y;
+        im += xr * g.y + xi * g.x;
+    }
+}
+
+template <int W, bool ADJ>
+__global__ __launch_bounds__(kBlock) void hop_kernel(HopParams p) {
+    __shared__ HopStage<W> stage_all[kBlock];
+    const int lane = threadIdx.x & 63;
+    HopStage<W>* stage = stage_all + (threadIdx.x & ~63);
+    const int64_t wave = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6;
+    const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * kBlock) >> 6;
+    const int64_t entries = p.counts[4];
+    const bool fermi = p.s.particle_cut == 1;
+    const bool plain = p.scounts[1] == 0;
+    for (int64_t base = wave * 64; base < entries; base += nwaves * 64) {
+        const int64_t e = base + lane;
+        const bool act = e < entries;
+        const uint2 range = act ? p.mrange[e] : make_uint2(0xFFFFFFFFu, 0u);
+        // the entries any row of the wave can meet: the union of its rows' groups
+        uint32_t lo = range.x, hi = range.y;
+        for (int d = 32; d > 0; d >>= 1) {
+            const uint32_t ol = __shfl_xor(lo, d);
+            const uint32_t oh = __shfl_xor(hi, d);
+            lo = ol < lo ? ol : lo;
+            hi = oh > hi ? oh : hi;
+        }
+        lo = __builtin_amdgcn_readfirstlane(lo);
+        hi = __builtin_amdgcn_readfirstlane(hi);
+        const int64_t ia = act ? static_cast<int64_t>(p.mrows[e]) : 0;
+        Cfg<W> ca;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            ca.w[w] = act ? p.mcfg[e * W + w] : 0ull;
+        }
+        double acc_re = 0.0, acc_im = 0.0;
+        // queued connections: table index in bits 0..23, stage slot in 24..29, sign in 31
+        uint32_t queue[kHopQueue];
+        int nq = 0;
+        auto drain = [&]() {
+            double2 x[kHopQueue];
+#pragma unroll
+            for (int k = 0; k < kHopQueue; ++k) {
+                const int64_t rq = static_cast<int64_t>(queue[k] & 0xFFFFFFu);
+                x[k] = make_double2(0.0, 0.0);
+                if (k < nq) {
+                    x[k] = p.s.a_re ? make_double2(p.s.a_re[rq], 0.0) : p.s.a[rq];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kHopQueue; ++k) {
+                if (k < nq) {
+                    const uint32_t it = queue[k];
+                    const double2 g = stage[(it >> 24) & 63u].g;
+                    const bool neg = (it >> 31) != 0u;
+                    hop_madd<ADJ>(neg ? -x[k].x : x[k].x, neg ? -x[k].y : x[k].y, g, acc_re, acc_im);
+                }
+            }
+            nq = 0;
+        };
+        for (uint32_t c0 = lo; c0 < hi; c0 += 64u) {
+            const uint32_t b = c0 + static_cast<uint32_t>(lane);
+            if (b < hi) {
+                HopStage<W> st;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    st.c[w] = p.mcfg[static_cast<int64_t>(b) * W + w];
+                }
+                st.g = p.s.in[p.mrows[b]];
+                stage[lane] = st;
+            }
+            __builtin_amdgcn_wave_barrier();
+            const int nb = static_cast<int>(hi - c0 < 64u ? hi - c0 : 64u);
+            // the pair tests, a few word operations per staged entry: bit u of `hit` marks a
+            // connection of this lane's row to entry u
+            uint64_t hit = 0ull;
+            for (int u = 0; u < nb; ++u) {
+                const HopStage<W>& st = stage[u];
+                int pc = 0, pa = 0;
+                uint64_t off = 0ull;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    const uint64_t d = ca.w[w] ^ st.c[w];
+                    pc += __popcll(d);
+                    pa += __popcll(d & ca.w[w]);
+                    off |= d & ~p.sites;
+                }
+                // two sites differ, both of the hop parity, one occupied in each config
+                hit |= (pc == 2 && pa == 1 && off == 0ull) ? (1ull << u) : 0ull;
+            }
+            hit = act ? hit : 0ull;
+            // each lane's own connections (a few): the element's table index and sign
+            while (hit) {
+                const int u = __builtin_ctzll(hit);
+                hit &= hit - 1ull;
+                const HopStage<W>& st = stage[u];
+                int r = 0, q = 0;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    const uint64_t d = ca.w[w] ^ st.c[w];
+                    const uint64_t da = d & ca.w[w], db = d & st.c[w];
+                    r = da ? 64 * w + __builtin_ctzll(da) : r;
+                    q = db ? 64 * w + __builtin_ctzll(db) : q;
+                }
+                Cfg<W> core = ca;
+                flip_site<W>(core, r);
+                Cfg<W> cb = core;
+                flip_site<W>(cb, q);
+                const int sg = fermi ? (ADJ ? single_parity<W>(ca, r, q) : single_parity<W>(cb, q, r)) : 0;
+                if (plain) {
+                    const uint32_t rq = static_cast<uint32_t>(ADJ ? r * p.s.n_bits + q : q * p.s.n_bits + r);
+                    const uint32_t item = rq | (static_cast<uint32_t>(u) << 24) | (static_cast<uint32_t>(sg) << 31);
+#pragma unroll
+                    for (int k = 0; k < kHopQueue; ++k) {
+                        queue[k] = k == nq ? item : queue[k];
+                    }
+                    ++nq;
+                    if (__ballot(nq == kHopQueue) != 0ull) {
+                        drain();
+                    }
+                } else {
+                    const double2 x = ADJ ? single_element<W, 8>(p.s, core, r, q) : single_element<W, 8>(p.s, core, q, r);
+                    hop_madd<ADJ>(sg ? -x.x : x.x, sg ? -x.y : x.y, st.g, acc_re, acc_im);
+                }
+            }
+            if (__ballot(nq != 0) != 0ull) {
+                drain();
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (act && (acc_re != 0.0 || acc_im != 0.0)) {
+            p.s.out[ia].x += acc_re;
+            p.s.out[ia].y += acc_im;
+        }
     }
 }
 
@@ -1929,6 +2227,170 @@ int apply_diagonal(const qmb_pairs* pairs, const qmb_targets* targets, const dou
     }
     QMB_CHECK_LAUNCH();
     return QMB_OK;
+}
+
+int strings_layout(Carver& cv, int n_qubytes, int64_t n, qmb_strings* out, void** scratch, size_t* scratch_bytes) {
+    QMB_TRY(check_qubytes(n_qubytes));
+    if (n_qubytes > kMaxPairBytes) {
+        return set_error(QMB_ERR_UNSUPPORTED, "the strings index supports n_qubytes <= %d, got %d", kMaxPairBytes, n_qubytes);
+    }
+    if (n < 0 || n >= (1ll << 32) - 1) {
+        return set_error(QMB_ERR_ARGUMENT, "n must be in [0, 2^32 - 1), got %lld", static_cast<long long>(n));
+    }
+    qmb_strings t{};
+    t.n_qubytes = n_qubytes;
+    t.n_words = n_words_of(n_qubytes);
+    t.n = n;
+    const size_t cap = static_cast<size_t>(n > 0 ? n : 1);
+    t.buffer = cv.base;
+    t.counts = cv.take<int64_t>(8);
+    t.keys = cv.take<uint32_t>(cap);
+    t.rows = cv.take<uint32_t>(cap);
+    t.group = cv.take<uint32_t>(cap);
+    t.starts = cv.take<uint32_t>(cap + 1);
+    t.mrows = cv.take<uint32_t>(cap);
+    t.mrange = cv.take<uint32_t>(2 * cap);
+    t.mcfg = cv.take<uint64_t>(cap * static_cast<size_t>(t.n_words));
+    size_t tmp = 0;
+    {
+        uint32_t* u = nullptr;
+        if (rocprim::radix_sort_pairs(nullptr, tmp, u, u, u, u, static_cast<unsigned>(cap), 0, 32) != hipSuccess) {
+            return set_error(QMB_ERR_HIP, "rocprim radix_sort_pairs size query failed");
+        }
+        size_t scan = 0;
+        if (rocprim::inclusive_scan(nullptr, scan, u, u, cap, rocprim::plus<uint32_t>()) != hipSuccess) {
+            return set_error(QMB_ERR_HIP, "rocprim inclusive_scan size query failed");
+        }
+        tmp = tmp > scan ? tmp : scan;
+        if (rocprim::exclusive_scan(nullptr, scan, u, u, 0u, cap, rocprim::plus<uint32_t>()) != hipSuccess) {
+            return set_error(QMB_ERR_HIP, "rocprim exclusive_scan size query failed");
+        }
+        tmp = tmp > scan ? tmp : scan;
+    }
+    const size_t s0 = cv.off;
+    void* s = cv.take_bytes(4 * cap * sizeof(uint32_t) + tmp + 4096);
+    if (scratch) {
+        *scratch = s;
+    }
+    if (scratch_bytes) {
+        *scratch_bytes = cv.off - s0;
+    }
+    t.bytes = cv.off;
+    if (out) {
+        *out = t;
+    }
+    return QMB_OK;
+}
+
+size_t strings_bytes(int n_qubytes, int64_t n) {
+    Carver cv{nullptr, 0};
+    return strings_layout(cv, n_qubytes, n, nullptr, nullptr, nullptr) == QMB_OK ? cv.off : 0;
+}
+
+int strings_build(const qmb_targets* targets, int parity, void* buffer, size_t bytes, qmb_strings* out, hipStream_t stream) {
+    if (!targets || !out) {
+        return set_error(QMB_ERR_ARGUMENT, "targets and out must be non-null");
+    }
+    if (parity != 0 && parity != 1) {
+        return set_error(QMB_ERR_ARGUMENT, "parity must be 0 (hops between even sites) or 1 (odd sites), got %d", parity);
+    }
+    Carver cv{static_cast<char*>(buffer), 0};
+    qmb_strings t{};
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    QMB_TRY(strings_layout(cv, targets->n_qubytes, targets->n, &t, &scratch, &scratch_bytes));
+    if (cv.off > bytes) {
+        return set_error(QMB_ERR_WORKSPACE, "strings index buffer too small: need %zu bytes, have %zu", cv.off, bytes);
+    }
+    t.bytes = bytes;
+    t.parity = parity;
+    QMB_TRY(launch_zero_f64(reinterpret_cast<double*>(t.counts), 8, stream));
+    const int64_t n = t.n;
+    if (n > 0) {
+        const size_t cap = static_cast<size_t>(n);
+        Carver sc{static_cast<char*>(scratch), 0};
+        uint32_t* keys_in = sc.take<uint32_t>(cap);
+        uint32_t* rows_in = sc.take<uint32_t>(cap);
+        uint32_t* flag = sc.take<uint32_t>(cap);
+        uint32_t* mpos = sc.take<uint32_t>(cap);
+        size_t tmp_bytes = scratch_bytes > sc.off + 512 ? scratch_bytes - sc.off - 512 : 0;
+        void* tmp = sc.take_bytes(tmp_bytes);
+        const unsigned nb = blocks_for(n);
+        if (t.n_words == 1) {
+            strings_emit_kernel<1><<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, n, parity, keys_in, rows_in);
+        } else {
+            strings_emit_kernel<2><<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, n, parity, keys_in, rows_in);
+        }
+        QMB_CHECK_LAUNCH();
+        size_t need = tmp_bytes;
+        QMB_HIP_TRY(rocprim::radix_sort_pairs(tmp, need, keys_in, t.keys, rows_in, t.rows, static_cast<unsigned>(cap), 0, 32, stream));
+        core_flag_kernel<<<nb, kBlock, 0, stream>>>(t.keys, n, flag);
+        QMB_CHECK_LAUNCH();
+        need = tmp_bytes;
+        QMB_HIP_TRY(rocprim::inclusive_scan(tmp, need, flag, t.group, cap, rocprim::plus<uint32_t>(), stream));
+        strings_start_kernel<<<nb, kBlock, 0, stream>>>(flag, t.group, n, t.starts, t.counts);
+        QMB_CHECK_LAUNCH();
+        strings_stats_kernel<<<nb < 512u ? nb : 512u, kBlock, 0, stream>>>(t.starts, t.counts);
+        QMB_CHECK_LAUNCH();
+        strings_multi_kernel<<<nb, kBlock, 0, stream>>>(t.group, t.starts, n, flag);
+        QMB_CHECK_LAUNCH();
+        need = tmp_bytes;
+        QMB_HIP_TRY(rocprim::exclusive_scan(tmp, need, flag, mpos, 0u, cap, rocprim::plus<uint32_t>(), stream));
+        auto* mrange = reinterpret_cast<uint2*>(t.mrange);
+        if (t.n_words == 1) {
+            strings_compact_kernel<1><<<nb, kBlock, 0, stream>>>(targets->keys, t.rows, t.group, t.starts, flag, mpos, n, t.mrows,
+                                                                 mrange, t.mcfg, t.counts);
+        } else {
+            strings_compact_kernel<2><<<nb, kBlock, 0, stream>>>(targets->keys, t.rows, t.group, t.starts, flag, mpos, n, t.mrows,
+                                                                 mrange, t.mcfg, t.counts);
+        }
+        QMB_CHECK_LAUNCH();
+    }
+    *out = t;
+    return QMB_OK;
+}
+
+int apply_hops(const qmb_singles* singles, const qmb_strings* strings, const qmb_targets* targets, const double* in, double* out,
+               int mode, hipStream_t stream) {
+    if (!singles || !strings || !targets || !in || !out) {
+        return set_error(QMB_ERR_ARGUMENT, "singles, strings, targets, in and out must be non-null");
+    }
+    if (singles->n_qubytes != strings->n_qubytes || strings->n_qubytes != targets->n_qubytes || strings->n != targets->n) {
+        return set_error(QMB_ERR_ARGUMENT, "singles, strings and targets describe different shapes");
+    }
+    if (mode != 0 && mode != 1) {
+        return set_error(QMB_ERR_ARGUMENT, "mode must be 0 (forward) or 1 (adjoint)");
+    }
+    if (strings->n == 0) {
+        return QMB_OK;
+    }
+    HopParams p{};
+    p.s.a = reinterpret_cast<const double2*>(singles->a);
+    p.s.kd = reinterpret_cast<const double2*>(singles->kd);
+    p.s.a_re = singles->real ? singles->a_re : nullptr;
+    p.s.kd_re = singles->real ? singles->kd_re : nullptr;
+    p.s.n_bits = singles->n_bits;
+    p.s.particle_cut = singles->particle_cut;
+    p.s.ckeys = targets->keys;
+    p.s.in = reinterpret_cast<const double2*>(in);
+    p.s.out = reinterpret_cast<double2*>(out);
+    p.scounts = singles->counts;
+    p.sites = parity_sites(strings->parity);
+    p.mrows = strings->mrows;
+    p.mrange = reinterpret_cast<const uint2*>(strings->mrange);
+    p.mcfg = strings->mcfg;
+    p.counts = strings->counts;
+    const int64_t want = (strings->n + 64 * 4 - 1) / (64 * 4);
+    const unsigned grid = static_cast<unsigned>(want < 8192 ? (want < 1 ? 1 : want) : 8192);
+    auto go = [&](auto kernel) -> int {
+        kernel<<<grid, kBlock, 0, stream>>>(p);
+        QMB_CHECK_LAUNCH();
+        return QMB_OK;
+    };
+    if (strings->n_words == 1) {
+        return mode == 1 ? go(&hop_kernel<1, true>) : go(&hop_kernel<1, false>);
+    }
+    return mode == 1 ? go(&hop_kernel<2, true>) : go(&hop_kernel<2, false>);
 }
 
 int apply_structured_dual(const qmb_pairs* pairs, const qmb_singles* singles, const qmb_cores* cores2, const qmb_cores* cores1,

@@ -127,6 +127,28 @@ class QmbCores(ctypes.Structure):
     ]
 
 
+class QmbStrings(ctypes.Structure):
+    """Mirror of ``struct qmb_strings`` (include/qmb_hip.h)."""
+
+    _fields_ = [
+        ("n_qubytes", _c_i32),
+        ("n_words", _c_i32),
+        ("parity", _c_i32),
+        ("pad", _c_i32),
+        ("n", _c_i64),
+        ("buffer", _vp),
+        ("bytes", _c_size),
+        ("keys", _vp),
+        ("rows", _vp),
+        ("group", _vp),
+        ("starts", _vp),
+        ("mrows", _vp),
+        ("mrange", _vp),
+        ("mcfg", _vp),
+        ("counts", _vp),
+    ]
+
+
 class QmbBatchItem(ctypes.Structure):
     """Mirror of ``struct qmb_batch_item`` (include/qmb_hip.h)."""
 
@@ -221,6 +243,13 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
         [ctypes.POINTER(QmbSingles), ctypes.POINTER(QmbCores), ctypes.POINTER(QmbTargets), _vp, _vp, _c_int, _vp],
     ),
     "qmb_apply_diagonal": (_c_int, [ctypes.POINTER(QmbPairs), ctypes.POINTER(QmbTargets), _vp, _vp, _c_int, _vp]),
+    "qmb_strings_workspace_size": (_c_int, [_c_int, _c_i64, ctypes.POINTER(_c_size)]),
+    "qmb_strings_build": (_c_int, [ctypes.POINTER(QmbTargets), _c_i32, _vp, _c_size, ctypes.POINTER(QmbStrings), _vp]),
+    "qmb_apply_hops": (
+        _c_int,
+        [ctypes.POINTER(QmbSingles), ctypes.POINTER(QmbStrings), ctypes.POINTER(QmbTargets), _vp, _vp, _c_int, _vp],
+    ),
+    "qmb_apply_diagonal_terms": (_c_int, [ctypes.POINTER(QmbTerms), ctypes.POINTER(QmbTargets), _vp, _vp, _c_int, _vp]),
     "qmb_apply_structured_dual": (
         _c_int,
         [ctypes.POINTER(QmbPairs), ctypes.POINTER(QmbSingles), ctypes.POINTER(QmbCores), ctypes.POINTER(QmbCores),

@@ -20,7 +20,8 @@ takes scratch memory from PyTorch's caching allocator and enqueues the C ABI cal
 stream.  Like the reference, find_relative synchronises to learn its output size; diagonal_term and
 single_relative do not.
 
-apply_within (and its adjoint) takes the structured path when the table has pair excitations:
+apply_within (and its adjoint) takes the structured path when the table has pair excitations (or
+hops between sites of one parity, without pair excitations):
 the term table is compiled once per (site, kind, coef) tensors and kept (a small cache keyed by
 the tensors and their versions), and the pair, single and diagonal kernels apply it within one
 config set -- configs_i itself when configs_j is the very same tensor, else configs_j joined with
@@ -133,16 +134,17 @@ def _prepared_terms(nq: int, pc: int, site: torch.Tensor, kind: torch.Tensor, co
 def _within_structured(nq: int, pc: int, configs: torch.Tensor, vec: torch.Tensor, site: torch.Tensor, kind: torch.Tensor,
                        coef: torch.Tensor, mode: int) -> torch.Tensor | None:
     """apply_within (mode 0) or its adjoint (mode 1) within one config set through the structured
-    path, or None when that path does not apply (no pair excitations, repeated rows, nq > 16)."""
+    path (pair-excitation core groups, or the hop path for tables of hops and diagonal terms), or
+    None when neither applies (no such terms, repeated rows, nq > 16)."""
     from . import prepared as pp  # pylint: disable=import-outside-toplevel
 
     if configs.size(0) == 0 or nq > pp.PAIR_MAX_QUBYTES:
         return None
     terms = _prepared_terms(nq, pc, site, kind, coef)
-    if terms.pairs_desc is None:  # type: ignore[attr-defined]
+    if terms.pairs_desc is None and not terms.hops:  # type: ignore[attr-defined]
         return None
     targets = pp.PreparedTargets(configs, nq)
-    if not pp.can_use_pairs(terms, targets, configs):  # type: ignore[arg-type]
+    if not (pp.can_use_pairs(terms, targets, configs) or pp.can_use_hops(terms, targets, configs)):  # type: ignore[arg-type]
         return None
     return pp.apply(terms, targets, configs, vec, mode)  # type: ignore[arg-type]
 

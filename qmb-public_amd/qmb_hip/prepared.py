@@ -30,6 +30,10 @@ PAIR_MAX_QUBYTES = 16
 EXCLUDE_PAIRS = 1
 EXCLUDE_SINGLES = 2
 EXCLUDE_DIAGONAL = 4
+EXCLUDE_HOPS = 8
+# The hop path tests the row pairs of each strings group; it is taken while those tests stay within
+# this many per (row, hop term) -- the enumeration's work per row is a few lookups per hop term.
+HOP_PAIRS_PER_TERM_ROW = 4
 
 
 def _device_count(buffer: torch.Tensor, counts_ptr: int) -> int:
@@ -142,6 +146,45 @@ class PreparedTerms:
                                                             ctypes.byref(self.residual_desc), _stream(device)),
                                "qmb_terms_build_residual")
                 self.n_residual_terms = _device_count(self.residual_buffer, self.residual_desc.counts)
+        # Hop path (a table without pair excitations whose off-diagonal terms include single
+        # excitations between two sites of one parity): the single-excitation table, the diagonal
+        # records of the full table, and the rest for enumeration.
+        self.hops = False
+        self.n_hop_terms = 0
+        if pairs and n_qubytes <= PAIR_MAX_QUBYTES and self.n_terms > 0 and self.pairs_desc is None:
+            self._prepare_hops(site, kind, coef)
+
+    def _prepare_hops(self, site: torch.Tensor, kind: torch.Tensor, coef: torch.Tensor) -> None:
+        lib = _lib.library()
+        device, nq, pc = self.device, self.n_qubytes, self.particle_cut
+        size = ctypes.c_size_t(0)
+        _lib.check(lib.qmb_terms_workspace_size(nq, self.n_terms, ctypes.byref(size)), "qmb_terms_workspace_size")
+        rbuf = _workspace(size.value, device)
+        rdesc = _lib.QmbTerms()
+        with torch.cuda.device(device):
+            _lib.check(lib.qmb_terms_build_residual(nq, pc, _ptr(site), _ptr(kind), _ptr(coef), self.n_terms, EXCLUDE_DIAGONAL | EXCLUDE_HOPS,
+                                                    _ptr(rbuf), rbuf.numel(), ctypes.byref(rdesc), _stream(device)),
+                       "qmb_terms_build_residual")
+        n_residual = _device_count(rbuf, rdesc.counts)
+        valid, diagonal, _ = self.counts()
+        n_hops = valid - diagonal - n_residual
+        if n_hops <= 0:
+            return
+        _lib.check(lib.qmb_singles_workspace_size(nq, ctypes.byref(size)), "qmb_singles_workspace_size")
+        sbuf = _workspace(size.value, device)
+        sdesc = _lib.QmbSingles()
+        with torch.cuda.device(device):
+            _lib.check(lib.qmb_singles_build(nq, pc, _ptr(site), _ptr(kind), _ptr(coef), self.n_terms, _ptr(sbuf), sbuf.numel(),
+                                             ctypes.byref(sdesc), _stream(device)), "qmb_singles_build")
+            if bool((coef[:, 1] == 0).all().item()):
+                # real table: the hop kernel reads 8-byte coefficients
+                _lib.check(lib.qmb_singles_set_real(ctypes.byref(sdesc), 1, _stream(device)), "qmb_singles_set_real")
+        self.singles_buffer, self.singles_desc = sbuf, sdesc
+        self.n_single_terms = _device_count(sbuf, sdesc.counts)
+        self.residual_buffer, self.residual_desc = rbuf, rdesc
+        self.n_residual_terms = n_residual
+        self.n_hop_terms = n_hops
+        self.hops = True
 
     def counts(self) -> tuple[int, int, int]:
         """(valid terms, diagonal terms, invalid terms) -- synchronises."""
@@ -178,6 +221,8 @@ class PreparedTargets:
         # holds by construction)
         self._occupied_hint = max_occupied is not None
         self._cores: dict[int, tuple[torch.Tensor, _lib.QmbCores]] = {}
+        self._strings: dict[int, tuple[torch.Tensor, _lib.QmbStrings]] = {}
+        self._strings_stats: dict[int, dict[str, int]] = {}
 
     def duplicates(self) -> int:
         """Rows equal to an earlier row (synchronises once)."""
@@ -219,6 +264,35 @@ class PreparedTargets:
         return self._cores[order][1]
 
 
+    def strings(self, parity: int) -> _lib.QmbStrings:
+        """Strings index of the set for the hops between sites of `parity` (0 even, 1 odd): the rows
+        grouped by their string on the other parity's sites, built on first use."""
+        if parity not in self._strings:
+            lib = _lib.library()
+            size = ctypes.c_size_t(0)
+            _lib.check(lib.qmb_strings_workspace_size(self.n_qubytes, self.n, ctypes.byref(size)), "qmb_strings_workspace_size")
+            buf = _workspace(size.value, self.device)
+            desc = _lib.QmbStrings()
+            with torch.cuda.device(self.device):
+                _lib.check(lib.qmb_strings_build(ctypes.byref(self.desc), parity, _ptr(buf), buf.numel(), ctypes.byref(desc),
+                                                 _stream(self.device)), "qmb_strings_build")
+            self._strings[parity] = (buf, desc)
+        return self._strings[parity][1]
+
+    def strings_stats(self, parity: int) -> dict[str, int]:
+        """Sizes of a strings index (synchronises): entries, groups, the row pairs the hop kernel
+        tests (sum of size^2 over the groups of two or more rows), the largest group and the entries
+        of the groups of two or more rows."""
+        if parity not in self._strings_stats:
+            desc = self.strings(parity)
+            buf = self._strings[parity][0]
+            off = int(desc.counts) - buf.data_ptr()
+            v = buf[off:off + 40].view(torch.int64).tolist()
+            self._strings_stats[parity] = {"entries": int(v[0]), "groups": int(v[1]), "pair_tests": int(v[2]), "largest": int(v[3]),
+                                           "multi_entries": int(v[4])}
+        return self._strings_stats[parity]
+
+
 def core_stats(targets: PreparedTargets, order: int) -> dict[str, int]:
     """Sizes of a set's core index (synchronises): entries, groups, and the entries and groups of the
     groups holding two or more entries (the only ones the group kernels walk)."""
@@ -248,15 +322,45 @@ def can_use_pairs(terms: PreparedTerms, targets: PreparedTargets, configs_i: tor
     return terms.pairs_desc is not None and targets.indexes(configs_i) and targets.duplicates() == 0
 
 
+def can_use_hops(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tensor) -> bool:
+    """The hop path applies within one distinct-row set indexed by `targets`, for a table with hops
+    and no pair excitations, while its pairwise tests (the strings indexes' sum of group size^2,
+    built here: a synchronisation) stay within HOP_PAIRS_PER_TERM_ROW per row and hop term."""
+    if not terms.hops or terms.pairs_desc is not None or not targets.indexes(configs_i) or targets.duplicates() != 0:
+        return False
+    targets.strings(0)
+    targets.strings(1)
+    tests = sum(targets.strings_stats(parity)["pair_tests"] for parity in (0, 1))
+    return tests <= HOP_PAIRS_PER_TERM_ROW * max(targets.n, 1) * terms.n_hop_terms
+
+
+def apply_hops(terms: PreparedTerms, targets: PreparedTargets, vec: torch.Tensor, mode: int, out: torch.Tensor) -> None:
+    """The hop path's kernels, accumulated into out: the diagonal terms, the hops of both parities
+    and the residual enumeration (terms outside those classes)."""
+    lib = _lib.library()
+    device = targets.device
+    stream = _stream(device)
+    with torch.cuda.device(device):
+        if terms.n_residual_terms > 0:
+            _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.residual_desc), ctypes.byref(targets.desc), _ptr(targets.configs),
+                                              targets.n, _ptr(vec), _ptr(out), mode, None, stream), "qmb_apply_prepared")
+        _lib.check(lib.qmb_apply_diagonal_terms(ctypes.byref(terms.desc), ctypes.byref(targets.desc), _ptr(vec), _ptr(out), mode, stream),
+                   "qmb_apply_diagonal_terms")
+        for parity in (0, 1):
+            _lib.check(lib.qmb_apply_hops(ctypes.byref(terms.singles_desc), ctypes.byref(targets.strings(parity)), ctypes.byref(targets.desc),
+                                          _ptr(vec), _ptr(out), mode, stream), "qmb_apply_hops")
+
+
 def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tensor, vec: torch.Tensor, mode: int,
           out: torch.Tensor | None = None, tile: int = 0, groups: int = 0, zero: bool = True, strategy: int = 0,
-          symmetric: bool | int | None = None, pairs: bool | None = None) -> torch.Tensor:
+          symmetric: bool | int | None = None, pairs: bool | None = None, hops: bool | None = None) -> torch.Tensor:
     """mode 0: psi_j = A psi_i (vec = psi_i as float64 [n_i, 2]); mode 1: grad_i = A^H grad_j.
 
     `out` (float64 [n, 2]) is accumulated into when given, after zeroing unless zero=False.
     symmetric=None uses the symmetric kernel whenever can_use_symmetric() holds.
     pairs=None takes the pair-excitation path whenever can_use_pairs() holds: the pair-excitation
     terms are applied by the core-group kernel and the rest by the enumeration kernel.
+    hops=None takes the hop path whenever can_use_hops() holds (and the pair path does not).
     """
     configs_i = configs_i.contiguous()
     vec = vec.contiguous()
@@ -272,6 +376,12 @@ def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tenso
     use_pairs = mode != MODE_COUNT and (pairs is None or pairs) and can_use_pairs(terms, targets, configs_i)
     if pairs and not use_pairs:
         raise RuntimeError("the pair-excitation path needs pair terms and configs_i equal to the indexed, distinct-row set.")
+    use_hops = not use_pairs and mode != MODE_COUNT and (hops is None or hops) and can_use_hops(terms, targets, configs_i)
+    if hops and not use_hops:
+        raise RuntimeError("the hop path needs hop terms, no pair excitations and configs_i equal to the indexed, distinct-row set.")
+    if use_hops:
+        apply_hops(terms, targets, vec, mode, out)
+        return out
     launch = _launch(tile, groups, strategy, symmetric)
     lib = _lib.library()
     stream = _stream(device)
