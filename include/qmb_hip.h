@@ -201,6 +201,13 @@ int qmb_targets_workspace_size(int n_qubytes, int64_t n, size_t* bytes);
 int qmb_targets_lookup(const qmb_targets* targets, const uint8_t* configs, int64_t n, int64_t* rows, void* stream);
 int qmb_targets_build(int n_qubytes, const uint8_t* configs, int64_t n, void* buffer, size_t bytes, qmb_targets* out,
                       void* stream);
+/* qmb_targets_build in two halves, so a caller whose kernels read only the key words and hash
+ * columns (the core, strings and hop paths) need not build the hash index: _keys lays out the
+ * whole descriptor and fills keys and cols; _index then fills table, filter and duplicates in the
+ * same buffer.  No reference counterpart. */
+int qmb_targets_build_keys(int n_qubytes, const uint8_t* configs, int64_t n, void* buffer, size_t bytes, qmb_targets* out,
+                           void* stream);
+int qmb_targets_build_index(const qmb_targets* targets, void* stream);
 
 /* Launch shape for the enumeration kernels (0 = library default). */
 typedef struct qmb_launch {
@@ -359,8 +366,10 @@ typedef struct qmb_strings {
     /* The entries of the groups of two or more rows (the only ones qmb_apply_hops walks), in index
      * order: */
     uint32_t* mrows;   /* [n] their rows */
-    uint32_t* mrange;  /* [n][2] their group as [first, end) of this order */
+    uint32_t* mrange;  /* [n][2] their group as [first, end) of this order; bit 31 of the end marks a
+                          group whose rows hold more than one other-parity string */
     uint64_t* mcfg;    /* [n][n_words] their config words */
+    uint64_t* mstr;    /* [n] their hop-parity string: bit j = site 2 j + parity */
     int64_t* counts;   /* device: {entries, groups, sum of size^2 over the groups of two or more rows (the
                           pairs qmb_apply_hops tests), largest group, entries of those groups, 0, 0, 0} */
 } qmb_strings;

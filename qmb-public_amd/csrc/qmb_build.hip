@@ -464,8 +464,8 @@ int targets_layout(Carver& cv, int n_qubytes, int64_t n, qmb_targets* out) {
     return QMB_OK;
 }
 
-int targets_build(int n_qubytes, const uint8_t* configs, int64_t n, void* buffer, size_t bytes, qmb_targets* out,
-                  hipStream_t stream) {
+int targets_build_keys(int n_qubytes, const uint8_t* configs, int64_t n, void* buffer, size_t bytes, qmb_targets* out,
+                       hipStream_t stream) {
     Carver cv{static_cast<char*>(buffer), 0};
     qmb_targets t{};
     QMB_TRY(targets_layout(cv, n_qubytes, n, &t));
@@ -474,15 +474,24 @@ int targets_build(int n_qubytes, const uint8_t* configs, int64_t n, void* buffer
     }
     t.bytes = bytes;
     QMB_TRY(keys_dispatch(t.n_words, configs, n, n_qubytes, t.keys, stream));
-    QMB_TRY(launch_fill_u64(t.table, kEmptySlot, static_cast<int64_t>(t.capacity), stream));
-    QMB_TRY(launch_fill_u64(t.filter, 0ull, static_cast<int64_t>(t.filter_words), stream));
-    QMB_TRY(launch_fill_u64(reinterpret_cast<uint64_t*>(t.duplicates), 0ull, 1, stream));
     hash_columns_kernel<<<1, 64 * kMaxWords, 0, stream>>>(t.cols, 64 * kMaxWords);
     QMB_CHECK_LAUNCH();
-    QMB_TRY(insert_dispatch(t.n_words, t.keys, n, t.table, t.capacity - 1, t.filter, t.filter_words - 1, t.cols, t.duplicates,
-                            stream));
     *out = t;
     return QMB_OK;
+}
+
+int targets_build_index(const qmb_targets* t, hipStream_t stream) {
+    QMB_TRY(launch_fill_u64(t->table, kEmptySlot, static_cast<int64_t>(t->capacity), stream));
+    QMB_TRY(launch_fill_u64(t->filter, 0ull, static_cast<int64_t>(t->filter_words), stream));
+    QMB_TRY(launch_fill_u64(reinterpret_cast<uint64_t*>(t->duplicates), 0ull, 1, stream));
+    return insert_dispatch(t->n_words, t->keys, t->n, t->table, t->capacity - 1, t->filter, t->filter_words - 1, t->cols, t->duplicates,
+                           stream);
+}
+
+int targets_build(int n_qubytes, const uint8_t* configs, int64_t n, void* buffer, size_t bytes, qmb_targets* out,
+                  hipStream_t stream) {
+    QMB_TRY(targets_build_keys(n_qubytes, configs, n, buffer, bytes, out, stream));
+    return targets_build_index(out, stream);
 }
 
 // --------------------------------------------------------------------------- config sort -----

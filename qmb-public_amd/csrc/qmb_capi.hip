@@ -297,6 +297,24 @@ int qmb_targets_build(int n_qubytes, const uint8_t* configs, int64_t n, void* bu
     return targets_build(n_qubytes, configs, n, buffer, bytes, out, as_stream(stream));
 }
 
+int qmb_targets_build_keys(int n_qubytes, const uint8_t* configs, int64_t n, void* buffer, size_t bytes, qmb_targets* out,
+                           void* stream) {
+    clear_error();
+    QMB_TRY(check_ptr(configs, n, "configs"));
+    if (!out || !buffer) {
+        return set_error(QMB_ERR_ARGUMENT, "buffer and out must be non-null");
+    }
+    return targets_build_keys(n_qubytes, configs, n, buffer, bytes, out, as_stream(stream));
+}
+
+int qmb_targets_build_index(const qmb_targets* targets, void* stream) {
+    clear_error();
+    if (!targets || !targets->buffer) {
+        return set_error(QMB_ERR_ARGUMENT, "targets must be a descriptor from qmb_targets_build_keys");
+    }
+    return targets_build_index(targets, as_stream(stream));
+}
+
 int qmb_apply_prepared(const qmb_terms* terms, const qmb_targets* targets, const uint8_t* configs_i, int64_t n_i, const double* in,
                        double* out, int mode, const qmb_launch* launch, void* stream) {
     clear_error();

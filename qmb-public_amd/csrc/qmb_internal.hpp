@@ -107,6 +107,11 @@ int terms_build(int n_qubytes, int particle_cut, const int16_t* site, const uint
 int targets_layout(Carver& cv, int n_qubytes, int64_t n, qmb_targets* out);
 int targets_build(int n_qubytes, const uint8_t* configs, int64_t n, void* buffer, size_t bytes, qmb_targets* out,
                   hipStream_t stream);
+// targets_build in two halves: the key words and hash columns, then the hash table, filter and
+// duplicate count over them.
+int targets_build_keys(int n_qubytes, const uint8_t* configs, int64_t n, void* buffer, size_t bytes, qmb_targets* out,
+                       hipStream_t stream);
+int targets_build_index(const qmb_targets* t, hipStream_t stream);
 
 // Sorting of config rows by bytes (radix over byte-swapped words).  keys_be: [n][W] output in
 // sorted order, perm: original row of each sorted position.

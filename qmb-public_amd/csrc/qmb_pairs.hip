@@ -1520,15 +1520,26 @@ __global__ void strings_multi_kernel(const uint32_t* __restrict__ group, const u
     mflag[e] = starts[g + 1] - starts[g] >= 2u ? 1u : 0u;
 }
 
-// The entries of groups of two or more rows, in index order, with their config words gathered and
-// each entry's group as a range of the compacted order (mpos: exclusive scan of mflag);
-// counts[4] = their number.
+// The sites of one parity of a config word, packed into 32 bits (site 2j + parity -> bit j).
+__device__ __forceinline__ uint64_t parity_bits(uint64_t x, int parity) {
+    x = (parity ? x >> 1 : x) & kEvenSites;
+    x = (x | (x >> 1)) & 0x3333333333333333ull;
+    x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+    return x;
+}
+
+// The entries of groups of two or more rows, in index order, with their config words and hop-parity
+// string (site 2j + parity -> bit j) gathered and each entry's group as a range of the compacted
+// order (mpos: exclusive scan of mflag); counts[4] = their number.
 template <int W>
 __global__ void strings_compact_kernel(const uint64_t* __restrict__ ckeys, const uint32_t* __restrict__ rows,
                                        const uint32_t* __restrict__ group, const uint32_t* __restrict__ starts,
-                                       const uint32_t* __restrict__ mflag, const uint32_t* __restrict__ mpos, int64_t n,
+                                       const uint32_t* __restrict__ mflag, const uint32_t* __restrict__ mpos, int64_t n, int parity,
                                        uint32_t* __restrict__ mrows, uint2* __restrict__ mrange, uint64_t* __restrict__ mcfg,
-                                       int64_t* __restrict__ counts) {
+                                       uint64_t* __restrict__ mstr, int64_t* __restrict__ counts) {
     const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (e >= n) {
         return;
@@ -1545,9 +1556,48 @@ __global__ void strings_compact_kernel(const uint64_t* __restrict__ ckeys, const
     const uint32_t row = rows[e];
     mrows[p] = row;
     mrange[p] = make_uint2(lo, lo + (starts[g + 1] - starts[g]));
+    uint64_t str = 0ull;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-        mcfg[static_cast<int64_t>(p) * W + w] = ckeys[static_cast<int64_t>(row) * W + w];
+        const uint64_t c = ckeys[static_cast<int64_t>(row) * W + w];
+        mcfg[static_cast<int64_t>(p) * W + w] = c;
+        str |= parity_bits(c, parity) << (32 * w);
+    }
+    mstr[p] = str;
+}
+
+// A compacted group whose rows hold more than one other-parity string (strings merged by equal
+// keys): mix[first entry] = 1.
+template <int W>
+__global__ void strings_mixed_kernel(const uint2* __restrict__ mrange, const uint64_t* __restrict__ mcfg, const int64_t* __restrict__ counts,
+                                     int parity, uint32_t* __restrict__ mix) {
+    const int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (p < 1 || p >= counts[4]) {
+        return;
+    }
+    const uint32_t lo = mrange[p].x;
+    if (lo == p) {
+        return;
+    }
+    const uint64_t other = parity_sites(parity ^ 1);
+    bool differ = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        differ = differ || ((mcfg[p * W + w] ^ mcfg[(p - 1) * W + w]) & other) != 0ull;
+    }
+    if (differ) {
+        mix[lo] = 1u;
+    }
+}
+
+// Marks the entries of merged groups: bit 31 of their range's end.
+__global__ void strings_mark_kernel(const uint32_t* __restrict__ mix, const int64_t* __restrict__ counts, uint2* __restrict__ mrange) {
+    const int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (p >= counts[4]) {
+        return;
+    }
+    if (mix[mrange[p].x]) {
+        mrange[p].y |= 0x80000000u;
     }
 }
 
@@ -1561,24 +1611,35 @@ struct HopParams {
     SingleParams s;          // the single-excitation tables, in, out and the set's config words
     const int64_t* scounts;  // the tables' counts: [1] = terms with a conditioned site
     uint64_t sites;          // the hop parity's sites, per word
+    int parity;
     const uint32_t* mrows;   // the compacted entries (groups of two or more rows): row,
-    const uint2* mrange;     // group as a range of the compacted order,
-    const uint64_t* mcfg;    // config words
+    const uint2* mrange;     // group as a range of the compacted order (bit 31 of the end: merged group),
+    const uint64_t* mcfg;    // config words,
+    const uint64_t* mstr;    // hop-parity string
     const int64_t* counts;   // [4] = number of compacted entries
 };
 
 // One wave per 64 consecutive compacted entries (rows of groups of two or more): lane = row a.  The
 // wave walks the entries of the union of its rows' groups (consecutive in the compacted order),
-// staged 64 at a time in LDS (config words read in order, amplitudes gathered) and read back as
-// broadcasts, and tests each pair.  A connected pair (a few per row) queues its table index, sign
-// and stage slot; the queue is drained once per staged chunk (or when a lane's queue is full), so
-// the coefficient loads of a chunk are in flight together instead of one load latency per step.
-// Tables with conditioned-site terms take the element at once (it sums a row of K over the core).
-// Each row has one entry, so its sum is added to out[a] without atomics.
+// staged 64 at a time in LDS (amplitudes gathered) and read back as broadcasts, and tests each
+// pair: in a wave of unmerged groups only the hop-parity strings of the rows of the lane's own
+// group (the other parity's strings are equal there), else the whole configs.  The connections a
+// chunk finds (a row may have many: the seed of a star of hops) are then listed in LDS and shared
+// out over the lanes, one (row, entry) per lane per round, so their elements and coefficient loads
+// run side by side; each row's sum collects in LDS and is added to out[a] without atomics (each row
+// has one entry).
 //   ADJ:  out[a] += conj(M_ba) in[b],  M_ba = (-1)^single_parity(c_a; r, q) (A[r][q] + sum_{k in core} K[r][q][k])
 //   !ADJ: out[a] += M_ab in[b],        M_ab = (-1)^single_parity(c_b; q, r) (A[q][r] + sum_{k in core} K[q][r][k])
 // with r the site of a, q the site of b (c_a = core + r, c_b = core + q), as in single_group.
-constexpr int kHopQueue = 8;
+constexpr int kHopList = 256;  // connections listed per round, per wave
+
+template <int W>
+struct HopWaveLds {
+    HopStage<W> stage[64];
+    uint64_t rowc[64][W];      // the lanes' row configs
+    double2 acc[64];           // the lanes' row sums
+    uint16_t list[kHopList];   // lane << 6 | entry slot
+};
 
 template <bool ADJ>
 __device__ __forceinline__ void hop_madd(double xr, double xi, const double2& g, double& re, double& im) {
@@ -1593,9 +1654,9 @@ __device__ __forceinline__ void hop_madd(double xr, double xi, const double2& g,
 
 template <int W, bool ADJ>
 __global__ __launch_bounds__(kBlock) void hop_kernel(HopParams p) {
-    __shared__ HopStage<W> stage_all[kBlock];
+    __shared__ HopWaveLds<W> lds_all[kBlock / 64];
     const int lane = threadIdx.x & 63;
-    HopStage<W>* stage = stage_all + (threadIdx.x & ~63);
+    HopWaveLds<W>& L = lds_all[threadIdx.x >> 6];
     const int64_t wave = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6;
     const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * kBlock) >> 6;
     const int64_t entries = p.counts[4];
@@ -1605,8 +1666,12 @@ __global__ __launch_bounds__(kBlock) void hop_kernel(HopParams p) {
         const int64_t e = base + lane;
         const bool act = e < entries;
         const uint2 range = act ? p.mrange[e] : make_uint2(0xFFFFFFFFu, 0u);
+        const uint32_t gs = range.x, ge = range.y & 0x7FFFFFFFu;
+        // a wave meeting a merged group compares whole configs; the others compare the hop-parity
+        // strings of the rows of their own groups
+        const bool merged = __ballot(act && (range.y >> 31) != 0u) != 0ull;
         // the entries any row of the wave can meet: the union of its rows' groups
-        uint32_t lo = range.x, hi = range.y;
+        uint32_t lo = gs, hi = ge;
         for (int d = 32; d > 0; d >>= 1) {
             const uint32_t ol = __shfl_xor(lo, d);
             const uint32_t oh = __shfl_xor(hi, d);
@@ -1620,106 +1685,137 @@ __global__ __launch_bounds__(kBlock) void hop_kernel(HopParams p) {
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             ca.w[w] = act ? p.mcfg[e * W + w] : 0ull;
+            L.rowc[lane][w] = ca.w[w];
         }
-        double acc_re = 0.0, acc_im = 0.0;
-        // queued connections: table index in bits 0..23, stage slot in 24..29, sign in 31
-        uint32_t queue[kHopQueue];
-        int nq = 0;
-        auto drain = [&]() {
-            double2 x[kHopQueue];
-#pragma unroll
-            for (int k = 0; k < kHopQueue; ++k) {
-                const int64_t rq = static_cast<int64_t>(queue[k] & 0xFFFFFFu);
-                x[k] = make_double2(0.0, 0.0);
-                if (k < nq) {
-                    x[k] = p.s.a_re ? make_double2(p.s.a_re[rq], 0.0) : p.s.a[rq];
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < kHopQueue; ++k) {
-                if (k < nq) {
-                    const uint32_t it = queue[k];
-                    const double2 g = stage[(it >> 24) & 63u].g;
-                    const bool neg = (it >> 31) != 0u;
-                    hop_madd<ADJ>(neg ? -x[k].x : x[k].x, neg ? -x[k].y : x[k].y, g, acc_re, acc_im);
-                }
-            }
-            nq = 0;
-        };
+        const uint64_t sa = act ? p.mstr[e] : 0ull;
+        L.acc[lane] = make_double2(0.0, 0.0);
         for (uint32_t c0 = lo; c0 < hi; c0 += 64u) {
             const uint32_t b = c0 + static_cast<uint32_t>(lane);
             if (b < hi) {
                 HopStage<W> st;
+                if (merged) {
 #pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    st.c[w] = p.mcfg[static_cast<int64_t>(b) * W + w];
+                    for (int w = 0; w < W; ++w) {
+                        st.c[w] = p.mcfg[static_cast<int64_t>(b) * W + w];
+                    }
+                } else {
+                    st.c[0] = p.mstr[b];
                 }
                 st.g = p.s.in[p.mrows[b]];
-                stage[lane] = st;
+                L.stage[lane] = st;
             }
             __builtin_amdgcn_wave_barrier();
             const int nb = static_cast<int>(hi - c0 < 64u ? hi - c0 : 64u);
-            // the pair tests, a few word operations per staged entry: bit u of `hit` marks a
-            // connection of this lane's row to entry u
+            // the pair tests: bit u of `hit` marks a connection of this lane's row to entry u
             uint64_t hit = 0ull;
-            for (int u = 0; u < nb; ++u) {
-                const HopStage<W>& st = stage[u];
-                int pc = 0, pa = 0;
-                uint64_t off = 0ull;
+            if (merged) {
+                for (int u = 0; u < nb; ++u) {
+                    const HopStage<W>& st = L.stage[u];
+                    int pc = 0, pa = 0;
+                    uint64_t off = 0ull;
 #pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    const uint64_t d = ca.w[w] ^ st.c[w];
-                    pc += __popcll(d);
-                    pa += __popcll(d & ca.w[w]);
-                    off |= d & ~p.sites;
+                    for (int w = 0; w < W; ++w) {
+                        const uint64_t d = ca.w[w] ^ st.c[w];
+                        pc += __popcll(d);
+                        pa += __popcll(d & ca.w[w]);
+                        off |= d & ~p.sites;
+                    }
+                    // two sites differ, both of the hop parity, one occupied in each config
+                    hit |= (pc == 2 && pa == 1 && off == 0ull) ? (1ull << u) : 0ull;
                 }
-                // two sites differ, both of the hop parity, one occupied in each config
-                hit |= (pc == 2 && pa == 1 && off == 0ull) ? (1ull << u) : 0ull;
+            } else {
+                // rows of one (unmerged) group share the other parity's string: two hop-parity
+                // sites differ, one occupied in each
+                for (int u = 0; u < nb; ++u) {
+                    const uint32_t bu = c0 + static_cast<uint32_t>(u);
+                    const uint64_t d = sa ^ L.stage[u].c[0];
+                    const bool ok = __popcll(d) == 2 && __popcll(d & sa) == 1 && bu >= gs && bu < ge;
+                    hit |= ok ? (1ull << u) : 0ull;
+                }
             }
             hit = act ? hit : 0ull;
-            // each lane's own connections (a few): the element's table index and sign
-            while (hit) {
-                const int u = __builtin_ctzll(hit);
-                hit &= hit - 1ull;
-                const HopStage<W>& st = stage[u];
-                int r = 0, q = 0;
-#pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    const uint64_t d = ca.w[w] ^ st.c[w];
-                    const uint64_t da = d & ca.w[w], db = d & st.c[w];
-                    r = da ? 64 * w + __builtin_ctzll(da) : r;
-                    q = db ? 64 * w + __builtin_ctzll(db) : q;
-                }
-                Cfg<W> core = ca;
-                flip_site<W>(core, r);
-                Cfg<W> cb = core;
-                flip_site<W>(cb, q);
-                const int sg = fermi ? (ADJ ? single_parity<W>(ca, r, q) : single_parity<W>(cb, q, r)) : 0;
-                if (plain) {
-                    const uint32_t rq = static_cast<uint32_t>(ADJ ? r * p.s.n_bits + q : q * p.s.n_bits + r);
-                    const uint32_t item = rq | (static_cast<uint32_t>(u) << 24) | (static_cast<uint32_t>(sg) << 31);
-#pragma unroll
-                    for (int k = 0; k < kHopQueue; ++k) {
-                        queue[k] = k == nq ? item : queue[k];
-                    }
-                    ++nq;
-                    if (__ballot(nq == kHopQueue) != 0ull) {
-                        drain();
-                    }
-                } else {
-                    const double2 x = ADJ ? single_element<W, 8>(p.s, core, r, q) : single_element<W, 8>(p.s, core, q, r);
-                    hop_madd<ADJ>(sg ? -x.x : x.x, sg ? -x.y : x.y, st.g, acc_re, acc_im);
-                }
+            // list the chunk's connections (lane-major) and share them out, kHopList per round
+            const int cnt = __popcll(hit);
+            int off = cnt;
+            for (int d = 1; d < 64; d <<= 1) {
+                const int o = __shfl_up(off, d);
+                off += lane >= d ? o : 0;
             }
-            if (__ballot(nq != 0) != 0ull) {
-                drain();
+            const int total = __shfl(off, 63);
+            off -= cnt;  // exclusive
+            for (int r0 = 0; r0 < total; r0 += kHopList) {
+                {
+                    uint64_t h = hit;
+                    int k = off;
+                    while (h) {
+                        const int u = __builtin_ctzll(h);
+                        h &= h - 1ull;
+                        if (k >= r0 && k < r0 + kHopList) {
+                            L.list[k - r0] = static_cast<uint16_t>((lane << 6) | u);
+                        }
+                        ++k;
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                const int m = total - r0 < kHopList ? total - r0 : kHopList;
+                for (int i = lane; i < m; i += 64) {
+                    const int item = L.list[i];
+                    const int la = item >> 6, u = item & 63;
+                    const HopStage<W>& st = L.stage[u];
+                    Cfg<W> cx;
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        cx.w[w] = L.rowc[la][w];
+                    }
+                    int r = 0, q = 0;
+                    if (merged) {
+#pragma unroll
+                        for (int w = 0; w < W; ++w) {
+                            const uint64_t d = cx.w[w] ^ st.c[w];
+                            const uint64_t da = d & cx.w[w], db = d & st.c[w];
+                            r = da ? 64 * w + __builtin_ctzll(da) : r;
+                            q = db ? 64 * w + __builtin_ctzll(db) : q;
+                        }
+                    } else {
+                        // the row's hop-parity string from its config words
+                        uint64_t sx = 0ull;
+#pragma unroll
+                        for (int w = 0; w < W; ++w) {
+                            sx |= parity_bits(cx.w[w], p.parity) << (32 * w);
+                        }
+                        const uint64_t d = sx ^ st.c[0];
+                        r = 2 * __builtin_ctzll(d & sx) + p.parity;
+                        q = 2 * __builtin_ctzll(d & st.c[0]) + p.parity;
+                    }
+                    Cfg<W> core = cx;
+                    flip_site<W>(core, r);
+                    Cfg<W> cb = core;
+                    flip_site<W>(cb, q);
+                    const int sg = fermi ? (ADJ ? single_parity<W>(cx, r, q) : single_parity<W>(cb, q, r)) : 0;
+                    double2 x;
+                    if (plain) {
+                        const int64_t rq = ADJ ? static_cast<int64_t>(r) * p.s.n_bits + q : static_cast<int64_t>(q) * p.s.n_bits + r;
+                        x = p.s.a_re ? make_double2(p.s.a_re[rq], 0.0) : p.s.a[rq];
+                    } else {
+                        x = ADJ ? single_element<W, 8>(p.s, core, r, q) : single_element<W, 8>(p.s, core, q, r);
+                    }
+                    double re = 0.0, im = 0.0;
+                    hop_madd<ADJ>(sg ? -x.x : x.x, sg ? -x.y : x.y, st.g, re, im);
+                    if (re != 0.0 || im != 0.0) {
+                        atomicAdd(&L.acc[la].x, re);
+                        atomicAdd(&L.acc[la].y, im);
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
             }
-            __builtin_amdgcn_wave_barrier();
         }
-        if (act && (acc_re != 0.0 || acc_im != 0.0)) {
-            p.s.out[ia].x += acc_re;
-            p.s.out[ia].y += acc_im;
+        __builtin_amdgcn_wave_barrier();
+        const double2 acc = L.acc[lane];
+        if (act && (acc.x != 0.0 || acc.y != 0.0)) {
+            p.s.out[ia].x += acc.x;
+            p.s.out[ia].y += acc.y;
         }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -2234,8 +2330,8 @@ int strings_layout(Carver& cv, int n_qubytes, int64_t n, qmb_strings* out, void*
     if (n_qubytes > kMaxPairBytes) {
         return set_error(QMB_ERR_UNSUPPORTED, "the strings index supports n_qubytes <= %d, got %d", kMaxPairBytes, n_qubytes);
     }
-    if (n < 0 || n >= (1ll << 32) - 1) {
-        return set_error(QMB_ERR_ARGUMENT, "n must be in [0, 2^32 - 1), got %lld", static_cast<long long>(n));
+    if (n < 0 || n >= (1ll << 31)) {
+        return set_error(QMB_ERR_ARGUMENT, "n must be in [0, 2^31), got %lld", static_cast<long long>(n));
     }
     qmb_strings t{};
     t.n_qubytes = n_qubytes;
@@ -2251,6 +2347,7 @@ int strings_layout(Carver& cv, int n_qubytes, int64_t n, qmb_strings* out, void*
     t.mrows = cv.take<uint32_t>(cap);
     t.mrange = cv.take<uint32_t>(2 * cap);
     t.mcfg = cv.take<uint64_t>(cap * static_cast<size_t>(t.n_words));
+    t.mstr = cv.take<uint64_t>(cap);
     size_t tmp = 0;
     {
         uint32_t* u = nullptr;
@@ -2338,12 +2435,22 @@ int strings_build(const qmb_targets* targets, int parity, void* buffer, size_t b
         QMB_HIP_TRY(rocprim::exclusive_scan(tmp, need, flag, mpos, 0u, cap, rocprim::plus<uint32_t>(), stream));
         auto* mrange = reinterpret_cast<uint2*>(t.mrange);
         if (t.n_words == 1) {
-            strings_compact_kernel<1><<<nb, kBlock, 0, stream>>>(targets->keys, t.rows, t.group, t.starts, flag, mpos, n, t.mrows,
-                                                                 mrange, t.mcfg, t.counts);
+            strings_compact_kernel<1><<<nb, kBlock, 0, stream>>>(targets->keys, t.rows, t.group, t.starts, flag, mpos, n, parity,
+                                                                 t.mrows, mrange, t.mcfg, t.mstr, t.counts);
         } else {
-            strings_compact_kernel<2><<<nb, kBlock, 0, stream>>>(targets->keys, t.rows, t.group, t.starts, flag, mpos, n, t.mrows,
-                                                                 mrange, t.mcfg, t.counts);
+            strings_compact_kernel<2><<<nb, kBlock, 0, stream>>>(targets->keys, t.rows, t.group, t.starts, flag, mpos, n, parity,
+                                                                 t.mrows, mrange, t.mcfg, t.mstr, t.counts);
         }
+        QMB_CHECK_LAUNCH();
+        // merged groups (flag is free again: it marks their first entries)
+        QMB_TRY(launch_zero_f64(reinterpret_cast<double*>(flag), static_cast<int64_t>((cap + 1) / 2), stream));
+        if (t.n_words == 1) {
+            strings_mixed_kernel<1><<<nb, kBlock, 0, stream>>>(mrange, t.mcfg, t.counts, parity, flag);
+        } else {
+            strings_mixed_kernel<2><<<nb, kBlock, 0, stream>>>(mrange, t.mcfg, t.counts, parity, flag);
+        }
+        QMB_CHECK_LAUNCH();
+        strings_mark_kernel<<<nb, kBlock, 0, stream>>>(flag, t.counts, mrange);
         QMB_CHECK_LAUNCH();
     }
     *out = t;
@@ -2376,9 +2483,11 @@ int apply_hops(const qmb_singles* singles, const qmb_strings* strings, const qmb
     p.s.out = reinterpret_cast<double2*>(out);
     p.scounts = singles->counts;
     p.sites = parity_sites(strings->parity);
+    p.parity = strings->parity;
     p.mrows = strings->mrows;
     p.mrange = reinterpret_cast<const uint2*>(strings->mrange);
     p.mcfg = strings->mcfg;
+    p.mstr = strings->mstr;
     p.counts = strings->counts;
     const int64_t want = (strings->n + 64 * 4 - 1) / (64 * 4);
     const unsigned grid = static_cast<unsigned>(want < 8192 ? (want < 1 ? 1 : want) : 8192);

@@ -145,6 +145,7 @@ class QmbStrings(ctypes.Structure):
         ("mrows", _vp),
         ("mrange", _vp),
         ("mcfg", _vp),
+        ("mstr", _vp),
         ("counts", _vp),
     ]
 
@@ -220,6 +221,8 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "qmb_terms_build": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _c_i64, _vp, _c_size, ctypes.POINTER(QmbTerms), _vp]),
     "qmb_targets_workspace_size": (_c_int, [_c_int, _c_i64, ctypes.POINTER(_c_size)]),
     "qmb_targets_build": (_c_int, [_c_int, _vp, _c_i64, _vp, _c_size, ctypes.POINTER(QmbTargets), _vp]),
+    "qmb_targets_build_keys": (_c_int, [_c_int, _vp, _c_i64, _vp, _c_size, ctypes.POINTER(QmbTargets), _vp]),
+    "qmb_targets_build_index": (_c_int, [ctypes.POINTER(QmbTargets), _vp]),
     "qmb_targets_lookup": (_c_int, [ctypes.POINTER(QmbTargets), _vp, _c_i64, _vp, _vp]),
     "qmb_apply_prepared": (
         _c_int,

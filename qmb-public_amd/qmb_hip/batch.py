@@ -69,7 +69,7 @@ class Batch:
             it.terms = ctypes.pointer(terms.desc)
             it.pairs = ctypes.pointer(terms.pairs_desc)
             it.singles = ctypes.pointer(terms.singles_desc) if terms.singles_desc is not None else None
-            it.targets = ctypes.pointer(targets.desc)
+            it.targets = ctypes.pointer(targets.keys_desc)
             it.cores2 = ctypes.pointer(targets.cores(2))
             it.cores1 = ctypes.pointer(targets.cores(1)) if terms.singles_desc is not None else None
             it.in_ = self.inp.data_ptr() + 16 * lo
@@ -87,6 +87,10 @@ class Batch:
                                           _stream(device)), "qmb_batch_plan")
         self.waves = 0
         self.residual = [k for k, (t, _) in enumerate(self.instances) if t.n_residual_terms > 0]
+        for k in self.residual:
+            # the residual enumeration looks targets up: build their hash index now, outside any
+            # graph capture of the launches
+            _ = self.instances[k][1].desc
 
     # ------------------------------------------------------------------ inputs and outputs
     def set_inputs(self, psis: typing.Sequence[torch.Tensor], second: typing.Sequence[torch.Tensor] | None = None) -> None:

@@ -211,10 +211,13 @@ class PreparedTargets:
         size = ctypes.c_size_t(0)
         _lib.check(lib.qmb_targets_workspace_size(n_qubytes, self.n, ctypes.byref(size)), "qmb_targets_workspace_size")
         self.buffer = _workspace(size.value, device)
-        self.desc = _lib.QmbTargets()
+        # the key words and hash columns now; the hash table, filter and duplicate count on first use
+        # of `desc` (the core, strings and hop paths read only keys_desc)
+        self._desc = _lib.QmbTargets()
+        self._indexed = False
         with torch.cuda.device(device):
-            _lib.check(lib.qmb_targets_build(n_qubytes, _ptr(configs), self.n, _ptr(self.buffer), self.buffer.numel(),
-                                             ctypes.byref(self.desc), _stream(device)), "qmb_targets_build")
+            _lib.check(lib.qmb_targets_build_keys(n_qubytes, _ptr(configs), self.n, _ptr(self.buffer), self.buffer.numel(),
+                                                  ctypes.byref(self._desc), _stream(device)), "qmb_targets_build_keys")
         self._duplicates: int | None = 0 if distinct else None
         self._max_occupied = max_occupied
         # a bound from the caller is checked against the rows once per core index (a measured one
@@ -223,6 +226,20 @@ class PreparedTargets:
         self._cores: dict[int, tuple[torch.Tensor, _lib.QmbCores]] = {}
         self._strings: dict[int, tuple[torch.Tensor, _lib.QmbStrings]] = {}
         self._strings_stats: dict[int, dict[str, int]] = {}
+
+    @property
+    def desc(self) -> _lib.QmbTargets:
+        """The descriptor with its hash index (built on first use)."""
+        if not self._indexed:
+            with torch.cuda.device(self.device):
+                _lib.check(_lib.library().qmb_targets_build_index(ctypes.byref(self._desc), _stream(self.device)), "qmb_targets_build_index")
+            self._indexed = True
+        return self._desc
+
+    @property
+    def keys_desc(self) -> _lib.QmbTargets:
+        """The descriptor for kernels that read only the key words and hash columns."""
+        return self._desc
 
     def duplicates(self) -> int:
         """Rows equal to an earlier row (synchronises once)."""
@@ -256,7 +273,7 @@ class PreparedTargets:
             buf = _workspace(size.value, self.device)
             desc = _lib.QmbCores()
             with torch.cuda.device(self.device):
-                _lib.check(lib.qmb_cores_build(ctypes.byref(self.desc), occ, order, _ptr(buf), buf.numel(), ctypes.byref(desc),
+                _lib.check(lib.qmb_cores_build(ctypes.byref(self.keys_desc), occ, order, _ptr(buf), buf.numel(), ctypes.byref(desc),
                                                _stream(self.device)), "qmb_cores_build")
             if self._occupied_hint:
                 check_core_overflow(_device_count(buf, desc.counts + 16), occ)
@@ -274,7 +291,7 @@ class PreparedTargets:
             buf = _workspace(size.value, self.device)
             desc = _lib.QmbStrings()
             with torch.cuda.device(self.device):
-                _lib.check(lib.qmb_strings_build(ctypes.byref(self.desc), parity, _ptr(buf), buf.numel(), ctypes.byref(desc),
+                _lib.check(lib.qmb_strings_build(ctypes.byref(self.keys_desc), parity, _ptr(buf), buf.numel(), ctypes.byref(desc),
                                                  _stream(self.device)), "qmb_strings_build")
             self._strings[parity] = (buf, desc)
         return self._strings[parity][1]
@@ -344,10 +361,11 @@ def apply_hops(terms: PreparedTerms, targets: PreparedTargets, vec: torch.Tensor
         if terms.n_residual_terms > 0:
             _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.residual_desc), ctypes.byref(targets.desc), _ptr(targets.configs),
                                               targets.n, _ptr(vec), _ptr(out), mode, None, stream), "qmb_apply_prepared")
-        _lib.check(lib.qmb_apply_diagonal_terms(ctypes.byref(terms.desc), ctypes.byref(targets.desc), _ptr(vec), _ptr(out), mode, stream),
+        _lib.check(lib.qmb_apply_diagonal_terms(ctypes.byref(terms.desc), ctypes.byref(targets.keys_desc), _ptr(vec), _ptr(out), mode,
+                                                stream),
                    "qmb_apply_diagonal_terms")
         for parity in (0, 1):
-            _lib.check(lib.qmb_apply_hops(ctypes.byref(terms.singles_desc), ctypes.byref(targets.strings(parity)), ctypes.byref(targets.desc),
+            _lib.check(lib.qmb_apply_hops(ctypes.byref(terms.singles_desc), ctypes.byref(targets.strings(parity)), ctypes.byref(targets.keys_desc),
                                           _ptr(vec), _ptr(out), mode, stream), "qmb_apply_hops")
 
 
@@ -396,12 +414,12 @@ def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tenso
             _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.residual_desc), ctypes.byref(targets.desc), _ptr(configs_i), n_i,
                                               _ptr(vec), _ptr(out), mode, ctypes.byref(launch) if launch else None, stream),
                        "qmb_apply_prepared")
-        _lib.check(lib.qmb_apply_diagonal(ctypes.byref(terms.pairs_desc), ctypes.byref(targets.desc), _ptr(vec), _ptr(out), pmode,
+        _lib.check(lib.qmb_apply_diagonal(ctypes.byref(terms.pairs_desc), ctypes.byref(targets.keys_desc), _ptr(vec), _ptr(out), pmode,
                                           stream), "qmb_apply_diagonal")
         if terms.singles_desc is not None:
-            _lib.check(lib.qmb_apply_singles(ctypes.byref(terms.singles_desc), ctypes.byref(targets.cores(1)), ctypes.byref(targets.desc),
+            _lib.check(lib.qmb_apply_singles(ctypes.byref(terms.singles_desc), ctypes.byref(targets.cores(1)), ctypes.byref(targets.keys_desc),
                                              _ptr(vec), _ptr(out), pmode, stream), "qmb_apply_singles")
-        _lib.check(lib.qmb_apply_pairs(ctypes.byref(terms.pairs_desc), ctypes.byref(targets.cores(2)), ctypes.byref(targets.desc),
+        _lib.check(lib.qmb_apply_pairs(ctypes.byref(terms.pairs_desc), ctypes.byref(targets.cores(2)), ctypes.byref(targets.keys_desc),
                                        _ptr(vec), _ptr(out), pmode, stream), "qmb_apply_pairs")
     return out
 
@@ -423,7 +441,7 @@ def apply_dual(terms: PreparedTerms, targets: PreparedTargets, vec: torch.Tensor
     cores1 = ctypes.byref(targets.cores(1)) if terms.singles_desc is not None else None
     with torch.cuda.device(targets.device):
         _lib.check(_lib.library().qmb_apply_structured_dual(ctypes.byref(terms.pairs_desc), singles, ctypes.byref(targets.cores(2)), cores1,
-                                                            ctypes.byref(targets.desc), _ptr(vec), _ptr(out), _ptr(vec2), _ptr(out2),
+                                                            ctypes.byref(targets.keys_desc), _ptr(vec), _ptr(out), _ptr(vec2), _ptr(out2),
                                                             mode, _stream(targets.device)), "qmb_apply_structured_dual")
     return out, out2
 

@@ -206,12 +206,26 @@ def test_strings_index_structure(cuda_device: torch.device) -> None:
         mrange = (buf[off:off + 8 * m].view(torch.int32).cpu().numpy().astype(np.int64) & 0xFFFFFFFF).reshape(m, 2)
         msizes = sizes[sizes > 1]
         mstart = np.repeat(np.concatenate(([0], np.cumsum(msizes)[:-1])), msizes)
-        assert (mrange[:, 0] == mstart).all() and (mrange[:, 1] == mstart + np.repeat(msizes, msizes)).all()
+        end, merged_bit = mrange[:, 1] & 0x7FFFFFFF, mrange[:, 1] >> 31
+        assert (mrange[:, 0] == mstart).all() and (end == mstart + np.repeat(msizes, msizes)).all()
         off = int(desc.mcfg) - base
         mcfg = buf[off:off + 16 * m].view(torch.int64).cpu().numpy().reshape(m, 2)
         toff = int(targets.desc.keys) - targets.buffer.data_ptr()
         words = targets.buffer[toff:toff + 16 * n].view(torch.int64).cpu().numpy().reshape(n, 2)
         assert (mcfg == words[mrows]).all()
+        # the hop-parity string: bit j = site 2 j + parity; merged groups flagged on every entry
+        off = int(desc.mstr) - base
+        mstr = buf[off:off + 8 * m].view(torch.int64).cpu().numpy().view(np.uint64)
+        mocc = occ[mrows]
+        want = np.zeros(m, dtype=np.uint64)
+        for j in range(40):
+            want |= mocc[:, 2 * j + parity].astype(np.uint64) << np.uint64(j)
+        assert (mstr == want).all()
+        mother = mocc[:, (1 - parity)::2]
+        for lo in np.unique(mrange[:, 0]):
+            hi = int(end[lo])
+            pure = bool((mother[lo:hi] == mother[lo]).all())
+            assert (merged_bit[lo:hi] == (0 if pure else 1)).all()
 
 
 def test_shared_string_set_falls_back_to_enumeration(cuda_device: torch.device) -> None:
