@@ -157,11 +157,12 @@ def hop_model(terms: object, targets: object, nq: int, n: int) -> dict[str, obje
       hop_kernel (one parity): per entry of a group of >= 2 rows its group range (8), row (4) and
                      config words (8 W), its row's amplitude (16) and output, read and written (32);
                      the single-excitation table (n_bits^2 entries) once;
-      diag_terms_kernel: per row its config words, amplitude and output (8 W + 32); the diagonal
-                     records once.
-    Per strings index build (one parity): per row its config words read (8 W), key and row emitted
-    (8), sorted (8 read + 8 written), group and start found (8); per entry of a group of >= 2 rows
-    its compacted row, range and config words written (12 + 8 W)."""
+      diag_scale_kernel: per row its diagonal value, amplitude and output (16 + 16 + 32).
+    Per set (once for the step's applications): the diagonal values, per row its config words read
+    and its value written (8 W + 16); per strings index build (one parity): per row its config words
+    read (8 W), key and row emitted (8), sorted (8 read + 8 written), group and start found (8); per
+    entry of a group of >= 2 rows its compacted row, range, config words and string written (20 +
+    8 W)."""
     words = (nq + 7) // 8
     n_bits = 8 * nq
     sb = 8 if terms.singles_desc.real else 16
@@ -170,10 +171,9 @@ def hop_model(terms: object, targets: object, nq: int, n: int) -> dict[str, obje
     for parity in (0, 1):
         st = targets.strings_stats(parity)
         out[f"hop_kernel_{parity}"] = dict(st, bytes=st["multi_entries"] * (8 + 4 + 8 * words + 16 + 32) + sb * n_bits ** 2)
-        build += n * (8 * words + 8 + 16 + 8) + st["multi_entries"] * (12 + 8 * words)
-    valid, diagonal, _ = terms.counts()
-    out["diag_terms_kernel"] = {"bytes": n * (8 * words + 32) + 24 * diagonal}
-    out["strings_build"] = {"bytes": build}
+        build += n * (8 * words + 8 + 16 + 8) + st["multi_entries"] * (20 + 8 * words)
+    out["diag_scale_kernel"] = {"bytes": n * 64}
+    out["strings_build"] = {"bytes": build + n * (8 * words + 16)}
     return out
 
 
@@ -988,8 +988,11 @@ def main() -> int:
         parts["hop_kernel_ms"] = [timed(lambda parity=parity: lib.qmb_apply_hops(
             ctypes.byref(terms.singles_desc), ctypes.byref(targets.strings(parity)), ctypes.byref(targets.desc), vin, vout,
             pp.MODE_FORWARD, s_ptr)) for parity in (0, 1)]
-        parts["diag_terms_kernel_ms"] = timed(lambda: lib.qmb_apply_diagonal_terms(ctypes.byref(terms.desc), ctypes.byref(targets.desc),
-                                                                                   vin, vout, pp.MODE_FORWARD, s_ptr))
+        dvals = targets.diagonal_values(terms)
+        parts["diagonal_values_ms"] = timed(lambda: lib.qmb_diagonal_values(ctypes.byref(terms.desc), ctypes.byref(targets.keys_desc),
+                                                                            ctypes.c_void_p(dvals.data_ptr()), s_ptr))
+        parts["diagonal_apply_ms"] = timed(lambda: lib.qmb_apply_diagonal_values(n, ctypes.c_void_p(dvals.data_ptr()), vin, vout,
+                                                                                 pp.MODE_FORWARD, s_ptr))
 
         def rebuild_strings() -> None:
             targets._strings.clear()  # pylint: disable=protected-access

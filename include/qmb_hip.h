@@ -351,9 +351,9 @@ int qmb_apply_diagonal(const qmb_pairs* pairs, const qmb_targets* targets, const
  * so the rows are grouped by that string and the rows of a group are tested pairwise.
  *
  * The strings index of parity s (0: hops between even sites, 1: between odd sites): one entry per
- * row, radix-sorted by a 32-bit key of the row's string on the sites of parity 1 - s (its core
- * hash mod 2^32 - 1; rows of different strings with equal keys share a group and are told apart
- * per pair). */
+ * row, radix-sorted by a 32-bit key of the row's string on the sites of parity 1 - s (a 64-bit
+ * hash of it mod 2^32 - 1; rows of different strings with equal keys share a group, flagged as
+ * merged, and are told apart per pair). */
 typedef struct qmb_strings {
     int32_t n_qubytes, n_words, parity, pad;
     int64_t n;
@@ -390,6 +390,12 @@ int qmb_apply_hops(const qmb_singles* singles, const qmb_strings* strings, const
  * where qmb_apply_diagonal's folded table costs a read per occupied pair. */
 int qmb_apply_diagonal_terms(const qmb_terms* terms, const qmb_targets* targets, const double* in, double* out, int mode,
                              void* stream);
+
+/* The same diagonal in two halves, for a set applied to more than once (the forward and backward of
+ * a step, a Krylov loop): d[i] (float64 [n][2]) = the diagonal value of row i, then out[i] +=
+ * d[i] in[i] (mode 0) or conj(d[i]) in[i] (mode 1) per application.  No reference counterpart. */
+int qmb_diagonal_values(const qmb_terms* terms, const qmb_targets* targets, double* d, void* stream);
+int qmb_apply_diagonal_values(int64_t n, const double* d, const double* in, double* out, int mode, void* stream);
 
 /* The diagonal, single- and pair-excitation parts of A (mode 0) or A^H (mode 1) on two vectors at
  * once, for one set: out += A in and out2 += A in2, each kernel walking its groups once for both

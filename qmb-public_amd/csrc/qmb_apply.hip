@@ -401,9 +401,19 @@ __device__ __forceinline__ void add_diag(int64_t i, double re, double im, const 
     }
 }
 
+// The row's diagonal value: added times the input (VALUES false) or written to out (VALUES true).
+template <bool ADJ, bool VALUES>
+__device__ __forceinline__ void put_diag(int64_t i, double re, double im, const double2* __restrict__ in, double2* __restrict__ out) {
+    if (VALUES) {
+        out[i] = make_double2(re, im);
+    } else {
+        add_diag<ADJ>(i, re, im, in, out);
+    }
+}
+
 // Grid-stride over rows.  Up to kDiagMaskTile diagonal terms are staged once per block; more are
 // walked a tile at a time per block of rows.
-template <int W, bool ADJ>
+template <int W, bool ADJ, bool VALUES>
 __global__ __launch_bounds__(kBlock) void diag_terms_kernel(const uint64_t* __restrict__ keys, int64_t n,
                                                            const uint64_t* __restrict__ drecs, const double2* __restrict__ dcoefs,
                                                            const int64_t* __restrict__ counts, const double2* __restrict__ in,
@@ -422,7 +432,7 @@ __global__ __launch_bounds__(kBlock) void diag_terms_kernel(const uint64_t* __re
             }
             double re = 0.0, im = 0.0;
             sum_diag_masks<W>(c, s_mask, static_cast<int>(td), re, im);
-            add_diag<ADJ>(i, re, im, in, out);
+            put_diag<ADJ, VALUES>(i, re, im, in, out);
         }
         return;
     }
@@ -443,9 +453,21 @@ __global__ __launch_bounds__(kBlock) void diag_terms_kernel(const uint64_t* __re
             sum_diag_masks<W>(c, s_mask, cnt, re, im);
         }
         if (valid) {
-            add_diag<ADJ>(i, re, im, in, out);
+            put_diag<ADJ, VALUES>(i, re, im, in, out);
         }
     }
+}
+
+// out[i] += d[i] in[i] (conj(d[i]) for the adjoint), d the rows' diagonal values.
+template <bool ADJ>
+__global__ __launch_bounds__(kBlock) void diag_scale_kernel(const double2* __restrict__ d, int64_t n, const double2* __restrict__ in,
+                                                           double2* __restrict__ out) {
+    const int64_t i = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const double2 v = d[i];
+    add_diag<ADJ>(i, v.x, v.y, in, out);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -689,13 +711,58 @@ int apply_diagonal_terms(const qmb_terms* terms, const qmb_targets* targets, con
         auto* vo = reinterpret_cast<double2*>(out);
         const unsigned grid = blocks_for(targets->n) < 2048u ? blocks_for(targets->n) : 2048u;
         if (mode == 1) {
-            diag_terms_kernel<W, true><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo);
+            diag_terms_kernel<W, true, false><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi,
+                                                                          vo);
         } else {
-            diag_terms_kernel<W, false><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi, vo);
+            diag_terms_kernel<W, false, false><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs, dc, terms->counts, vi,
+                                                                           vo);
         }
         QMB_CHECK_LAUNCH();
         return QMB_OK;
     });
+}
+
+int diagonal_values(const qmb_terms* terms, const qmb_targets* targets, double* d, hipStream_t stream) {
+    if (!terms || !targets || !d) {
+        return set_error(QMB_ERR_ARGUMENT, "terms, targets and d must be non-null");
+    }
+    if (terms->n_qubytes != targets->n_qubytes) {
+        return set_error(QMB_ERR_ARGUMENT, "terms built for %d bytes, targets for %d", terms->n_qubytes, targets->n_qubytes);
+    }
+    if (targets->n == 0) {
+        return QMB_OK;
+    }
+    return dispatch_words(terms->n_words, [&](auto wc) -> int {
+        constexpr int W = decltype(wc)::value;
+        const unsigned grid = blocks_for(targets->n) < 2048u ? blocks_for(targets->n) : 2048u;
+        diag_terms_kernel<W, false, true><<<grid, kBlock, 0, stream>>>(targets->keys, targets->n, terms->drecs,
+                                                                      reinterpret_cast<const double2*>(terms->dcoefs), terms->counts,
+                                                                      nullptr, reinterpret_cast<double2*>(d));
+        QMB_CHECK_LAUNCH();
+        return QMB_OK;
+    });
+}
+
+int apply_diagonal_values(int64_t n, const double* d, const double* in, double* out, int mode, hipStream_t stream) {
+    if (n < 0 || (n > 0 && (!d || !in || !out))) {
+        return set_error(QMB_ERR_ARGUMENT, "d, in and out must be non-null and n >= 0");
+    }
+    if (mode != 0 && mode != 1) {
+        return set_error(QMB_ERR_ARGUMENT, "mode must be 0 (forward) or 1 (adjoint)");
+    }
+    if (n == 0) {
+        return QMB_OK;
+    }
+    const auto* dv = reinterpret_cast<const double2*>(d);
+    const auto* vi = reinterpret_cast<const double2*>(in);
+    auto* vo = reinterpret_cast<double2*>(out);
+    if (mode == 1) {
+        diag_scale_kernel<true><<<blocks_for(n), kBlock, 0, stream>>>(dv, n, vi, vo);
+    } else {
+        diag_scale_kernel<false><<<blocks_for(n), kBlock, 0, stream>>>(dv, n, vi, vo);
+    }
+    QMB_CHECK_LAUNCH();
+    return QMB_OK;
 }
 
 size_t single_relative_scratch_bytes(int64_t n_terms) {

@@ -471,6 +471,16 @@ int qmb_apply_diagonal_terms(const qmb_terms* terms, const qmb_targets* targets,
     return apply_diagonal_terms(terms, targets, in, out, mode, as_stream(stream));
 }
 
+int qmb_diagonal_values(const qmb_terms* terms, const qmb_targets* targets, double* d, void* stream) {
+    clear_error();
+    return diagonal_values(terms, targets, d, as_stream(stream));
+}
+
+int qmb_apply_diagonal_values(int64_t n, const double* d, const double* in, double* out, int mode, void* stream) {
+    clear_error();
+    return apply_diagonal_values(n, d, in, out, mode, as_stream(stream));
+}
+
 int qmb_apply_structured_dual(const qmb_pairs* pairs, const qmb_singles* singles, const qmb_cores* cores2, const qmb_cores* cores1,
                               const qmb_targets* targets, const double* in, double* out, const double* in2, double* out2, int mode,
                               void* stream) {

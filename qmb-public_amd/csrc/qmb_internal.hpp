@@ -140,6 +140,8 @@ int apply_plain(int n_qubytes, int particle_cut, const uint8_t* configs_i, const
 int diagonal_prepared(const qmb_terms* terms, const uint8_t* configs, int64_t n, double* out, hipStream_t stream);
 int apply_diagonal_terms(const qmb_terms* terms, const qmb_targets* targets, const double* in, double* out, int mode,
                          hipStream_t stream);
+int diagonal_values(const qmb_terms* terms, const qmb_targets* targets, double* d, hipStream_t stream);
+int apply_diagonal_values(int64_t n, const double* d, const double* in, double* out, int mode, hipStream_t stream);
 
 int single_relative_prepared(int n_qubytes, int particle_cut, const uint8_t* configs, int64_t n, const int16_t* site,
                              const uint8_t* kind, const double* coef, int64_t n_terms, uint64_t seed, const double* keys,

@@ -1431,7 +1431,7 @@ __global__ __launch_bounds__(kBlock) void single_kernel(SingleParams p) {
 // spin: even = up, odd = down) leaves the string on the other parity's sites unchanged, so the two
 // configs it connects hold equal strings there.  The strings index of parity s groups a set's rows
 // by their string on the sites of parity 1 - s: one entry per row, sorted by a 32-bit key of that
-// string (its hash mod 2^32 - 1).  Inside a group two rows are connected by such a hop iff their
+// string (a hash of it mod 2^32 - 1).  Inside a group two rows are connected by such a hop iff their
 // configs differ in exactly two sites, both of parity s, one occupied in each; rows of different
 // strings merged by equal keys fail that test.  Building it sorts n keys, where the order-1 core
 // index sorts n per occupied site.
@@ -1439,21 +1439,33 @@ constexpr uint64_t kEvenSites = 0x5555555555555555ull;
 
 __host__ __device__ __forceinline__ uint64_t parity_sites(int parity) { return parity ? ~kEvenSites : kEvenSites; }
 
+// The sites of one parity of a config word, packed into 32 bits (site 2j + parity -> bit j).
+__device__ __forceinline__ uint64_t parity_bits(uint64_t x, int parity) {
+    x = (parity ? x >> 1 : x) & kEvenSites;
+    x = (x | (x >> 1)) & 0x3333333333333333ull;
+    x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+    return x;
+}
+
+// The key of a row: its other-parity string packed (bit 32 w + j = site 64 w + 2 j + 1 - parity),
+// mixed by the 64-bit finaliser, mod 2^32 - 1 (the key no entry holds marks nothing here).
 template <int W>
-__global__ void strings_emit_kernel(const uint64_t* __restrict__ keys, const uint64_t* __restrict__ cols, int64_t n, int parity,
-                                    uint32_t* __restrict__ okeys, uint32_t* __restrict__ orows) {
+__global__ void strings_emit_kernel(const uint64_t* __restrict__ keys, int64_t n, int parity, uint32_t* __restrict__ okeys,
+                                    uint32_t* __restrict__ orows) {
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i >= n) {
         return;
     }
-    Cfg<W> c = load_key<W>(keys, i);
-    const uint64_t other = parity_sites(parity ^ 1);
+    const Cfg<W> c = load_key<W>(keys, i);
+    uint64_t other = 0ull;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-        c.w[w] &= other;
+        other |= parity_bits(c.w[w], parity ^ 1) << (32 * w);
     }
-    // below 2^32 - 1: the key that no entry holds marks nothing here, every row has its entry
-    okeys[i] = static_cast<uint32_t>(hash_cfg<W>(c, cols) % 0xFFFFFFFFull);
+    okeys[i] = static_cast<uint32_t>(fmix64(other ^ 0x2545f4914f6cdd1dull) % 0xFFFFFFFFull);
     orows[i] = static_cast<uint32_t>(i);
 }
 
@@ -1518,17 +1530,6 @@ __global__ void strings_multi_kernel(const uint32_t* __restrict__ group, const u
     }
     const uint32_t g = group[e];
     mflag[e] = starts[g + 1] - starts[g] >= 2u ? 1u : 0u;
-}
-
-// The sites of one parity of a config word, packed into 32 bits (site 2j + parity -> bit j).
-__device__ __forceinline__ uint64_t parity_bits(uint64_t x, int parity) {
-    x = (parity ? x >> 1 : x) & kEvenSites;
-    x = (x | (x >> 1)) & 0x3333333333333333ull;
-    x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
-    x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
-    x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
-    x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
-    return x;
 }
 
 // The entries of groups of two or more rows, in index order, with their config words and hop-parity
@@ -2414,9 +2415,9 @@ int strings_build(const qmb_targets* targets, int parity, void* buffer, size_t b
         void* tmp = sc.take_bytes(tmp_bytes);
         const unsigned nb = blocks_for(n);
         if (t.n_words == 1) {
-            strings_emit_kernel<1><<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, n, parity, keys_in, rows_in);
+            strings_emit_kernel<1><<<nb, kBlock, 0, stream>>>(targets->keys, n, parity, keys_in, rows_in);
         } else {
-            strings_emit_kernel<2><<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, n, parity, keys_in, rows_in);
+            strings_emit_kernel<2><<<nb, kBlock, 0, stream>>>(targets->keys, n, parity, keys_in, rows_in);
         }
         QMB_CHECK_LAUNCH();
         size_t need = tmp_bytes;

@@ -253,6 +253,8 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
         [ctypes.POINTER(QmbSingles), ctypes.POINTER(QmbStrings), ctypes.POINTER(QmbTargets), _vp, _vp, _c_int, _vp],
     ),
     "qmb_apply_diagonal_terms": (_c_int, [ctypes.POINTER(QmbTerms), ctypes.POINTER(QmbTargets), _vp, _vp, _c_int, _vp]),
+    "qmb_diagonal_values": (_c_int, [ctypes.POINTER(QmbTerms), ctypes.POINTER(QmbTargets), _vp, _vp]),
+    "qmb_apply_diagonal_values": (_c_int, [_c_i64, _vp, _vp, _vp, _c_int, _vp]),
     "qmb_apply_structured_dual": (
         _c_int,
         [ctypes.POINTER(QmbPairs), ctypes.POINTER(QmbSingles), ctypes.POINTER(QmbCores), ctypes.POINTER(QmbCores),

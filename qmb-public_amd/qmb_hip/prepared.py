@@ -226,6 +226,7 @@ class PreparedTargets:
         self._cores: dict[int, tuple[torch.Tensor, _lib.QmbCores]] = {}
         self._strings: dict[int, tuple[torch.Tensor, _lib.QmbStrings]] = {}
         self._strings_stats: dict[int, dict[str, int]] = {}
+        self._diagonal: dict[int, tuple[PreparedTerms, torch.Tensor]] = {}
 
     @property
     def desc(self) -> _lib.QmbTargets:
@@ -296,6 +297,18 @@ class PreparedTargets:
             self._strings[parity] = (buf, desc)
         return self._strings[parity][1]
 
+    def diagonal_values(self, terms: "PreparedTerms") -> torch.Tensor:
+        """The diagonal value of every row under `terms` (float64 [n, 2]), computed on first use and
+        kept with this set, so that each further application adds it with one pass."""
+        key = id(terms)
+        if key not in self._diagonal:
+            d = torch.empty((max(self.n, 1), 2), dtype=torch.float64, device=self.device)
+            with torch.cuda.device(self.device):
+                _lib.check(_lib.library().qmb_diagonal_values(ctypes.byref(terms.desc), ctypes.byref(self.keys_desc), _ptr(d),
+                                                              _stream(self.device)), "qmb_diagonal_values")
+            self._diagonal[key] = (terms, d)
+        return self._diagonal[key][1]
+
     def strings_stats(self, parity: int) -> dict[str, int]:
         """Sizes of a strings index (synchronises): entries, groups, the row pairs the hop kernel
         tests (sum of size^2 over the groups of two or more rows), the largest group and the entries
@@ -352,8 +365,9 @@ def can_use_hops(terms: PreparedTerms, targets: PreparedTargets, configs_i: torc
 
 
 def apply_hops(terms: PreparedTerms, targets: PreparedTargets, vec: torch.Tensor, mode: int, out: torch.Tensor) -> None:
-    """The hop path's kernels, accumulated into out: the diagonal terms, the hops of both parities
-    and the residual enumeration (terms outside those classes)."""
+    """The hop path's kernels, accumulated into out: the diagonal terms (their per-row values kept
+    with the set), the hops of both parities and the residual enumeration (terms outside those
+    classes)."""
     lib = _lib.library()
     device = targets.device
     stream = _stream(device)
@@ -361,9 +375,8 @@ def apply_hops(terms: PreparedTerms, targets: PreparedTargets, vec: torch.Tensor
         if terms.n_residual_terms > 0:
             _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.residual_desc), ctypes.byref(targets.desc), _ptr(targets.configs),
                                               targets.n, _ptr(vec), _ptr(out), mode, None, stream), "qmb_apply_prepared")
-        _lib.check(lib.qmb_apply_diagonal_terms(ctypes.byref(terms.desc), ctypes.byref(targets.keys_desc), _ptr(vec), _ptr(out), mode,
-                                                stream),
-                   "qmb_apply_diagonal_terms")
+        _lib.check(lib.qmb_apply_diagonal_values(targets.n, _ptr(targets.diagonal_values(terms)), _ptr(vec), _ptr(out), mode, stream),
+                   "qmb_apply_diagonal_values")
         for parity in (0, 1):
             _lib.check(lib.qmb_apply_hops(ctypes.byref(terms.singles_desc), ctypes.byref(targets.strings(parity)), ctypes.byref(targets.keys_desc),
                                           _ptr(vec), _ptr(out), mode, stream), "qmb_apply_hops")

@@ -156,6 +156,32 @@ def test_spinful_table_with_conditioned_and_cross_hops(cuda_device: torch.device
     _check_oracle_rows(w, 7600 + n_sites, rows=4)
 
 
+def test_diagonal_values_match_oracle_diagonal_term(cuda_device: torch.device) -> None:
+    """qmb_diagonal_values on a set equals the oracle's diagonal_term of its rows (the same terms
+    summed in original order, so bit-exact), for a table with n_p and n_p n_q terms over 256 of
+    them (two LDS tiles), and qmb_apply_diagonal_values adds d * in (conj(d) for the adjoint)."""
+    site, kind, coef = _spinful_random_terms(40, 7350, 700)
+    cfg = _hubbard_configs(40, 3000, 7351)
+    w = _prepare(site, kind, coef, cfg, 10, cuda_device)
+    assert w["terms"].counts()[1] > 256
+    d = torch.view_as_complex(w["targets"].diagonal_values(w["terms"])).cpu().numpy()
+    want = orc.diagonal_term(1, cfg, site, kind, coef)
+    assert np.array_equal(d, want)
+    from qmb_hip import _lib  # pylint: disable=import-outside-toplevel
+
+    lib = _lib.library()
+    vec = sy.random_psi(len(cfg), 7352)
+    v = torch.view_as_real(torch.from_numpy(vec).to(cuda_device)).contiguous()
+    for mode, dd in ((0, d), (1, np.conj(d))):
+        out = torch.zeros_like(v)
+        _lib.check(lib.qmb_apply_diagonal_values(len(cfg), ctypes.c_void_p(w["targets"].diagonal_values(w["terms"]).data_ptr()),
+                                                 ctypes.c_void_p(v.data_ptr()), ctypes.c_void_p(out.data_ptr()), mode, None),
+                   "qmb_apply_diagonal_values")
+        torch.cuda.synchronize()
+        got = torch.view_as_complex(out).cpu().numpy()
+        assert float(np.abs(got - dd * vec).max()) <= 1e-14 * float(np.abs(dd * vec).max())
+
+
 def test_strings_index_structure(cuda_device: torch.device) -> None:
     cfg = _hubbard_configs(40, 20000, 7700)
     cfg_t = torch.from_numpy(cfg).to(cuda_device)
