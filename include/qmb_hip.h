@@ -78,13 +78,18 @@ double qmb_philox_uniform_host(uint64_t seed, uint64_t index);
  * synchronises after every call) and, when the table has pair excitations, the rows are distinct and
  * the workspace holds the core indexes for the set's most occupied row, applies the structured path
  * (pair and single excitations by core groups, the folded diagonal, the remaining terms enumerated).
- * Otherwise -- two config sets, or a workspace of qmb_apply_within_workspace_size -- it enumerates
- * every connection with hash lookups and does not synchronise.  Both give the same results (to
- * FP64 summation order). */
+ * Failing that, with a workspace of at least qmb_apply_within_hops_workspace_size, a table with
+ * hops between sites of one parity and distinct rows take the hop path (rows grouped by the other
+ * parity's string, two more synchronisations: the term classes, then the pair-test count, which
+ * must stay within 4 per row and hop term).  Otherwise -- two config sets, or a workspace of
+ * qmb_apply_within_workspace_size -- it enumerates every connection with hash lookups and does
+ * not synchronise.  All give the same results (to FP64 summation order). */
 int qmb_apply_within_workspace_size(int n_qubytes, int64_t n_i, int64_t n_j, int64_t n_terms, size_t* bytes);
 /* Workspace for the structured path on one set of n configs whose rows hold at most max_occupied
  * occupied sites (e.g. the particle number); n_qubytes <= 16.  No reference counterpart. */
 int qmb_apply_within_structured_workspace_size(int n_qubytes, int64_t n, int64_t n_terms, int32_t max_occupied, size_t* bytes);
+/* Workspace for the hop path on one set of n configs; n_qubytes <= 16.  No reference counterpart. */
+int qmb_apply_within_hops_workspace_size(int n_qubytes, int64_t n, int64_t n_terms, size_t* bytes);
 int qmb_apply_within(int n_qubytes, int particle_cut, const uint8_t* configs_i, const double* psi_i, int64_t n_i,
                      const uint8_t* configs_j, int64_t n_j, const int16_t* site, const uint8_t* kind, const double* coef,
                      int64_t n_terms, double* psi_j, void* workspace, size_t workspace_bytes, void* stream);

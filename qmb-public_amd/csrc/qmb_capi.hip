@@ -776,6 +776,108 @@ int apply_structured(int nq, int pc, const uint8_t* configs, int64_t n, const in
     return 0;
 }
 
+// The hop path behind the C ABI (one config set, same-parity hops): the full term table (its
+// diagonal records), the target index (its duplicate count), the single-excitation table, the terms
+// left to enumeration, and a strings index per parity.  Two synchronisations: one reads the term
+// classes and repeated rows, one the strings indexes' pair-test counts, which must stay within
+// kHopTestsPerTermRow per (row, hop term) or the call enumerates instead.
+constexpr int64_t kHopTestsPerTermRow = 4;
+
+struct HopLayout {
+    size_t terms, targets, singles, residual, strings, total;
+};
+
+HopLayout hop_layout(int nq, int64_t n, int64_t n_terms) {
+    HopLayout l{};
+    l.terms = terms_bytes(nq, n_terms);
+    l.targets = targets_bytes(nq, n);
+    l.singles = singles_bytes(nq);
+    l.residual = terms_bytes(nq, n_terms);
+    l.strings = strings_bytes(nq, n);
+    l.total = l.terms + l.targets + l.singles + l.residual + 2 * l.strings + 8 * 256 + 1024;
+    return l;
+}
+
+// plan[0] valid terms, [1] diagonal terms, [2] terms left to enumeration, [3] repeated rows,
+// [4] coefficients with a nonzero imaginary part (from plan_scan_kernel)
+__global__ void hop_plan_gather_kernel(const int64_t* __restrict__ term_counts, const int64_t* __restrict__ residual_counts,
+                                       const int64_t* __restrict__ duplicates, int64_t* __restrict__ plan) {
+    if (threadIdx.x == 0) {
+        plan[0] = term_counts[0];
+        plan[1] = term_counts[1];
+        plan[2] = residual_counts[0];
+        plan[3] = duplicates[0];
+    }
+}
+
+__global__ void hop_tests_kernel(const int64_t* __restrict__ counts0, const int64_t* __restrict__ counts1, int64_t* __restrict__ plan) {
+    if (threadIdx.x == 0) {
+        plan[5] = counts0[2] + counts1[2];
+    }
+}
+
+// 0: applied through the hop path; 1: not applicable (the caller enumerates); else an error.
+int apply_hop_path(int nq, int pc, const uint8_t* configs, int64_t n, const int16_t* site, const uint8_t* kind, const double* coef,
+                   int64_t n_terms, const double* in, double* out, int mode, void* workspace, size_t workspace_bytes, hipStream_t s) {
+    if (nq > kPairMaxQubytes || n <= 0 || n_terms <= 0 || n >= (int64_t{1} << 31)) {
+        return 1;
+    }
+    const HopLayout lo = hop_layout(nq, n, n_terms);
+    if (workspace_bytes < lo.total) {
+        return 1;
+    }
+    Regions r{static_cast<char*>(workspace), 0, workspace_bytes};
+    qmb_terms terms{};
+    QMB_TRY(terms_build(nq, pc, site, kind, coef, n_terms, r.next(lo.terms), lo.terms, &terms, s));
+    qmb_targets targets{};
+    QMB_TRY(targets_build(nq, configs, n, r.next(lo.targets), lo.targets, &targets, s));
+    qmb_singles singles{};
+    QMB_TRY(singles_build(nq, pc, site, kind, coef, n_terms, r.next(lo.singles), lo.singles, &singles, s));
+    qmb_terms residual{};
+    QMB_TRY(terms_build(nq, pc, site, kind, coef, n_terms, r.next(lo.residual), lo.residual, &residual, s,
+                        kExcludeDiagonal | kExcludeHops));
+    int64_t* plan = static_cast<int64_t*>(r.next(8 * sizeof(int64_t)));
+    plan_zero_kernel<<<1, 64, 0, s>>>(plan);
+    QMB_CHECK_LAUNCH();
+    // plan[3] and plan[4] of plan_scan (most occupied sites, imaginary coefficients); [3] is then
+    // overwritten with the repeated rows
+    const int64_t reach = std::max(n, n_terms);
+    plan_scan_kernel<<<static_cast<unsigned>((reach + 255) / 256), 256, 0, s>>>(configs, n, nq, coef, n_terms, plan);
+    QMB_CHECK_LAUNCH();
+    hop_plan_gather_kernel<<<1, 64, 0, s>>>(terms.counts, residual.counts, targets.duplicates, plan);
+    QMB_CHECK_LAUNCH();
+    int64_t host[8] = {};
+    QMB_HIP_TRY(hipMemcpyAsync(host, plan, 5 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    QMB_HIP_TRY(hipStreamSynchronize(s));
+    const int64_t n_hops = host[0] - host[1] - host[2];
+    if (n_hops <= 0 || host[3] != 0) {
+        return 1;
+    }
+    if (host[4] == 0) {
+        QMB_TRY(singles_set_real(&singles, 1, s));
+    }
+    qmb_strings strings[2]{};
+    for (int parity = 0; parity < 2; ++parity) {
+        QMB_TRY(strings_build(&targets, parity, r.next(lo.strings), lo.strings, &strings[parity], s));
+    }
+    hop_tests_kernel<<<1, 64, 0, s>>>(strings[0].counts, strings[1].counts, plan);
+    QMB_CHECK_LAUNCH();
+    QMB_HIP_TRY(hipMemcpyAsync(host + 5, plan + 5, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    QMB_HIP_TRY(hipStreamSynchronize(s));
+    if (host[5] > kHopTestsPerTermRow * n * n_hops) {
+        return 1;
+    }
+    QMB_TRY(launch_zero_f64(out, 2 * n, s));
+    QMB_TRY(apply_diagonal_terms(&terms, &targets, in, out, mode, s));
+    for (int parity = 0; parity < 2; ++parity) {
+        QMB_TRY(apply_hops(&singles, &strings[parity], &targets, in, out, mode, s));
+    }
+    if (host[2] > 0) {
+        QMB_TRY(apply_prepared(&residual, &targets, configs, n, in, out, mode, nullptr, s));
+    }
+    return 0;
+}
+
 }  // namespace
 }  // namespace qmb
 
@@ -798,6 +900,22 @@ int qmb_apply_within_structured_workspace_size(int n_qubytes, int64_t n, int64_t
     return QMB_OK;
 }
 
+int qmb_apply_within_hops_workspace_size(int n_qubytes, int64_t n, int64_t n_terms, size_t* bytes) {
+    clear_error();
+    QMB_TRY(check_qubytes(n_qubytes));
+    if (!bytes) {
+        return set_error(QMB_ERR_ARGUMENT, "bytes must be non-null");
+    }
+    if (n < 0 || n_terms < 0) {
+        return set_error(QMB_ERR_ARGUMENT, "n and n_terms must be non-negative");
+    }
+    if (n_qubytes > kPairMaxQubytes) {
+        return set_error(QMB_ERR_UNSUPPORTED, "the hop path covers configs of up to %d bytes", kPairMaxQubytes);
+    }
+    *bytes = std::max(hop_layout(n_qubytes, n, n_terms).total, terms_bytes(n_qubytes, n_terms) + targets_bytes(n_qubytes, n) + 512);
+    return QMB_OK;
+}
+
 static int apply_oneshot(int n_qubytes, int particle_cut, const uint8_t* configs_i, int64_t n_i, const uint8_t* configs_j,
                          int64_t n_j, const int16_t* site, const uint8_t* kind, const double* coef, int64_t n_terms,
                          const double* in, double* out, int mode, void* workspace, size_t workspace_bytes, void* stream) {
@@ -813,8 +931,13 @@ static int apply_oneshot(int n_qubytes, int particle_cut, const uint8_t* configs
     QMB_TRY(need_workspace(tb + gb + 512, workspace_bytes, workspace));
     hipStream_t s = as_stream(stream);
     if (configs_i == configs_j && n_i == n_j) {
-        const int rc = apply_structured(n_qubytes, particle_cut, configs_i, n_i, site, kind, coef, n_terms, in, out, mode, workspace,
-                                        workspace_bytes, s);
+        int rc = apply_structured(n_qubytes, particle_cut, configs_i, n_i, site, kind, coef, n_terms, in, out, mode, workspace,
+                                  workspace_bytes, s);
+        if (rc != 1) {
+            return rc;
+        }
+        rc = apply_hop_path(n_qubytes, particle_cut, configs_i, n_i, site, kind, coef, n_terms, in, out, mode, workspace,
+                            workspace_bytes, s);
         if (rc != 1) {
             return rc;
         }

@@ -202,6 +202,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "qmb_philox_uniform_host": (ctypes.c_double, [_c_u64, _c_u64]),
     "qmb_apply_within_workspace_size": (_c_int, [_c_int, _c_i64, _c_i64, _c_i64, ctypes.POINTER(_c_size)]),
     "qmb_apply_within_structured_workspace_size": (_c_int, [_c_int, _c_i64, _c_i64, _c_i32, ctypes.POINTER(_c_size)]),
+    "qmb_apply_within_hops_workspace_size": (_c_int, [_c_int, _c_i64, _c_i64, ctypes.POINTER(_c_size)]),
     "qmb_apply_within": (_c_int, [_c_int, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _vp, _c_size, _vp]),
     "qmb_find_relative_workspace_size": (_c_int, [_c_int, _c_i64, _c_i64, _c_i64, _c_i64, ctypes.POINTER(_c_size)]),
     "qmb_find_relative": (

@@ -14,6 +14,9 @@ one parity (qmb/hubbard.py's spinful hops) and diagonal terms.
   - A set whose rows all share one string falls back to the enumeration (the pair tests would be
     n^2).
   - The 64-site Hubbard chain at 4M configs: the hop path equals the enumeration on every row.
+  - The C ABI: qmb_apply_within / qmb_apply_within_adjoint on one config buffer with a workspace of
+    qmb_apply_within_hops_workspace_size take the hop path and equal the enumeration they run with a
+    workspace of qmb_apply_within_workspace_size.
 """
 
 from __future__ import annotations
@@ -297,3 +300,57 @@ def test_hop_entry_points_check_arguments(cuda_device: torch.device) -> None:
     assert rc == _lib.QMB_ERR_ARGUMENT
     size = ctypes.c_size_t(0)
     assert lib.qmb_strings_workspace_size(17, 100, ctypes.byref(size)) != 0
+
+
+def _c_abi_call(lib: object, nq: int, cfg_t: torch.Tensor, terms: tuple[torch.Tensor, ...], vec: np.ndarray, adjoint: bool,
+                ws_bytes: int, dev: torch.device) -> tuple[np.ndarray, float]:
+    import time  # pylint: disable=import-outside-toplevel
+
+    from qmb_hip import _lib  # pylint: disable=import-outside-toplevel
+
+    n = cfg_t.size(0)
+    v = torch.view_as_real(torch.from_numpy(vec).to(dev)).contiguous()
+    out = torch.empty((n, 2), dtype=torch.float64, device=dev)
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    cp = ctypes.c_void_p(cfg_t.data_ptr())
+    tp = [ctypes.c_void_p(t.data_ptr()) for t in terms]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    if adjoint:
+        rc = lib.qmb_apply_within_adjoint(nq, 1, cp, n, cp, ctypes.c_void_p(v.data_ptr()), n, *tp, terms[0].size(0),
+                                          ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ws.data_ptr()), ws_bytes, stream)
+    else:
+        rc = lib.qmb_apply_within(nq, 1, cp, ctypes.c_void_p(v.data_ptr()), n, cp, n, *tp, terms[0].size(0),
+                                  ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ws.data_ptr()), ws_bytes, stream)
+    _lib.check(rc, "qmb_apply_within" + ("_adjoint" if adjoint else ""))
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t0
+    return torch.view_as_complex(out).cpu().numpy(), seconds
+
+
+@pytest.mark.parametrize("sites,count", [(40, 20000), (64, 4_000_000)], ids=["chain-80so", "hubbard128"])
+def test_c_abi_takes_hop_path(cuda_device: torch.device, sites: int, count: int) -> None:
+    from qmb_hip import _lib  # pylint: disable=import-outside-toplevel
+
+    lib = _lib.library()
+    site, kind, coef = (x.contiguous().to(cuda_device) for x in sy.terms_from_dict(sy.hubbard_dict(sites, 1, 1.0, 4.0)))
+    nq = (2 * sites + 7) // 8
+    cfg = sy.hubbard_chain_configs(sites, count, 7990 + sites)
+    cfg_t = torch.from_numpy(cfg).to(cuda_device)
+    n = len(cfg)
+    hops = ctypes.c_size_t(0)
+    _lib.check(lib.qmb_apply_within_hops_workspace_size(nq, n, site.size(0), ctypes.byref(hops)), "hops size")
+    small = ctypes.c_size_t(0)
+    _lib.check(lib.qmb_apply_within_workspace_size(nq, n, n, site.size(0), ctypes.byref(small)), "enumeration size")
+    assert hops.value > small.value
+    modes = (False, True) if count < 100_000 else (False,)
+    for adjoint in modes:
+        vec = sy.random_psi(n, 7995 + int(adjoint))
+        _c_abi_call(lib, nq, cfg_t, (site, kind, coef), vec, adjoint, hops.value, cuda_device)  # first call: page in
+        got, t_hop = _c_abi_call(lib, nq, cfg_t, (site, kind, coef), vec, adjoint, hops.value, cuda_device)
+        want, t_enum = _c_abi_call(lib, nq, cfg_t, (site, kind, coef), vec, adjoint, small.value, cuda_device)
+        print(f"{sites} sites, {n} configs, adjoint={adjoint}: hop path {t_hop * 1e3:.2f} ms, enumeration {t_enum * 1e3:.2f} ms",
+              flush=True)
+        scale = float(np.abs(want).max())
+        assert float(np.abs(got - want).max()) <= 1e-12 * scale
