@@ -313,6 +313,9 @@ def run_batch(args: argparse.Namespace, dev: object) -> int:
     b = qb.Batch(instances, two_vectors=True)
     b.set_inputs(psis, grads)
     waves = b.tune()
+    # every (waves, split) the autotuner timed on this batch, for the record
+    tuning = [entry for entry in qb.TUNER.cache.values() if "split" in entry["params"]]
+    tried = sorted(tuning[-1]["tried"], key=lambda x: x[1])[:8] if tuning else []
     b1 = qb.Batch(instances)
     b1.set_inputs(psis)
     b1.waves = waves
@@ -414,7 +417,8 @@ def run_batch(args: argparse.Namespace, dev: object) -> int:
         "data": "synthetic (seeded)",
         "config": {"workload": "batch64", "instances": len(raw), "spin_orbitals": [w["n_so"] for w in raw],
                    "n_configs": [int(len(w["configs"])) for w in raw], "n_terms_total": int(sum(len(w["site"]) for w in raw)),
-                   "M_gen": m_gen, "M_hit": m_hit, "waves": waves},
+                   "M_gen": m_gen, "M_hit": m_hit, "waves": waves, "split": b.split,
+                   "tuning_fastest": [{"waves": p["waves"], "split": p["split"], "ms": ms} for p, ms in tried]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": None, "kernel": "batch_kernel<adjoint, one vector> (one direction, all instances)", "kernel_ms": fwd_only,
                      "algorithmic_bytes": model_bytes,

@@ -5,9 +5,14 @@ Each instance is a PreparedTerms (with the structured tables: pair and single ex
 diagonal records) and a PreparedTargets over its distinct config set.  The inputs and outputs of all
 instances are packed into two device buffers, so the plan (qmb_batch_plan) is built once and every
 launch (qmb_apply_batch) is a single persistent kernel over all instances' work; the whole step
-can be captured in a hipGraph (torch.cuda.CUDAGraph) and replayed.  The number of persistent
-waves is picked by the autotuner.  Instances whose tables keep terms outside the structured
-classes get those applied by one enumeration launch each after the batch.
+can be captured in a hipGraph (torch.cuda.CUDAGraph) and replayed.  Instances whose tables keep
+terms outside the structured classes get those applied by one enumeration launch each after the
+batch.
+
+The autotuner picks two launch parameters: the number of persistent waves, and a split -- the
+`split` instances with the most core-index entries run their own structured launches (the
+single-set kernels, which fill the GPU alone and keep more registers than the batch kernel), the
+rest share the persistent launch (one plan per split, built on first use).
 """
 
 from __future__ import annotations
@@ -23,6 +28,7 @@ from .autotune import TUNER
 from .ops import _ptr, _stream, _workspace
 
 WAVE_CANDIDATES = (1024, 2048, 4096, 8192, 16384, 32768)
+SPLIT_CANDIDATES = (0, 1, 2, 4, 8, 16)
 
 
 class Batch:
@@ -61,36 +67,84 @@ class Batch:
                 raise ValueError("a two-vector batch needs Hermitian instances (both vectors take the same map)")
             self.inp2 = torch.zeros_like(self.inp)
             self.out2 = torch.zeros_like(self.out)
-        n = len(self.instances)
-        self._items = (_lib.QmbBatchItem * n)()
-        for k, (terms, targets) in enumerate(self.instances):
-            lo = self.rows[k][0]
-            it = self._items[k]
-            it.terms = ctypes.pointer(terms.desc)
-            it.pairs = ctypes.pointer(terms.pairs_desc)
-            it.singles = ctypes.pointer(terms.singles_desc) if terms.singles_desc is not None else None
-            it.targets = ctypes.pointer(targets.keys_desc)
-            it.cores2 = ctypes.pointer(targets.cores(2))
-            it.cores1 = ctypes.pointer(targets.cores(1)) if terms.singles_desc is not None else None
-            it.in_ = self.inp.data_ptr() + 16 * lo
-            it.out = self.out.data_ptr() + 16 * lo
-            if two_vectors:
-                it.in2 = self.inp2.data_ptr() + 16 * lo
-                it.out2 = self.out2.data_ptr() + 16 * lo
-        lib = _lib.library()
-        size = ctypes.c_size_t(0)
-        _lib.check(lib.qmb_batch_workspace_size(n, ctypes.byref(size)), "qmb_batch_workspace_size")
-        self.buffer = _workspace(size.value, device)
-        self.plan = _lib.QmbBatch()
-        with torch.cuda.device(device):
-            _lib.check(lib.qmb_batch_plan(self._items, n, _ptr(self.buffer), self.buffer.numel(), ctypes.byref(self.plan),
-                                          _stream(device)), "qmb_batch_plan")
+        # instances by decreasing core-index entries (the work of their group walks)
+        work = [targets.cores(2).capacity + (targets.cores(1).capacity if terms.singles_desc is not None else 0)
+                for terms, targets in self.instances]
+        self.order = sorted(range(len(self.instances)), key=lambda k: (-work[k], k))
+        self._plans: dict[int, tuple[object, torch.Tensor, _lib.QmbBatch]] = {}
+        self.split = 0
+        self.plan = self._plan(0)
         self.waves = 0
         self.residual = [k for k, (t, _) in enumerate(self.instances) if t.n_residual_terms > 0]
         for k in self.residual:
             # the residual enumeration looks targets up: build their hash index now, outside any
             # graph capture of the launches
             _ = self.instances[k][1].desc
+
+    def _plan(self, split: int) -> _lib.QmbBatch:
+        """The persistent launch's plan over every instance but the `split` largest (built once)."""
+        if split not in self._plans:
+            members = sorted(self.order[split:])
+            n = len(members)
+            items = (_lib.QmbBatchItem * max(n, 1))()
+            for j, k in enumerate(members):
+                terms, targets = self.instances[k]
+                lo = self.rows[k][0]
+                it = items[j]
+                it.terms = ctypes.pointer(terms.desc)
+                it.pairs = ctypes.pointer(terms.pairs_desc)
+                it.singles = ctypes.pointer(terms.singles_desc) if terms.singles_desc is not None else None
+                it.targets = ctypes.pointer(targets.keys_desc)
+                it.cores2 = ctypes.pointer(targets.cores(2))
+                it.cores1 = ctypes.pointer(targets.cores(1)) if terms.singles_desc is not None else None
+                it.in_ = self.inp.data_ptr() + 16 * lo
+                it.out = self.out.data_ptr() + 16 * lo
+                if self.two_vectors:
+                    it.in2 = self.inp2.data_ptr() + 16 * lo
+                    it.out2 = self.out2.data_ptr() + 16 * lo
+            plan = _lib.QmbBatch()
+            buffer = torch.empty(0, dtype=torch.uint8, device=self.device)
+            if n > 0:
+                lib = _lib.library()
+                size = ctypes.c_size_t(0)
+                _lib.check(lib.qmb_batch_workspace_size(n, ctypes.byref(size)), "qmb_batch_workspace_size")
+                buffer = _workspace(size.value, self.device)
+                with torch.cuda.device(self.device):
+                    _lib.check(lib.qmb_batch_plan(items, n, _ptr(buffer), buffer.numel(), ctypes.byref(plan), _stream(self.device)),
+                               "qmb_batch_plan")
+            self._plans[split] = (items, buffer, plan)
+        return self._plans[split][2]
+
+    def _solo(self, mode: int, stream: ctypes.c_void_p) -> None:
+        """The `split` largest instances through the single-set structured kernels."""
+        lib = _lib.library()
+        for k in self.order[:self.split]:
+            terms, targets = self.instances[k]
+            lo = self.rows[k][0]
+            singles = ctypes.byref(terms.singles_desc) if terms.singles_desc is not None else None
+            cores1 = ctypes.byref(targets.cores(1)) if terms.singles_desc is not None else None
+            vin, vout = ctypes.c_void_p(self.inp.data_ptr() + 16 * lo), ctypes.c_void_p(self.out.data_ptr() + 16 * lo)
+            if self.two_vectors:
+                _lib.check(lib.qmb_apply_structured_dual(ctypes.byref(terms.pairs_desc), singles, ctypes.byref(targets.cores(2)), cores1,
+                                                         ctypes.byref(targets.keys_desc), vin, vout,
+                                                         ctypes.c_void_p(self.inp2.data_ptr() + 16 * lo),
+                                                         ctypes.c_void_p(self.out2.data_ptr() + 16 * lo), mode, stream),
+                           "qmb_apply_structured_dual")
+            else:
+                _lib.check(lib.qmb_apply_diagonal(ctypes.byref(terms.pairs_desc), ctypes.byref(targets.keys_desc), vin, vout, mode, stream),
+                           "qmb_apply_diagonal")
+                if terms.singles_desc is not None:
+                    _lib.check(lib.qmb_apply_singles(ctypes.byref(terms.singles_desc), cores1, ctypes.byref(targets.keys_desc), vin, vout,
+                                                     mode, stream), "qmb_apply_singles")
+                _lib.check(lib.qmb_apply_pairs(ctypes.byref(terms.pairs_desc), ctypes.byref(targets.cores(2)),
+                                               ctypes.byref(targets.keys_desc), vin, vout, mode, stream), "qmb_apply_pairs")
+
+    def configure(self, split: int) -> None:
+        """Runs the `split` largest instances by their own launches and the rest batched."""
+        if not 0 <= split <= len(self.instances):
+            raise ValueError(f"split must be in [0, {len(self.instances)}], got {split}")
+        self.plan = self._plan(split)
+        self.split = split
 
     # ------------------------------------------------------------------ inputs and outputs
     def set_inputs(self, psis: typing.Sequence[torch.Tensor], second: typing.Sequence[torch.Tensor] | None = None) -> None:
@@ -129,6 +183,7 @@ class Batch:
         stream = _stream(self.device)
         w = self.waves if waves is None else waves
         with torch.cuda.device(self.device):
+            self._solo(pp.HERMITIAN_MODE, stream)
             _lib.check(lib.qmb_apply_batch(ctypes.byref(self.plan), pp.HERMITIAN_MODE, w, stream), "qmb_apply_batch")
             for k in self.residual:
                 for inp, out in ((self.inp, self.out), (self.inp2, self.out2)):
@@ -154,15 +209,26 @@ class Batch:
         pmode = pp.HERMITIAN_MODE if self.hermitian else mode
         w = self.waves if waves is None else waves
         with torch.cuda.device(self.device):
+            self._solo(pmode, stream)
             _lib.check(lib.qmb_apply_batch(ctypes.byref(self.plan), pmode, w, stream), "qmb_apply_batch")
             for k in self.residual:
                 self._residual(k, mode, self.inp, self.out, stream)
 
     def tune(self, retune: bool = False) -> int:
-        """Picks the number of persistent waves with the autotuner (runs the batch on the current
-        inputs; overwrites the outputs)."""
-        key = f"batch:n={len(self.instances)}:rows={self.n_rows}:units={self.plan.total_units}:two={int(self.two_vectors)}"
-        run = (lambda c: self.launch_both(c["waves"])) if self.two_vectors else (lambda c: self.launch(pp.MODE_FORWARD, c["waves"]))
-        best = TUNER.best(key, [{"waves": w} for w in WAVE_CANDIDATES], run, retune=retune)
+        """Picks the number of persistent waves and the split with the autotuner (runs the step on
+        the current inputs; overwrites the outputs).  Returns the waves."""
+        key = (f"batch:n={len(self.instances)}:rows={self.n_rows}:units={self._plan(0).total_units}:two={int(self.two_vectors)}"
+               ":split")
+        splits = [k for k in SPLIT_CANDIDATES if k < len(self.instances)]
+
+        def run(c: dict[str, int]) -> None:
+            self.configure(int(c["split"]))
+            if self.two_vectors:
+                self.launch_both(int(c["waves"]))
+            else:
+                self.launch(pp.MODE_FORWARD, int(c["waves"]))
+
+        best = TUNER.best(key, [{"waves": w, "split": k} for k in splits for w in WAVE_CANDIDATES], run, retune=retune)
+        self.configure(int(best["split"]))
         self.waves = int(best["waves"])
         return self.waves

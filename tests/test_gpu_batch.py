@@ -212,9 +212,37 @@ def test_batch_autotune_keeps_result(cuda_device: torch.device) -> None:
     b.launch(pp.MODE_FORWARD, waves=qb.WAVE_CANDIDATES[0])
     first = b.out.clone()
     waves = b.tune(retune=True)
-    assert waves in qb.WAVE_CANDIDATES
+    assert waves in qb.WAVE_CANDIDATES and b.split in qb.SPLIT_CANDIDATES
     b.launch(pp.MODE_FORWARD)
     assert (b.out - first).abs().max().item() <= 1e-12 * first.abs().max().item()
+
+
+@pytest.mark.parametrize("two", [False, True], ids=["one-vector", "two-vector"])
+def test_batch_split_matches_full_batch(cuda_device: torch.device, two: bool) -> None:
+    """Every split of a batch (the largest instances by their own structured launches, the rest
+    batched) gives each instance's own application, both vectors of a two-vector batch and both
+    modes of a one-vector one; the instances are ordered by core-index entries, largest first."""
+    specs = [(6, 3, 3, 300, 51), (8, 3, 3, 1500, 52), (4, 2, 2, 36, 53), (8, 4, 3, 900, 54)]
+    instances = [_molecular(*spec, cuda_device) for spec in specs]
+    b = qb.Batch(instances, two_vectors=two)
+    work = [t.cores(2).capacity + t.cores(1).capacity for _, t in instances]
+    assert [work[k] for k in b.order] == sorted(work, reverse=True)
+    psis = [torch.from_numpy(sy.random_psi(t.n, 700 + k)).to(cuda_device) for k, (_, t) in enumerate(instances)]
+    grads = [torch.from_numpy(sy.random_psi(t.n, 800 + k)).to(cuda_device) for k, (_, t) in enumerate(instances)]
+    b.set_inputs(psis, grads if two else None)
+    for split in (0, 1, 3, 4):
+        b.configure(split)
+        modes = (pp.MODE_FORWARD,) if two else (pp.MODE_FORWARD, pp.MODE_ADJOINT)
+        for mode in modes:
+            if two:
+                b.launch_both(4096)
+            else:
+                b.launch(mode, 4096)
+            for k, (terms, targets) in enumerate(instances):
+                for second, vec in (((False, psis[k]), (True, grads[k])) if two else ((False, psis[k]),)):
+                    ref = _reference(terms, targets, vec, mode)
+                    got = b.output(k, second).cpu().numpy()
+                    np.testing.assert_array_less(np.abs(got - ref), 1e-12 * _scale(ref, 10.0, vec) + 1e-300)
 
 
 def test_per_instance_two_vector_matches_two_applications(cuda_device: torch.device) -> None:
