@@ -216,9 +216,9 @@ __device__ __forceinline__ uint32_t core_key(uint64_t h, uint32_t cls) {
 // among its row's pairs grows with its slot (ties where no core site lies between two slots), and
 // the stable sort leaves each group's entries close to ascending slots: the lanes of a wave then
 // read nearby columns of a coefficient table row.
-template <int W, int ORDER>
+template <int W, int ORDER, bool PACKED>
 __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64_t* __restrict__ cols, int64_t n, int n_bits,
-                                 int per_config, uint32_t* __restrict__ okeys, uint64_t* __restrict__ ovals,
+                                 int per_config, int sub_bits, uint32_t* __restrict__ okeys, void* __restrict__ ovals,
                                  int64_t* __restrict__ counts) {
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i >= n) {
@@ -233,10 +233,18 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
         even += __popcll(c.w[w] & kEven);
     }
     int64_t k = 0;
+    // the value: row << 16 | y << 8 | x, or packed into 32 bits as row << sub_bits | (pair slot of
+    // (x, y), or the site x); the packed value halves what the sort moves per value
     auto emit = [&](uint64_t key, uint64_t val) {
         if (k < per_config) {
             okeys[k * n + i] = static_cast<uint32_t>(key);
-            ovals[k * n + i] = (static_cast<uint64_t>(i) << 16) | val;
+            if (PACKED) {
+                const uint32_t sub = ORDER == 2 ? static_cast<uint32_t>(pair_slot(static_cast<int>(val & 0xff), static_cast<int>(val >> 8), n_bits))
+                                                : static_cast<uint32_t>(val);
+                static_cast<uint32_t*>(ovals)[k * n + i] = (static_cast<uint32_t>(i) << sub_bits) | sub;
+            } else {
+                static_cast<uint64_t*>(ovals)[k * n + i] = (static_cast<uint64_t>(i) << 16) | val;
+            }
         }
         ++k;
     };
@@ -305,8 +313,46 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
     }
     for (; k < per_config; ++k) {
         okeys[k * n + i] = kKeySentinel;
-        ovals[k * n + i] = 0;
+        if (PACKED) {
+            static_cast<uint32_t*>(ovals)[k * n + i] = 0u;
+        } else {
+            static_cast<uint64_t*>(ovals)[k * n + i] = 0ull;
+        }
     }
+}
+
+// inv[pair_slot(x, y)] = y << 8 | x for every pair x < y of n_bits sites.
+__global__ void pair_inverse_kernel(int n_bits, uint32_t* __restrict__ inv) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_bits * n_bits) {
+        return;
+    }
+    const int x = t % n_bits, y = t / n_bits;
+    if (x < y) {
+        inv[pair_slot(x, y, n_bits)] = (static_cast<uint32_t>(y) << 8) | static_cast<uint32_t>(x);
+    }
+}
+
+// core_flag_kernel for packed values: also writes each entry's value in the kernels' form
+// (row << 16 | y << 8 | x).
+template <int ORDER>
+__global__ void core_flag_unpack_kernel(const uint32_t* __restrict__ keys, int64_t cap, uint32_t* __restrict__ flag,
+                                        const uint32_t* __restrict__ packed, int sub_bits, const uint32_t* __restrict__ inv,
+                                        uint64_t* __restrict__ vals) {
+    const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (e >= cap) {
+        return;
+    }
+    const uint32_t k = keys[e];
+    const bool used = k != kKeySentinel;
+    flag[e] = (used && (e == 0 || keys[e - 1] != k)) ? 1u : 0u;
+    uint64_t v = 0ull;
+    if (used) {
+        const uint32_t pv = packed[e];
+        const uint32_t sub = pv & ((1u << sub_bits) - 1u);
+        v = (static_cast<uint64_t>(pv >> sub_bits) << 16) | (ORDER == 2 ? inv[sub] : sub);
+    }
+    vals[e] = v;
 }
 
 __global__ void core_flag_kernel(const uint32_t* __restrict__ keys, int64_t cap, uint32_t* __restrict__ flag) {
@@ -339,6 +385,20 @@ __global__ void core_start_kernel(const uint32_t* __restrict__ keys, const uint3
         counts[1] = groups;
     }
 }
+
+// Bits of the sub field of a packed core value (order 2: pair slots, order 1: sites) and of the
+// row; the 32-bit value applies when both fit.
+inline int bits_for(int64_t range) {
+    int b = 0;
+    while ((int64_t{1} << b) < range) {
+        ++b;
+    }
+    return b;
+}
+inline int core_sub_bits(int n_bits, int order) {
+    return bits_for(order == 2 ? static_cast<int64_t>(n_bits) * (n_bits - 1) / 2 : n_bits);
+}
+inline bool core_packed(int64_t n, int n_bits, int order) { return bits_for(n) + core_sub_bits(n_bits, order) <= 32; }
 
 int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, int order, qmb_cores* out, void** scratch,
                  size_t* scratch_bytes) {
@@ -376,6 +436,11 @@ int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, int ord
         if (rocprim::radix_sort_pairs(nullptr, tmp, u, u, v, v, static_cast<unsigned>(cap), 0, kKeyBits) != hipSuccess) {
             return set_error(QMB_ERR_HIP, "rocprim radix_sort_pairs size query failed");
         }
+        size_t packed = 0;
+        if (rocprim::radix_sort_pairs(nullptr, packed, u, u, u, u, static_cast<unsigned>(cap), 0, kKeyBits) != hipSuccess) {
+            return set_error(QMB_ERR_HIP, "rocprim radix_sort_pairs size query failed");
+        }
+        tmp = tmp > packed ? tmp : packed;
         size_t scan = 0;
         if (rocprim::exclusive_scan(nullptr, scan, u, u, 0u, cap, rocprim::plus<uint32_t>()) != hipSuccess) {
             return set_error(QMB_ERR_HIP, "rocprim exclusive_scan size query failed");
@@ -383,7 +448,8 @@ int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, int ord
         tmp = tmp > scan ? tmp : scan;
     }
     const size_t s0 = cv.off;
-    void* s = cv.take_bytes(cap * sizeof(uint64_t) + 3 * cap * sizeof(uint32_t) + tmp + 4096);
+    const size_t inv_bytes = static_cast<size_t>(8 * n_qubytes) * (8 * n_qubytes) / 2 * sizeof(uint32_t) + 256;
+    void* s = cv.take_bytes(cap * sizeof(uint64_t) + 3 * cap * sizeof(uint32_t) + inv_bytes + tmp + 4096);
     if (scratch) {
         *scratch = s;
     }
@@ -2139,28 +2205,64 @@ int cores_build(const qmb_targets* targets, int max_occupied, int order, void* b
         uint64_t* vals_in = sc.take<uint64_t>(cap);
         uint32_t* flag = sc.take<uint32_t>(cap);
         uint32_t* gid = sc.take<uint32_t>(cap);
+        const int n_bits = 8 * c.n_qubytes;
+        uint32_t* inv = sc.take<uint32_t>(static_cast<size_t>(n_bits) * n_bits / 2);
         size_t tmp_bytes = scratch_bytes > sc.off + 512 ? scratch_bytes - sc.off - 512 : 0;
         void* tmp = sc.take_bytes(tmp_bytes);
         const unsigned nb = blocks_for(c.n);
+        // packed values (row and pair slot in 32 bits) when they fit: the two u32 arrays share the
+        // u64 value buffer
+        const bool packed = core_packed(c.n, n_bits, order);
+        const int sub_bits = core_sub_bits(n_bits, order);
+        uint32_t* pin = reinterpret_cast<uint32_t*>(vals_in);
+        uint32_t* pout = pin + cap;
         auto emit = [&](auto kernel) {
-            kernel<<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, 8 * c.n_qubytes, c.per_config, keys_in, vals_in,
-                                              c.counts);
+            kernel<<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, n_bits, c.per_config, sub_bits, keys_in,
+                                              packed ? static_cast<void*>(pin) : static_cast<void*>(vals_in), c.counts);
         };
-        switch (c.n_words * 2 + (order - 1)) {
-            case 2: emit(&core_emit_kernel<1, 1>); break;
-            case 3: emit(&core_emit_kernel<1, 2>); break;
-            case 4: emit(&core_emit_kernel<2, 1>); break;
-            case 5: emit(&core_emit_kernel<2, 2>); break;
-            case 6: emit(&core_emit_kernel<3, 1>); break;
-            case 7: emit(&core_emit_kernel<3, 2>); break;
-            case 8: emit(&core_emit_kernel<4, 1>); break;
-            default: emit(&core_emit_kernel<4, 2>); break;
+        const int shape = c.n_words * 2 + (order - 1);
+        if (packed) {
+            switch (shape) {
+                case 2: emit(&core_emit_kernel<1, 1, true>); break;
+                case 3: emit(&core_emit_kernel<1, 2, true>); break;
+                case 4: emit(&core_emit_kernel<2, 1, true>); break;
+                case 5: emit(&core_emit_kernel<2, 2, true>); break;
+                case 6: emit(&core_emit_kernel<3, 1, true>); break;
+                case 7: emit(&core_emit_kernel<3, 2, true>); break;
+                case 8: emit(&core_emit_kernel<4, 1, true>); break;
+                default: emit(&core_emit_kernel<4, 2, true>); break;
+            }
+        } else {
+            switch (shape) {
+                case 2: emit(&core_emit_kernel<1, 1, false>); break;
+                case 3: emit(&core_emit_kernel<1, 2, false>); break;
+                case 4: emit(&core_emit_kernel<2, 1, false>); break;
+                case 5: emit(&core_emit_kernel<2, 2, false>); break;
+                case 6: emit(&core_emit_kernel<3, 1, false>); break;
+                case 7: emit(&core_emit_kernel<3, 2, false>); break;
+                case 8: emit(&core_emit_kernel<4, 1, false>); break;
+                default: emit(&core_emit_kernel<4, 2, false>); break;
+            }
         }
         QMB_CHECK_LAUNCH();
         size_t need = tmp_bytes;
-        QMB_HIP_TRY(rocprim::radix_sort_pairs(tmp, need, keys_in, c.keys, vals_in, c.vals, static_cast<unsigned>(cap), 0, kKeyBits,
-                                              stream));
-        core_flag_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, c.capacity, flag);
+        if (packed) {
+            QMB_HIP_TRY(rocprim::radix_sort_pairs(tmp, need, keys_in, c.keys, pin, pout, static_cast<unsigned>(cap), 0, kKeyBits,
+                                                  stream));
+            if (order == 2) {
+                pair_inverse_kernel<<<blocks_for(static_cast<int64_t>(n_bits) * n_bits), kBlock, 0, stream>>>(n_bits, inv);
+                QMB_CHECK_LAUNCH();
+                core_flag_unpack_kernel<2><<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, c.capacity, flag, pout, sub_bits, inv,
+                                                                                          c.vals);
+            } else {
+                core_flag_unpack_kernel<1><<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, c.capacity, flag, pout, sub_bits, inv,
+                                                                                          c.vals);
+            }
+        } else {
+            QMB_HIP_TRY(rocprim::radix_sort_pairs(tmp, need, keys_in, c.keys, vals_in, c.vals, static_cast<unsigned>(cap), 0, kKeyBits,
+                                                  stream));
+            core_flag_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, c.capacity, flag);
+        }
         QMB_CHECK_LAUNCH();
         need = tmp_bytes;
         QMB_HIP_TRY(rocprim::exclusive_scan(tmp, need, flag, gid, 0u, cap, rocprim::plus<uint32_t>(), stream));
