@@ -35,6 +35,11 @@ constexpr int kBlock = 256;
 constexpr uint32_t kHashRange = 0x55555555u;  // 3 * kHashRange = 2^32 - 1
 constexpr uint32_t kKeySentinel = 0xFFFFFFFFu;
 constexpr int kKeyBits = 32;
+// A packed core entry is one u64: key << kPackedBits | row << sub_bits | sub, sorted on bits
+// [kPackedBits, kPackedBits + 32) alone.  The range ends at bit 63, not 64: rocPRIM's radix sort
+// takes a merge path for small inputs whose comparison mask is (1 << end_bit) - 1 in the key type,
+// which a 64-bit end_bit overflows (it then compared the low bits).
+constexpr int kPackedBits = 31;
 constexpr int kMaxPairBytes = 16;            // the dense table is n_pairs^2 entries
 
 inline unsigned blocks_for(int64_t n, int block = kBlock) {
@@ -237,12 +242,12 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
     // (x, y), or the site x); the packed value halves what the sort moves per value
     auto emit = [&](uint64_t key, uint64_t val) {
         if (k < per_config) {
-            okeys[k * n + i] = static_cast<uint32_t>(key);
             if (PACKED) {
                 const uint32_t sub = ORDER == 2 ? static_cast<uint32_t>(pair_slot(static_cast<int>(val & 0xff), static_cast<int>(val >> 8), n_bits))
                                                 : static_cast<uint32_t>(val);
-                static_cast<uint32_t*>(ovals)[k * n + i] = (static_cast<uint32_t>(i) << sub_bits) | sub;
+                static_cast<uint64_t*>(ovals)[k * n + i] = (key << kPackedBits) | ((static_cast<uint32_t>(i) << sub_bits) | sub);
             } else {
+                okeys[k * n + i] = static_cast<uint32_t>(key);
                 static_cast<uint64_t*>(ovals)[k * n + i] = (static_cast<uint64_t>(i) << 16) | val;
             }
         }
@@ -312,10 +317,10 @@ __global__ void core_emit_kernel(const uint64_t* __restrict__ keys, const uint64
         atomicAdd(reinterpret_cast<unsigned long long*>(&counts[2]), 1ull);
     }
     for (; k < per_config; ++k) {
-        okeys[k * n + i] = kKeySentinel;
         if (PACKED) {
-            static_cast<uint32_t*>(ovals)[k * n + i] = 0u;
+            static_cast<uint64_t*>(ovals)[k * n + i] = static_cast<uint64_t>(kKeySentinel) << kPackedBits;
         } else {
+            okeys[k * n + i] = kKeySentinel;
             static_cast<uint64_t*>(ovals)[k * n + i] = 0ull;
         }
     }
@@ -333,25 +338,26 @@ __global__ void pair_inverse_kernel(int n_bits, uint32_t* __restrict__ inv) {
     }
 }
 
-// core_flag_kernel for packed values: also writes each entry's value in the kernels' form
-// (row << 16 | y << 8 | x).
+// core_flag_kernel for packed entries (key << 31 | row << sub_bits | sub, sorted on the key): also
+// splits each entry into its key and its value in the kernels' form (row << 16 | y << 8 | x).
 template <int ORDER>
-__global__ void core_flag_unpack_kernel(const uint32_t* __restrict__ keys, int64_t cap, uint32_t* __restrict__ flag,
-                                        const uint32_t* __restrict__ packed, int sub_bits, const uint32_t* __restrict__ inv,
-                                        uint64_t* __restrict__ vals) {
+__global__ void core_flag_unpack_kernel(const uint64_t* __restrict__ entries, int64_t cap, uint32_t* __restrict__ flag, int sub_bits,
+                                        const uint32_t* __restrict__ inv, uint32_t* __restrict__ keys, uint64_t* __restrict__ vals) {
     const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (e >= cap) {
         return;
     }
-    const uint32_t k = keys[e];
+    const uint64_t x = entries[e];
+    const uint32_t k = static_cast<uint32_t>(x >> kPackedBits);
     const bool used = k != kKeySentinel;
-    flag[e] = (used && (e == 0 || keys[e - 1] != k)) ? 1u : 0u;
+    flag[e] = (used && (e == 0 || static_cast<uint32_t>(entries[e - 1] >> kPackedBits) != k)) ? 1u : 0u;
     uint64_t v = 0ull;
     if (used) {
-        const uint32_t pv = packed[e];
+        const uint32_t pv = static_cast<uint32_t>(x & ((uint64_t{1} << kPackedBits) - 1u));
         const uint32_t sub = pv & ((1u << sub_bits) - 1u);
         v = (static_cast<uint64_t>(pv >> sub_bits) << 16) | (ORDER == 2 ? inv[sub] : sub);
     }
+    keys[e] = k;
     vals[e] = v;
 }
 
@@ -398,7 +404,7 @@ inline int bits_for(int64_t range) {
 inline int core_sub_bits(int n_bits, int order) {
     return bits_for(order == 2 ? static_cast<int64_t>(n_bits) * (n_bits - 1) / 2 : n_bits);
 }
-inline bool core_packed(int64_t n, int n_bits, int order) { return bits_for(n) + core_sub_bits(n_bits, order) <= 32; }
+inline bool core_packed(int64_t n, int n_bits, int order) { return bits_for(n) + core_sub_bits(n_bits, order) <= kPackedBits; }
 
 int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, int order, qmb_cores* out, void** scratch,
                  size_t* scratch_bytes) {
@@ -437,8 +443,9 @@ int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, int ord
             return set_error(QMB_ERR_HIP, "rocprim radix_sort_pairs size query failed");
         }
         size_t packed = 0;
-        if (rocprim::radix_sort_pairs(nullptr, packed, u, u, u, u, static_cast<unsigned>(cap), 0, kKeyBits) != hipSuccess) {
-            return set_error(QMB_ERR_HIP, "rocprim radix_sort_pairs size query failed");
+        if (rocprim::radix_sort_keys(nullptr, packed, v, v, static_cast<unsigned>(cap), kPackedBits, kPackedBits + kKeyBits) !=
+            hipSuccess) {
+            return set_error(QMB_ERR_HIP, "rocprim radix_sort_keys size query failed");
         }
         tmp = tmp > packed ? tmp : packed;
         size_t scan = 0;
@@ -449,7 +456,7 @@ int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, int ord
     }
     const size_t s0 = cv.off;
     const size_t inv_bytes = static_cast<size_t>(8 * n_qubytes) * (8 * n_qubytes) / 2 * sizeof(uint32_t) + 256;
-    void* s = cv.take_bytes(cap * sizeof(uint64_t) + 3 * cap * sizeof(uint32_t) + inv_bytes + tmp + 4096);
+    void* s = cv.take_bytes(2 * cap * sizeof(uint64_t) + 2 * cap * sizeof(uint32_t) + inv_bytes + tmp + 4096);
     if (scratch) {
         *scratch = s;
     }
@@ -2201,7 +2208,9 @@ int cores_build(const qmb_targets* targets, int max_occupied, int order, void* b
     if (c.capacity > 0) {
         const size_t cap = static_cast<size_t>(c.capacity);
         Carver sc{static_cast<char*>(scratch), 0};
-        uint32_t* keys_in = sc.take<uint32_t>(cap);
+        // a u64 per entry: the emitted keys (u32) or, packed, the emitted entries
+        uint64_t* entries_in = sc.take<uint64_t>(cap);
+        uint32_t* keys_in = reinterpret_cast<uint32_t*>(entries_in);
         uint64_t* vals_in = sc.take<uint64_t>(cap);
         uint32_t* flag = sc.take<uint32_t>(cap);
         uint32_t* gid = sc.take<uint32_t>(cap);
@@ -2210,15 +2219,13 @@ int cores_build(const qmb_targets* targets, int max_occupied, int order, void* b
         size_t tmp_bytes = scratch_bytes > sc.off + 512 ? scratch_bytes - sc.off - 512 : 0;
         void* tmp = sc.take_bytes(tmp_bytes);
         const unsigned nb = blocks_for(c.n);
-        // packed values (row and pair slot in 32 bits) when they fit: the two u32 arrays share the
-        // u64 value buffer
+        // packed entries (key << 31 | row and pair slot) when row and slot fit in 31 bits: a
+        // keys-only sort of one u64 per entry, from entries_in into vals_in
         const bool packed = core_packed(c.n, n_bits, order);
         const int sub_bits = core_sub_bits(n_bits, order);
-        uint32_t* pin = reinterpret_cast<uint32_t*>(vals_in);
-        uint32_t* pout = pin + cap;
         auto emit = [&](auto kernel) {
             kernel<<<nb, kBlock, 0, stream>>>(targets->keys, targets->cols, c.n, n_bits, c.per_config, sub_bits, keys_in,
-                                              packed ? static_cast<void*>(pin) : static_cast<void*>(vals_in), c.counts);
+                                              packed ? static_cast<void*>(entries_in) : static_cast<void*>(vals_in), c.counts);
         };
         const int shape = c.n_words * 2 + (order - 1);
         if (packed) {
@@ -2247,15 +2254,15 @@ int cores_build(const qmb_targets* targets, int max_occupied, int order, void* b
         QMB_CHECK_LAUNCH();
         size_t need = tmp_bytes;
         if (packed) {
-            QMB_HIP_TRY(rocprim::radix_sort_pairs(tmp, need, keys_in, c.keys, pin, pout, static_cast<unsigned>(cap), 0, kKeyBits,
-                                                  stream));
+            QMB_HIP_TRY(rocprim::radix_sort_keys(tmp, need, entries_in, vals_in, static_cast<unsigned>(cap), kPackedBits,
+                                                 kPackedBits + kKeyBits, stream));
             if (order == 2) {
                 pair_inverse_kernel<<<blocks_for(static_cast<int64_t>(n_bits) * n_bits), kBlock, 0, stream>>>(n_bits, inv);
                 QMB_CHECK_LAUNCH();
-                core_flag_unpack_kernel<2><<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, c.capacity, flag, pout, sub_bits, inv,
+                core_flag_unpack_kernel<2><<<blocks_for(c.capacity), kBlock, 0, stream>>>(vals_in, c.capacity, flag, sub_bits, inv, c.keys,
                                                                                           c.vals);
             } else {
-                core_flag_unpack_kernel<1><<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, c.capacity, flag, pout, sub_bits, inv,
+                core_flag_unpack_kernel<1><<<blocks_for(c.capacity), kBlock, 0, stream>>>(vals_in, c.capacity, flag, sub_bits, inv, c.keys,
                                                                                           c.vals);
             }
         } else {
