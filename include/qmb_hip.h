@@ -309,12 +309,14 @@ int qmb_terms_build_residual(int n_qubytes, int particle_cut, const int16_t* sit
 
 /* Core index of the set indexed by a qmb_targets: one entry per (config, `order` occupied sites)
  * -- order 2: every occupied pair (cores of pair excitations), order 1: every occupied site
- * (cores of single excitations) -- radix-sorted by a class of the core (order 2: even sites of
- * the core mod 3; order 1: 0), a 34-bit hash of the core and, within a group of equal keys, the
- * removed pair (site); and the start of every group.  Different cores with equal class and hash
- * share a group; the apply calls compare cores exactly there.  max_occupied must bound the number
- * of occupied sites of every config (a row with more sets counts[2], and the apply calls then
- * write NaN into out[0] instead of a wrong result). */
+ * (cores of single excitations) -- radix-sorted by a 32-bit key of the core (its class, order 2:
+ * even sites of the core mod 3, order 1: 0, and a hash of it), stable within a group of equal keys;
+ * and the start of every group.  Different cores with equal keys share a group: the build compares
+ * the cores of neighbouring entries exactly and flags such a mixed group in bit 31 of its start,
+ * and the apply calls compare cores per pair only there (elsewhere every row of a group has the
+ * first entry's core).  capacity must be below 2^31.  max_occupied must bound the number of
+ * occupied sites of every config (a row with more sets counts[2], and the apply calls then write
+ * NaN into out[0] instead of a wrong result). */
 typedef struct qmb_cores {
     int32_t n_qubytes, n_words, max_occupied, per_config; /* per_config = C(max_occupied, order) */
     int32_t order, key_bits;  /* key_bits: width of the sorted keys, 32 (set by qmb_cores_build) */
@@ -326,7 +328,7 @@ typedef struct qmb_cores {
                           64-bit core hash mod 0x55555555; a group is a run of equal keys; unused entries last
                           (0xFFFFFFFF) */
     uint64_t* vals;    /* [capacity] row << 16 | site1 << 8 | site0 of the removed sites (order 1: site0) */
-    uint32_t* starts;  /* [capacity + 1] first entry of each group, then the end */
+    uint32_t* starts;  /* [capacity + 1] first entry of each group (bit 31: the group is mixed), then the end */
     int64_t* counts;   /* device: {entries, groups, overflow, 0} */
 } qmb_cores;
 

@@ -11,6 +11,7 @@
 // reaches HBM with one FP64 atomic per group it belongs to.
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
 
 #include <vector>
 
@@ -208,6 +209,12 @@ __device__ __forceinline__ void flip_site(Cfg<W>& c, int s) {
     }
 }
 
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int lane) {
+    const uint32_t lo = __shfl(static_cast<uint32_t>(v), lane);
+    const uint32_t hi = __shfl(static_cast<uint32_t>(v >> 32), lane);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 __device__ __forceinline__ uint32_t core_key(uint64_t h, uint32_t cls) {
     return cls * kHashRange + static_cast<uint32_t>(h % kHashRange);
 }
@@ -338,57 +345,187 @@ __global__ void pair_inverse_kernel(int n_bits, uint32_t* __restrict__ inv) {
     }
 }
 
-// core_flag_kernel for packed entries (key << 31 | row << sub_bits | sub, sorted on the key): also
-// splits each entry into its key and its value in the kernels' form (row << 16 | y << 8 | x).
-template <int ORDER>
-__global__ void core_flag_unpack_kernel(const uint64_t* __restrict__ entries, int64_t cap, uint32_t* __restrict__ flag, int sub_bits,
-                                        const uint32_t* __restrict__ inv, uint32_t* __restrict__ keys, uint64_t* __restrict__ vals) {
-    const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (e >= cap) {
-        return;
+// Flags of a sorted core entry: the first entry of its key (a group head), an entry whose core differs
+// from the previous entry's under the same key (two cores merged by equal keys), an unused slot.
+constexpr uint32_t kFlagHead = 1u;
+constexpr uint32_t kFlagBreak = 2u;
+constexpr uint32_t kFlagUnused = 4u;
+// Bit 31 of starts[g] (g < G): group g holds more than one core (its rows' cores are compared exactly).
+constexpr uint32_t kGroupMixed = 0x80000000u;
+constexpr uint32_t kGroupStart = 0x7FFFFFFFu;
+
+// The core of an entry (row << 16 | y << 8 | x; order 1: row << 16 | x): its row's config without
+// the entry's sites.
+template <int W, int ORDER>
+__device__ __forceinline__ Cfg<W> entry_core(const uint64_t* __restrict__ ckeys, uint64_t v) {
+    Cfg<W> c = load_key<W>(ckeys, static_cast<int64_t>(v >> 16));
+    flip_site<W>(c, static_cast<int>(v & 0xff));
+    if (ORDER == 2) {
+        flip_site<W>(c, static_cast<int>((v >> 8) & 0xff));
     }
-    const uint64_t x = entries[e];
-    const uint32_t k = static_cast<uint32_t>(x >> kPackedBits);
-    const bool used = k != kKeySentinel;
-    flag[e] = (used && (e == 0 || static_cast<uint32_t>(entries[e - 1] >> kPackedBits) != k)) ? 1u : 0u;
-    uint64_t v = 0ull;
-    if (used) {
-        const uint32_t pv = static_cast<uint32_t>(x & ((uint64_t{1} << kPackedBits) - 1u));
-        const uint32_t sub = pv & ((1u << sub_bits) - 1u);
-        v = (static_cast<uint64_t>(pv >> sub_bits) << 16) | (ORDER == 2 ? inv[sub] : sub);
-    }
-    keys[e] = k;
-    vals[e] = v;
+    return c;
 }
 
-__global__ void core_flag_kernel(const uint32_t* __restrict__ keys, int64_t cap, uint32_t* __restrict__ flag) {
+template <int ORDER>
+__device__ __forceinline__ uint64_t unpack_entry(uint64_t x, int sub_bits, const uint32_t* __restrict__ inv) {
+    const uint32_t pv = static_cast<uint32_t>(x & ((uint64_t{1} << kPackedBits) - 1u));
+    const uint32_t sub = pv & ((1u << sub_bits) - 1u);
+    return (static_cast<uint64_t>(pv >> sub_bits) << 16) | (ORDER == 2 ? inv[sub] : sub);
+}
+
+// Flag of sorted entry e (key k, value v; kp, kn the neighbours' keys, vp the previous value).  Only
+// entries of groups of two or more read their config: each compares its core with the previous
+// entry's, taken from the lane below (lane 0 reads it itself).  Every lane runs the shuffles.
+template <int W, int ORDER>
+__device__ __forceinline__ uint32_t entry_flag(const uint64_t* __restrict__ ckeys, int lane, uint32_t k, uint32_t kp, uint32_t kn,
+                                               uint64_t v, uint64_t vp) {
+    const bool used = k != kKeySentinel;
+    const bool head = used && kp != k;
+    const bool multi = used && (!head || kn == k);
+    Cfg<W> core{};
+    if (multi) {
+        core = entry_core<W, ORDER>(ckeys, v);
+    }
+    Cfg<W> prev;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        prev.w[w] = shfl_u64(core.w[w], (lane + 63) & 63);
+    }
+    if (lane == 0 && used && !head) {
+        prev = entry_core<W, ORDER>(ckeys, vp);
+    }
+    bool same = true;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        same = same && core.w[w] == prev.w[w];
+    }
+    return !used ? kFlagUnused : (head ? kFlagHead : (same ? 0u : kFlagBreak));
+}
+
+// Flags of packed entries (key << 31 | row << sub_bits | sub, sorted on the key); also splits each
+// entry into its key and its value in the kernels' form (row << 16 | y << 8 | x).  Launched with
+// whole waves (entries past cap count as unused).
+template <int W, int ORDER>
+__global__ __launch_bounds__(kBlock) void core_flag_unpack_kernel(const uint64_t* __restrict__ entries, int64_t cap,
+                                                                  uint32_t* __restrict__ flag, int sub_bits, const uint32_t* __restrict__ inv,
+                                                                  const uint64_t* __restrict__ ckeys, uint32_t* __restrict__ keys,
+                                                                  uint64_t* __restrict__ vals) {
+    const int lane = threadIdx.x & 63;
     const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (e >= cap) {
+    constexpr uint64_t kUnusedEntry = static_cast<uint64_t>(kKeySentinel) << kPackedBits;
+    const uint64_t x = e < cap ? entries[e] : kUnusedEntry;
+    const uint64_t xp = e > 0 && e <= cap ? entries[e - 1] : kUnusedEntry;
+    const uint64_t xn = e + 1 < cap ? entries[e + 1] : kUnusedEntry;
+    const uint32_t k = static_cast<uint32_t>(x >> kPackedBits);
+    const uint32_t kp = static_cast<uint32_t>(xp >> kPackedBits);
+    const uint32_t kn = static_cast<uint32_t>(xn >> kPackedBits);
+    const bool used = k != kKeySentinel;
+    const uint64_t v = used ? unpack_entry<ORDER>(x, sub_bits, inv) : 0ull;
+    // the previous entry is used whenever it shares this entry's key
+    const uint64_t vp = used && kp == k ? unpack_entry<ORDER>(xp, sub_bits, inv) : 0ull;
+    const uint32_t f = entry_flag<W, ORDER>(ckeys, lane, k, kp, kn, v, vp);
+    if (e < cap) {
+        flag[e] = f;
+        keys[e] = k;
+        vals[e] = v;
+    }
+}
+
+// Flags of sorted (key, value) pairs, as core_flag_unpack_kernel.
+template <int W, int ORDER>
+__global__ __launch_bounds__(kBlock) void core_flag_kernel(const uint32_t* __restrict__ keys, const uint64_t* __restrict__ vals, int64_t cap,
+                                                           const uint64_t* __restrict__ ckeys, uint32_t* __restrict__ flag) {
+    const int lane = threadIdx.x & 63;
+    const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    const uint32_t k = e < cap ? keys[e] : kKeySentinel;
+    const uint32_t kp = e > 0 && e <= cap ? keys[e - 1] : kKeySentinel;
+    const uint32_t kn = e + 1 < cap ? keys[e + 1] : kKeySentinel;
+    const bool used = k != kKeySentinel;
+    const uint64_t v = used ? vals[e] : 0ull;
+    const uint64_t vp = used && kp == k ? vals[e - 1] : 0ull;
+    const uint32_t f = entry_flag<W, ORDER>(ckeys, lane, k, kp, kn, v, vp);
+    if (e < cap) {
+        flag[e] = f;
+    }
+}
+
+// flag[e] = 1 where a sorted key differs from the previous one (the strings index).
+__global__ void key_flag_kernel(const uint32_t* __restrict__ keys, int64_t n, uint32_t* __restrict__ flag) {
+    const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (e >= n) {
         return;
     }
     const uint32_t k = keys[e];
     flag[e] = (k != kKeySentinel && (e == 0 || keys[e - 1] != k)) ? 1u : 0u;
 }
 
-// starts[g] = first entry of group g; starts[G] = number of used entries; counts {entries, G}.
-__global__ void core_start_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ flag,
-                                  const uint32_t* __restrict__ gid, int64_t cap, uint32_t* __restrict__ starts, int64_t* __restrict__ counts) {
-    const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (e >= cap) {
+struct HeadBit {
+    __host__ __device__ uint32_t operator()(uint32_t f) const { return f & kFlagHead; }
+};
+using HeadIter = rocprim::transform_iterator<const uint32_t*, HeadBit, uint32_t>;
+
+// starts[g] = first entry of group g, with kGroupMixed when a break lies inside the group;
+// starts[G] = number of used entries; counts {entries, G}.  gid = exclusive scan of the head bits.
+// One wave reads 64 consecutive flags: a head finds its group's end (the next head or unused slot)
+// and the breaks before it from the wave's ballots; the last head of the wave, whose group may run
+// past the wave, reads on 64 flags at a time with the whole wave.  Launched with whole waves.
+__global__ __launch_bounds__(kBlock) void core_start_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ gid, int64_t cap,
+                                                            uint32_t* __restrict__ starts, int64_t* __restrict__ counts) {
+    const int lane = threadIdx.x & 63;
+    const int64_t base = blockIdx.x * static_cast<int64_t>(blockDim.x) + (threadIdx.x & ~63);
+    const int64_t e = base + lane;
+    const uint32_t f = e < cap ? flag[e] : kFlagUnused;
+    const uint64_t stop = __ballot((f & (kFlagHead | kFlagUnused)) != 0u);
+    const uint64_t brk = __ballot((f & kFlagBreak) != 0u);
+    const uint64_t unused = __ballot((f & kFlagUnused) != 0u);
+    const bool head = (f & kFlagHead) != 0u;
+    const uint64_t after = lane == 63 ? 0ull : ~0ull << (lane + 1);  // lanes above this one
+    const uint64_t next = stop & after;
+    // the group of the wave's last head runs on past the wave when no stop follows it
+    const bool open = head && next == 0ull;
+    const uint64_t open_mask = __ballot(open);
+    bool tail_break = false;
+    int64_t tail_end = 0;
+    bool tail_unused = false;
+    if (open_mask != 0ull) {
+        for (int64_t j0 = base + 64;; j0 += 64) {
+            const int64_t j = j0 + lane;
+            const uint32_t fj = j < cap ? flag[j] : kFlagUnused;
+            const uint64_t st = __ballot((fj & (kFlagHead | kFlagUnused)) != 0u);
+            const uint64_t bk = __ballot((fj & kFlagBreak) != 0u);
+            if (st != 0ull) {
+                const int nx = __builtin_ctzll(st);
+                tail_break = tail_break || (bk & ((1ull << nx) - 1ull)) != 0ull;
+                tail_end = j0 + nx;
+                const uint32_t fe = __shfl(fj, nx);
+                tail_unused = (fe & kFlagUnused) != 0u;
+                break;
+            }
+            tail_break = tail_break || bk != 0ull;
+        }
+    }
+    if (!head) {
         return;
     }
-    const uint32_t k = keys[e];
-    if (k == kKeySentinel) {
-        return;
+    bool mixed;
+    int64_t end;
+    bool last;
+    if (open) {
+        mixed = (brk & after) != 0ull || tail_break;
+        end = tail_end;
+        last = tail_unused;
+    } else {
+        const int nx = __builtin_ctzll(next);
+        mixed = (brk & after & ((1ull << nx) - 1ull)) != 0ull;
+        end = base + nx;
+        last = ((unused >> nx) & 1ull) != 0ull;
     }
-    if (flag[e]) {
-        starts[gid[e]] = static_cast<uint32_t>(e);
-    }
-    if (e == cap - 1 || keys[e + 1] == kKeySentinel) {
-        const uint32_t groups = gid[e] + flag[e];
-        starts[groups] = static_cast<uint32_t>(e + 1);
-        counts[0] = e + 1;
-        counts[1] = groups;
+    const uint32_t g = gid[e];
+    starts[g] = static_cast<uint32_t>(e) | (mixed ? kGroupMixed : 0u);
+    if (last) {
+        starts[g + 1] = static_cast<uint32_t>(end);
+        counts[0] = end;
+        counts[1] = static_cast<int64_t>(g) + 1;
     }
 }
 
@@ -426,8 +563,8 @@ int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, int ord
     c.per_config = order == 2 ? max_occupied * (max_occupied - 1) / 2 : max_occupied;
     c.n = n;
     c.capacity = n * c.per_config;
-    if (c.capacity >= (1ll << 32) - 1) {
-        return set_error(QMB_ERR_UNSUPPORTED, "core index of %lld entries exceeds 2^32", static_cast<long long>(c.capacity));
+    if (c.capacity >= (1ll << 31)) {
+        return set_error(QMB_ERR_UNSUPPORTED, "core index of %lld entries exceeds 2^31 - 1", static_cast<long long>(c.capacity));
     }
     const size_t cap = static_cast<size_t>(c.capacity > 0 ? c.capacity : 1);
     c.buffer = cv.base;
@@ -449,7 +586,7 @@ int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, int ord
         }
         tmp = tmp > packed ? tmp : packed;
         size_t scan = 0;
-        if (rocprim::exclusive_scan(nullptr, scan, u, u, 0u, cap, rocprim::plus<uint32_t>()) != hipSuccess) {
+        if (rocprim::exclusive_scan(nullptr, scan, HeadIter(u, HeadBit{}), u, 0u, cap, rocprim::plus<uint32_t>()) != hipSuccess) {
             return set_error(QMB_ERR_HIP, "rocprim exclusive_scan size query failed");
         }
         tmp = tmp > scan ? tmp : scan;
@@ -545,12 +682,6 @@ struct PairStage {
 constexpr int kWavesPerBlock = kBlock / 64;
 constexpr size_t kPairStageBytes = kWavesPerBlock * 64 * sizeof(PairStage<2>);
 static_assert(sizeof(PairStage<1>) <= sizeof(PairStage<2>) && sizeof(PairStage<2>) % 16 == 0, "stage slot layout");
-
-__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int lane) {
-    const uint32_t lo = __shfl(static_cast<uint32_t>(v), lane);
-    const uint32_t hi = __shfl(static_cast<uint32_t>(v >> 32), lane);
-    return (static_cast<uint64_t>(hi) << 32) | lo;
-}
 
 __device__ __forceinline__ double2 shfl_d2(const double2& v, int lane) {
     return make_double2(__shfl(v.x, lane), __shfl(v.y, lane));
@@ -670,11 +801,9 @@ __device__ __forceinline__ bool pair_term(const PairParams& p, const PairRow<W>&
 }
 
 template <int W>
-__device__ __forceinline__ PairRow<W> pair_row(const PairParams& p, const Cfg<W>& ca, int r0, int r1, bool adj, bool fermi) {
+__device__ __forceinline__ PairRow<W> pair_row(const PairParams& p, const Cfg<W>& core, int r0, int r1, bool adj, bool fermi) {
     PairRow<W> ra;
-    ra.core = ca;
-    flip_site<W>(ra.core, r0);
-    flip_site<W>(ra.core, r1);
+    ra.core = core;
     ra.bits = site_bits<W>(r0, r1);
     ra.mr = range_mask<W>(r0, r1);
     const int64_t slot = pair_slot(r0, r1, p.n_bits);
@@ -703,41 +832,24 @@ __device__ __forceinline__ void pair_madd(const double2& x, const double2& g, do
 //   ADJ:  out[a] += sum_b conj(A_ba) in[b]
 //   !ADJ: out[a] += sum_b A_ab in[b]
 // !ADJ reads T[Pb][Pa]: one table row per entry, at the columns of the lanes' rows.
+// s_raw: the group's start with its kGroupMixed bit.  In a group of one core every row's core is
+// the first entry's, so only that entry's config is read; a mixed group (different cores with
+// equal keys) reads every row's own config and checks every pair exactly.
 template <int W, bool ADJ, int B, int NV, bool REAL>
-__device__ void pair_group(const PairParams& p, uint32_t s, uint32_t e, int lane, PairStage<W>* stage) {
+__device__ void pair_group(const PairParams& p, uint32_t s_raw, uint32_t e, int lane, PairStage<W>* stage) {
+    const uint32_t s = s_raw & kGroupStart;
+    const bool mixed = (s_raw & kGroupMixed) != 0u;
     const uint64_t v0 = p.vals[s];
-    Cfg<W> core0 = load_key<W>(p.ckeys, static_cast<int64_t>(v0 >> 16));
-    flip_site<W>(core0, static_cast<int>(v0 & 0xff));
-    flip_site<W>(core0, static_cast<int>((v0 >> 8) & 0xff));
+    const Cfg<W> core0 = entry_core<W, 2>(p.ckeys, v0);
     const bool fermi = p.particle_cut == 1;
     const int fmask = fermi ? 1 : 0;
-    // Different cores with equal hashes share a group: then every pair is checked exactly.  The test
-    // covers all entries of the group, not one 64-row block: a block whose rows all hold the first
-    // core still meets entries of the other cores in the other blocks.
-    bool mixed = false;
-    for (uint32_t m0 = s; m0 < e; m0 += 64) {
-        const uint32_t ml = m0 + lane;
-        bool diff = false;
-        if (ml < e) {
-            const uint64_t vm = p.vals[ml];
-            Cfg<W> cm = load_key<W>(p.ckeys, static_cast<int64_t>(vm >> 16));
-            flip_site<W>(cm, static_cast<int>(vm & 0xff));
-            flip_site<W>(cm, static_cast<int>((vm >> 8) & 0xff));
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                diff = diff || cm.w[w] != core0.w[w];
-            }
-        }
-        const bool any_diff = __ballot(diff) != 0ull;
-        mixed = mixed || any_diff;
-    }
     for (uint32_t a0 = s; a0 < e; a0 += 64) {
         const uint32_t a = a0 + lane;
         const bool act = a < e;
         const uint64_t va = act ? p.vals[a] : v0;
+        const PairRow<W> ra = pair_row<W>(p, mixed ? entry_core<W, 2>(p.ckeys, va) : core0, static_cast<int>(va & 0xff),
+                                          static_cast<int>((va >> 8) & 0xff), ADJ, fermi);
         const int64_t ia = static_cast<int64_t>(va >> 16);
-        const PairRow<W> ra = pair_row<W>(p, load_key<W>(p.ckeys, ia), static_cast<int>(va & 0xff), static_cast<int>((va >> 8) & 0xff),
-                                          ADJ, fermi);
         double acc_re[NV] = {}, acc_im[NV] = {};
         for (uint32_t b0 = s; b0 < e; b0 += 64) {
             const uint32_t bl = b0 + lane;
@@ -799,12 +911,15 @@ __device__ void pair_group_packed(const PairParams& p, uint64_t mask, uint32_t g
             mask &= mask - 1;
             const uint32_t sj = __builtin_amdgcn_readlane(gs_l, l);
             const uint32_t ej = __builtin_amdgcn_readlane(ge_l, l);
-            nmax = static_cast<int>(ej - sj) > nmax ? static_cast<int>(ej - sj) : nmax;
+            const int nj = static_cast<int>(ej - (sj & kGroupStart));
+            nmax = nj > nmax ? nj : nmax;
             if (gi == j) {
                 s = sj;
                 e = ej;
             }
         }
+        const bool mixed = (s & kGroupMixed) != 0u;
+        s &= kGroupStart;
         const int g = static_cast<int>(e - s);  // 0 for a slot without a group
         const bool act = ri < g;
         const uint32_t a = s + static_cast<uint32_t>(ri);
@@ -812,21 +927,23 @@ __device__ void pair_group_packed(const PairParams& p, uint64_t mask, uint32_t g
         const int64_t ia = static_cast<int64_t>(va >> 16);
         const int r0 = static_cast<int>(va & 0xff);
         const int r1 = static_cast<int>((va >> 8) & 0xff);
-        Cfg<W> ca;
+        // the group's first row reads its config for the group (a mixed group: every row its own)
+        Cfg<W> own;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            ca.w[w] = act ? p.ckeys[ia * W + w] : 0ull;
+            own.w[w] = act && (mixed || ri == 0) ? p.ckeys[ia * W + w] : 0ull;
         }
-        const PairRow<W> ra = pair_row<W>(p, ca, r0, r1, ADJ, fermi);
+        if (act) {
+            flip_site<W>(own, r0);
+            flip_site<W>(own, r1);
+        }
         // the group's first core, from the lane holding its first row (every lane runs every shuffle)
         Cfg<W> core0;
-        bool same = true;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            core0.w[w] = shfl_u64(ra.core.w[w], gi * H);
-            same = same & (core0.w[w] == ra.core.w[w]);
+            core0.w[w] = shfl_u64(own.w[w], gi * H);
         }
-        const bool mixed = __ballot(act && !same) != 0ull;
+        const PairRow<W> ra = pair_row<W>(p, mixed ? own : core0, r0, r1, ADJ, fermi);
         // this lane's entry (its own row), staged for the rows of its group
         stage[lane] = stage_entry<W, ADJ, NV>(p, act, va, core0, fermi);
         double acc_re[NV] = {}, acc_im[NV] = {};
@@ -890,6 +1007,8 @@ __device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_
                 e = ej;
             }
         }
+        const bool mixed = (s & kGroupMixed) != 0u;
+        s &= kGroupStart;
         const uint32_t g = e - s;
         const bool rowa = static_cast<uint32_t>(ai) < g;
         const bool rowb = static_cast<uint32_t>(bi) < g;
@@ -912,29 +1031,37 @@ __device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_
         for (int v = 0; v < NV; ++v) {
             gv[v] = (v == 0 ? p.in : p.in2)[ib];
         }
-        Cfg<W> ca;
+        // the rows of the group's first row read its config for the group (a mixed group: every row
+        // its own)
+        Cfg<W> own;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            ca.w[w] = rowa ? p.ckeys[ia * W + w] : 0ull;
+            own.w[w] = rowa && (mixed || ai == 0) ? p.ckeys[ia * W + w] : 0ull;
         }
-        Cfg<W> core = ca;
         if (rowa) {
-            flip_site<W>(core, r0);
-            flip_site<W>(core, r1);
+            flip_site<W>(own, r0);
+            flip_site<W>(own, r1);
         }
-        // the core of row b: computed by the lanes whose row is b
+        // the group's first core, and the core of row b (computed by the lanes whose row is b);
+        // every lane runs every shuffle (a short-circuit would leave lanes out)
         const int src = (blk << (2 * lg)) + (bi << lg);
+        Cfg<W> core;
         bool same = true;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            const bool eq = shfl_u64(core.w[w], src) == core.w[w];
-            same = same & eq;  // every lane runs every shuffle (a short-circuit would leave lanes out)
+            const uint64_t c0 = shfl_u64(own.w[w], blk << (2 * lg));
+            core.w[w] = mixed ? own.w[w] : c0;
+            const bool eq = shfl_u64(own.w[w], src) == own.w[w];
+            same = same & eq;
         }
-        const bool ok = pre && same;
+        const bool ok = pre && (same || !mixed);
         double acc_re[NV] = {}, acc_im[NV] = {};
         if (ok) {
             int sg;
             if (ADJ) {
+                Cfg<W> ca = core;
+                flip_site<W>(ca, r0);
+                flip_site<W>(ca, r1);
                 sg = fermi ? pair_parity<W>(ca, r0, r1, q0, q1) : 0;
             } else {
                 Cfg<W> cb = core;
@@ -992,12 +1119,12 @@ __device__ __forceinline__ void pair_chunk_groups(const PairParams& p, uint32_t 
 template <int W, bool ADJ, int B, int NV, bool PACK = true>
 __device__ __forceinline__ void pair_chunk(const PairParams& p, int64_t groups, int64_t g0, int lane, char* stage_bytes) {
     const int64_t gg = g0 + lane;
-    uint32_t gs = 0, ge = 0;
+    uint32_t gs = 0, ge = 0;  // gs keeps the group's kGroupMixed bit for the group functions
     if (gg < groups) {
         gs = p.starts[gg];
-        ge = p.starts[gg + 1];
+        ge = p.starts[gg + 1] & kGroupStart;
     }
-    const uint32_t sz = ge - gs;
+    const uint32_t sz = ge - (gs & kGroupStart);
     pair_tiny_class<W, ADJ, NV>(p, __ballot(sz > 4u && sz <= 8u), gs, ge, 8, lane);
     pair_tiny_class<W, ADJ, NV>(p, __ballot(sz > 2u && sz <= 4u), gs, ge, 4, lane);
     pair_tiny_class<W, ADJ, NV>(p, __ballot(sz == 2u), gs, ge, 2, lane);
@@ -1165,38 +1292,23 @@ __device__ __forceinline__ double2 single_element(const SingleParams& p, const C
 // (config = core + q), q != r:
 //   ADJ:  out[a] += conj(M_ba) in[b],  M_ba = (-1)^single_parity(c_a; r, q) (A[r][q] + sum_{k in core} K[r][q][k])
 //   !ADJ: out[a] += M_ab in[b],        M_ab = (-1)^single_parity(c_b; q, r) (A[q][r] + sum_{k in core} K[q][r][k])
+// s_raw: the group's start with its kGroupMixed bit (see pair_group).
 template <int W, bool ADJ, int NV, int CH>
-__device__ void single_group(const SingleParams& p, uint32_t s, uint32_t e, int lane) {
+__device__ void single_group(const SingleParams& p, uint32_t s_raw, uint32_t e, int lane) {
+    const uint32_t s = s_raw & kGroupStart;
+    const bool mixed = (s_raw & kGroupMixed) != 0u;
     const uint64_t v0 = p.vals[s];
-    Cfg<W> core0 = load_key<W>(p.ckeys, static_cast<int64_t>(v0 >> 16));
-    flip_site<W>(core0, static_cast<int>(v0 & 0xff));
+    const Cfg<W> core0 = entry_core<W, 1>(p.ckeys, v0);
     const bool fermi = p.particle_cut == 1;
-    // merged cores: the test covers every entry of the group (see pair_group)
-    bool mixed = false;
-    for (uint32_t m0 = s; m0 < e; m0 += 64) {
-        const uint32_t ml = m0 + lane;
-        bool diff = false;
-        if (ml < e) {
-            const uint64_t vm = p.vals[ml];
-            Cfg<W> cm = load_key<W>(p.ckeys, static_cast<int64_t>(vm >> 16));
-            flip_site<W>(cm, static_cast<int>(vm & 0xff));
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                diff = diff || cm.w[w] != core0.w[w];
-            }
-        }
-        const bool any_diff = __ballot(diff) != 0ull;
-        mixed = mixed || any_diff;
-    }
     for (uint32_t a0 = s; a0 < e; a0 += 64) {
         const uint32_t a = a0 + lane;
         const bool act = a < e;
         const uint64_t va = act ? p.vals[a] : v0;
         const int64_t ia = static_cast<int64_t>(va >> 16);
         const int r = static_cast<int>(va & 0xff);
-        const Cfg<W> ca = load_key<W>(p.ckeys, ia);
-        Cfg<W> core = ca;
-        flip_site<W>(core, r);
+        const Cfg<W> core = mixed ? entry_core<W, 1>(p.ckeys, va) : core0;
+        Cfg<W> ca = core;
+        flip_site<W>(ca, r);
         double acc_re[NV] = {}, acc_im[NV] = {};
         for (uint32_t b0 = s; b0 < e; b0 += 64) {
             const uint32_t bl = b0 + lane;
@@ -1276,33 +1388,37 @@ __device__ void single_group_packed(const SingleParams& p, uint64_t mask, uint32
             mask &= mask - 1;
             const uint32_t sj = __builtin_amdgcn_readlane(gs_l, l);
             const uint32_t ej = __builtin_amdgcn_readlane(ge_l, l);
-            nmax = static_cast<int>(ej - sj) > nmax ? static_cast<int>(ej - sj) : nmax;
+            const int nj = static_cast<int>(ej - (sj & kGroupStart));
+            nmax = nj > nmax ? nj : nmax;
             if (gi == j) {
                 s = sj;
                 e = ej;
             }
         }
+        const bool mixed = (s & kGroupMixed) != 0u;
+        s &= kGroupStart;
         const int g = static_cast<int>(e - s);  // 0 for a slot without a group
         const bool act = ri < g;
         const uint64_t va = act ? p.vals[s + static_cast<uint32_t>(ri)] : 0ull;
         const int64_t ia = static_cast<int64_t>(va >> 16);
         const int r = static_cast<int>(va & 0xff);
-        Cfg<W> ca;
+        // the group's first row reads its config for the group (a mixed group: every row its own)
+        Cfg<W> own;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            ca.w[w] = act ? p.ckeys[ia * W + w] : 0ull;
+            own.w[w] = act && (mixed || ri == 0) ? p.ckeys[ia * W + w] : 0ull;
         }
-        Cfg<W> core = ca;
         if (act) {
-            flip_site<W>(core, r);
+            flip_site<W>(own, r);
         }
-        bool same = true;
+        Cfg<W> core;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            const bool eq = shfl_u64(core.w[w], gi * H) == core.w[w];
-            same = same & eq;  // every lane runs every shuffle (a short-circuit would leave lanes out)
+            const uint64_t c0 = shfl_u64(own.w[w], gi * H);  // every lane runs every shuffle
+            core.w[w] = mixed ? own.w[w] : c0;
         }
-        const bool mixed = __ballot(act && !same) != 0ull;
+        Cfg<W> ca = core;
+        flip_site<W>(ca, r);
         const double2 gin = act ? p.in[ia] : make_double2(0.0, 0.0);
         const double2 gin2 = (NV == 2 && act) ? p.in2[ia] : make_double2(0.0, 0.0);
         double acc_re[NV] = {}, acc_im[NV] = {};
@@ -1383,6 +1499,8 @@ __device__ void single_tiny_class(const SingleParams& p, uint64_t mask, uint32_t
                 e = ej;
             }
         }
+        const bool mixed = (s & kGroupMixed) != 0u;
+        s &= kGroupStart;
         const uint32_t g = e - s;
         const bool rowa = static_cast<uint32_t>(ai) < g;
         const bool rowb = static_cast<uint32_t>(bi) < g;
@@ -1399,23 +1517,30 @@ __device__ void single_tiny_class(const SingleParams& p, uint64_t mask, uint32_t
         for (int v = 0; v < NV; ++v) {
             gv[v] = (v == 0 ? p.in : p.in2)[ib];
         }
-        Cfg<W> ca;
+        // the rows of the group's first row read its config for the group (a mixed group: every row
+        // its own)
+        Cfg<W> own;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            ca.w[w] = rowa ? p.ckeys[ia * W + w] : 0ull;
+            own.w[w] = rowa && (mixed || ai == 0) ? p.ckeys[ia * W + w] : 0ull;
         }
-        Cfg<W> core = ca;
         if (rowa) {
-            flip_site<W>(core, r);
+            flip_site<W>(own, r);
         }
+        // every lane runs every shuffle (a short-circuit would leave lanes out)
         const int src = (blk << (2 * lg)) + (bi << lg);
+        Cfg<W> core;
         bool same = true;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            const bool eq = shfl_u64(core.w[w], src) == core.w[w];
-            same = same & eq;  // every lane runs every shuffle (a short-circuit would leave lanes out)
+            const uint64_t c0 = shfl_u64(own.w[w], blk << (2 * lg));
+            core.w[w] = mixed ? own.w[w] : c0;
+            const bool eq = shfl_u64(own.w[w], src) == own.w[w];
+            same = same & eq;
         }
-        const bool ok = pre && same;
+        Cfg<W> ca = core;
+        flip_site<W>(ca, r);
+        const bool ok = pre && (same || !mixed);
         double acc_re[NV] = {}, acc_im[NV] = {};
         if (ok) {
             double2 x;
@@ -1460,12 +1585,12 @@ __device__ void single_tiny_class(const SingleParams& p, uint64_t mask, uint32_t
 template <int W, bool ADJ, int NV, int CH, bool PACK = true>
 __device__ __forceinline__ void single_chunk(const SingleParams& p, int64_t groups, int64_t g0, int lane) {
     const int64_t gg = g0 + lane;
-    uint32_t gs = 0, ge = 0;
+    uint32_t gs = 0, ge = 0;  // gs keeps the group's kGroupMixed bit for the group functions
     if (gg < groups) {
         gs = p.starts[gg];
-        ge = p.starts[gg + 1];
+        ge = p.starts[gg + 1] & kGroupStart;
     }
-    const uint32_t sz = ge - gs;
+    const uint32_t sz = ge - (gs & kGroupStart);
     single_tiny_class<W, ADJ, NV, CH>(p, __ballot(sz > 4u && sz <= 8u), gs, ge, 8, lane);
     single_tiny_class<W, ADJ, NV, CH>(p, __ballot(sz > 2u && sz <= 4u), gs, ge, 4, lane);
     single_tiny_class<W, ADJ, NV, CH>(p, __ballot(sz == 2u), gs, ge, 2, lane);
@@ -2259,21 +2384,43 @@ int cores_build(const qmb_targets* targets, int max_occupied, int order, void* b
             if (order == 2) {
                 pair_inverse_kernel<<<blocks_for(static_cast<int64_t>(n_bits) * n_bits), kBlock, 0, stream>>>(n_bits, inv);
                 QMB_CHECK_LAUNCH();
-                core_flag_unpack_kernel<2><<<blocks_for(c.capacity), kBlock, 0, stream>>>(vals_in, c.capacity, flag, sub_bits, inv, c.keys,
-                                                                                          c.vals);
-            } else {
-                core_flag_unpack_kernel<1><<<blocks_for(c.capacity), kBlock, 0, stream>>>(vals_in, c.capacity, flag, sub_bits, inv, c.keys,
-                                                                                          c.vals);
+            }
+            auto unpack = [&](auto kernel) {
+                kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(vals_in, c.capacity, flag, sub_bits, inv, targets->keys, c.keys,
+                                                                      c.vals);
+            };
+            switch (shape) {
+                case 2: unpack(&core_flag_unpack_kernel<1, 1>); break;
+                case 3: unpack(&core_flag_unpack_kernel<1, 2>); break;
+                case 4: unpack(&core_flag_unpack_kernel<2, 1>); break;
+                case 5: unpack(&core_flag_unpack_kernel<2, 2>); break;
+                case 6: unpack(&core_flag_unpack_kernel<3, 1>); break;
+                case 7: unpack(&core_flag_unpack_kernel<3, 2>); break;
+                case 8: unpack(&core_flag_unpack_kernel<4, 1>); break;
+                default: unpack(&core_flag_unpack_kernel<4, 2>); break;
             }
         } else {
             QMB_HIP_TRY(rocprim::radix_sort_pairs(tmp, need, keys_in, c.keys, vals_in, c.vals, static_cast<unsigned>(cap), 0, kKeyBits,
                                                   stream));
-            core_flag_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, c.capacity, flag);
+            auto flags = [&](auto kernel) {
+                kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, c.vals, c.capacity, targets->keys, flag);
+            };
+            switch (shape) {
+                case 2: flags(&core_flag_kernel<1, 1>); break;
+                case 3: flags(&core_flag_kernel<1, 2>); break;
+                case 4: flags(&core_flag_kernel<2, 1>); break;
+                case 5: flags(&core_flag_kernel<2, 2>); break;
+                case 6: flags(&core_flag_kernel<3, 1>); break;
+                case 7: flags(&core_flag_kernel<3, 2>); break;
+                case 8: flags(&core_flag_kernel<4, 1>); break;
+                default: flags(&core_flag_kernel<4, 2>); break;
+            }
         }
         QMB_CHECK_LAUNCH();
         need = tmp_bytes;
-        QMB_HIP_TRY(rocprim::exclusive_scan(tmp, need, flag, gid, 0u, cap, rocprim::plus<uint32_t>(), stream));
-        core_start_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, flag, gid, c.capacity, c.starts, c.counts);
+        QMB_HIP_TRY(rocprim::exclusive_scan(tmp, need, HeadIter(flag, HeadBit{}), gid, 0u, cap, rocprim::plus<uint32_t>(), stream));
+        // whole waves: the start kernel reads the flags of 64 entries per wave (unused past cap)
+        core_start_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(flag, gid, c.capacity, c.starts, c.counts);
         c.key_bits = kKeyBits;
         QMB_CHECK_LAUNCH();
     }
@@ -2531,7 +2678,7 @@ int strings_build(const qmb_targets* targets, int parity, void* buffer, size_t b
         QMB_CHECK_LAUNCH();
         size_t need = tmp_bytes;
         QMB_HIP_TRY(rocprim::radix_sort_pairs(tmp, need, keys_in, t.keys, rows_in, t.rows, static_cast<unsigned>(cap), 0, 32, stream));
-        core_flag_kernel<<<nb, kBlock, 0, stream>>>(t.keys, n, flag);
+        key_flag_kernel<<<nb, kBlock, 0, stream>>>(t.keys, n, flag);
         QMB_CHECK_LAUNCH();
         need = tmp_bytes;
         QMB_HIP_TRY(rocprim::inclusive_scan(tmp, need, flag, t.group, cap, rocprim::plus<uint32_t>(), stream));

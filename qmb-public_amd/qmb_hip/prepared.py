@@ -332,7 +332,7 @@ def core_stats(targets: PreparedTargets, order: int) -> dict[str, int]:
     counts = buf[int(cores.counts) - base:int(cores.counts) - base + 16].view(torch.int64).tolist()
     entries, groups = int(counts[0]), int(counts[1])
     off = int(cores.starts) - base
-    starts = buf[off:off + 4 * (groups + 1)].view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    starts = buf[off:off + 4 * (groups + 1)].view(torch.int32).to(torch.int64) & 0x7FFFFFFF  # bit 31: mixed group
     sizes = starts[1:] - starts[:-1]
     multi = sizes > 1
     return {"entries": entries, "groups": groups, "multi_entries": int(sizes[multi].sum().item()),

@@ -252,3 +252,89 @@ def test_too_low_max_occupied_raises(cuda_device: torch.device) -> None:
         pp.apply(terms, low, cfg, vec, pp.MODE_FORWARD)
     exact = pp.PreparedTargets(cfg, 2, distinct=True, max_occupied=6)
     assert torch.isfinite(pp.apply(terms, exact, cfg, vec, pp.MODE_FORWARD)).all()
+
+
+def _core_index(targets: pp.PreparedTargets, order: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """(vals, starts with their flag bit, counts) of a built core index, copied to the host."""
+    desc = targets.cores(order)
+    buf = targets._cores[order][0]  # pylint: disable=protected-access
+    base = buf.data_ptr()
+    counts = buf[int(desc.counts) - base:int(desc.counts) - base + 32].view(torch.int64).cpu().numpy()
+    entries, groups = int(counts[0]), int(counts[1])
+    vals = buf[int(desc.vals) - base:int(desc.vals) - base + 8 * entries].view(torch.int64).cpu().numpy().astype(np.uint64)
+    off = int(desc.starts) - base
+    starts = buf[off:off + 4 * (groups + 1)].view(torch.int32).cpu().numpy().astype(np.uint32)
+    return vals, starts, counts
+
+
+def _build_cores_with_narrow_hash(targets: pp.PreparedTargets, bits: int, seed: int) -> None:
+    """Builds both core indexes of the set from random `bits`-bit hash columns, so that many
+    different cores share a key, then puts the set's own columns back (its hash index, built later
+    on first use, keeps them)."""
+    off = int(targets.keys_desc.cols) - targets.buffer.data_ptr()
+    view = targets.buffer[off:off + 8 * 256].view(torch.int64)
+    own = view.clone()
+    rng = np.random.default_rng(seed)
+    view.copy_(torch.from_numpy(rng.integers(0, 1 << bits, size=256, dtype=np.int64)))
+    targets.cores(2)
+    targets.cores(1)
+    view.copy_(own)
+
+
+MERGE_CASES = [c for c in CASES if c[0] in ("molecular-20so", "random-two-words-fermi")]
+
+
+@pytest.mark.parametrize("bits", [3, 9, 12, 14])
+@pytest.mark.parametrize("name,pc,case_fn", MERGE_CASES, ids=[c[0] for c in MERGE_CASES])
+def test_merged_core_groups(cuda_device: torch.device, name: str, pc: int, case_fn: object, bits: int) -> None:
+    """Core keys from narrowed hash columns merge many different cores into one group (3 bits: a
+    few thousand entries a group; 14 bits: a few).  The build flags exactly the groups holding more
+    than one core, and the forward and adjoint through those mixed groups -- every group class:
+    2..8 entries, 9..32 packed, larger -- still equal the oracle and the enumeration."""
+    (s, k, c), cfg = case_fn()
+    nq = cfg.shape[1]
+    cfg_t = _dev(cfg, cuda_device)
+    terms = pp.PreparedTerms(_dev(s, cuda_device), _dev(k, cuda_device), _dev(c, cuda_device), nq, pc)
+    targets = pp.PreparedTargets(cfg_t, nq, distinct=True)
+    _build_cores_with_narrow_hash(targets, bits, 7 + bits)
+    bitsof = np.unpackbits(cfg, axis=1, bitorder="little")
+    sizes_seen = set()
+    n_mixed = 0
+    for order in (2, 1):
+        vals, starts, _ = _core_index(targets, order)
+        flag = starts[:-1] >> 31
+        st = (starts & 0x7FFFFFFF).astype(np.int64)
+        sizes = np.diff(st)
+        assert st[0] == 0 and st[-1] == len(vals) and (sizes > 0).all()
+        # the core of every entry: its row's sites without the removed ones
+        rows = (vals >> np.uint64(16)).astype(np.int64)
+        core = bitsof[rows].copy()
+        e = np.arange(len(vals))
+        core[e, (vals & np.uint64(0xFF)).astype(np.int64)] = 0
+        if order == 2:
+            core[e, ((vals >> np.uint64(8)) & np.uint64(0xFF)).astype(np.int64)] = 0
+        packed = np.packbits(core, axis=1, bitorder="little")
+        first = np.repeat(st[:-1], sizes)
+        differs = (packed != packed[first]).any(axis=1).astype(np.int64)
+        mixed = np.add.reduceat(differs, st[:-1]) > 0
+        np.testing.assert_array_equal(flag.astype(bool), mixed)
+        n_mixed += int(mixed.sum())
+        sizes_seen.update(np.unique(np.minimum(sizes[mixed], 33)).tolist())
+    if name.startswith("molecular"):
+        # ~40k distinct cores: merged at every width, down to groups of 2..4 entries at 14 bits
+        assert n_mixed > 0
+        if bits == 14:
+            assert {2, 3, 4}.issubset(sizes_seen), sizes_seen
+    elif bits <= 9:
+        assert n_mixed > 0  # a few hundred distinct cores merge only under narrow keys
+    psi = sy.random_psi(len(cfg), 61)
+    vec = torch.view_as_real(_dev(psi, cuda_device)).contiguous()
+    want, asum, _ = orc.apply_within(pc, cfg, psi, cfg, s, k, c)
+    got = pp.apply(terms, targets, cfg_t, vec, pp.MODE_FORWARD, symmetric=pp.SYMMETRIC_OFF, pairs=True).cpu().numpy()
+    err = np.abs(got[:, 0] + 1j * got[:, 1] - want)
+    assert np.all(err <= 1e-12 * asum + 1e-300), (name, bits, err.max())
+    ref = pp.apply(terms, pp.PreparedTargets(cfg_t, nq), cfg_t, vec, pp.MODE_ADJOINT, symmetric=False, pairs=False).cpu().numpy()
+    got_a = pp.apply(terms, targets, cfg_t, vec, pp.MODE_ADJOINT, symmetric=False, pairs=True).cpu().numpy()
+    scale = np.abs(ref).sum(axis=1) + np.abs(c).sum(axis=1).max() * np.abs(psi).sum()
+    assert np.all(np.abs(got_a - ref).max(axis=1) <= 1e-12 * scale), (name, bits)
+
