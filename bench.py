@@ -122,31 +122,38 @@ def read_traffic(workload: str, kernel: str) -> tuple[float | None, float | None
 def structured_model(pp: object, terms: object, targets: object, nq: int, n: int) -> dict[str, object]:
     """Bytes the structured application must move at least once (synchronises to size the core
     indexes).  Per kernel launch:
-      pair_kernel:   4 per group (its start) + per entry of a group of >= 2 entries: the entry (8,
-                     row and removed sites), its row's config words (8 W) and amplitude (16); the
+      pair_kernel:   4 per group (its start) + per group of >= 2 entries its first entry's config
+                     words (8 W: every row of a group of one core has that core) + per entry of such a
+                     group: the entry (8, row and removed sites) and its row's amplitude (16); the
                      coefficient table once (8 or 16 per entry of n_pairs^2); the output once (16 per
                      row);
       single_kernel: the same over the order-1 index, with the single-excitation tables (n_bits^2 +
                      n_bits^3 entries) once;
       diag_apply:    per row its config words, amplitude and output (8 W + 32); the folded table once.
-    Per core index build: every entry emitted (12 written: 4 key + 8 value), sorted (12 read + 12
-    written), its group start found (4) -- the sort's extra passes and the hash index are not
+    Per core index build (packed entries): every entry emitted (8 written), sorted once (8 read + 8
+    written), split and flagged (8 read; 4 key + 8 value + 4 flag written; the config words of each
+    entry of a group of >= 2 entries read, 8 W), its flag scanned (4 read + 4 written) and read for
+    the starts (4), and 4 per group start -- the sort's extra passes and the hash index are not
     counted."""
     words = (nq + 7) // 8
-    per_entry = 8 + 8 * words + 16
     out: dict[str, object] = {}
     coef_b = 8 if terms.pairs_desc.real else 16
     n_bits = 8 * nq
+
+    def walk(st: dict[str, int]) -> int:
+        return 4 * st["groups"] + 8 * words * st["multi_groups"] + 24 * st["multi_entries"]
+
+    def build(st: dict[str, int]) -> int:
+        return 60 * st["entries"] + 8 * words * st["multi_entries"] + 4 * st["groups"]
+
     st2 = pp.core_stats(targets, 2)
-    out["pair_kernel"] = dict(st2, bytes=4 * st2["groups"] + per_entry * st2["multi_entries"] + coef_b * terms.pairs_desc.n_pairs ** 2
-                              + 16 * n)
-    index_bytes = 40 * st2["entries"] + 4 * st2["groups"]
+    out["pair_kernel"] = dict(st2, bytes=walk(st2) + coef_b * terms.pairs_desc.n_pairs ** 2 + 16 * n)
+    index_bytes = build(st2)
     if terms.singles_desc is not None:
         st1 = pp.core_stats(targets, 1)
         sb = 8 if terms.singles_desc.real else 16
-        out["single_kernel"] = dict(st1, bytes=4 * st1["groups"] + per_entry * st1["multi_entries"]
-                                    + sb * (n_bits ** 2 + n_bits ** 3) + 16 * n)
-        index_bytes += 40 * st1["entries"] + 4 * st1["groups"]
+        out["single_kernel"] = dict(st1, bytes=walk(st1) + sb * (n_bits ** 2 + n_bits ** 3) + 16 * n)
+        index_bytes += build(st1)
     out["diag_apply_kernel"] = {"bytes": n * (8 * words + 32) + 16 * (1 + n_bits + terms.pairs_desc.n_pairs)}
     out["core_index_build"] = {"bytes": index_bytes}
     return out
@@ -422,8 +429,9 @@ def run_batch(args: argparse.Namespace, dev: object) -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": None, "kernel": "batch_kernel<adjoint, one vector> (one direction, all instances)", "kernel_ms": fwd_only,
                      "algorithmic_bytes": model_bytes,
-                     "model": "structured path, bytes moved at least once: group starts, entries of groups of >= 2 entries with their "
-                              "rows' config words and amplitudes, the coefficient tables, the outputs (summed over the instances)",
+                     "model": "structured path, bytes moved at least once: group starts, one config per group of >= 2 entries, "
+                              "the entries of those groups with their rows' amplitudes, the coefficient tables, the outputs "
+                              "(summed over the instances)",
                      "lookup_model": {"what": "one key read per generated connection (the enumeration path's bound, not this path's)",
                                       "bytes": algo_bytes, "gbps_equivalent": algo_bytes / (fwd_only * 1e-3) / 1e9}},
         "batch": {"batched_graph_ms": graph_batch, "batched_eager_ms": eager_batch, "batched_two_launches_ms": two_launches,
@@ -1087,8 +1095,8 @@ def main() -> int:
         label = f"pair_kernel<{(nq + 7) // 8}, {'true' if pmode == pp.MODE_ADJOINT else 'false'}, 1>"
         app_bytes = sum(v["bytes"] for k, v in model.items() if k != "core_index_build")
         step_bytes = model["core_index_build"]["bytes"] + 2 * app_bytes
-        model_desc = ("structured path, bytes moved at least once per launch: group starts, the entries of groups of >= 2 "
-                      "entries with their rows' config words and amplitudes, the coefficient table, the output")
+        model_desc = ("structured path, bytes moved at least once per launch: group starts, one config per group of >= 2 "
+                      "entries, the entries of those groups with their rows' amplitudes, the coefficient table, the output")
     elif use_hops:
         model = hop_model(terms, targets, nq, n)
         kname = "hop_kernel"

@@ -11,7 +11,6 @@
 // reaches HBM with one FP64 atomic per group it belongs to.
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
-#include <rocprim/iterator/transform_iterator.hpp>
 
 #include <vector>
 
@@ -345,11 +344,13 @@ __global__ void pair_inverse_kernel(int n_bits, uint32_t* __restrict__ inv) {
     }
 }
 
-// Flags of a sorted core entry: the first entry of its key (a group head), an entry whose core differs
-// from the previous entry's under the same key (two cores merged by equal keys), an unused slot.
+// Flags of a used sorted core entry: the first entry of its key (a group head, bit 0) and an entry
+// whose core differs from the previous entry's under the same key (two cores merged by equal keys,
+// bit 31).  Unused slots (the sentinel keys, sorted last) get 0; the flag pass writes the number of
+// used entries to counts[0].  An exclusive scan of the flags with 32-bit wrap-around gives each
+// entry's group in bits 0..30 (capacity < 2^31), whatever the break bits add to bit 31.
 constexpr uint32_t kFlagHead = 1u;
-constexpr uint32_t kFlagBreak = 2u;
-constexpr uint32_t kFlagUnused = 4u;
+constexpr uint32_t kFlagBreak = 0x80000000u;
 // Bit 31 of starts[g] (g < G): group g holds more than one core (its rows' cores are compared exactly).
 constexpr uint32_t kGroupMixed = 0x80000000u;
 constexpr uint32_t kGroupStart = 0x7FFFFFFFu;
@@ -373,79 +374,133 @@ __device__ __forceinline__ uint64_t unpack_entry(uint64_t x, int sub_bits, const
     return (static_cast<uint64_t>(pv >> sub_bits) << 16) | (ORDER == 2 ? inv[sub] : sub);
 }
 
-// Flag of sorted entry e (key k, value v; kp, kn the neighbours' keys, vp the previous value).  Only
-// entries of groups of two or more read their config: each compares its core with the previous
-// entry's, taken from the lane below (lane 0 reads it itself).  Every lane runs the shuffles.
-template <int W, int ORDER>
-__device__ __forceinline__ uint32_t entry_flag(const uint64_t* __restrict__ ckeys, int lane, uint32_t k, uint32_t kp, uint32_t kn,
-                                               uint64_t v, uint64_t vp) {
-    const bool used = k != kKeySentinel;
-    const bool head = used && kp != k;
-    const bool multi = used && (!head || kn == k);
-    Cfg<W> core{};
-    if (multi) {
-        core = entry_core<W, ORDER>(ckeys, v);
-    }
-    Cfg<W> prev;
+// Flags of kFlagUnroll * 64 consecutive sorted entries per wave: entry base + 64 t + lane in slot t of
+// each lane.  A wave loads its entries once (the neighbours' keys come from the lanes beside, across
+// slots from lane 63 / lane 0 of the slot beside, and the wave's outer neighbours are read by lanes 0
+// and 63).  Only entries of groups of two or more read their config: each compares its core with
+// the previous entry's, taken the same way; the slots' config loads are independent, so a lane keeps
+// kFlagUnroll of them in flight.  Every lane runs every shuffle.  load(j, k, v) reads entry j < cap.
+constexpr int kFlagUnroll = 4;
+
+template <int W, int ORDER, typename Load>
+__device__ __forceinline__ void core_flags(const uint64_t* __restrict__ ckeys, int64_t cap, Load load, uint32_t (&f)[kFlagUnroll],
+                                           uint32_t (&k)[kFlagUnroll], uint64_t (&v)[kFlagUnroll], int64_t base, int lane,
+                                           int64_t* __restrict__ counts) {
+    constexpr int U = kFlagUnroll;
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-        prev.w[w] = shfl_u64(core.w[w], (lane + 63) & 63);
+    for (int t = 0; t < U; ++t) {
+        k[t] = kKeySentinel;
+        v[t] = 0ull;
+        const int64_t e = base + 64 * t + lane;
+        if (e < cap) {
+            load(e, k[t], v[t]);
+        }
     }
-    if (lane == 0 && used && !head) {
-        prev = entry_core<W, ORDER>(ckeys, vp);
+    // the wave's outer neighbours
+    uint32_t k_before = kKeySentinel, k_after = kKeySentinel;
+    uint64_t v_before = 0ull;
+    if (lane == 0 && base > 0 && base <= cap) {
+        load(base - 1, k_before, v_before);
     }
-    bool same = true;
+    if (lane == 63 && base + 64 * U < cap) {
+        uint64_t v_after;
+        load(base + 64 * U, k_after, v_after);
+    }
+    uint32_t kp[U], kn[U];
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-        same = same && core.w[w] == prev.w[w];
+    for (int t = 0; t < U; ++t) {
+        const uint32_t below = __shfl(k[t], (lane + 63) & 63);
+        const uint32_t above = __shfl(k[t], (lane + 1) & 63);
+        const uint32_t prev_slot = __shfl(k[t > 0 ? t - 1 : 0], 63);
+        const uint32_t next_slot = __shfl(k[t < U - 1 ? t + 1 : U - 1], 0);
+        kp[t] = lane != 0 ? below : (t > 0 ? prev_slot : k_before);
+        kn[t] = lane != 63 ? above : (t < U - 1 ? next_slot : k_after);
     }
-    return !used ? kFlagUnused : (head ? kFlagHead : (same ? 0u : kFlagBreak));
+    Cfg<W> core[U];
+#pragma unroll
+    for (int t = 0; t < U; ++t) {
+        const bool used = k[t] != kKeySentinel;
+        const bool multi = used && (kp[t] == k[t] || kn[t] == k[t]);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            core[t].w[w] = 0ull;
+        }
+        if (multi) {
+            core[t] = entry_core<W, ORDER>(ckeys, v[t]);
+        }
+    }
+    Cfg<W> core_before{};
+    if (lane == 0 && k_before != kKeySentinel && k_before == k[0]) {
+        core_before = entry_core<W, ORDER>(ckeys, v_before);
+    }
+#pragma unroll
+    for (int t = 0; t < U; ++t) {
+        bool same = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const uint64_t below = shfl_u64(core[t].w[w], (lane + 63) & 63);
+            const uint64_t prev_slot = shfl_u64(core[t > 0 ? t - 1 : 0].w[w], 63);
+            const uint64_t prev = lane != 0 ? below : (t > 0 ? prev_slot : core_before.w[w]);
+            same = same & (prev == core[t].w[w]);
+        }
+        const bool used = k[t] != kKeySentinel;
+        const bool head = used && kp[t] != k[t];
+        f[t] = !used ? 0u : (head ? kFlagHead : (same ? 0u : kFlagBreak));
+        if (used && kn[t] == kKeySentinel) {
+            counts[0] = base + 64 * t + lane + 1;  // the last used entry
+        }
+    }
 }
 
 // Flags of packed entries (key << 31 | row << sub_bits | sub, sorted on the key); also splits each
-// entry into its key and its value in the kernels' form (row << 16 | y << 8 | x).  Launched with
-// whole waves (entries past cap count as unused).
+// entry into its key and its value in the kernels' form (row << 16 | y << 8 | x).  Grid: whole
+// waves of kFlagUnroll * 64 entries.
 template <int W, int ORDER>
 __global__ __launch_bounds__(kBlock) void core_flag_unpack_kernel(const uint64_t* __restrict__ entries, int64_t cap,
                                                                   uint32_t* __restrict__ flag, int sub_bits, const uint32_t* __restrict__ inv,
                                                                   const uint64_t* __restrict__ ckeys, uint32_t* __restrict__ keys,
-                                                                  uint64_t* __restrict__ vals) {
+                                                                  uint64_t* __restrict__ vals, int64_t* __restrict__ counts) {
     const int lane = threadIdx.x & 63;
-    const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    constexpr uint64_t kUnusedEntry = static_cast<uint64_t>(kKeySentinel) << kPackedBits;
-    const uint64_t x = e < cap ? entries[e] : kUnusedEntry;
-    const uint64_t xp = e > 0 && e <= cap ? entries[e - 1] : kUnusedEntry;
-    const uint64_t xn = e + 1 < cap ? entries[e + 1] : kUnusedEntry;
-    const uint32_t k = static_cast<uint32_t>(x >> kPackedBits);
-    const uint32_t kp = static_cast<uint32_t>(xp >> kPackedBits);
-    const uint32_t kn = static_cast<uint32_t>(xn >> kPackedBits);
-    const bool used = k != kKeySentinel;
-    const uint64_t v = used ? unpack_entry<ORDER>(x, sub_bits, inv) : 0ull;
-    // the previous entry is used whenever it shares this entry's key
-    const uint64_t vp = used && kp == k ? unpack_entry<ORDER>(xp, sub_bits, inv) : 0ull;
-    const uint32_t f = entry_flag<W, ORDER>(ckeys, lane, k, kp, kn, v, vp);
-    if (e < cap) {
-        flag[e] = f;
-        keys[e] = k;
-        vals[e] = v;
+    const int64_t base = (blockIdx.x * static_cast<int64_t>(blockDim.x) + (threadIdx.x & ~63)) * kFlagUnroll;
+    auto load = [&](int64_t j, uint32_t& kj, uint64_t& vj) {
+        const uint64_t xj = entries[j];
+        kj = static_cast<uint32_t>(xj >> kPackedBits);
+        vj = kj != kKeySentinel ? unpack_entry<ORDER>(xj, sub_bits, inv) : 0ull;
+    };
+    uint32_t f[kFlagUnroll], k[kFlagUnroll];
+    uint64_t v[kFlagUnroll];
+    core_flags<W, ORDER>(ckeys, cap, load, f, k, v, base, lane, counts);
+#pragma unroll
+    for (int t = 0; t < kFlagUnroll; ++t) {
+        const int64_t e = base + 64 * t + lane;
+        if (e < cap) {
+            flag[e] = f[t];
+            keys[e] = k[t];
+            vals[e] = v[t];
+        }
     }
 }
 
 // Flags of sorted (key, value) pairs, as core_flag_unpack_kernel.
 template <int W, int ORDER>
 __global__ __launch_bounds__(kBlock) void core_flag_kernel(const uint32_t* __restrict__ keys, const uint64_t* __restrict__ vals, int64_t cap,
-                                                           const uint64_t* __restrict__ ckeys, uint32_t* __restrict__ flag) {
+                                                           const uint64_t* __restrict__ ckeys, uint32_t* __restrict__ flag,
+                                                           int64_t* __restrict__ counts) {
     const int lane = threadIdx.x & 63;
-    const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    const uint32_t k = e < cap ? keys[e] : kKeySentinel;
-    const uint32_t kp = e > 0 && e <= cap ? keys[e - 1] : kKeySentinel;
-    const uint32_t kn = e + 1 < cap ? keys[e + 1] : kKeySentinel;
-    const bool used = k != kKeySentinel;
-    const uint64_t v = used ? vals[e] : 0ull;
-    const uint64_t vp = used && kp == k ? vals[e - 1] : 0ull;
-    const uint32_t f = entry_flag<W, ORDER>(ckeys, lane, k, kp, kn, v, vp);
-    if (e < cap) {
-        flag[e] = f;
+    const int64_t base = (blockIdx.x * static_cast<int64_t>(blockDim.x) + (threadIdx.x & ~63)) * kFlagUnroll;
+    auto load = [&](int64_t j, uint32_t& kj, uint64_t& vj) {
+        kj = keys[j];
+        vj = vals[j];
+    };
+    uint32_t f[kFlagUnroll], k[kFlagUnroll];
+    uint64_t v[kFlagUnroll];
+    core_flags<W, ORDER>(ckeys, cap, load, f, k, v, base, lane, counts);
+#pragma unroll
+    for (int t = 0; t < kFlagUnroll; ++t) {
+        const int64_t e = base + 64 * t + lane;
+        if (e < cap) {
+            flag[e] = f[t];
+        }
     }
 }
 
@@ -459,25 +514,21 @@ __global__ void key_flag_kernel(const uint32_t* __restrict__ keys, int64_t n, ui
     flag[e] = (k != kKeySentinel && (e == 0 || keys[e - 1] != k)) ? 1u : 0u;
 }
 
-struct HeadBit {
-    __host__ __device__ uint32_t operator()(uint32_t f) const { return f & kFlagHead; }
-};
-using HeadIter = rocprim::transform_iterator<const uint32_t*, HeadBit, uint32_t>;
-
 // starts[g] = first entry of group g, with kGroupMixed when a break lies inside the group;
-// starts[G] = number of used entries; counts {entries, G}.  gid = exclusive scan of the head bits.
-// One wave reads 64 consecutive flags: a head finds its group's end (the next head or unused slot)
-// and the breaks before it from the wave's ballots; the last head of the wave, whose group may run
-// past the wave, reads on 64 flags at a time with the whole wave.  Launched with whole waves.
-__global__ __launch_bounds__(kBlock) void core_start_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ gid, int64_t cap,
+// starts[G] = number of used entries; counts {entries, G}.  gid = exclusive scan of the flags.
+// One wave reads 64 consecutive flags: a head finds its group's end (the next head or the end of
+// the used entries) and the breaks before it from the wave's ballots; the last head of the wave,
+// whose group may run past the wave, reads on 64 flags at a time with the whole wave.  Launched
+// with whole waves.
+__global__ __launch_bounds__(kBlock) void core_start_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ gid,
                                                             uint32_t* __restrict__ starts, int64_t* __restrict__ counts) {
     const int lane = threadIdx.x & 63;
+    const int64_t used = counts[0];
     const int64_t base = blockIdx.x * static_cast<int64_t>(blockDim.x) + (threadIdx.x & ~63);
     const int64_t e = base + lane;
-    const uint32_t f = e < cap ? flag[e] : kFlagUnused;
-    const uint64_t stop = __ballot((f & (kFlagHead | kFlagUnused)) != 0u);
+    const uint32_t f = e < used ? flag[e] : 0u;
+    const uint64_t stop = __ballot(e >= used || (f & kFlagHead) != 0u);
     const uint64_t brk = __ballot((f & kFlagBreak) != 0u);
-    const uint64_t unused = __ballot((f & kFlagUnused) != 0u);
     const bool head = (f & kFlagHead) != 0u;
     const uint64_t after = lane == 63 ? 0ull : ~0ull << (lane + 1);  // lanes above this one
     const uint64_t next = stop & after;
@@ -486,19 +537,16 @@ __global__ __launch_bounds__(kBlock) void core_start_kernel(const uint32_t* __re
     const uint64_t open_mask = __ballot(open);
     bool tail_break = false;
     int64_t tail_end = 0;
-    bool tail_unused = false;
     if (open_mask != 0ull) {
         for (int64_t j0 = base + 64;; j0 += 64) {
             const int64_t j = j0 + lane;
-            const uint32_t fj = j < cap ? flag[j] : kFlagUnused;
-            const uint64_t st = __ballot((fj & (kFlagHead | kFlagUnused)) != 0u);
+            const uint32_t fj = j < used ? flag[j] : 0u;
+            const uint64_t st = __ballot(j >= used || (fj & kFlagHead) != 0u);
             const uint64_t bk = __ballot((fj & kFlagBreak) != 0u);
             if (st != 0ull) {
                 const int nx = __builtin_ctzll(st);
                 tail_break = tail_break || (bk & ((1ull << nx) - 1ull)) != 0ull;
                 tail_end = j0 + nx;
-                const uint32_t fe = __shfl(fj, nx);
-                tail_unused = (fe & kFlagUnused) != 0u;
                 break;
             }
             tail_break = tail_break || bk != 0ull;
@@ -509,22 +557,18 @@ __global__ __launch_bounds__(kBlock) void core_start_kernel(const uint32_t* __re
     }
     bool mixed;
     int64_t end;
-    bool last;
     if (open) {
         mixed = (brk & after) != 0ull || tail_break;
         end = tail_end;
-        last = tail_unused;
     } else {
         const int nx = __builtin_ctzll(next);
         mixed = (brk & after & ((1ull << nx) - 1ull)) != 0ull;
         end = base + nx;
-        last = ((unused >> nx) & 1ull) != 0ull;
     }
-    const uint32_t g = gid[e];
+    const uint32_t g = gid[e] & kGroupStart;
     starts[g] = static_cast<uint32_t>(e) | (mixed ? kGroupMixed : 0u);
-    if (last) {
+    if (end == used) {
         starts[g + 1] = static_cast<uint32_t>(end);
-        counts[0] = end;
         counts[1] = static_cast<int64_t>(g) + 1;
     }
 }
@@ -586,7 +630,7 @@ int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, int ord
         }
         tmp = tmp > packed ? tmp : packed;
         size_t scan = 0;
-        if (rocprim::exclusive_scan(nullptr, scan, HeadIter(u, HeadBit{}), u, 0u, cap, rocprim::plus<uint32_t>()) != hipSuccess) {
+        if (rocprim::exclusive_scan(nullptr, scan, u, u, 0u, cap, rocprim::plus<uint32_t>()) != hipSuccess) {
             return set_error(QMB_ERR_HIP, "rocprim exclusive_scan size query failed");
         }
         tmp = tmp > scan ? tmp : scan;
@@ -944,6 +988,9 @@ __device__ void pair_group_packed(const PairParams& p, uint64_t mask, uint32_t g
             core0.w[w] = shfl_u64(own.w[w], gi * H);
         }
         const PairRow<W> ra = pair_row<W>(p, mixed ? own : core0, r0, r1, ADJ, fermi);
+        // the exact per-pair check runs wave-wide when any group of the pass is mixed: a row of a
+        // group of one core holds its true core (core0) and passes it for its connections
+        const bool wmixed = __ballot(mixed) != 0ull;
         // this lane's entry (its own row), staged for the rows of its group
         stage[lane] = stage_entry<W, ADJ, NV>(p, act, va, core0, fermi);
         double acc_re[NV] = {}, acc_im[NV] = {};
@@ -957,7 +1004,7 @@ __device__ void pair_group_packed(const PairParams& p, uint64_t mask, uint32_t g
                 const PairStage<W>& st = stage[gi * H + (uu & (H - 1))];
                 gv[k] = st.g;
                 gv2[k] = NV == 2 ? st.g2 : make_double2(0.0, 0.0);
-                ok[k] = pair_term<W, ADJ, REAL>(p, ra, st, mixed, fmask, act && uu < g, x[k]);
+                ok[k] = pair_term<W, ADJ, REAL>(p, ra, st, wmixed, fmask, act && uu < g, x[k]);
             }
 #pragma unroll
             for (int k = 0; k < B; ++k) {
@@ -1417,6 +1464,9 @@ __device__ void single_group_packed(const SingleParams& p, uint64_t mask, uint32
             const uint64_t c0 = shfl_u64(own.w[w], gi * H);  // every lane runs every shuffle
             core.w[w] = mixed ? own.w[w] : c0;
         }
+        // the exact per-pair check runs wave-wide when any group of the pass is mixed (a row of a
+        // group of one core holds its true core and passes it)
+        const bool wmixed = __ballot(mixed) != 0ull;
         Cfg<W> ca = core;
         flip_site<W>(ca, r);
         const double2 gin = act ? p.in[ia] : make_double2(0.0, 0.0);
@@ -1429,7 +1479,7 @@ __device__ void single_group_packed(const SingleParams& p, uint64_t mask, uint32
             const double2 gb2 = NV == 2 ? shfl_d2(gin2, src) : make_double2(0.0, 0.0);
             const int q = static_cast<int>(vb & 0xff);
             bool ok = act && u < g && u != ri && q != r;
-            if (mixed && ok) {
+            if (wmixed && ok) {
                 Cfg<W> cb = load_key<W>(p.ckeys, static_cast<int64_t>(vb >> 16));
                 flip_site<W>(cb, q);
 #pragma unroll
@@ -2386,8 +2436,8 @@ int cores_build(const qmb_targets* targets, int max_occupied, int order, void* b
                 QMB_CHECK_LAUNCH();
             }
             auto unpack = [&](auto kernel) {
-                kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(vals_in, c.capacity, flag, sub_bits, inv, targets->keys, c.keys,
-                                                                      c.vals);
+                kernel<<<blocks_for(c.capacity, kBlock * kFlagUnroll), kBlock, 0, stream>>>(vals_in, c.capacity, flag, sub_bits, inv,
+                                                                                         targets->keys, c.keys, c.vals, c.counts);
             };
             switch (shape) {
                 case 2: unpack(&core_flag_unpack_kernel<1, 1>); break;
@@ -2403,7 +2453,8 @@ int cores_build(const qmb_targets* targets, int max_occupied, int order, void* b
             QMB_HIP_TRY(rocprim::radix_sort_pairs(tmp, need, keys_in, c.keys, vals_in, c.vals, static_cast<unsigned>(cap), 0, kKeyBits,
                                                   stream));
             auto flags = [&](auto kernel) {
-                kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(c.keys, c.vals, c.capacity, targets->keys, flag);
+                kernel<<<blocks_for(c.capacity, kBlock * kFlagUnroll), kBlock, 0, stream>>>(c.keys, c.vals, c.capacity, targets->keys,
+                                                                                         flag, c.counts);
             };
             switch (shape) {
                 case 2: flags(&core_flag_kernel<1, 1>); break;
@@ -2418,9 +2469,9 @@ int cores_build(const qmb_targets* targets, int max_occupied, int order, void* b
         }
         QMB_CHECK_LAUNCH();
         need = tmp_bytes;
-        QMB_HIP_TRY(rocprim::exclusive_scan(tmp, need, HeadIter(flag, HeadBit{}), gid, 0u, cap, rocprim::plus<uint32_t>(), stream));
+        QMB_HIP_TRY(rocprim::exclusive_scan(tmp, need, flag, gid, 0u, cap, rocprim::plus<uint32_t>(), stream));
         // whole waves: the start kernel reads the flags of 64 entries per wave (unused past cap)
-        core_start_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(flag, gid, c.capacity, c.starts, c.counts);
+        core_start_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(flag, gid, c.starts, c.counts);
         c.key_bits = kKeyBits;
         QMB_CHECK_LAUNCH();
     }
