@@ -224,6 +224,7 @@ class PreparedTargets:
         # holds by construction)
         self._occupied_hint = max_occupied is not None
         self._cores: dict[int, tuple[torch.Tensor, _lib.QmbCores]] = {}
+        self._unchecked: set[int] = set()
         self._strings: dict[int, tuple[torch.Tensor, _lib.QmbStrings]] = {}
         self._strings_stats: dict[int, dict[str, int]] = {}
         self._diagonal: dict[int, tuple[PreparedTerms, torch.Tensor]] = {}
@@ -263,9 +264,15 @@ class PreparedTargets:
                 self._max_occupied = int(table[self.configs.long()].sum(dim=1).max().item())
         return self._max_occupied
 
-    def cores(self, order: int = 2) -> _lib.QmbCores:
+    def has_cores(self, order: int) -> bool:
+        """True once the core index of this order is built (or being built on a stream)."""
+        return order in self._cores
+
+    def cores(self, order: int = 2, check: bool = True) -> _lib.QmbCores:
         """Core index of the set (order 2: pair-excitation cores, 1: single-excitation cores), built
-        on first use."""
+        on first use on the current stream.  With a max_occupied bound from the caller, the build's
+        overflow count is read (a sync) on the first call with check=True: an index built on a
+        side stream is checked after the caller has joined that stream."""
         if order not in self._cores:
             lib = _lib.library()
             occ = self.max_occupied()
@@ -276,9 +283,13 @@ class PreparedTargets:
             with torch.cuda.device(self.device):
                 _lib.check(lib.qmb_cores_build(ctypes.byref(self.keys_desc), occ, order, _ptr(buf), buf.numel(), ctypes.byref(desc),
                                                _stream(self.device)), "qmb_cores_build")
-            if self._occupied_hint:
-                check_core_overflow(_device_count(buf, desc.counts + 16), occ)
             self._cores[order] = (buf, desc)
+            if self._occupied_hint:
+                self._unchecked.add(order)
+        if check and order in self._unchecked:
+            buf, desc = self._cores[order]
+            self._unchecked.discard(order)
+            check_core_overflow(_device_count(buf, desc.counts + 16), self.max_occupied())
         return self._cores[order][1]
 
 
@@ -382,6 +393,17 @@ def apply_hops(terms: PreparedTerms, targets: PreparedTargets, vec: torch.Tensor
                                           _ptr(vec), _ptr(out), mode, stream), "qmb_apply_hops")
 
 
+_SIDE_STREAMS: dict[int, torch.cuda.Stream] = {}
+
+
+def _side_stream(device: torch.device) -> torch.cuda.Stream:
+    """A second stream per device (default priority) for index builds beside the apply kernels."""
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    if idx not in _SIDE_STREAMS:
+        _SIDE_STREAMS[idx] = torch.cuda.Stream(device=torch.device("cuda", idx))
+    return _SIDE_STREAMS[idx]
+
+
 def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tensor, vec: torch.Tensor, mode: int,
           out: torch.Tensor | None = None, tile: int = 0, groups: int = 0, zero: bool = True, strategy: int = 0,
           symmetric: bool | int | None = None, pairs: bool | None = None, hops: bool | None = None) -> torch.Tensor:
@@ -427,13 +449,26 @@ def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tenso
             _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.residual_desc), ctypes.byref(targets.desc), _ptr(configs_i), n_i,
                                               _ptr(vec), _ptr(out), mode, ctypes.byref(launch) if launch else None, stream),
                        "qmb_apply_prepared")
+        # A first application builds the order-1 index on a side stream, beside the order-2 build and
+        # the pair kernel (which leaves memory bandwidth and room on the CUs unused); the single
+        # kernel runs after the join.
+        side = None
+        if terms.singles_desc is not None and not targets.has_cores(1):
+            side = _side_stream(device)
+            side.wait_stream(torch.cuda.current_stream(device))
+            with torch.cuda.stream(side):
+                targets.cores(1, check=False)
+            # the index's memory is allocated on the side stream and read on this one
+            targets._cores[1][0].record_stream(torch.cuda.current_stream(device))  # pylint: disable=protected-access
         _lib.check(lib.qmb_apply_diagonal(ctypes.byref(terms.pairs_desc), ctypes.byref(targets.keys_desc), _ptr(vec), _ptr(out), pmode,
                                           stream), "qmb_apply_diagonal")
-        if terms.singles_desc is not None:
-            _lib.check(lib.qmb_apply_singles(ctypes.byref(terms.singles_desc), ctypes.byref(targets.cores(1)), ctypes.byref(targets.keys_desc),
-                                             _ptr(vec), _ptr(out), pmode, stream), "qmb_apply_singles")
         _lib.check(lib.qmb_apply_pairs(ctypes.byref(terms.pairs_desc), ctypes.byref(targets.cores(2)), ctypes.byref(targets.keys_desc),
                                        _ptr(vec), _ptr(out), pmode, stream), "qmb_apply_pairs")
+        if terms.singles_desc is not None:
+            if side is not None:
+                torch.cuda.current_stream(device).wait_stream(side)
+            _lib.check(lib.qmb_apply_singles(ctypes.byref(terms.singles_desc), ctypes.byref(targets.cores(1)), ctypes.byref(targets.keys_desc),
+                                             _ptr(vec), _ptr(out), pmode, stream), "qmb_apply_singles")
     return out
 
 
