@@ -761,7 +761,7 @@ __device__ __forceinline__ PairStage<W> stage_entry(const PairParams& p, bool ac
         const Cfg<W> bq = site_bits<W>(q0, q1);
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            st.m[w] = mq.w[w];
+            st.m[w] = fermi ? mq.w[w] : 0ull;  // zero for hard-core bosons: no sign
             st.bits[w] = bq.w[w];
         }
         const int64_t slot = pair_slot(q0, q1, p.n_bits);
@@ -800,8 +800,8 @@ struct PairRow {
 //   !ADJ: A_ab = (-1)^(rbit ^ ebit ^ parity(bits_b & [r0, r1))) T[Pb][Pa]
 // with ebit taken from the row's own core in a mixed group.
 template <int W, bool ADJ, bool REAL>
-__device__ __forceinline__ bool pair_term(const PairParams& p, const PairRow<W>& ra, const PairStage<W>& st, bool mixed, int fermi,
-                                          bool en, double2& x) {
+__device__ __forceinline__ bool pair_term(const PairParams& p, const PairRow<W>& ra, const PairStage<W>& st, bool mixed, bool en,
+                                          double2& x) {
     uint64_t meet = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
@@ -830,16 +830,19 @@ __device__ __forceinline__ bool pair_term(const PairParams& p, const PairRow<W>&
         const int64_t idx = ra.part + static_cast<int64_t>(st.part);
         x = REAL ? make_double2(p.coef_re[idx], 0.0) : p.coef[idx];
     }
-    int c = 0;
+    // The sign's parity in bit 0: rbit ^ ebit plus the other pair's sites inside this pair's range
+    // (the range masks, rbit and ebit are zero for hard-core bosons).  Adding par << 31 to the high
+    // word flips the sign bit by bit 0 of par alone, without a branch.
+    uint32_t par = static_cast<uint32_t>(ra.rbit ^ ebit);
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-        c += __popcll(ADJ ? (ra.bits.w[w] & st.m[w]) : (st.bits[w] & ra.mr.w[w]));
+        const uint64_t m = ADJ ? (ra.bits.w[w] & st.m[w]) : (st.bits[w] & ra.mr.w[w]);
+        par += static_cast<uint32_t>(__popc(static_cast<uint32_t>(m))) + static_cast<uint32_t>(__popc(static_cast<uint32_t>(m >> 32)));
     }
-    // (-1)^parity as the sign bit, applied without a branch (fermi = 0 for hard-core bosons)
-    const uint64_t sign = static_cast<uint64_t>((ra.rbit ^ ebit ^ c) & fermi) << 63;
-    x.x = __longlong_as_double(__double_as_longlong(x.x) ^ static_cast<long long>(sign));
+    const int flip = static_cast<int>(par << 31);
+    x.x = __hiloint2double(__double2hiint(x.x) + flip, __double2loint(x.x));
     if (!REAL) {
-        x.y = __longlong_as_double(__double_as_longlong(x.y) ^ static_cast<long long>(sign));
+        x.y = __hiloint2double(__double2hiint(x.y) + flip, __double2loint(x.y));
     }
     return ok;
 }
@@ -853,6 +856,12 @@ __device__ __forceinline__ PairRow<W> pair_row(const PairParams& p, const Cfg<W>
     const int64_t slot = pair_slot(r0, r1, p.n_bits);
     ra.part = adj ? slot * p.n_pairs : slot;
     ra.rbit = fermi ? (masked_parity<W>(ra.core, ra.mr) ^ 1) : 0;
+    if (!fermi) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            ra.mr.w[w] = 0ull;  // no sign for hard-core bosons
+        }
+    }
     return ra;
 }
 
@@ -886,7 +895,6 @@ __device__ void pair_group(const PairParams& p, uint32_t s_raw, uint32_t e, int 
     const uint64_t v0 = p.vals[s];
     const Cfg<W> core0 = entry_core<W, 2>(p.ckeys, v0);
     const bool fermi = p.particle_cut == 1;
-    const int fmask = fermi ? 1 : 0;
     for (uint32_t a0 = s; a0 < e; a0 += 64) {
         const uint32_t a = a0 + lane;
         const bool act = a < e;
@@ -909,7 +917,7 @@ __device__ void pair_group(const PairParams& p, uint32_t s_raw, uint32_t e, int 
                     g[k] = st.g;
                     g2[k] = NV == 2 ? st.g2 : make_double2(0.0, 0.0);
                     // slots past nb hold entries of the previous chunk or nothing: bounded by u + k < nb
-                    ok[k] = pair_term<W, ADJ, REAL>(p, ra, st, mixed, fmask, act && (u + k < nb), x[k]);
+                    ok[k] = pair_term<W, ADJ, REAL>(p, ra, st, mixed, act && (u + k < nb), x[k]);
                 }
 #pragma unroll
                 for (int k = 0; k < B; ++k) {
@@ -946,7 +954,6 @@ __device__ void pair_group_packed(const PairParams& p, uint64_t mask, uint32_t g
     const int gi = lane / H;
     const int ri = lane % H;
     const bool fermi = p.particle_cut == 1;
-    const int fmask = fermi ? 1 : 0;
     while (mask) {
         uint32_t s = 0, e = 0;
         int nmax = 0;
@@ -1004,7 +1011,7 @@ __device__ void pair_group_packed(const PairParams& p, uint64_t mask, uint32_t g
                 const PairStage<W>& st = stage[gi * H + (uu & (H - 1))];
                 gv[k] = st.g;
                 gv2[k] = NV == 2 ? st.g2 : make_double2(0.0, 0.0);
-                ok[k] = pair_term<W, ADJ, REAL>(p, ra, st, wmixed, fmask, act && uu < g, x[k]);
+                ok[k] = pair_term<W, ADJ, REAL>(p, ra, st, wmixed, act && uu < g, x[k]);
             }
 #pragma unroll
             for (int k = 0; k < B; ++k) {
