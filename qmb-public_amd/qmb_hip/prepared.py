@@ -293,6 +293,14 @@ class PreparedTargets:
         return self._cores[order][1]
 
 
+    def has_strings(self, parity: int) -> bool:
+        """True once the strings index of this parity is built (or being built on a stream)."""
+        return parity in self._strings
+
+    def has_diagonal_values(self, terms: "PreparedTerms") -> bool:
+        """True once the diagonal values under `terms` are computed (or being computed)."""
+        return id(terms) in self._diagonal
+
     def strings(self, parity: int) -> _lib.QmbStrings:
         """Strings index of the set for the hops between sites of `parity` (0 even, 1 odd): the rows
         grouped by their string on the other parity's sites, built on first use."""
@@ -369,8 +377,16 @@ def can_use_hops(terms: PreparedTerms, targets: PreparedTargets, configs_i: torc
     built here: a synchronisation) stay within HOP_PAIRS_PER_TERM_ROW per row and hop term."""
     if not terms.hops or terms.pairs_desc is not None or not targets.indexes(configs_i) or targets.duplicates() != 0:
         return False
-    targets.strings(0)
-    targets.strings(1)
+    # the two parities' indexes build side by side (one on a second stream), then one sync
+    if not targets.has_strings(1):
+        device = targets.device
+        side = _side_stream(device)
+        side.wait_stream(torch.cuda.current_stream(device))
+        with torch.cuda.stream(side):
+            targets.strings(1)
+        targets._strings[1][0].record_stream(torch.cuda.current_stream(device))  # pylint: disable=protected-access
+        targets.strings(0)
+        torch.cuda.current_stream(device).wait_stream(side)
     tests = sum(targets.strings_stats(parity)["pair_tests"] for parity in (0, 1))
     return tests <= HOP_PAIRS_PER_TERM_ROW * max(targets.n, 1) * terms.n_hop_terms
 
@@ -383,14 +399,24 @@ def apply_hops(terms: PreparedTerms, targets: PreparedTargets, vec: torch.Tensor
     device = targets.device
     stream = _stream(device)
     with torch.cuda.device(device):
+        # a first application computes the rows' diagonal values on a second stream beside the hop
+        # kernels, and adds them after the join
+        side = None
+        if not targets.has_diagonal_values(terms):
+            side = _side_stream(device)
+            side.wait_stream(torch.cuda.current_stream(device))
+            with torch.cuda.stream(side):
+                targets.diagonal_values(terms).record_stream(torch.cuda.current_stream(device))
         if terms.n_residual_terms > 0:
             _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.residual_desc), ctypes.byref(targets.desc), _ptr(targets.configs),
                                               targets.n, _ptr(vec), _ptr(out), mode, None, stream), "qmb_apply_prepared")
-        _lib.check(lib.qmb_apply_diagonal_values(targets.n, _ptr(targets.diagonal_values(terms)), _ptr(vec), _ptr(out), mode, stream),
-                   "qmb_apply_diagonal_values")
         for parity in (0, 1):
             _lib.check(lib.qmb_apply_hops(ctypes.byref(terms.singles_desc), ctypes.byref(targets.strings(parity)), ctypes.byref(targets.keys_desc),
                                           _ptr(vec), _ptr(out), mode, stream), "qmb_apply_hops")
+        if side is not None:
+            torch.cuda.current_stream(device).wait_stream(side)
+        _lib.check(lib.qmb_apply_diagonal_values(targets.n, _ptr(targets.diagonal_values(terms)), _ptr(vec), _ptr(out), mode, stream),
+                   "qmb_apply_diagonal_values")
 
 
 _SIDE_STREAMS: dict[int, torch.cuda.Stream] = {}
