@@ -475,9 +475,10 @@ def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tenso
             _lib.check(lib.qmb_apply_prepared(ctypes.byref(terms.residual_desc), ctypes.byref(targets.desc), _ptr(configs_i), n_i,
                                               _ptr(vec), _ptr(out), mode, ctypes.byref(launch) if launch else None, stream),
                        "qmb_apply_prepared")
-        # A first application builds the order-1 index on a side stream, beside the order-2 build and
-        # the pair kernel (which leaves memory bandwidth and room on the CUs unused); the single
-        # kernel runs after the join.
+        # A first application builds the order-1 index on a side stream beside the pair kernel (which
+        # leaves memory bandwidth and room on the CUs unused): forked once the order-2 index is
+        # built, joined before the single kernel.
+        cores2 = targets.cores(2)
         side = None
         if terms.singles_desc is not None and not targets.has_cores(1):
             side = _side_stream(device)
@@ -488,7 +489,7 @@ def apply(terms: PreparedTerms, targets: PreparedTargets, configs_i: torch.Tenso
             targets._cores[1][0].record_stream(torch.cuda.current_stream(device))  # pylint: disable=protected-access
         _lib.check(lib.qmb_apply_diagonal(ctypes.byref(terms.pairs_desc), ctypes.byref(targets.keys_desc), _ptr(vec), _ptr(out), pmode,
                                           stream), "qmb_apply_diagonal")
-        _lib.check(lib.qmb_apply_pairs(ctypes.byref(terms.pairs_desc), ctypes.byref(targets.cores(2)), ctypes.byref(targets.keys_desc),
+        _lib.check(lib.qmb_apply_pairs(ctypes.byref(terms.pairs_desc), ctypes.byref(cores2), ctypes.byref(targets.keys_desc),
                                        _ptr(vec), _ptr(out), pmode, stream), "qmb_apply_pairs")
         if terms.singles_desc is not None:
             if side is not None:
