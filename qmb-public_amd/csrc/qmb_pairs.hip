@@ -1962,10 +1962,15 @@ __global__ __launch_bounds__(kBlock) void hop_kernel(HopParams p) {
             }
             __builtin_amdgcn_wave_barrier();
             const int nb = static_cast<int>(hi - c0 < 64u ? hi - c0 : 64u);
-            // the pair tests: bit u of `hit` marks a connection of this lane's row to entry u
+            // the pair tests: bit u of `hit` marks a connection of this lane's row to entry u.  A row
+            // meets only the entries of its own group, so each lane walks its group's part of the
+            // chunk: the wave's walk is the longest such part, not the whole union.
+            const int ulo = static_cast<int>(gs) - static_cast<int>(c0) > 0 ? static_cast<int>(gs) - static_cast<int>(c0) : 0;
+            const int uend = static_cast<int>(ge) - static_cast<int>(c0);
+            const int uhi = act ? (uend < nb ? uend : nb) : 0;
             uint64_t hit = 0ull;
             if (merged) {
-                for (int u = 0; u < nb; ++u) {
+                for (int u = ulo; u < uhi; ++u) {
                     const HopStage<W>& st = L.stage[u];
                     int pc = 0, pa = 0;
                     uint64_t off = 0ull;
@@ -1982,10 +1987,9 @@ __global__ __launch_bounds__(kBlock) void hop_kernel(HopParams p) {
             } else {
                 // rows of one (unmerged) group share the other parity's string: two hop-parity
                 // sites differ, one occupied in each
-                for (int u = 0; u < nb; ++u) {
-                    const uint32_t bu = c0 + static_cast<uint32_t>(u);
+                for (int u = ulo; u < uhi; ++u) {
                     const uint64_t d = sa ^ L.stage[u].c[0];
-                    const bool ok = __popcll(d) == 2 && __popcll(d & sa) == 1 && bu >= gs && bu < ge;
+                    const bool ok = __popcll(d) == 2 && __popcll(d & sa) == 1;
                     hit |= ok ? (1ull << u) : 0ull;
                 }
             }
