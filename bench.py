@@ -1084,7 +1084,7 @@ def main() -> int:
         cps.append(a.elapsed_time(b))
     copy_gbps = 2 * src.numel() * 8 / (float(np.median(cps)) * 1e-3) / 1e9
     del src, dst
-    # The survey's lookup model (one key read per generated connection): the bound of the
+    # The lookup model (one key read per generated connection): the bound of the
     # enumeration path, reported for the structured path only as a labelled comparison.
     lookup_bytes = n * (nq + 16) + n * (nq + 16) + n_terms * 28 + m_gen * nq + m_hit * 16
     log(f"M_gen={m_gen} M_hit={m_hit}; forward {kernel_ms:.3f} ms, adjoint {adjoint_ms:.3f} ms, copy {copy_gbps:.0f} GB/s")

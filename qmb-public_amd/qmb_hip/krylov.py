@@ -149,7 +149,7 @@ def lanczos(operator: typing.Callable[[torch.Tensor], torch.Tensor], psi: torch.
 
 @dataclasses.dataclass
 class DynamicLanczos:
-    """Lanczos on a basis that can grow between runs (the driver of qmb/haar.py:21-183, same
+    """Lanczos on a basis that can grow between runs (the Lanczos loop of qmb/haar.py:21-183, same
     fields and yields).  Every run of the recurrence starts from the stored start vector `psi`,
     zero-padded onto the current basis, and `run()` yields (energy, basis, Ritz vector) tuples:
 

@@ -4,7 +4,7 @@ Host-side checks of the Lanczos driver (qmb_hip.krylov) that need no GPU:
   - the tridiagonal solve (qmb/haar.py:168-183: SciPy stebz, lowest pair only) returns the lowest
     eigenvalue of the dense tridiagonal matrix and combines the Krylov vectors with its eigenvector;
     with no beta it returns the first alpha and vector unchanged;
-  - the driver validates its arguments as the reference's arithmetic would fail on them
+  - DynamicLanczos validates its arguments as the reference's arithmetic would fail on them
     (batch_count_apply_within < 1 divides by zero there, a psi of the wrong length);
   - qmb_lanczos_step rejects null, aliased and short-workspace arguments before any device work.
 """
