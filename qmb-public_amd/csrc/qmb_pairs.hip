@@ -127,29 +127,38 @@ __device__ void fold_diagonal(uint64_t rec, int particle_cut, int n_bits, const 
     }
 }
 
+// *counter += the lanes of the wave where `add` holds, as one atomic from the wave's first such lane
+// (a same-address atomic per thread serialises: 600k of them took 7 ms).  Every lane must call it.
+__device__ __forceinline__ void wave_count(int64_t* counter, bool add) {
+    const uint64_t m = __ballot(add);
+    if (m != 0ull && static_cast<int>(threadIdx.x & 63u) == __builtin_ctzll(m)) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(counter), static_cast<unsigned long long>(__popcll(m)));
+    }
+}
+
+// Launched with whole waves: threads past n_terms take part in the counts only.
 __global__ void pairs_fill_kernel(const int16_t* __restrict__ site, const uint8_t* __restrict__ kind,
                                   const double2* __restrict__ coef, int64_t n_terms, int particle_cut, int n_bits,
                                   int64_t n_pairs, double2* __restrict__ table, double2* __restrict__ diag,
                                   int64_t* __restrict__ counts) {
     const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (t >= n_terms) {
-        return;
+    bool is_diag = false, is_pair = false;
+    if (t < n_terms) {
+        const CompiledTerm c = compile_term(site + 4 * t, kind + 4 * t, particle_cut, n_bits);
+        PairExcitation e;
+        if (c.status == kCompileOk && c.diagonal) {
+            fold_diagonal(c.rec, particle_cut, n_bits, coef[t], diag);
+            is_diag = true;
+        } else if (c.status == kCompileOk && pair_excitation(c.rec, particle_cut, &e)) {
+            const double2 v = coef[t];
+            double2* dst = table + pair_slot(e.r0, e.r1, n_bits) * n_pairs + pair_slot(e.c0, e.c1, n_bits);
+            unsafeAtomicAdd(&dst->x, e.delta ? -v.x : v.x);
+            unsafeAtomicAdd(&dst->y, e.delta ? -v.y : v.y);
+            is_pair = true;
+        }
     }
-    const CompiledTerm c = compile_term(site + 4 * t, kind + 4 * t, particle_cut, n_bits);
-    if (c.status == kCompileOk && c.diagonal) {
-        fold_diagonal(c.rec, particle_cut, n_bits, coef[t], diag);
-        atomicAdd(reinterpret_cast<unsigned long long*>(&counts[1]), 1ull);
-        return;
-    }
-    PairExcitation e;
-    if (c.status != kCompileOk || !pair_excitation(c.rec, particle_cut, &e)) {
-        return;
-    }
-    const double2 v = coef[t];
-    double2* dst = table + pair_slot(e.r0, e.r1, n_bits) * n_pairs + pair_slot(e.c0, e.c1, n_bits);
-    unsafeAtomicAdd(&dst->x, e.delta ? -v.x : v.x);
-    unsafeAtomicAdd(&dst->y, e.delta ? -v.y : v.y);
-    atomicAdd(reinterpret_cast<unsigned long long*>(&counts[0]), 1ull);
+    wave_count(&counts[0], is_pair);
+    wave_count(&counts[1], is_diag);
 }
 
 int pairs_layout(Carver& cv, int n_qubytes, qmb_pairs* out) {
@@ -1221,33 +1230,35 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(PairParams p) {
 }
 
 // ------------------------------------------------------------------------- single excitations --
+// Launched with whole waves: threads past n_terms take part in the counts only.
 __global__ void singles_fill_kernel(const int16_t* __restrict__ site, const uint8_t* __restrict__ kind,
                                     const double2* __restrict__ coef, int64_t n_terms, int particle_cut, int n_bits,
                                     double2* __restrict__ a, double2* __restrict__ kd, int64_t* __restrict__ counts) {
     const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (t >= n_terms) {
-        return;
+    bool is_single = false, is_dressed = false;
+    if (t < n_terms) {
+        const CompiledTerm c = compile_term(site + 4 * t, kind + 4 * t, particle_cut, n_bits);
+        SingleExcitation e;
+        if (c.status == kCompileOk && single_excitation(c.rec, particle_cut, &e)) {
+            const double2 v0 = coef[t];
+            const double2 v = e.delta ? make_double2(-v0.x, -v0.y) : v0;
+            const int64_t rp = static_cast<int64_t>(e.r) * n_bits + e.p;
+            if (e.k < 0 || !e.kocc) {
+                // holds whenever k is empty
+                unsafeAtomicAdd(&a[rp].x, v.x);
+                unsafeAtomicAdd(&a[rp].y, v.y);
+            }
+            if (e.k >= 0) {
+                double2* dk = kd + rp * n_bits + e.k;
+                unsafeAtomicAdd(&dk->x, e.kocc ? v.x : -v.x);
+                unsafeAtomicAdd(&dk->y, e.kocc ? v.y : -v.y);
+                is_dressed = true;
+            }
+            is_single = true;
+        }
     }
-    const CompiledTerm c = compile_term(site + 4 * t, kind + 4 * t, particle_cut, n_bits);
-    SingleExcitation e;
-    if (c.status != kCompileOk || !single_excitation(c.rec, particle_cut, &e)) {
-        return;
-    }
-    const double2 v0 = coef[t];
-    const double2 v = e.delta ? make_double2(-v0.x, -v0.y) : v0;
-    const int64_t rp = static_cast<int64_t>(e.r) * n_bits + e.p;
-    if (e.k < 0 || !e.kocc) {
-        // holds whenever k is empty
-        unsafeAtomicAdd(&a[rp].x, v.x);
-        unsafeAtomicAdd(&a[rp].y, v.y);
-    }
-    if (e.k >= 0) {
-        double2* dk = kd + rp * n_bits + e.k;
-        unsafeAtomicAdd(&dk->x, e.kocc ? v.x : -v.x);
-        unsafeAtomicAdd(&dk->y, e.kocc ? v.y : -v.y);
-        atomicAdd(reinterpret_cast<unsigned long long*>(&counts[1]), 1ull);
-    }
-    atomicAdd(reinterpret_cast<unsigned long long*>(&counts[0]), 1ull);
+    wave_count(&counts[0], is_single);
+    wave_count(&counts[1], is_dressed);
 }
 
 int singles_layout(Carver& cv, int n_qubytes, qmb_singles* out) {
