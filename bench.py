@@ -18,7 +18,7 @@ Workloads (--workload):
   hubbard128:            open 1-D Hubbard chain, 64 sites (128 spin orbitals), t=1, U=4, half
                          filling, 4,000,000 configs (random seeds + single hops), configs_j = configs_i.
   molecular20:           20 spin orbitals, 5a+5b, 10,000 configs uniform in the sector.
-  batch64:               64 seeded molecular instances, forward + backward in one persistent launch.
+  batch64:               64 seeded molecular instances, forward + backward in one batched pass.
   lanczos52:             Lanczos iterations on the molecular52 basis (application + vector step).
   pt2_52:                the perturbation step (E0, find_relative, application into the external set).
   sampler52:             the config producer: unique sampling of 1M configs from a 26-site network.

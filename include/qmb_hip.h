@@ -414,11 +414,12 @@ int qmb_apply_structured_dual(const qmb_pairs* pairs, const qmb_singles* singles
                               void* stream);
 
 /* ---------------- batched persistent launch (no reference counterpart) -------------------------
- * Many instances -- a term table and a config set each, of any sizes -- applied in ONE persistent
- * kernel launch: the diagonal, single- and pair-excitation work of every instance is cut into
- * units of 64 core groups or 64 rows, and persistent waves take the units round-robin (pair units
- * of all instances first).  For many small instances this replaces a few kernels per instance by
- * one launch that fills the GPU; the launch is capturable in a hipGraph (no allocation, no sync).
+ * Many instances -- a term table and a config set each, of any sizes -- applied by two persistent
+ * kernel launches for all of them: the diagonal, single- and pair-excitation work of every instance
+ * is cut into units of 64 core groups or 64 rows; the first launch's waves take the pair units of
+ * all instances round-robin, the second's the single and diagonal units.  For many small instances
+ * this replaces a few kernels per instance by two launches that fill the GPU; both are capturable
+ * in a hipGraph (no allocation, no sync).
  * An instance whose table has terms outside those classes (its residual table) needs that part
  * applied separately. */
 typedef struct qmb_batch_item {
@@ -447,11 +448,12 @@ typedef struct qmb_batch {
 int qmb_batch_workspace_size(int n_items, size_t* bytes);
 /* Copies the item table to the device; synchronises `stream` once (set-up, not per launch). */
 int qmb_batch_plan(const qmb_batch_item* items, int n_items, void* buffer, size_t bytes, qmb_batch* out, void* stream);
-/* One persistent launch of `waves` wavefronts (0: 8192) over all units; mode 0 forward, 1 adjoint
+/* Two persistent launches of `waves` wavefronts each (0: 8192) over all units (the pair units, then
+ * the single and diagonal units); mode 0 forward, 1 adjoint
  * (a Hermitian instance gives the same map in both).  A two-vector batch (dual) computes
  * out = A in and out2 = A in2 (mode 0; A^H with mode 1) in the same pass, sharing every group walk
  * and coefficient read: for Hermitian instances that is the forward and the backward of a
- * training step in one launch. */
+ * training step in one pass. */
 int qmb_apply_batch(const qmb_batch* batch, int mode, int waves, void* stream);
 
 /* ---------------------------------------------------------------------------------------------

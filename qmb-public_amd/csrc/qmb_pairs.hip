@@ -2228,9 +2228,11 @@ __device__ void diag_rows(const BatchItemDev& it, int64_t r0, int lane) {
     }
 }
 
-template <int W, bool ADJ, int NV>
+// PART 1: the pair units (type 0), with the groups of 9..32 entries packed several to a wave;
+// PART 2: the single-excitation (type 1) and diagonal (type 2) units.
+template <int W, bool ADJ, int NV, int PART>
 __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int lane, char* stage) {
-    if (type == 0) {
+    if (PART == 1 && type == 0) {
         PairParams p{};
         p.coef = it.pcoef;
         p.coef_re = it.pcoef_re;
@@ -2247,9 +2249,9 @@ __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int 
         p.out2 = it.out2;
         const int64_t groups = it.pcounts[1];
         if (local * 64 < groups && it.pcounts[2] == 0) {
-            pair_chunk<W, ADJ, kPairBatchInBatch, NV, false>(p, groups, local * 64, lane, stage);
+            pair_chunk<W, ADJ, kPairBatchInBatch, NV, true>(p, groups, local * 64, lane, stage);
         }
-    } else if (type == 1) {
+    } else if (PART == 2 && type == 1) {
         SingleParams p{};
         p.a = it.sa;
         p.kd = it.skd;
@@ -2269,7 +2271,7 @@ __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int 
         if (local * 64 < groups && it.scounts[2] == 0) {
             single_chunk<W, ADJ, NV, 4, false>(p, groups, local * 64, lane);
         }
-    } else {
+    } else if (PART == 2 && type == 2) {
         diag_rows<W, ADJ, NV>(it, local * 64, lane);
         if (local == 0 && lane == 0 &&
             ((it.has_pairs && it.pcounts[2] != 0) || (it.has_singles && it.scounts[2] != 0))) {
@@ -2281,9 +2283,12 @@ __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int 
     }
 }
 
-// At most 128 VGPRs (4 waves per SIMD): the two-vector kernel otherwise takes 133 and runs 3 waves,
-// which measured ~20% slower on a 64-instance molecular batch than 4 waves with 2 spilled registers.
-template <bool ADJ, int NV>
+// One batch application is two persistent launches over the units, the pair units (PART 1), then the
+// single and diagonal units (PART 2): each kernel holds only its own unit code, so the pair units
+// keep the packed groups within 128 VGPRs (4 waves per SIMD; 3 waves measured ~20% slower).  As one
+// launch over all unit types the kernel spilled at that limit and had to walk the 9..32-entry groups
+// one per wave (64 molecular instances, forward + backward: 115.0 ms against 110.3 ms split).
+template <bool ADJ, int NV, int PART>
 __global__ __launch_bounds__(kBlock, 4) void batch_kernel(const BatchItemDev* __restrict__ items, const int64_t* __restrict__ offsets,
                                                        int n_items, int64_t total) {
     const int lane = threadIdx.x & 63;
@@ -2291,7 +2296,9 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(const BatchItemDev* __
     const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * kBlock) >> 6;
     __shared__ __align__(16) char stage[kPairStageBytes];
     const int slots = 3 * n_items;
-    for (int64_t u = wave; u < total; u += nwaves) {
+    const int64_t first = PART == 2 ? offsets[n_items] : 0;
+    const int64_t last = PART == 1 ? offsets[n_items] : total;
+    for (int64_t u = first + wave; u < last; u += nwaves) {
         // last slot whose first unit is <= u (offsets[slots] == total > u)
         int lo = 0, hi = slots;
         while (hi - lo > 1) {
@@ -2306,9 +2313,9 @@ __global__ __launch_bounds__(kBlock, 4) void batch_kernel(const BatchItemDev* __
         const BatchItemDev& it = items[lo - type * n_items];
         const int64_t local = u - offsets[lo];
         if (it.n_words == 1) {
-            batch_unit<1, ADJ, NV>(it, type, local, lane, stage);
+            batch_unit<1, ADJ, NV, PART>(it, type, local, lane, stage);
         } else {
-            batch_unit<2, ADJ, NV>(it, type, local, lane, stage);
+            batch_unit<2, ADJ, NV, PART>(it, type, local, lane, stage);
         }
     }
 }
@@ -3043,14 +3050,18 @@ int apply_batch(const qmb_batch* batch, int mode, int waves, hipStream_t stream)
     const unsigned g = static_cast<unsigned>(blocks);
     if (batch->dual) {
         if (mode == 1) {
-            batch_kernel<true, 2><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
+            batch_kernel<true, 2, 1><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
+            batch_kernel<true, 2, 2><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
         } else {
-            batch_kernel<false, 2><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
+            batch_kernel<false, 2, 1><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
+            batch_kernel<false, 2, 2><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
         }
     } else if (mode == 1) {
-        batch_kernel<true, 1><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
+        batch_kernel<true, 1, 1><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
+        batch_kernel<true, 1, 2><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
     } else {
-        batch_kernel<false, 1><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
+        batch_kernel<false, 1, 1><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
+        batch_kernel<false, 1, 2><<<g, kBlock, 0, stream>>>(items, batch->offsets, batch->n_items, batch->total_units);
     }
     QMB_CHECK_LAUNCH();
     return QMB_OK;

@@ -1,10 +1,11 @@
 """
-Batched application of many Hamiltonians to many config sets in one persistent launch.
+Batched application of many Hamiltonians to many config sets in two persistent launches.
 
 Each instance is a PreparedTerms (with the structured tables: pair and single excitations,
 diagonal records) and a PreparedTargets over its distinct config set.  The inputs and outputs of all
 instances are packed into two device buffers, so the plan (qmb_batch_plan) is built once and every
-launch (qmb_apply_batch) is a single persistent kernel over all instances' work; the whole step
+launch (qmb_apply_batch) is two persistent kernels over all instances' work (the pair units, then the
+single and diagonal units); the whole step
 can be captured in a hipGraph (torch.cuda.CUDAGraph) and replayed.  Instances whose tables keep
 terms outside the structured classes get those applied by one enumeration launch each after the
 batch.
@@ -12,7 +13,7 @@ batch.
 The autotuner picks two launch parameters: the number of persistent waves, and a split -- the
 `split` instances with the most core-index entries run their own structured launches (the
 single-set kernels, which fill the GPU alone and keep more registers than the batch kernel), the
-rest share the persistent launch (one plan per split, built on first use).
+rest share the batched launches (one plan per split, built on first use).
 """
 
 from __future__ import annotations
@@ -173,7 +174,7 @@ class Batch:
 
     # ------------------------------------------------------------------------------ launch
     def launch_both(self, waves: int | None = None) -> None:
-        """out = H in and out2 = H in2 for every (Hermitian) instance in ONE persistent launch, plus
+        """out = H in and out2 = H in2 for every (Hermitian) instance in one batched pass, plus
         the residual enumeration per instance that needs it.  Enqueue-only (graph capturable)."""
         if not self.two_vectors:
             raise RuntimeError("launch_both needs a batch built with two_vectors=True")
@@ -199,7 +200,7 @@ class Batch:
                                                      ctypes.byref(launch) if launch else None, stream), "qmb_apply_prepared")
 
     def launch(self, mode: int, waves: int | None = None) -> None:
-        """out = A in (mode 0) or A^H in (mode 1) for every instance: one persistent launch plus one
+        """out = A in (mode 0) or A^H in (mode 1) for every instance: one batched pass plus one
         enumeration launch per instance with residual terms.  Enqueue-only (graph capturable)."""
         if self.two_vectors:
             raise RuntimeError("a two-vector batch runs with launch_both()")
