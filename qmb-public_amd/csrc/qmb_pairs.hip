@@ -1193,8 +1193,7 @@ __device__ __forceinline__ void pair_chunk(const PairParams& p, int64_t groups, 
     pair_tiny_class<W, ADJ, NV>(p, __ballot(sz == 2u), gs, ge, 2, lane);
     // each wave owns 64 slots of the largest stage size: waves of one block may run different W
     PairStage<W>* stage = reinterpret_cast<PairStage<W>*>(stage_bytes + (threadIdx.x >> 6) * 64 * sizeof(PairStage<2>));
-    // PACK: groups of 9..32 entries several to a wave (the batch kernel, at its register limit,
-    // keeps them one per wave)
+    // PACK: groups of 9..32 entries several to a wave (else one group per wave)
     // a real table (wave-uniform) takes the loops specialised for 8-byte coefficients
     if (p.coef_re != nullptr) {
         pair_chunk_groups<W, ADJ, B, NV, PACK, true>(p, gs, ge, sz, lane, stage);
@@ -1662,7 +1661,7 @@ __device__ __forceinline__ void single_chunk(const SingleParams& p, int64_t grou
     single_tiny_class<W, ADJ, NV, CH>(p, __ballot(sz > 4u && sz <= 8u), gs, ge, 8, lane);
     single_tiny_class<W, ADJ, NV, CH>(p, __ballot(sz > 2u && sz <= 4u), gs, ge, 4, lane);
     single_tiny_class<W, ADJ, NV, CH>(p, __ballot(sz == 2u), gs, ge, 2, lane);
-    // PACK: groups of 9..32 entries several to a wave (not in the batch kernel: register limit)
+    // PACK: groups of 9..32 entries several to a wave (else one group per wave)
     if (PACK) {
         single_group_packed<W, ADJ, NV, CH, 16>(p, __ballot(sz > 8u && sz <= 16u), gs, ge, lane);
         single_group_packed<W, ADJ, NV, CH, 32>(p, __ballot(sz > 16u && sz <= 32u), gs, ge, lane);
@@ -2228,8 +2227,8 @@ __device__ void diag_rows(const BatchItemDev& it, int64_t r0, int lane) {
     }
 }
 
-// PART 1: the pair units (type 0), with the groups of 9..32 entries packed several to a wave;
-// PART 2: the single-excitation (type 1) and diagonal (type 2) units.
+// PART 1: the pair units (type 0); PART 2: the single-excitation (type 1) and diagonal (type 2)
+// units.  Both pack the groups of 9..32 entries several to a wave.
 template <int W, bool ADJ, int NV, int PART>
 __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int lane, char* stage) {
     if (PART == 1 && type == 0) {
@@ -2269,7 +2268,7 @@ __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int 
         p.out2 = it.out2;
         const int64_t groups = it.scounts[1];
         if (local * 64 < groups && it.scounts[2] == 0) {
-            single_chunk<W, ADJ, NV, 4, false>(p, groups, local * 64, lane);
+            single_chunk<W, ADJ, NV, 4, true>(p, groups, local * 64, lane);
         }
     } else if (PART == 2 && type == 2) {
         diag_rows<W, ADJ, NV>(it, local * 64, lane);
@@ -2284,10 +2283,10 @@ __device__ void batch_unit(const BatchItemDev& it, int type, int64_t local, int 
 }
 
 // One batch application is two persistent launches over the units, the pair units (PART 1), then the
-// single and diagonal units (PART 2): each kernel holds only its own unit code, so the pair units
-// keep the packed groups within 128 VGPRs (4 waves per SIMD; 3 waves measured ~20% slower).  As one
-// launch over all unit types the kernel spilled at that limit and had to walk the 9..32-entry groups
-// one per wave (64 molecular instances, forward + backward: 115.0 ms against 110.3 ms split).
+// single and diagonal units (PART 2): each kernel holds only its own unit code, so both keep the
+// packed groups within 128 VGPRs (4 waves per SIMD; 3 waves measured ~20% slower).  As one launch
+// over all unit types the kernel spilled at that limit and had to walk the 9..32-entry groups one
+// per wave (64 molecular instances, forward + backward: 115.0 ms against 108.2 ms split).
 template <bool ADJ, int NV, int PART>
 __global__ __launch_bounds__(kBlock, 4) void batch_kernel(const BatchItemDev* __restrict__ items, const int64_t* __restrict__ offsets,
                                                        int n_items, int64_t total) {
