@@ -377,12 +377,16 @@ def can_use_hops(terms: PreparedTerms, targets: PreparedTargets, configs_i: torc
     built here: a synchronisation) stay within HOP_PAIRS_PER_TERM_ROW per row and hop term."""
     if not terms.hops or terms.pairs_desc is not None or not targets.indexes(configs_i) or targets.duplicates() != 0:
         return False
-    # the two parities' indexes build side by side (one on a second stream), then one sync
+    # the two parities' indexes build side by side (one on a second stream), then one sync; the
+    # second stream first computes the rows' diagonal values the hop path adds next, a single kernel
+    # that fills the GPU while the host enqueues the indexes' many small launches
     if not targets.has_strings(1):
         device = targets.device
         side = _side_stream(device)
         side.wait_stream(torch.cuda.current_stream(device))
         with torch.cuda.stream(side):
+            if not targets.has_diagonal_values(terms):
+                targets.diagonal_values(terms).record_stream(torch.cuda.current_stream(device))
             targets.strings(1)
         targets._strings[1][0].record_stream(torch.cuda.current_stream(device))  # pylint: disable=protected-access
         targets.strings(0)
