@@ -356,8 +356,8 @@ __global__ void pair_inverse_kernel(int n_bits, uint32_t* __restrict__ inv) {
 // Flags of a used sorted core entry: the first entry of its key (a group head, bit 0) and an entry
 // whose core differs from the previous entry's under the same key (two cores merged by equal keys,
 // bit 31).  Unused slots (the sentinel keys, sorted last) get 0; the flag pass writes the number of
-// used entries to counts[0].  An exclusive scan of the flags with 32-bit wrap-around gives each
-// entry's group in bits 0..30 (capacity < 2^31), whatever the break bits add to bit 31.
+// used entries to counts[0] and each wave's number of heads to its chunk's counter; a head's group
+// is the heads before its chunk (an exclusive scan of the counters) plus those before it in the chunk.
 constexpr uint32_t kFlagHead = 1u;
 constexpr uint32_t kFlagBreak = 0x80000000u;
 // Bit 31 of starts[g] (g < G): group g holds more than one core (its rows' cores are compared exactly).
@@ -461,12 +461,32 @@ __device__ __forceinline__ void core_flags(const uint64_t* __restrict__ ckeys, i
     }
 }
 
+// Entries per flag chunk: one wave's kFlagUnroll * 64 consecutive entries.
+constexpr int kFlagChunk = 64 * kFlagUnroll;
+
+inline int64_t flag_chunks(int64_t cap) { return static_cast<int64_t>(blocks_for(cap, kBlock * kFlagUnroll)) * (kBlock / 64); }
+
+// chunk_heads[c] = number of group heads among the entries of chunk c (the wave's own).  Every wave
+// of the grid writes its chunk, also past the entries.
+__device__ __forceinline__ void store_chunk_heads(const uint32_t (&f)[kFlagUnroll], int64_t base, int lane,
+                                                  uint32_t* __restrict__ chunk_heads) {
+    uint32_t heads = 0;
+#pragma unroll
+    for (int t = 0; t < kFlagUnroll; ++t) {
+        heads += static_cast<uint32_t>(__popcll(__ballot((f[t] & kFlagHead) != 0u)));
+    }
+    if (lane == 0) {
+        chunk_heads[base / kFlagChunk] = heads;
+    }
+}
+
 // Flags of packed entries (key << 31 | row << sub_bits | sub, sorted on the key); also splits each
-// entry into its key and its value in the kernels' form (row << 16 | y << 8 | x).  Grid: whole
-// waves of kFlagUnroll * 64 entries.
+// entry into its key and its value in the kernels' form (row << 16 | y << 8 | x), and counts each
+// chunk's heads.  Grid: flag_chunks(cap) waves of kFlagChunk entries.
 template <int W, int ORDER>
 __global__ __launch_bounds__(kBlock) void core_flag_unpack_kernel(const uint64_t* __restrict__ entries, int64_t cap,
-                                                                  uint32_t* __restrict__ flag, int sub_bits, const uint32_t* __restrict__ inv,
+                                                                  uint32_t* __restrict__ flag, uint32_t* __restrict__ chunk_heads,
+                                                                  int sub_bits, const uint32_t* __restrict__ inv,
                                                                   const uint64_t* __restrict__ ckeys, uint32_t* __restrict__ keys,
                                                                   uint64_t* __restrict__ vals, int64_t* __restrict__ counts) {
     const int lane = threadIdx.x & 63;
@@ -479,6 +499,7 @@ __global__ __launch_bounds__(kBlock) void core_flag_unpack_kernel(const uint64_t
     uint32_t f[kFlagUnroll], k[kFlagUnroll];
     uint64_t v[kFlagUnroll];
     core_flags<W, ORDER>(ckeys, cap, load, f, k, v, base, lane, counts);
+    store_chunk_heads(f, base, lane, chunk_heads);
 #pragma unroll
     for (int t = 0; t < kFlagUnroll; ++t) {
         const int64_t e = base + 64 * t + lane;
@@ -490,11 +511,11 @@ __global__ __launch_bounds__(kBlock) void core_flag_unpack_kernel(const uint64_t
     }
 }
 
-// Flags of sorted (key, value) pairs, as core_flag_unpack_kernel.
+// Flags of sorted (key, value) pairs and each chunk's heads, as core_flag_unpack_kernel.
 template <int W, int ORDER>
 __global__ __launch_bounds__(kBlock) void core_flag_kernel(const uint32_t* __restrict__ keys, const uint64_t* __restrict__ vals, int64_t cap,
                                                            const uint64_t* __restrict__ ckeys, uint32_t* __restrict__ flag,
-                                                           int64_t* __restrict__ counts) {
+                                                           uint32_t* __restrict__ chunk_heads, int64_t* __restrict__ counts) {
     const int lane = threadIdx.x & 63;
     const int64_t base = (blockIdx.x * static_cast<int64_t>(blockDim.x) + (threadIdx.x & ~63)) * kFlagUnroll;
     auto load = [&](int64_t j, uint32_t& kj, uint64_t& vj) {
@@ -504,6 +525,7 @@ __global__ __launch_bounds__(kBlock) void core_flag_kernel(const uint32_t* __res
     uint32_t f[kFlagUnroll], k[kFlagUnroll];
     uint64_t v[kFlagUnroll];
     core_flags<W, ORDER>(ckeys, cap, load, f, k, v, base, lane, counts);
+    store_chunk_heads(f, base, lane, chunk_heads);
 #pragma unroll
     for (int t = 0; t < kFlagUnroll; ++t) {
         const int64_t e = base + 64 * t + lane;
@@ -524,61 +546,74 @@ __global__ void key_flag_kernel(const uint32_t* __restrict__ keys, int64_t n, ui
 }
 
 // starts[g] = first entry of group g, with kGroupMixed when a break lies inside the group;
-// starts[G] = number of used entries; counts {entries, G}.  gid = exclusive scan of the flags.
-// One wave reads 64 consecutive flags: a head finds its group's end (the next head or the end of
-// the used entries) and the breaks before it from the wave's ballots; the last head of the wave,
-// whose group may run past the wave, reads on 64 flags at a time with the whole wave.  Launched
-// with whole waves.
-__global__ __launch_bounds__(kBlock) void core_start_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ gid,
+// starts[G] = number of used entries; counts {entries, G}.  One wave per flag chunk, 64 flags at a
+// time: chunk_off[c] = heads before chunk c (the exclusive scan of the chunks' head counts), so a
+// head's group is that plus the heads before it in the chunk.  A head finds its group's end (the
+// next head or the end of the used entries) and the breaks before it from the wave's ballots; the
+// last head of the 64, whose group may run past them, reads on 64 flags at a time with the whole
+// wave.  Grid: flag_chunks(capacity) waves.
+__global__ __launch_bounds__(kBlock) void core_start_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ chunk_off,
                                                             uint32_t* __restrict__ starts, int64_t* __restrict__ counts) {
     const int lane = threadIdx.x & 63;
     const int64_t used = counts[0];
-    const int64_t base = blockIdx.x * static_cast<int64_t>(blockDim.x) + (threadIdx.x & ~63);
-    const int64_t e = base + lane;
-    const uint32_t f = e < used ? flag[e] : 0u;
-    const uint64_t stop = __ballot(e >= used || (f & kFlagHead) != 0u);
-    const uint64_t brk = __ballot((f & kFlagBreak) != 0u);
-    const bool head = (f & kFlagHead) != 0u;
+    const int64_t chunk = (blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x) >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;                    // lanes below this one
     const uint64_t after = lane == 63 ? 0ull : ~0ull << (lane + 1);  // lanes above this one
-    const uint64_t next = stop & after;
-    // the group of the wave's last head runs on past the wave when no stop follows it
-    const bool open = head && next == 0ull;
-    const uint64_t open_mask = __ballot(open);
-    bool tail_break = false;
-    int64_t tail_end = 0;
-    if (open_mask != 0ull) {
-        for (int64_t j0 = base + 64;; j0 += 64) {
-            const int64_t j = j0 + lane;
-            const uint32_t fj = j < used ? flag[j] : 0u;
-            const uint64_t st = __ballot(j >= used || (fj & kFlagHead) != 0u);
-            const uint64_t bk = __ballot((fj & kFlagBreak) != 0u);
-            if (st != 0ull) {
-                const int nx = __builtin_ctzll(st);
-                tail_break = tail_break || (bk & ((1ull << nx) - 1ull)) != 0ull;
-                tail_end = j0 + nx;
-                break;
-            }
-            tail_break = tail_break || bk != 0ull;
+    if (chunk * kFlagChunk >= used) {
+        return;  // the whole wave
+    }
+    uint32_t g0 = chunk_off[chunk];
+    for (int t = 0; t < kFlagUnroll; ++t) {
+        const int64_t base = chunk * kFlagChunk + 64 * t;
+        if (base >= used) {
+            break;  // the whole wave
         }
-    }
-    if (!head) {
-        return;
-    }
-    bool mixed;
-    int64_t end;
-    if (open) {
-        mixed = (brk & after) != 0ull || tail_break;
-        end = tail_end;
-    } else {
-        const int nx = __builtin_ctzll(next);
-        mixed = (brk & after & ((1ull << nx) - 1ull)) != 0ull;
-        end = base + nx;
-    }
-    const uint32_t g = gid[e] & kGroupStart;
-    starts[g] = static_cast<uint32_t>(e) | (mixed ? kGroupMixed : 0u);
-    if (end == used) {
-        starts[g + 1] = static_cast<uint32_t>(end);
-        counts[1] = static_cast<int64_t>(g) + 1;
+        const int64_t e = base + lane;
+        const uint32_t f = e < used ? flag[e] : 0u;
+        const bool head = (f & kFlagHead) != 0u;
+        const uint64_t heads = __ballot(head);
+        const uint64_t stop = __ballot(e >= used || head);
+        const uint64_t brk = __ballot((f & kFlagBreak) != 0u);
+        const uint64_t next = stop & after;
+        // the group of the 64's last head runs on past them when no stop follows it
+        const bool open = head && next == 0ull;
+        const uint64_t open_mask = __ballot(open);
+        bool tail_break = false;
+        int64_t tail_end = 0;
+        if (open_mask != 0ull) {
+            for (int64_t j0 = base + 64;; j0 += 64) {
+                const int64_t j = j0 + lane;
+                const uint32_t fj = j < used ? flag[j] : 0u;
+                const uint64_t st = __ballot(j >= used || (fj & kFlagHead) != 0u);
+                const uint64_t bk = __ballot((fj & kFlagBreak) != 0u);
+                if (st != 0ull) {
+                    const int nx = __builtin_ctzll(st);
+                    tail_break = tail_break || (bk & ((1ull << nx) - 1ull)) != 0ull;
+                    tail_end = j0 + nx;
+                    break;
+                }
+                tail_break = tail_break || bk != 0ull;
+            }
+        }
+        if (head) {
+            bool mixed;
+            int64_t end;
+            if (open) {
+                mixed = (brk & after) != 0ull || tail_break;
+                end = tail_end;
+            } else {
+                const int nx = __builtin_ctzll(next);
+                mixed = (brk & after & ((1ull << nx) - 1ull)) != 0ull;
+                end = base + nx;
+            }
+            const uint32_t g = g0 + static_cast<uint32_t>(__popcll(heads & below));
+            starts[g] = static_cast<uint32_t>(e) | (mixed ? kGroupMixed : 0u);
+            if (end == used) {
+                starts[g + 1] = static_cast<uint32_t>(end);
+                counts[1] = static_cast<int64_t>(g) + 1;
+            }
+        }
+        g0 += static_cast<uint32_t>(__popcll(heads));
     }
 }
 
@@ -646,7 +681,8 @@ int cores_layout(Carver& cv, int n_qubytes, int64_t n, int max_occupied, int ord
     }
     const size_t s0 = cv.off;
     const size_t inv_bytes = static_cast<size_t>(8 * n_qubytes) * (8 * n_qubytes) / 2 * sizeof(uint32_t) + 256;
-    void* s = cv.take_bytes(2 * cap * sizeof(uint64_t) + 2 * cap * sizeof(uint32_t) + inv_bytes + tmp + 4096);
+    const size_t chunk_bytes = 2 * static_cast<size_t>(flag_chunks(static_cast<int64_t>(cap))) * sizeof(uint32_t) + 512;
+    void* s = cv.take_bytes(2 * cap * sizeof(uint64_t) + cap * sizeof(uint32_t) + chunk_bytes + inv_bytes + tmp + 4096);
     if (scratch) {
         *scratch = s;
     }
@@ -2416,7 +2452,9 @@ int cores_build(const qmb_targets* targets, int max_occupied, int order, void* b
         uint32_t* keys_in = reinterpret_cast<uint32_t*>(entries_in);
         uint64_t* vals_in = sc.take<uint64_t>(cap);
         uint32_t* flag = sc.take<uint32_t>(cap);
-        uint32_t* gid = sc.take<uint32_t>(cap);
+        const int64_t chunks = flag_chunks(c.capacity);
+        uint32_t* chunk_heads = sc.take<uint32_t>(static_cast<size_t>(chunks));
+        uint32_t* chunk_off = sc.take<uint32_t>(static_cast<size_t>(chunks));
         const int n_bits = 8 * c.n_qubytes;
         uint32_t* inv = sc.take<uint32_t>(static_cast<size_t>(n_bits) * n_bits / 2);
         size_t tmp_bytes = scratch_bytes > sc.off + 512 ? scratch_bytes - sc.off - 512 : 0;
@@ -2464,8 +2502,8 @@ int cores_build(const qmb_targets* targets, int max_occupied, int order, void* b
                 QMB_CHECK_LAUNCH();
             }
             auto unpack = [&](auto kernel) {
-                kernel<<<blocks_for(c.capacity, kBlock * kFlagUnroll), kBlock, 0, stream>>>(vals_in, c.capacity, flag, sub_bits, inv,
-                                                                                         targets->keys, c.keys, c.vals, c.counts);
+                kernel<<<blocks_for(c.capacity, kBlock * kFlagUnroll), kBlock, 0, stream>>>(vals_in, c.capacity, flag, chunk_heads, sub_bits,
+                                                                                         inv, targets->keys, c.keys, c.vals, c.counts);
             };
             switch (shape) {
                 case 2: unpack(&core_flag_unpack_kernel<1, 1>); break;
@@ -2482,7 +2520,7 @@ int cores_build(const qmb_targets* targets, int max_occupied, int order, void* b
                                                   stream));
             auto flags = [&](auto kernel) {
                 kernel<<<blocks_for(c.capacity, kBlock * kFlagUnroll), kBlock, 0, stream>>>(c.keys, c.vals, c.capacity, targets->keys,
-                                                                                         flag, c.counts);
+                                                                                         flag, chunk_heads, c.counts);
             };
             switch (shape) {
                 case 2: flags(&core_flag_kernel<1, 1>); break;
@@ -2497,9 +2535,10 @@ int cores_build(const qmb_targets* targets, int max_occupied, int order, void* b
         }
         QMB_CHECK_LAUNCH();
         need = tmp_bytes;
-        QMB_HIP_TRY(rocprim::exclusive_scan(tmp, need, flag, gid, 0u, cap, rocprim::plus<uint32_t>(), stream));
-        // whole waves: the start kernel reads the flags of 64 entries per wave (unused past cap)
-        core_start_kernel<<<blocks_for(c.capacity), kBlock, 0, stream>>>(flag, gid, c.starts, c.counts);
+        QMB_HIP_TRY(rocprim::exclusive_scan(tmp, need, chunk_heads, chunk_off, 0u, static_cast<size_t>(chunks), rocprim::plus<uint32_t>(),
+                                            stream));
+        // one wave per flag chunk, as the flag kernel
+        core_start_kernel<<<blocks_for(c.capacity, kBlock * kFlagUnroll), kBlock, 0, stream>>>(flag, chunk_off, c.starts, c.counts);
         c.key_bits = kKeyBits;
         QMB_CHECK_LAUNCH();
     }
