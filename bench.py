@@ -132,8 +132,8 @@ def structured_model(pp: object, terms: object, targets: object, nq: int, n: int
       diag_apply:    per row its config words, amplitude and output (8 W + 32); the folded table once.
     Per core index build (packed entries): every entry emitted (8 written), sorted once (8 read + 8
     written), split and flagged (8 read; 4 key + 8 value + 4 flag written; the config words of each
-    entry of a group of >= 2 entries read, 8 W), its flag scanned (4 read + 4 written) and read for
-    the starts (4), and 4 per group start -- the sort's extra passes and the hash index are not
+    entry of a group of >= 2 entries read, 8 W), its flag read again for the starts (4), and 4 per
+    group start -- the sort's extra passes, the per-chunk head counts and the hash index are not
     counted."""
     words = (nq + 7) // 8
     out: dict[str, object] = {}
@@ -144,7 +144,7 @@ def structured_model(pp: object, terms: object, targets: object, nq: int, n: int
         return 4 * st["groups"] + 8 * words * st["multi_groups"] + 24 * st["multi_entries"]
 
     def build(st: dict[str, int]) -> int:
-        return 60 * st["entries"] + 8 * words * st["multi_entries"] + 4 * st["groups"]
+        return 52 * st["entries"] + 8 * words * st["multi_entries"] + 4 * st["groups"]
 
     st2 = pp.core_stats(targets, 2)
     out["pair_kernel"] = dict(st2, bytes=walk(st2) + coef_b * terms.pairs_desc.n_pairs ** 2 + 16 * n)

@@ -45,8 +45,8 @@ def test_structured_model_terms(bench: types.ModuleType, nq: int, real: bool) ->
     assert out["pair_kernel"]["bytes"] == 4 * 355 + 8 * words * 125 + 24 * 680 + coef * n_pairs ** 2 + 16 * n
     assert out["single_kernel"]["bytes"] == 4 * 93 + 8 * words * 20 + 24 * 66 + coef * (n_bits ** 2 + n_bits ** 3) + 16 * n
     assert out["diag_apply_kernel"]["bytes"] == n * (8 * words + 32) + 16 * (1 + n_bits + n_pairs)
-    # per build: 60 per entry, a config per entry of a group of >= 2, a start per group
-    assert out["core_index_build"]["bytes"] == sum(60 * s["entries"] + 8 * words * s["multi_entries"] + 4 * s["groups"]
+    # per build: 52 per entry, a config per entry of a group of >= 2, a start per group
+    assert out["core_index_build"]["bytes"] == sum(52 * s["entries"] + 8 * words * s["multi_entries"] + 4 * s["groups"]
                                                    for s in st.values())
     assert out["pair_kernel"]["multi_groups"] == 125
 
