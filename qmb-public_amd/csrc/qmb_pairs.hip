@@ -1079,47 +1079,82 @@ __device__ void pair_group_packed(const PairParams& p, uint64_t mask, uint32_t g
     }
 }
 
-// Groups of 2..8 entries, one size class at a time (gp = 8, 4, 2 lanes per row): one lane per
-// (row a, entry b) of a gp x gp block, 64 / gp^2 groups per pass.  Each lane checks its pair exactly
-// (cores compared through a shuffle from the lane holding row b), the rows are summed over b with a
-// butterfly inside the block, and the first lane of each row adds the sum.  `mask` holds the lanes
-// of the 64-group chunk whose group is in this class; (gs_l, ge_l) is that lane's group.
+// A pass over groups of 2..8 entries with their ordered (row, entry) pairs side by side: whole
+// groups are taken from `mask` in lane order while their pairs fit the 64 lanes.  A group of g
+// entries takes g (g - 1) lanes from `lead`, row a's g - 1 pairs consecutive (entry b = j, or j + 1
+// from j = a on), so no lane pairs a row with itself; a 3-entry group takes 6 lanes instead of a
+// 4 x 4 block.  Lanes past the pass's groups get s = e = 0.
+struct FlatLane {
+    uint32_t s, e;  // the lane's group (s with its kGroupMixed bit)
+    int ai, bi;     // row and entry in the group
+    int j, d;       // position in row ai's run of d = g - 1 lanes
+    int lead;       // lane of the group's first pair (row 0, j = 0)
+    int rowb;       // lane of row bi's first pair
+};
+
+__device__ __forceinline__ FlatLane flat_pass(uint64_t& mask, uint32_t gs_l, uint32_t ge_l, int lane) {
+    FlatLane f{0u, 0u, 0, 0, 0, 1, 0, 0};
+    int used = 0;
+    while (mask) {
+        const int l = __builtin_ctzll(mask);
+        const uint32_t sj = __builtin_amdgcn_readlane(gs_l, l);
+        const uint32_t ej = __builtin_amdgcn_readlane(ge_l, l);
+        const int nj = static_cast<int>(ej - (sj & kGroupStart));
+        const int len = nj * (nj - 1);
+        if (used + len > 64) {
+            break;
+        }
+        mask &= mask - 1;
+        if (lane >= used && lane < used + len) {
+            f.s = sj;
+            f.e = ej;
+            f.lead = used;
+        }
+        used += len;
+    }
+    const int g = static_cast<int>(f.e - (f.s & kGroupStart));
+    f.d = g > 1 ? g - 1 : 1;
+    const uint32_t k = static_cast<uint32_t>(lane - f.lead);
+    f.ai = static_cast<int>(k / static_cast<uint32_t>(f.d));
+    f.j = static_cast<int>(k) - f.ai * f.d;
+    f.bi = f.j + (f.j >= f.ai ? 1 : 0);
+    f.rowb = f.lead + f.bi * f.d;
+    return f;
+}
+
+// Sums each row's run of d lanes into its first lane (j = 0); d <= 7.
+__device__ __forceinline__ void flat_row_sum(double& re, double& im, int j, int d) {
+    for (int m = 1; m < 8; m <<= 1) {
+        const double ore = __shfl_down(re, m);
+        const double oim = __shfl_down(im, m);
+        if (j + m < d) {
+            re += ore;
+            im += oim;
+        }
+    }
+}
+
+// Groups of 2..8 entries laid out by flat_pass: one lane per ordered (row a, entry b) pair.  Each
+// lane checks its pair exactly (cores compared through a shuffle from the lane holding row b), the
+// rows are summed over their runs, and the first lane of each row adds the sum.  `mask` holds the
+// lanes of the 64-group chunk whose group has 2..8 entries; (gs_l, ge_l) is that lane's group.
 template <int W, bool ADJ, int NV>
-__device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_l, uint32_t ge_l, int gp, int lane) {
-    const int lg = gp == 8 ? 3 : (gp == 4 ? 2 : 1);
-    const int bs = gp * gp;
-    const int per = 64 / bs;
-    const int blk = lane >> (2 * lg);
-    const int sub = lane & (bs - 1);
-    const int ai = sub >> lg;
-    const int bi = sub & (gp - 1);
+__device__ void pair_flat_class(const PairParams& p, uint64_t mask, uint32_t gs_l, uint32_t ge_l, int lane) {
     const bool fermi = p.particle_cut == 1;
     while (mask) {
-        uint32_t s = 0, e = 0;
-        for (int j = 0; j < per && mask != 0ull; ++j) {
-            const int l = __builtin_ctzll(mask);
-            mask &= mask - 1;
-            const uint32_t sj = __builtin_amdgcn_readlane(gs_l, l);
-            const uint32_t ej = __builtin_amdgcn_readlane(ge_l, l);
-            if (blk == j) {
-                s = sj;
-                e = ej;
-            }
-        }
-        const bool mixed = (s & kGroupMixed) != 0u;
-        s &= kGroupStart;
-        const uint32_t g = e - s;
-        const bool rowa = static_cast<uint32_t>(ai) < g;
-        const bool rowb = static_cast<uint32_t>(bi) < g;
-        const uint64_t va = rowa ? p.vals[s + ai] : 0ull;
-        const uint64_t vb = rowb ? p.vals[s + bi] : 0ull;
+        const FlatLane f = flat_pass(mask, gs_l, ge_l, lane);
+        const bool mixed = (f.s & kGroupMixed) != 0u;
+        const uint32_t s = f.s & kGroupStart;
+        const bool act = f.e > s;
+        const uint64_t va = act ? p.vals[s + f.ai] : 0ull;
+        const uint64_t vb = act ? p.vals[s + f.bi] : 0ull;
         const int64_t ia = static_cast<int64_t>(va >> 16);
         const int r0 = static_cast<int>(va & 0xff), r1 = static_cast<int>((va >> 8) & 0xff);
         const int q0 = static_cast<int>(vb & 0xff), q1 = static_cast<int>((vb >> 8) & 0xff);
-        // Everything that needs only the entries: the coefficient and the input row are loaded now,
-        // beside the stored configs, and dropped below when the cores differ (entry 0 of the
-        // table and row 0 of the input are always there).
-        const bool pre = rowa && rowb && ai != bi && q0 != r0 && q0 != r1 && q1 != r0 && q1 != r1;
+        // the coefficient and the input row need only the entries: loaded beside the stored configs
+        // and dropped below when the cores differ (entry 0 of the table and row 0 of the input are
+        // always there)
+        const bool pre = act && q0 != r0 && q0 != r1 && q1 != r0 && q1 != r1;
         const int64_t idx = !pre ? 0
                                  : (ADJ ? pair_slot(r0, r1, p.n_bits) * p.n_pairs + pair_slot(q0, q1, p.n_bits)
                                         : pair_slot(q0, q1, p.n_bits) * p.n_pairs + pair_slot(r0, r1, p.n_bits));
@@ -1130,27 +1165,25 @@ __device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_
         for (int v = 0; v < NV; ++v) {
             gv[v] = (v == 0 ? p.in : p.in2)[ib];
         }
-        // the rows of the group's first row read its config for the group (a mixed group: every row
-        // its own)
+        // the lanes of the group's first row read its config for the group (a mixed group: every
+        // row its own)
         Cfg<W> own;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            own.w[w] = rowa && (mixed || ai == 0) ? p.ckeys[ia * W + w] : 0ull;
+            own.w[w] = act && (mixed || f.ai == 0) ? p.ckeys[ia * W + w] : 0ull;
         }
-        if (rowa) {
+        if (act) {
             flip_site<W>(own, r0);
             flip_site<W>(own, r1);
         }
-        // the group's first core, and the core of row b (computed by the lanes whose row is b);
         // every lane runs every shuffle (a short-circuit would leave lanes out)
-        const int src = (blk << (2 * lg)) + (bi << lg);
         Cfg<W> core;
         bool same = true;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            const uint64_t c0 = shfl_u64(own.w[w], blk << (2 * lg));
+            const uint64_t c0 = shfl_u64(own.w[w], f.lead);
             core.w[w] = mixed ? own.w[w] : c0;
-            const bool eq = shfl_u64(own.w[w], src) == own.w[w];
+            const bool eq = shfl_u64(own.w[w], f.rowb) == own.w[w];
             same = same & eq;
         }
         const bool ok = pre && (same || !mixed);
@@ -1183,11 +1216,8 @@ __device__ void pair_tiny_class(const PairParams& p, uint64_t mask, uint32_t gs_
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            for (int m = 1; m < gp; m <<= 1) {
-                acc_re[v] += __shfl_xor(acc_re[v], m);
-                acc_im[v] += __shfl_xor(acc_im[v], m);
-            }
-            if (bi == 0 && rowa && (acc_re[v] != 0.0 || acc_im[v] != 0.0)) {
+            flat_row_sum(acc_re[v], acc_im[v], f.j, f.d);
+            if (f.j == 0 && act && (acc_re[v] != 0.0 || acc_im[v] != 0.0)) {
                 double2* o = v == 0 ? p.out : p.out2;
                 unsafeAtomicAdd(&o[ia].x, acc_re[v]);
                 unsafeAtomicAdd(&o[ia].y, acc_im[v]);
@@ -1213,8 +1243,8 @@ __device__ __forceinline__ void pair_chunk_groups(const PairParams& p, uint32_t 
     }
 }
 
-// The pair work of the groups [g0, g0 + 64): groups of up to 8 entries through the tiny-group
-// classes, larger ones a row block at a time (pair_group).
+// The pair work of the groups [g0, g0 + 64): groups of up to 8 entries with their ordered pairs
+// side by side (pair_flat_class), larger ones packed or a row block at a time (pair_group).
 template <int W, bool ADJ, int B, int NV, bool PACK = true>
 __device__ __forceinline__ void pair_chunk(const PairParams& p, int64_t groups, int64_t g0, int lane, char* stage_bytes) {
     const int64_t gg = g0 + lane;
@@ -1224,9 +1254,7 @@ __device__ __forceinline__ void pair_chunk(const PairParams& p, int64_t groups, 
         ge = p.starts[gg + 1] & kGroupStart;
     }
     const uint32_t sz = ge - (gs & kGroupStart);
-    pair_tiny_class<W, ADJ, NV>(p, __ballot(sz > 4u && sz <= 8u), gs, ge, 8, lane);
-    pair_tiny_class<W, ADJ, NV>(p, __ballot(sz > 2u && sz <= 4u), gs, ge, 4, lane);
-    pair_tiny_class<W, ADJ, NV>(p, __ballot(sz == 2u), gs, ge, 2, lane);
+    pair_flat_class<W, ADJ, NV>(p, __ballot(sz >= 2u && sz <= 8u), gs, ge, lane);
     // each wave owns 64 slots of the largest stage size: waves of one block may run different W
     PairStage<W>* stage = reinterpret_cast<PairStage<W>*>(stage_bytes + (threadIdx.x >> 6) * 64 * sizeof(PairStage<2>));
     // PACK: groups of 9..32 entries several to a wave (else one group per wave)
@@ -1579,76 +1607,56 @@ __device__ void single_group_packed(const SingleParams& p, uint64_t mask, uint32
     }
 }
 
-// Single-excitation groups of 2..8 entries: the lane-per-(row, entry) scheme of pair_tiny_class.
+// Single-excitation groups of 2..8 entries laid out by flat_pass, as pair_flat_class.
 template <int W, bool ADJ, int NV, int CH>
-__device__ void single_tiny_class(const SingleParams& p, uint64_t mask, uint32_t gs_l, uint32_t ge_l, int gp, int lane) {
-    const int lg = gp == 8 ? 3 : (gp == 4 ? 2 : 1);
-    const int bs = gp * gp;
-    const int per = 64 / bs;
-    const int blk = lane >> (2 * lg);
-    const int sub = lane & (bs - 1);
-    const int ai = sub >> lg;
-    const int bi = sub & (gp - 1);
+__device__ void single_flat_class(const SingleParams& p, uint64_t mask, uint32_t gs_l, uint32_t ge_l, int lane) {
     const bool fermi = p.particle_cut == 1;
     while (mask) {
-        uint32_t s = 0, e = 0;
-        for (int j = 0; j < per && mask != 0ull; ++j) {
-            const int l = __builtin_ctzll(mask);
-            mask &= mask - 1;
-            const uint32_t sj = __builtin_amdgcn_readlane(gs_l, l);
-            const uint32_t ej = __builtin_amdgcn_readlane(ge_l, l);
-            if (blk == j) {
-                s = sj;
-                e = ej;
-            }
-        }
-        const bool mixed = (s & kGroupMixed) != 0u;
-        s &= kGroupStart;
-        const uint32_t g = e - s;
-        const bool rowa = static_cast<uint32_t>(ai) < g;
-        const bool rowb = static_cast<uint32_t>(bi) < g;
-        const uint64_t va = rowa ? p.vals[s + ai] : 0ull;
-        const uint64_t vb = rowb ? p.vals[s + bi] : 0ull;
+        const FlatLane f = flat_pass(mask, gs_l, ge_l, lane);
+        const bool mixed = (f.s & kGroupMixed) != 0u;
+        const uint32_t s = f.s & kGroupStart;
+        const bool act = f.e > s;
+        const uint64_t va = act ? p.vals[s + f.ai] : 0ull;
+        const uint64_t vb = act ? p.vals[s + f.bi] : 0ull;
         const int64_t ia = static_cast<int64_t>(va >> 16);
         const int r = static_cast<int>(va & 0xff);
         const int q = static_cast<int>(vb & 0xff);
         // the input row needs only the entry: loaded beside the stored configs (row 0 when unused)
-        const bool pre = rowa && rowb && ai != bi && q != r;
+        const bool pre = act && q != r;
         const int64_t ib = pre ? static_cast<int64_t>(vb >> 16) : 0;
         double2 gv[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             gv[v] = (v == 0 ? p.in : p.in2)[ib];
         }
-        // the rows of the group's first row read its config for the group (a mixed group: every row
-        // its own)
+        // the lanes of the group's first row read its config for the group (a mixed group: every
+        // row its own)
         Cfg<W> own;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            own.w[w] = rowa && (mixed || ai == 0) ? p.ckeys[ia * W + w] : 0ull;
+            own.w[w] = act && (mixed || f.ai == 0) ? p.ckeys[ia * W + w] : 0ull;
         }
-        if (rowa) {
+        if (act) {
             flip_site<W>(own, r);
         }
         // every lane runs every shuffle (a short-circuit would leave lanes out)
-        const int src = (blk << (2 * lg)) + (bi << lg);
         Cfg<W> core;
         bool same = true;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            const uint64_t c0 = shfl_u64(own.w[w], blk << (2 * lg));
+            const uint64_t c0 = shfl_u64(own.w[w], f.lead);
             core.w[w] = mixed ? own.w[w] : c0;
-            const bool eq = shfl_u64(own.w[w], src) == own.w[w];
+            const bool eq = shfl_u64(own.w[w], f.rowb) == own.w[w];
             same = same & eq;
         }
-        Cfg<W> ca = core;
-        flip_site<W>(ca, r);
         const bool ok = pre && (same || !mixed);
         double acc_re[NV] = {}, acc_im[NV] = {};
         if (ok) {
             double2 x;
             int sg;
             if (ADJ) {
+                Cfg<W> ca = core;
+                flip_site<W>(ca, r);
                 x = single_element<W, CH>(p, core, r, q);
                 sg = fermi ? single_parity<W>(ca, r, q) : 0;
             } else {
@@ -1672,11 +1680,8 @@ __device__ void single_tiny_class(const SingleParams& p, uint64_t mask, uint32_t
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            for (int m = 1; m < gp; m <<= 1) {
-                acc_re[v] += __shfl_xor(acc_re[v], m);
-                acc_im[v] += __shfl_xor(acc_im[v], m);
-            }
-            if (bi == 0 && rowa && (acc_re[v] != 0.0 || acc_im[v] != 0.0)) {
+            flat_row_sum(acc_re[v], acc_im[v], f.j, f.d);
+            if (f.j == 0 && act && (acc_re[v] != 0.0 || acc_im[v] != 0.0)) {
                 double2* o = v == 0 ? p.out : p.out2;
                 unsafeAtomicAdd(&o[ia].x, acc_re[v]);
                 unsafeAtomicAdd(&o[ia].y, acc_im[v]);
@@ -1694,9 +1699,7 @@ __device__ __forceinline__ void single_chunk(const SingleParams& p, int64_t grou
         ge = p.starts[gg + 1] & kGroupStart;
     }
     const uint32_t sz = ge - (gs & kGroupStart);
-    single_tiny_class<W, ADJ, NV, CH>(p, __ballot(sz > 4u && sz <= 8u), gs, ge, 8, lane);
-    single_tiny_class<W, ADJ, NV, CH>(p, __ballot(sz > 2u && sz <= 4u), gs, ge, 4, lane);
-    single_tiny_class<W, ADJ, NV, CH>(p, __ballot(sz == 2u), gs, ge, 2, lane);
+    single_flat_class<W, ADJ, NV, CH>(p, __ballot(sz >= 2u && sz <= 8u), gs, ge, lane);
     // PACK: groups of 9..32 entries several to a wave (else one group per wave)
     if (PACK) {
         single_group_packed<W, ADJ, NV, CH, 16>(p, __ballot(sz > 8u && sz <= 16u), gs, ge, lane);
