@@ -989,37 +989,52 @@ __device__ void pair_group(const PairParams& p, uint32_t s_raw, uint32_t e, int 
     }
 }
 
-// Groups of 9..H entries (H = 16 or 32), 64 / H of them per pass: lane l serves row l % H of group
-// l / H.  A group's entries are its rows, so each lane stages its own entry in its LDS slot and the
-// rows of a group walk the slots of that group, as in pair_group.  Packing keeps the lanes busy
-// where one group per wave would leave most of them idle (a 20-entry group uses 20 of 64).
-template <int W, bool ADJ, int B, int NV, int H, bool REAL>
-__device__ void pair_group_packed(const PairParams& p, uint64_t mask, uint32_t gs_l, uint32_t ge_l, int lane, PairStage<W>* stage) {
-    constexpr int kPer = 64 / H;
-    const int gi = lane / H;
-    const int ri = lane % H;
+// A pass over groups of 9..32 entries, lane = row: whole groups are taken from `mask` in lane order
+// while their rows fit the 64 lanes, a group's rows consecutive from `lead`.  nmax: the largest
+// group of the pass.  Lanes past the pass's groups get s = e = 0.
+struct RowLane {
+    uint32_t s, e;  // the lane's group (s with its kGroupMixed bit)
+    int lead;       // lane of the group's first row
+    int nmax;
+};
+
+__device__ __forceinline__ RowLane row_pass(uint64_t& mask, uint32_t gs_l, uint32_t ge_l, int lane) {
+    RowLane f{0u, 0u, 0, 0};
+    int used = 0;
+    while (mask) {
+        const int l = __builtin_ctzll(mask);
+        const uint32_t sj = __builtin_amdgcn_readlane(gs_l, l);
+        const uint32_t ej = __builtin_amdgcn_readlane(ge_l, l);
+        const int nj = static_cast<int>(ej - (sj & kGroupStart));
+        if (used + nj > 64) {
+            break;
+        }
+        mask &= mask - 1;
+        if (lane >= used && lane < used + nj) {
+            f.s = sj;
+            f.e = ej;
+            f.lead = used;
+        }
+        used += nj;
+        f.nmax = nj > f.nmax ? nj : f.nmax;
+    }
+    return f;
+}
+
+// Groups of 9..32 entries laid out by row_pass (a 9-entry group takes 9 lanes, so 7 of them share a
+// pass).  A group's entries are its rows, so each lane stages its own entry in its LDS slot and the
+// rows of a group walk the slots of that group, as in pair_group.
+template <int W, bool ADJ, int B, int NV, bool REAL>
+__device__ void pair_group_rows(const PairParams& p, uint64_t mask, uint32_t gs_l, uint32_t ge_l, int lane, PairStage<W>* stage) {
     const bool fermi = p.particle_cut == 1;
     while (mask) {
-        uint32_t s = 0, e = 0;
-        int nmax = 0;
-        for (int j = 0; j < kPer && mask != 0ull; ++j) {
-            const int l = __builtin_ctzll(mask);
-            mask &= mask - 1;
-            const uint32_t sj = __builtin_amdgcn_readlane(gs_l, l);
-            const uint32_t ej = __builtin_amdgcn_readlane(ge_l, l);
-            const int nj = static_cast<int>(ej - (sj & kGroupStart));
-            nmax = nj > nmax ? nj : nmax;
-            if (gi == j) {
-                s = sj;
-                e = ej;
-            }
-        }
-        const bool mixed = (s & kGroupMixed) != 0u;
-        s &= kGroupStart;
-        const int g = static_cast<int>(e - s);  // 0 for a slot without a group
+        const RowLane f = row_pass(mask, gs_l, ge_l, lane);
+        const bool mixed = (f.s & kGroupMixed) != 0u;
+        const uint32_t s = f.s & kGroupStart;
+        const int g = static_cast<int>(f.e - s);  // 0 for a lane without a group
+        const int ri = lane - f.lead;
         const bool act = ri < g;
-        const uint32_t a = s + static_cast<uint32_t>(ri);
-        const uint64_t va = act ? p.vals[a] : 0ull;
+        const uint64_t va = act ? p.vals[s + static_cast<uint32_t>(ri)] : 0ull;
         const int64_t ia = static_cast<int64_t>(va >> 16);
         const int r0 = static_cast<int>(va & 0xff);
         const int r1 = static_cast<int>((va >> 8) & 0xff);
@@ -1037,7 +1052,7 @@ __device__ void pair_group_packed(const PairParams& p, uint64_t mask, uint32_t g
         Cfg<W> core0;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            core0.w[w] = shfl_u64(own.w[w], gi * H);
+            core0.w[w] = shfl_u64(own.w[w], f.lead);
         }
         const PairRow<W> ra = pair_row<W>(p, mixed ? own : core0, r0, r1, ADJ, fermi);
         // the exact per-pair check runs wave-wide when any group of the pass is mixed: a row of a
@@ -1047,13 +1062,14 @@ __device__ void pair_group_packed(const PairParams& p, uint64_t mask, uint32_t g
         stage[lane] = stage_entry<W, ADJ, NV>(p, act, va, core0, fermi);
         double acc_re[NV] = {}, acc_im[NV] = {};
         __builtin_amdgcn_wave_barrier();
-        for (int u = 0; u < nmax; u += B) {
+        for (int u = 0; u < f.nmax; u += B) {
             double2 x[B], gv[B], gv2[B];
             bool ok[B];
 #pragma unroll
             for (int k = 0; k < B; ++k) {
                 const int uu = u + k;
-                const PairStage<W>& st = stage[gi * H + (uu & (H - 1))];
+                // slots past the group belong to other groups or to no group: bounded by uu < g
+                const PairStage<W>& st = stage[(f.lead + uu) & 63];
                 gv[k] = st.g;
                 gv2[k] = NV == 2 ? st.g2 : make_double2(0.0, 0.0);
                 ok[k] = pair_term<W, ADJ, REAL>(p, ra, st, wmixed, act && uu < g, x[k]);
@@ -1232,8 +1248,7 @@ template <int W, bool ADJ, int B, int NV, bool PACK, bool REAL>
 __device__ __forceinline__ void pair_chunk_groups(const PairParams& p, uint32_t gs, uint32_t ge, uint32_t sz, int lane,
                                                   PairStage<W>* stage) {
     if (PACK) {
-        pair_group_packed<W, ADJ, B, NV, 16, REAL>(p, __ballot(sz > 8u && sz <= 16u), gs, ge, lane, stage);
-        pair_group_packed<W, ADJ, B, NV, 32, REAL>(p, __ballot(sz > 16u && sz <= 32u), gs, ge, lane, stage);
+        pair_group_rows<W, ADJ, B, NV, REAL>(p, __ballot(sz > 8u && sz <= 32u), gs, ge, lane, stage);
     }
     uint64_t big = __ballot(sz > (PACK ? 32u : 8u));
     while (big) {
@@ -1499,33 +1514,18 @@ __device__ void single_group(const SingleParams& p, uint32_t s_raw, uint32_t e, 
     }
 }
 
-// Single-excitation groups of 9..H entries (H = 16 or 32), 64 / H of them per pass: lane l serves
-// row l % H of group l / H; a group's entries are its rows, read from the lanes of the group by
-// shuffles (as single_group reads its staged entries), so several groups share a wave.
-template <int W, bool ADJ, int NV, int CH, int H>
-__device__ void single_group_packed(const SingleParams& p, uint64_t mask, uint32_t gs_l, uint32_t ge_l, int lane) {
-    constexpr int kPer = 64 / H;
-    const int gi = lane / H;
-    const int ri = lane % H;
+// Single-excitation groups of 9..32 entries laid out by row_pass; a group's entries are its rows,
+// read from the lanes of the group by shuffles (as single_group reads its staged entries), so
+// several groups share a wave.
+template <int W, bool ADJ, int NV, int CH>
+__device__ void single_group_rows(const SingleParams& p, uint64_t mask, uint32_t gs_l, uint32_t ge_l, int lane) {
     const bool fermi = p.particle_cut == 1;
     while (mask) {
-        uint32_t s = 0, e = 0;
-        int nmax = 0;
-        for (int j = 0; j < kPer && mask != 0ull; ++j) {
-            const int l = __builtin_ctzll(mask);
-            mask &= mask - 1;
-            const uint32_t sj = __builtin_amdgcn_readlane(gs_l, l);
-            const uint32_t ej = __builtin_amdgcn_readlane(ge_l, l);
-            const int nj = static_cast<int>(ej - (sj & kGroupStart));
-            nmax = nj > nmax ? nj : nmax;
-            if (gi == j) {
-                s = sj;
-                e = ej;
-            }
-        }
-        const bool mixed = (s & kGroupMixed) != 0u;
-        s &= kGroupStart;
-        const int g = static_cast<int>(e - s);  // 0 for a slot without a group
+        const RowLane f = row_pass(mask, gs_l, ge_l, lane);
+        const bool mixed = (f.s & kGroupMixed) != 0u;
+        const uint32_t s = f.s & kGroupStart;
+        const int g = static_cast<int>(f.e - s);  // 0 for a lane without a group
+        const int ri = lane - f.lead;
         const bool act = ri < g;
         const uint64_t va = act ? p.vals[s + static_cast<uint32_t>(ri)] : 0ull;
         const int64_t ia = static_cast<int64_t>(va >> 16);
@@ -1542,7 +1542,7 @@ __device__ void single_group_packed(const SingleParams& p, uint64_t mask, uint32
         Cfg<W> core;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            const uint64_t c0 = shfl_u64(own.w[w], gi * H);  // every lane runs every shuffle
+            const uint64_t c0 = shfl_u64(own.w[w], f.lead);  // every lane runs every shuffle
             core.w[w] = mixed ? own.w[w] : c0;
         }
         // the exact per-pair check runs wave-wide when any group of the pass is mixed (a row of a
@@ -1553,8 +1553,9 @@ __device__ void single_group_packed(const SingleParams& p, uint64_t mask, uint32
         const double2 gin = act ? p.in[ia] : make_double2(0.0, 0.0);
         const double2 gin2 = (NV == 2 && act) ? p.in2[ia] : make_double2(0.0, 0.0);
         double acc_re[NV] = {}, acc_im[NV] = {};
-        for (int u = 0; u < nmax; ++u) {
-            const int src = gi * H + (u & (H - 1));
+        for (int u = 0; u < f.nmax; ++u) {
+            // lanes past the group belong to other groups or to no group: bounded by u < g
+            const int src = (f.lead + u) & 63;
             const uint64_t vb = shfl_u64(va, src);
             const double2 gb = shfl_d2(gin, src);
             const double2 gb2 = NV == 2 ? shfl_d2(gin2, src) : make_double2(0.0, 0.0);
@@ -1702,8 +1703,7 @@ __device__ __forceinline__ void single_chunk(const SingleParams& p, int64_t grou
     single_flat_class<W, ADJ, NV, CH>(p, __ballot(sz >= 2u && sz <= 8u), gs, ge, lane);
     // PACK: groups of 9..32 entries several to a wave (else one group per wave)
     if (PACK) {
-        single_group_packed<W, ADJ, NV, CH, 16>(p, __ballot(sz > 8u && sz <= 16u), gs, ge, lane);
-        single_group_packed<W, ADJ, NV, CH, 32>(p, __ballot(sz > 16u && sz <= 32u), gs, ge, lane);
+        single_group_rows<W, ADJ, NV, CH>(p, __ballot(sz > 8u && sz <= 32u), gs, ge, lane);
     }
     uint64_t big = __ballot(sz > (PACK ? 32u : 8u));
     while (big) {
