@@ -776,10 +776,11 @@ __device__ __forceinline__ double2 shfl_d2(const double2& v, int lane) {
     return make_double2(__shfl(v.x, lane), __shfl(v.y, lane));
 }
 
-// Coefficient gathers kept in flight per lane in pair_group: 4 in the pair kernel; 2 inside the
-// batched kernel, whose other paths already hold many registers (fewer registers, more waves).
+// Coefficient gathers kept in flight per lane in pair_group: 4 in the one-vector pair kernel; 3 in
+// the two-vector pair kernel and the batched kernel, whose accumulators and other paths hold more
+// registers (4 there spills).
 constexpr int kPairBatch = 4;
-constexpr int kPairBatchInBatch = 2;
+constexpr int kPairBatchInBatch = 3;
 
 template <int W>
 __device__ __forceinline__ Cfg<W> site_bits(int a, int b) {
