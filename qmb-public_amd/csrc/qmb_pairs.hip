@@ -1994,9 +1994,14 @@ __global__ __launch_bounds__(kBlock) void hop_kernel(HopParams p) {
             L.rowc[lane][w] = ca.w[w];
         }
         const uint64_t sa = act ? p.mstr[e] : 0ull;
+        // the particle count of the wave's first row's string, and whether every row of the wave
+        // holds as many (lane 0 always has a row)
+        const int pc_ref = __builtin_amdgcn_readfirstlane(__popcll(sa));
+        const bool rows_even = __ballot(act && __popcll(sa) != pc_ref) == 0ull;
         L.acc[lane] = make_double2(0.0, 0.0);
         for (uint32_t c0 = lo; c0 < hi; c0 += 64u) {
             const uint32_t b = c0 + static_cast<uint32_t>(lane);
+            bool uneven = false;
             if (b < hi) {
                 HopStage<W> st;
                 if (merged) {
@@ -2006,10 +2011,13 @@ __global__ __launch_bounds__(kBlock) void hop_kernel(HopParams p) {
                     }
                 } else {
                     st.c[0] = p.mstr[b];
+                    uneven = __popcll(st.c[0]) != pc_ref;
                 }
                 st.g = p.s.in[p.mrows[b]];
                 L.stage[lane] = st;
             }
+            // every string the chunk's pair tests meet holds pc_ref particles
+            const bool even = rows_even && __ballot(uneven) == 0ull;
             __builtin_amdgcn_wave_barrier();
             const int nb = static_cast<int>(hi - c0 < 64u ? hi - c0 : 64u);
             // the pair tests: bit u of `hit` marks a connection of this lane's row to entry u.  A row
@@ -2033,6 +2041,13 @@ __global__ __launch_bounds__(kBlock) void hop_kernel(HopParams p) {
                     }
                     // two sites differ, both of the hop parity, one occupied in each config
                     hit |= (pc == 2 && pa == 1 && off == 0ull) ? (1ull << u) : 0ull;
+                }
+            } else if (even) {
+                // rows of one (unmerged) group share the other parity's string, and strings of
+                // equal particle counts that differ in two sites hold one of them each: the count
+                // of differing sites decides
+                for (int u = ulo; u < uhi; ++u) {
+                    hit |= static_cast<uint64_t>(__popcll(sa ^ L.stage[u].c[0]) == 2) << u;
                 }
             } else {
                 // rows of one (unmerged) group share the other parity's string: two hop-parity

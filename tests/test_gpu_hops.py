@@ -84,16 +84,17 @@ def test_hubbard_hops_match_enumeration_and_oracle(cuda_device: torch.device, m:
     _check_oracle_rows(w, 7200 + sites)
 
 
-def _hubbard_configs(sites: int, count: int, seed: int) -> np.ndarray:
-    """Half-filled configs: random seeds plus all their same-spin moves between any two sites (so
-    lattice hops meet their targets), distinct, sorted."""
+def _hubbard_configs(sites: int, count: int, seed: int, n_up: int | None = None, n_dn: int | None = None) -> np.ndarray:
+    """Configs of n_up + n_dn particles (default: half filling): random seeds plus all their
+    same-spin moves between any two sites (so lattice hops meet their targets), distinct, sorted."""
     rng = np.random.default_rng(seed)
-    n_up = sites // 2
+    n_up = sites // 2 if n_up is None else n_up
+    n_dn = sites - sites // 2 if n_dn is None else n_dn
     n_seed = max(4, count // (2 * n_up * (sites - n_up)) + 1)
     occ = np.zeros((n_seed, 2 * sites), dtype=np.uint8)
     for r in range(n_seed):
         occ[r, 2 * rng.choice(sites, n_up, replace=False)] = 1
-        occ[r, 2 * rng.choice(sites, sites - n_up, replace=False) + 1] = 1
+        occ[r, 2 * rng.choice(sites, n_dn, replace=False) + 1] = 1
     parts = [occ]
     for spin in (0, 1):
         for a in range(sites):
@@ -107,6 +108,25 @@ def _hubbard_configs(sites: int, count: int, seed: int) -> np.ndarray:
     if len(allc) > count:
         allc = allc[np.sort(rng.choice(len(allc), size=count, replace=False))]
     return allc
+
+
+def test_hubbard_hops_mixed_particle_counts(cuda_device: torch.device) -> None:
+    """Configs of three (up, down) particle counts: rows of one group then hold hop-parity strings of
+    different particle counts, where the hop kernel checks that one differing site is occupied in
+    each string (with equal counts the number of differing sites decides)."""
+    m = 12
+    site, kind, coef = (x.numpy() for x in sy.terms_from_dict(sy.hubbard_dict(m, 1, 1.0, 4.0)))
+    nq = (2 * m + 7) // 8
+    parts = [_hubbard_configs(m, 4000, 7900 + k, n_up=m // 2 + du, n_dn=m // 2 + dd)
+             for k, (du, dd) in enumerate(((0, 0), (1, 0), (0, 1)))]
+    cfg = sy.sort_unique_configs(np.concatenate(parts))
+    occ = np.unpackbits(cfg, axis=1, bitorder="little")[:, :2 * m]
+    counts = {(int(u), int(d)) for u, d in zip(occ[:, 0::2].sum(axis=1), occ[:, 1::2].sum(axis=1))}
+    assert counts == {(6, 6), (7, 6), (6, 7)}
+    w = _prepare(site, kind, coef, cfg, nq, cuda_device)
+    assert w["terms"].hops and w["terms"].n_residual_terms == 0
+    _check_against_enumeration(w, 7910)
+    _check_oracle_rows(w, 7920)
 
 
 def _spinful_random_terms(n_sites: int, seed: int, n_diag_pairs: int | None = None) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
