@@ -2016,8 +2016,10 @@ __global__ __launch_bounds__(kBlock) void hop_kernel(HopParams p) {
                 st.g = p.s.in[p.mrows[b]];
                 L.stage[lane] = st;
             }
-            // every string the chunk's pair tests meet holds pc_ref particles
-            const bool even = rows_even && __ballot(uneven) == 0ull;
+            // every string the chunk's pair tests meet holds pc_ref particles (every lane runs the
+            // ballot)
+            const bool staged_even = __ballot(uneven) == 0ull;
+            const bool even = rows_even && staged_even;
             __builtin_amdgcn_wave_barrier();
             const int nb = static_cast<int>(hi - c0 < 64u ? hi - c0 : 64u);
             // the pair tests: bit u of `hit` marks a connection of this lane's row to entry u.  A row
