@@ -1388,9 +1388,12 @@ __device__ __forceinline__ int single_parity(const Cfg<W>& c, int r, int p) {
 // Sum of row[k] over the set bits k of c, ascending k.  Up to CH indices are taken from the mask
 // before their loads are issued, so the loads of one element overlap instead of each waiting for the
 // last add (same order of additions, same result as a bit-by-bit walk).  The single-set kernels use
-// CH = 8; the batch kernel, held to 128 VGPRs, uses 4 (measured on 64 molecular instances: 126.8 ms
+// CH = kSingleLoads (13: the 13 core sites of a 14-electron config in one batch; single kernel 1.68
+// -> 1.40 ms at 52 spin orbitals and 7a+7b against 8, while 16 ran 1.67 ms); the batch kernel, held
+// to 128 VGPRs, uses 4 (measured on 64 molecular instances: 126.8 ms
 // per step against 128.8 ms with 2 and 130.7 ms with the plain walk).  row must hold at least
 // 64 * (W - 1) + 1 entries (a lane without an index reads row[64 w] and drops it).
+constexpr int kSingleLoads = 13;
 __device__ __forceinline__ double zero_like(double) { return 0.0; }
 __device__ __forceinline__ double2 zero_like(double2) { return make_double2(0.0, 0.0); }
 __device__ __forceinline__ double add_to(double a, double b) { return a + b; }
@@ -1727,7 +1730,7 @@ __global__ __launch_bounds__(kBlock) void single_kernel(SingleParams p) {
     }
     const int64_t groups = p.counts[1];
     for (int64_t g0 = wave * 64; g0 < groups; g0 += nwaves * 64) {
-        single_chunk<W, ADJ, NV, 8>(p, groups, g0, lane);
+        single_chunk<W, ADJ, NV, kSingleLoads>(p, groups, g0, lane);
     }
 }
 
